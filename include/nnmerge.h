@@ -1,0 +1,194 @@
+/*
+ * nnmerge.h -- C ABI of the MI355X (gfx950) NN-Descent local-join library.
+ *
+ * The reference (wlzhao/nn-merge) is a single C++11 program without an FFI:
+ * its merge classes call the local join through a virtual method
+ * (AbstractMerge::localJoin, src/abstractmerge.h:30) once per iteration of
+ * AbstractMerge::nnDescent (src/abstractmerge.cpp:78-192), and every pair goes
+ * through NNList::join (src/pvetex.h:111-125), the distance pointer DISTFUNC
+ * (src/abstractgraph.h:11-12) and NNList::parallel_try_insert
+ * (src/pvetex.h:99-109).  This header is the boundary the host classes
+ * (Smerge, Jmerge, KGraph, LyNNDescent, LySearch in nn-merge_amd/host) call in
+ * place of those in-process calls.  Each entry point names the reference
+ * routine it replaces.
+ *
+ * Conventions: plain pointers and sizes, no exceptions across the ABI, every
+ * call returns an int status (NNM_OK = 0, negative = error) and
+ * nnm_last_error() returns the text of the last failure on that context.
+ * Host arrays are copied into device memory owned by the context; the library
+ * never takes ownership of caller memory.  Calls are synchronous.
+ * A context is used by one host thread at a time.
+ */
+#ifndef NNMERGE_H
+#define NNMERGE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NNM_OK 0
+#define NNM_ERR_INVALID -1     /* bad argument or shape */
+#define NNM_ERR_HIP -2         /* HIP runtime failure */
+#define NNM_ERR_NOMEM -3       /* device allocation failed or over the cap */
+#define NNM_ERR_STATE -4       /* call out of order (no data, no graph, ...) */
+#define NNM_ERR_UNSUPPORTED -5 /* valid request this build does not handle */
+
+/* Subgraph label of rows that arrive without a k-NN graph (Gid == 1 rows of
+ * Jmerge, src/jmerge.cpp:160-163; the new block of LyNNDescent::appdNwSmpl,
+ * src/lynndescent.cpp:418-445). */
+#define NNM_RAW 0xFFFFFFFFu
+
+/* Distance precision of the local join. */
+#define NNM_PREC_FP32 0    /* FP32 difference form on the vector ALUs */
+#define NNM_PREC_BF16 1    /* BF16 MFMA norm form, FP32 re-rank */
+#define NNM_PREC_LEARNED 2 /* learned projection prefilter, FP32 re-rank */
+
+/* Pair rule of the local join. */
+#define NNM_RULE_ALL 0   /* every pair with distinct ids: KGraph::join
+                            (src/kgraph.cpp:191-210), verse = 1 in
+                            src/lynndescent.cpp:330 */
+#define NNM_RULE_MERGE 1 /* skip pairs inside one built subgraph:
+                            Smerge::localJoin (src/smerge.cpp:224),
+                            Jmerge::localJoin (src/jmerge.cpp:189-191),
+                            LyNNDescent::localJoin (src/lynndescent.cpp:330) */
+
+/* Re-rank policy for approximate precisions. */
+#define NNM_RERANK_AUTO 0   /* skip when the approximate form is exact */
+#define NNM_RERANK_ALWAYS 1
+#define NNM_RERANK_NEVER 2
+
+typedef struct nnm_ctx nnm_ctx;
+
+/* NN-Descent parameters.  Defaults (nnm_default_params) equal the reference
+ * constants: K = 20, rho = 1 (every new entry joins), reverse cap 100
+ * (smplNum, src/abstractmerge.cpp:81), no iteration cap, delta = 0.002
+ * (default_delta, src/abstractgraph.cpp:4). */
+typedef struct nnm_params {
+    uint32_t K;         /* list length */
+    float rho;          /* forward sampling: at most round(rho*K) new entries */
+    uint32_t rev_cap;   /* reverse list cap (new and old each) */
+    uint32_t max_iters; /* iteration cap, 0 = until delta */
+    float delta;        /* stop when mean fraction of new entries < delta */
+    uint32_t rule;      /* NNM_RULE_* */
+    uint32_t prec;      /* NNM_PREC_* */
+    uint32_t rerank;    /* NNM_RERANK_* */
+    uint32_t slots;     /* update slots per vertex per iteration (power of
+                           two, 16..64); 0 = 64 */
+    uint32_t cut;       /* merge set-up: far entries replaced by cross
+                           samples; 0 = K/2 (cutSize, src/smerge.cpp:99) */
+    uint64_t seed;      /* every random choice derives from this seed */
+} nnm_params;
+
+/* Counters of one call.  dists follows the reference's nCmps: one per pair
+ * that passes the pair rule (src/smerge.cpp:226-227). */
+typedef struct nnm_stats {
+    uint32_t iters;
+    uint32_t converged;    /* 1 when the delta test ended the loop */
+    uint64_t dists;        /* join distances evaluated */
+    uint64_t init_dists;   /* distances spent by set-up calls */
+    uint64_t updates;      /* list entries replaced */
+    uint64_t reranked;     /* candidates recomputed in exact FP32 */
+    uint64_t list_entries; /* candidate-list entries joined */
+    double last_delta;     /* mean fraction of new entries after the loop */
+    double secs;           /* device time of the call */
+    double join_secs;      /* device time spent in the join kernels */
+} nnm_stats;
+
+void nnm_default_params(nnm_params* p);
+
+/* --- context ---------------------------------------------------------- */
+int nnm_create(nnm_ctx** out, int device);
+int nnm_destroy(nnm_ctx* ctx);
+const char* nnm_last_error(const nnm_ctx* ctx);
+/* Device-memory cap in bytes for this context (0 = no cap).  Allocations
+ * past the cap fail with NNM_ERR_NOMEM. */
+int nnm_set_mem_cap(nnm_ctx* ctx, uint64_t bytes);
+uint64_t nnm_mem_used(const nnm_ctx* ctx);
+
+/* --- data ------------------------------------------------------------- */
+/* Row-major n x d float32 matrix (AbstractGraph::mat, src/abstractgraph.h:28,
+ * loaded by IODelegator::load_refSet, src/iodelegator.cpp:159-182).
+ * Replaces the matrix every distance call indexes through DISTFUNC
+ * (src/abstractgraph.h:11-12).  Resets graph and partition state. */
+int nnm_set_data(nnm_ctx* ctx, const float* X, uint64_t n, uint32_t d);
+
+/* Partition of the rows into nsub contiguous subgraphs: subgraph s covers
+ * rows [sub_begin[s], sub_begin[s+1]) (sub_begin has nsub+1 entries, the
+ * last equal to n).  sub_built[s] = 1 for a subgraph that has a k-NN graph
+ * (its rows carry label s), 0 for raw rows (label NNM_RAW).  sub_group[s]
+ * puts subgraphs into merge groups: random samples are drawn inside a group
+ * and groups never exchange neighbours, so several merges (or several
+ * independent builds) run in one launch.  Replaces the Gid bookkeeping of
+ * src/abstractmerge.cpp:107-111 and LyNNDescent::getGraphId
+ * (src/lynndescent.cpp:189-199).  Groups must be contiguous runs of
+ * subgraphs. */
+int nnm_set_partition(nnm_ctx* ctx, const uint32_t* sub_begin, const uint8_t* sub_built,
+                      const uint32_t* sub_group, uint32_t nsub);
+
+/* --- graph ------------------------------------------------------------ */
+/* Random initial lists: K distinct random neighbours per row inside its
+ * group, self excluded, sorted by distance, all flagged new.  Replaces
+ * KGraph::InitialGraph (src/kgraph.cpp:56-96). */
+int nnm_graph_random_init(nnm_ctx* ctx, uint32_t K, uint64_t seed, nnm_stats* st);
+
+/* Upload lists (n x K ids, global row numbers).  dist == NULL recomputes the
+ * distances (AbstractMerge::InitSGraph, src/abstractmerge.cpp:55-76); flags
+ * == NULL marks every entry new.  Rows are sorted by (dist, id) on upload. */
+int nnm_set_graph(nnm_ctx* ctx, const uint32_t* ids, const float* dist, const uint8_t* flags,
+                  uint32_t K, nnm_stats* st);
+
+/* Download lists: ids and dist are n x K (either may be NULL), flags n x K
+ * (may be NULL).  Replaces the pool read-out of AbstractMerge::saveKGraph
+ * (src/abstractmerge.cpp:278-291) and KGraph::buildKGraph
+ * (src/kgraph.cpp:228-238). */
+int nnm_get_graph(nnm_ctx* ctx, uint32_t* ids, float* dist, uint8_t* flags);
+
+/* --- NN-Descent ------------------------------------------------------- */
+/* The NN-Descent loop: list build + reverse sampling + local join + delta
+ * test, until delta or max_iters.  Replaces AbstractMerge::nnDescent
+ * (src/abstractmerge.cpp:78-192), KGraph::nndescent (src/kgraph.cpp:98-189)
+ * and LyNNDescent::nnDescent (src/lynndescent.cpp:201-310), with the pair
+ * rule of the matching localJoin. */
+int nnm_nndescent(nnm_ctx* ctx, const nnm_params* p, nnm_stats* st);
+
+/* One iteration of the loop above (list build, localJoin, delta).  The
+ * per-iteration counterpart of AbstractMerge::localJoin
+ * (src/abstractmerge.h:30). */
+int nnm_local_join(nnm_ctx* ctx, const nnm_params* p, uint32_t iter, nnm_stats* st);
+
+/* --- merges ----------------------------------------------------------- */
+/* Merge set-up over the current partition.  Rows of built subgraphs park
+ * their far `cut` entries and take `cut` fresh random samples from the other
+ * subgraphs of their group (Smerge::InitialGraph src/smerge.cpp:94-209,
+ * Jmerge::InitialGraph src/jmerge.cpp:95-140, LyNNDescent::appdNwSmpl
+ * src/lynndescent.cpp:371-415); raw rows get K random neighbours over their
+ * group (src/jmerge.cpp:147-171, src/lynndescent.cpp:421-445).  Requires a
+ * graph for the built rows (nnm_set_graph or an earlier build). */
+int nnm_merge_init(nnm_ctx* ctx, const nnm_params* p, nnm_stats* st);
+
+/* Merge the parked entries back: AbstractMerge::merge2one
+ * (src/abstractmerge.cpp:232-277), LyNNDescent::merge2one
+ * (src/lynndescent.cpp:455-500).  dedup = 1 drops repeated ids (the
+ * reference keeps them). */
+int nnm_merge_finish(nnm_ctx* ctx, int dedup);
+
+/* --- brute force ------------------------------------------------------ */
+/* Exact k-NN (FP32 difference form, ties broken by id) of nq query rows
+ * against all rows.  Q == NULL: the queries are rows `rows[0..nq)` of the
+ * data and exclude_self drops each query's own row.  Q != NULL: nq x d query
+ * vectors.  Output nq x k.  The reference reads this from a precomputed
+ * ground-truth file (Evaluator::loadGrdKNNGraph, src/evaluator.cpp:343-378). */
+int nnm_bruteforce_knn(nnm_ctx* ctx, const float* Q, const uint32_t* rows, uint64_t nq,
+                       uint32_t k, int exclude_self, uint32_t* ids, float* dist);
+
+/* Squared L2 between row pairs (a[i], b[i]) in the library's FP32 order:
+ * the DISTFUNC call of src/abstractgraph.h:11-12 batched. */
+int nnm_pair_dists(nnm_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t m, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNMERGE_H */

@@ -1,0 +1,202 @@
+// Exact k-NN by brute force (FP32 difference form, ties broken by id).
+//
+// The reference has no such routine: it reads exact k-NN lists from a
+// precomputed file (Evaluator::loadGrdKNNGraph, src/evaluator.cpp:343-378)
+// and compares merged graphs against it (Evaluator::getKGraphRecall,
+// src/evaluator.cpp:258-295).  This kernel produces those lists on the GPU.
+//
+// Layout: each lane owns one query row (held in registers) and a sorted
+// top-KMAX list in registers; database rows stream through LDS in tiles of
+// BT rows, read by every lane of the block at the same address.  The
+// database is split into nsplit ranges over blockIdx.y; a second kernel
+// merges the per-range lists.  Distances use the library order (0..d-1,
+// fused multiply-add), so they match nnm_l2_rows bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "nnm_internal.hpp"
+
+#define BF_BT 32
+#define BF_KMAX 32
+
+__device__ inline bool bf_lt(float d1, uint32_t i1, float d2, uint32_t i2) {
+    return d1 < d2 || (d1 == d2 && i1 < i2);
+}
+
+template <int KM>
+__device__ inline void bf_insert(float (&td)[KM], uint32_t (&ti)[KM], float d, uint32_t id) {
+    if (!bf_lt(d, id, td[KM - 1], ti[KM - 1])) return;
+    td[KM - 1] = d;
+    ti[KM - 1] = id;
+#pragma unroll
+    for (int p = KM - 1; p > 0; --p) {
+        const bool sw = bf_lt(td[p], ti[p], td[p - 1], ti[p - 1]);
+        const float a = td[p], b = td[p - 1];
+        const uint32_t ia = ti[p], ib = ti[p - 1];
+        td[p] = sw ? b : a;
+        td[p - 1] = sw ? a : b;
+        ti[p] = sw ? ib : ia;
+        ti[p - 1] = sw ? ia : ib;
+    }
+}
+
+template <int LDMAX>
+__global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X, uint32_t ld, uint64_t n,
+                                                    const float* __restrict__ Q, const uint32_t* __restrict__ rows,
+                                                    uint64_t nq, int exclude_self, uint32_t nsplit,
+                                                    float* __restrict__ part_d, uint32_t* __restrict__ part_i) {
+    __shared__ __align__(16) float tile[BF_BT * LDMAX];
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t split = blockIdx.y;
+    const uint64_t per = (n + nsplit - 1) / nsplit;
+    const uint64_t lo = split * per;
+    const uint64_t hi = std::min<uint64_t>(n, lo + per);
+    const bool active = q < nq;
+
+    float qv[LDMAX];
+    uint32_t self = 0xFFFFFFFFu;
+    {
+        const float* src = nullptr;
+        if (active) {
+            if (Q) {
+                src = Q + q * ld;
+            } else {
+                const uint32_t r = rows[q];
+                src = X + (uint64_t)r * ld;
+                if (exclude_self) self = r;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < LDMAX; c += 4) {
+            float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (active && (uint32_t)c < ld) t = *reinterpret_cast<const float4*>(src + c);
+            qv[c] = t.x; qv[c + 1] = t.y; qv[c + 2] = t.z; qv[c + 3] = t.w;
+        }
+    }
+    float td[BF_KMAX];
+    uint32_t ti[BF_KMAX];
+#pragma unroll
+    for (int i = 0; i < BF_KMAX; ++i) { td[i] = INFINITY; ti[i] = 0xFFFFFFFFu; }
+
+    const uint32_t row_f4 = ld / 4;
+    for (uint64_t r0 = lo; r0 < hi; r0 += BF_BT) {
+        const uint32_t nt = (uint32_t)std::min<uint64_t>(BF_BT, hi - r0);
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < nt * row_f4; e += blockDim.x) {
+            const uint32_t rr = e / row_f4, cc = e % row_f4;
+            *reinterpret_cast<float4*>(&tile[rr * LDMAX + cc * 4]) =
+                *reinterpret_cast<const float4*>(X + (r0 + rr) * ld + cc * 4);
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (uint32_t rr = 0; rr < nt; ++rr) {
+            const float* tr = &tile[rr * LDMAX];
+            float acc = 0.f;
+#pragma unroll
+            for (int c = 0; c < LDMAX; c += 4) {
+                if ((uint32_t)c < ld) {
+                    const float4 y = *reinterpret_cast<const float4*>(tr + c);
+                    float t = qv[c] - y.x; acc = __builtin_fmaf(t, t, acc);
+                    t = qv[c + 1] - y.y; acc = __builtin_fmaf(t, t, acc);
+                    t = qv[c + 2] - y.z; acc = __builtin_fmaf(t, t, acc);
+                    t = qv[c + 3] - y.w; acc = __builtin_fmaf(t, t, acc);
+                }
+            }
+            const uint32_t id = (uint32_t)(r0 + rr);
+            if (id == self) continue;
+            bf_insert(td, ti, acc, id);
+        }
+    }
+    if (!active) return;
+    float* od = part_d + (q * nsplit + split) * BF_KMAX;
+    uint32_t* oi = part_i + (q * nsplit + split) * BF_KMAX;
+#pragma unroll
+    for (int i = 0; i < BF_KMAX; ++i) { od[i] = td[i]; oi[i] = ti[i]; }
+}
+
+__global__ void k_bf_merge(uint64_t nq, uint32_t nsplit, uint32_t k, const float* __restrict__ part_d,
+                           const uint32_t* __restrict__ part_i, float* __restrict__ out_d,
+                           uint32_t* __restrict__ out_i) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    float td[BF_KMAX];
+    uint32_t ti[BF_KMAX];
+#pragma unroll
+    for (int i = 0; i < BF_KMAX; ++i) { td[i] = INFINITY; ti[i] = 0xFFFFFFFFu; }
+    for (uint32_t s = 0; s < nsplit; ++s) {
+        const float* pd = part_d + (q * nsplit + s) * BF_KMAX;
+        const uint32_t* pi = part_i + (q * nsplit + s) * BF_KMAX;
+        for (uint32_t i = 0; i < BF_KMAX; ++i) {
+            if (pi[i] == 0xFFFFFFFFu) break;
+            bf_insert(td, ti, pd[i], pi[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < BF_KMAX; ++i)
+        if ((uint32_t)i < k) { out_d[q * k + i] = td[i]; out_i[q * k + i] = ti[i]; }
+}
+
+extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* rows, uint64_t nq, uint32_t k,
+                                  int exclude_self, uint32_t* ids, float* dist) {
+    if (!c || !ids || !dist || nq == 0) return nnm_fail(c, NNM_ERR_INVALID, "nnm_bruteforce_knn: bad arguments");
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    if (k == 0 || k > BF_KMAX) return nnm_fail(c, NNM_ERR_INVALID, "k must be in [1, 32]");
+    if (!Q && !rows) return nnm_fail(c, NNM_ERR_INVALID, "need query vectors or query rows");
+    if (c->ld > 256) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "brute force supports d <= 256");
+    if (!Q)
+        for (uint64_t i = 0; i < nq; ++i)
+            if (rows[i] >= c->n) return nnm_fail(c, NNM_ERR_INVALID, "query row out of range");
+    NNM_HIP(c, hipSetDevice(c->device));
+    const uint64_t qblocks = (nq + 255) / 256;
+    uint64_t want = (uint64_t)c->num_cus * 4;
+    uint32_t nsplit = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want / qblocks, 512));
+    nsplit = (uint32_t)std::min<uint64_t>(nsplit, (c->n + BF_BT - 1) / BF_BT);
+    float* dQ = nullptr;
+    uint32_t* dR = nullptr;
+    float *pd = nullptr, *od = nullptr;
+    uint32_t *pi = nullptr, *oi = nullptr;
+    int rc = NNM_OK;
+    auto cleanup = [&]() {
+        nnm_free(c, dQ); nnm_free(c, dR); nnm_free(c, pd); nnm_free(c, pi); nnm_free(c, od); nnm_free(c, oi);
+    };
+    if (Q) {
+        rc = nnm_alloc(c, (void**)&dQ, nq * c->ld * sizeof(float));
+        if (rc) { cleanup(); return rc; }
+        if (c->ld != c->d) hipMemsetAsync(dQ, 0, nq * c->ld * sizeof(float), c->stream);
+        hipMemcpy2DAsync(dQ, c->ld * sizeof(float), Q, c->d * sizeof(float), c->d * sizeof(float), nq,
+                         hipMemcpyHostToDevice, c->stream);
+    } else {
+        rc = nnm_alloc(c, (void**)&dR, nq * sizeof(uint32_t));
+        if (rc) { cleanup(); return rc; }
+        hipMemcpyAsync(dR, rows, nq * 4, hipMemcpyHostToDevice, c->stream);
+    }
+    if ((rc = nnm_alloc(c, (void**)&pd, nq * nsplit * BF_KMAX * 4)) ||
+        (rc = nnm_alloc(c, (void**)&pi, nq * nsplit * BF_KMAX * 4)) ||
+        (rc = nnm_alloc(c, (void**)&od, nq * k * 4)) || (rc = nnm_alloc(c, (void**)&oi, nq * k * 4))) {
+        cleanup();
+        return rc;
+    }
+    dim3 grid((uint32_t)qblocks, nsplit);
+    if (c->ld <= 64)
+        hipLaunchKernelGGL(k_bruteforce<64>, grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,
+                           exclude_self, nsplit, pd, pi);
+    else if (c->ld <= 128)
+        hipLaunchKernelGGL(k_bruteforce<128>, grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,
+                           exclude_self, nsplit, pd, pi);
+    else
+        hipLaunchKernelGGL(k_bruteforce<256>, grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,
+                           exclude_self, nsplit, pd, pi);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { cleanup(); return nnm_hip_fail(c, e, "k_bruteforce launch"); }
+    hipLaunchKernelGGL(k_bf_merge, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, c->stream, nq, nsplit, k, pd,
+                       pi, od, oi);
+    e = hipGetLastError();
+    if (e != hipSuccess) { cleanup(); return nnm_hip_fail(c, e, "k_bf_merge launch"); }
+    hipMemcpyAsync(ids, oi, nq * k * 4, hipMemcpyDeviceToHost, c->stream);
+    hipMemcpyAsync(dist, od, nq * k * 4, hipMemcpyDeviceToHost, c->stream);
+    e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return nnm_hip_fail(c, e, "nnm_bruteforce_knn");
+    return NNM_OK;
+}

@@ -1,0 +1,1159 @@
+// Context, data, partition, graph set-up and the NN-Descent driver loop.
+//
+// Kernel map (reference routine each one replaces):
+//   k_random_init     KGraph::InitialGraph        src/kgraph.cpp:56-96
+//   k_set_graph       AbstractMerge::InitSGraph   src/abstractmerge.cpp:55-76
+//   k_merge_init      Smerge::InitialGraph        src/smerge.cpp:94-209
+//                     Jmerge::InitialGraph        src/jmerge.cpp:83-176
+//                     LyNNDescent::appdNwSmpl     src/lynndescent.cpp:354-452
+//   k_merge_finish    AbstractMerge::merge2one    src/abstractmerge.cpp:232-277
+//   k_lb_count/fill   list build of nnDescent     src/abstractmerge.cpp:86-137
+//   k_lb_select       reverse sampling            src/abstractmerge.cpp:139-166
+//   k_update          UpdateKnnList per pair      src/pvetex.h:40-85, 99-109
+//                     + EvaluateDelta             src/abstractmerge.cpp:194-202
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "nnm_internal.hpp"
+
+// ================================================================ host utils
+
+int nnm_fail(nnm_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int nnm_hip_fail(nnm_ctx* c, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) return nnm_fail(c, NNM_ERR_NOMEM, m);
+    return nnm_fail(c, NNM_ERR_HIP, m);
+}
+
+int nnm_alloc(nnm_ctx* c, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~size_t(255);
+    if (c->mem_cap && c->mem_used + bytes > c->mem_cap)
+        return nnm_fail(c, NNM_ERR_NOMEM, "device memory cap exceeded (" +
+                                              std::to_string(c->mem_used + bytes) + " > " +
+                                              std::to_string(c->mem_cap) + " bytes)");
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) { *p = nullptr; return nnm_hip_fail(c, e, "hipMalloc"); }
+    c->mem_used += bytes;
+    c->allocs.emplace_back(*p, bytes);
+    return NNM_OK;
+}
+
+void nnm_free(nnm_ctx* c, void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < c->allocs.size(); ++i) {
+        if (c->allocs[i].first == p) {
+            c->mem_used -= c->allocs[i].second;
+            c->allocs.erase(c->allocs.begin() + i);
+            break;
+        }
+    }
+    hipFree(p);
+}
+
+template <typename T>
+static int alloc_n(nnm_ctx* c, T** p, size_t n) {
+    return nnm_alloc(c, reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+#define NNM_TRY(expr)              \
+    do {                           \
+        int _r = (expr);           \
+        if (_r != NNM_OK) return _r; \
+    } while (0)
+
+// Fill kernels are used instead of hipMemsetAsync in the iteration loop so the
+// loop can be captured into a HIP graph.
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 65535u * 16u) {
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+static int fill_u32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) {
+    if (!n) return NNM_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, p, n, v);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+static int fill_u64(nnm_ctx* c, uint64_t* p, uint64_t n, uint64_t v) {
+    if (!n) return NNM_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, p, n, v);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+// ================================================================ context
+
+extern "C" void nnm_default_params(nnm_params* p) {
+    if (!p) return;
+    p->K = 20;
+    p->rho = 1.0f;
+    p->rev_cap = 100;
+    p->max_iters = 0;
+    p->delta = 0.002f;
+    p->rule = NNM_RULE_ALL;
+    p->prec = NNM_PREC_FP32;
+    p->rerank = NNM_RERANK_AUTO;
+    p->slots = 64;
+    p->cut = 0;
+    p->seed = 1;
+}
+
+extern "C" int nnm_create(nnm_ctx** out, int device) {
+    if (!out) return NNM_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return NNM_ERR_HIP;
+    if (device < 0 || device >= ndev) return NNM_ERR_INVALID;
+    nnm_ctx* c = new nnm_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return NNM_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return NNM_ERR_HIP; }
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    hipEventCreate(&c->evj0);
+    hipEventCreate(&c->evj1);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (alloc_n(c, &c->counters, CNT_N) != NNM_OK) { delete c; return NNM_ERR_NOMEM; }
+    *out = c;
+    return NNM_OK;
+}
+
+extern "C" int nnm_destroy(nnm_ctx* c) {
+    if (!c) return NNM_ERR_INVALID;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& a : c->allocs) hipFree(a.first);
+    c->allocs.clear();
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipEventDestroy(c->evj0);
+    hipEventDestroy(c->evj1);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return NNM_OK;
+}
+
+extern "C" const char* nnm_last_error(const nnm_ctx* c) {
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+extern "C" int nnm_set_mem_cap(nnm_ctx* c, uint64_t bytes) {
+    if (!c) return NNM_ERR_INVALID;
+    c->mem_cap = bytes;
+    return NNM_OK;
+}
+
+extern "C" uint64_t nnm_mem_used(const nnm_ctx* c) { return c ? c->mem_used : 0; }
+
+static void free_graph(nnm_ctx* c) {
+    nnm_free(c, c->g_ids); nnm_free(c, c->g_dist); nnm_free(c, c->g_flag);
+    c->g_ids = nullptr; c->g_dist = nullptr; c->g_flag = nullptr;
+    nnm_free(c, c->c_ids); nnm_free(c, c->c_dist); nnm_free(c, c->c_flag); nnm_free(c, c->has_cpn);
+    c->c_ids = nullptr; c->c_dist = nullptr; c->c_flag = nullptr; c->has_cpn = nullptr;
+    c->K = 0; c->cut = 0; c->cpn_live = 0;
+}
+
+static void free_iter(nnm_ctx* c) {
+    void** ps[] = {(void**)&c->l_new, (void**)&c->l_old, (void**)&c->n_new, (void**)&c->n_old,
+                   (void**)&c->newmask, (void**)&c->rcnt, (void**)&c->roff, (void**)&c->rcur,
+                   (void**)&c->rbuf, (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->thr};
+    for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
+    c->cap_new = c->cap_old = c->rev_cap = c->slots_C = 0;
+}
+
+static void free_partition(nnm_ctx* c) {
+    void** ps[] = {(void**)&c->label, (void**)&c->vsub, (void**)&c->sub_begin, (void**)&c->sub_grp,
+                   (void**)&c->sub_built, (void**)&c->grp_begin, (void**)&c->grp_end,
+                   (void**)&c->grp_flags, (void**)&c->grp_active};
+    for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
+    c->nsub = c->ngrp = 0;
+    c->h_sub_begin.clear(); c->h_sub_group.clear(); c->h_sub_built.clear(); c->h_grp_first_sub.clear();
+}
+
+// ================================================================ data
+
+__global__ void k_to_bf16(const float* __restrict__ X, uint16_t* __restrict__ Xb, float* __restrict__ norms,
+                          uint64_t n, uint32_t ld) {
+    // one wave per row: round to nearest even bf16, norm of the rounded row
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    const uint32_t lane = threadIdx.x % NNM_WAVE;
+    if (row >= n) return;
+    float acc = 0.f;
+    for (uint32_t k = lane; k < ld; k += NNM_WAVE) {
+        float x = X[row * ld + k];
+        uint32_t u = __float_as_uint(x);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        uint16_t h = (uint16_t)(u >> 16);
+        Xb[row * ld + k] = h;
+        float r = __uint_as_float((uint32_t)h << 16);
+        acc = __builtin_fmaf(r, r, acc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, NNM_WAVE);
+    if (lane == 0) norms[row] = acc;
+}
+
+int nnm_ensure_bf16(nnm_ctx* c) {
+    if (c->Xb) return NNM_OK;
+    NNM_TRY(alloc_n(c, &c->Xb, c->n * c->ld));
+    NNM_TRY(alloc_n(c, &c->norms, c->n));
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream,
+                       c->X, c->Xb, c->norms, c->n, c->ld);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) {
+    if (!c || !X || n < 2 || d == 0 || d > 1024) return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_data: bad arguments");
+    if (n >= 0xFFFFFFFFull) return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_data: n must be < 2^32-1");
+    NNM_HIP(c, hipSetDevice(c->device));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    free_graph(c);
+    free_iter(c);
+    free_partition(c);
+    nnm_free(c, c->X); c->X = nullptr;
+    nnm_free(c, c->Xb); c->Xb = nullptr;
+    nnm_free(c, c->norms); c->norms = nullptr;
+    c->n = n;
+    c->d = d;
+    c->ld = (d + 31) / 32 * 32;
+    NNM_TRY(alloc_n(c, &c->X, n * c->ld));
+    if (c->ld != d) NNM_HIP(c, hipMemsetAsync(c->X, 0, n * c->ld * sizeof(float), c->stream));
+    NNM_HIP(c, hipMemcpy2DAsync(c->X, c->ld * sizeof(float), X, d * sizeof(float), d * sizeof(float), n,
+                                hipMemcpyHostToDevice, c->stream));
+    // bf16 exactness: integers of magnitude <= 256 are exact in bf16, and
+    // with d <= 128 every partial sum of the norm form stays below 2^24
+    int exact = (d <= 128) ? 1 : 0;
+    const uint64_t total = n * (uint64_t)d;
+    for (uint64_t i = 0; i < total && exact; ++i) {
+        float v = X[i];
+        if (!(v >= -256.f && v <= 256.f) || v != std::nearbyint(v)) exact = 0;
+    }
+    c->data_exact_bf16 = exact;
+    // default partition: one built subgraph, one group
+    uint32_t sb[2] = {0u, (uint32_t)n};
+    uint8_t bu[1] = {1};
+    uint32_t gr[1] = {0};
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    return nnm_set_partition(c, sb, bu, gr, 1);
+}
+
+// ================================================================ partition
+
+__global__ void k_fill_labels(uint32_t* __restrict__ label, uint32_t* __restrict__ vsub,
+                              const uint32_t* __restrict__ sub_begin, const uint8_t* __restrict__ built,
+                              uint32_t nsub, uint64_t n) {
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t lo = 0, hi = nsub;  // find s with sub_begin[s] <= v < sub_begin[s+1]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (sub_begin[mid] <= v) lo = mid; else hi = mid;
+    }
+    vsub[v] = lo;
+    label[v] = built[lo] ? lo : NNM_RAW;
+}
+
+extern "C" int nnm_set_partition(nnm_ctx* c, const uint32_t* sub_begin, const uint8_t* sub_built,
+                                 const uint32_t* sub_group, uint32_t nsub) {
+    if (!c || !sub_begin || !sub_built || !sub_group || nsub == 0)
+        return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: bad arguments");
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "nnm_set_partition: no data");
+    if (sub_begin[0] != 0 || sub_begin[nsub] != c->n)
+        return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: sub_begin must run from 0 to n");
+    for (uint32_t s = 0; s < nsub; ++s)
+        if (sub_begin[s + 1] <= sub_begin[s])
+            return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: empty or unordered subgraph");
+    std::vector<uint32_t> gb, ge, first;
+    for (uint32_t s = 0; s < nsub; ++s) {
+        if (s == 0 || sub_group[s] != sub_group[s - 1]) {
+            if (sub_group[s] != (uint32_t)gb.size())
+                return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: groups must be numbered 0.. in row order");
+            gb.push_back(sub_begin[s]);
+            ge.push_back(sub_begin[s + 1]);
+            first.push_back(s);
+        } else {
+            ge.back() = sub_begin[s + 1];
+        }
+    }
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    free_partition(c);
+    c->nsub = nsub;
+    c->ngrp = (uint32_t)gb.size();
+    c->h_sub_begin.assign(sub_begin, sub_begin + nsub + 1);
+    c->h_sub_built.assign(sub_built, sub_built + nsub);
+    c->h_sub_group.assign(sub_group, sub_group + nsub);
+    c->h_grp_first_sub = first;
+    NNM_TRY(alloc_n(c, &c->label, c->n));
+    NNM_TRY(alloc_n(c, &c->vsub, c->n));
+    NNM_TRY(alloc_n(c, &c->sub_begin, nsub + 1));
+    NNM_TRY(alloc_n(c, &c->sub_grp, nsub));
+    NNM_TRY(alloc_n(c, &c->sub_built, nsub));
+    NNM_TRY(alloc_n(c, &c->grp_begin, c->ngrp));
+    NNM_TRY(alloc_n(c, &c->grp_end, c->ngrp));
+    NNM_TRY(alloc_n(c, &c->grp_flags, c->ngrp));
+    NNM_TRY(alloc_n(c, &c->grp_active, c->ngrp));
+    std::vector<uint8_t> built01(nsub);
+    for (uint32_t s = 0; s < nsub; ++s) built01[s] = sub_built[s] ? 1 : 0;
+    NNM_HIP(c, hipMemcpyAsync(c->sub_begin, sub_begin, (nsub + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->sub_grp, sub_group, nsub * 4, hipMemcpyHostToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->sub_built, built01.data(), nsub, hipMemcpyHostToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->grp_begin, gb.data(), gb.size() * 4, hipMemcpyHostToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->grp_end, ge.data(), ge.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_fill_labels, dim3(grid_for(c->n, 256, 1u << 30)), dim3(256), 0, c->stream, c->label,
+                       c->vsub, c->sub_begin, c->sub_built, nsub, c->n);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    return NNM_OK;
+}
+
+// ================================================================ graph kernels
+
+// Write lanes [0, K) of a row: (key, flag) per lane, sorted by key first.
+// Ids inside a row are distinct, so the flag of each sorted entry is found
+// by id.
+__device__ inline void wave_write_sorted_row(uint64_t key, uint32_t flag, uint32_t K,
+                                             uint32_t* __restrict__ ids, float* __restrict__ dist,
+                                             uint8_t* __restrict__ flg) {
+    const uint32_t lane = nnm_lane();
+    uint64_t s = nnm_wave_sort64(lane < K ? key : NNM_EMPTY_KEY);
+    uint32_t sid = nnm_key_id(s);
+    uint32_t myf = 1;
+    for (uint32_t p = 0; p < K; ++p) {
+        uint32_t pid = (uint32_t)__shfl(key, (int)p, NNM_WAVE);
+        uint32_t pf = __shfl(flag, (int)p, NNM_WAVE);
+        if (pid == sid) myf = pf;
+    }
+    if (lane < K) {
+        ids[lane] = sid;
+        dist[lane] = nnm_key_dist(s);
+        flg[lane] = (uint8_t)myf;
+    }
+}
+
+// K random distinct neighbours inside the row's group, self excluded.
+// Draws K+1 values like getRndSds(rng, K0+1, seeds, N) (src/kgraph.cpp:71).
+__device__ inline void wave_random_row(const float* __restrict__ X, uint32_t ld, uint64_t v, uint32_t gb,
+                                       uint32_t gsize, uint32_t K, uint64_t seed, uint32_t* __restrict__ ids,
+                                       float* __restrict__ dist, uint8_t* __restrict__ flg, uint64_t* nd) {
+    const uint32_t lane = nnm_lane();
+    uint32_t r = nnm_wave_rnd_sds(seed, v, K + 1, gsize);
+    uint32_t id = gb + r;
+    bool ok = lane < K + 1 && id != (uint32_t)v;
+    uint64_t m = nnm_ballot(ok);
+    uint32_t rank = nnm_mask_rank(m);
+    bool take = ok && rank < K;
+    // move the taken ids to lanes [0, K)
+    uint32_t src_lane = 0;
+    // lane t < K receives the id of the lane whose rank is t
+    uint64_t mm = m;
+    for (uint32_t t = 0; t < K; ++t) {
+        uint32_t l = (uint32_t)__ffsll((long long)mm) - 1;
+        mm &= mm - 1;
+        if (lane == t) src_lane = l;
+    }
+    (void)take;
+    uint32_t myid = (uint32_t)__shfl((int)id, (int)src_lane, NNM_WAVE);
+    float dd = 0.f;
+    if (lane < K) dd = nnm_l2_rows(X + v * ld, X + (uint64_t)myid * ld, ld);
+    uint64_t key = nnm_key(dd, myid);
+    if (lane < K) *nd += 1;
+    wave_write_sorted_row(key, 1u, K, ids, dist, flg);
+}
+
+__global__ void k_random_init(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K,
+                              const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_grp,
+                              const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
+                              uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                              uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    if (v >= n) return;
+    const uint32_t g = sub_grp[vsub[v]];
+    const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
+    uint64_t nd = 0;
+    wave_random_row(X, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd);
+    uint32_t tot = nnm_wave_sum32((uint32_t)nd);
+    if (nnm_lane() == 0) atomicAdd((unsigned long long*)&counters[CNT_DISTS], (unsigned long long)tot);
+}
+
+// Upload fix-up: optional distance recompute, optional flags, sort rows.
+__global__ void k_set_graph(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K, int recompute,
+                            int have_flags, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                            uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters,
+                            uint32_t* __restrict__ bad) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    if (v >= n) return;
+    const uint32_t lane = nnm_lane();
+    uint32_t id = 0;
+    float dd = 0.f;
+    uint32_t f = 1;
+    if (lane < K) {
+        id = g_ids[v * K + lane];
+        if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
+        dd = recompute ? nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld) : g_dist[v * K + lane];
+        f = have_flags ? g_flag[v * K + lane] : 1u;
+    }
+    if (recompute) {
+        uint32_t tot = nnm_wave_sum32(lane < K ? 1u : 0u);
+        if (lane == 0) atomicAdd((unsigned long long*)&counters[CNT_DISTS], (unsigned long long)tot);
+    }
+    wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+}
+
+// Merge set-up, one wave per row.
+__global__ void k_merge_init(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K, uint32_t cut,
+                             const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_begin,
+                             const uint32_t* __restrict__ sub_grp, const uint8_t* __restrict__ sub_built,
+                             const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
+                             uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                             uint8_t* __restrict__ g_flag, uint32_t* __restrict__ c_ids,
+                             float* __restrict__ c_dist, uint8_t* __restrict__ c_flag,
+                             uint8_t* __restrict__ has_cpn, uint64_t* __restrict__ counters) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    if (v >= n) return;
+    const uint32_t lane = nnm_lane();
+    const uint32_t s = vsub[v];
+    const uint32_t g = sub_grp[s];
+    const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
+    uint32_t* ids = g_ids + v * K;
+    float* dist = g_dist + v * K;
+    uint8_t* flg = g_flag + v * K;
+    uint64_t nd = 0;
+    if (!sub_built[s]) {
+        if (lane == 0) has_cpn[v] = 0;
+        wave_random_row(X, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd);
+    } else {
+        const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
+        const uint32_t remain = K - cut;
+        uint32_t pid = 0, pf = 0;
+        float pd = 0.f;
+        if (lane < K) { pid = ids[lane]; pd = dist[lane]; pf = flg[lane]; }
+        // park the far entries (cpnGraph)
+        if (lane >= remain && lane < K) {
+            c_ids[v * cut + (lane - remain)] = pid;
+            c_dist[v * cut + (lane - remain)] = pd;
+            c_flag[v * cut + (lane - remain)] = (uint8_t)pf;
+        }
+        if (lane == 0) has_cpn[v] = 1;
+        const uint32_t others = gsize - ssize;
+        uint32_t fresh_n = 0;
+        uint32_t cand = 0;
+        bool fresh = false;
+        if (others > 0) {
+            uint32_t draw = 2 * cut < others ? 2 * cut : others;
+            uint32_t r = nnm_wave_rnd_sds(seed, v, draw, others);
+            cand = r < (sb - gb) ? gb + r : gb + r + ssize;
+            bool dup = false;
+            for (uint32_t p = 0; p < K; ++p) {
+                uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
+                dup |= (q == cand);
+            }
+            fresh = lane < draw && !dup;
+            uint64_t m = nnm_ballot(fresh);
+            uint32_t rank = nnm_mask_rank(m);
+            fresh = fresh && rank < cut;
+            fresh_n = (uint32_t)__popcll(m) < cut ? (uint32_t)__popcll(m) : cut;
+            // route fresh candidate of rank j to lane remain + j
+            uint32_t src = 0;
+            uint64_t mm = m;
+            for (uint32_t t = 0; t < fresh_n; ++t) {
+                uint32_t l = (uint32_t)__ffsll((long long)mm) - 1;
+                mm &= mm - 1;
+                if (lane == remain + t) src = l;
+            }
+            uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
+            if (lane >= remain && lane < remain + fresh_n) {
+                pid = got;
+                pd = nnm_l2_rows(X + (uint64_t)got * ld, X + v * ld, ld);
+                pf = 1;
+                nd += 1;
+            }
+        }
+        (void)fresh;
+        wave_write_sorted_row(nnm_key(pd, pid), pf, K, ids, dist, flg);
+    }
+    uint32_t tot = nnm_wave_sum32((uint32_t)nd);
+    if (lane == 0) atomicAdd((unsigned long long*)&counters[CNT_DISTS], (unsigned long long)tot);
+}
+
+// merge2one: pool (K, sorted) + parked (cut, sorted) -> first K by (dist, id).
+__global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, const uint8_t* __restrict__ has_cpn,
+                               uint32_t* __restrict__ g_ids, float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
+                               const uint32_t* __restrict__ c_ids, const float* __restrict__ c_dist,
+                               const uint8_t* __restrict__ c_flag) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    if (v >= n || !has_cpn[v]) return;
+    const uint32_t lane = nnm_lane();
+    uint64_t key = NNM_EMPTY_KEY;
+    uint32_t f = 0;
+    uint32_t id = 0xFFFFFFFFu;
+    if (lane < K) {
+        id = g_ids[v * K + lane];
+        key = nnm_key(g_dist[v * K + lane], id);
+        f = g_flag[v * K + lane];
+    } else if (lane < K + cut) {
+        id = c_ids[v * cut + lane - K];
+        key = nnm_key(c_dist[v * cut + lane - K], id);
+        f = c_flag[v * cut + lane - K];
+    }
+    if (dedup && lane >= K && lane < K + cut) {
+        for (uint32_t p = 0; p < K; ++p) {
+            uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
+            if (q == id) key = NNM_EMPTY_KEY;
+        }
+    }
+    uint64_t s = nnm_wave_sort64(key);
+    uint32_t sid = nnm_key_id(s);
+    uint32_t myf = 0;
+    for (uint32_t p = 0; p < K + cut; ++p) {
+        uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
+        uint32_t qf = __shfl(f, (int)p, NNM_WAVE);
+        if (q == sid) myf = qf;
+    }
+    if (lane < K) {
+        g_ids[v * K + lane] = sid;
+        g_dist[v * K + lane] = nnm_key_dist(s);
+        g_flag[v * K + lane] = (uint8_t)myf;
+    }
+}
+
+__global__ void k_pair_dists(const float* __restrict__ X, uint32_t ld, const uint32_t* __restrict__ a,
+                             const uint32_t* __restrict__ b, uint64_t m, float* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    out[i] = nnm_l2_rows(X + (uint64_t)a[i] * ld, X + (uint64_t)b[i] * ld, ld);
+}
+
+// ================================================================ graph API
+
+static int ensure_graph(nnm_ctx* c, uint32_t K) {
+    if (K == 0 || K > 32) return nnm_fail(c, NNM_ERR_INVALID, "K must be in [1, 32]");
+    if (c->K == K && c->g_ids) return NNM_OK;
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    free_graph(c);
+    free_iter(c);
+    c->K = K;
+    NNM_TRY(alloc_n(c, &c->g_ids, c->n * K));
+    NNM_TRY(alloc_n(c, &c->g_dist, c->n * K));
+    NNM_TRY(alloc_n(c, &c->g_flag, c->n * K));
+    return NNM_OK;
+}
+
+static int check_groups(nnm_ctx* c, uint32_t K) {
+    for (uint32_t g = 0; g < c->ngrp; ++g) {
+        uint32_t s0 = c->h_grp_first_sub[g];
+        uint32_t s1 = (g + 1 < c->ngrp) ? c->h_grp_first_sub[g + 1] : c->nsub;
+        uint32_t size = c->h_sub_begin[s1] - c->h_sub_begin[s0];
+        if (size < K + 1)
+            return nnm_fail(c, NNM_ERR_INVALID, "every group needs at least K+1 rows");
+    }
+    return NNM_OK;
+}
+
+static void stats_reset(nnm_stats* st) {
+    if (st) std::memset(st, 0, sizeof(*st));
+}
+
+static int read_counters(nnm_ctx* c, uint64_t* h) {
+    NNM_HIP(c, hipMemcpyAsync(h, c->counters, CNT_N * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    return NNM_OK;
+}
+
+extern "C" int nnm_graph_random_init(nnm_ctx* c, uint32_t K, uint64_t seed, nnm_stats* st) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    stats_reset(st);
+    NNM_TRY(check_groups(c, K));
+    NNM_TRY(ensure_graph(c, K));
+    NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_random_init, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld,
+                       c->n, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
+                       c->g_flag, c->counters);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    uint64_t h[CNT_N];
+    NNM_TRY(read_counters(c, h));
+    if (st) {
+        st->init_dists = h[CNT_DISTS];
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        st->secs = ms * 1e-3;
+    }
+    return NNM_OK;
+}
+
+extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist, const uint8_t* flags, uint32_t K,
+                             nnm_stats* st) {
+    if (!c || !ids) return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_graph: bad arguments");
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    stats_reset(st);
+    NNM_TRY(ensure_graph(c, K));
+    const uint64_t nk = c->n * K;
+    NNM_HIP(c, hipMemcpyAsync(c->g_ids, ids, nk * 4, hipMemcpyHostToDevice, c->stream));
+    if (dist) NNM_HIP(c, hipMemcpyAsync(c->g_dist, dist, nk * 4, hipMemcpyHostToDevice, c->stream));
+    if (flags) NNM_HIP(c, hipMemcpyAsync(c->g_flag, flags, nk, hipMemcpyHostToDevice, c->stream));
+    NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
+    uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_set_graph, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
+                       K, dist ? 0 : 1, flags ? 1 : 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    uint64_t h[CNT_N];
+    NNM_TRY(read_counters(c, h));
+    c->cpn_live = 0;
+    if (st) {
+        st->init_dists = h[CNT_DISTS];
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        st->secs = ms * 1e-3;
+    }
+    if ((uint32_t)h[CNT_TOTAL_REV] != 0)
+        return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_graph: ids out of range or self loops");
+    return NNM_OK;
+}
+
+extern "C" int nnm_get_graph(nnm_ctx* c, uint32_t* ids, float* dist, uint8_t* flags) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no graph");
+    const uint64_t nk = c->n * c->K;
+    if (ids) NNM_HIP(c, hipMemcpyAsync(ids, c->g_ids, nk * 4, hipMemcpyDeviceToHost, c->stream));
+    if (dist) NNM_HIP(c, hipMemcpyAsync(dist, c->g_dist, nk * 4, hipMemcpyDeviceToHost, c->stream));
+    if (flags) NNM_HIP(c, hipMemcpyAsync(flags, c->g_flag, nk, hipMemcpyDeviceToHost, c->stream));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    return NNM_OK;
+}
+
+extern "C" int nnm_pair_dists(nnm_ctx* c, const uint32_t* a, const uint32_t* b, uint64_t m, float* out) {
+    if (!c || !a || !b || !out) return nnm_fail(c, NNM_ERR_INVALID, "nnm_pair_dists: bad arguments");
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    if (m == 0) return NNM_OK;
+    for (uint64_t i = 0; i < m; ++i)
+        if (a[i] >= c->n || b[i] >= c->n) return nnm_fail(c, NNM_ERR_INVALID, "nnm_pair_dists: row out of range");
+    uint32_t *da = nullptr, *db = nullptr;
+    float* dout = nullptr;
+    NNM_TRY(alloc_n(c, &da, m));
+    NNM_TRY(alloc_n(c, &db, m));
+    NNM_TRY(alloc_n(c, &dout, m));
+    NNM_HIP(c, hipMemcpyAsync(da, a, m * 4, hipMemcpyHostToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(db, b, m * 4, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_pair_dists, dim3(grid_for(m, 256, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, da,
+                       db, m, dout);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipMemcpyAsync(out, dout, m * 4, hipMemcpyDeviceToHost, c->stream));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    nnm_free(c, da);
+    nnm_free(c, db);
+    nnm_free(c, dout);
+    return NNM_OK;
+}
+
+// ================================================================ merges
+
+extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
+    if (!c || !p) return NNM_ERR_INVALID;
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    stats_reset(st);
+    const uint32_t K = p->K;
+    const uint32_t cut = p->cut ? p->cut : K / 2;
+    if (cut == 0 || cut >= K) return nnm_fail(c, NNM_ERR_INVALID, "cut must be in [1, K)");
+    if (K + cut > NNM_WAVE) return nnm_fail(c, NNM_ERR_INVALID, "K + cut must be <= 64");
+    NNM_TRY(check_groups(c, K));
+    bool any_built = false;
+    for (uint8_t b : c->h_sub_built) any_built |= (b != 0);
+    if (any_built && (!c->g_ids || c->K != K))
+        return nnm_fail(c, NNM_ERR_STATE, "nnm_merge_init: built subgraphs need a graph with the same K");
+    NNM_TRY(ensure_graph(c, K));
+    if (c->cut != cut || !c->c_ids) {
+        nnm_free(c, c->c_ids); nnm_free(c, c->c_dist); nnm_free(c, c->c_flag); nnm_free(c, c->has_cpn);
+        c->c_ids = nullptr; c->c_dist = nullptr; c->c_flag = nullptr; c->has_cpn = nullptr;
+        c->cut = cut;
+        NNM_TRY(alloc_n(c, &c->c_ids, c->n * cut));
+        NNM_TRY(alloc_n(c, &c->c_dist, c->n * cut));
+        NNM_TRY(alloc_n(c, &c->c_flag, c->n * cut));
+        NNM_TRY(alloc_n(c, &c->has_cpn, c->n));
+    }
+    NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_merge_init, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
+                       K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin, c->grp_end,
+                       p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
+                       c->c_flag, c->has_cpn, c->counters);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    uint64_t h[CNT_N];
+    NNM_TRY(read_counters(c, h));
+    c->cpn_live = 1;
+    if (st) {
+        st->init_dists = h[CNT_DISTS];
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        st->secs = ms * 1e-3;
+    }
+    return NNM_OK;
+}
+
+extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!c->cpn_live) return nnm_fail(c, NNM_ERR_STATE, "nnm_merge_finish: no merge in progress");
+    hipLaunchKernelGGL(k_merge_finish, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->n, c->K,
+                       c->cut, dedup, c->has_cpn, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist, c->c_flag);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    c->cpn_live = 0;
+    return NNM_OK;
+}
+
+// ================================================================ list build
+
+// Forward lists, new-sample mask, reverse counts.  One thread per row.
+__global__ void k_lb_count(uint64_t n, uint32_t K, uint32_t S, const uint32_t* __restrict__ g_ids,
+                           const float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
+                           uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
+                           uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
+                           uint32_t* __restrict__ newmask, uint32_t* __restrict__ rcnt) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t* ids = g_ids + v * K;
+    const float* dd = g_dist + v * K;
+    uint8_t* fl = g_flag + v * K;
+    uint32_t nn = 0, no = 0, mask = 0;
+    for (uint32_t e = 0; e < K; ++e) {
+        const uint32_t u = ids[e];
+        const float de = dd[e];
+        const float kth = g_dist[(uint64_t)u * K + K - 1];
+        if (fl[e]) {
+            if (nn >= S) continue;  // forward sampling: nearest S new entries
+            l_new[v * cap_new + nn++] = u;
+            mask |= 1u << e;
+            fl[e] = 0;
+            if (kth < de) atomicAdd(&rcnt[u], 1u);
+        } else {
+            l_old[v * cap_old + no++] = u;
+            if (kth < de) atomicAdd(&rcnt[n + u], 1u);
+        }
+    }
+    n_new[v] = nn;
+    n_old[v] = no;
+    newmask[v] = mask;
+}
+
+// Reverse entries into CSR buckets.
+__global__ void k_lb_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g_ids,
+                          const float* __restrict__ g_dist, const uint8_t* __restrict__ g_flag,
+                          const uint32_t* __restrict__ newmask, const uint32_t* __restrict__ roff,
+                          uint32_t* __restrict__ rcur, uint32_t* __restrict__ rbuf) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t* ids = g_ids + v * K;
+    const float* dd = g_dist + v * K;
+    const uint8_t* fl = g_flag + v * K;
+    const uint32_t mask = newmask[v];
+    for (uint32_t e = 0; e < K; ++e) {
+        const bool is_new = (mask >> e) & 1u;
+        const bool is_old = !is_new && fl[e] == 0;
+        if (!is_new && !is_old) continue;
+        const uint32_t u = ids[e];
+        if (!(g_dist[(uint64_t)u * K + K - 1] < dd[e])) continue;
+        const uint64_t b = is_new ? u : n + u;
+        uint32_t slot = atomicAdd(&rcur[b], 1u);
+        rbuf[roff[b] + slot] = (uint32_t)v;
+    }
+}
+
+// Append the reverse entries to the candidate lists.  More than cap entries:
+// keep the cap entries with the smallest random key (a seeded, order-free
+// stand-in for random_shuffle + resize, src/abstractmerge.cpp:149-159).
+__device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t cap,
+                                               uint32_t salt, uint32_t* __restrict__ dst) {
+    const uint32_t lane = nnm_lane();
+    if (cnt <= cap) {
+        for (uint32_t i = lane; i < cnt; i += NNM_WAVE) dst[i] = src[i];
+        return cnt;
+    }
+    // find the cap-th smallest 64-bit key (hash << 32 | id), bit by bit
+    uint64_t res = 0;
+    for (int b = 63; b >= 0; --b) {
+        const uint64_t trial = res + (1ull << b);
+        uint32_t below = 0;
+        for (uint32_t i = lane; i < cnt; i += NNM_WAVE) {
+            const uint32_t id = src[i];
+            const uint64_t key = ((uint64_t)nnm_hash32(id, salt) << 32) | id;
+            below += key < trial ? 1u : 0u;
+        }
+        below = nnm_wave_sum32(below);
+        if (below < cap) res = trial;
+    }
+    // res is the cap-th smallest key: keep keys <= res (exactly cap of them)
+    uint32_t out = 0;
+    for (uint32_t base = 0; base < cnt; base += NNM_WAVE) {
+        const uint32_t i = base + lane;
+        bool keep = false;
+        uint32_t id = 0;
+        if (i < cnt) {
+            id = src[i];
+            const uint64_t key = ((uint64_t)nnm_hash32(id, salt) << 32) | id;
+            keep = key <= res;
+        }
+        const uint64_t m = nnm_ballot(keep);
+        if (keep) dst[out + nnm_mask_rank(m)] = id;
+        out += (uint32_t)__popcll(m);
+    }
+    return out;
+}
+
+__global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ roff,
+                            const uint32_t* __restrict__ rbuf, uint32_t cap, uint32_t salt,
+                            uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
+                            uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
+                            uint64_t* __restrict__ counters) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    if (v >= n) return;
+    const uint32_t lane = nnm_lane();
+    const uint32_t fn = n_new[v], fo = n_old[v];
+    const uint32_t an = wave_append_reverse(rbuf + roff[v], rcnt[v], cap, nnm_hash32((uint32_t)v, salt),
+                                            l_new + v * cap_new + fn);
+    const uint32_t ao = wave_append_reverse(rbuf + roff[n + v], rcnt[n + v], cap,
+                                            nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u), l_old + v * cap_old + fo);
+    if (lane == 0) {
+        n_new[v] = fn + an;
+        n_old[v] = fo + ao;
+    }
+}
+
+// Exclusive scan of 2n counts (three passes: tile sums, scan of sums, add).
+#define SCAN_TILE 2048
+__global__ void k_scan_tiles(const uint32_t* __restrict__ in, uint64_t m, uint32_t* __restrict__ tile_sum) {
+    __shared__ uint32_t red[256];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint32_t s = 0;
+    for (uint32_t i = threadIdx.x; i < SCAN_TILE; i += 256)
+        if (base + i < m) s += in[base + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = red[0];
+}
+
+__global__ void k_scan_sums(uint32_t* __restrict__ tile_sum, uint32_t ntiles, uint32_t* __restrict__ total) {
+    // single block of 1024 threads, sequential chunks
+    __shared__ uint32_t buf[1024];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += 1024) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < ntiles ? tile_sum[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t o = 1; o < 1024; o <<= 1) {
+            uint32_t t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < ntiles) tile_sum[i] = carry + buf[threadIdx.x] - v;
+        carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_apply(const uint32_t* __restrict__ in, uint64_t m, const uint32_t* __restrict__ tile_off,
+                             uint32_t* __restrict__ out) {
+    __shared__ uint32_t wsum[256];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    // each thread owns 8 consecutive elements
+    const uint32_t t = threadIdx.x;
+    uint32_t loc[8];
+    uint32_t s = 0;
+    for (int j = 0; j < 8; ++j) {
+        uint64_t i = base + t * 8 + j;
+        loc[j] = i < m ? in[i] : 0;
+        s += loc[j];
+    }
+    wsum[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 256; o <<= 1) {
+        uint32_t x = t >= o ? wsum[t - o] : 0;
+        __syncthreads();
+        wsum[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = tile_off[blockIdx.x] + wsum[t] - s;
+    for (int j = 0; j < 8; ++j) {
+        uint64_t i = base + t * 8 + j;
+        if (i < m) out[i] = run;
+        run += loc[j];
+    }
+}
+
+// ================================================================ update (merge slots into lists)
+
+// One wave per row: pool (lanes < K) and slot candidates (lanes < C) merged
+// by rank into the sorted list.  A slot candidate enters when it is not
+// already in the list and its (dist, id) key is below the k-th entry; that is
+// UpdateKnnList's rule applied to the whole batch of the iteration.  Slots
+// are reset to empty.  With rerank, candidates are recomputed in exact FP32
+// before ranking.
+__global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X, uint32_t ld,
+                         int rerank, float rerank_slack, uint32_t* __restrict__ g_ids,
+                         float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
+                         uint64_t* __restrict__ counters) {
+    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    if (v >= n) return;
+    const uint32_t lane = nnm_lane();
+    uint64_t pkey = NNM_EMPTY_KEY;
+    uint32_t pid = 0xFFFFFFFFu, pf = 0;
+    if (lane < K) {
+        pid = g_ids[v * K + lane];
+        pkey = nnm_key(g_dist[v * K + lane], pid);
+        pf = g_flag[v * K + lane];
+    }
+    uint64_t skey = NNM_EMPTY_KEY;
+    if (lane < C) {
+        skey = slots[v * C + lane];
+        if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
+    }
+    const uint64_t kth_key = __shfl(pkey, (int)(K - 1), NNM_WAVE);
+    bool valid = skey != NNM_EMPTY_KEY;
+    const uint32_t sid = nnm_key_id(skey);
+    if (__ballot(valid)) {
+        for (uint32_t p = 0; p < K; ++p) {
+            uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
+            valid = valid && (q != sid);
+        }
+        uint32_t rr = 0;
+        if (rerank) {
+            // approximate keys carry a bounded error: re-rank every candidate
+            // that can still cross the k-th entry
+            const float kd = nnm_key_dist(kth_key);
+            valid = valid && (nnm_key_dist(skey) <= kd + rerank_slack);
+            if (valid) {
+                float ex = nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
+                skey = nnm_key(ex, sid);
+                rr = 1;
+            }
+            rr = nnm_wave_sum32(rr);
+        }
+        valid = valid && (skey < kth_key);
+        const uint64_t vm = __ballot(valid);
+        uint32_t pos_p = lane, pos_s = 0;
+        // rank against the other valid candidates
+        uint64_t mm = vm;
+        while (mm) {
+            const uint32_t e = (uint32_t)__ffsll((long long)mm) - 1;
+            mm &= mm - 1;
+            const uint64_t ek = __shfl(skey, (int)e, NNM_WAVE);
+            pos_p += (ek < pkey) ? 1u : 0u;
+            pos_s += (ek < skey) ? 1u : 0u;
+        }
+        // rank against the pool
+        for (uint32_t p = 0; p < K; ++p) {
+            const uint64_t q = __shfl(pkey, (int)p, NNM_WAVE);
+            pos_s += (q < skey) ? 1u : 0u;
+        }
+        const bool ins = valid && pos_s < K;
+        const bool keep = lane < K && pos_p < K;
+        if (keep && pos_p != lane) {
+            g_ids[v * K + pos_p] = pid;
+            g_dist[v * K + pos_p] = nnm_key_dist(pkey);
+            g_flag[v * K + pos_p] = (uint8_t)pf;
+        }
+        if (ins) {
+            g_ids[v * K + pos_s] = sid;
+            g_dist[v * K + pos_s] = nnm_key_dist(skey);
+            g_flag[v * K + pos_s] = 1;
+        }
+        const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
+        const uint32_t n_flag = (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
+        if (lane == 0) {
+            if (n_ins) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)n_ins);
+            atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)n_flag);
+            if (rr) atomicAdd((unsigned long long*)&counters[CNT_RERANKED], (unsigned long long)rr);
+        }
+    } else {
+        const uint32_t n_flag = (uint32_t)__popcll(__ballot(lane < K && pf));
+        if (lane == 0 && n_flag) atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)n_flag);
+    }
+}
+
+// ================================================================ NN-Descent
+
+int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
+    const uint32_t K = c->K;
+    uint32_t S = (uint32_t)std::lround((double)p->rho * K);
+    S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
+    const uint32_t R = p->rev_cap;
+    const uint32_t cap_new = S + R, cap_old = K + R;
+    const uint32_t C = p->slots ? p->slots : 64;
+    if (c->l_new && c->cap_new == cap_new && c->cap_old == cap_old && c->slots_C == C) return NNM_OK;
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    free_iter(c);
+    c->cap_new = cap_new;
+    c->cap_old = cap_old;
+    c->rev_cap = R;
+    c->slots_C = C;
+    const uint64_t n = c->n;
+    NNM_TRY(alloc_n(c, &c->l_new, n * cap_new));
+    NNM_TRY(alloc_n(c, &c->l_old, n * cap_old));
+    NNM_TRY(alloc_n(c, &c->n_new, n));
+    NNM_TRY(alloc_n(c, &c->n_old, n));
+    NNM_TRY(alloc_n(c, &c->newmask, n));
+    NNM_TRY(alloc_n(c, &c->rcnt, 2 * n));
+    NNM_TRY(alloc_n(c, &c->roff, 2 * n + 1));
+    NNM_TRY(alloc_n(c, &c->rcur, 2 * n));
+    NNM_TRY(alloc_n(c, &c->rbuf, n * K));
+    NNM_TRY(alloc_n(c, &c->scan_tmp, (2 * n + SCAN_TILE - 1) / SCAN_TILE + 1));
+    NNM_TRY(alloc_n(c, &c->slots, n * C));
+    NNM_TRY(fill_u64(c, c->slots, n * C, NNM_EMPTY_KEY));
+    return NNM_OK;
+}
+
+static int validate_params(nnm_ctx* c, const nnm_params* p) {
+    if (!p) return nnm_fail(c, NNM_ERR_INVALID, "null params");
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    if (!c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no graph");
+    if (p->K != c->K) return nnm_fail(c, NNM_ERR_INVALID, "params K differs from the graph's K");
+    if (!(p->rho > 0.f && p->rho <= 1.f)) return nnm_fail(c, NNM_ERR_INVALID, "rho must be in (0, 1]");
+    if (p->rev_cap == 0 || p->rev_cap > 4096) return nnm_fail(c, NNM_ERR_INVALID, "rev_cap must be in [1, 4096]");
+    const uint32_t C = p->slots ? p->slots : 64;
+    if (C < 16 || C > 64 || (C & (C - 1))) return nnm_fail(c, NNM_ERR_INVALID, "slots must be 16, 32 or 64");
+    if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
+    if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
+    if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
+    return NNM_OK;
+}
+
+// One iteration; leaves the flag count in counters[CNT_FLAGS].
+static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double* join_ms) {
+    const uint64_t n = c->n;
+    const uint32_t K = c->K;
+    uint32_t S = (uint32_t)std::lround((double)p->rho * K);
+    S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
+    const uint64_t iseed = nnm_rand(p->seed, 0xC0FFEEull, iter);
+    const uint32_t salt = (uint32_t)iseed, salt2 = (uint32_t)(iseed >> 32);
+    NNM_TRY(fill_u32(c, c->rcnt, 2 * n, 0));
+    NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
+    hipLaunchKernelGGL(k_lb_count, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, S, c->g_ids,
+                       c->g_dist, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
+                       c->newmask, c->rcnt);
+    NNM_LAUNCH_CHECK(c);
+    const uint64_t m = 2 * n;
+    const uint32_t ntiles = (uint32_t)((m + SCAN_TILE - 1) / SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, c->stream, c->rcnt, m, c->scan_tmp);
+    NNM_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, c->scan_tmp, ntiles, c->roff + m);
+    NNM_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, c->stream, c->rcnt, m, c->scan_tmp, c->roff);
+    NNM_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_lb_fill, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_ids,
+                       c->g_dist, c->g_flag, c->newmask, c->roff, c->rcur, c->rbuf);
+    NNM_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_lb_select, dim3(grid_for(n, 4, 1u << 30)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
+                       c->rbuf, c->rev_cap, salt2, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
+                       c->counters);
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
+    NNM_TRY(nnm_launch_join(c, p, iter, salt));
+    NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
+    int rerank = 0;
+    float slack = 0.f;
+    if (p->prec != NNM_PREC_FP32) {
+        rerank = (p->rerank == NNM_RERANK_ALWAYS) ||
+                 (p->rerank == NNM_RERANK_AUTO && !(p->prec == NNM_PREC_BF16 && c->data_exact_bf16));
+        slack = 0.f;
+    }
+    hipLaunchKernelGGL(k_update, dim3(grid_for(n, 4, 1u << 30)), dim3(256), 0, c->stream, n, K, c->slots_C, c->X,
+                       c->ld, rerank, slack, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
+    NNM_LAUNCH_CHECK(c);
+    if (join_ms) {
+        NNM_HIP(c, hipEventSynchronize(c->evj1));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->evj0, c->evj1);
+        *join_ms += ms;
+    }
+    return NNM_OK;
+}
+
+static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool single, nnm_stats* st) {
+    NNM_TRY(validate_params(c, p));
+    NNM_TRY(nnm_ensure_iter_buffers(c, p));
+    if (p->prec == NNM_PREC_BF16) NNM_TRY(nnm_ensure_bf16(c));
+    stats_reset(st);
+    NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    double join_ms = 0.0;
+    uint32_t it = 0;
+    int converged = 0;
+    double last_delta = 1.0;
+    uint64_t prev[CNT_N] = {0};
+    for (;;) {
+        const uint32_t iter = single ? only_iter : it;
+        NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
+        NNM_TRY(fill_u64(c, c->counters + CNT_WORK, 1, 0));
+        NNM_TRY(run_iteration(c, p, iter, &join_ms));
+        uint64_t h[CNT_N];
+        NNM_TRY(read_counters(c, h));
+        ++it;
+        last_delta = (double)h[CNT_FLAGS] / ((double)c->n * c->K);
+        (void)prev;
+        if (last_delta < p->delta) { converged = 1; break; }
+        if (single) break;
+        if (p->max_iters && it >= p->max_iters) break;
+    }
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    uint64_t h[CNT_N];
+    NNM_TRY(read_counters(c, h));
+    if (st) {
+        st->iters = it;
+        st->converged = converged;
+        st->dists = h[CNT_DISTS];
+        st->updates = h[CNT_UPDATES];
+        st->reranked = h[CNT_RERANKED];
+        st->list_entries = h[CNT_LIST_ENTRIES];
+        st->last_delta = last_delta;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        st->secs = ms * 1e-3;
+        st->join_secs = join_ms * 1e-3;
+    }
+    return NNM_OK;
+}
+
+extern "C" int nnm_nndescent(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
+    if (!c) return NNM_ERR_INVALID;
+    return descent(c, p, 0, false, st);
+}
+
+extern "C" int nnm_local_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, nnm_stats* st) {
+    if (!c) return NNM_ERR_INVALID;
+    return descent(c, p, iter, true, st);
+}

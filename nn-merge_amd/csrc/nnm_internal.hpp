@@ -1,0 +1,256 @@
+// Internal state and device helpers of the gfx950 NN-Descent library.
+//
+// Device layout (structure of arrays, row-major per vertex):
+//   X      n x ld  float32, rows zero-padded to ld (multiple of 64) so every
+//                  row starts on a 256-byte boundary and padding adds 0 to L2
+//   g_ids  n x K   uint32  neighbour ids, each row sorted by (dist, id)
+//   g_dist n x K   float32 squared L2
+//   g_flag n x K   uint8   1 = entry is new (not yet joined)
+//   label  n       uint32  subgraph label: s for rows of built subgraph s,
+//                          NNM_RAW for raw rows
+//   slots  n x C   uint64  per-iteration update slots, key = dist bits << 32 | id
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/nnmerge.h"
+
+#define NNM_WAVE 64
+#define NNM_MAX_K 64
+
+// Slot keys: squared distances are >= 0, so their IEEE bits order like the
+// values; the low word breaks ties by id.  An empty slot holds all ones.
+#define NNM_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+enum CounterIdx {
+    CNT_DISTS = 0,
+    CNT_UPDATES = 1,
+    CNT_FLAGS = 2,
+    CNT_RERANKED = 3,
+    CNT_LIST_ENTRIES = 4,
+    CNT_WORK = 5,      // work-queue cursor of the persistent join
+    CNT_TOTAL_REV = 6,
+    CNT_N = 16
+};
+
+struct nnm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evj0 = nullptr, evj1 = nullptr;
+    std::string err;
+    uint64_t mem_cap = 0;
+    uint64_t mem_used = 0;
+    int num_cus = 256;
+
+    // data
+    uint64_t n = 0;
+    uint32_t d = 0, ld = 0;
+    float* X = nullptr;
+    uint16_t* Xb = nullptr;   // bf16 copy (lazy)
+    float* norms = nullptr;   // squared norms of the bf16 rows (lazy)
+    int data_exact_bf16 = 0;  // every value an integer in [-256, 256]
+
+    // partition
+    uint32_t nsub = 0, ngrp = 0;
+    std::vector<uint32_t> h_sub_begin, h_sub_group, h_grp_first_sub;
+    std::vector<uint8_t> h_sub_built;
+    uint32_t* label = nullptr;      // n
+    uint32_t* vsub = nullptr;       // n: subgraph of each row
+    uint32_t* sub_begin = nullptr;  // nsub + 1
+    uint32_t* sub_grp = nullptr;    // nsub
+    uint8_t* sub_built = nullptr;   // nsub
+    uint32_t* grp_begin = nullptr;  // ngrp
+    uint32_t* grp_end = nullptr;    // ngrp
+
+    // graph
+    uint32_t K = 0;
+    uint32_t* g_ids = nullptr;
+    float* g_dist = nullptr;
+    uint8_t* g_flag = nullptr;
+
+    // parked far entries of a merge (cpnGraph)
+    uint32_t cut = 0;
+    uint32_t* c_ids = nullptr;
+    float* c_dist = nullptr;
+    uint8_t* c_flag = nullptr;
+    uint8_t* has_cpn = nullptr;     // n
+    int cpn_live = 0;
+
+    // iteration scratch
+    uint32_t cap_new = 0, cap_old = 0, rev_cap = 0, slots_C = 0;
+    uint32_t* l_new = nullptr;      // n x cap_new
+    uint32_t* l_old = nullptr;      // n x cap_old
+    uint32_t* n_new = nullptr;      // n
+    uint32_t* n_old = nullptr;      // n
+    uint32_t* newmask = nullptr;    // n
+    uint32_t* rcnt = nullptr;       // 2n counts (new, old)
+    uint32_t* roff = nullptr;       // 2n + 1 offsets
+    uint32_t* rcur = nullptr;       // 2n cursors
+    uint32_t* rbuf = nullptr;       // n*K reverse entries
+    uint32_t* scan_tmp = nullptr;   // block sums
+    uint64_t* slots = nullptr;      // n x slots_C
+    uint64_t* counters = nullptr;   // CNT_N
+    uint32_t* thr = nullptr;        // n: k-th distance bits at iteration start
+    uint32_t* grp_flags = nullptr;  // ngrp: per-group new-flag counts
+    uint8_t* grp_active = nullptr;  // ngrp
+
+    std::vector<std::pair<void*, size_t>> allocs;
+};
+
+// ------------------------------------------------------------ host helpers
+
+int nnm_fail(nnm_ctx* c, int code, const std::string& msg);
+int nnm_hip_fail(nnm_ctx* c, hipError_t e, const char* what);
+int nnm_alloc(nnm_ctx* c, void** p, size_t bytes);
+void nnm_free(nnm_ctx* c, void* p);
+int nnm_ensure_bf16(nnm_ctx* c);
+int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
+
+#define NNM_HIP(c, expr)                                  \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return nnm_hip_fail((c), _e, #expr); \
+    } while (0)
+
+#define NNM_LAUNCH_CHECK(c)                                  \
+    do {                                                     \
+        hipError_t _e = hipGetLastError();                   \
+        if (_e != hipSuccess) return nnm_hip_fail((c), _e, "kernel launch"); \
+    } while (0)
+
+// Kernel launchers shared between translation units.
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t salt);
+
+// ---------------------------------------------------------- device helpers
+
+__host__ __device__ inline uint64_t nnm_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Counter-based random stream: the value for (seed, a, b) is fixed, so the
+// same draws come out whatever the launch shape or thread order.
+__host__ __device__ inline uint64_t nnm_rand(uint64_t seed, uint64_t a, uint64_t b) {
+    return nnm_mix64(seed ^ nnm_mix64(a * 0xD1B54A32D192ED03ull + b));
+}
+
+__host__ __device__ inline uint32_t nnm_hash32(uint32_t x, uint32_t salt) {
+    x ^= salt;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ inline uint64_t nnm_key(float dist, uint32_t id) {
+    return ((uint64_t)__float_as_uint(dist) << 32) | id;
+}
+__device__ inline float nnm_key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+__device__ inline uint32_t nnm_key_id(uint64_t k) { return (uint32_t)k; }
+
+__device__ inline uint32_t nnm_lane() { return __lane_id(); }
+
+// The library's FP32 squared L2: elements in order 0..d-1, one accumulator,
+// each step acc = fma(a-b, a-b, acc).  Every kernel that produces a stored
+// distance uses this order, so one pair always gives the same bits.
+__device__ inline float nnm_l2_rows(const float* __restrict__ a, const float* __restrict__ b,
+                                    uint32_t d) {
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    float acc = 0.f;
+    for (uint32_t i = 0; i < d / 4; ++i) {
+        float4 x = a4[i], y = b4[i];
+        float t;
+        t = x.x - y.x; acc = __builtin_fmaf(t, t, acc);
+        t = x.y - y.y; acc = __builtin_fmaf(t, t, acc);
+        t = x.z - y.z; acc = __builtin_fmaf(t, t, acc);
+        t = x.w - y.w; acc = __builtin_fmaf(t, t, acc);
+    }
+    for (uint32_t i = (d / 4) * 4; i < d; ++i) {
+        float t = a[i] - b[i];
+        acc = __builtin_fmaf(t, t, acc);
+    }
+    return acc;
+}
+
+// Wave-wide bitonic sort of one 64-bit key per lane, ascending by lane.
+__device__ inline uint64_t nnm_wave_sort64(uint64_t key) {
+    const uint32_t lane = nnm_lane();
+#pragma unroll
+    for (uint32_t k = 2; k <= NNM_WAVE; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            uint64_t other = __shfl_xor(key, (int)j, NNM_WAVE);
+            bool up = ((lane & k) == 0);
+            bool lower = ((lane & j) == 0);
+            uint64_t mn = key < other ? key : other;
+            uint64_t mx = key < other ? other : key;
+            key = (lower == up) ? mn : mx;
+        }
+    }
+    return key;
+}
+
+__device__ inline uint32_t nnm_wave_sort32(uint32_t key) {
+    const uint32_t lane = nnm_lane();
+#pragma unroll
+    for (uint32_t k = 2; k <= NNM_WAVE; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            uint32_t other = __shfl_xor(key, (int)j, NNM_WAVE);
+            bool up = ((lane & k) == 0);
+            bool lower = ((lane & j) == 0);
+            uint32_t mn = key < other ? key : other;
+            uint32_t mx = key < other ? other : key;
+            key = (lower == up) ? mn : mx;
+        }
+    }
+    return key;
+}
+
+__device__ inline uint32_t nnm_wave_sum32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NNM_WAVE);
+    return v;
+}
+
+__device__ inline uint64_t nnm_ballot(bool p) { return __ballot(p); }
+
+__device__ inline uint32_t nnm_mask_rank(uint64_t mask) {
+    // number of set bits of mask below this lane
+    const uint32_t lane = nnm_lane();
+    uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    return (uint32_t)__popcll(below);
+}
+
+// Distinct random values in [0, M) for lanes [0, k), k <= 64, k <= M: the
+// draw of AbstractGraph::getRndSds (src/abstractgraph.cpp:7-35) -- k values
+// in [0, M-k), sorted, bumped apart, rotated by a random offset -- computed
+// across the wave (sort, prefix max, rotate).  Lanes >= k return M.
+__device__ inline uint32_t nnm_wave_rnd_sds(uint64_t seed, uint64_t stream, uint32_t k, uint32_t M) {
+    const uint32_t lane = nnm_lane();
+    if (M == k) return lane < k ? lane : M;
+    uint32_t r = 0xFFFFFFFFu;
+    if (lane < k) r = (uint32_t)(nnm_rand(seed, stream, lane) % (uint64_t)(M - k));
+    r = nnm_wave_sort32(r);
+    // bump: a'[i] = i + max_{j<=i}(a[j] - j)   (a[i] sorted, i < k)
+    int32_t t = lane < k ? (int32_t)r - (int32_t)lane : -0x7FFFFFFF;
+#pragma unroll
+    for (int o = 1; o < NNM_WAVE; o <<= 1) {
+        int32_t u = __shfl_up(t, (unsigned)o, NNM_WAVE);
+        if ((int)lane >= o) t = t > u ? t : u;
+    }
+    uint32_t off = (uint32_t)(nnm_rand(seed, stream, 0xFFFFu) % M);
+    if (lane >= k) return M;
+    uint64_t v = (uint64_t)(t + (int32_t)lane) + off;
+    return (uint32_t)(v % M);
+}

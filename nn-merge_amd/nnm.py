@@ -1,0 +1,234 @@
+"""ctypes binding of libnnmerge.so (the C ABI in include/nnmerge.h).
+
+Loading fails loudly when the library is missing; there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnnmerge.so")
+
+NNM_OK = 0
+NNM_ERR_INVALID = -1
+NNM_ERR_HIP = -2
+NNM_ERR_NOMEM = -3
+NNM_ERR_STATE = -4
+NNM_ERR_UNSUPPORTED = -5
+RAW = 0xFFFFFFFF
+PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
+RULE_ALL, RULE_MERGE = 0, 1
+RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
+
+
+class Params(C.Structure):
+    _fields_ = [("K", C.c_uint32), ("rho", C.c_float), ("rev_cap", C.c_uint32), ("max_iters", C.c_uint32),
+                ("delta", C.c_float), ("rule", C.c_uint32), ("prec", C.c_uint32), ("rerank", C.c_uint32),
+                ("slots", C.c_uint32), ("cut", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("converged", C.c_uint32), ("dists", C.c_uint64),
+                ("init_dists", C.c_uint64), ("updates", C.c_uint64), ("reranked", C.c_uint64),
+                ("list_entries", C.c_uint64), ("last_delta", C.c_double), ("secs", C.c_double),
+                ("join_secs", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class NNMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nnmerge error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# exported entry points and their signatures (every symbol of include/nnmerge.h)
+_vp = C.c_void_p
+_SIGS = {
+    "nnm_default_params": (None, [C.POINTER(Params)]),
+    "nnm_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "nnm_destroy": (C.c_int, [_vp]),
+    "nnm_last_error": (C.c_char_p, [_vp]),
+    "nnm_set_mem_cap": (C.c_int, [_vp, C.c_uint64]),
+    "nnm_mem_used": (C.c_uint64, [_vp]),
+    "nnm_set_data": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32]),
+    "nnm_set_partition": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32]),
+    "nnm_graph_random_init": (C.c_int, [_vp, C.c_uint32, C.c_uint64, C.POINTER(Stats)]),
+    "nnm_set_graph": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(Stats)]),
+    "nnm_get_graph": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "nnm_nndescent": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Stats)]),
+    "nnm_local_join": (C.c_int, [_vp, C.POINTER(Params), C.c_uint32, C.POINTER(Stats)]),
+    "nnm_merge_init": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Stats)]),
+    "nnm_merge_finish": (C.c_int, [_vp, C.c_int]),
+    "nnm_bruteforce_knn": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_uint32, C.c_int, _vp, _vp]),
+    "nnm_pair_dists": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp]),
+}
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run python nn-merge_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_params(**kw):
+    p = Params()
+    lib().nnm_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One device context (one HIP stream) of the library."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().nnm_create(C.byref(self.h), device)
+        if rc != NNM_OK:
+            raise NNMError(rc, "nnm_create failed")
+        self.n = 0
+        self.d = 0
+        self.K = 0
+
+    def close(self):
+        if self.h:
+            lib().nnm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != NNM_OK:
+            raise NNMError(rc, lib().nnm_last_error(self.h).decode())
+        return rc
+
+    def last_error(self):
+        return lib().nnm_last_error(self.h).decode()
+
+    def set_mem_cap(self, nbytes):
+        self._chk(lib().nnm_set_mem_cap(self.h, nbytes))
+
+    def mem_used(self):
+        return lib().nnm_mem_used(self.h)
+
+    def set_data(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        self._chk(lib().nnm_set_data(self.h, _ptr(X), X.shape[0], X.shape[1]))
+        self.n, self.d = X.shape
+        self.K = 0
+
+    def set_partition(self, sub_begin, sub_built, sub_group=None):
+        sb = np.ascontiguousarray(sub_begin, dtype=np.uint32)
+        bu = np.ascontiguousarray(sub_built, dtype=np.uint8)
+        gr = np.ascontiguousarray(np.zeros(bu.size) if sub_group is None else sub_group, dtype=np.uint32)
+        self._chk(lib().nnm_set_partition(self.h, _ptr(sb), _ptr(bu), _ptr(gr), bu.size))
+
+    def random_init(self, K, seed=1):
+        st = Stats()
+        self._chk(lib().nnm_graph_random_init(self.h, K, seed, C.byref(st)))
+        self.K = K
+        return st.as_dict()
+
+    def set_graph(self, ids, dist=None, flags=None):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        K = ids.shape[1]
+        dist = None if dist is None else np.ascontiguousarray(dist, dtype=np.float32)
+        flags = None if flags is None else np.ascontiguousarray(flags, dtype=np.uint8)
+        st = Stats()
+        self._chk(lib().nnm_set_graph(self.h, _ptr(ids), _ptr(dist), _ptr(flags), K, C.byref(st)))
+        self.K = K
+        return st.as_dict()
+
+    def get_graph(self):
+        ids = np.empty((self.n, self.K), np.uint32)
+        dist = np.empty((self.n, self.K), np.float32)
+        flags = np.empty((self.n, self.K), np.uint8)
+        self._chk(lib().nnm_get_graph(self.h, _ptr(ids), _ptr(dist), _ptr(flags)))
+        return ids, dist, flags
+
+    def nndescent(self, p):
+        st = Stats()
+        self._chk(lib().nnm_nndescent(self.h, C.byref(p), C.byref(st)))
+        return st.as_dict()
+
+    def local_join(self, p, it=0):
+        st = Stats()
+        self._chk(lib().nnm_local_join(self.h, C.byref(p), it, C.byref(st)))
+        return st.as_dict()
+
+    def merge_init(self, p):
+        st = Stats()
+        self._chk(lib().nnm_merge_init(self.h, C.byref(p), C.byref(st)))
+        return st.as_dict()
+
+    def merge_finish(self, dedup=True):
+        self._chk(lib().nnm_merge_finish(self.h, 1 if dedup else 0))
+
+    def bruteforce(self, k, rows=None, Q=None, exclude_self=True):
+        if Q is not None:
+            Q = np.ascontiguousarray(Q, dtype=np.float32)
+            nq = Q.shape[0]
+            r = None
+        else:
+            r = np.ascontiguousarray(np.arange(self.n) if rows is None else rows, dtype=np.uint32)
+            nq = r.size
+        ids = np.empty((nq, k), np.uint32)
+        dist = np.empty((nq, k), np.float32)
+        self._chk(lib().nnm_bruteforce_knn(self.h, _ptr(Q), _ptr(r), nq, k, 1 if exclude_self else 0,
+                                           _ptr(ids), _ptr(dist)))
+        return ids, dist
+
+    def pair_dists(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        out = np.empty(a.size, np.float32)
+        self._chk(lib().nnm_pair_dists(self.h, _ptr(a), _ptr(b), a.size, _ptr(out)))
+        return out
+
+
+def recall(ids, gt, rows, k=10, unique=True):
+    """recall@k of lists `ids` (n x K) on `rows` against ground truth `gt`
+    (len(rows) x >=k): Evaluator::getKGraphRecall semantics
+    (src/evaluator.cpp:258-295); unique=False counts repeated ids each time."""
+    L = ids[np.asarray(rows)][:, :k]
+    G = gt[:, :k]
+    hits = 0
+    for i in range(L.shape[0]):
+        row = L[i]
+        if unique:
+            row = np.unique(row)
+        hits += np.isin(row, G[i]).sum()
+    return hits / (L.shape[0] * k)

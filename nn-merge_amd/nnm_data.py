@@ -1,0 +1,71 @@
+"""Seeded synthetic inputs for the merge workloads.
+
+Every generator is deterministic in its seed (numpy PCG64).  The workloads:
+
+* ``sift_like(n, d, centers, sigma)``: integer-valued float32 rows in
+  [0, 255], drawn from a Gaussian mixture whose centres are uniform in
+  [0, 255]^d; x = round(clip(c + N(0, sigma^2), 0, 255)).  Every value is an
+  integer <= 255, so squared L2 is exact in float32 for d <= 256 and the
+  rows are exactly representable in bfloat16.
+* ``unit_mixture(n, d, centers, sigma)``: Gaussian mixture with centres drawn
+  from N(0, I), x = c + N(0, sigma^2 I), then L2-normalised.
+* ``log_uniform_leaves(total, lo, hi)``: leaf sizes drawn log-uniformly in
+  [lo, hi] until they sum to ``total`` (the last leaf takes the remainder,
+  merged into its neighbour if it falls under ``lo``).
+"""
+import numpy as np
+
+SEED_BASE = 20261203
+
+
+def sift_like(n, d=128, centers=64, sigma=20.0, seed=SEED_BASE + 1, chunk=1 << 20):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(0.0, 255.0, size=(centers, d)).astype(np.float32)
+    lab = rng.integers(0, centers, size=n)
+    X = np.empty((n, d), np.float32)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        blk = rng.standard_normal(size=(e - s, d), dtype=np.float32)
+        blk *= np.float32(sigma)
+        blk += C[lab[s:e]]
+        np.clip(blk, 0.0, 255.0, out=blk)
+        np.rint(blk, out=blk)
+        X[s:e] = blk
+    return X
+
+
+def unit_mixture(n, d=96, centers=512, sigma=0.35, seed=SEED_BASE + 3, chunk=1 << 20):
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal(size=(centers, d)).astype(np.float32)
+    lab = rng.integers(0, centers, size=n)
+    X = np.empty((n, d), np.float32)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        blk = rng.standard_normal(size=(e - s, d), dtype=np.float32)
+        blk *= np.float32(sigma)
+        blk += C[lab[s:e]]
+        blk /= np.linalg.norm(blk, axis=1, keepdims=True)
+        X[s:e] = blk
+    return X
+
+
+def log_uniform_leaves(total, lo=5000, hi=80000, seed=SEED_BASE + 33):
+    rng = np.random.default_rng(seed)
+    sizes = []
+    left = total
+    while left > 0:
+        s = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        s = min(s, left)
+        sizes.append(s)
+        left -= s
+    if len(sizes) > 1 and sizes[-1] < lo:
+        sizes[-2] += sizes[-1]
+        sizes.pop()
+    return np.array(sizes, dtype=np.int64)
+
+
+def random_partition_perm(n, parts, seed):
+    """A random assignment of n rows to `parts` equal-size partitions, as a
+    permutation that makes each partition a contiguous block."""
+    rng = np.random.default_rng(seed)
+    return rng.permutation(n)

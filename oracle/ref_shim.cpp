@@ -1,0 +1,18 @@
+// C entry points over the reference's own distance routines, compiled from
+// the reference sources in place (see Makefile, target _ref/libref_dist.so).
+// TEST INFRASTRUCTURE ONLY: used by tests to pin the oracle's restatement of
+// VMath::l2norm (src/vmath.cpp:56-88) and DistOpt::L2SqrSIMDExt
+// (src/distopt.cpp:11-80).
+#include "vmath.h"
+#include "distopt.h"
+
+extern "C" float ref_l2norm(const float* v1, unsigned long s1, const float* v2,
+                            unsigned long s2, unsigned d) {
+    return VMath::l2norm(v1, s1, v2, s2, d);
+}
+
+extern "C" float ref_l2sqr_simd_ext(const float* v1, unsigned loc1, const float* v2,
+                                    unsigned loc2, unsigned d) {
+    alignas(16) float tmp[4];
+    return DistOpt::L2SqrSIMDExt(v1, loc1, v2, loc2, d, tmp);
+}

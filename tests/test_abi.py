@@ -1,0 +1,43 @@
+"""The C-ABI library loads on the CPU and exports every entry point declared
+in include/nnmerge.h (no device calls are made here)."""
+import ctypes
+import os
+import re
+
+import nnm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "nnmerge.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nnm_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "nnm_create" in syms and "nnm_nndescent" in syms and "nnm_bruteforce_knn" in syms
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(nnm.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert sorted(nnm.exported_symbols()) == declared_symbols()
+
+
+def test_default_params_are_reference_constants():
+    p = nnm.default_params()
+    assert p.K == 20 and p.rev_cap == 100 and abs(p.delta - 0.002) < 1e-9 and p.rho == 1.0
+    assert p.max_iters == 0 and p.rule == nnm.RULE_ALL and p.prec == nnm.PREC_FP32
+
+
+def test_params_struct_layout_matches_header():
+    # nnm_params: 10 x 4-byte fields then a uint64 seed (offset 40), size 48
+    assert ctypes.sizeof(nnm.Params) == 48
+    assert nnm.Params.seed.offset == 40
+    assert ctypes.sizeof(nnm.Stats) == 8 + 5 * 8 + 3 * 8

@@ -1,0 +1,117 @@
+"""GPU parity of the basic entry points against the CPU oracle.
+
+Distances: the library's FP32 order (0..d-1, fused multiply-add) is exact on
+integer-valued rows (every partial sum is an integer below 2^24), so on
+sift-like data the GPU must equal the FP64 oracle exactly and neighbour ids
+must match bit for bit with ties broken by id.  On real-valued rows the
+tolerance is 1e-5 relative to FP64.
+"""
+import numpy as np
+import pytest
+
+import nnm
+import nnm_data
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with nnm.Context(0) as c:
+        yield c
+
+
+def test_bruteforce_integer_rows_bit_exact(ctx):
+    X = nnm_data.sift_like(3000, 128, centers=32, seed=11)
+    ctx.set_data(X)
+    rows = np.arange(0, 3000, 15, dtype=np.uint32)
+    gi, gd = ctx.bruteforce(20, rows=rows)
+    oi, od = orc.bruteforce(X, 20, rows=rows)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gd.astype(np.float64), od)
+
+
+def test_bruteforce_real_rows_within_1e5(ctx):
+    X = nnm_data.unit_mixture(2500, 96, centers=16, sigma=0.3, seed=12)
+    ctx.set_data(X)
+    rows = np.arange(0, 2500, 25, dtype=np.uint32)
+    gi, gd = ctx.bruteforce(10, rows=rows)
+    oi, od = orc.bruteforce(X, 10, rows=rows)
+    rel = np.abs(gd.astype(np.float64) - od) / np.maximum(od, 1e-30)
+    assert rel.max() < 1e-5
+    # ids agree wherever the FP64 distances are not within 1e-5 of a neighbour
+    gap_ok = np.ones_like(od, dtype=bool)
+    gap = np.diff(od, axis=1) / np.maximum(od[:, 1:], 1e-30)
+    gap_ok[:, 1:] &= gap > 1e-5
+    gap_ok[:, :-1] &= gap > 1e-5
+    assert np.array_equal(gi[gap_ok], oi[gap_ok])
+
+
+def test_bruteforce_query_vectors(ctx):
+    X = nnm_data.sift_like(2000, 128, centers=16, seed=13)
+    Q = nnm_data.sift_like(64, 128, centers=16, seed=14)
+    ctx.set_data(X)
+    gi, gd = ctx.bruteforce(8, Q=Q)
+    oi, od = orc.bruteforce(X, 8, Q=Q)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gd.astype(np.float64), od)
+
+
+def test_pair_dists_match_reference_order_on_integers(ctx):
+    X = nnm_data.sift_like(500, 128, centers=8, seed=15)
+    ctx.set_data(X)
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 500, 300).astype(np.uint32)
+    b = rng.integers(0, 500, 300).astype(np.uint32)
+    g = ctx.pair_dists(a, b)
+    ref = np.array([orc.l2_seq(X[i], X[j]) for i, j in zip(a, b)], np.float32)
+    assert np.array_equal(g, ref)
+
+
+def _check_rows(ids, dist, X, K, lo=None, hi=None):
+    n = ids.shape[0]
+    for v in range(n):
+        row = ids[v]
+        assert len(set(row.tolist())) == K, f"duplicate ids in row {v}"
+        assert v not in row
+        if lo is not None:
+            assert (row >= lo[v]).all() and (row < hi[v]).all()
+        keys = list(zip(dist[v].tolist(), row.tolist()))
+        assert keys == sorted(keys), f"row {v} not sorted by (dist, id)"
+
+
+def test_random_init_invariants(ctx):
+    X = nnm_data.sift_like(1200, 128, centers=8, seed=16)
+    ctx.set_data(X)
+    ctx.set_partition([0, 500, 1200], [1, 1], [0, 1])
+    ctx.random_init(20, seed=5)
+    ids, dist, flags = ctx.get_graph()
+    lo = np.where(np.arange(1200) < 500, 0, 500)
+    hi = np.where(np.arange(1200) < 500, 500, 1200)
+    _check_rows(ids, dist, X, 20, lo, hi)
+    assert flags.all()
+    v = np.repeat(np.arange(1200), 20)
+    exact = np.array([orc.l2_seq(X[i], X[j]) for i, j in zip(v[:4000], ids.ravel()[:4000])], np.float32)
+    assert np.array_equal(dist.ravel()[:4000], exact)
+
+
+def test_set_graph_recomputes_and_sorts(ctx):
+    X = nnm_data.sift_like(800, 128, centers=8, seed=17)
+    ctx.set_data(X)
+    rng = np.random.default_rng(3)
+    ids = np.stack([rng.choice(np.delete(np.arange(800), v), 12, replace=False) for v in range(800)]).astype(np.uint32)
+    ctx.set_graph(ids)
+    gi, gd, gf = ctx.get_graph()
+    _check_rows(gi, gd, X, 12)
+    for v in range(0, 800, 50):
+        assert set(gi[v].tolist()) == set(ids[v].tolist())
+
+
+def test_bad_graph_rejected(ctx):
+    X = nnm_data.sift_like(100, 128, centers=4, seed=18)
+    ctx.set_data(X)
+    ids = np.tile(np.arange(10, dtype=np.uint32), (100, 1))
+    with pytest.raises(nnm.NNMError) as e:
+        ctx.set_graph(ids)  # row v < 10 contains itself
+    assert e.value.code == nnm.NNM_ERR_INVALID

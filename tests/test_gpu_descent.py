@@ -1,0 +1,162 @@
+"""GPU NN-Descent and merges against the CPU restatement of the reference.
+
+Parity is statistical, as the reference's own runs are (it seeds with
+time(NULL) and shuffles with rand()): merged-graph recall@10 against exact
+brute force must be within 0.005 of the CPU restatement's on the same seeded
+input and the same starting subgraphs.  Graph invariants are exact.
+"""
+import numpy as np
+import pytest
+
+import nnm
+import nnm_data
+import orc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 0.005
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with nnm.Context(0) as c:
+        yield c
+
+
+def _invariants(ids, dist, K):
+    n = ids.shape[0]
+    assert ids.shape == (n, K)
+    s = np.sort(ids, axis=1)
+    assert (np.diff(s.astype(np.int64), axis=1) != 0).all(), "duplicate ids"
+    assert (ids != np.arange(n)[:, None]).all(), "self loop"
+    d64 = dist.astype(np.float64)
+    ok = (np.diff(d64, axis=1) > 0) | ((np.diff(d64, axis=1) == 0) & (np.diff(ids.astype(np.int64), axis=1) > 0))
+    assert ok.all(), "rows not sorted by (dist, id)"
+
+
+@pytest.fixture(scope="module")
+def data20k():
+    X = nnm_data.sift_like(20000, 128, centers=64, sigma=20.0, seed=21)
+    rows = np.arange(0, 20000, 10, dtype=np.uint32)
+    gt, _ = orc.bruteforce(X, 10, rows=rows)
+    return X, rows, gt
+
+
+def test_kgraph_recall_parity(ctx, data20k):
+    X, rows, gt = data20k
+    K = 20
+    ctx.set_data(X)
+    ctx.random_init(K, seed=7)
+    p = nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=7, max_iters=30)
+    st = ctx.nndescent(p)
+    ids, dist, _ = ctx.get_graph()
+    _invariants(ids, dist, K)
+    r_gpu = nnm.recall(ids, gt, rows, 10)
+    oi, _, ost = orc.kgraph(X, K=K, seed=7, max_iters=30)
+    r_cpu = orc.recall(oi, rows, gt, 10)
+    print("kgraph", st, "gpu", r_gpu, "cpu", r_cpu, ost)
+    assert st["dists"] > 0 and st["iters"] >= 2
+    assert r_gpu >= r_cpu - TOL
+
+
+def _build_halves(ctx, X, K, seed, parts):
+    n = X.shape[0]
+    b = np.linspace(0, n, parts + 1).astype(np.int64)
+    ctx.set_data(X)
+    ctx.set_partition(b, [1] * parts, np.arange(parts))
+    ctx.random_init(K, seed=seed)
+    st = ctx.nndescent(nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=seed, max_iters=30))
+    ids, dist, _ = ctx.get_graph()
+    for i in range(parts):
+        blk = ids[b[i]:b[i + 1]]
+        assert (blk >= b[i]).all() and (blk < b[i + 1]).all(), "subgraph build crossed its group"
+    return b, ids, st
+
+
+def test_smerge_recall_parity(ctx, data20k):
+    X, rows, gt = data20k
+    K = 20
+    b, sub_ids, _ = _build_halves(ctx, X, K, 3, 2)
+    r_sub = nnm.recall(sub_ids, gt, rows, 10)
+    # merge on the GPU from the GPU subgraphs
+    ctx.set_partition(b, [1, 1], [0, 0])
+    ctx.set_graph(sub_ids)
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, seed=9, max_iters=30)
+    ist = ctx.merge_init(p)
+    st = ctx.nndescent(p)
+    ctx.merge_finish(True)
+    ids, dist, _ = ctx.get_graph()
+    _invariants(ids, dist, K)
+    r_gpu = nnm.recall(ids, gt, rows, 10)
+    # CPU restatement from the same subgraphs
+    oi, _, ost = orc.merge(X, b, [1, 1], sub_ids, K=K, seed=9, max_iters=30)
+    r_cpu = orc.recall(oi, rows, gt, 10)
+    print("smerge sub", r_sub, "gpu", r_gpu, st, ist, "cpu", r_cpu, ost)
+    assert r_gpu >= r_cpu - TOL
+
+
+def test_jmerge_recall_parity(ctx, data20k):
+    X, rows, gt = data20k
+    K = 20
+    n = X.shape[0]
+    half = n // 2
+    ctx.set_data(X[:half])
+    ctx.random_init(K, seed=4)
+    ctx.nndescent(nnm.default_params(K=K, seed=4, max_iters=30))
+    g0, _, _ = ctx.get_graph()
+    init = np.zeros((n, K), np.uint32)
+    init[:half] = g0
+    ctx.set_data(X)
+    ctx.set_partition([0, half, n], [1, 0], [0, 0])
+    dummy = ((np.arange(n)[:, None] + 1 + np.arange(K)[None, :]) % n).astype(np.uint32)
+    init[half:] = dummy[half:]
+    ctx.set_graph(init)
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, seed=6, max_iters=30)
+    ctx.merge_init(p)
+    st = ctx.nndescent(p)
+    ctx.merge_finish(True)
+    ids, dist, _ = ctx.get_graph()
+    _invariants(ids, dist, K)
+    r_gpu = nnm.recall(ids, gt, rows, 10)
+    oi, _, ost = orc.merge(X, [0, half, n], [1, 0], init, K=K, seed=6, max_iters=30)
+    r_cpu = orc.recall(oi, rows, gt, 10)
+    print("jmerge gpu", r_gpu, st, "cpu", r_cpu, ost)
+    assert r_gpu >= r_cpu - TOL
+
+
+def test_batched_builds_match_single(ctx):
+    """Two subgraphs built in one launch give the same recall as separate builds."""
+    X = nnm_data.sift_like(8000, 128, centers=32, seed=22)
+    K = 16
+    b, ids, st = _build_halves(ctx, X, K, 5, 4)
+    for i in range(4):
+        Xi = X[b[i]:b[i + 1]]
+        rows = np.arange(0, Xi.shape[0], 7, dtype=np.uint32)
+        gt, _ = orc.bruteforce(Xi, 10, rows=rows)
+        r = nnm.recall(ids[b[i]:b[i + 1]] - b[i], gt, rows, 10)
+        assert r > 0.97
+
+
+def test_single_iteration_counts(ctx):
+    """nCmps of one iteration from random lists equals the closed-form pair
+    count C(n_new, 2) + n_new * n_old summed over the rows, when every list
+    entry is new (rho = 1) and every pair passes the rule."""
+    X = nnm_data.sift_like(600, 128, centers=4, seed=23)
+    ctx.set_data(X)
+    ctx.random_init(10, seed=2)
+    p = nnm.default_params(K=10, rule=nnm.RULE_ALL, seed=2, rev_cap=1000)
+    ids, dist, _ = ctx.get_graph()
+    st = ctx.local_join(p, 0)
+    # expected lists: all K entries are new (rho = 1); reverse entries v -> u
+    # when u's k-th distance is below d(v, u); no old entries yet
+    n = 600
+    kth = dist[:, -1]
+    n_new = np.full(n, 10, np.int64)
+    for v in range(n):
+        for e in range(10):
+            u = ids[v, e]
+            if kth[u] < dist[v, e]:
+                n_new[u] += 1
+    assert st["list_entries"] == int(n_new.sum())
+    assert st["dists"] == int((n_new * (n_new - 1) // 2).sum())
+    assert st["iters"] == 1
