@@ -91,6 +91,7 @@ typedef struct nnm_stats {
     uint64_t updates;      /* list entries replaced */
     uint64_t reranked;     /* candidates recomputed in exact FP32 */
     uint64_t list_entries; /* candidate-list entries joined */
+    uint64_t offers;       /* update offers that passed the k-th prefilter */
     double last_delta;     /* mean fraction of new entries after the loop */
     double secs;           /* device time of the call */
     double join_secs;      /* device time spent in the join kernels */
@@ -145,6 +146,15 @@ int nnm_set_graph(nnm_ctx* ctx, const uint32_t* ids, const float* dist, const ui
  * (src/kgraph.cpp:228-238). */
 int nnm_get_graph(nnm_ctx* ctx, uint32_t* ids, float* dist, uint8_t* flags);
 
+/* Device-side checkpoint of the lists: nnm_graph_save copies ids, dists and
+ * flags into a second device buffer, nnm_graph_restore copies them back
+ * (recompute = 1 also recomputes every distance, as InitSGraph does,
+ * src/abstractmerge.cpp:55-76).  The reference's only checkpoint is the text
+ * dump of AbstractMerge::saveKGraph (src/abstractmerge.cpp:278-291); this
+ * keeps the copy in HBM so repeated merges start without a host transfer. */
+int nnm_graph_save(nnm_ctx* ctx);
+int nnm_graph_restore(nnm_ctx* ctx, int recompute, nnm_stats* st);
+
 /* --- NN-Descent ------------------------------------------------------- */
 /* The NN-Descent loop: list build + reverse sampling + local join + delta
  * test, until delta or max_iters.  Replaces AbstractMerge::nnDescent
@@ -186,6 +196,21 @@ int nnm_bruteforce_knn(nnm_ctx* ctx, const float* Q, const uint32_t* rows, uint6
 /* Squared L2 between row pairs (a[i], b[i]) in the library's FP32 order:
  * the DISTFUNC call of src/abstractgraph.h:11-12 batched. */
 int nnm_pair_dists(nnm_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t m, float* out);
+
+/* --- host-class drivers ----------------------------------------------- */
+/* Runs one host class end to end (nn-merge_amd/host): kind 0 = KGraph
+ * (KGraph::buildKGraph, src/kgraph.cpp:217-245), 1 = Smerge
+ * (Smerge::buildKGraph, src/smerge.cpp:46-91), 2 = Jmerge
+ * (Jmerge::buildKGraph, src/jmerge.cpp:43-80), 3 = P-way merge of `parts`
+ * equal partitions.  The merges build their partitions first (all in one
+ * launch, not timed, as the reference does not time them) and then run the
+ * timed merge.  Outputs n x K ids (and dists when dist != NULL); st gets the
+ * merge's counters (dists = the reference's nCmps, secs = device time of
+ * the timed region); build_secs the device time of the partition builds.
+ * Replaces the test() drivers' buildKGraph calls (src/smerge.cpp:238-261,
+ * src/jmerge.cpp:205-224, src/kgraph.cpp:263-285). */
+int nnm_host_build(int kind, const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t parts,
+                   uint32_t* ids, float* dist, nnm_stats* st, double* build_secs);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,14 @@ static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 65535u * 16u
     return (uint32_t)g;
 }
 
+// Blocks of 256 threads (4 waves) for persistent wave loops: enough to fill
+// every CU a few times over, never more than one wave per row.
+static uint32_t wave_grid(nnm_ctx* c, uint64_t rows) {
+    uint64_t want = (uint64_t)c->num_cus * 8;
+    uint64_t need = (rows + 3) / 4;
+    return (uint32_t)std::max<uint64_t>(1, std::min(want, need));
+}
+
 static int fill_u32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) {
     if (!n) return NNM_OK;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, p, n, v);
@@ -172,6 +180,8 @@ extern "C" uint64_t nnm_mem_used(const nnm_ctx* c) { return c ? c->mem_used : 0;
 static void free_graph(nnm_ctx* c) {
     nnm_free(c, c->g_ids); nnm_free(c, c->g_dist); nnm_free(c, c->g_flag);
     c->g_ids = nullptr; c->g_dist = nullptr; c->g_flag = nullptr;
+    nnm_free(c, c->s_ids); nnm_free(c, c->s_dist); nnm_free(c, c->s_flag);
+    c->s_ids = nullptr; c->s_dist = nullptr; c->s_flag = nullptr;
     nnm_free(c, c->c_ids); nnm_free(c, c->c_dist); nnm_free(c, c->c_flag); nnm_free(c, c->has_cpn);
     c->c_ids = nullptr; c->c_dist = nullptr; c->c_flag = nullptr; c->has_cpn = nullptr;
     c->K = 0; c->cut = 0; c->cpn_live = 0;
@@ -333,6 +343,19 @@ extern "C" int nnm_set_partition(nnm_ctx* c, const uint32_t* sub_begin, const ui
 
 // ================================================================ graph kernels
 
+// Persistent wave loop: wave w of the grid takes rows w, w + W, w + 2W, ...
+#define NNM_WAVE_LOOP(v, n)                                                                       \
+    for (uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE; v < (n); \
+         v += (uint64_t)gridDim.x * (blockDim.x / NNM_WAVE))
+
+// One atomic per wave for a per-lane count.
+__device__ inline void wave_count(uint64_t* ctr, uint64_t per_lane) {
+    uint64_t t = per_lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, NNM_WAVE);
+    if (nnm_lane() == 0 && t) atomicAdd((unsigned long long*)ctr, (unsigned long long)t);
+}
+
 // Write lanes [0, K) of a row: (key, flag) per lane, sorted by key first.
 // Ids inside a row are distinct, so the flag of each sorted entry is found
 // by id.
@@ -390,14 +413,13 @@ __global__ void k_random_init(const float* __restrict__ X, uint32_t ld, uint64_t
                               const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
                               uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                               uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters) {
-    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    if (v >= n) return;
-    const uint32_t g = sub_grp[vsub[v]];
-    const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
     uint64_t nd = 0;
-    wave_random_row(X, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd);
-    uint32_t tot = nnm_wave_sum32((uint32_t)nd);
-    if (nnm_lane() == 0) atomicAdd((unsigned long long*)&counters[CNT_DISTS], (unsigned long long)tot);
+    NNM_WAVE_LOOP(v, n) {
+        const uint32_t g = sub_grp[vsub[v]];
+        const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
+        wave_random_row(X, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd);
+    }
+    wave_count(counters + CNT_DISTS, nd);
 }
 
 // Upload fix-up: optional distance recompute, optional flags, sort rows.
@@ -405,23 +427,22 @@ __global__ void k_set_graph(const float* __restrict__ X, uint32_t ld, uint64_t n
                             int have_flags, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                             uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters,
                             uint32_t* __restrict__ bad) {
-    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    if (v >= n) return;
     const uint32_t lane = nnm_lane();
-    uint32_t id = 0;
-    float dd = 0.f;
-    uint32_t f = 1;
-    if (lane < K) {
-        id = g_ids[v * K + lane];
-        if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
-        dd = recompute ? nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld) : g_dist[v * K + lane];
-        f = have_flags ? g_flag[v * K + lane] : 1u;
+    uint64_t nd = 0;
+    NNM_WAVE_LOOP(v, n) {
+        uint32_t id = 0;
+        float dd = 0.f;
+        uint32_t f = 1;
+        if (lane < K) {
+            id = g_ids[v * K + lane];
+            if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
+            dd = recompute ? nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld) : g_dist[v * K + lane];
+            f = have_flags ? g_flag[v * K + lane] : 1u;
+            nd += recompute ? 1 : 0;
+        }
+        wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
-    if (recompute) {
-        uint32_t tot = nnm_wave_sum32(lane < K ? 1u : 0u);
-        if (lane == 0) atomicAdd((unsigned long long*)&counters[CNT_DISTS], (unsigned long long)tot);
-    }
-    wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+    wave_count(counters + CNT_DISTS, nd);
 }
 
 // Merge set-up, one wave per row.
@@ -433,71 +454,70 @@ __global__ void k_merge_init(const float* __restrict__ X, uint32_t ld, uint64_t 
                              uint8_t* __restrict__ g_flag, uint32_t* __restrict__ c_ids,
                              float* __restrict__ c_dist, uint8_t* __restrict__ c_flag,
                              uint8_t* __restrict__ has_cpn, uint64_t* __restrict__ counters) {
-    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    if (v >= n) return;
     const uint32_t lane = nnm_lane();
-    const uint32_t s = vsub[v];
-    const uint32_t g = sub_grp[s];
-    const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
-    uint32_t* ids = g_ids + v * K;
-    float* dist = g_dist + v * K;
-    uint8_t* flg = g_flag + v * K;
     uint64_t nd = 0;
-    if (!sub_built[s]) {
-        if (lane == 0) has_cpn[v] = 0;
-        wave_random_row(X, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd);
-    } else {
-        const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
-        const uint32_t remain = K - cut;
-        uint32_t pid = 0, pf = 0;
-        float pd = 0.f;
-        if (lane < K) { pid = ids[lane]; pd = dist[lane]; pf = flg[lane]; }
-        // park the far entries (cpnGraph)
-        if (lane >= remain && lane < K) {
-            c_ids[v * cut + (lane - remain)] = pid;
-            c_dist[v * cut + (lane - remain)] = pd;
-            c_flag[v * cut + (lane - remain)] = (uint8_t)pf;
+    NNM_WAVE_LOOP(v, n) {
+        const uint32_t s = vsub[v];
+        const uint32_t g = sub_grp[s];
+        const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
+        uint32_t* ids = g_ids + v * K;
+        float* dist = g_dist + v * K;
+        uint8_t* flg = g_flag + v * K;
+        if (!sub_built[s]) {
+            if (lane == 0) has_cpn[v] = 0;
+            wave_random_row(X, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd);
+        } else {
+            const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
+            const uint32_t remain = K - cut;
+            uint32_t pid = 0, pf = 0;
+            float pd = 0.f;
+            if (lane < K) { pid = ids[lane]; pd = dist[lane]; pf = flg[lane]; }
+            // park the far entries (cpnGraph)
+            if (lane >= remain && lane < K) {
+                c_ids[v * cut + (lane - remain)] = pid;
+                c_dist[v * cut + (lane - remain)] = pd;
+                c_flag[v * cut + (lane - remain)] = (uint8_t)pf;
+            }
+            if (lane == 0) has_cpn[v] = 1;
+            const uint32_t others = gsize - ssize;
+            uint32_t fresh_n = 0;
+            uint32_t cand = 0;
+            bool fresh = false;
+            if (others > 0) {
+                uint32_t draw = 2 * cut < others ? 2 * cut : others;
+                uint32_t r = nnm_wave_rnd_sds(seed, v, draw, others);
+                cand = r < (sb - gb) ? gb + r : gb + r + ssize;
+                bool dup = false;
+                for (uint32_t p = 0; p < K; ++p) {
+                    uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
+                    dup |= (q == cand);
+                }
+                fresh = lane < draw && !dup;
+                uint64_t m = nnm_ballot(fresh);
+                uint32_t rank = nnm_mask_rank(m);
+                fresh = fresh && rank < cut;
+                fresh_n = (uint32_t)__popcll(m) < cut ? (uint32_t)__popcll(m) : cut;
+                // route fresh candidate of rank j to lane remain + j
+                uint32_t src = 0;
+                uint64_t mm = m;
+                for (uint32_t t = 0; t < fresh_n; ++t) {
+                    uint32_t l = (uint32_t)__ffsll((long long)mm) - 1;
+                    mm &= mm - 1;
+                    if (lane == remain + t) src = l;
+                }
+                uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
+                if (lane >= remain && lane < remain + fresh_n) {
+                    pid = got;
+                    pd = nnm_l2_rows(X + (uint64_t)got * ld, X + v * ld, ld);
+                    pf = 1;
+                    nd += 1;
+                }
+            }
+            (void)fresh;
+            wave_write_sorted_row(nnm_key(pd, pid), pf, K, ids, dist, flg);
         }
-        if (lane == 0) has_cpn[v] = 1;
-        const uint32_t others = gsize - ssize;
-        uint32_t fresh_n = 0;
-        uint32_t cand = 0;
-        bool fresh = false;
-        if (others > 0) {
-            uint32_t draw = 2 * cut < others ? 2 * cut : others;
-            uint32_t r = nnm_wave_rnd_sds(seed, v, draw, others);
-            cand = r < (sb - gb) ? gb + r : gb + r + ssize;
-            bool dup = false;
-            for (uint32_t p = 0; p < K; ++p) {
-                uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
-                dup |= (q == cand);
-            }
-            fresh = lane < draw && !dup;
-            uint64_t m = nnm_ballot(fresh);
-            uint32_t rank = nnm_mask_rank(m);
-            fresh = fresh && rank < cut;
-            fresh_n = (uint32_t)__popcll(m) < cut ? (uint32_t)__popcll(m) : cut;
-            // route fresh candidate of rank j to lane remain + j
-            uint32_t src = 0;
-            uint64_t mm = m;
-            for (uint32_t t = 0; t < fresh_n; ++t) {
-                uint32_t l = (uint32_t)__ffsll((long long)mm) - 1;
-                mm &= mm - 1;
-                if (lane == remain + t) src = l;
-            }
-            uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
-            if (lane >= remain && lane < remain + fresh_n) {
-                pid = got;
-                pd = nnm_l2_rows(X + (uint64_t)got * ld, X + v * ld, ld);
-                pf = 1;
-                nd += 1;
-            }
-        }
-        (void)fresh;
-        wave_write_sorted_row(nnm_key(pd, pid), pf, K, ids, dist, flg);
     }
-    uint32_t tot = nnm_wave_sum32((uint32_t)nd);
-    if (lane == 0) atomicAdd((unsigned long long*)&counters[CNT_DISTS], (unsigned long long)tot);
+    wave_count(counters + CNT_DISTS, nd);
 }
 
 // merge2one: pool (K, sorted) + parked (cut, sorted) -> first K by (dist, id).
@@ -505,39 +525,40 @@ __global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, 
                                uint32_t* __restrict__ g_ids, float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
                                const uint32_t* __restrict__ c_ids, const float* __restrict__ c_dist,
                                const uint8_t* __restrict__ c_flag) {
-    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    if (v >= n || !has_cpn[v]) return;
     const uint32_t lane = nnm_lane();
-    uint64_t key = NNM_EMPTY_KEY;
-    uint32_t f = 0;
-    uint32_t id = 0xFFFFFFFFu;
-    if (lane < K) {
-        id = g_ids[v * K + lane];
-        key = nnm_key(g_dist[v * K + lane], id);
-        f = g_flag[v * K + lane];
-    } else if (lane < K + cut) {
-        id = c_ids[v * cut + lane - K];
-        key = nnm_key(c_dist[v * cut + lane - K], id);
-        f = c_flag[v * cut + lane - K];
-    }
-    if (dedup && lane >= K && lane < K + cut) {
-        for (uint32_t p = 0; p < K; ++p) {
-            uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
-            if (q == id) key = NNM_EMPTY_KEY;
+    NNM_WAVE_LOOP(v, n) {
+        if (!has_cpn[v]) continue;
+        uint64_t key = NNM_EMPTY_KEY;
+        uint32_t f = 0;
+        uint32_t id = 0xFFFFFFFFu;
+        if (lane < K) {
+            id = g_ids[v * K + lane];
+            key = nnm_key(g_dist[v * K + lane], id);
+            f = g_flag[v * K + lane];
+        } else if (lane < K + cut) {
+            id = c_ids[v * cut + lane - K];
+            key = nnm_key(c_dist[v * cut + lane - K], id);
+            f = c_flag[v * cut + lane - K];
         }
-    }
-    uint64_t s = nnm_wave_sort64(key);
-    uint32_t sid = nnm_key_id(s);
-    uint32_t myf = 0;
-    for (uint32_t p = 0; p < K + cut; ++p) {
-        uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
-        uint32_t qf = __shfl(f, (int)p, NNM_WAVE);
-        if (q == sid) myf = qf;
-    }
-    if (lane < K) {
-        g_ids[v * K + lane] = sid;
-        g_dist[v * K + lane] = nnm_key_dist(s);
-        g_flag[v * K + lane] = (uint8_t)myf;
+        if (dedup && lane >= K && lane < K + cut) {
+            for (uint32_t p = 0; p < K; ++p) {
+                uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
+                if (q == id) key = NNM_EMPTY_KEY;
+            }
+        }
+        uint64_t s = nnm_wave_sort64(key);
+        uint32_t sid = nnm_key_id(s);
+        uint32_t myf = 0;
+        for (uint32_t p = 0; p < K + cut; ++p) {
+            uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
+            uint32_t qf = __shfl(f, (int)p, NNM_WAVE);
+            if (q == sid) myf = qf;
+        }
+        if (lane < K) {
+            g_ids[v * K + lane] = sid;
+            g_dist[v * K + lane] = nnm_key_dist(s);
+            g_flag[v * K + lane] = (uint8_t)myf;
+        }
     }
 }
 
@@ -592,7 +613,7 @@ extern "C" int nnm_graph_random_init(nnm_ctx* c, uint32_t K, uint64_t seed, nnm_
     NNM_TRY(ensure_graph(c, K));
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_random_init, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld,
+    hipLaunchKernelGGL(k_random_init, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld,
                        c->n, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
                        c->g_flag, c->counters);
     NNM_LAUNCH_CHECK(c);
@@ -621,7 +642,7 @@ extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist,
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_set_graph, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
+    hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
                        K, dist ? 0 : 1, flags ? 1 : 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -647,6 +668,51 @@ extern "C" int nnm_get_graph(nnm_ctx* c, uint32_t* ids, float* dist, uint8_t* fl
     if (dist) NNM_HIP(c, hipMemcpyAsync(dist, c->g_dist, nk * 4, hipMemcpyDeviceToHost, c->stream));
     if (flags) NNM_HIP(c, hipMemcpyAsync(flags, c->g_flag, nk, hipMemcpyDeviceToHost, c->stream));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
+    return NNM_OK;
+}
+
+extern "C" int nnm_graph_save(nnm_ctx* c) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no graph");
+    const uint64_t nk = c->n * c->K;
+    if (!c->s_ids) {
+        NNM_TRY(alloc_n(c, &c->s_ids, nk));
+        NNM_TRY(alloc_n(c, &c->s_dist, nk));
+        NNM_TRY(alloc_n(c, &c->s_flag, nk));
+    }
+    NNM_HIP(c, hipMemcpyAsync(c->s_ids, c->g_ids, nk * 4, hipMemcpyDeviceToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->s_dist, c->g_dist, nk * 4, hipMemcpyDeviceToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->s_flag, c->g_flag, nk, hipMemcpyDeviceToDevice, c->stream));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    return NNM_OK;
+}
+
+extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!c->s_ids) return nnm_fail(c, NNM_ERR_STATE, "no saved graph");
+    stats_reset(st);
+    const uint64_t nk = c->n * c->K;
+    NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->g_ids, c->s_ids, nk * 4, hipMemcpyDeviceToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->g_dist, c->s_dist, nk * 4, hipMemcpyDeviceToDevice, c->stream));
+    NNM_HIP(c, hipMemcpyAsync(c->g_flag, c->s_flag, nk, hipMemcpyDeviceToDevice, c->stream));
+    if (recompute) {
+        uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
+        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld,
+                           c->n, c->K, 1, 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+        NNM_LAUNCH_CHECK(c);
+    }
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    uint64_t h[CNT_N];
+    NNM_TRY(read_counters(c, h));
+    c->cpn_live = 0;
+    if (st) {
+        st->init_dists = h[CNT_DISTS];
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        st->secs = ms * 1e-3;
+    }
     return NNM_OK;
 }
 
@@ -701,7 +767,7 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
     }
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_merge_init, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
+    hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
                        K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin, c->grp_end,
                        p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
                        c->c_flag, c->has_cpn, c->counters);
@@ -722,7 +788,7 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
 extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
     if (!c) return NNM_ERR_INVALID;
     if (!c->cpn_live) return nnm_fail(c, NNM_ERR_STATE, "nnm_merge_finish: no merge in progress");
-    hipLaunchKernelGGL(k_merge_finish, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->n, c->K,
+    hipLaunchKernelGGL(k_merge_finish, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->n, c->K,
                        c->cut, dedup, c->has_cpn, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist, c->c_flag);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipStreamSynchronize(c->stream));
@@ -833,17 +899,18 @@ __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const
                             uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
                             uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                             uint64_t* __restrict__ counters) {
-    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    if (v >= n) return;
     const uint32_t lane = nnm_lane();
-    const uint32_t fn = n_new[v], fo = n_old[v];
-    const uint32_t an = wave_append_reverse(rbuf + roff[v], rcnt[v], cap, nnm_hash32((uint32_t)v, salt),
-                                            l_new + v * cap_new + fn);
-    const uint32_t ao = wave_append_reverse(rbuf + roff[n + v], rcnt[n + v], cap,
-                                            nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u), l_old + v * cap_old + fo);
-    if (lane == 0) {
-        n_new[v] = fn + an;
-        n_old[v] = fo + ao;
+    NNM_WAVE_LOOP(v, n) {
+        const uint32_t fn = n_new[v], fo = n_old[v];
+        const uint32_t an = wave_append_reverse(rbuf + roff[v], rcnt[v], cap, nnm_hash32((uint32_t)v, salt),
+                                                l_new + v * cap_new + fn);
+        const uint32_t ao = wave_append_reverse(rbuf + roff[n + v], rcnt[n + v], cap,
+                                                nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u),
+                                                l_old + v * cap_old + fo);
+        if (lane == 0) {
+            n_new[v] = fn + an;
+            n_old[v] = fo + ao;
+        }
     }
 }
 
@@ -927,81 +994,84 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
                          int rerank, float rerank_slack, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
                          uint64_t* __restrict__ counters) {
-    const uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    if (v >= n) return;
     const uint32_t lane = nnm_lane();
-    uint64_t pkey = NNM_EMPTY_KEY;
-    uint32_t pid = 0xFFFFFFFFu, pf = 0;
-    if (lane < K) {
-        pid = g_ids[v * K + lane];
-        pkey = nnm_key(g_dist[v * K + lane], pid);
-        pf = g_flag[v * K + lane];
-    }
-    uint64_t skey = NNM_EMPTY_KEY;
-    if (lane < C) {
-        skey = slots[v * C + lane];
-        if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
-    }
-    const uint64_t kth_key = __shfl(pkey, (int)(K - 1), NNM_WAVE);
-    bool valid = skey != NNM_EMPTY_KEY;
-    const uint32_t sid = nnm_key_id(skey);
-    if (__ballot(valid)) {
-        for (uint32_t p = 0; p < K; ++p) {
-            uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
-            valid = valid && (q != sid);
+    uint64_t c_upd = 0, c_flags = 0, c_rr = 0;  // wave-uniform
+    NNM_WAVE_LOOP(v, n) {
+        uint64_t pkey = NNM_EMPTY_KEY;
+        uint32_t pid = 0xFFFFFFFFu, pf = 0;
+        if (lane < K) {
+            pid = g_ids[v * K + lane];
+            pkey = nnm_key(g_dist[v * K + lane], pid);
+            pf = g_flag[v * K + lane];
         }
-        uint32_t rr = 0;
-        if (rerank) {
-            // approximate keys carry a bounded error: re-rank every candidate
-            // that can still cross the k-th entry
-            const float kd = nnm_key_dist(kth_key);
-            valid = valid && (nnm_key_dist(skey) <= kd + rerank_slack);
-            if (valid) {
-                float ex = nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
-                skey = nnm_key(ex, sid);
-                rr = 1;
+        uint64_t skey = NNM_EMPTY_KEY;
+        if (lane < C) {
+            skey = slots[v * C + lane];
+            if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
+        }
+        const uint64_t kth_key = __shfl(pkey, (int)(K - 1), NNM_WAVE);
+        bool valid = skey != NNM_EMPTY_KEY;
+        const uint32_t sid = nnm_key_id(skey);
+        if (__ballot(valid)) {
+            for (uint32_t p = 0; p < K; ++p) {
+                uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
+                valid = valid && (q != sid);
             }
-            rr = nnm_wave_sum32(rr);
+            uint32_t rr = 0;
+            if (rerank) {
+                // approximate keys carry a bounded error: re-rank every candidate
+                // that can still cross the k-th entry
+                const float kd = nnm_key_dist(kth_key);
+                valid = valid && (nnm_key_dist(skey) <= kd + rerank_slack);
+                if (valid) {
+                    float ex = nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
+                    skey = nnm_key(ex, sid);
+                    rr = 1;
+                }
+                rr = nnm_wave_sum32(rr);
+            }
+            valid = valid && (skey < kth_key);
+            const uint64_t vm = __ballot(valid);
+            uint32_t pos_p = lane, pos_s = 0;
+            // rank against the other valid candidates
+            uint64_t mm = vm;
+            while (mm) {
+                const uint32_t e = (uint32_t)__ffsll((long long)mm) - 1;
+                mm &= mm - 1;
+                const uint64_t ek = __shfl(skey, (int)e, NNM_WAVE);
+                pos_p += (ek < pkey) ? 1u : 0u;
+                pos_s += (ek < skey) ? 1u : 0u;
+            }
+            // rank against the pool
+            for (uint32_t p = 0; p < K; ++p) {
+                const uint64_t q = __shfl(pkey, (int)p, NNM_WAVE);
+                pos_s += (q < skey) ? 1u : 0u;
+            }
+            const bool ins = valid && pos_s < K;
+            const bool keep = lane < K && pos_p < K;
+            if (keep && pos_p != lane) {
+                g_ids[v * K + pos_p] = pid;
+                g_dist[v * K + pos_p] = nnm_key_dist(pkey);
+                g_flag[v * K + pos_p] = (uint8_t)pf;
+            }
+            if (ins) {
+                g_ids[v * K + pos_s] = sid;
+                g_dist[v * K + pos_s] = nnm_key_dist(skey);
+                g_flag[v * K + pos_s] = 1;
+            }
+            const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
+            const uint32_t n_flag = (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
+            c_upd += n_ins;
+            c_flags += n_flag;
+            c_rr += rr;
+        } else {
+            c_flags += (uint32_t)__popcll(__ballot(lane < K && pf));
         }
-        valid = valid && (skey < kth_key);
-        const uint64_t vm = __ballot(valid);
-        uint32_t pos_p = lane, pos_s = 0;
-        // rank against the other valid candidates
-        uint64_t mm = vm;
-        while (mm) {
-            const uint32_t e = (uint32_t)__ffsll((long long)mm) - 1;
-            mm &= mm - 1;
-            const uint64_t ek = __shfl(skey, (int)e, NNM_WAVE);
-            pos_p += (ek < pkey) ? 1u : 0u;
-            pos_s += (ek < skey) ? 1u : 0u;
-        }
-        // rank against the pool
-        for (uint32_t p = 0; p < K; ++p) {
-            const uint64_t q = __shfl(pkey, (int)p, NNM_WAVE);
-            pos_s += (q < skey) ? 1u : 0u;
-        }
-        const bool ins = valid && pos_s < K;
-        const bool keep = lane < K && pos_p < K;
-        if (keep && pos_p != lane) {
-            g_ids[v * K + pos_p] = pid;
-            g_dist[v * K + pos_p] = nnm_key_dist(pkey);
-            g_flag[v * K + pos_p] = (uint8_t)pf;
-        }
-        if (ins) {
-            g_ids[v * K + pos_s] = sid;
-            g_dist[v * K + pos_s] = nnm_key_dist(skey);
-            g_flag[v * K + pos_s] = 1;
-        }
-        const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
-        const uint32_t n_flag = (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
-        if (lane == 0) {
-            if (n_ins) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)n_ins);
-            atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)n_flag);
-            if (rr) atomicAdd((unsigned long long*)&counters[CNT_RERANKED], (unsigned long long)rr);
-        }
-    } else {
-        const uint32_t n_flag = (uint32_t)__popcll(__ballot(lane < K && pf));
-        if (lane == 0 && n_flag) atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)n_flag);
+    }
+    if (lane == 0) {
+        if (c_upd) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)c_upd);
+        if (c_flags) atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)c_flags);
+        if (c_rr) atomicAdd((unsigned long long*)&counters[CNT_RERANKED], (unsigned long long)c_rr);
     }
 }
 
@@ -1077,7 +1147,7 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
     hipLaunchKernelGGL(k_lb_fill, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_ids,
                        c->g_dist, c->g_flag, c->newmask, c->roff, c->rcur, c->rbuf);
     NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_lb_select, dim3(grid_for(n, 4, 1u << 30)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
+    hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
                        c->rbuf, c->rev_cap, salt2, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
                        c->counters);
     NNM_LAUNCH_CHECK(c);
@@ -1091,7 +1161,7 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
                  (p->rerank == NNM_RERANK_AUTO && !(p->prec == NNM_PREC_BF16 && c->data_exact_bf16));
         slack = 0.f;
     }
-    hipLaunchKernelGGL(k_update, dim3(grid_for(n, 4, 1u << 30)), dim3(256), 0, c->stream, n, K, c->slots_C, c->X,
+    hipLaunchKernelGGL(k_update, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, c->slots_C, c->X,
                        c->ld, rerank, slack, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
     NNM_LAUNCH_CHECK(c);
     if (join_ms) {
@@ -1119,6 +1189,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         const uint32_t iter = single ? only_iter : it;
         NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
         NNM_TRY(fill_u64(c, c->counters + CNT_WORK, 1, 0));
+        NNM_TRY(fill_u64(c, c->counters + CNT_WORK2, 1, 0));
         NNM_TRY(run_iteration(c, p, iter, &join_ms));
         uint64_t h[CNT_N];
         NNM_TRY(read_counters(c, h));
@@ -1139,6 +1210,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->updates = h[CNT_UPDATES];
         st->reranked = h[CNT_RERANKED];
         st->list_entries = h[CNT_LIST_ENTRIES];
+        st->offers = h[CNT_OFFERS];
         st->last_delta = last_delta;
         float ms = 0.f;
         hipEventElapsedTime(&ms, c->ev0, c->ev1);

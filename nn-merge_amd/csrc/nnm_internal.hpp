@@ -35,6 +35,8 @@ enum CounterIdx {
     CNT_LIST_ENTRIES = 4,
     CNT_WORK = 5,      // work-queue cursor of the persistent join
     CNT_TOTAL_REV = 6,
+    CNT_OFFERS = 7,    // slot offers issued by the join
+    CNT_WORK2 = 8,     // work-queue cursor of the long-list join
     CNT_N = 16
 };
 
@@ -72,6 +74,9 @@ struct nnm_ctx {
     uint32_t* g_ids = nullptr;
     float* g_dist = nullptr;
     uint8_t* g_flag = nullptr;
+    uint32_t* s_ids = nullptr;      // checkpoint copy
+    float* s_dist = nullptr;
+    uint8_t* s_flag = nullptr;
 
     // parked far entries of a merge (cpnGraph)
     uint32_t cut = 0;

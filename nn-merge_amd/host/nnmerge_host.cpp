@@ -1,0 +1,209 @@
+// Host classes over the C ABI; see nnmerge_host.hpp for the reference map.
+#include "nnmerge_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+
+namespace nnmerge {
+
+float l2norm(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0) {
+    // VMath::l2norm, src/vmath.cpp:56-88: one accumulator, elements in order
+    const float* a = v1 + s1 * d0;
+    const float* b = v2 + s2 * d0;
+    float res = 0.f;
+    for (unsigned i = 0; i < d0; ++i) {
+        float t = a[i] - b[i];
+        res += t * t;
+    }
+    return res;
+}
+
+AbstractGraph::AbstractGraph(SpaceInterface<float>* space, int device) {
+    (void)space;  // the device path computes squared L2
+    nnm_default_params(&params);
+    int rc = nnm_create(&ctx_, device);
+    if (rc != NNM_OK) throw Error(rc, "nnm_create failed");
+}
+
+AbstractGraph::~AbstractGraph() {
+    if (ctx_) nnm_destroy(ctx_);
+}
+
+void AbstractGraph::check(int rc) const {
+    if (rc != NNM_OK) throw Error(rc, nnm_last_error(ctx_));
+}
+
+void AbstractGraph::fetch() {
+    ids_.resize((size_t)row * K0);
+    dist_.resize((size_t)row * K0);
+    flag_.resize((size_t)row * K0);
+    check(nnm_get_graph(ctx_, ids_.data(), dist_.data(), flag_.data()));
+}
+
+unsigned* AbstractGraph::exportIds() const {
+    // caller-owned copy, as KGraph::buildKGraph returns (src/kgraph.cpp:228-238)
+    unsigned* out = new unsigned[(size_t)row * K0];
+    std::copy(ids_.begin(), ids_.end(), out);
+    return out;
+}
+
+std::vector<Neighbor> AbstractGraph::getNeighbors(unsigned i) const {
+    std::vector<Neighbor> v;
+    if (i >= row) return v;
+    v.reserve(K0);
+    for (unsigned j = 0; j < K0; ++j) {
+        const size_t k = (size_t)i * K0 + j;
+        v.push_back(Neighbor{ids_[k], dist_[k], 0u, flag_[k] != 0});
+    }
+    return v;
+}
+
+void AbstractGraph::saveKGraph(const char* dst) {
+    // text format of AbstractMerge::saveKGraph, src/abstractmerge.cpp:278-291
+    std::ofstream out(dst);
+    if (!out) throw Error(NNM_ERR_INVALID, std::string("cannot open ") + dst);
+    out << row << " " << K0 << "\n";
+    for (unsigned i = 0; i < row; ++i) {
+        out << i << " " << K0 << " ";
+        for (unsigned j = 0; j < K0; ++j) out << ids_[(size_t)i * K0 + j] << " ";
+        out << "\n";
+    }
+}
+
+unsigned* KGraph::buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) {
+    K0 = K;
+    row = N;
+    col = D;
+    params.K = K;
+    params.rule = NNM_RULE_ALL;
+    check(nnm_set_data(ctx_, mat, N, D));
+    nnm_stats s0{}, s1{};
+    check(nnm_graph_random_init(ctx_, K, params.seed, &s0));
+    check(nnm_nndescent(ctx_, &params, &s1));
+    lastStats = s1;
+    nCmps = s0.init_dists + s1.dists;
+    buildSecs = s0.secs + s1.secs;
+    mergeSecs = 0.0;
+    fetch();
+    return exportIds();
+}
+
+unsigned AbstractMerge::nnDescent() {
+    nnm_params p = params;
+    p.rule = NNM_RULE_MERGE;
+    nnm_stats st{};
+    check(nnm_nndescent(ctx_, &p, &st));
+    lastStats = st;
+    nCmps += st.dists;
+    mergeSecs += st.secs;
+    return 0;
+}
+
+unsigned AbstractMerge::localJoin() {
+    nnm_params p = params;
+    p.rule = NNM_RULE_MERGE;
+    nnm_stats st{};
+    check(nnm_local_join(ctx_, &p, iter_++, &st));
+    lastStats = st;
+    nCmps += st.dists;
+    mergeSecs += st.secs;
+    return 0;
+}
+
+unsigned AbstractMerge::merge2one() {
+    check(nnm_merge_finish(ctx_, 1));
+    return 1;
+}
+
+void AbstractMerge::mergePartitions(float* mat, unsigned K, unsigned N, unsigned D,
+                                    const std::vector<uint32_t>& begin, const std::vector<uint8_t>& built) {
+    K0 = K;
+    row = N;
+    col = D;
+    params.K = K;
+    const uint32_t P = (uint32_t)built.size();
+    check(nnm_set_data(ctx_, mat, N, D));
+    // 1. NN-Descent over every built partition at once: each partition its
+    //    own group (KGraph::buildKGraph per subgraph, src/smerge.cpp:59-66)
+    std::vector<uint32_t> grp(P);
+    for (uint32_t s = 0; s < P; ++s) grp[s] = s;
+    std::vector<uint8_t> all(P, 1);
+    check(nnm_set_partition(ctx_, begin.data(), all.data(), grp.data(), P));
+    nnm_stats s0{}, s1{};
+    check(nnm_graph_random_init(ctx_, K, params.seed ^ 0xB01DULL, &s0));
+    nnm_params bp = params;
+    bp.rule = NNM_RULE_ALL;
+    bp.max_iters = 0;  // subgraphs are built to convergence, as KGraph does
+    bp.rho = 1.0f;
+    check(nnm_nndescent(ctx_, &bp, &s1));
+    buildSecs = s0.secs + s1.secs;
+    // 2. the timed merge (src/smerge.cpp:75-81, src/jmerge.cpp:63-70)
+    std::fill(grp.begin(), grp.end(), 0u);
+    check(nnm_set_partition(ctx_, begin.data(), built.data(), grp.data(), P));
+    nnm_stats r{}, i{};
+    check(nnm_graph_save(ctx_));
+    check(nnm_graph_restore(ctx_, 1, &r));  // InitSGraph: recompute distances
+    nnm_params mp = params;
+    mp.rule = NNM_RULE_MERGE;
+    check(nnm_merge_init(ctx_, &mp, &i));
+    nCmps = i.init_dists;
+    mergeSecs = r.secs + i.secs;
+    nnDescent();
+    merge2one();
+    fetch();
+}
+
+unsigned* Smerge::buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) {
+    const uint32_t pre = N * 5 / 10;  // preSize, src/smerge.cpp:53
+    mergePartitions(mat, K, N, D, {0u, pre, N}, {1, 1});
+    return exportIds();
+}
+
+unsigned* Jmerge::buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) {
+    const uint32_t pre = N * 5 / 10;  // preSize, src/jmerge.cpp:47
+    mergePartitions(mat, K, N, D, {0u, pre, N}, {1, 0});
+    return exportIds();
+}
+
+unsigned* PMerge::buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) {
+    if (parts_ < 2 || parts_ > N / (K + 1)) throw Error(NNM_ERR_INVALID, "PMerge: bad partition count");
+    std::vector<uint32_t> b(parts_ + 1);
+    for (unsigned s = 0; s <= parts_; ++s) b[s] = (uint32_t)((uint64_t)N * s / parts_);
+    mergePartitions(mat, K, N, D, b, std::vector<uint8_t>(parts_, 1));
+    return exportIds();
+}
+
+}  // namespace nnmerge
+
+extern "C" int nnm_host_build(int kind, const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t parts,
+                              uint32_t* ids, float* dist, nnm_stats* st, double* build_secs) {
+    if (!X || !p || !ids || n == 0 || n > 0xFFFFFFF0ull) return NNM_ERR_INVALID;
+    try {
+        nnmerge::AbstractGraph* g = nullptr;
+        switch (kind) {
+            case 0: g = new nnmerge::KGraph(); break;
+            case 1: g = new nnmerge::Smerge(); break;
+            case 2: g = new nnmerge::Jmerge(); break;
+            case 3: g = new nnmerge::PMerge(parts); break;
+            default: return NNM_ERR_INVALID;
+        }
+        g->params = *p;
+        unsigned* out = g->buildKGraph(const_cast<float*>(X), p->K, (unsigned)n, d);
+        std::memcpy(ids, out, n * p->K * sizeof(uint32_t));
+        delete[] out;
+        if (dist) std::memcpy(dist, g->dists().data(), n * p->K * sizeof(float));
+        if (st) {
+            *st = g->lastStats;
+            st->dists = g->nCmps;
+            st->secs = g->mergeSecs;
+        }
+        if (build_secs) *build_secs = g->buildSecs;
+        delete g;
+        return NNM_OK;
+    } catch (const nnmerge::Error& e) {
+        return e.code;
+    } catch (...) {
+        return NNM_ERR_HIP;
+    }
+}

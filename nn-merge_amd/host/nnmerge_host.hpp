@@ -1,0 +1,142 @@
+// Host classes with the reference's entry points, running the local-join
+// path on the GPU through the C ABI (include/nnmerge.h).
+//
+//   KGraph::buildKGraph(mat, K, N, D)      src/kgraph.cpp:217-245
+//   Smerge::buildKGraph(mat, K, N, D)      src/smerge.cpp:46-91
+//   Jmerge::buildKGraph(mat, K, N, D)      src/jmerge.cpp:43-80
+//   AbstractMerge::nnDescent / localJoin   src/abstractmerge.cpp:78-192,
+//                                          src/abstractmerge.h:29-30
+//   AbstractMerge::merge2one               src/abstractmerge.cpp:232-277
+//   saveKGraph (text "N K" / "i K ids")    src/abstractmerge.cpp:278-291
+//   getNeighbors -> the Neighbor list view src/pvetex.h:18-30
+//
+// PMerge generalises the two-way merges to P contiguous partitions merged
+// jointly (pairs inside one built partition skipped), all partition builds
+// in one batched launch.
+//
+// Ownership differs on purpose: the reference's graph objects delete the
+// caller's matrix (src/abstractgraph.cpp:43-47, src/kgraph.cpp:28-32); these
+// classes copy it to the device and never free caller memory.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nnmerge.h"
+
+namespace nnmerge {
+
+// DISTFUNC and SpaceInterface of src/abstractgraph.h:11-19, kept for CPU
+// callers that pass a distance to the host classes; the device path always
+// computes squared L2 (L2space, src/space_lp.h:9-23).
+template <typename MTYPE>
+using DISTFUNC = MTYPE (*)(const float*, const unsigned long, const float*, const unsigned long, const unsigned int);
+
+template <typename MTYPE>
+class SpaceInterface {
+public:
+    virtual ~SpaceInterface() {}
+    virtual DISTFUNC<MTYPE> get_dist_func() = 0;
+};
+
+float l2norm(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0);
+
+class L2space : public SpaceInterface<float> {
+public:
+    DISTFUNC<float> get_dist_func() override { return l2norm; }
+};
+
+// Host view of one list entry (mergeDataType::Neighbor, src/pvetex.h:18-30).
+struct Neighbor {
+    unsigned id;
+    float dist;
+    unsigned gd;
+    bool flag;
+};
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& m) : std::runtime_error(m), code(code) {}
+    int code;
+};
+
+class AbstractGraph {
+public:
+    explicit AbstractGraph(SpaceInterface<float>* space = nullptr, int device = 0);
+    virtual ~AbstractGraph();
+    AbstractGraph(const AbstractGraph&) = delete;
+    AbstractGraph& operator=(const AbstractGraph&) = delete;
+
+    virtual unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) = 0;
+    virtual void saveKGraph(const char* dst);
+    std::vector<Neighbor> getNeighbors(unsigned i) const;
+    const std::vector<uint32_t>& ids() const { return ids_; }
+    const std::vector<float>& dists() const { return dist_; }
+
+    nnm_params params;          // K, rho, rev_cap, T, delta, prec, seed ...
+    unsigned long nCmps = 0;    // distances of the timed region (reference nCmps)
+    double mergeSecs = 0.0;     // device time of the timed region
+    double buildSecs = 0.0;     // device time of the subgraph builds (not timed by the reference)
+    nnm_stats lastStats{};
+
+protected:
+    void check(int rc) const;
+    void fetch();
+    unsigned* exportIds() const;
+
+    nnm_ctx* ctx_ = nullptr;
+    unsigned K0 = 0, row = 0, col = 0;
+    std::vector<uint32_t> ids_;
+    std::vector<float> dist_;
+    std::vector<uint8_t> flag_;
+};
+
+class KGraph : public AbstractGraph {
+public:
+    using AbstractGraph::AbstractGraph;
+    unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) override;
+};
+
+class AbstractMerge : public AbstractGraph {
+public:
+    using AbstractGraph::AbstractGraph;
+    // One NN-Descent loop over the current merge state (list build, local
+    // join, delta test), as AbstractMerge::nnDescent.
+    virtual unsigned nnDescent();
+    // One iteration of the loop: the per-iteration local join.
+    virtual unsigned localJoin();
+    unsigned merge2one();
+
+protected:
+    // Builds every partition with NN-Descent in one launch, then runs the
+    // timed merge: InitSGraph + InitialGraph + nnDescent + merge2one.
+    void mergePartitions(float* mat, unsigned K, unsigned N, unsigned D, const std::vector<uint32_t>& begin,
+                         const std::vector<uint8_t>& built);
+    unsigned iter_ = 0;
+};
+
+class Smerge : public AbstractMerge {
+public:
+    using AbstractMerge::AbstractMerge;
+    unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) override;
+};
+
+class Jmerge : public AbstractMerge {
+public:
+    using AbstractMerge::AbstractMerge;
+    unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) override;
+};
+
+class PMerge : public AbstractMerge {
+public:
+    PMerge(unsigned parts, SpaceInterface<float>* space = nullptr, int device = 0)
+        : AbstractMerge(space, device), parts_(parts) {}
+    unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) override;
+
+private:
+    unsigned parts_;
+};
+
+}  // namespace nnmerge

@@ -31,7 +31,7 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_uint32), ("converged", C.c_uint32), ("dists", C.c_uint64),
                 ("init_dists", C.c_uint64), ("updates", C.c_uint64), ("reranked", C.c_uint64),
-                ("list_entries", C.c_uint64), ("last_delta", C.c_double), ("secs", C.c_double),
+                ("list_entries", C.c_uint64), ("offers", C.c_uint64), ("last_delta", C.c_double), ("secs", C.c_double),
                 ("join_secs", C.c_double)]
 
     def as_dict(self):
@@ -66,6 +66,10 @@ _SIGS = {
     "nnm_merge_finish": (C.c_int, [_vp, C.c_int]),
     "nnm_bruteforce_knn": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_uint32, C.c_int, _vp, _vp]),
     "nnm_pair_dists": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp]),
+    "nnm_graph_save": (C.c_int, [_vp]),
+    "nnm_graph_restore": (C.c_int, [_vp, C.c_int, C.POINTER(Stats)]),
+    "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
+                                 C.POINTER(Stats), C.POINTER(C.c_double)]),
 }
 
 
@@ -179,6 +183,14 @@ class Context:
         self._chk(lib().nnm_get_graph(self.h, _ptr(ids), _ptr(dist), _ptr(flags)))
         return ids, dist, flags
 
+    def graph_save(self):
+        self._chk(lib().nnm_graph_save(self.h))
+
+    def graph_restore(self, recompute=True):
+        st = Stats()
+        self._chk(lib().nnm_graph_restore(self.h, 1 if recompute else 0, C.byref(st)))
+        return st.as_dict()
+
     def nndescent(self, p):
         st = Stats()
         self._chk(lib().nnm_nndescent(self.h, C.byref(p), C.byref(st)))
@@ -217,6 +229,24 @@ class Context:
         out = np.empty(a.size, np.float32)
         self._chk(lib().nnm_pair_dists(self.h, _ptr(a), _ptr(b), a.size, _ptr(out)))
         return out
+
+
+HOST_KGRAPH, HOST_SMERGE, HOST_JMERGE, HOST_PMERGE = 0, 1, 2, 3
+
+
+def host_build(kind, X, p, parts=0):
+    """Run a host class (KGraph / Smerge / Jmerge / PMerge) end to end."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    ids = np.empty((n, p.K), np.uint32)
+    dist = np.empty((n, p.K), np.float32)
+    st = Stats()
+    bs = C.c_double()
+    rc = lib().nnm_host_build(kind, _ptr(X), n, d, C.byref(p), parts, _ptr(ids), _ptr(dist), C.byref(st),
+                              C.byref(bs))
+    if rc != NNM_OK:
+        raise NNMError(rc, "nnm_host_build failed")
+    return ids, dist, st.as_dict(), bs.value
 
 
 def recall(ids, gt, rows, k=10, unique=True):

@@ -40,4 +40,4 @@ def test_params_struct_layout_matches_header():
     # nnm_params: 10 x 4-byte fields then a uint64 seed (offset 40), size 48
     assert ctypes.sizeof(nnm.Params) == 48
     assert nnm.Params.seed.offset == 40
-    assert ctypes.sizeof(nnm.Stats) == 8 + 5 * 8 + 3 * 8
+    assert ctypes.sizeof(nnm.Stats) == 8 + 6 * 8 + 3 * 8

@@ -160,3 +160,23 @@ def test_single_iteration_counts(ctx):
     assert st["list_entries"] == int(n_new.sum())
     assert st["dists"] == int((n_new * (n_new - 1) // 2).sum())
     assert st["iters"] == 1
+
+
+@pytest.mark.parametrize("d", [128, 96, 64, 200])
+def test_join_distances_exact(ctx, d):
+    """Every distance the join writes into a list equals the library's
+    canonical pair distance bit for bit (checks the row staging of every
+    join path: long and short lists, several row lengths)."""
+    X = nnm_data.sift_like(3000, d, centers=8, seed=40 + d)
+    ctx.set_data(X)
+    ctx.random_init(20, seed=3)
+    p = nnm.default_params(K=20, rule=nnm.RULE_ALL, seed=3, rev_cap=100)
+    for it in range(3):
+        ctx.local_join(p, it)
+    ids, dist, _ = ctx.get_graph()
+    v = np.repeat(np.arange(3000, dtype=np.uint32), 20)
+    ref = ctx.pair_dists(v, ids.ravel())
+    assert np.array_equal(ref, dist.ravel())
+    # integer rows: the canonical order is exact
+    ex = ((X[v].astype(np.float64) - X[ids.ravel()].astype(np.float64)) ** 2).sum(1)
+    assert np.array_equal(ex, dist.ravel().astype(np.float64))
