@@ -40,7 +40,7 @@
 #define MAXT ((JW / TM) * (JW / TN))  // 512 micro-tiles per window pair
 #define MAXS (MAXT / NNM_WAVE)        // 8 micro-tiles per lane
 #define JOIN_CHUNK 16                 // rows taken from the work queue at a time
-#define ROW_LDS_BYTES 24576           // LDS for staged rows (both paths)
+#define ROW_LDS_BYTES 24320           // LDS for staged rows (both paths)
 #define RND 2                         // fast path: micro-tiles per lane per round
 
 struct JoinArgs {
@@ -242,141 +242,363 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
     }
 }
 
-// Fast-path row address in LDS: row r, 16-byte chunk c of a row of CPR
-// chunks, stored at chunk c ^ (r & 7).
-template <int CPR>
-__device__ inline const float4* lds_chunk(const float* rows, uint32_t r, uint32_t c) {
-    return reinterpret_cast<const float4*>(rows) + r * CPR + (c ^ (r & 7u));
-}
-
+// Fast path.  Consecutive short lists of the work queue are packed into one
+// LDS window of at most js rows (their concatenated lists), so their
+// micro-tiles fill the wave together.  Rows are copied into LDS by
+// direct-to-LDS loads, one row per instruction, at a pitch of ld + 4 floats
+// (row r starts 4r banks further, so lanes reading different rows at one
+// offset hit different banks and every LDS read uses an immediate offset).
 template <uint32_t JS>
 struct FastLds {
     float rows[ROW_LDS_BYTES / 4];
-    uint32_t metaB[3][JW];
-    uint16_t tiles[((JS + TM - 1) / TM) * ((JS + TN - 1) / TN)];
+    uint32_t meta[3][JW];                  // id, label, k-th distance bits per window row
+    uint16_t tiles[((JS + TM - 1) / TM) * ((JS + TN - 1) / TN) + 64];
 };
+
+#define FB 4  // most lists packed into one window
+
+// Joins the list heads of `nb_v` vertices packed in the window: window rows
+// [off[b], off[b] + m[b]) hold vertex b's list, new entries first.
+template <int CPR, uint32_t JS>
+__device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, uint32_t lane, uint32_t nbv,
+                                   const uint32_t (&off)[FB], const uint32_t (&nnv)[FB], const uint32_t (&mv)[FB],
+                                   uint64_t& my_dists, uint64_t& my_offers) {
+    constexpr uint32_t PF = 4 * CPR + 4;  // LDS pitch in floats
+    // micro-tile list: (vertex, row block, column block) packed as b << 12 | rb << 6 | cb
+    uint32_t ntile = 0;
+    for (uint32_t b = 0; b < nbv; ++b) {
+        const uint32_t RBn = (nnv[b] + TM - 1) / TM, CBn = (mv[b] + TN - 1) / TN;
+        const uint32_t T = RBn * CBn;
+        for (uint32_t base = 0; base < T; base += NNM_WAVE) {
+            const uint32_t t = base + lane;
+            bool use = false;
+            uint32_t code = 0;
+            if (t < T) {
+                const uint32_t rb = t / CBn, cb = t % CBn;
+                const uint32_t jmax = min(cb * TN + TN - 1, mv[b] - 1);
+                use = jmax > rb * TM;
+                code = (b << 12) | (rb << 6) | cb;
+            }
+            const uint64_t msk = __ballot(use);
+            const uint32_t slot = ntile + nnm_mask_rank(msk);
+            // packed lists hold at most ceil(js/2) * ceil(js/4) + 3 micro-tiles
+            if (use && slot < sizeof(L.tiles) / sizeof(L.tiles[0])) L.tiles[slot] = (uint16_t)code;
+            ntile += (uint32_t)__popcll(msk);
+        }
+    }
+    ntile = min(ntile, (uint32_t)(sizeof(L.tiles) / sizeof(L.tiles[0])));
+    wave_lds_sync();
+    for (uint32_t t0 = 0; t0 < ntile; t0 += NNM_WAVE * RND) {
+        float acc[RND][TM * TN];
+        uint32_t code[RND];
+        bool on[RND];
+#pragma unroll
+        for (int s = 0; s < RND; ++s) {
+            const uint32_t ti = t0 + lane + NNM_WAVE * s;
+            on[s] = ti < ntile;
+            code[s] = on[s] ? L.tiles[ti] : 0u;
+#pragma unroll
+            for (int q = 0; q < TM * TN; ++q) acc[s][q] = 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < RND; ++s) {
+            if (!on[s]) continue;
+            const uint32_t b = code[s] >> 12, rb = (code[s] >> 6) & 63u, cb = code[s] & 63u;
+            const float* pa = L.rows + (off[b] + rb * TM) * PF;
+            const float* pb = L.rows + (off[b] + cb * TN) * PF;
+#pragma unroll
+            for (uint32_t c = 0; c < (uint32_t)CPR; ++c) {
+                float4 x[TM], y[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) x[i] = *reinterpret_cast<const float4*>(pa + i * PF + 4 * c);
+#pragma unroll
+                for (int j = 0; j < TN; ++j) y[j] = *reinterpret_cast<const float4*>(pb + j * PF + 4 * c);
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) {
+                        float t1 = x[i].x - y[j].x;
+                        float r = __builtin_fmaf(t1, t1, acc[s][i * TN + j]);
+                        t1 = x[i].y - y[j].y;
+                        r = __builtin_fmaf(t1, t1, r);
+                        t1 = x[i].z - y[j].z;
+                        r = __builtin_fmaf(t1, t1, r);
+                        t1 = x[i].w - y[j].w;
+                        acc[s][i * TN + j] = __builtin_fmaf(t1, t1, r);
+                    }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < RND; ++s) {
+            if (!on[s]) continue;
+            const uint32_t b = code[s] >> 12, rb = (code[s] >> 6) & 63u, cb = code[s] & 63u;
+            tile_epilogue(a, acc[s], rb, cb, nnv[b], mv[b], true,
+                          reinterpret_cast<const uint32_t(*)[JW]>(&L.meta[0][off[b]]),
+                          reinterpret_cast<const uint32_t(*)[JW]>(&L.meta[0][off[b]]), my_dists, my_offers);
+        }
+    }
+}
 
 template <int CPR>  // 16-byte chunks per row: ld = 4 * CPR
 __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
     constexpr uint32_t LD = 4 * CPR;
-    constexpr uint32_t JS_ROWS = ROW_LDS_BYTES / (LD * 4);
-    constexpr uint32_t JS = JS_ROWS < 64 ? JS_ROWS : 64;   // fast-path window
-    constexpr uint32_t RPI = NNM_WAVE / CPR;                // rows per direct-to-LDS instruction
-    static_assert(CPR % 8 == 0 && RPI >= 1, "row length must be a multiple of 32 floats, at most 256");
+    constexpr uint32_t PF = LD + 4;
+    constexpr uint32_t JS_ROWS = ROW_LDS_BYTES / (PF * 4);
+    constexpr uint32_t JS = JS_ROWS < 64 ? JS_ROWS : 64;  // window rows
+    static_assert(CPR % 8 == 0 && CPR <= 64, "row length must be a multiple of 32 floats, at most 256");
     __shared__ __align__(16) FastLds<JS> L;
 
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
 
-    // work queue: chunks of JOIN_CHUNK rows, one atomic per chunk
-    uint64_t v = 0, vend = 0;
-    {
+    for (;;) {
         uint64_t base = 0;
         if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)JOIN_CHUNK);
-        v = __shfl(base, 0, NNM_WAVE);
-        vend = min(a.n, v + JOIN_CHUNK);
-    }
-    // prefetched list head of vertex v: counts and the first 64 ids of each list
-    uint32_t p_nn = 0, p_no = 0, p_rn = 0, p_ro = 0;
-    if (v < a.n) {
-        p_nn = a.n_new[v];
-        p_no = a.n_old[v];
-        if (lane < a.cap_new) p_rn = a.l_new[v * a.cap_new + lane];
-        if (lane < a.cap_old) p_ro = a.l_old[v * a.cap_old + lane];
-    }
-    while (v < a.n) {
-        const uint64_t vcur = v;
-        const uint32_t nn = p_nn, no = p_no, rn = p_rn, ro = p_ro;
-        // advance and prefetch the next vertex's list head
-        ++v;
-        if (v >= vend) {
-            uint64_t base = 0;
-            if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)JOIN_CHUNK);
-            v = __shfl(base, 0, NNM_WAVE);
-            vend = min(a.n, v + JOIN_CHUNK);
+        base = __shfl(base, 0, NNM_WAVE);
+        if (base >= a.n) break;
+        const uint32_t cnt = (uint32_t)min((uint64_t)JOIN_CHUNK, a.n - base);
+        // list sizes of the chunk: lane i < cnt holds vertex base + i
+        uint32_t cn = 0, co = 0;
+        if (lane < cnt) {
+            cn = a.n_new[base + lane];
+            co = a.n_old[base + lane];
         }
-        if (v < a.n) {
-            p_nn = a.n_new[v];
-            p_no = a.n_old[v];
-            if (lane < a.cap_new) p_rn = a.l_new[v * a.cap_new + lane];
-            if (lane < a.cap_old) p_ro = a.l_old[v * a.cap_old + lane];
-        }
-        const uint32_t m = nn + no;
-        if (m > JS) continue;  // long lists: k_join_f32_windows
-        if (lane == 0) my_entries += m;
-        if (nn == 0) continue;
-        // window ids: lane j holds entry j of new ++ old
-        const uint32_t src = lane < nn ? lane : lane - nn;
-        const uint32_t from_new = (uint32_t)__shfl((int)rn, (int)(src & 63), NNM_WAVE);
-        const uint32_t from_old = (uint32_t)__shfl((int)ro, (int)(src & 63), NNM_WAVE);
-        const uint32_t wid = lane < nn ? from_new : from_old;
-        wave_lds_sync();  // previous vertex's readers are done with LDS
-        if (lane < m) {
-            L.metaB[0][lane] = wid;
-            L.metaB[1][lane] = a.label[wid];
-            L.metaB[2][lane] = __float_as_uint(a.g_dist[(uint64_t)wid * a.K + a.K - 1]);
-        }
-        // rows -> LDS, RPI rows per instruction, chunk c of row r at c ^ (r & 7)
-        {
-            const uint32_t lrow = lane / CPR, slot = lane % CPR;
-            for (uint32_t r0 = 0; r0 < m; r0 += RPI) {
-                const uint32_t r = r0 + lrow;
-                const uint32_t id = (uint32_t)__shfl((int)wid, (int)(r & 63), NNM_WAVE);
-                if (lane < RPI * CPR && r < m) {
-                    const uint32_t q = slot ^ (r & 7u);
-                    const float* g = a.X + (uint64_t)id * LD + 4 * q;
+        uint32_t i = 0;
+        while (i < cnt) {
+            // greedy packing of consecutive short lists
+            uint32_t nbv = 0, rows = 0;
+            uint32_t off[FB], nnv[FB], mv[FB], vid[FB];
+#pragma unroll
+            for (int b = 0; b < FB; ++b) { off[b] = 0; nnv[b] = 0; mv[b] = 0; vid[b] = 0; }
+            while (i < cnt && nbv < FB) {
+                const uint32_t nn = (uint32_t)__shfl((int)cn, (int)i, NNM_WAVE);
+                const uint32_t m = nn + (uint32_t)__shfl((int)co, (int)i, NNM_WAVE);
+                if (m > JS || nn == 0) {  // long lists: k_join_f32_windows; no new entries: no pairs
+                    if (m <= JS && lane == 0) my_entries += m;
+                    ++i;
+                    continue;
+                }
+                if (rows + m > JS) break;
+#pragma unroll
+                for (int b = 0; b < FB; ++b)
+                    if ((uint32_t)b == nbv) { off[b] = rows; nnv[b] = nn; mv[b] = m; vid[b] = i; }
+                rows += m;
+                ++nbv;
+                ++i;
+                if (lane == 0) my_entries += m;
+            }
+            if (nbv == 0) continue;
+            // window ids: lane j < rows holds window row j
+            wave_lds_sync();  // previous batch's readers are done with LDS
+            uint32_t wid = 0;
+            {
+                uint32_t b = 0;
+#pragma unroll
+                for (int q = 1; q < FB; ++q)
+                    if ((uint32_t)q < nbv && lane >= off[q]) b = q;
+                uint32_t ob = 0, nb_ = 0, vb = 0;
+#pragma unroll
+                for (int q = 0; q < FB; ++q)
+                    if ((uint32_t)q == b) { ob = off[q]; nb_ = nnv[q]; vb = vid[q]; }
+                if (lane < rows) {
+                    const uint64_t v = base + vb;
+                    const uint32_t e = lane - ob;
+                    wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
+                    L.meta[0][lane] = wid;
+                    L.meta[1][lane] = a.label[wid];
+                    L.meta[2][lane] = __float_as_uint(a.g_dist[(uint64_t)wid * a.K + a.K - 1]);
+                }
+            }
+            // rows -> LDS, one row per direct-to-LDS instruction
+            for (uint32_t r = 0; r < rows; ++r) {
+                const uint32_t id = (uint32_t)__shfl((int)wid, (int)r, NNM_WAVE);
+                if (lane < (uint32_t)CPR) {
+                    const float* g = a.X + (uint64_t)id * LD + 4 * lane;
                     __builtin_amdgcn_global_load_lds(
-                        (const void*)g, (__attribute__((address_space(3))) void*)(L.rows + r0 * LD), 16, 0, 0);
+                        (const void*)g, (__attribute__((address_space(3))) void*)(L.rows + r * PF), 16, 0, 0);
                 }
             }
+            __builtin_amdgcn_s_waitcnt(0);
+            wave_lds_sync();
+            join_packed<CPR, JS>(a, L, lane, nbv, off, nnv, mv, my_dists, my_offers);
         }
-        __builtin_amdgcn_s_waitcnt(0);
-        wave_lds_sync();
-        const uint32_t ntile = build_tiles(L.tiles, nn, m, true, lane);
-        wave_lds_sync();
-        const uint32_t CBn = (m + TN - 1) / TN;
-        for (uint32_t t0 = 0; t0 < ntile; t0 += NNM_WAVE * RND) {
-            float acc[RND][TM * TN];
-            uint32_t rb[RND], cb[RND];
-            bool on[RND];
+    }
 #pragma unroll
-            for (int s = 0; s < RND; ++s) {
-                const uint32_t ti = t0 + lane + NNM_WAVE * s;
-                on[s] = ti < ntile;
-                const uint32_t t = on[s] ? L.tiles[ti] : 0u;
-                rb[s] = t / CBn;
-                cb[s] = t % CBn;
+    for (int o = 32; o > 0; o >>= 1) {
+        my_dists += __shfl_xor(my_dists, o, NNM_WAVE);
+        my_entries += __shfl_xor(my_entries, o, NNM_WAVE);
+        my_offers += __shfl_xor(my_offers, o, NNM_WAVE);
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+    }
+}
+
+// ---------------------------------------------------------------- BF16 MFMA join
+//
+// Chosen-precision form of the fast path: the Gram of a packed window
+// (concatenated short lists) on MFMA (v_mfma_f32_16x16x16_bf16, FP32
+// accumulation) and distances as n_i + n_j - 2 x_i.x_j with n the squared
+// norms of the bf16 rows.  On integer rows of magnitude <= 256 with d <= 128
+// every product and partial sum is an integer below 2^24, so the value equals
+// the FP32 difference form exactly.  Otherwise the offered key is the lower
+// bound d - eps, eps = s (2 sqrt(d) + s) + 1e-5 (n_i + n_j), s = 2^-8 (|x_i| +
+// |x_j|) (bf16 rounding moves each element by at most 2^-8 of itself), and the
+// update kernel recomputes every offered candidate in exact FP32.
+//
+// LDS: bf16 rows at a pitch of exactly one row, copied by 4-byte direct-to-LDS
+// loads (one row per instruction); the 8-byte unit u of row r sits at
+// u ^ (r & SWM), so the 16 lanes that read one MFMA operand column (16 rows,
+// one unit) hit 16 different bank pairs.
+typedef short nnm_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float nnm_f32x4 __attribute__((ext_vector_type(4)));
+
+#define BFB 8          // most lists packed into one bf16 window
+#define BJS 64         // bf16 window rows (one meta entry per lane)
+
+struct Bf16Lds {
+    uint2 rows[(BJS + 16) * 32];   // (window + 16 slack rows) x 32 units of 8 bytes
+    uint32_t meta[4][BJS];         // id, label, k-th distance bits, norm bits
+};
+
+template <int U>  // 8-byte units per bf16 row: ld = 4 * U
+__global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb,
+                                                  const float* __restrict__ norms, int exact) {
+    static_assert(U % 8 == 0 && U <= 32, "bf16 rows of 32..128 elements");
+    constexpr uint32_t SWM = (U % 16 == 0) ? 15u : 7u;
+    constexpr uint32_t DW = 2 * U;  // dwords per row
+    __shared__ __align__(16) Bf16Lds L;
+    const uint32_t lane = threadIdx.x;
+    uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
+    const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
+
+    for (;;) {
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)JOIN_CHUNK);
+        base = __shfl(base, 0, NNM_WAVE);
+        if (base >= a.n) break;
+        const uint32_t cnt = (uint32_t)min((uint64_t)JOIN_CHUNK, a.n - base);
+        uint32_t cn = 0, co = 0;
+        if (lane < cnt) {
+            cn = a.n_new[base + lane];
+            co = a.n_old[base + lane];
+        }
+        uint32_t i = 0;
+        while (i < cnt) {
+            uint32_t nbv = 0, rows = 0;
+            uint32_t off[BFB], nnv[BFB], mv[BFB], vid[BFB];
 #pragma unroll
-                for (int q = 0; q < TM * TN; ++q) acc[s][q] = 0.f;
+            for (int b = 0; b < BFB; ++b) { off[b] = 0; nnv[b] = 0; mv[b] = 0; vid[b] = 0; }
+            while (i < cnt && nbv < BFB) {
+                const uint32_t nn = (uint32_t)__shfl((int)cn, (int)i, NNM_WAVE);
+                const uint32_t m = nn + (uint32_t)__shfl((int)co, (int)i, NNM_WAVE);
+                if (m > BJS || nn == 0) {
+                    if (m <= BJS && lane == 0) my_entries += m;
+                    ++i;
+                    continue;
+                }
+                if (rows + m > BJS) break;
+#pragma unroll
+                for (int b = 0; b < BFB; ++b)
+                    if ((uint32_t)b == nbv) { off[b] = rows; nnv[b] = nn; mv[b] = m; vid[b] = i; }
+                rows += m;
+                ++nbv;
+                ++i;
+                if (lane == 0) my_entries += m;
             }
+            if (nbv == 0) continue;
+            wave_lds_sync();
+            uint32_t wid = 0;
+            {
+                uint32_t ob = 0, nb_ = 0, vb = 0;
 #pragma unroll
-            for (int s = 0; s < RND; ++s) {
-                if (!on[s]) continue;
-                const uint32_t ra = rb[s] * TM, rc = cb[s] * TN;
-                for (uint32_t c = 0; c < CPR; ++c) {
-                    float4 x[TM], y[TN];
-#pragma unroll
-                    for (int i = 0; i < TM; ++i) x[i] = *lds_chunk<CPR>(L.rows, ra + i, c);
-#pragma unroll
-                    for (int j = 0; j < TN; ++j) y[j] = *lds_chunk<CPR>(L.rows, rc + j, c);
-#pragma unroll
-                    for (int i = 0; i < TM; ++i)
-#pragma unroll
-                        for (int j = 0; j < TN; ++j) {
-                            float t1 = x[i].x - y[j].x;
-                            float r = __builtin_fmaf(t1, t1, acc[s][i * TN + j]);
-                            t1 = x[i].y - y[j].y;
-                            r = __builtin_fmaf(t1, t1, r);
-                            t1 = x[i].z - y[j].z;
-                            r = __builtin_fmaf(t1, t1, r);
-                            t1 = x[i].w - y[j].w;
-                            acc[s][i * TN + j] = __builtin_fmaf(t1, t1, r);
-                        }
+                for (int q = 0; q < BFB; ++q)
+                    if ((uint32_t)q < nbv && lane >= off[q]) { ob = off[q]; nb_ = nnv[q]; vb = vid[q]; }
+                if (lane < rows) {
+                    const uint64_t v = base + vb;
+                    const uint32_t e = lane - ob;
+                    wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
+                    L.meta[0][lane] = wid;
+                    L.meta[1][lane] = a.label[wid];
+                    L.meta[2][lane] = __float_as_uint(a.g_dist[(uint64_t)wid * a.K + a.K - 1]);
+                    L.meta[3][lane] = __float_as_uint(norms[wid]);
                 }
             }
+            // rows -> LDS: lane l of a row instruction fetches dword 2u + (l & 1), u = (l/2) ^ (r & SWM)
+            for (uint32_t r = 0; r < rows; ++r) {
+                const uint32_t id = (uint32_t)__shfl((int)wid, (int)r, NNM_WAVE);
+                if (lane < DW) {
+                    const uint32_t u = (lane >> 1) ^ (r & SWM);
+                    const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
+                    __builtin_amdgcn_global_load_lds((const void*)g,
+                                                     (__attribute__((address_space(3))) void*)(L.rows + r * U), 4, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            wave_lds_sync();
+            const uint32_t lr = lane & 15u, lg = lane >> 4;
+            for (uint32_t b = 0; b < nbv; ++b) {
+                uint32_t ob = 0, nb_ = 0, mb = 0;
 #pragma unroll
-            for (int s = 0; s < RND; ++s)
-                if (on[s]) tile_epilogue(a, acc[s], rb[s], cb[s], nn, m, true, L.metaB, L.metaB, my_dists, my_offers);
+                for (int q = 0; q < BFB; ++q)
+                    if ((uint32_t)q == b) { ob = off[q]; nb_ = nnv[q]; mb = mv[q]; }
+                const uint32_t RB = (nb_ + 15) / 16, CB = (mb + 15) / 16;
+                for (uint32_t rb = 0; rb < RB; ++rb) {
+                    for (uint32_t cb = rb; cb < CB; ++cb) {
+                        const uint32_t ra = ob + rb * 16 + lr, rc = ob + cb * 16 + lr;
+                        nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
+                            const uint32_t u = ks * 4 + lg;
+                            const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
+                            const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
+                            nnm_bf16x4 A, B;
+                            A[0] = (short)(av.x & 0xFFFFu); A[1] = (short)(av.x >> 16);
+                            A[2] = (short)(av.y & 0xFFFFu); A[3] = (short)(av.y >> 16);
+                            B[0] = (short)(bv.x & 0xFFFFu); B[1] = (short)(bv.x >> 16);
+                            B[2] = (short)(bv.y & 0xFFFFu); B[3] = (short)(bv.y >> 16);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc, 0, 0, 0);
+                        }
+                        // lane holds G[4 lg + q][lr] of the block
+                        const uint32_t cj = cb * 16 + lr;
+                        const bool colok = cj < mb;
+                        uint32_t bid = 0, blab = 0;
+                        float bthr = 0.f, bn = 0.f;
+                        if (colok) {
+                            bid = L.meta[0][ob + cj];
+                            blab = L.meta[1][ob + cj];
+                            bthr = __uint_as_float(L.meta[2][ob + cj]);
+                            bn = __uint_as_float(L.meta[3][ob + cj]);
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const uint32_t ri = rb * 16 + 4 * lg + q;
+                            if (!colok || ri >= nb_ || cj <= ri) continue;
+                            const uint32_t aid = L.meta[0][ob + ri], alab = L.meta[1][ob + ri];
+                            if (aid == bid) continue;
+                            if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
+                            const float an = __uint_as_float(L.meta[3][ob + ri]);
+                            float dd = fmaxf((an + bn) - 2.f * acc[q], 0.f);
+                            if (!exact) {
+                                const float s = 0.00390625f * (sqrtf(an) + sqrtf(bn)) * 1.01f;
+                                const float eps = s * (2.f * sqrtf(dd) + s) + 1e-5f * (an + bn);
+                                dd = fmaxf(dd - eps, 0.f);
+                            }
+                            ++my_dists;
+                            if (dd < bthr) {
+                                offer(a.slots, a.C, a.salt, bid, aid, dd);
+                                ++my_offers;
+                            }
+                            if (dd < __uint_as_float(L.meta[2][ob + ri])) {
+                                offer(a.slots, a.C, a.salt, aid, bid, dd);
+                                ++my_offers;
+                            }
+                        }
+                    }
+                }
+            }
         }
     }
 #pragma unroll
@@ -425,7 +647,7 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
 }
 
 static uint32_t fast_window(uint32_t ld) {
-    const uint32_t rows = ROW_LDS_BYTES / (ld * 4);
+    const uint32_t rows = ROW_LDS_BYTES / ((ld + 4) * 4);
     return rows < 64 ? rows : 64;
 }
 
@@ -456,6 +678,28 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
     a.g_dist = c->g_dist;
     a.slots = c->slots;
     a.counters = c->counters;
+    if (p->prec == NNM_PREC_BF16) {
+        if (c->ld > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
+        const int ex = c->data_exact_bf16;
+#define NNM_BF(U)                                                                                          \
+    case 4 * U:                                                                                            \
+        hipLaunchKernelGGL(k_join_bf16<U>, dim3(join_grid(c, k_join_bf16<U>)), dim3(64), 0, c->stream, a,  \
+                           c->Xb, c->norms, ex);                                                           \
+        break;
+        switch (c->ld) {
+            NNM_BF(8)
+            NNM_BF(16)
+            NNM_BF(24)
+            NNM_BF(32)
+            default: return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join row length");
+        }
+#undef NNM_BF
+        NNM_LAUNCH_CHECK(c);
+        hipLaunchKernelGGL(k_join_f32_windows, dim3(join_grid(c, k_join_f32_windows)), dim3(64), 0, c->stream, a,
+                           (uint32_t)BJS);
+        NNM_LAUNCH_CHECK(c);
+        return NNM_OK;
+    }
     if (p->prec != NNM_PREC_FP32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "join precision not built yet");
     // fast path for lists of at most js entries, windows kernel for the rest
     uint32_t js = 0;

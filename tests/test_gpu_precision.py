@@ -1,0 +1,84 @@
+"""Chosen precision: the BF16 MFMA join against the plain FP32 join.
+
+On integer rows of magnitude <= 256 (d <= 128) the bf16 norm form is exact,
+so the BF16 run must reproduce the FP32 run bit for bit (same distances
+counted, same lists).  On real-valued rows the join offers lower bounds and
+every candidate that reaches a list is recomputed in exact FP32: stored
+distances must equal the canonical FP32 pair distance, and recall must stay
+within 0.005 of the FP32 run.
+"""
+import numpy as np
+import pytest
+
+import nnm
+import nnm_data
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with nnm.Context(0) as c:
+        yield c
+
+
+def _run(ctx, X, prec, rule=nnm.RULE_ALL, seed=5, iters=8):
+    ctx.set_data(X)
+    ctx.random_init(20, seed=seed)
+    st = ctx.nndescent(nnm.default_params(K=20, rule=rule, prec=prec, seed=seed, max_iters=iters))
+    ids, dist, flags = ctx.get_graph()
+    return ids, dist, flags, st
+
+
+@pytest.mark.parametrize("d", [128, 64])
+def test_bf16_equals_fp32_on_integer_rows(ctx, d):
+    X = nnm_data.sift_like(20000, d, centers=64, seed=50 + d)
+    i32, d32, f32, s32 = _run(ctx, X, nnm.PREC_FP32)
+    i16, d16, f16, s16 = _run(ctx, X, nnm.PREC_BF16)
+    assert s16["dists"] == s32["dists"] and s16["iters"] == s32["iters"]
+    assert s16["reranked"] == 0
+    assert np.array_equal(i16, i32) and np.array_equal(d16, d32) and np.array_equal(f16, f32)
+
+
+def test_bf16_merge_equals_fp32_on_integer_rows(ctx):
+    X = nnm_data.sift_like(16000, 128, centers=64, seed=61)
+    n, h, K = 16000, 8000, 20
+    ctx.set_data(X)
+    ctx.set_partition([0, h, n], [1, 1], [0, 1])
+    ctx.random_init(K, seed=2)
+    ctx.nndescent(nnm.default_params(K=K, seed=2))
+    sub, _, _ = ctx.get_graph()
+    out = {}
+    for prec in (nnm.PREC_FP32, nnm.PREC_BF16):
+        ctx.set_partition([0, h, n], [1, 0], [0, 0])
+        ctx.set_graph(sub)
+        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, prec=prec, rho=0.5, max_iters=10, seed=3)
+        ctx.merge_init(p)
+        st = ctx.nndescent(p)
+        ctx.merge_finish(True)
+        out[prec] = (ctx.get_graph(), st)
+    (g32, s32), (g16, s16) = out[nnm.PREC_FP32], out[nnm.PREC_BF16]
+    assert s32["dists"] == s16["dists"]
+    for a, b in zip(g32, g16):
+        assert np.array_equal(a, b)
+
+
+def test_bf16_real_rows_rerank_exact(ctx):
+    X = nnm_data.unit_mixture(20000, 96, centers=64, sigma=0.35, seed=62)
+    rows = np.arange(0, 20000, 10, dtype=np.uint32)
+    ctx.set_data(X)
+    gt, _ = ctx.bruteforce(10, rows=rows)
+    i32, d32, _, s32 = _run(ctx, X, nnm.PREC_FP32, iters=12)
+    i16, d16, _, s16 = _run(ctx, X, nnm.PREC_BF16, iters=12)
+    assert s16["reranked"] > 0
+    v = np.repeat(np.arange(20000, dtype=np.uint32), 20)
+    ref = ctx.pair_dists(v, i16.ravel())
+    assert np.array_equal(ref, d16.ravel()), "a stored distance is not the exact FP32 value"
+    # FP32 values within 1e-5 relative of FP64
+    sel = np.arange(0, v.size, 37)
+    ex = ((X[v[sel]].astype(np.float64) - X[i16.ravel()[sel]].astype(np.float64)) ** 2).sum(1)
+    assert (np.abs(d16.ravel()[sel] - ex) <= 1e-5 * np.maximum(ex, 1e-6)).all()
+    r32, r16 = nnm.recall(i32, gt, rows), nnm.recall(i16, gt, rows)
+    print("fp32", r32, s32, "bf16", r16, s16)
+    assert r16 >= r32 - 0.005
