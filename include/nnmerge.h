@@ -43,7 +43,9 @@ extern "C" {
 /* Distance precision of the local join. */
 #define NNM_PREC_FP32 0    /* FP32 difference form on the vector ALUs */
 #define NNM_PREC_BF16 1    /* BF16 MFMA norm form, FP32 re-rank */
-#define NNM_PREC_LEARNED 2 /* learned projection prefilter, FP32 re-rank */
+#define NNM_PREC_LEARNED 2 /* learned projection prefilter, FP32 re-rank;
+                              without a trained model every iteration uses
+                              the plain BF16 join */
 
 /* Pair rule of the local join. */
 #define NNM_RULE_ALL 0   /* every pair with distinct ids: KGraph::join
@@ -78,6 +80,8 @@ typedef struct nnm_params {
                            two, 16..64); 0 = 64 */
     uint32_t cut;       /* merge set-up: far entries replaced by cross
                            samples; 0 = K/2 (cutSize, src/smerge.cpp:99) */
+    uint32_t plain_tail;/* NNM_PREC_LEARNED with max_iters: the last
+                           plain_tail iterations use the plain BF16 join */
     uint64_t seed;      /* every random choice derives from this seed */
 } nnm_params;
 
@@ -95,6 +99,8 @@ typedef struct nnm_stats {
     double last_delta;     /* mean fraction of new entries after the loop */
     double secs;           /* device time of the call */
     double join_secs;      /* device time spent in the join kernels */
+    uint32_t learned_iters;/* iterations joined through the learned prefilter */
+    uint32_t plain_iters;  /* iterations joined with the plain join */
 } nnm_stats;
 
 void nnm_default_params(nnm_params* p);
@@ -196,6 +202,30 @@ int nnm_bruteforce_knn(nnm_ctx* ctx, const float* Q, const uint32_t* rows, uint6
 /* Squared L2 between row pairs (a[i], b[i]) in the library's FP32 order:
  * the DISTFUNC call of src/abstractgraph.h:11-12 batched. */
 int nnm_pair_dists(nnm_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t m, float* out);
+
+/* --- learned candidate pruning ---------------------------------------- */
+/* The reference has no learned component; these entry points serve the
+ * NNM_PREC_LEARNED join.  A model f(x) = W2 relu(W1 a x + b1) + b2
+ * (d -> hidden -> d_out, FP32 master weights) is trained on the GPU so that
+ * |f(x) - f(y)|^2 ~ a^2 |x - y|^2 on pairs taken from the current lists (list
+ * neighbours and two-hop candidates, the pairs the local join evaluates).
+ * nnm_mlp_init: He-initialised model, input scale a from the data.
+ * nnm_mlp_train: `steps` Adam steps on `batch` fresh pairs each; afterwards the
+ *   prefilter margin tau is the `keep` quantile of p / kth over true
+ *   improvements of a fresh sample.
+ * nnm_mlp_loss_grad: loss and gradients of one batch given by the caller
+ *   (rows already scaled; bf16 = 0 uses FP32 GEMMs), for checking.
+ * nnm_mlp_project: projection of every row (used by the learned join);
+ *   out (n x d_out floats) may be NULL. */
+int nnm_mlp_init(nnm_ctx* ctx, uint32_t hidden, uint32_t d_out, uint64_t seed);
+int nnm_mlp_set_weights(nnm_ctx* ctx, const float* W1, const float* b1, const float* W2, const float* b2,
+                        float alpha);
+int nnm_mlp_get_weights(nnm_ctx* ctx, float* W1, float* b1, float* W2, float* b2, float* alpha, float* tau);
+int nnm_mlp_loss_grad(nnm_ctx* ctx, const float* xa, const float* xb, const float* t, const float* s, uint32_t B,
+                      int bf16, float* loss, float* gW1, float* gb1, float* gW2, float* gb2);
+int nnm_mlp_train(nnm_ctx* ctx, uint32_t steps, uint32_t batch, float lr, float keep, uint64_t seed,
+                  float* loss_first, float* loss_last, double* secs);
+int nnm_mlp_project(nnm_ctx* ctx, float* out, double* secs);
 
 /* --- host-class drivers ----------------------------------------------- */
 /* Runs one host class end to end (nn-merge_amd/host): kind 0 = KGraph

@@ -65,11 +65,6 @@ static int alloc_n(nnm_ctx* c, T** p, size_t n) {
     return nnm_alloc(c, reinterpret_cast<void**>(p), n * sizeof(T));
 }
 
-#define NNM_TRY(expr)              \
-    do {                           \
-        int _r = (expr);           \
-        if (_r != NNM_OK) return _r; \
-    } while (0)
 
 // Fill kernels are used instead of hipMemsetAsync in the iteration loop so the
 // loop can be captured into a HIP graph.
@@ -124,6 +119,7 @@ extern "C" void nnm_default_params(nnm_params* p) {
     p->rerank = NNM_RERANK_AUTO;
     p->slots = 64;
     p->cut = 0;
+    p->plain_tail = 1;
     p->seed = 1;
 }
 
@@ -153,6 +149,7 @@ extern "C" int nnm_destroy(nnm_ctx* c) {
     if (!c) return NNM_ERR_INVALID;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    nnm_mlp_free(c);
     for (auto& a : c->allocs) hipFree(a.first);
     c->allocs.clear();
     hipEventDestroy(c->ev0);
@@ -190,7 +187,7 @@ static void free_graph(nnm_ctx* c) {
 static void free_iter(nnm_ctx* c) {
     void** ps[] = {(void**)&c->l_new, (void**)&c->l_old, (void**)&c->n_new, (void**)&c->n_old,
                    (void**)&c->newmask, (void**)&c->rcnt, (void**)&c->roff, (void**)&c->rcur,
-                   (void**)&c->rbuf, (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->thr};
+                   (void**)&c->rbuf, (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->kth};
     for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
     c->cap_new = c->cap_old = c->rev_cap = c->slots_C = 0;
 }
@@ -245,6 +242,7 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
     free_graph(c);
     free_iter(c);
     free_partition(c);
+    nnm_mlp_invalidate_projection(c);
     nnm_free(c, c->X); c->X = nullptr;
     nnm_free(c, c->Xb); c->Xb = nullptr;
     nnm_free(c, c->norms); c->norms = nullptr;
@@ -798,9 +796,17 @@ extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
 
 // ================================================================ list build
 
+// k-th distance of every list, compact (n floats stay in the last-level cache
+// for the random lookups of the list build and the join).
+__global__ void k_kth(uint64_t n, uint32_t K, const float* __restrict__ g_dist, float* __restrict__ kth) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) kth[v] = g_dist[v * K + K - 1];
+}
+
 // Forward lists, new-sample mask, reverse counts.  One thread per row.
 __global__ void k_lb_count(uint64_t n, uint32_t K, uint32_t S, const uint32_t* __restrict__ g_ids,
-                           const float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
+                           const float* __restrict__ g_dist, const float* __restrict__ kthv,
+                           uint8_t* __restrict__ g_flag,
                            uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
                            uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                            uint32_t* __restrict__ newmask, uint32_t* __restrict__ rcnt) {
@@ -813,7 +819,7 @@ __global__ void k_lb_count(uint64_t n, uint32_t K, uint32_t S, const uint32_t* _
     for (uint32_t e = 0; e < K; ++e) {
         const uint32_t u = ids[e];
         const float de = dd[e];
-        const float kth = g_dist[(uint64_t)u * K + K - 1];
+        const float kth = kthv[u];
         if (fl[e]) {
             if (nn >= S) continue;  // forward sampling: nearest S new entries
             l_new[v * cap_new + nn++] = u;
@@ -832,7 +838,8 @@ __global__ void k_lb_count(uint64_t n, uint32_t K, uint32_t S, const uint32_t* _
 
 // Reverse entries into CSR buckets.
 __global__ void k_lb_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g_ids,
-                          const float* __restrict__ g_dist, const uint8_t* __restrict__ g_flag,
+                          const float* __restrict__ g_dist, const float* __restrict__ kthv,
+                          const uint8_t* __restrict__ g_flag,
                           const uint32_t* __restrict__ newmask, const uint32_t* __restrict__ roff,
                           uint32_t* __restrict__ rcur, uint32_t* __restrict__ rbuf) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -846,7 +853,7 @@ __global__ void k_lb_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g
         const bool is_old = !is_new && fl[e] == 0;
         if (!is_new && !is_old) continue;
         const uint32_t u = ids[e];
-        if (!(g_dist[(uint64_t)u * K + K - 1] < dd[e])) continue;
+        if (!(kthv[u] < dd[e])) continue;
         const uint64_t b = is_new ? u : n + u;
         uint32_t slot = atomicAdd(&rcur[b], 1u);
         rbuf[roff[b] + slot] = (uint32_t)v;
@@ -991,7 +998,7 @@ __global__ void k_scan_apply(const uint32_t* __restrict__ in, uint64_t m, const 
 // are reset to empty.  With rerank, candidates are recomputed in exact FP32
 // before ranking.
 __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X, uint32_t ld,
-                         int rerank, float rerank_slack, uint32_t* __restrict__ g_ids,
+                         int rerank, float rerank_tau, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
                          uint64_t* __restrict__ counters) {
     const uint32_t lane = nnm_lane();
@@ -1022,7 +1029,7 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
                 // approximate keys carry a bounded error: re-rank every candidate
                 // that can still cross the k-th entry
                 const float kd = nnm_key_dist(kth_key);
-                valid = valid && (nnm_key_dist(skey) <= kd + rerank_slack);
+                valid = valid && (nnm_key_dist(skey) <= kd * rerank_tau);
                 if (valid) {
                     float ex = nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
                     skey = nnm_key(ex, sid);
@@ -1103,6 +1110,7 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     NNM_TRY(alloc_n(c, &c->rbuf, n * K));
     NNM_TRY(alloc_n(c, &c->scan_tmp, (2 * n + SCAN_TILE - 1) / SCAN_TILE + 1));
     NNM_TRY(alloc_n(c, &c->slots, n * C));
+    NNM_TRY(alloc_n(c, &c->kth, n));
     NNM_TRY(fill_u64(c, c->slots, n * C, NNM_EMPTY_KEY));
     return NNM_OK;
 }
@@ -1132,8 +1140,10 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
     const uint32_t salt = (uint32_t)iseed, salt2 = (uint32_t)(iseed >> 32);
     NNM_TRY(fill_u32(c, c->rcnt, 2 * n, 0));
     NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
+    hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->kth);
+    NNM_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_lb_count, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, S, c->g_ids,
-                       c->g_dist, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
+                       c->g_dist, c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
                        c->newmask, c->rcnt);
     NNM_LAUNCH_CHECK(c);
     const uint64_t m = 2 * n;
@@ -1145,7 +1155,7 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
     hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, c->stream, c->rcnt, m, c->scan_tmp, c->roff);
     NNM_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_lb_fill, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_ids,
-                       c->g_dist, c->g_flag, c->newmask, c->roff, c->rcur, c->rbuf);
+                       c->g_dist, c->kth, c->g_flag, c->newmask, c->roff, c->rcur, c->rbuf);
     NNM_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
                        c->rbuf, c->rev_cap, salt2, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
@@ -1155,14 +1165,16 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
     NNM_TRY(nnm_launch_join(c, p, iter, salt));
     NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
     int rerank = 0;
-    float slack = 0.f;
-    if (p->prec != NNM_PREC_FP32) {
-        rerank = (p->rerank == NNM_RERANK_ALWAYS) ||
-                 (p->rerank == NNM_RERANK_AUTO && !(p->prec == NNM_PREC_BF16 && c->data_exact_bf16));
-        slack = 0.f;
+    float tau = 1.f;
+    if (p->prec == NNM_PREC_BF16) {
+        rerank = (p->rerank == NNM_RERANK_ALWAYS) || (p->rerank == NNM_RERANK_AUTO && !c->data_exact_bf16);
+    } else if (p->prec == NNM_PREC_LEARNED) {
+        rerank = 1;  // projected distances are estimates: every offer is recomputed
+        tau = nnm_mlp_tau(c);
     }
+    if (p->rerank == NNM_RERANK_NEVER) rerank = 0;
     hipLaunchKernelGGL(k_update, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, c->slots_C, c->X,
-                       c->ld, rerank, slack, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
+                       c->ld, rerank, tau, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
     NNM_LAUNCH_CHECK(c);
     if (join_ms) {
         NNM_HIP(c, hipEventSynchronize(c->evj1));
@@ -1176,26 +1188,36 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
 static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool single, nnm_stats* st) {
     NNM_TRY(validate_params(c, p));
     NNM_TRY(nnm_ensure_iter_buffers(c, p));
-    if (p->prec == NNM_PREC_BF16) NNM_TRY(nnm_ensure_bf16(c));
+    // learned prefilter: needs a trained model; otherwise the plain BF16 join
+    // runs every iteration (the fallback)
+    bool learned = p->prec == NNM_PREC_LEARNED && nnm_mlp_trained(c);
+    if (learned) NNM_TRY(nnm_mlp_ensure_projection(c, nullptr));
+    if (p->prec != NNM_PREC_FP32) NNM_TRY(nnm_ensure_bf16(c));
     stats_reset(st);
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
     double join_ms = 0.0;
-    uint32_t it = 0;
+    uint32_t it = 0, n_learned = 0, n_plain = 0;
     int converged = 0;
     double last_delta = 1.0;
-    uint64_t prev[CNT_N] = {0};
     for (;;) {
         const uint32_t iter = single ? only_iter : it;
+        // the last plain_tail iterations of a capped learned run use the
+        // plain join, which catches what the prefilter dropped
+        nnm_params q = *p;
+        if (p->prec == NNM_PREC_LEARNED) {
+            const bool tail = p->max_iters && iter + p->plain_tail >= p->max_iters;
+            q.prec = (learned && !tail) ? NNM_PREC_LEARNED : NNM_PREC_BF16;
+        }
+        if (q.prec == NNM_PREC_LEARNED) ++n_learned; else ++n_plain;
         NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
         NNM_TRY(fill_u64(c, c->counters + CNT_WORK, 1, 0));
-        NNM_TRY(fill_u64(c, c->counters + CNT_WORK2, 1, 0));
-        NNM_TRY(run_iteration(c, p, iter, &join_ms));
+        NNM_TRY(fill_u64(c, c->counters + CNT_WORK2, 2, 0));
+        NNM_TRY(run_iteration(c, &q, iter, &join_ms));
         uint64_t h[CNT_N];
         NNM_TRY(read_counters(c, h));
         ++it;
         last_delta = (double)h[CNT_FLAGS] / ((double)c->n * c->K);
-        (void)prev;
         if (last_delta < p->delta) { converged = 1; break; }
         if (single) break;
         if (p->max_iters && it >= p->max_iters) break;
@@ -1212,6 +1234,8 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->list_entries = h[CNT_LIST_ENTRIES];
         st->offers = h[CNT_OFFERS];
         st->last_delta = last_delta;
+        st->learned_iters = n_learned;
+        st->plain_iters = n_plain;
         float ms = 0.f;
         hipEventElapsedTime(&ms, c->ev0, c->ev1);
         st->secs = ms * 1e-3;

@@ -37,6 +37,7 @@ enum CounterIdx {
     CNT_TOTAL_REV = 6,
     CNT_OFFERS = 7,    // slot offers issued by the join
     CNT_WORK2 = 8,     // work-queue cursor of the long-list join
+    CNT_WORK3 = 9,     // work-queue cursor of the window join
     CNT_N = 16
 };
 
@@ -100,9 +101,11 @@ struct nnm_ctx {
     uint32_t* scan_tmp = nullptr;   // block sums
     uint64_t* slots = nullptr;      // n x slots_C
     uint64_t* counters = nullptr;   // CNT_N
-    uint32_t* thr = nullptr;        // n: k-th distance bits at iteration start
+    float* kth = nullptr;           // n: k-th distance of every list at iteration start
     uint32_t* grp_flags = nullptr;  // ngrp: per-group new-flag counts
     uint8_t* grp_active = nullptr;  // ngrp
+
+    void* mlp = nullptr;            // learned projection state (nnm_mlp.hip)
 
     std::vector<std::pair<void*, size_t>> allocs;
 };
@@ -128,8 +131,26 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
         if (_e != hipSuccess) return nnm_hip_fail((c), _e, "kernel launch"); \
     } while (0)
 
+#define NNM_TRY(expr)                \
+    do {                             \
+        int _r = (expr);             \
+        if (_r != NNM_OK) return _r; \
+    } while (0)
+#define NNM_TRY_MLP NNM_TRY
+
 // Kernel launchers shared between translation units.
 int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t salt);
+
+// Learned projection (nnm_mlp.hip)
+void nnm_mlp_free(nnm_ctx* c);
+int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs);
+void nnm_mlp_invalidate_projection(nnm_ctx* c);
+const uint16_t* nnm_mlp_rows(nnm_ctx* c);
+const float* nnm_mlp_norms(nnm_ctx* c);
+uint32_t nnm_mlp_dout(nnm_ctx* c);
+float nnm_mlp_scale(nnm_ctx* c);
+float nnm_mlp_tau(nnm_ctx* c);
+int nnm_mlp_trained(nnm_ctx* c);
 
 // ---------------------------------------------------------- device helpers
 

@@ -59,6 +59,7 @@ struct JoinArgs {
     const uint32_t* n_old;
     const uint32_t* label;
     const float* g_dist;
+    const float* kth;
     uint64_t* slots;
     uint64_t* counters;
 };
@@ -154,13 +155,13 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
                 const uint32_t id = idx < nn ? Ln[idx] : Lo[idx - nn];
                 L.metaB[0][lane] = id;
                 L.metaB[1][lane] = a.label[id];
-                L.metaB[2][lane] = __float_as_uint(a.g_dist[(uint64_t)id * a.K + a.K - 1]);
+                L.metaB[2][lane] = __float_as_uint(a.kth[id]);
             }
             if (!diag && lane < na) {
                 const uint32_t id = Ln[a0 + lane];
                 L.metaA[0][lane] = id;
                 L.metaA[1][lane] = a.label[id];
-                L.metaA[2][lane] = __float_as_uint(a.g_dist[(uint64_t)id * a.K + a.K - 1]);
+                L.metaA[2][lane] = __float_as_uint(a.kth[id]);
             }
             const float* rA = diag ? rowsB : rowsA;
             const uint32_t(*mA)[JW] = diag ? L.metaB : L.metaA;
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
                     L.meta[0][lane] = wid;
                     L.meta[1][lane] = a.label[wid];
-                    L.meta[2][lane] = __float_as_uint(a.g_dist[(uint64_t)wid * a.K + a.K - 1]);
+                    L.meta[2][lane] = __float_as_uint(a.kth[wid]);
                 }
             }
             // rows -> LDS, one row per direct-to-LDS instruction
@@ -456,6 +457,23 @@ typedef short nnm_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float nnm_f32x4 __attribute__((ext_vector_type(4)));
 
 #define BFB 8          // most lists packed into one bf16 window
+
+#define KEY_EXACT 0    // bf16 norm form is exact (integer rows)
+#define KEY_BOUND 1    // offer the lower bound d - eps; exact re-rank later
+#define KEY_LEARN 2    // learned projection: estimate kscale * d_proj, margin tau
+
+// Offer key of a pair from the squared norms and the Gram entry.
+__device__ inline float offer_key(float an, float bn, float g, int mode, float kscale) {
+    float dd = fmaxf((an + bn) - 2.f * g, 0.f);
+    if (mode == KEY_BOUND) {
+        const float s = 0.00390625f * (sqrtf(an) + sqrtf(bn)) * 1.01f;
+        const float eps = s * (2.f * sqrtf(dd) + s) + 1e-5f * (an + bn);
+        dd = fmaxf(dd - eps, 0.f);
+    } else if (mode == KEY_LEARN) {
+        dd *= kscale;
+    }
+    return dd;
+}
 #define BJS 64         // bf16 window rows (one meta entry per lane)
 
 struct Bf16Lds {
@@ -465,9 +483,9 @@ struct Bf16Lds {
 
 template <int U>  // 8-byte units per bf16 row: ld = 4 * U
 __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb,
-                                                  const float* __restrict__ norms, int exact) {
-    static_assert(U % 8 == 0 && U <= 32, "bf16 rows of 32..128 elements");
-    constexpr uint32_t SWM = (U % 16 == 0) ? 15u : 7u;
+                                                  const float* __restrict__ norms, int mode, float kscale, float tau) {
+    static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
+    constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;  // dwords per row
     __shared__ __align__(16) Bf16Lds L;
     const uint32_t lane = threadIdx.x;
@@ -522,7 +540,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
                     L.meta[0][lane] = wid;
                     L.meta[1][lane] = a.label[wid];
-                    L.meta[2][lane] = __float_as_uint(a.g_dist[(uint64_t)wid * a.K + a.K - 1]);
+                    L.meta[2][lane] = __float_as_uint(a.kth[wid]);
                     L.meta[3][lane] = __float_as_uint(norms[wid]);
                 }
             }
@@ -580,18 +598,13 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                             if (aid == bid) continue;
                             if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
                             const float an = __uint_as_float(L.meta[3][ob + ri]);
-                            float dd = fmaxf((an + bn) - 2.f * acc[q], 0.f);
-                            if (!exact) {
-                                const float s = 0.00390625f * (sqrtf(an) + sqrtf(bn)) * 1.01f;
-                                const float eps = s * (2.f * sqrtf(dd) + s) + 1e-5f * (an + bn);
-                                dd = fmaxf(dd - eps, 0.f);
-                            }
+                            const float dd = offer_key(an, bn, acc[q], mode, kscale);
                             ++my_dists;
-                            if (dd < bthr) {
+                            if (dd < tau * bthr) {
                                 offer(a.slots, a.C, a.salt, bid, aid, dd);
                                 ++my_offers;
                             }
-                            if (dd < __uint_as_float(L.meta[2][ob + ri])) {
+                            if (dd < tau * __uint_as_float(L.meta[2][ob + ri])) {
                                 offer(a.slots, a.C, a.salt, aid, bid, dd);
                                 ++my_offers;
                             }
@@ -614,6 +627,138 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
     }
 }
 
+// Long lists (BJS < m <= BLM) on BF16: one workgroup of 4 waves per vertex.
+// The whole list is staged in LDS (one row per 4-byte direct-to-LDS
+// instruction, same unit swizzle as the short-list kernel) and the 16x16
+// MFMA blocks of the upper part of the (new x list) Gram are dealt to the
+// four waves.
+#define BLM 256
+
+struct Bf16LongLds {
+    uint2 rows[(BLM + 16) * 32];
+    uint32_t meta[4][BLM];
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb,
+                                                        const float* __restrict__ norms, int mode, float kscale, float tau) {
+    constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
+    constexpr uint32_t DW = 2 * U;
+    __shared__ __align__(16) Bf16LongLds L;
+    __shared__ uint64_t vshare;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
+    const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
+    uint64_t cur = 0, end = 0;
+    for (;;) {
+        // block-wide queue over vertices: thread 0 takes chunks of rows
+        if (tid == 0) {
+            uint64_t v = cur;
+            for (;;) {
+                if (v >= end) {
+                    v = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(JOIN_CHUNK * 16));
+                    end = min(a.n, v + JOIN_CHUNK * 16);
+                    if (v >= a.n) break;
+                }
+                const uint32_t m = a.n_new[v] + a.n_old[v];
+                if (m > BJS && m <= BLM) break;
+                ++v;
+            }
+            vshare = v;
+            cur = v + 1;
+        }
+        __syncthreads();
+        const uint64_t v = vshare;
+        __syncthreads();
+        if (v >= a.n) break;
+        const uint32_t nn = a.n_new[v], no = a.n_old[v], m = nn + no;
+        if (tid == 0) my_entries += m;
+        for (uint32_t j = tid; j < m; j += blockDim.x) {
+            const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
+            L.meta[0][j] = id;
+            L.meta[1][j] = a.label[id];
+            L.meta[2][j] = __float_as_uint(a.kth[id]);
+            L.meta[3][j] = __float_as_uint(norms[id]);
+        }
+        __syncthreads();
+        // each wave copies every 4th row
+        for (uint32_t r = wave; r < m; r += 4) {
+            const uint32_t id = L.meta[0][r];
+            if (lane < DW) {
+                const uint32_t u = (lane >> 1) ^ (r & SWM);
+                const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
+                __builtin_amdgcn_global_load_lds((const void*)g,
+                                                 (__attribute__((address_space(3))) void*)(L.rows + r * U), 4, 0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const uint32_t lr = lane & 15u, lg = lane >> 4;
+        const uint32_t RB = (nn + 15) / 16, CB = (m + 15) / 16;
+        uint32_t blk = 0;
+        for (uint32_t rb = 0; rb < RB; ++rb) {
+            for (uint32_t cb = rb; cb < CB; ++cb, ++blk) {
+                if ((blk & 3u) != wave) continue;
+                const uint32_t ra = rb * 16 + lr, rc = cb * 16 + lr;
+                nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
+                    const uint32_t u = ks * 4 + lg;
+                    const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
+                    const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
+                    nnm_bf16x4 A, B;
+                    A[0] = (short)(av.x & 0xFFFFu); A[1] = (short)(av.x >> 16);
+                    A[2] = (short)(av.y & 0xFFFFu); A[3] = (short)(av.y >> 16);
+                    B[0] = (short)(bv.x & 0xFFFFu); B[1] = (short)(bv.x >> 16);
+                    B[2] = (short)(bv.y & 0xFFFFu); B[3] = (short)(bv.y >> 16);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc, 0, 0, 0);
+                }
+                const uint32_t cj = cb * 16 + lr;
+                const bool colok = cj < m;
+                uint32_t bid = 0, blab = 0;
+                float bthr = 0.f, bn = 0.f;
+                if (colok) {
+                    bid = L.meta[0][cj];
+                    blab = L.meta[1][cj];
+                    bthr = __uint_as_float(L.meta[2][cj]);
+                    bn = __uint_as_float(L.meta[3][cj]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t ri = rb * 16 + 4 * lg + q;
+                    if (!colok || ri >= nn || cj <= ri) continue;
+                    const uint32_t aid = L.meta[0][ri], alab = L.meta[1][ri];
+                    if (aid == bid) continue;
+                    if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
+                    const float an = __uint_as_float(L.meta[3][ri]);
+                    const float dd = offer_key(an, bn, acc[q], mode, kscale);
+                    ++my_dists;
+                    if (dd < tau * bthr) {
+                        offer(a.slots, a.C, a.salt, bid, aid, dd);
+                        ++my_offers;
+                    }
+                    if (dd < tau * __uint_as_float(L.meta[2][ri])) {
+                        offer(a.slots, a.C, a.salt, aid, bid, dd);
+                        ++my_offers;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // LDS reused by the next vertex
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        my_dists += __shfl_xor(my_dists, o, NNM_WAVE);
+        my_entries += __shfl_xor(my_entries, o, NNM_WAVE);
+        my_offers += __shfl_xor(my_offers, o, NNM_WAVE);
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+    }
+}
+
 // Lists longer than min_m entries (every list when min_m = 0): windows of 64.
 __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t min_m) {
     __shared__ __align__(16) JoinLds L;
@@ -622,7 +767,7 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
     for (;;) {
         uint64_t base = 0;
         if (lane == 0)
-            base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(JOIN_CHUNK * 16));
+            base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK3], (unsigned long long)(JOIN_CHUNK * 16));
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
         const uint64_t end = min(a.n, base + JOIN_CHUNK * 16);
@@ -658,6 +803,13 @@ static uint32_t join_grid(nnm_ctx* c, Kern k) {
     return (uint32_t)c->num_cus * (uint32_t)per_cu;
 }
 
+template <typename Kern>
+static uint32_t join_grid256(nnm_ctx* c, Kern k) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return (uint32_t)c->num_cus * (uint32_t)per_cu;
+}
+
 int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t salt) {
     (void)iter;
     JoinArgs a;
@@ -676,17 +828,35 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
     a.n_old = c->n_old;
     a.label = c->label;
     a.g_dist = c->g_dist;
+    a.kth = c->kth;
     a.slots = c->slots;
     a.counters = c->counters;
-    if (p->prec == NNM_PREC_BF16) {
-        if (c->ld > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
-        const int ex = c->data_exact_bf16;
+    if (p->prec == NNM_PREC_BF16 || p->prec == NNM_PREC_LEARNED) {
+        const uint16_t* rows = c->Xb;
+        const float* nrm = c->norms;
+        uint32_t rl = c->ld;
+        int mode = c->data_exact_bf16 ? KEY_EXACT : KEY_BOUND;
+        float kscale = 1.f, tau = 1.f;
+        if (p->prec == NNM_PREC_LEARNED) {
+            rows = nnm_mlp_rows(c);
+            nrm = nnm_mlp_norms(c);
+            rl = nnm_mlp_dout(c);
+            mode = KEY_LEARN;
+            kscale = nnm_mlp_scale(c);
+            tau = nnm_mlp_tau(c);
+            if (!rows) return nnm_fail(c, NNM_ERR_STATE, "learned join without a projection");
+        }
+        if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
 #define NNM_BF(U)                                                                                          \
     case 4 * U:                                                                                            \
         hipLaunchKernelGGL(k_join_bf16<U>, dim3(join_grid(c, k_join_bf16<U>)), dim3(64), 0, c->stream, a,  \
-                           c->Xb, c->norms, ex);                                                           \
+                           rows, nrm, mode, kscale, tau);                                                  \
+        NNM_LAUNCH_CHECK(c);                                                                               \
+        hipLaunchKernelGGL(k_join_bf16_long<U>, dim3(join_grid256(c, k_join_bf16_long<U>)), dim3(256), 0,  \
+                           c->stream, a, rows, nrm, mode, kscale, tau);                                    \
         break;
-        switch (c->ld) {
+        switch (rl) {
+            NNM_BF(4)
             NNM_BF(8)
             NNM_BF(16)
             NNM_BF(24)
@@ -695,9 +865,12 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
         }
 #undef NNM_BF
         NNM_LAUNCH_CHECK(c);
-        hipLaunchKernelGGL(k_join_f32_windows, dim3(join_grid(c, k_join_f32_windows)), dim3(64), 0, c->stream, a,
-                           (uint32_t)BJS);
-        NNM_LAUNCH_CHECK(c);
+        // lists beyond the long-list LDS window (rev_cap above 118) stay on FP32
+        if (c->cap_new + c->cap_old > BLM) {
+            hipLaunchKernelGGL(k_join_f32_windows, dim3(join_grid(c, k_join_f32_windows)), dim3(64), 0, c->stream,
+                               a, (uint32_t)BLM);
+            NNM_LAUNCH_CHECK(c);
+        }
         return NNM_OK;
     }
     if (p->prec != NNM_PREC_FP32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "join precision not built yet");

@@ -1,0 +1,584 @@
+// Learned candidate pruning: a small MLP projection f(x) = W2 relu(W1 a x + b1) + b2
+// (d_in -> hidden -> d_out, FP32 master weights, BF16 MFMA or FP32 GEMMs),
+// trained so that |f(x) - f(y)|^2 approximates a^2 |x - y|^2 on pairs that
+// NN-Descent itself produces (list neighbours and two-hop candidates).  The
+// projected rows feed the same join kernels as the BF16 path; every
+// candidate they offer is recomputed in exact FP32 before it can enter a
+// list.  The reference has no learned component; the pair loss, gradients
+// and optimizer are restated in FP64 by the CPU oracle for the tests.
+//
+// Loss over a batch of B pairs (a_i, b_i) with exact squared distance t_i and
+// anchor k-th distance s_i (both scaled by a^2):
+//   p_i = |f(x_a) - f(x_b)|^2,  r_i = (p_i - t_i) / (t_i + s_i),  L = mean r_i^2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nnm_internal.hpp"
+
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+
+struct nnm_mlp_state {
+    uint32_t din = 0, hid = 0, dout = 0;
+    float alpha = 1.f;  // input scale
+    float tau = 1.f;    // prefilter margin on projected distances
+    int trained = 0;
+    // master weights and Adam moments (FP32), gradients
+    float *W1 = nullptr, *b1 = nullptr, *W2 = nullptr, *b2 = nullptr;
+    float *m = nullptr, *v = nullptr, *g = nullptr;  // packed [W1 b1 W2 b2]
+    uint64_t nparam = 0;
+    uint64_t step = 0;
+    // batch scratch
+    uint32_t cap_rows = 0;
+    float *xs = nullptr, *pre = nullptr, *h = nullptr, *z = nullptr, *dz = nullptr, *dh = nullptr;
+    float *t = nullptr, *s = nullptr, *loss = nullptr;
+    uint32_t* pairs = nullptr;
+    // projection of every row (bf16) and its squared norms
+    uint16_t* P = nullptr;
+    float* pnorm = nullptr;
+    uint64_t projected_n = 0;
+};
+
+static nnm_mlp_state* mlp_of(nnm_ctx* c) { return reinterpret_cast<nnm_mlp_state*>(c->mlp); }
+
+__device__ inline short mlp_to_bf16(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+// C[M x N] = op(A)[M x K] * op(B)[K x N] (+ bias[n]) with A(m,k) = A[m*am + k*ak],
+// B(k,n) = B[k*bk + n*bn]; M, N, K multiples of 16.  One wave per 16x16 tile.
+// BF16: operands rounded to bf16, MFMA with FP32 accumulation; otherwise FP32
+// fused multiply-add in k order.
+template <bool BF16>
+__global__ void k_gemm(uint32_t M, uint32_t N, uint32_t K, const float* __restrict__ A, uint32_t am, uint32_t ak,
+                       const float* __restrict__ B, uint32_t bk, uint32_t bn, const float* __restrict__ bias,
+                       float* __restrict__ C, uint32_t cm) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / NNM_WAVE;
+    const uint32_t lane = threadIdx.x % NNM_WAVE;
+    const uint32_t tiles_n = N / 16;
+    if (wave >= (M / 16) * tiles_n) return;
+    const uint32_t m0 = (wave / tiles_n) * 16, n0 = (wave % tiles_n) * 16;
+    const uint32_t lr = lane & 15u, lg = lane >> 4;
+    if (BF16) {
+        mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k0 = 0; k0 < K; k0 += 16) {
+            mlp_bf16x4 a, b;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t k = k0 + 4 * lg + q;
+                a[q] = mlp_to_bf16(A[(uint64_t)(m0 + lr) * am + (uint64_t)k * ak]);
+                b[q] = mlp_to_bf16(B[(uint64_t)k * bk + (uint64_t)(n0 + lr) * bn]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t mm = m0 + 4 * lg + q, nn = n0 + lr;
+            C[(uint64_t)mm * cm + nn] = acc[q] + (bias ? bias[nn] : 0.f);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t mm = m0 + 4 * lg + q, nn = n0 + lr;
+            float acc = 0.f;
+            for (uint32_t k = 0; k < K; ++k)
+                acc = __builtin_fmaf(A[(uint64_t)mm * am + (uint64_t)k * ak], B[(uint64_t)k * bk + (uint64_t)nn * bn], acc);
+            C[(uint64_t)mm * cm + nn] = acc + (bias ? bias[nn] : 0.f);
+        }
+    }
+}
+
+__global__ void k_relu(const float* __restrict__ pre, float* __restrict__ h, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) h[i] = fmaxf(pre[i], 0.f);
+}
+
+// Pair loss and dL/dz for B pairs stored as rows (2i, 2i+1) of z.
+__global__ void k_pair_loss(uint32_t B, uint32_t dout, const float* __restrict__ z, const float* __restrict__ t,
+                            const float* __restrict__ s, float* __restrict__ dz, float* __restrict__ loss) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const float* za = z + (uint64_t)(2 * i) * dout;
+    const float* zb = za + dout;
+    float p = 0.f;
+    for (uint32_t j = 0; j < dout; ++j) {
+        const float e = za[j] - zb[j];
+        p = __builtin_fmaf(e, e, p);
+    }
+    const float den = t[i] + s[i];
+    const float r = (p - t[i]) / den;
+    loss[i] = r * r;
+    const float g = 2.f * r / den / (float)B;  // dL/dp
+    float* da = dz + (uint64_t)(2 * i) * dout;
+    float* db = da + dout;
+    for (uint32_t j = 0; j < dout; ++j) {
+        const float e = 2.f * g * (za[j] - zb[j]);
+        da[j] = e;
+        db[j] = -e;
+    }
+}
+
+__global__ void k_relu_grad(const float* __restrict__ pre, float* __restrict__ dh, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && pre[i] <= 0.f) dh[i] = 0.f;
+}
+
+// Column sums: out[j] = sum_r in[r * cols + j]
+__global__ void k_colsum(const float* __restrict__ in, uint32_t rows, uint32_t cols, float* __restrict__ out) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    float acc = 0.f;
+    for (uint32_t r = 0; r < rows; ++r) acc += in[(uint64_t)r * cols + j];
+    out[j] = acc;
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       uint64_t n, float lr, float b1, float b2, float c1, float c2) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// Training pairs from the current lists: pair i has anchor a; even i pairs it
+// with one of its list neighbours, odd i with a neighbour of a neighbour (a
+// join candidate).  Rows are gathered scaled by alpha; t = alpha^2 |xa - xb|^2
+// in the library's FP32 order, s = alpha^2 kth(a).
+__global__ void k_mlp_pairs(uint32_t B, uint64_t n, uint32_t K, uint32_t ld, const float* __restrict__ X,
+                            const uint32_t* __restrict__ ids, const float* __restrict__ g_dist,
+                            const uint32_t* __restrict__ label, float alpha, uint64_t seed,
+                            float* __restrict__ xs, float* __restrict__ t, float* __restrict__ s) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const uint64_t r1 = nnm_rand(seed, i, 1), r2 = nnm_rand(seed, i, 2);
+    // anchors from rows that carry a built list (raw rows of a merge have none yet)
+    uint32_t a = (uint32_t)(nnm_rand(seed, i, 0) % n);
+    for (uint32_t tr = 3; tr < 19 && label[a] == NNM_RAW; ++tr) a = (uint32_t)(nnm_rand(seed, i, tr) % n);
+    uint32_t b = ids[(uint64_t)a * K + (uint32_t)(r1 % K)];
+    if (i & 1u) {
+        const uint32_t c = ids[(uint64_t)b * K + (uint32_t)(r2 % K)];
+        if (c != a) b = c;
+    }
+    const float* xa = X + (uint64_t)a * ld;
+    const float* xb = X + (uint64_t)b * ld;
+    float* oa = xs + (uint64_t)(2 * i) * ld;
+    float* ob = oa + ld;
+    for (uint32_t k = 0; k < ld; ++k) {
+        oa[k] = alpha * xa[k];
+        ob[k] = alpha * xb[k];
+    }
+    t[i] = alpha * alpha * nnm_l2_rows(xa, xb, ld);
+    s[i] = alpha * alpha * g_dist[(uint64_t)a * K + K - 1];
+}
+
+// Projection of every row: 16 rows per wave, hidden layer in LDS (bf16),
+// output rows in bf16 plus their squared norms (of the bf16 values).
+template <uint32_t HID, uint32_t DOUT>
+__global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X, uint64_t n, uint32_t ld,
+                                                     float alpha, const float* __restrict__ W1,
+                                                     const float* __restrict__ b1, const float* __restrict__ W2,
+                                                     const float* __restrict__ b2, uint16_t* __restrict__ P,
+                                                     float* __restrict__ pnorm) {
+    __shared__ short hsh[4][16][HID + 4];
+    const uint32_t wv = threadIdx.x / NNM_WAVE, lane = threadIdx.x % NNM_WAVE;
+    const uint32_t lr = lane & 15u, lg = lane >> 4;
+    const uint64_t nblk = (n + 15) / 16;
+    for (uint64_t blk = (uint64_t)blockIdx.x * 4 + wv; blk < nblk; blk += (uint64_t)gridDim.x * 4) {
+        const uint64_t r0 = blk * 16;
+        const uint64_t myrow = min(r0 + lr, n - 1);
+        // layer 1: H[16 x HID] = relu(a X W1^T + b1)
+        for (uint32_t n0 = 0; n0 < HID; n0 += 16) {
+            mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (uint32_t k0 = 0; k0 < ld; k0 += 16) {
+                mlp_bf16x4 a, b;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t k = k0 + 4 * lg + q;
+                    a[q] = mlp_to_bf16(alpha * X[myrow * ld + k]);
+                    b[q] = mlp_to_bf16(W1[(uint64_t)(n0 + lr) * ld + k]);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) hsh[wv][4 * lg + q][n0 + lr] = mlp_to_bf16(fmaxf(acc[q] + b1[n0 + lr], 0.f));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // layer 2: Z[16 x DOUT] = H W2^T + b2
+        float zq[DOUT / 16][4];
+#pragma unroll
+        for (uint32_t t = 0; t < DOUT / 16; ++t) {
+            mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (uint32_t k0 = 0; k0 < HID; k0 += 16) {
+                mlp_bf16x4 a, b;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t k = k0 + 4 * lg + q;
+                    a[q] = hsh[wv][lr][k];
+                    b[q] = mlp_to_bf16(W2[(uint64_t)(t * 16 + lr) * HID + k]);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) zq[t][q] = acc[q] + b2[t * 16 + lr];
+        }
+        // lane holds Z[4 lg + q][t*16 + lr]; write bf16 and accumulate norms
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t row = r0 + 4 * lg + q;
+            float nrm = 0.f;
+#pragma unroll
+            for (uint32_t t = 0; t < DOUT / 16; ++t) {
+                const short hb = mlp_to_bf16(zq[t][q]);
+                const float r = __uint_as_float((uint32_t)(uint16_t)hb << 16);
+                nrm = __builtin_fmaf(r, r, nrm);
+                if (row < n) P[row * DOUT + t * 16 + lr] = (uint16_t)hb;
+            }
+            // sum over the 16 lanes of this row group
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) nrm += __shfl_xor(nrm, o, NNM_WAVE);
+            if (lr == 0 && row < n) pnorm[row] = nrm;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+template <typename T>
+static int mlp_alloc(nnm_ctx* c, T** p, size_t n) {
+    return nnm_alloc(c, reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+void nnm_mlp_free(nnm_ctx* c) {
+    nnm_mlp_state* M = mlp_of(c);
+    if (!M) return;
+    void* ps[] = {M->W1, M->m, M->v, M->g, M->xs, M->pre, M->h, M->z, M->dz, M->dh, M->t, M->s, M->loss,
+                  M->pairs, M->P, M->pnorm};
+    for (void* p : ps) nnm_free(c, p);
+    delete M;
+    c->mlp = nullptr;
+}
+
+static int gemm(nnm_ctx* c, bool bf16, uint32_t M, uint32_t N, uint32_t K, const float* A, uint32_t am, uint32_t ak,
+                const float* B, uint32_t bk, uint32_t bn, const float* bias, float* C, uint32_t cm) {
+    const uint32_t tiles = (M / 16) * (N / 16);
+    const uint32_t blocks = (tiles + 3) / 4;
+    if (bf16)
+        hipLaunchKernelGGL(k_gemm<true>, dim3(blocks), dim3(256), 0, c->stream, M, N, K, A, am, ak, B, bk, bn, bias, C, cm);
+    else
+        hipLaunchKernelGGL(k_gemm<false>, dim3(blocks), dim3(256), 0, c->stream, M, N, K, A, am, ak, B, bk, bn, bias, C, cm);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+static int ensure_batch(nnm_ctx* c, uint32_t B) {
+    nnm_mlp_state* M = mlp_of(c);
+    const uint32_t rows = 2 * B;
+    if (M->cap_rows >= rows) return NNM_OK;
+    void* ps[] = {M->xs, M->pre, M->h, M->z, M->dz, M->dh, M->t, M->s, M->loss};
+    for (void* p : ps) nnm_free(c, p);
+    int r;
+    if ((r = mlp_alloc(c, &M->xs, (size_t)rows * M->din)) || (r = mlp_alloc(c, &M->pre, (size_t)rows * M->hid)) ||
+        (r = mlp_alloc(c, &M->h, (size_t)rows * M->hid)) || (r = mlp_alloc(c, &M->z, (size_t)rows * M->dout)) ||
+        (r = mlp_alloc(c, &M->dz, (size_t)rows * M->dout)) || (r = mlp_alloc(c, &M->dh, (size_t)rows * M->hid)) ||
+        (r = mlp_alloc(c, &M->t, B)) || (r = mlp_alloc(c, &M->s, B)) || (r = mlp_alloc(c, &M->loss, B)))
+        return r;
+    M->cap_rows = rows;
+    return NNM_OK;
+}
+
+// forward + loss + backward on the batch already in xs / t / s; gradients in M->g
+static int forward_backward(nnm_ctx* c, uint32_t B, bool bf16) {
+    nnm_mlp_state* M = mlp_of(c);
+    if (2 * B > M->cap_rows) return nnm_fail(c, NNM_ERR_STATE, "mlp batch larger than its buffers");
+    const uint32_t R = 2 * B, D = M->din, H = M->hid, O = M->dout;
+    float* gW1 = M->g;
+    float* gb1 = gW1 + (size_t)H * D;
+    float* gW2 = gb1 + H;
+    float* gb2 = gW2 + (size_t)O * H;
+    // pre = xs W1^T + b1 ; h = relu(pre) ; z = h W2^T + b2
+    NNM_TRY_MLP(gemm(c, bf16, R, H, D, M->xs, D, 1, M->W1, 1, D, M->b1, M->pre, H));
+    hipLaunchKernelGGL(k_relu, dim3((R * H + 255) / 256), dim3(256), 0, c->stream, M->pre, M->h, (uint64_t)R * H);
+    NNM_TRY_MLP(gemm(c, bf16, R, O, H, M->h, H, 1, M->W2, 1, H, M->b2, M->z, O));
+    hipLaunchKernelGGL(k_pair_loss, dim3((B + 255) / 256), dim3(256), 0, c->stream, B, O, M->z, M->t, M->s, M->dz,
+                       M->loss);
+    // gW2 = dz^T h ; gb2 = colsum dz ; dh = dz W2 (relu mask) ; gW1 = dh^T xs ; gb1 = colsum dh
+    NNM_TRY_MLP(gemm(c, bf16, O, H, R, M->dz, 1, O, M->h, H, 1, nullptr, gW2, H));
+    hipLaunchKernelGGL(k_colsum, dim3((O + 255) / 256), dim3(256), 0, c->stream, M->dz, R, O, gb2);
+    NNM_TRY_MLP(gemm(c, bf16, R, H, O, M->dz, O, 1, M->W2, H, 1, nullptr, M->dh, H));
+    hipLaunchKernelGGL(k_relu_grad, dim3((R * H + 255) / 256), dim3(256), 0, c->stream, M->pre, M->dh, (uint64_t)R * H);
+    NNM_TRY_MLP(gemm(c, bf16, H, D, R, M->dh, 1, H, M->xs, D, 1, nullptr, gW1, D));
+    hipLaunchKernelGGL(k_colsum, dim3((H + 255) / 256), dim3(256), 0, c->stream, M->dh, R, H, gb1);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+extern "C" int nnm_mlp_init(nnm_ctx* c, uint32_t hidden, uint32_t d_out, uint64_t seed) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    if (hidden == 0 || d_out == 0 || hidden % 16 || d_out % 16 || hidden > 1024 || d_out > 64)
+        return nnm_fail(c, NNM_ERR_INVALID, "hidden and d_out must be multiples of 16 (d_out <= 64)");
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    nnm_mlp_free(c);
+    nnm_mlp_state* M = new nnm_mlp_state();
+    c->mlp = M;
+    M->din = c->ld;
+    M->hid = hidden;
+    M->dout = d_out;
+    M->nparam = (uint64_t)hidden * M->din + hidden + (uint64_t)d_out * hidden + d_out;
+    int r;
+    if ((r = mlp_alloc(c, &M->W1, M->nparam)) || (r = mlp_alloc(c, &M->m, M->nparam)) ||
+        (r = mlp_alloc(c, &M->v, M->nparam)) || (r = mlp_alloc(c, &M->g, M->nparam)))
+        return r;
+    M->b1 = M->W1 + (size_t)hidden * M->din;
+    M->W2 = M->b1 + hidden;
+    M->b2 = M->W2 + (size_t)d_out * hidden;
+    // He-normal init from the counter-based stream (Box-Muller), zero biases
+    std::vector<float> w(M->nparam, 0.f);
+    auto gauss = [&](uint64_t i) {
+        const double u1 = ((nnm_rand(seed, i, 0) >> 11) + 1.0) / 9007199254740993.0;
+        const double u2 = (nnm_rand(seed, i, 1) >> 11) / 9007199254740992.0;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    const double s1 = std::sqrt(2.0 / c->d), s2 = std::sqrt(1.0 / hidden);
+    for (uint32_t o = 0; o < hidden; ++o)
+        for (uint32_t k = 0; k < c->d; ++k) w[(size_t)o * M->din + k] = (float)(s1 * gauss((size_t)o * M->din + k));
+    const size_t w2 = (size_t)hidden * M->din + hidden;
+    for (size_t i = 0; i < (size_t)d_out * hidden; ++i) w[w2 + i] = (float)(s2 * gauss(w2 + i));
+    NNM_HIP(c, hipMemcpy(M->W1, w.data(), M->nparam * 4, hipMemcpyHostToDevice));
+    NNM_HIP(c, hipMemset(M->m, 0, M->nparam * 4));
+    NNM_HIP(c, hipMemset(M->v, 0, M->nparam * 4));
+    // input scale: a^2 mean |x|^2 = d over a sample of rows
+    {
+        const uint64_t ns = std::min<uint64_t>(c->n, 4096);
+        std::vector<float> rows(ns * c->ld);
+        double acc = 0.0;
+        for (uint64_t i = 0; i < ns; ++i) {
+            const uint64_t r = (c->n * i) / ns;
+            NNM_HIP(c, hipMemcpy(rows.data() + i * c->ld, c->X + r * c->ld, c->ld * 4, hipMemcpyDeviceToHost));
+        }
+        for (float x : rows) acc += (double)x * x;
+        acc /= (double)ns;
+        M->alpha = acc > 0 ? (float)std::sqrt((double)c->d / acc) : 1.f;
+    }
+    M->step = 0;
+    M->trained = 0;
+    M->tau = 1.f;
+    return NNM_OK;
+}
+
+extern "C" int nnm_mlp_set_weights(nnm_ctx* c, const float* W1, const float* b1, const float* W2, const float* b2,
+                                   float alpha) {
+    if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
+    nnm_mlp_state* M = mlp_of(c);
+    NNM_HIP(c, hipMemcpy(M->W1, W1, (size_t)M->hid * M->din * 4, hipMemcpyHostToDevice));
+    NNM_HIP(c, hipMemcpy(M->b1, b1, (size_t)M->hid * 4, hipMemcpyHostToDevice));
+    NNM_HIP(c, hipMemcpy(M->W2, W2, (size_t)M->dout * M->hid * 4, hipMemcpyHostToDevice));
+    NNM_HIP(c, hipMemcpy(M->b2, b2, (size_t)M->dout * 4, hipMemcpyHostToDevice));
+    if (alpha > 0.f) M->alpha = alpha;
+    return NNM_OK;
+}
+
+extern "C" int nnm_mlp_get_weights(nnm_ctx* c, float* W1, float* b1, float* W2, float* b2, float* alpha,
+                                   float* tau) {
+    if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
+    nnm_mlp_state* M = mlp_of(c);
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    if (W1) NNM_HIP(c, hipMemcpy(W1, M->W1, (size_t)M->hid * M->din * 4, hipMemcpyDeviceToHost));
+    if (b1) NNM_HIP(c, hipMemcpy(b1, M->b1, (size_t)M->hid * 4, hipMemcpyDeviceToHost));
+    if (W2) NNM_HIP(c, hipMemcpy(W2, M->W2, (size_t)M->dout * M->hid * 4, hipMemcpyDeviceToHost));
+    if (b2) NNM_HIP(c, hipMemcpy(b2, M->b2, (size_t)M->dout * 4, hipMemcpyDeviceToHost));
+    if (alpha) *alpha = M->alpha;
+    if (tau) *tau = M->tau;
+    return NNM_OK;
+}
+
+extern "C" int nnm_mlp_loss_grad(nnm_ctx* c, const float* xa, const float* xb, const float* t, const float* s,
+                                 uint32_t B, int bf16, float* loss, float* gW1, float* gb1, float* gW2, float* gb2) {
+    if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
+    if (!xa || !xb || !t || !s || B == 0 || B % 8) return nnm_fail(c, NNM_ERR_INVALID, "B must be a multiple of 8");
+    nnm_mlp_state* M = mlp_of(c);
+    NNM_TRY_MLP(ensure_batch(c, B));
+    // rows 2i = xa[i], 2i+1 = xb[i] (already scaled by the caller); d_in columns, padded with zeros
+    std::vector<float> xs((size_t)2 * B * M->din, 0.f);
+    for (uint32_t i = 0; i < B; ++i)
+        for (uint32_t k = 0; k < c->d; ++k) {
+            xs[(size_t)(2 * i) * M->din + k] = xa[(size_t)i * c->d + k];
+            xs[(size_t)(2 * i + 1) * M->din + k] = xb[(size_t)i * c->d + k];
+        }
+    NNM_HIP(c, hipMemcpy(M->xs, xs.data(), xs.size() * 4, hipMemcpyHostToDevice));
+    NNM_HIP(c, hipMemcpy(M->t, t, B * 4, hipMemcpyHostToDevice));
+    NNM_HIP(c, hipMemcpy(M->s, s, B * 4, hipMemcpyHostToDevice));
+    NNM_TRY_MLP(forward_backward(c, B, bf16 != 0));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<float> lv(B);
+    NNM_HIP(c, hipMemcpy(lv.data(), M->loss, B * 4, hipMemcpyDeviceToHost));
+    double acc = 0;
+    for (float x : lv) acc += x;
+    if (loss) *loss = (float)(acc / B);
+    const float* g = M->g;
+    const size_t nW1 = (size_t)M->hid * M->din, nW2 = (size_t)M->dout * M->hid;
+    // report gW1 without the padded input columns
+    if (gW1) {
+        std::vector<float> tmp(nW1);
+        NNM_HIP(c, hipMemcpy(tmp.data(), g, nW1 * 4, hipMemcpyDeviceToHost));
+        for (uint32_t o = 0; o < M->hid; ++o)
+            std::memcpy(gW1 + (size_t)o * c->d, tmp.data() + (size_t)o * M->din, c->d * 4);
+    }
+    if (gb1) NNM_HIP(c, hipMemcpy(gb1, g + nW1, M->hid * 4, hipMemcpyDeviceToHost));
+    if (gW2) NNM_HIP(c, hipMemcpy(gW2, g + nW1 + M->hid, nW2 * 4, hipMemcpyDeviceToHost));
+    if (gb2) NNM_HIP(c, hipMemcpy(gb2, g + nW1 + M->hid + nW2, M->dout * 4, hipMemcpyDeviceToHost));
+    return NNM_OK;
+}
+
+// Prefilter margin: the tau-quantile rule.  On fresh pairs, find the smallest
+// tau such that a fraction `keep` of the pairs with t < s (true improvements
+// near the boundary) has p < tau * s... computed on the host from a sample.
+static int calibrate_tau(nnm_ctx* c, uint32_t B, float keep, uint64_t seed) {
+    nnm_mlp_state* M = mlp_of(c);
+    NNM_TRY_MLP(ensure_batch(c, B));
+    hipLaunchKernelGGL(k_mlp_pairs, dim3((B + 255) / 256), dim3(256), 0, c->stream, B, c->n, c->K, c->ld, c->X,
+                       c->g_ids, c->g_dist, c->label, M->alpha, seed, M->xs, M->t, M->s);
+    NNM_LAUNCH_CHECK(c);
+    NNM_TRY_MLP(forward_backward(c, B, true));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<float> z((size_t)2 * B * M->dout), t(B), s(B);
+    NNM_HIP(c, hipMemcpy(z.data(), M->z, z.size() * 4, hipMemcpyDeviceToHost));
+    NNM_HIP(c, hipMemcpy(t.data(), M->t, B * 4, hipMemcpyDeviceToHost));
+    NNM_HIP(c, hipMemcpy(s.data(), M->s, B * 4, hipMemcpyDeviceToHost));
+    std::vector<float> ratio;
+    for (uint32_t i = 0; i < B; ++i) {
+        if (!(t[i] < s[i]) || s[i] <= 0.f) continue;
+        double p = 0;
+        for (uint32_t j = 0; j < M->dout; ++j) {
+            const double e = (double)z[(size_t)(2 * i) * M->dout + j] - z[(size_t)(2 * i + 1) * M->dout + j];
+            p += e * e;
+        }
+        ratio.push_back((float)(p / s[i]));
+    }
+    if (ratio.size() < 16) { M->tau = 2.f; return NNM_OK; }
+    std::sort(ratio.begin(), ratio.end());
+    const size_t q = std::min(ratio.size() - 1, (size_t)(keep * ratio.size()));
+    M->tau = std::max(1.f, ratio[q]);
+    return NNM_OK;
+}
+
+extern "C" int nnm_mlp_train(nnm_ctx* c, uint32_t steps, uint32_t batch, float lr, float keep, uint64_t seed,
+                             float* loss_first, float* loss_last, double* secs) {
+    if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
+    if (!c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "training pairs come from a graph: none is set");
+    if (batch == 0 || batch % 8 || steps == 0) return nnm_fail(c, NNM_ERR_INVALID, "bad batch or steps");
+    nnm_mlp_state* M = mlp_of(c);
+    NNM_TRY_MLP(ensure_batch(c, batch));
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    float lf = 0.f, ll = 0.f;
+    std::vector<float> lv(batch);
+    for (uint32_t it = 0; it < steps; ++it) {
+        if (2 * batch > M->cap_rows) return nnm_fail(c, NNM_ERR_STATE, "mlp batch larger than its buffers");
+        hipLaunchKernelGGL(k_mlp_pairs, dim3((batch + 255) / 256), dim3(256), 0, c->stream, batch, c->n, c->K,
+                           c->ld, c->X, c->g_ids, c->g_dist, c->label, M->alpha, nnm_rand(seed, it, 77), M->xs,
+                           M->t, M->s);
+        NNM_LAUNCH_CHECK(c);
+        NNM_TRY_MLP(forward_backward(c, batch, true));
+        ++M->step;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)M->step), c2 = 1.f - std::pow(b2, (float)M->step);
+        hipLaunchKernelGGL(k_adam, dim3((M->nparam + 255) / 256), dim3(256), 0, c->stream, M->W1, M->g, M->m, M->v,
+                           M->nparam, lr, b1, b2, c1, c2);
+        NNM_LAUNCH_CHECK(c);
+        if (it == 0 || it + 1 == steps) {
+            NNM_HIP(c, hipMemcpyAsync(lv.data(), M->loss, batch * 4, hipMemcpyDeviceToHost, c->stream));
+            NNM_HIP(c, hipStreamSynchronize(c->stream));
+            double acc = 0;
+            for (float x : lv) acc += x;
+            (it == 0 ? lf : ll) = (float)(acc / batch);
+        }
+    }
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    NNM_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    NNM_TRY_MLP(calibrate_tau(c, 8192, keep, nnm_rand(seed, 0xCA11B, 0)));
+    M->trained = 1;
+    if (loss_first) *loss_first = lf;
+    if (loss_last) *loss_last = ll;
+    if (secs) *secs = ms * 1e-3;
+    return NNM_OK;
+}
+
+int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs) {
+    nnm_mlp_state* M = mlp_of(c);
+    if (!M) return nnm_fail(c, NNM_ERR_STATE, "learned join needs a model (nnm_mlp_init / nnm_mlp_train)");
+    if (M->din != c->ld) return nnm_fail(c, NNM_ERR_STATE, "model was built for another row length");
+    if (M->projected_n == c->n && M->P) return NNM_OK;
+    nnm_free(c, M->P);
+    nnm_free(c, M->pnorm);
+    M->P = nullptr;
+    M->pnorm = nullptr;
+    NNM_TRY_MLP(mlp_alloc(c, &M->P, (size_t)c->n * M->dout));
+    NNM_TRY_MLP(mlp_alloc(c, &M->pnorm, c->n));
+    NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((c->n + 63) / 64, (uint64_t)c->num_cus * 16);
+    if (M->hid == 256 && M->dout == 32)
+        hipLaunchKernelGGL((k_mlp_project<256, 32>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha,
+                           M->W1, M->b1, M->W2, M->b2, M->P, M->pnorm);
+    else if (M->hid == 128 && M->dout == 32)
+        hipLaunchKernelGGL((k_mlp_project<128, 32>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha,
+                           M->W1, M->b1, M->W2, M->b2, M->P, M->pnorm);
+    else if (M->hid == 256 && M->dout == 16)
+        hipLaunchKernelGGL((k_mlp_project<256, 16>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha,
+                           M->W1, M->b1, M->W2, M->b2, M->P, M->pnorm);
+    else
+        return nnm_fail(c, NNM_ERR_UNSUPPORTED, "projection kernel built for (hidden, d_out) in {(256,32),(128,32),(256,16)}");
+    NNM_LAUNCH_CHECK(c);
+    NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    NNM_HIP(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (secs) *secs = ms * 1e-3;
+    M->projected_n = c->n;
+    return NNM_OK;
+}
+
+void nnm_mlp_invalidate_projection(nnm_ctx* c) {
+    nnm_mlp_state* M = mlp_of(c);
+    if (M) M->projected_n = 0;
+}
+
+extern "C" int nnm_mlp_project(nnm_ctx* c, float* out, double* secs) {
+    if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
+    nnm_mlp_state* M = mlp_of(c);
+    M->projected_n = 0;
+    NNM_TRY_MLP(nnm_mlp_ensure_projection(c, secs));
+    if (out) {
+        std::vector<uint16_t> h((size_t)c->n * M->dout);
+        NNM_HIP(c, hipMemcpy(h.data(), M->P, h.size() * 2, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h.size(); ++i) {
+            uint32_t u = (uint32_t)h[i] << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            out[i] = f;
+        }
+    }
+    return NNM_OK;
+}
+
+// accessors for the join launcher
+const uint16_t* nnm_mlp_rows(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->P : nullptr; }
+const float* nnm_mlp_norms(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->pnorm : nullptr; }
+uint32_t nnm_mlp_dout(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->dout : 0; }
+float nnm_mlp_scale(nnm_ctx* c) { return mlp_of(c) ? 1.f / (mlp_of(c)->alpha * mlp_of(c)->alpha) : 1.f; }
+float nnm_mlp_tau(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->tau : 1.f; }
+int nnm_mlp_trained(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->trained : 0; }

@@ -25,14 +25,14 @@ RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
 class Params(C.Structure):
     _fields_ = [("K", C.c_uint32), ("rho", C.c_float), ("rev_cap", C.c_uint32), ("max_iters", C.c_uint32),
                 ("delta", C.c_float), ("rule", C.c_uint32), ("prec", C.c_uint32), ("rerank", C.c_uint32),
-                ("slots", C.c_uint32), ("cut", C.c_uint32), ("seed", C.c_uint64)]
+                ("slots", C.c_uint32), ("cut", C.c_uint32), ("plain_tail", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_uint32), ("converged", C.c_uint32), ("dists", C.c_uint64),
                 ("init_dists", C.c_uint64), ("updates", C.c_uint64), ("reranked", C.c_uint64),
                 ("list_entries", C.c_uint64), ("offers", C.c_uint64), ("last_delta", C.c_double), ("secs", C.c_double),
-                ("join_secs", C.c_double)]
+                ("join_secs", C.c_double), ("learned_iters", C.c_uint32), ("plain_iters", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -68,6 +68,14 @@ _SIGS = {
     "nnm_pair_dists": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp]),
     "nnm_graph_save": (C.c_int, [_vp]),
     "nnm_graph_restore": (C.c_int, [_vp, C.c_int, C.POINTER(Stats)]),
+    "nnm_mlp_init": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "nnm_mlp_set_weights": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_float]),
+    "nnm_mlp_get_weights": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "nnm_mlp_loss_grad": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_int, C.POINTER(C.c_float), _vp, _vp,
+                                    _vp, _vp]),
+    "nnm_mlp_train": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint64,
+                                C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+    "nnm_mlp_project": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
     "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
                                  C.POINTER(Stats), C.POINTER(C.c_double)]),
 }
@@ -222,6 +230,52 @@ class Context:
         self._chk(lib().nnm_bruteforce_knn(self.h, _ptr(Q), _ptr(r), nq, k, 1 if exclude_self else 0,
                                            _ptr(ids), _ptr(dist)))
         return ids, dist
+
+    # ---- learned projection
+    def mlp_init(self, hidden=256, d_out=32, seed=1):
+        self._chk(lib().nnm_mlp_init(self.h, hidden, d_out, seed))
+        self.mlp_shape = (hidden, d_out)
+
+    def mlp_weights(self):
+        hid, dout = self.mlp_shape
+        W1 = np.empty((hid, (self.d + 31) // 32 * 32), np.float32)
+        b1 = np.empty(hid, np.float32)
+        W2 = np.empty((dout, hid), np.float32)
+        b2 = np.empty(dout, np.float32)
+        al, tau = C.c_float(), C.c_float()
+        self._chk(lib().nnm_mlp_get_weights(self.h, _ptr(W1), _ptr(b1), _ptr(W2), _ptr(b2), C.byref(al),
+                                            C.byref(tau)))
+        return W1, b1, W2, b2, al.value, tau.value
+
+    def mlp_set_weights(self, W1, b1, W2, b2, alpha=0.0):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (W1, b1, W2, b2)]
+        self._chk(lib().nnm_mlp_set_weights(self.h, *[_ptr(a) for a in arrs], alpha))
+
+    def mlp_loss_grad(self, xa, xb, t, s, bf16=True):
+        hid, dout = self.mlp_shape
+        xa, xb, t, s = [np.ascontiguousarray(a, np.float32) for a in (xa, xb, t, s)]
+        B = t.size
+        gW1 = np.empty((hid, self.d), np.float32)
+        gb1 = np.empty(hid, np.float32)
+        gW2 = np.empty((dout, hid), np.float32)
+        gb2 = np.empty(dout, np.float32)
+        loss = C.c_float()
+        self._chk(lib().nnm_mlp_loss_grad(self.h, _ptr(xa), _ptr(xb), _ptr(t), _ptr(s), B, 1 if bf16 else 0,
+                                          C.byref(loss), _ptr(gW1), _ptr(gb1), _ptr(gW2), _ptr(gb2)))
+        return loss.value, gW1, gb1, gW2, gb2
+
+    def mlp_train(self, steps=1000, batch=4096, lr=1e-3, keep=0.98, seed=1):
+        lf, ll, secs = C.c_float(), C.c_float(), C.c_double()
+        self._chk(lib().nnm_mlp_train(self.h, steps, batch, lr, keep, seed, C.byref(lf), C.byref(ll),
+                                      C.byref(secs)))
+        return {"loss_first": lf.value, "loss_last": ll.value, "secs": secs.value}
+
+    def mlp_project(self, want=False):
+        hid, dout = self.mlp_shape
+        out = np.empty((self.n, dout), np.float32) if want else None
+        secs = C.c_double()
+        self._chk(lib().nnm_mlp_project(self.h, _ptr(out), C.byref(secs)))
+        return out, secs.value
 
     def pair_dists(self, a, b):
         a = np.ascontiguousarray(a, dtype=np.uint32)

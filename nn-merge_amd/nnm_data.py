@@ -7,6 +7,10 @@ Every generator is deterministic in its seed (numpy PCG64).  The workloads:
   [0, 255]^d; x = round(clip(c + N(0, sigma^2), 0, 255)).  Every value is an
   integer <= 255, so squared L2 is exact in float32 for d <= 256 and the
   rows are exactly representable in bfloat16.
+* ``sift_like(..., rank=r)``: the same mixture with SIFT-like low intrinsic
+  dimension: x = round(clip(c + A_c z + e, 0, 255)) with z ~ N(0, I_r), a
+  random d x r factor A_c per centre scaled so the total per-coordinate
+  spread stays sigma, and isotropic noise e ~ N(0, noise^2).
 * ``unit_mixture(n, d, centers, sigma)``: Gaussian mixture with centres drawn
   from N(0, I), x = c + N(0, sigma^2 I), then L2-normalised.
 * ``log_uniform_leaves(total, lo, hi)``: leaf sizes drawn log-uniformly in
@@ -18,16 +22,31 @@ import numpy as np
 SEED_BASE = 20261203
 
 
-def sift_like(n, d=128, centers=64, sigma=20.0, seed=SEED_BASE + 1, chunk=1 << 20):
+def sift_like(n, d=128, centers=64, sigma=20.0, seed=SEED_BASE + 1, chunk=1 << 20, rank=None, noise=2.0):
     rng = np.random.default_rng(seed)
     C = rng.uniform(0.0, 255.0, size=(centers, d)).astype(np.float32)
     lab = rng.integers(0, centers, size=n)
+    if rank is not None:
+        scale = np.sqrt(max(sigma * sigma - noise * noise, 0.0) / rank)
+        A = (rng.standard_normal(size=(centers, d, rank)) * scale).astype(np.float32)
     X = np.empty((n, d), np.float32)
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
-        blk = rng.standard_normal(size=(e - s, d), dtype=np.float32)
-        blk *= np.float32(sigma)
-        blk += C[lab[s:e]]
+        lb = lab[s:e]
+        if rank is None:
+            blk = rng.standard_normal(size=(e - s, d), dtype=np.float32)
+            blk *= np.float32(sigma)
+        else:
+            z = rng.standard_normal(size=(e - s, rank), dtype=np.float32)
+            blk = rng.standard_normal(size=(e - s, d), dtype=np.float32)
+            blk *= np.float32(noise)
+            order = np.argsort(lb, kind="stable")
+            bounds = np.searchsorted(lb[order], np.arange(centers + 1))
+            for k in range(centers):
+                idx = order[bounds[k]:bounds[k + 1]]
+                if idx.size:
+                    blk[idx] += z[idx] @ A[k].T
+        blk += C[lb]
         np.clip(blk, 0.0, 255.0, out=blk)
         np.rint(blk, out=blk)
         X[s:e] = blk

@@ -1,0 +1,129 @@
+"""Learned candidate pruning: GPU training kernels against the FP64 oracle,
+the projection kernel, and the ablation of the learned prefilter against
+the plain join (which it falls back to)."""
+import time
+
+import numpy as np
+import pytest
+
+import mlp
+import nnm
+import nnm_data
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with nnm.Context(0) as c:
+        yield c
+
+
+def _batch(X, B, rng):
+    a = rng.integers(0, X.shape[0], B)
+    b = rng.integers(0, X.shape[0], B)
+    xa, xb = X[a] / 128.0, X[b] / 128.0
+    t = ((xa.astype(np.float64) - xb) ** 2).sum(1)
+    s = rng.uniform(0.5, 1.5, B) * np.median(t)
+    return xa.astype(np.float32), xb.astype(np.float32), t.astype(np.float32), s.astype(np.float32)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.mark.parametrize("bf16,tol", [(False, 1e-4), (True, 4e-2)])
+def test_mlp_gradients_match_oracle(ctx, bf16, tol):
+    X = nnm_data.sift_like(2000, 128, centers=8, seed=71)
+    ctx.set_data(X)
+    ctx.mlp_init(256, 32, seed=3)
+    W1, b1, W2, b2, alpha, _ = ctx.mlp_weights()
+    rng = np.random.default_rng(0)
+    b1r = rng.standard_normal(256).astype(np.float32) * 0.05
+    b2r = rng.standard_normal(32).astype(np.float32) * 0.05
+    ctx.mlp_set_weights(W1, b1r, W2, b2r)
+    xa, xb, t, s = _batch(X, 64, rng)
+    L, gW1, gb1, gW2, gb2 = ctx.mlp_loss_grad(xa, xb, t, s, bf16=bf16)
+    oL, ogW1, ogb1, ogW2, ogb2 = mlp.loss_grad(W1[:, :128], b1r, W2, b2r, xa, xb, t, s)
+    assert abs(L - oL) <= tol * abs(oL)
+    for g, o in [(gW1, ogW1), (gb1, ogb1), (gW2, ogW2), (gb2, ogb2)]:
+        assert _rel(g, o) < tol
+
+
+def test_mlp_projection_matches_oracle(ctx):
+    X = nnm_data.sift_like(3000, 128, centers=8, seed=72)
+    ctx.set_data(X)
+    ctx.mlp_init(256, 32, seed=4)
+    W1, b1, W2, b2, alpha, _ = ctx.mlp_weights()
+    P, secs = ctx.mlp_project(want=True)
+    _, _, z = mlp.forward(W1[:, :128], b1, W2, b2, X.astype(np.float64) * alpha)
+    assert _rel(P, z) < 2e-2
+
+
+def test_mlp_training_reduces_loss(ctx):
+    X = nnm_data.sift_like(30000, 128, centers=64, seed=73)
+    ctx.set_data(X)
+    ctx.random_init(20, seed=1)
+    ctx.nndescent(nnm.default_params(K=20, seed=1, max_iters=8))
+    ctx.mlp_init(256, 32, seed=5)
+    st = ctx.mlp_train(steps=300, batch=2048, lr=2e-3, keep=0.98, seed=2)
+    print(st)
+    assert st["loss_last"] < 0.5 * st["loss_first"]
+    _, _, _, _, _, tau = ctx.mlp_weights()
+    assert 1.0 <= tau < 10.0
+
+
+def test_learned_join_ablation(ctx):
+    """J-Merge of 100k low-rank sift-like rows: plain BF16 join vs the learned
+    prefilter with a trained network vs with the untrained (random) network.
+    The learned join must keep recall within 0.01 of the plain join (its
+    fallback for the last iteration); the trained network must not do worse
+    than the untrained one, and must offer fewer candidates for exact re-rank."""
+    n, K = 100000, 20
+    half = n // 2
+    X = nnm_data.sift_like(n, 128, centers=256, seed=nnm_data.SEED_BASE + 4, rank=12)
+    rows = np.arange(0, n, 25, dtype=np.uint32)
+    ctx.set_data(X)
+    gt, _ = ctx.bruteforce(10, rows=rows)
+    ctx.set_data(X[:half])
+    ctx.random_init(K, seed=3)
+    ctx.nndescent(nnm.default_params(K=K, seed=3))
+    g0, _, _ = ctx.get_graph()
+    ctx.set_data(X)
+    ctx.set_partition([0, half, n], [1, 0], [0, 0])
+    init = np.vstack([g0, ((np.arange(half, n)[:, None] + 1 + np.arange(K)[None, :]) % n).astype(np.uint32)])
+    res = {}
+
+    def merge(prec):
+        ctx.set_graph(init)
+        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=0.5, max_iters=10, prec=prec, seed=5)
+        ctx.merge_init(p)
+        st = ctx.nndescent(p)
+        ctx.merge_finish(True)
+        ids, _, _ = ctx.get_graph()
+        return nnm.recall(ids, gt, rows), st
+
+    res["plain"] = merge(nnm.PREC_BF16)
+    # network trained on pairs of the built half (the plain NN-Descent's own lists)
+    ctx.set_graph(init)
+    ctx.mlp_init(256, 32, seed=7)
+    res["untrained_tau"] = None
+    ctx.mlp_train(steps=1, batch=1024, lr=0.0, keep=0.98, seed=8)  # calibrates tau only
+    res["untrained"] = merge(nnm.PREC_LEARNED)
+    ctx.set_graph(init)
+    ctx.mlp_init(256, 32, seed=7)
+    tst = ctx.mlp_train(steps=600, batch=4096, lr=2e-3, keep=0.98, seed=9)
+    res["trained"] = merge(nnm.PREC_LEARNED)
+    for k in ("plain", "untrained", "trained"):
+        r, st = res[k]
+        print(k, round(r, 4), "iters", st["iters"], "learned", st["learned_iters"], "exact", st["reranked"],
+              "pairs", st["dists"], "secs", round(st["secs"], 4))
+    print("train", tst)
+    r_plain, s_plain = res["plain"]
+    r_tr, s_tr = res["trained"]
+    r_un, s_un = res["untrained"]
+    assert s_tr["learned_iters"] > 0 and s_tr["plain_iters"] >= 1
+    assert r_tr >= r_plain - 0.01
+    assert r_tr >= r_un - 0.002
+    assert s_tr["reranked"] <= s_un["reranked"]

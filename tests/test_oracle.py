@@ -160,3 +160,44 @@ def test_recall_semantics():
     # reference counts the repeated id twice (src/evaluator.cpp:568-579)
     assert orc.recall(ids, rows, gt, 4, unique=False) == 3 / 4
     assert orc.recall(ids, rows, gt, 4, unique=True) == 2 / 4
+
+
+def test_mlp_oracle_gradients_match_finite_differences():
+    import mlp
+    rng = np.random.default_rng(9)
+    d, H, O, B = 12, 16, 4, 10
+    W1 = rng.standard_normal((H, d)) * 0.4
+    b1 = rng.standard_normal(H) * 0.1
+    W2 = rng.standard_normal((O, H)) * 0.3
+    b2 = rng.standard_normal(O) * 0.1
+    xa = rng.standard_normal((B, d))
+    xb = rng.standard_normal((B, d))
+    t = rng.uniform(0.5, 3.0, B)
+    s = rng.uniform(0.5, 2.0, B)
+    L, gW1, gb1, gW2, gb2 = mlp.loss_grad(W1, b1, W2, b2, xa, xb, t, s)
+    params = [W1, b1, W2, b2]
+    grads = [gW1, gb1, gW2, gb2]
+    h = 1e-6
+    for P, G in zip(params, grads):
+        flat = P.reshape(-1)
+        for idx in rng.choice(flat.size, min(12, flat.size), replace=False):
+            old = flat[idx]
+            flat[idx] = old + h
+            lp = mlp.loss(*params, xa, xb, t, s)
+            flat[idx] = old - h
+            lm = mlp.loss(*params, xa, xb, t, s)
+            flat[idx] = old
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - G.reshape(-1)[idx]) <= 1e-6 + 1e-5 * abs(fd)
+
+
+def test_mlp_oracle_loss_closed_form():
+    import mlp
+    # identity-like network on 2-d points: f(x) = x (W1 = I, W2 = I) on the
+    # positive quadrant, so p = |xa - xb|^2 and L = 0 when t = p
+    W1 = np.eye(2); b1 = np.zeros(2); W2 = np.eye(2); b2 = np.zeros(2)
+    xa = np.array([[1.0, 2.0], [3.0, 1.0]]); xb = np.array([[2.0, 4.0], [1.0, 1.0]])
+    t = ((xa - xb) ** 2).sum(1)
+    assert mlp.loss(W1, b1, W2, b2, xa, xb, t, np.ones(2)) == 0.0
+    L = mlp.loss(W1, b1, W2, b2, xa, xb, 2 * t, np.zeros(2))  # p = t/2 of target: r = -1/2
+    assert abs(L - 0.25) < 1e-15
