@@ -242,6 +242,20 @@ int nnm_mlp_project(nnm_ctx* ctx, float* out, double* secs);
 int nnm_host_build(int kind, const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t parts,
                    uint32_t* ids, float* dist, nnm_stats* st, double* build_secs);
 
+/* Hierarchical construction through the LyNNDescent host class.
+ * leaf_begin == NULL: the reference's doubling schedule,
+ * LyNNDescent::buildhgraph (src/lynndescent.cpp:735-831): NN-Descent over the
+ * first N0 = 64 rows, then blocks as large as the current graph J-Merged in
+ * (appdNwSmpl + nnDescent(verse = 0) + merge2one).  Otherwise nleaves
+ * contiguous leaves [leaf_begin[i], leaf_begin[i+1]) are built together and
+ * merged pairwise level by level, every pair of a level in one launch per
+ * iteration.  Per-level rows covered, merges run together, device seconds
+ * and distances go to the level_* arrays (up to max_levels entries). */
+int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t nleaves,
+                    const uint32_t* leaf_begin, uint32_t* ids, float* dist, nnm_stats* st, uint32_t* level_rows,
+                    uint32_t* level_groups, double* level_secs, uint64_t* level_dists, uint32_t max_levels,
+                    uint32_t* nlevels);
+
 #ifdef __cplusplus
 }
 #endif

@@ -291,8 +291,8 @@ extern "C" int nnm_set_partition(nnm_ctx* c, const uint32_t* sub_begin, const ui
     if (!c || !sub_begin || !sub_built || !sub_group || nsub == 0)
         return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: bad arguments");
     if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "nnm_set_partition: no data");
-    if (sub_begin[0] != 0 || sub_begin[nsub] != c->n)
-        return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: sub_begin must run from 0 to n");
+    if (sub_begin[0] != 0 || sub_begin[nsub] > c->n)
+        return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: sub_begin must run from 0 to at most n");
     for (uint32_t s = 0; s < nsub; ++s)
         if (sub_begin[s + 1] <= sub_begin[s])
             return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_partition: empty or unordered subgraph");
@@ -332,8 +332,9 @@ extern "C" int nnm_set_partition(nnm_ctx* c, const uint32_t* sub_begin, const ui
     NNM_HIP(c, hipMemcpyAsync(c->sub_built, built01.data(), nsub, hipMemcpyHostToDevice, c->stream));
     NNM_HIP(c, hipMemcpyAsync(c->grp_begin, gb.data(), gb.size() * 4, hipMemcpyHostToDevice, c->stream));
     NNM_HIP(c, hipMemcpyAsync(c->grp_end, ge.data(), ge.size() * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_fill_labels, dim3(grid_for(c->n, 256, 1u << 30)), dim3(256), 0, c->stream, c->label,
-                       c->vsub, c->sub_begin, c->sub_built, nsub, c->n);
+    c->n_act = sub_begin[nsub];
+    hipLaunchKernelGGL(k_fill_labels, dim3(grid_for(c->n_act, 256, 1u << 30)), dim3(256), 0, c->stream, c->label,
+                       c->vsub, c->sub_begin, c->sub_built, nsub, c->n_act);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     return NNM_OK;
@@ -611,8 +612,8 @@ extern "C" int nnm_graph_random_init(nnm_ctx* c, uint32_t K, uint64_t seed, nnm_
     NNM_TRY(ensure_graph(c, K));
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_random_init, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld,
-                       c->n, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
+    hipLaunchKernelGGL(k_random_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld,
+                       c->n_act, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
                        c->g_flag, c->counters);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -697,8 +698,8 @@ extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
     NNM_HIP(c, hipMemcpyAsync(c->g_flag, c->s_flag, nk, hipMemcpyDeviceToDevice, c->stream));
     if (recompute) {
         uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
-        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld,
-                           c->n, c->K, 1, 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld,
+                           c->n_act, c->K, 1, 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
         NNM_LAUNCH_CHECK(c);
     }
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -765,7 +766,7 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
     }
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
+    hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld, c->n_act,
                        K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin, c->grp_end,
                        p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
                        c->c_flag, c->has_cpn, c->counters);
@@ -786,7 +787,7 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
 extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
     if (!c) return NNM_ERR_INVALID;
     if (!c->cpn_live) return nnm_fail(c, NNM_ERR_STATE, "nnm_merge_finish: no merge in progress");
-    hipLaunchKernelGGL(k_merge_finish, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->n, c->K,
+    hipLaunchKernelGGL(k_merge_finish, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
                        c->cut, dedup, c->has_cpn, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist, c->c_flag);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipStreamSynchronize(c->stream));
@@ -1132,7 +1133,7 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
 
 // One iteration; leaves the flag count in counters[CNT_FLAGS].
 static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double* join_ms) {
-    const uint64_t n = c->n;
+    const uint64_t n = c->n_act;
     const uint32_t K = c->K;
     uint32_t S = (uint32_t)std::lround((double)p->rho * K);
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
@@ -1200,14 +1201,16 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     uint32_t it = 0, n_learned = 0, n_plain = 0;
     int converged = 0;
     double last_delta = 1.0;
+    // Learned runs end on the plain join, which catches what the prefilter
+    // dropped: the last plain_tail iterations of a capped run, and every
+    // iteration after the learned phase meets the delta test.
+    bool tail = !learned;
     for (;;) {
         const uint32_t iter = single ? only_iter : it;
-        // the last plain_tail iterations of a capped learned run use the
-        // plain join, which catches what the prefilter dropped
         nnm_params q = *p;
         if (p->prec == NNM_PREC_LEARNED) {
-            const bool tail = p->max_iters && iter + p->plain_tail >= p->max_iters;
-            q.prec = (learned && !tail) ? NNM_PREC_LEARNED : NNM_PREC_BF16;
+            if (p->max_iters && iter + p->plain_tail >= p->max_iters) tail = true;
+            q.prec = tail ? NNM_PREC_BF16 : NNM_PREC_LEARNED;
         }
         if (q.prec == NNM_PREC_LEARNED) ++n_learned; else ++n_plain;
         NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
@@ -1217,8 +1220,15 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         uint64_t h[CNT_N];
         NNM_TRY(read_counters(c, h));
         ++it;
-        last_delta = (double)h[CNT_FLAGS] / ((double)c->n * c->K);
-        if (last_delta < p->delta) { converged = 1; break; }
+        last_delta = (double)h[CNT_FLAGS] / ((double)c->n_act * c->K);
+        if (last_delta < p->delta) {
+            if (q.prec == NNM_PREC_LEARNED && p->plain_tail > 0) {
+                tail = true;
+            } else {
+                converged = 1;
+                break;
+            }
+        }
         if (single) break;
         if (p->max_iters && it >= p->max_iters) break;
     }

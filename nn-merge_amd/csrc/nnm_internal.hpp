@@ -51,7 +51,8 @@ struct nnm_ctx {
     int num_cus = 256;
 
     // data
-    uint64_t n = 0;
+    uint64_t n = 0;       // rows held
+    uint64_t n_act = 0;   // rows covered by the partition (a prefix of the rows)
     uint32_t d = 0, ld = 0;
     float* X = nullptr;
     uint16_t* Xb = nullptr;   // bf16 copy (lazy)

@@ -815,7 +815,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
     JoinArgs a;
     a.X = c->X;
     a.ld = c->ld;
-    a.n = c->n;
+    a.n = c->n_act;
     a.K = c->K;
     a.C = c->slots_C;
     a.rule = p->rule;

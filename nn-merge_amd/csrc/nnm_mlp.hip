@@ -449,7 +449,7 @@ extern "C" int nnm_mlp_loss_grad(nnm_ctx* c, const float* xa, const float* xb, c
 static int calibrate_tau(nnm_ctx* c, uint32_t B, float keep, uint64_t seed) {
     nnm_mlp_state* M = mlp_of(c);
     NNM_TRY_MLP(ensure_batch(c, B));
-    hipLaunchKernelGGL(k_mlp_pairs, dim3((B + 255) / 256), dim3(256), 0, c->stream, B, c->n, c->K, c->ld, c->X,
+    hipLaunchKernelGGL(k_mlp_pairs, dim3((B + 255) / 256), dim3(256), 0, c->stream, B, c->n_act, c->K, c->ld, c->X,
                        c->g_ids, c->g_dist, c->label, M->alpha, seed, M->xs, M->t, M->s);
     NNM_LAUNCH_CHECK(c);
     NNM_TRY_MLP(forward_backward(c, B, true));
@@ -487,7 +487,7 @@ extern "C" int nnm_mlp_train(nnm_ctx* c, uint32_t steps, uint32_t batch, float l
     std::vector<float> lv(batch);
     for (uint32_t it = 0; it < steps; ++it) {
         if (2 * batch > M->cap_rows) return nnm_fail(c, NNM_ERR_STATE, "mlp batch larger than its buffers");
-        hipLaunchKernelGGL(k_mlp_pairs, dim3((batch + 255) / 256), dim3(256), 0, c->stream, batch, c->n, c->K,
+        hipLaunchKernelGGL(k_mlp_pairs, dim3((batch + 255) / 256), dim3(256), 0, c->stream, batch, c->n_act, c->K,
                            c->ld, c->X, c->g_ids, c->g_dist, c->label, M->alpha, nnm_rand(seed, it, 77), M->xs,
                            M->t, M->s);
         NNM_LAUNCH_CHECK(c);

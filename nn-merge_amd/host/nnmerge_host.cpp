@@ -174,7 +174,147 @@ unsigned* PMerge::buildKGraph(float* mat, const unsigned K, const unsigned N, co
     return exportIds();
 }
 
+void LyNNDescent::runLevel(const std::vector<uint32_t>& begin, const std::vector<uint8_t>& built,
+                           const std::vector<uint32_t>& group, unsigned cut) {
+    check(nnm_set_partition(ctx_, begin.data(), built.data(), group.data(), (uint32_t)built.size()));
+    nnm_params mp = params;
+    mp.rule = NNM_RULE_MERGE;
+    mp.cut = cut;
+    nnm_stats i{}, st{};
+    check(nnm_merge_init(ctx_, &mp, &i));
+    check(nnm_nndescent(ctx_, &mp, &st));
+    check(nnm_merge_finish(ctx_, 1));
+    LevelStat ls{begin.back(), group.empty() ? 0u : group.back() + 1, st.iters, i.init_dists + st.dists,
+                 i.secs + st.secs};
+    levels.push_back(ls);
+    nCmps += ls.dists;
+    mergeSecs += ls.secs;
+    lastStats = st;
+}
+
+unsigned LyNNDescent::buildhgraph(float* mat, unsigned N, unsigned D, unsigned topk) {
+    K0 = topk;
+    row = N;
+    col = D;
+    params.K = topk;
+    levels.clear();
+    nCmps = 0;
+    mergeSecs = 0.0;
+    if (N < N0 + 1) throw Error(NNM_ERR_INVALID, "buildhgraph needs more than N0 rows");
+    check(nnm_set_data(ctx_, mat, N, D));
+    // seed graph: plain NN-Descent over the first N0 rows (init + nnDescent(verse = 1))
+    {
+        uint32_t b[2] = {0u, N0};
+        uint8_t bu[1] = {1};
+        uint32_t g[1] = {0};
+        check(nnm_set_partition(ctx_, b, bu, g, 1));
+        nnm_stats s0{}, s1{};
+        check(nnm_graph_random_init(ctx_, topk, params.seed, &s0));
+        nnm_params bp = params;
+        bp.rule = NNM_RULE_ALL;
+        check(nnm_nndescent(ctx_, &bp, &s1));
+        levels.push_back(LevelStat{N0, 1, s1.iters, s0.init_dists + s1.dists, s0.secs + s1.secs});
+        nCmps += s0.init_dists + s1.dists;
+        mergeSecs += s0.secs + s1.secs;
+    }
+    // appdNwSmpl(K/2, smplNum) + nnDescent(0) + merge2one, smplNum doubling
+    uint32_t cur = N0, smpl = N0;
+    while (cur < N) {
+        const uint32_t b = (N - cur > smpl) ? smpl : (N - cur);
+        runLevel({0u, cur, cur + b}, {1, 0}, {0, 0}, topk - topk / 2);
+        cur += b;
+        smpl = cur;
+    }
+    fetch();
+    return 0;
+}
+
+unsigned LyNNDescent::buildLeafHierarchy(float* mat, unsigned N, unsigned D, unsigned K,
+                                         const std::vector<uint32_t>& leaf_begin) {
+    K0 = K;
+    row = N;
+    col = D;
+    params.K = K;
+    levels.clear();
+    nCmps = 0;
+    mergeSecs = 0.0;
+    if (leaf_begin.size() < 2 || leaf_begin.front() != 0 || leaf_begin.back() != N)
+        throw Error(NNM_ERR_INVALID, "leaf_begin must run from 0 to N");
+    check(nnm_set_data(ctx_, mat, N, D));
+    std::vector<uint32_t> begin = leaf_begin;
+    const uint32_t L = (uint32_t)begin.size() - 1;
+    // every leaf built by NN-Descent in the same launches, each its own group
+    {
+        std::vector<uint8_t> bu(L, 1);
+        std::vector<uint32_t> g(L);
+        for (uint32_t s = 0; s < L; ++s) g[s] = s;
+        check(nnm_set_partition(ctx_, begin.data(), bu.data(), g.data(), L));
+        nnm_stats s0{}, s1{};
+        check(nnm_graph_random_init(ctx_, K, params.seed ^ 0x1EAFull, &s0));
+        nnm_params bp = params;
+        bp.rule = NNM_RULE_ALL;
+        check(nnm_nndescent(ctx_, &bp, &s1));
+        buildSecs = s0.secs + s1.secs;
+        levels.push_back(LevelStat{N, L, s1.iters, s0.init_dists + s1.dists, s0.secs + s1.secs});
+    }
+    // levels: adjacent pairs merged together (an odd last subgraph waits)
+    while (begin.size() > 2) {
+        const uint32_t S = (uint32_t)begin.size() - 1;
+        std::vector<uint8_t> bu(S, 1);
+        std::vector<uint32_t> g(S);
+        for (uint32_t s = 0; s < S; ++s) g[s] = s / 2;
+        runLevel(begin, bu, g, K / 2);
+        std::vector<uint32_t> nb;
+        for (uint32_t s = 0; s < S; s += 2) nb.push_back(begin[s]);
+        nb.push_back(N);
+        begin.swap(nb);
+    }
+    fetch();
+    return 0;
+}
+
+unsigned* LyNNDescent::buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) {
+    buildhgraph(mat, N, D, K);
+    return exportIds();
+}
+
 }  // namespace nnmerge
+
+extern "C" int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t nleaves,
+                               const uint32_t* leaf_begin, uint32_t* ids, float* dist, nnm_stats* st,
+                               uint32_t* level_rows, uint32_t* level_groups, double* level_secs,
+                               uint64_t* level_dists, uint32_t max_levels, uint32_t* nlevels) {
+    if (!X || !p || !ids || n == 0 || n > 0xFFFFFFF0ull) return NNM_ERR_INVALID;
+    try {
+        nnmerge::LyNNDescent g;
+        g.params = *p;
+        if (nleaves && leaf_begin)
+            g.buildLeafHierarchy(const_cast<float*>(X), (unsigned)n, d, p->K,
+                                 std::vector<uint32_t>(leaf_begin, leaf_begin + nleaves + 1));
+        else
+            g.buildhgraph(const_cast<float*>(X), (unsigned)n, d, p->K);
+        std::memcpy(ids, g.ids().data(), n * p->K * sizeof(uint32_t));
+        if (dist) std::memcpy(dist, g.dists().data(), n * p->K * sizeof(float));
+        if (st) {
+            *st = g.lastStats;
+            st->dists = g.nCmps;
+            st->secs = g.mergeSecs;
+        }
+        const uint32_t L = (uint32_t)g.levels.size();
+        if (nlevels) *nlevels = L;
+        for (uint32_t i = 0; i < L && i < max_levels; ++i) {
+            if (level_rows) level_rows[i] = g.levels[i].rows;
+            if (level_groups) level_groups[i] = g.levels[i].groups;
+            if (level_secs) level_secs[i] = g.levels[i].secs;
+            if (level_dists) level_dists[i] = g.levels[i].dists;
+        }
+        return NNM_OK;
+    } catch (const nnmerge::Error& e) {
+        return e.code;
+    } catch (...) {
+        return NNM_ERR_HIP;
+    }
+}
 
 extern "C" int nnm_host_build(int kind, const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t parts,
                               uint32_t* ids, float* dist, nnm_stats* st, double* build_secs) {

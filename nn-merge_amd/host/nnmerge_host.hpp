@@ -139,4 +139,34 @@ private:
     unsigned parts_;
 };
 
+// LyNNDescent (src/lynndescent.h:19-86): hierarchical construction.
+//   buildhgraph: the reference's doubling schedule (src/lynndescent.cpp:735-831):
+//     NN-Descent over the first N0 = 64 rows, then repeatedly append a block
+//     as large as the current graph (the rest at the end), J-Merge it in
+//     (appdNwSmpl + nnDescent(verse = 0) + merge2one) until every row is in.
+//   buildLeafHierarchy: uneven leaves built together in one batched launch,
+//     then levels that merge adjacent pairs of subgraphs, every pair of a
+//     level in the same launches (S-Merge rule: both sides built).
+struct LevelStat {
+    uint32_t rows;       // rows covered after the level
+    uint32_t groups;     // merges run together
+    uint32_t iters;
+    uint64_t dists;
+    double secs;         // device time of the level's merge
+};
+
+class LyNNDescent : public AbstractGraph {
+public:
+    static const unsigned N0 = 64;  // src/lynndescent.cpp:23
+    using AbstractGraph::AbstractGraph;
+    unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) override;
+    unsigned buildhgraph(float* mat, unsigned N, unsigned D, unsigned topk);
+    unsigned buildLeafHierarchy(float* mat, unsigned N, unsigned D, unsigned K, const std::vector<uint32_t>& leaf_begin);
+    std::vector<LevelStat> levels;
+
+private:
+    void runLevel(const std::vector<uint32_t>& begin, const std::vector<uint8_t>& built,
+                  const std::vector<uint32_t>& group, unsigned cut);
+};
+
 }  // namespace nnmerge

@@ -76,6 +76,8 @@ _SIGS = {
     "nnm_mlp_train": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint64,
                                 C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_double)]),
     "nnm_mlp_project": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
+    "nnm_host_hmerge": (C.c_int, [_vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp, _vp,
+                                  C.POINTER(Stats), _vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_uint32)]),
     "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
                                  C.POINTER(Stats), C.POINTER(C.c_double)]),
 }
@@ -301,6 +303,30 @@ def host_build(kind, X, p, parts=0):
     if rc != NNM_OK:
         raise NNMError(rc, "nnm_host_build failed")
     return ids, dist, st.as_dict(), bs.value
+
+
+def host_hmerge(X, p, leaf_begin=None, max_levels=64):
+    """LyNNDescent: doubling schedule (leaf_begin None) or leaf hierarchy."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    ids = np.empty((n, p.K), np.uint32)
+    dist = np.empty((n, p.K), np.float32)
+    st = Stats()
+    lb = None if leaf_begin is None else np.ascontiguousarray(leaf_begin, np.uint32)
+    nl = 0 if lb is None else lb.size - 1
+    rows = np.zeros(max_levels, np.uint32)
+    groups = np.zeros(max_levels, np.uint32)
+    secs = np.zeros(max_levels, np.float64)
+    dists = np.zeros(max_levels, np.uint64)
+    L = C.c_uint32()
+    rc = lib().nnm_host_hmerge(_ptr(X), n, d, C.byref(p), nl, _ptr(lb), _ptr(ids), _ptr(dist), C.byref(st),
+                               _ptr(rows), _ptr(groups), _ptr(secs), _ptr(dists), max_levels, C.byref(L))
+    if rc != NNM_OK:
+        raise NNMError(rc, "nnm_host_hmerge failed")
+    m = min(L.value, max_levels)
+    levels = [{"rows": int(rows[i]), "groups": int(groups[i]), "secs": float(secs[i]), "dists": int(dists[i])}
+              for i in range(m)]
+    return ids, dist, st.as_dict(), levels
 
 
 def recall(ids, gt, rows, k=10, unique=True):
