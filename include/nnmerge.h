@@ -56,6 +56,12 @@ extern "C" {
                             Jmerge::localJoin (src/jmerge.cpp:189-191),
                             LyNNDescent::localJoin (src/lynndescent.cpp:330) */
 
+/* nnm_params.flags */
+#define NNM_FLAG_GRAPH 1u    /* replay each iteration from a captured HIP graph */
+#define NNM_FLAG_AUTOTUNE 2u /* time the join launch shapes on the first
+                                iteration and keep the fastest (results do not
+                                depend on the shape) */
+
 /* Re-rank policy for approximate precisions. */
 #define NNM_RERANK_AUTO 0   /* skip when the approximate form is exact */
 #define NNM_RERANK_ALWAYS 1
@@ -82,6 +88,7 @@ typedef struct nnm_params {
                            samples; 0 = K/2 (cutSize, src/smerge.cpp:99) */
     uint32_t plain_tail;/* NNM_PREC_LEARNED with max_iters: the last
                            plain_tail iterations use the plain BF16 join */
+    uint32_t flags;     /* NNM_FLAG_* */
     uint64_t seed;      /* every random choice derives from this seed */
 } nnm_params;
 
@@ -101,6 +108,10 @@ typedef struct nnm_stats {
     double join_secs;      /* device time spent in the join kernels */
     uint32_t learned_iters;/* iterations joined through the learned prefilter */
     uint32_t plain_iters;  /* iterations joined with the plain join */
+    uint32_t tile_js;      /* join window rows used (autotuned or default) */
+    uint32_t tile_chunk;   /* rows per work-queue grab used */
+    uint32_t graph_iters;  /* iterations replayed from a HIP graph */
+    uint32_t pad_;
 } nnm_stats;
 
 void nnm_default_params(nnm_params* p);

@@ -120,6 +120,7 @@ extern "C" void nnm_default_params(nnm_params* p) {
     p->slots = 64;
     p->cut = 0;
     p->plain_tail = 1;
+    p->flags = 0;
     p->seed = 1;
 }
 
@@ -903,11 +904,13 @@ __device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ src,
 }
 
 __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ roff,
-                            const uint32_t* __restrict__ rbuf, uint32_t cap, uint32_t salt,
+                            const uint32_t* __restrict__ rbuf, uint32_t cap, uint64_t seed,
                             uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
                             uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
-                            uint64_t* __restrict__ counters) {
+                            const uint64_t* __restrict__ counters) {
     const uint32_t lane = nnm_lane();
+    // sampling salt of this iteration (the iteration number lives on the device)
+    const uint32_t salt = (uint32_t)(nnm_rand(seed, 0xC0FFEEull, counters[CNT_ITER]) >> 32);
     NNM_WAVE_LOOP(v, n) {
         const uint32_t fn = n_new[v], fo = n_old[v];
         const uint32_t an = wave_append_reverse(rbuf + roff[v], rcnt[v], cap, nnm_hash32((uint32_t)v, salt),
@@ -1128,17 +1131,16 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
+    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE)) return nnm_fail(c, NNM_ERR_INVALID, "unknown flags");
     return NNM_OK;
 }
 
-// One iteration; leaves the flag count in counters[CNT_FLAGS].
-static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double* join_ms) {
+// Candidate lists of one iteration: forward samples, reverse lists, caps.
+static int build_lists(nnm_ctx* c, const nnm_params* p) {
     const uint64_t n = c->n_act;
     const uint32_t K = c->K;
     uint32_t S = (uint32_t)std::lround((double)p->rho * K);
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
-    const uint64_t iseed = nnm_rand(p->seed, 0xC0FFEEull, iter);
-    const uint32_t salt = (uint32_t)iseed, salt2 = (uint32_t)(iseed >> 32);
     NNM_TRY(fill_u32(c, c->rcnt, 2 * n, 0));
     NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
     hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->kth);
@@ -1159,12 +1161,14 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
                        c->g_dist, c->kth, c->g_flag, c->newmask, c->roff, c->rcur, c->rbuf);
     NNM_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
-                       c->rbuf, c->rev_cap, salt2, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
+                       c->rbuf, c->rev_cap, p->seed, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
                        c->counters);
     NNM_LAUNCH_CHECK(c);
-    NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
-    NNM_TRY(nnm_launch_join(c, p, iter, salt));
-    NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
+    return NNM_OK;
+}
+
+// Merge the slots into the lists (with the re-rank the precision asks for).
+static int update_lists(nnm_ctx* c, const nnm_params* p) {
     int rerank = 0;
     float tau = 1.f;
     if (p->prec == NNM_PREC_BF16) {
@@ -1174,14 +1178,104 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t iter, double*
         tau = nnm_mlp_tau(c);
     }
     if (p->rerank == NNM_RERANK_NEVER) rerank = 0;
-    hipLaunchKernelGGL(k_update, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, c->slots_C, c->X,
-                       c->ld, rerank, tau, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
+    hipLaunchKernelGGL(k_update, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
+                       c->slots_C, c->X, c->ld, rerank, tau, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
     NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+static int reset_queues(nnm_ctx* c) {
+    NNM_TRY(fill_u64(c, c->counters + CNT_WORK, 1, 0));
+    NNM_TRY(fill_u64(c, c->counters + CNT_WORK2, 2, 0));
+    return NNM_OK;
+}
+
+// One iteration; leaves the flag count in counters[CNT_FLAGS].  The
+// iteration number is read from counters[CNT_ITER], which the caller sets.
+// join_ms == nullptr: no events or waits, so the sequence can be captured.
+static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, double* join_ms) {
+    NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
+    NNM_TRY(reset_queues(c));
+    NNM_TRY(build_lists(c, p));
+    if (join_ms) NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
+    NNM_TRY(nnm_launch_join(c, p, js, chunk));
+    if (join_ms) NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
+    NNM_TRY(update_lists(c, p));
     if (join_ms) {
         NNM_HIP(c, hipEventSynchronize(c->evj1));
         float ms = 0.f;
         hipEventElapsedTime(&ms, c->evj0, c->evj1);
         *join_ms += ms;
+    }
+    return NNM_OK;
+}
+
+// One iteration that times every join launch shape on the lists it builds
+// and keeps the fastest for this precision.  The join only lowers slot keys
+// (atomicMin), so repeated launches leave the slots as one launch does; the
+// work counters are reset before and the tallies restored after each trial.
+static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) {
+    NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
+    NNM_TRY(reset_queues(c));
+    NNM_TRY(build_lists(c, p));
+    uint64_t h[CNT_N];
+    NNM_TRY(read_counters(c, h));
+    std::vector<std::pair<uint32_t, uint32_t>> space;
+    nnm_join_tune_space(c, p->prec, &space);
+    uint32_t best_js = 0, best_ch = 0;
+    float best = 3.4e38f;
+    for (int rep = 0; rep < 2; ++rep) {
+        for (const auto& cand : space) {
+            NNM_TRY(reset_queues(c));
+            NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
+            NNM_TRY(nnm_launch_join(c, p, cand.first, cand.second));
+            NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
+            NNM_HIP(c, hipEventSynchronize(c->evj1));
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c->evj0, c->evj1);
+            if (rep == 1 && ms < best) { best = ms; best_js = cand.first; best_ch = cand.second; }
+        }
+    }
+    c->tune_js[p->prec] = best_js;
+    c->tune_chunk[p->prec] = best_ch;
+    c->tune_ld[p->prec] = c->ld;
+    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS}) NNM_TRY(fill_u64(c, c->counters + i, 1, h[i]));
+    NNM_TRY(reset_queues(c));
+    NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
+    NNM_TRY(nnm_launch_join(c, p, best_js, best_ch));
+    NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
+    NNM_TRY(update_lists(c, p));
+    NNM_HIP(c, hipEventSynchronize(c->evj1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->evj0, c->evj1);
+    *join_ms += ms;
+    return NNM_OK;
+}
+
+// Instantiated iteration graphs of one descent call (one per join precision).
+struct IterGraphs {
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+    ~IterGraphs() {
+        for (auto& e : exec)
+            if (e) hipGraphExecDestroy(e);
+    }
+};
+
+static int capture_iteration(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, hipGraphExec_t* out) {
+    NNM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int rc = run_iteration(c, p, js, chunk, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != NNM_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return nnm_hip_fail(c, e, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        *out = nullptr;
+        return nnm_hip_fail(c, ei, "hipGraphInstantiate");
     }
     return NNM_OK;
 }
@@ -1198,9 +1292,13 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
     double join_ms = 0.0;
-    uint32_t it = 0, n_learned = 0, n_plain = 0;
+    uint32_t it = 0, n_learned = 0, n_plain = 0, n_graph = 0;
+    uint32_t used_js = 0, used_chunk = 0;
     int converged = 0;
     double last_delta = 1.0;
+    const bool autotune = (p->flags & NNM_FLAG_AUTOTUNE) != 0;
+    const bool graph = (p->flags & NNM_FLAG_GRAPH) != 0;
+    IterGraphs graphs;
     // Learned runs end on the plain join, which catches what the prefilter
     // dropped: the last plain_tail iterations of a capped run, and every
     // iteration after the learned phase meets the delta test.
@@ -1213,10 +1311,22 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
             q.prec = tail ? NNM_PREC_BF16 : NNM_PREC_LEARNED;
         }
         if (q.prec == NNM_PREC_LEARNED) ++n_learned; else ++n_plain;
-        NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
-        NNM_TRY(fill_u64(c, c->counters + CNT_WORK, 1, 0));
-        NNM_TRY(fill_u64(c, c->counters + CNT_WORK2, 2, 0));
-        NNM_TRY(run_iteration(c, &q, iter, &join_ms));
+        NNM_TRY(fill_u64(c, c->counters + CNT_ITER, 1, iter));
+        const bool tuned = c->tune_ld[q.prec] == c->ld;
+        const uint32_t js = (autotune && tuned) ? c->tune_js[q.prec] : 0;
+        const uint32_t chunk = (autotune && tuned) ? c->tune_chunk[q.prec] : 0;
+        if (autotune && !tuned) {
+            NNM_TRY(autotune_iteration(c, &q, &join_ms));
+        } else if (graph) {
+            hipGraphExec_t& ge = graphs.exec[q.prec];
+            if (!ge) NNM_TRY(capture_iteration(c, &q, js, chunk, &ge));
+            NNM_HIP(c, hipGraphLaunch(ge, c->stream));
+            ++n_graph;
+        } else {
+            NNM_TRY(run_iteration(c, &q, js, chunk, &join_ms));
+        }
+        used_js = autotune ? c->tune_js[q.prec] : js;
+        used_chunk = autotune ? c->tune_chunk[q.prec] : chunk;
         uint64_t h[CNT_N];
         NNM_TRY(read_counters(c, h));
         ++it;
@@ -1246,6 +1356,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->last_delta = last_delta;
         st->learned_iters = n_learned;
         st->plain_iters = n_plain;
+        st->graph_iters = n_graph;
+        st->tile_js = used_js ? used_js : nnm_join_default_js(c, p->prec == NNM_PREC_FP32 ? NNM_PREC_FP32 : NNM_PREC_BF16);
+        st->tile_chunk = used_chunk ? used_chunk : nnm_join_default_chunk();
         float ms = 0.f;
         hipEventElapsedTime(&ms, c->ev0, c->ev1);
         st->secs = ms * 1e-3;

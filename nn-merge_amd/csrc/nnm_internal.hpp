@@ -38,6 +38,7 @@ enum CounterIdx {
     CNT_OFFERS = 7,    // slot offers issued by the join
     CNT_WORK2 = 8,     // work-queue cursor of the long-list join
     CNT_WORK3 = 9,     // work-queue cursor of the window join
+    CNT_ITER = 10,     // iteration number (salts derive from it on the device)
     CNT_N = 16
 };
 
@@ -108,6 +109,11 @@ struct nnm_ctx {
 
     void* mlp = nullptr;            // learned projection state (nnm_mlp.hip)
 
+    // join launch configuration (autotuned): window rows of the short-list
+    // kernels and rows taken from the work queue at a time
+    // per precision (NNM_PREC_*); tune_ld = 0 means not tuned yet
+    uint32_t tune_js[3] = {0, 0, 0}, tune_chunk[3] = {0, 0, 0}, tune_ld[3] = {0, 0, 0};
+
     std::vector<std::pair<void*, size_t>> allocs;
 };
 
@@ -140,7 +146,10 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
 #define NNM_TRY_MLP NNM_TRY
 
 // Kernel launchers shared between translation units.
-int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t salt);
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk);
+void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32_t, uint32_t>>* out);
+uint32_t nnm_join_default_js(nnm_ctx* c, uint32_t prec);
+uint32_t nnm_join_default_chunk();
 
 // Learned projection (nnm_mlp.hip)
 void nnm_mlp_free(nnm_ctx* c);

@@ -50,7 +50,10 @@ struct JoinArgs {
     uint32_t K;
     uint32_t C;
     uint32_t rule;
-    uint32_t salt;
+    uint32_t salt;    // set inside each kernel from seed and the iteration counter
+    uint64_t seed;
+    uint32_t js;      // short-list window rows (<= the kernel's compiled window)
+    uint32_t chunk;   // rows per work-queue grab
     const uint32_t* l_new;
     uint32_t cap_new;
     const uint32_t* l_old;
@@ -75,6 +78,12 @@ __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Slot salt of the iteration: derived on the device from the seed and the
+// iteration counter, so a captured iteration replays with fresh salts.
+__device__ inline uint32_t join_salt(const JoinArgs& a) {
+    return (uint32_t)nnm_rand(a.seed, 0xC0FFEEull, a.counters[CNT_ITER]);
 }
 
 __device__ inline void offer(uint64_t* __restrict__ slots, uint32_t C, uint32_t salt, uint32_t target,
@@ -342,22 +351,24 @@ __device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, uint32_t l
 
 template <int CPR>  // 16-byte chunks per row: ld = 4 * CPR
 __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
+    a.salt = join_salt(a);
     constexpr uint32_t LD = 4 * CPR;
     constexpr uint32_t PF = LD + 4;
     constexpr uint32_t JS_ROWS = ROW_LDS_BYTES / (PF * 4);
     constexpr uint32_t JS = JS_ROWS < 64 ? JS_ROWS : 64;  // window rows
     static_assert(CPR % 8 == 0 && CPR <= 64, "row length must be a multiple of 32 floats, at most 256");
     __shared__ __align__(16) FastLds<JS> L;
+    const uint32_t js = min(a.js, JS);
 
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
 
     for (;;) {
         uint64_t base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)JOIN_CHUNK);
+        if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)a.chunk);
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
-        const uint32_t cnt = (uint32_t)min((uint64_t)JOIN_CHUNK, a.n - base);
+        const uint32_t cnt = (uint32_t)min((uint64_t)a.chunk, a.n - base);
         // list sizes of the chunk: lane i < cnt holds vertex base + i
         uint32_t cn = 0, co = 0;
         if (lane < cnt) {
@@ -374,12 +385,12 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
             while (i < cnt && nbv < FB) {
                 const uint32_t nn = (uint32_t)__shfl((int)cn, (int)i, NNM_WAVE);
                 const uint32_t m = nn + (uint32_t)__shfl((int)co, (int)i, NNM_WAVE);
-                if (m > JS || nn == 0) {  // long lists: k_join_f32_windows; no new entries: no pairs
-                    if (m <= JS && lane == 0) my_entries += m;
+                if (m > js || nn == 0) {  // long lists: k_join_f32_windows; no new entries: no pairs
+                    if (m <= js && lane == 0) my_entries += m;
                     ++i;
                     continue;
                 }
-                if (rows + m > JS) break;
+                if (rows + m > js) break;
 #pragma unroll
                 for (int b = 0; b < FB; ++b)
                     if ((uint32_t)b == nbv) { off[b] = rows; nnv[b] = nn; mv[b] = m; vid[b] = i; }
@@ -484,20 +495,22 @@ struct Bf16Lds {
 template <int U>  // 8-byte units per bf16 row: ld = 4 * U
 __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb,
                                                   const float* __restrict__ norms, int mode, float kscale, float tau) {
+    a.salt = join_salt(a);
     static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;  // dwords per row
     __shared__ __align__(16) Bf16Lds L;
+    const uint32_t js = min(a.js, (uint32_t)BJS);
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
     const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
 
     for (;;) {
         uint64_t base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)JOIN_CHUNK);
+        if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)a.chunk);
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
-        const uint32_t cnt = (uint32_t)min((uint64_t)JOIN_CHUNK, a.n - base);
+        const uint32_t cnt = (uint32_t)min((uint64_t)a.chunk, a.n - base);
         uint32_t cn = 0, co = 0;
         if (lane < cnt) {
             cn = a.n_new[base + lane];
@@ -512,12 +525,12 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
             while (i < cnt && nbv < BFB) {
                 const uint32_t nn = (uint32_t)__shfl((int)cn, (int)i, NNM_WAVE);
                 const uint32_t m = nn + (uint32_t)__shfl((int)co, (int)i, NNM_WAVE);
-                if (m > BJS || nn == 0) {
-                    if (m <= BJS && lane == 0) my_entries += m;
+                if (m > js || nn == 0) {
+                    if (m <= js && lane == 0) my_entries += m;
                     ++i;
                     continue;
                 }
-                if (rows + m > BJS) break;
+                if (rows + m > js) break;
 #pragma unroll
                 for (int b = 0; b < BFB; ++b)
                     if ((uint32_t)b == nbv) { off[b] = rows; nnv[b] = nn; mv[b] = m; vid[b] = i; }
@@ -642,6 +655,7 @@ struct Bf16LongLds {
 template <int U>
 __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb,
                                                         const float* __restrict__ norms, int mode, float kscale, float tau) {
+    a.salt = join_salt(a);
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;
     __shared__ __align__(16) Bf16LongLds L;
@@ -656,12 +670,12 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
             uint64_t v = cur;
             for (;;) {
                 if (v >= end) {
-                    v = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(JOIN_CHUNK * 16));
-                    end = min(a.n, v + JOIN_CHUNK * 16);
+                    v = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(a.chunk * 16));
+                    end = min(a.n, v + a.chunk * 16);
                     if (v >= a.n) break;
                 }
                 const uint32_t m = a.n_new[v] + a.n_old[v];
-                if (m > BJS && m <= BLM) break;
+                if (m > min(a.js, (uint32_t)BJS) && m <= BLM) break;
                 ++v;
             }
             vshare = v;
@@ -761,6 +775,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
 
 // Lists longer than min_m entries (every list when min_m = 0): windows of 64.
 __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t min_m) {
+    a.salt = join_salt(a);
     __shared__ __align__(16) JoinLds L;
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
@@ -810,8 +825,21 @@ static uint32_t join_grid256(nnm_ctx* c, Kern k) {
     return (uint32_t)c->num_cus * (uint32_t)per_cu;
 }
 
-int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t salt) {
-    (void)iter;
+uint32_t nnm_join_default_js(nnm_ctx* c, uint32_t prec) {
+    return prec == NNM_PREC_FP32 ? fast_window(c->ld) : (uint32_t)BJS;
+}
+uint32_t nnm_join_default_chunk() { return JOIN_CHUNK; }
+
+void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32_t, uint32_t>>* out) {
+    out->clear();
+    const uint32_t top = (prec == NNM_PREC_FP32) ? fast_window(c->ld) : (uint32_t)BJS;
+    const uint32_t cand[3] = {top, top * 2 / 3, top / 2};
+    for (uint32_t js : cand)
+        for (uint32_t ch : {4u, 16u, 64u})
+            if (js >= 8) out->emplace_back(js, ch);
+}
+
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t chunk) {
     JoinArgs a;
     a.X = c->X;
     a.ld = c->ld;
@@ -819,7 +847,9 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
     a.K = c->K;
     a.C = c->slots_C;
     a.rule = p->rule;
-    a.salt = salt;
+    a.salt = 0;
+    a.seed = p->seed;
+    a.chunk = chunk ? chunk : JOIN_CHUNK;
     a.l_new = c->l_new;
     a.cap_new = c->cap_new;
     a.l_old = c->l_old;
@@ -847,6 +877,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
             if (!rows) return nnm_fail(c, NNM_ERR_STATE, "learned join without a projection");
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
+        a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;
 #define NNM_BF(U)                                                                                          \
     case 4 * U:                                                                                            \
         hipLaunchKernelGGL(k_join_bf16<U>, dim3(join_grid(c, k_join_bf16<U>)), dim3(64), 0, c->stream, a,  \
@@ -876,10 +907,11 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t iter, uint32_t sal
     if (p->prec != NNM_PREC_FP32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "join precision not built yet");
     // fast path for lists of at most js entries, windows kernel for the rest
     uint32_t js = 0;
+    a.js = js_req ? min(js_req, fast_window(c->ld)) : fast_window(c->ld);
 #define NNM_FAST(CPR)                                                                                        \
     case 4 * CPR:                                                                                            \
         hipLaunchKernelGGL(k_join_f32<CPR>, dim3(join_grid(c, k_join_f32<CPR>)), dim3(64), 0, c->stream, a); \
-        js = fast_window(4 * CPR);                                                                           \
+        js = a.js;                                                                                           \
         break;
     switch (c->ld) {
         NNM_FAST(8)

@@ -20,19 +20,22 @@ RAW = 0xFFFFFFFF
 PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
 RULE_ALL, RULE_MERGE = 0, 1
 RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
+FLAG_GRAPH, FLAG_AUTOTUNE = 1, 2
 
 
 class Params(C.Structure):
     _fields_ = [("K", C.c_uint32), ("rho", C.c_float), ("rev_cap", C.c_uint32), ("max_iters", C.c_uint32),
                 ("delta", C.c_float), ("rule", C.c_uint32), ("prec", C.c_uint32), ("rerank", C.c_uint32),
-                ("slots", C.c_uint32), ("cut", C.c_uint32), ("plain_tail", C.c_uint32), ("seed", C.c_uint64)]
+                ("slots", C.c_uint32), ("cut", C.c_uint32), ("plain_tail", C.c_uint32), ("flags", C.c_uint32),
+                ("seed", C.c_uint64)]
 
 
 class Stats(C.Structure):
     _fields_ = [("iters", C.c_uint32), ("converged", C.c_uint32), ("dists", C.c_uint64),
                 ("init_dists", C.c_uint64), ("updates", C.c_uint64), ("reranked", C.c_uint64),
                 ("list_entries", C.c_uint64), ("offers", C.c_uint64), ("last_delta", C.c_double), ("secs", C.c_double),
-                ("join_secs", C.c_double), ("learned_iters", C.c_uint32), ("plain_iters", C.c_uint32)]
+                ("join_secs", C.c_double), ("learned_iters", C.c_uint32), ("plain_iters", C.c_uint32),
+                ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
