@@ -112,6 +112,15 @@ typedef struct nnm_stats {
     uint32_t tile_chunk;   /* rows per work-queue grab used */
     uint32_t graph_iters;  /* iterations replayed from a HIP graph */
     uint32_t pad_;
+    uint64_t long_dists;   /* part of dists / list_entries / offers that the */
+    uint64_t long_entries; /* long-list join kernels account for; the rest  */
+    uint64_t long_offers;  /* is the short-list (packed) join kernel's      */
+    double join_main_secs; /* short-list join kernel time (eager iterations) */
+    uint64_t join_main_bytes; /* its algorithmic bytes: per list entry the row,
+                                 id, label, k-th distance (+ norm in the norm
+                                 form); 8 per offer; 8 per vertex */
+    uint64_t join_main_flops; /* its distance flops: 3d per FP32 distance
+                                 (difference form), 2d per MFMA distance */
 } nnm_stats;
 
 void nnm_default_params(nnm_params* p);

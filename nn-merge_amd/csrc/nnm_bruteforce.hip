@@ -163,13 +163,13 @@ extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* ro
     if (Q) {
         rc = nnm_alloc(c, (void**)&dQ, nq * c->ld * sizeof(float));
         if (rc) { cleanup(); return rc; }
-        if (c->ld != c->d) hipMemsetAsync(dQ, 0, nq * c->ld * sizeof(float), c->stream);
-        hipMemcpy2DAsync(dQ, c->ld * sizeof(float), Q, c->d * sizeof(float), c->d * sizeof(float), nq,
-                         hipMemcpyHostToDevice, c->stream);
+        rc = nnm_h2d_rows(c, dQ, c->ld, Q, c->d, nq);
+        if (rc) { cleanup(); return rc; }
     } else {
         rc = nnm_alloc(c, (void**)&dR, nq * sizeof(uint32_t));
         if (rc) { cleanup(); return rc; }
-        hipMemcpyAsync(dR, rows, nq * 4, hipMemcpyHostToDevice, c->stream);
+        rc = nnm_h2d(c, dR, rows, nq * 4);
+        if (rc) { cleanup(); return rc; }
     }
     if ((rc = nnm_alloc(c, (void**)&pd, nq * nsplit * BF_KMAX * 4)) ||
         (rc = nnm_alloc(c, (void**)&pi, nq * nsplit * BF_KMAX * 4)) ||
@@ -193,10 +193,9 @@ extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* ro
                        pi, od, oi);
     e = hipGetLastError();
     if (e != hipSuccess) { cleanup(); return nnm_hip_fail(c, e, "k_bf_merge launch"); }
-    hipMemcpyAsync(ids, oi, nq * k * 4, hipMemcpyDeviceToHost, c->stream);
-    hipMemcpyAsync(dist, od, nq * k * 4, hipMemcpyDeviceToHost, c->stream);
-    e = hipStreamSynchronize(c->stream);
+    rc = nnm_d2h(c, ids, oi, nq * k * 4);
+    if (rc == NNM_OK) rc = nnm_d2h(c, dist, od, nq * k * 4);
     cleanup();
-    if (e != hipSuccess) return nnm_hip_fail(c, e, "nnm_bruteforce_knn");
+    if (rc != NNM_OK) return rc;
     return NNM_OK;
 }

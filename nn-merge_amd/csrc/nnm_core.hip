@@ -60,6 +60,90 @@ void nnm_free(nnm_ctx* c, void* p) {
     hipFree(p);
 }
 
+// ---------------------------------------------------------------- transfers
+// Every host <-> device copy goes through two pinned staging buffers that the
+// context allocates once: the runtime's path for pageable memory pins the
+// caller's pages for large copies, which not every host allows, and pinned
+// staging also keeps the copy engine at full PCIe rate.
+#define NNM_STAGE_BYTES (size_t(16) << 20)
+
+static int ensure_stage(nnm_ctx* c) {
+    if (c->stage[0]) return NNM_OK;
+    for (int i = 0; i < 2; ++i) {
+        hipError_t e = hipHostMalloc(&c->stage[i], NNM_STAGE_BYTES, hipHostMallocDefault);
+        if (e != hipSuccess) { c->stage[i] = nullptr; return nnm_hip_fail(c, e, "hipHostMalloc (staging)"); }
+        e = hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming);
+        if (e != hipSuccess) return nnm_hip_fail(c, e, "hipEventCreate (staging)");
+        // recorded once so the first wait on it returns at once
+        NNM_HIP(c, hipEventRecord(c->stage_ev[i], c->stream));
+    }
+    return NNM_OK;
+}
+
+static void free_stage(nnm_ctx* c) {
+    for (int i = 0; i < 2; ++i) {
+        if (c->stage[i]) hipHostFree(c->stage[i]);
+        if (c->stage_ev[i]) hipEventDestroy(c->stage_ev[i]);
+        c->stage[i] = nullptr;
+        c->stage_ev[i] = nullptr;
+    }
+}
+
+int nnm_h2d(nnm_ctx* c, void* dst, const void* src, size_t bytes) {
+    NNM_TRY(ensure_stage(c));
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += NNM_STAGE_BYTES, k ^= 1) {
+        const size_t m = std::min(NNM_STAGE_BYTES, bytes - off);
+        NNM_HIP(c, hipEventSynchronize(c->stage_ev[k]));  // buffer k is free again
+        std::memcpy(c->stage[k], s + off, m);
+        NNM_HIP(c, hipMemcpyAsync(d + off, c->stage[k], m, hipMemcpyHostToDevice, c->stream));
+        NNM_HIP(c, hipEventRecord(c->stage_ev[k], c->stream));
+    }
+    return NNM_OK;
+}
+
+int nnm_h2d_rows(nnm_ctx* c, float* dst, uint32_t ld, const float* src, uint32_t d, uint64_t n) {
+    NNM_TRY(ensure_stage(c));
+    const uint64_t per = std::max<uint64_t>(1, NNM_STAGE_BYTES / ((size_t)ld * 4));
+    if ((size_t)ld * 4 > NNM_STAGE_BYTES) return nnm_fail(c, NNM_ERR_INVALID, "row too long for staging");
+    int k = 0;
+    for (uint64_t r0 = 0; r0 < n; r0 += per, k ^= 1) {
+        const uint64_t m = std::min(per, n - r0);
+        NNM_HIP(c, hipEventSynchronize(c->stage_ev[k]));
+        float* st = static_cast<float*>(c->stage[k]);
+        for (uint64_t r = 0; r < m; ++r) {
+            std::memcpy(st + r * ld, src + (r0 + r) * d, (size_t)d * 4);
+            if (ld > d) std::memset(st + r * ld + d, 0, (size_t)(ld - d) * 4);
+        }
+        NNM_HIP(c, hipMemcpyAsync(dst + r0 * ld, st, m * ld * 4, hipMemcpyHostToDevice, c->stream));
+        NNM_HIP(c, hipEventRecord(c->stage_ev[k], c->stream));
+    }
+    return NNM_OK;
+}
+
+int nnm_d2h(nnm_ctx* c, void* dst, const void* src, size_t bytes) {
+    NNM_TRY(ensure_stage(c));
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    const size_t nch = (bytes + NNM_STAGE_BYTES - 1) / NNM_STAGE_BYTES;
+    auto issue = [&](size_t i) -> int {
+        const size_t off = i * NNM_STAGE_BYTES, m = std::min(NNM_STAGE_BYTES, bytes - off);
+        NNM_HIP(c, hipMemcpyAsync(c->stage[i & 1], s + off, m, hipMemcpyDeviceToHost, c->stream));
+        NNM_HIP(c, hipEventRecord(c->stage_ev[i & 1], c->stream));
+        return NNM_OK;
+    };
+    if (nch) NNM_TRY(issue(0));
+    for (size_t i = 0; i < nch; ++i) {
+        if (i + 1 < nch) NNM_TRY(issue(i + 1));
+        NNM_HIP(c, hipEventSynchronize(c->stage_ev[i & 1]));
+        const size_t off = i * NNM_STAGE_BYTES;
+        std::memcpy(d + off, c->stage[i & 1], std::min(NNM_STAGE_BYTES, bytes - off));
+    }
+    return NNM_OK;
+}
+
 template <typename T>
 static int alloc_n(nnm_ctx* c, T** p, size_t n) {
     return nnm_alloc(c, reinterpret_cast<void**>(p), n * sizeof(T));
@@ -139,6 +223,7 @@ extern "C" int nnm_create(nnm_ctx** out, int device) {
     hipEventCreate(&c->ev1);
     hipEventCreate(&c->evj0);
     hipEventCreate(&c->evj1);
+    hipEventCreate(&c->evjm);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     if (alloc_n(c, &c->counters, CNT_N) != NNM_OK) { delete c; return NNM_ERR_NOMEM; }
@@ -151,12 +236,14 @@ extern "C" int nnm_destroy(nnm_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     nnm_mlp_free(c);
+    free_stage(c);
     for (auto& a : c->allocs) hipFree(a.first);
     c->allocs.clear();
     hipEventDestroy(c->ev0);
     hipEventDestroy(c->ev1);
     hipEventDestroy(c->evj0);
     hipEventDestroy(c->evj1);
+    hipEventDestroy(c->evjm);
     hipStreamDestroy(c->stream);
     delete c;
     return NNM_OK;
@@ -251,9 +338,7 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
     c->d = d;
     c->ld = (d + 31) / 32 * 32;
     NNM_TRY(alloc_n(c, &c->X, n * c->ld));
-    if (c->ld != d) NNM_HIP(c, hipMemsetAsync(c->X, 0, n * c->ld * sizeof(float), c->stream));
-    NNM_HIP(c, hipMemcpy2DAsync(c->X, c->ld * sizeof(float), X, d * sizeof(float), d * sizeof(float), n,
-                                hipMemcpyHostToDevice, c->stream));
+    NNM_TRY(nnm_h2d_rows(c, c->X, c->ld, X, d, n));
     // bf16 exactness: integers of magnitude <= 256 are exact in bf16, and
     // with d <= 128 every partial sum of the norm form stays below 2^24
     int exact = (d <= 128) ? 1 : 0;
@@ -328,11 +413,11 @@ extern "C" int nnm_set_partition(nnm_ctx* c, const uint32_t* sub_begin, const ui
     NNM_TRY(alloc_n(c, &c->grp_active, c->ngrp));
     std::vector<uint8_t> built01(nsub);
     for (uint32_t s = 0; s < nsub; ++s) built01[s] = sub_built[s] ? 1 : 0;
-    NNM_HIP(c, hipMemcpyAsync(c->sub_begin, sub_begin, (nsub + 1) * 4, hipMemcpyHostToDevice, c->stream));
-    NNM_HIP(c, hipMemcpyAsync(c->sub_grp, sub_group, nsub * 4, hipMemcpyHostToDevice, c->stream));
-    NNM_HIP(c, hipMemcpyAsync(c->sub_built, built01.data(), nsub, hipMemcpyHostToDevice, c->stream));
-    NNM_HIP(c, hipMemcpyAsync(c->grp_begin, gb.data(), gb.size() * 4, hipMemcpyHostToDevice, c->stream));
-    NNM_HIP(c, hipMemcpyAsync(c->grp_end, ge.data(), ge.size() * 4, hipMemcpyHostToDevice, c->stream));
+    NNM_TRY(nnm_h2d(c, c->sub_begin, sub_begin, (nsub + 1) * 4));
+    NNM_TRY(nnm_h2d(c, c->sub_grp, sub_group, nsub * 4));
+    NNM_TRY(nnm_h2d(c, c->sub_built, built01.data(), nsub));
+    NNM_TRY(nnm_h2d(c, c->grp_begin, gb.data(), gb.size() * 4));
+    NNM_TRY(nnm_h2d(c, c->grp_end, ge.data(), ge.size() * 4));
     c->n_act = sub_begin[nsub];
     hipLaunchKernelGGL(k_fill_labels, dim3(grid_for(c->n_act, 256, 1u << 30)), dim3(256), 0, c->stream, c->label,
                        c->vsub, c->sub_begin, c->sub_built, nsub, c->n_act);
@@ -600,7 +685,7 @@ static void stats_reset(nnm_stats* st) {
 }
 
 static int read_counters(nnm_ctx* c, uint64_t* h) {
-    NNM_HIP(c, hipMemcpyAsync(h, c->counters, CNT_N * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    NNM_TRY(nnm_d2h(c, h, c->counters, CNT_N * sizeof(uint64_t)));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     return NNM_OK;
 }
@@ -636,9 +721,9 @@ extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist,
     stats_reset(st);
     NNM_TRY(ensure_graph(c, K));
     const uint64_t nk = c->n * K;
-    NNM_HIP(c, hipMemcpyAsync(c->g_ids, ids, nk * 4, hipMemcpyHostToDevice, c->stream));
-    if (dist) NNM_HIP(c, hipMemcpyAsync(c->g_dist, dist, nk * 4, hipMemcpyHostToDevice, c->stream));
-    if (flags) NNM_HIP(c, hipMemcpyAsync(c->g_flag, flags, nk, hipMemcpyHostToDevice, c->stream));
+    NNM_TRY(nnm_h2d(c, c->g_ids, ids, nk * 4));
+    if (dist) NNM_TRY(nnm_h2d(c, c->g_dist, dist, nk * 4));
+    if (flags) NNM_TRY(nnm_h2d(c, c->g_flag, flags, nk));
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -664,9 +749,9 @@ extern "C" int nnm_get_graph(nnm_ctx* c, uint32_t* ids, float* dist, uint8_t* fl
     if (!c) return NNM_ERR_INVALID;
     if (!c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no graph");
     const uint64_t nk = c->n * c->K;
-    if (ids) NNM_HIP(c, hipMemcpyAsync(ids, c->g_ids, nk * 4, hipMemcpyDeviceToHost, c->stream));
-    if (dist) NNM_HIP(c, hipMemcpyAsync(dist, c->g_dist, nk * 4, hipMemcpyDeviceToHost, c->stream));
-    if (flags) NNM_HIP(c, hipMemcpyAsync(flags, c->g_flag, nk, hipMemcpyDeviceToHost, c->stream));
+    if (ids) NNM_TRY(nnm_d2h(c, ids, c->g_ids, nk * 4));
+    if (dist) NNM_TRY(nnm_d2h(c, dist, c->g_dist, nk * 4));
+    if (flags) NNM_TRY(nnm_d2h(c, flags, c->g_flag, nk));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     return NNM_OK;
 }
@@ -727,12 +812,12 @@ extern "C" int nnm_pair_dists(nnm_ctx* c, const uint32_t* a, const uint32_t* b, 
     NNM_TRY(alloc_n(c, &da, m));
     NNM_TRY(alloc_n(c, &db, m));
     NNM_TRY(alloc_n(c, &dout, m));
-    NNM_HIP(c, hipMemcpyAsync(da, a, m * 4, hipMemcpyHostToDevice, c->stream));
-    NNM_HIP(c, hipMemcpyAsync(db, b, m * 4, hipMemcpyHostToDevice, c->stream));
+    NNM_TRY(nnm_h2d(c, da, a, m * 4));
+    NNM_TRY(nnm_h2d(c, db, b, m * 4));
     hipLaunchKernelGGL(k_pair_dists, dim3(grid_for(m, 256, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, da,
                        db, m, dout);
     NNM_LAUNCH_CHECK(c);
-    NNM_HIP(c, hipMemcpyAsync(out, dout, m * 4, hipMemcpyDeviceToHost, c->stream));
+    NNM_TRY(nnm_d2h(c, out, dout, m * 4));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     nnm_free(c, da);
     nnm_free(c, db);
@@ -1190,6 +1275,17 @@ static int reset_queues(nnm_ctx* c) {
     return NNM_OK;
 }
 
+// join_ms[0] += whole join, join_ms[1] += its short-list kernel (evj0..evjm)
+static int add_join_times(nnm_ctx* c, double* join_ms) {
+    NNM_HIP(c, hipEventSynchronize(c->evj1));
+    float ms = 0.f, mm = 0.f;
+    hipEventElapsedTime(&ms, c->evj0, c->evj1);
+    hipEventElapsedTime(&mm, c->evj0, c->evjm);
+    join_ms[0] += ms;
+    join_ms[1] += mm;
+    return NNM_OK;
+}
+
 // One iteration; leaves the flag count in counters[CNT_FLAGS].  The
 // iteration number is read from counters[CNT_ITER], which the caller sets.
 // join_ms == nullptr: no events or waits, so the sequence can be captured.
@@ -1198,15 +1294,10 @@ static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t 
     NNM_TRY(reset_queues(c));
     NNM_TRY(build_lists(c, p));
     if (join_ms) NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
-    NNM_TRY(nnm_launch_join(c, p, js, chunk));
+    NNM_TRY(nnm_launch_join(c, p, js, chunk, join_ms ? c->evjm : nullptr));
     if (join_ms) NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
     NNM_TRY(update_lists(c, p));
-    if (join_ms) {
-        NNM_HIP(c, hipEventSynchronize(c->evj1));
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, c->evj0, c->evj1);
-        *join_ms += ms;
-    }
+    if (join_ms) NNM_TRY(add_join_times(c, join_ms));
     return NNM_OK;
 }
 
@@ -1239,17 +1330,14 @@ static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) 
     c->tune_js[p->prec] = best_js;
     c->tune_chunk[p->prec] = best_ch;
     c->tune_ld[p->prec] = c->ld;
-    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS}) NNM_TRY(fill_u64(c, c->counters + i, 1, h[i]));
+    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS, CNT_LONG_DISTS, CNT_LONG_ENTRIES, CNT_LONG_OFFERS})
+        NNM_TRY(fill_u64(c, c->counters + i, 1, h[i]));
     NNM_TRY(reset_queues(c));
     NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
-    NNM_TRY(nnm_launch_join(c, p, best_js, best_ch));
+    NNM_TRY(nnm_launch_join(c, p, best_js, best_ch, c->evjm));
     NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
     NNM_TRY(update_lists(c, p));
-    NNM_HIP(c, hipEventSynchronize(c->evj1));
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, c->evj0, c->evj1);
-    *join_ms += ms;
-    return NNM_OK;
+    return add_join_times(c, join_ms);
 }
 
 // Instantiated iteration graphs of one descent call (one per join precision).
@@ -1291,9 +1379,10 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     stats_reset(st);
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    double join_ms = 0.0;
+    double join_ms[2] = {0.0, 0.0};
     uint32_t it = 0, n_learned = 0, n_plain = 0, n_graph = 0;
     uint32_t used_js = 0, used_chunk = 0;
+    uint64_t prev_se = 0, prev_sd = 0, prev_so = 0, main_bytes = 0, main_flops = 0;
     int converged = 0;
     double last_delta = 1.0;
     const bool autotune = (p->flags & NNM_FLAG_AUTOTUNE) != 0;
@@ -1316,19 +1405,44 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         const uint32_t js = (autotune && tuned) ? c->tune_js[q.prec] : 0;
         const uint32_t chunk = (autotune && tuned) ? c->tune_chunk[q.prec] : 0;
         if (autotune && !tuned) {
-            NNM_TRY(autotune_iteration(c, &q, &join_ms));
+            NNM_TRY(autotune_iteration(c, &q, join_ms));
         } else if (graph) {
             hipGraphExec_t& ge = graphs.exec[q.prec];
             if (!ge) NNM_TRY(capture_iteration(c, &q, js, chunk, &ge));
             NNM_HIP(c, hipGraphLaunch(ge, c->stream));
             ++n_graph;
         } else {
-            NNM_TRY(run_iteration(c, &q, js, chunk, &join_ms));
+            NNM_TRY(run_iteration(c, &q, js, chunk, join_ms));
         }
         used_js = autotune ? c->tune_js[q.prec] : js;
         used_chunk = autotune ? c->tune_chunk[q.prec] : chunk;
         uint64_t h[CNT_N];
         NNM_TRY(read_counters(c, h));
+        {
+            // algorithmic traffic and work of the short-list join kernel in
+            // this iteration: per list entry one row, its id, label and
+            // k-th distance (and norm for the norm form); per offer one
+            // 8-byte slot update; per vertex the two list sizes
+            const uint64_t se = (h[CNT_LIST_ENTRIES] - h[CNT_LONG_ENTRIES]) - prev_se;
+            const uint64_t sd = (h[CNT_DISTS] - h[CNT_LONG_DISTS]) - prev_sd;
+            const uint64_t so = (h[CNT_OFFERS] - h[CNT_LONG_OFFERS]) - prev_so;
+            prev_se += se;
+            prev_sd += sd;
+            prev_so += so;
+            uint64_t row_bytes, flops_per_dist;
+            if (q.prec == NNM_PREC_FP32) {
+                row_bytes = (uint64_t)c->d * 4 + 12;
+                flops_per_dist = 3ull * c->d;
+            } else if (q.prec == NNM_PREC_LEARNED) {
+                row_bytes = (uint64_t)nnm_mlp_dout(c) * 2 + 16;
+                flops_per_dist = 2ull * nnm_mlp_dout(c);
+            } else {
+                row_bytes = (uint64_t)c->d * 2 + 16;
+                flops_per_dist = 2ull * c->d;
+            }
+            main_bytes += se * row_bytes + so * 8 + c->n_act * 8;
+            main_flops += sd * flops_per_dist;
+        }
         ++it;
         last_delta = (double)h[CNT_FLAGS] / ((double)c->n_act * c->K);
         if (last_delta < p->delta) {
@@ -1362,7 +1476,13 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         float ms = 0.f;
         hipEventElapsedTime(&ms, c->ev0, c->ev1);
         st->secs = ms * 1e-3;
-        st->join_secs = join_ms * 1e-3;
+        st->join_secs = join_ms[0] * 1e-3;
+        st->join_main_secs = join_ms[1] * 1e-3;
+        st->long_dists = h[CNT_LONG_DISTS];
+        st->long_entries = h[CNT_LONG_ENTRIES];
+        st->long_offers = h[CNT_LONG_OFFERS];
+        st->join_main_bytes = main_bytes;
+        st->join_main_flops = main_flops;
     }
     return NNM_OK;
 }

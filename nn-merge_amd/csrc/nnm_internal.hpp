@@ -39,13 +39,16 @@ enum CounterIdx {
     CNT_WORK2 = 8,     // work-queue cursor of the long-list join
     CNT_WORK3 = 9,     // work-queue cursor of the window join
     CNT_ITER = 10,     // iteration number (salts derive from it on the device)
+    CNT_LONG_DISTS = 11,    // share of DISTS / LIST_ENTRIES / OFFERS that the
+    CNT_LONG_ENTRIES = 12,  // long-list join kernels account for (the rest is
+    CNT_LONG_OFFERS = 13,   // the short-list kernel's)
     CNT_N = 16
 };
 
 struct nnm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evj0 = nullptr, evj1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evj0 = nullptr, evj1 = nullptr, evjm = nullptr;
     std::string err;
     uint64_t mem_cap = 0;
     uint64_t mem_used = 0;
@@ -115,6 +118,10 @@ struct nnm_ctx {
     uint32_t tune_js[3] = {0, 0, 0}, tune_chunk[3] = {0, 0, 0}, tune_ld[3] = {0, 0, 0};
 
     std::vector<std::pair<void*, size_t>> allocs;
+
+    // pinned staging buffers for host <-> device transfers (see nnm_h2d)
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 // ------------------------------------------------------------ host helpers
@@ -123,6 +130,13 @@ int nnm_fail(nnm_ctx* c, int code, const std::string& msg);
 int nnm_hip_fail(nnm_ctx* c, hipError_t e, const char* what);
 int nnm_alloc(nnm_ctx* c, void** p, size_t bytes);
 void nnm_free(nnm_ctx* c, void* p);
+// Host <-> device copies on the context stream through the context's pinned
+// staging buffers.  nnm_h2d returns once the source has been read (the copy
+// may still be in flight); nnm_d2h returns with the data in dst.
+// nnm_h2d_rows uploads n rows of d floats into rows of pitch ld (zero padded).
+int nnm_h2d(nnm_ctx* c, void* dst, const void* src, size_t bytes);
+int nnm_d2h(nnm_ctx* c, void* dst, const void* src, size_t bytes);
+int nnm_h2d_rows(nnm_ctx* c, float* dst, uint32_t ld, const float* src, uint32_t d, uint64_t n);
 int nnm_ensure_bf16(nnm_ctx* c);
 int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
 
@@ -146,7 +160,7 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
 #define NNM_TRY_MLP NNM_TRY
 
 // Kernel launchers shared between translation units.
-int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk);
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, hipEvent_t mid = nullptr);
 void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32_t, uint32_t>>* out);
 uint32_t nnm_join_default_js(nnm_ctx* c, uint32_t prec);
 uint32_t nnm_join_default_chunk();

@@ -770,6 +770,9 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
         atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
         atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)my_dists);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)my_entries);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)my_offers);
     }
 }
 
@@ -803,6 +806,9 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
         atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
         atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)my_dists);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)my_entries);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)my_offers);
     }
 }
 
@@ -839,7 +845,7 @@ void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32
             if (js >= 8) out->emplace_back(js, ch);
 }
 
-int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t chunk) {
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t chunk, hipEvent_t mid) {
     JoinArgs a;
     a.X = c->X;
     a.ld = c->ld;
@@ -883,6 +889,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
         hipLaunchKernelGGL(k_join_bf16<U>, dim3(join_grid(c, k_join_bf16<U>)), dim3(64), 0, c->stream, a,  \
                            rows, nrm, mode, kscale, tau);                                                  \
         NNM_LAUNCH_CHECK(c);                                                                               \
+        if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));                                              \
         hipLaunchKernelGGL(k_join_bf16_long<U>, dim3(join_grid256(c, k_join_bf16_long<U>)), dim3(256), 0,  \
                            c->stream, a, rows, nrm, mode, kscale, tau);                                    \
         break;
@@ -926,6 +933,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
     }
 #undef NNM_FAST
     NNM_LAUNCH_CHECK(c);
+    if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));
     hipLaunchKernelGGL(k_join_f32_windows, dim3(join_grid(c, k_join_f32_windows)), dim3(64), 0, c->stream, a, js);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;

@@ -129,13 +129,23 @@ __global__ void k_relu_grad(const float* __restrict__ pre, float* __restrict__ d
     if (i < n && pre[i] <= 0.f) dh[i] = 0.f;
 }
 
-// Column sums: out[j] = sum_r in[r * cols + j]
-__global__ void k_colsum(const float* __restrict__ in, uint32_t rows, uint32_t cols, float* __restrict__ out) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
+// Column sums: out[j] = sum_r in[r * cols + j].  Block (64, 16): 64 columns,
+// 16 row phases summed in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ in, uint32_t rows, uint32_t cols,
+                                                 float* __restrict__ out) {
+    __shared__ float part[16][64];
+    const uint32_t tx = threadIdx.x, ty = threadIdx.y;
+    const uint32_t j = blockIdx.x * 64 + tx;
     float acc = 0.f;
-    for (uint32_t r = 0; r < rows; ++r) acc += in[(uint64_t)r * cols + j];
-    out[j] = acc;
+    if (j < cols)
+        for (uint32_t r = ty; r < rows; r += 16) acc += in[(uint64_t)r * cols + j];
+    part[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && j < cols) {
+        float s = 0.f;
+        for (int q = 0; q < 16; ++q) s += part[q][tx];
+        out[j] = s;
+    }
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -316,11 +326,11 @@ static int forward_backward(nnm_ctx* c, uint32_t B, bool bf16) {
                        M->loss);
     // gW2 = dz^T h ; gb2 = colsum dz ; dh = dz W2 (relu mask) ; gW1 = dh^T xs ; gb1 = colsum dh
     NNM_TRY_MLP(gemm(c, bf16, O, H, R, M->dz, 1, O, M->h, H, 1, nullptr, gW2, H));
-    hipLaunchKernelGGL(k_colsum, dim3((O + 255) / 256), dim3(256), 0, c->stream, M->dz, R, O, gb2);
+    hipLaunchKernelGGL(k_colsum, dim3((O + 63) / 64), dim3(64, 16), 0, c->stream, M->dz, R, O, gb2);
     NNM_TRY_MLP(gemm(c, bf16, R, H, O, M->dz, O, 1, M->W2, H, 1, nullptr, M->dh, H));
     hipLaunchKernelGGL(k_relu_grad, dim3((R * H + 255) / 256), dim3(256), 0, c->stream, M->pre, M->dh, (uint64_t)R * H);
     NNM_TRY_MLP(gemm(c, bf16, H, D, R, M->dh, 1, H, M->xs, D, 1, nullptr, gW1, D));
-    hipLaunchKernelGGL(k_colsum, dim3((H + 255) / 256), dim3(256), 0, c->stream, M->dh, R, H, gb1);
+    hipLaunchKernelGGL(k_colsum, dim3((H + 63) / 64), dim3(64, 16), 0, c->stream, M->dh, R, H, gb1);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
@@ -357,17 +367,20 @@ extern "C" int nnm_mlp_init(nnm_ctx* c, uint32_t hidden, uint32_t d_out, uint64_
         for (uint32_t k = 0; k < c->d; ++k) w[(size_t)o * M->din + k] = (float)(s1 * gauss((size_t)o * M->din + k));
     const size_t w2 = (size_t)hidden * M->din + hidden;
     for (size_t i = 0; i < (size_t)d_out * hidden; ++i) w[w2 + i] = (float)(s2 * gauss(w2 + i));
-    NNM_HIP(c, hipMemcpy(M->W1, w.data(), M->nparam * 4, hipMemcpyHostToDevice));
-    NNM_HIP(c, hipMemset(M->m, 0, M->nparam * 4));
-    NNM_HIP(c, hipMemset(M->v, 0, M->nparam * 4));
+    NNM_TRY_MLP(nnm_h2d(c, M->W1, w.data(), M->nparam * 4));
+    NNM_HIP(c, hipMemsetAsync(M->m, 0, M->nparam * 4, c->stream));
+    NNM_HIP(c, hipMemsetAsync(M->v, 0, M->nparam * 4, c->stream));
     // input scale: a^2 mean |x|^2 = d over a sample of rows
     {
         const uint64_t ns = std::min<uint64_t>(c->n, 4096);
         std::vector<float> rows(ns * c->ld);
         double acc = 0.0;
-        for (uint64_t i = 0; i < ns; ++i) {
-            const uint64_t r = (c->n * i) / ns;
-            NNM_HIP(c, hipMemcpy(rows.data() + i * c->ld, c->X + r * c->ld, c->ld * 4, hipMemcpyDeviceToHost));
+        // the first rows of ns evenly spaced blocks of the matrix
+        const uint64_t blk = 64, nb = (ns + blk - 1) / blk;
+        for (uint64_t b = 0; b < nb; ++b) {
+            const uint64_t r = (c->n - blk) * b / std::max<uint64_t>(1, nb - 1);
+            const uint64_t m = std::min(blk, ns - b * blk);
+            NNM_TRY_MLP(nnm_d2h(c, rows.data() + b * blk * c->ld, c->X + r * c->ld, m * c->ld * 4));
         }
         for (float x : rows) acc += (double)x * x;
         acc /= (double)ns;
@@ -383,10 +396,10 @@ extern "C" int nnm_mlp_set_weights(nnm_ctx* c, const float* W1, const float* b1,
                                    float alpha) {
     if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
     nnm_mlp_state* M = mlp_of(c);
-    NNM_HIP(c, hipMemcpy(M->W1, W1, (size_t)M->hid * M->din * 4, hipMemcpyHostToDevice));
-    NNM_HIP(c, hipMemcpy(M->b1, b1, (size_t)M->hid * 4, hipMemcpyHostToDevice));
-    NNM_HIP(c, hipMemcpy(M->W2, W2, (size_t)M->dout * M->hid * 4, hipMemcpyHostToDevice));
-    NNM_HIP(c, hipMemcpy(M->b2, b2, (size_t)M->dout * 4, hipMemcpyHostToDevice));
+    NNM_TRY_MLP(nnm_h2d(c, M->W1, W1, (size_t)M->hid * M->din * 4));
+    NNM_TRY_MLP(nnm_h2d(c, M->b1, b1, (size_t)M->hid * 4));
+    NNM_TRY_MLP(nnm_h2d(c, M->W2, W2, (size_t)M->dout * M->hid * 4));
+    NNM_TRY_MLP(nnm_h2d(c, M->b2, b2, (size_t)M->dout * 4));
     if (alpha > 0.f) M->alpha = alpha;
     return NNM_OK;
 }
@@ -396,10 +409,10 @@ extern "C" int nnm_mlp_get_weights(nnm_ctx* c, float* W1, float* b1, float* W2, 
     if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
     nnm_mlp_state* M = mlp_of(c);
     NNM_HIP(c, hipStreamSynchronize(c->stream));
-    if (W1) NNM_HIP(c, hipMemcpy(W1, M->W1, (size_t)M->hid * M->din * 4, hipMemcpyDeviceToHost));
-    if (b1) NNM_HIP(c, hipMemcpy(b1, M->b1, (size_t)M->hid * 4, hipMemcpyDeviceToHost));
-    if (W2) NNM_HIP(c, hipMemcpy(W2, M->W2, (size_t)M->dout * M->hid * 4, hipMemcpyDeviceToHost));
-    if (b2) NNM_HIP(c, hipMemcpy(b2, M->b2, (size_t)M->dout * 4, hipMemcpyDeviceToHost));
+    if (W1) NNM_TRY_MLP(nnm_d2h(c, W1, M->W1, (size_t)M->hid * M->din * 4));
+    if (b1) NNM_TRY_MLP(nnm_d2h(c, b1, M->b1, (size_t)M->hid * 4));
+    if (W2) NNM_TRY_MLP(nnm_d2h(c, W2, M->W2, (size_t)M->dout * M->hid * 4));
+    if (b2) NNM_TRY_MLP(nnm_d2h(c, b2, M->b2, (size_t)M->dout * 4));
     if (alpha) *alpha = M->alpha;
     if (tau) *tau = M->tau;
     return NNM_OK;
@@ -418,13 +431,13 @@ extern "C" int nnm_mlp_loss_grad(nnm_ctx* c, const float* xa, const float* xb, c
             xs[(size_t)(2 * i) * M->din + k] = xa[(size_t)i * c->d + k];
             xs[(size_t)(2 * i + 1) * M->din + k] = xb[(size_t)i * c->d + k];
         }
-    NNM_HIP(c, hipMemcpy(M->xs, xs.data(), xs.size() * 4, hipMemcpyHostToDevice));
-    NNM_HIP(c, hipMemcpy(M->t, t, B * 4, hipMemcpyHostToDevice));
-    NNM_HIP(c, hipMemcpy(M->s, s, B * 4, hipMemcpyHostToDevice));
+    NNM_TRY_MLP(nnm_h2d(c, M->xs, xs.data(), xs.size() * 4));
+    NNM_TRY_MLP(nnm_h2d(c, M->t, t, B * 4));
+    NNM_TRY_MLP(nnm_h2d(c, M->s, s, B * 4));
     NNM_TRY_MLP(forward_backward(c, B, bf16 != 0));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     std::vector<float> lv(B);
-    NNM_HIP(c, hipMemcpy(lv.data(), M->loss, B * 4, hipMemcpyDeviceToHost));
+    NNM_TRY_MLP(nnm_d2h(c, lv.data(), M->loss, B * 4));
     double acc = 0;
     for (float x : lv) acc += x;
     if (loss) *loss = (float)(acc / B);
@@ -433,13 +446,13 @@ extern "C" int nnm_mlp_loss_grad(nnm_ctx* c, const float* xa, const float* xb, c
     // report gW1 without the padded input columns
     if (gW1) {
         std::vector<float> tmp(nW1);
-        NNM_HIP(c, hipMemcpy(tmp.data(), g, nW1 * 4, hipMemcpyDeviceToHost));
+        NNM_TRY_MLP(nnm_d2h(c, tmp.data(), g, nW1 * 4));
         for (uint32_t o = 0; o < M->hid; ++o)
             std::memcpy(gW1 + (size_t)o * c->d, tmp.data() + (size_t)o * M->din, c->d * 4);
     }
-    if (gb1) NNM_HIP(c, hipMemcpy(gb1, g + nW1, M->hid * 4, hipMemcpyDeviceToHost));
-    if (gW2) NNM_HIP(c, hipMemcpy(gW2, g + nW1 + M->hid, nW2 * 4, hipMemcpyDeviceToHost));
-    if (gb2) NNM_HIP(c, hipMemcpy(gb2, g + nW1 + M->hid + nW2, M->dout * 4, hipMemcpyDeviceToHost));
+    if (gb1) NNM_TRY_MLP(nnm_d2h(c, gb1, g + nW1, M->hid * 4));
+    if (gW2) NNM_TRY_MLP(nnm_d2h(c, gW2, g + nW1 + M->hid, nW2 * 4));
+    if (gb2) NNM_TRY_MLP(nnm_d2h(c, gb2, g + nW1 + M->hid + nW2, M->dout * 4));
     return NNM_OK;
 }
 
@@ -455,9 +468,9 @@ static int calibrate_tau(nnm_ctx* c, uint32_t B, float keep, uint64_t seed) {
     NNM_TRY_MLP(forward_backward(c, B, true));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     std::vector<float> z((size_t)2 * B * M->dout), t(B), s(B);
-    NNM_HIP(c, hipMemcpy(z.data(), M->z, z.size() * 4, hipMemcpyDeviceToHost));
-    NNM_HIP(c, hipMemcpy(t.data(), M->t, B * 4, hipMemcpyDeviceToHost));
-    NNM_HIP(c, hipMemcpy(s.data(), M->s, B * 4, hipMemcpyDeviceToHost));
+    NNM_TRY_MLP(nnm_d2h(c, z.data(), M->z, z.size() * 4));
+    NNM_TRY_MLP(nnm_d2h(c, t.data(), M->t, B * 4));
+    NNM_TRY_MLP(nnm_d2h(c, s.data(), M->s, B * 4));
     std::vector<float> ratio;
     for (uint32_t i = 0; i < B; ++i) {
         if (!(t[i] < s[i]) || s[i] <= 0.f) continue;
@@ -499,8 +512,7 @@ extern "C" int nnm_mlp_train(nnm_ctx* c, uint32_t steps, uint32_t batch, float l
                            M->nparam, lr, b1, b2, c1, c2);
         NNM_LAUNCH_CHECK(c);
         if (it == 0 || it + 1 == steps) {
-            NNM_HIP(c, hipMemcpyAsync(lv.data(), M->loss, batch * 4, hipMemcpyDeviceToHost, c->stream));
-            NNM_HIP(c, hipStreamSynchronize(c->stream));
+            NNM_TRY_MLP(nnm_d2h(c, lv.data(), M->loss, batch * 4));
             double acc = 0;
             for (float x : lv) acc += x;
             (it == 0 ? lf : ll) = (float)(acc / batch);
@@ -564,7 +576,7 @@ extern "C" int nnm_mlp_project(nnm_ctx* c, float* out, double* secs) {
     NNM_TRY_MLP(nnm_mlp_ensure_projection(c, secs));
     if (out) {
         std::vector<uint16_t> h((size_t)c->n * M->dout);
-        NNM_HIP(c, hipMemcpy(h.data(), M->P, h.size() * 2, hipMemcpyDeviceToHost));
+        NNM_TRY_MLP(nnm_d2h(c, h.data(), M->P, h.size() * 2));
         for (size_t i = 0; i < h.size(); ++i) {
             uint32_t u = (uint32_t)h[i] << 16;
             float f;

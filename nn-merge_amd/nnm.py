@@ -35,7 +35,9 @@ class Stats(C.Structure):
                 ("init_dists", C.c_uint64), ("updates", C.c_uint64), ("reranked", C.c_uint64),
                 ("list_entries", C.c_uint64), ("offers", C.c_uint64), ("last_delta", C.c_double), ("secs", C.c_double),
                 ("join_secs", C.c_double), ("learned_iters", C.c_uint32), ("plain_iters", C.c_uint32),
-                ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32)]
+                ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32),
+                ("long_dists", C.c_uint64), ("long_entries", C.c_uint64), ("long_offers", C.c_uint64),
+                ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
