@@ -276,6 +276,75 @@ int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm_params* p,
                     uint32_t* level_groups, double* level_secs, uint64_t* level_dists, uint32_t max_levels,
                     uint32_t* nlevels);
 
+/* ------------------------------------------------------------ DPG layers */
+
+/* Build a DPG layer (LyNNDescent::dpg + DelaunayGraph,
+ * src/lynndescent.cpp:502-540, 640-710) from the context's k-NN lists over
+ * the active rows: M is the occlusion list bound (the reference's M_ =
+ * K0/2), maxM the reverse-edge bound (_M on the base layer, _M/2 above).
+ * carry = 1 starts rows present in the previous nnm_dpg_build from that
+ * layer's lists (the reference keeps LyNNDescent::graph across layers).
+ * The reverse pass takes every offer from the occlusion-pruned lists in
+ * increasing source order per target (order-free restatement of the
+ * reference's in-place pass).  st->dists counts the distances computed. */
+int nnm_dpg_build(nnm_ctx* ctx, uint32_t M, uint32_t maxM, int carry, nnm_stats* st);
+
+/* The last DPG layer as CSR: offs[rows + 1] (rows = active rows at build
+ * time), adj[offs[rows]] and dist[offs[rows]] (each list sorted by
+ * distance; adj/dist may be null).  The reference writes the same lists as
+ * text, "N" then "i deg ids..." (LyNNDescent::saveDPGraph,
+ * src/lynndescent.cpp:625-638). */
+int nnm_dpg_get(nnm_ctx* ctx, uint32_t* offs, uint32_t* adj, float* dist);
+
+/* ------------------------------------------------------------ search */
+
+#define NNM_SEARCH_LAYERED 0 /* LySearch::knnSearch: greedy through upper layers, best-first on the last */
+#define NNM_SEARCH_FLAT 1    /* LySearch::nndFlatSrch: best-first on the last layer from seeds */
+#define NNM_SEARCH_SRC_KNN 0
+#define NNM_SEARCH_SRC_DPG 1
+
+typedef struct nnm_search_params {
+    uint32_t mode;   /* NNM_SEARCH_LAYERED or NNM_SEARCH_FLAT */
+    uint32_t beam;   /* result-list size, the reference's topk (<= 256) */
+    uint32_t k;      /* results per query (<= min(beam, 64)) */
+    uint32_t width;  /* list entries expanded per step (1..4; 1 = the reference's order) */
+    uint32_t prec;   /* NNM_PREC_FP32, or NNM_PREC_BF16: bf16 rows during the search,
+                        the first `rerank` list entries recomputed in FP32 at the end */
+    uint32_t rerank; /* 0 = min(beam, 64) */
+    uint32_t entry;  /* layered mode: entry row of layer 0 (the reference draws
+                        rand() % rows, src/lysearch.cpp:378) */
+    uint32_t pad_;
+    uint64_t seed;   /* flat mode without explicit seeds: seeds drawn from it */
+} nnm_search_params;
+
+typedef struct nnm_search_stats {
+    uint64_t dists;       /* distances computed (the reference's nCmps, src/lysearch.cpp:281) */
+    uint64_t reranked;    /* FP32 re-rank distances (bf16 mode) */
+    uint64_t hash_resets; /* visited-hash refills (revisited rows are recomputed) */
+    uint64_t capped;      /* queries stopped by the step cap (an error) */
+    uint64_t steps;       /* list entries expanded, all queries */
+    double secs;          /* search kernel time */
+} nnm_search_stats;
+
+/* Search layers as CSR (the reference's augKnnGraph + knnLocs,
+ * src/lysearch.cpp:112-186): layer g covers rows [0, rows[g]), offs[g] has
+ * rows[g] + 1 entries, adj[g] the neighbour ids.  Layers grow (rows[g] <=
+ * rows[g + 1]); the last one is the base layer. */
+int nnm_search_set_layers(nnm_ctx* ctx, uint32_t nlayers, const uint32_t* rows, const uint32_t* const* offs,
+                          const uint32_t* const* adj);
+
+/* The context's own graph as the single search layer: NNM_SEARCH_SRC_DPG
+ * (the last nnm_dpg_build) or NNM_SEARCH_SRC_KNN (the k-NN lists). */
+int nnm_search_use_graph(nnm_ctx* ctx, int source);
+
+/* Batched search of nq queries (nq x d float32, host): LySearch::knnSearch
+ * (src/lysearch.cpp:363-437) + nndHNSWSrch (src/lysearch.cpp:236-322), or
+ * nndFlatSrch (src/lysearch.cpp:439-544) in flat mode with seeds (nq x
+ * beam, may be null).  Writes the first k results per query sorted by
+ * (dist, id) into ids / dist (nq x k); missing results are 0xFFFFFFFF. */
+int nnm_search(nnm_ctx* ctx, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* seeds,
+               uint32_t* ids, float* dist, nnm_search_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

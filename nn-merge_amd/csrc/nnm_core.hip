@@ -188,6 +188,8 @@ static int fill_u64(nnm_ctx* c, uint64_t* p, uint64_t n, uint64_t v) {
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
+int nnm_fill_u64(nnm_ctx* c, uint64_t* p, uint64_t n, uint64_t v) { return fill_u64(c, p, n, v); }
+int nnm_fill_u32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) { return fill_u32(c, p, n, v); }
 
 // ================================================================ context
 
@@ -236,6 +238,8 @@ extern "C" int nnm_destroy(nnm_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     nnm_mlp_free(c);
+    nnm_search_free(c);
+    nnm_dpg_free(c);
     free_stage(c);
     for (auto& a : c->allocs) hipFree(a.first);
     c->allocs.clear();
@@ -331,6 +335,8 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
     free_iter(c);
     free_partition(c);
     nnm_mlp_invalidate_projection(c);
+    nnm_search_free(c);
+    nnm_dpg_free(c);
     nnm_free(c, c->X); c->X = nullptr;
     nnm_free(c, c->Xb); c->Xb = nullptr;
     nnm_free(c, c->norms); c->norms = nullptr;
@@ -1217,6 +1223,18 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
     if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE)) return nnm_fail(c, NNM_ERR_INVALID, "unknown flags");
+    return NNM_OK;
+}
+
+int nnm_scan_u32(nnm_ctx* c, const uint32_t* in, uint64_t m, uint32_t* out, uint32_t* tmp) {
+    const uint32_t ntiles = (uint32_t)((m + SCAN_TILE - 1) / SCAN_TILE);
+    if (ntiles == 0) return nnm_fill_u32(c, out, 1, 0);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, c->stream, in, m, tmp);
+    NNM_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, tmp, ntiles, out + m);
+    NNM_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, c->stream, in, m, tmp, out);
+    NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
 

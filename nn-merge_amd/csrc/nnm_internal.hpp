@@ -111,6 +111,8 @@ struct nnm_ctx {
     uint8_t* grp_active = nullptr;  // ngrp
 
     void* mlp = nullptr;            // learned projection state (nnm_mlp.hip)
+    void* search = nullptr;         // search layers (nnm_search.hip)
+    void* dpg = nullptr;            // DPG layer lists (nnm_dpg.hip)
 
     // join launch configuration (autotuned): window rows of the short-list
     // kernels and rows taken from the work queue at a time
@@ -167,6 +169,14 @@ uint32_t nnm_join_default_chunk();
 
 // Learned projection (nnm_mlp.hip)
 void nnm_mlp_free(nnm_ctx* c);
+void nnm_search_free(nnm_ctx* c);
+int nnm_search_adopt_layer(nnm_ctx* c, uint32_t rows, uint32_t* offs, uint32_t* adj);
+void nnm_dpg_free(nnm_ctx* c);
+int nnm_fill_u64(nnm_ctx* c, uint64_t* p, uint64_t n, uint64_t v);
+int nnm_fill_u32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v);
+// exclusive scan of m u32 counts into out[0..m] (out[m] = total); tmp holds
+// (m + 2047) / 2048 + 1 words
+int nnm_scan_u32(nnm_ctx* c, const uint32_t* in, uint64_t m, uint32_t* out, uint32_t* tmp);
 int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs);
 void nnm_mlp_invalidate_projection(nnm_ctx* c);
 const uint16_t* nnm_mlp_rows(nnm_ctx* c);

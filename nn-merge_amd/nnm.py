@@ -21,6 +21,8 @@ PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
 RULE_ALL, RULE_MERGE = 0, 1
 RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
 FLAG_GRAPH, FLAG_AUTOTUNE = 1, 2
+SEARCH_LAYERED, SEARCH_FLAT = 0, 1
+SEARCH_SRC_KNN, SEARCH_SRC_DPG = 0, 1
 
 
 class Params(C.Structure):
@@ -38,6 +40,20 @@ class Stats(C.Structure):
                 ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32),
                 ("long_dists", C.c_uint64), ("long_entries", C.c_uint64), ("long_offers", C.c_uint64),
                 ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SearchParams(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("beam", C.c_uint32), ("k", C.c_uint32), ("width", C.c_uint32),
+                ("prec", C.c_uint32), ("rerank", C.c_uint32), ("entry", C.c_uint32), ("pad_", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [("dists", C.c_uint64), ("reranked", C.c_uint64), ("hash_resets", C.c_uint64),
+                ("capped", C.c_uint64), ("steps", C.c_uint64), ("secs", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -85,6 +101,12 @@ _SIGS = {
                                   C.POINTER(Stats), _vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_uint32)]),
     "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
                                  C.POINTER(Stats), C.POINTER(C.c_double)]),
+    "nnm_dpg_build": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Stats)]),
+    "nnm_dpg_get": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "nnm_search_set_layers": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "nnm_search_use_graph": (C.c_int, [_vp, C.c_int]),
+    "nnm_search": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(SearchParams), _vp, _vp, _vp,
+                             C.POINTER(SearchStats)]),
 }
 
 
@@ -167,6 +189,7 @@ class Context:
         X = np.ascontiguousarray(X, dtype=np.float32)
         self._chk(lib().nnm_set_data(self.h, _ptr(X), X.shape[0], X.shape[1]))
         self.n, self.d = X.shape
+        self.n_act = self.n
         self.K = 0
 
     def set_partition(self, sub_begin, sub_built, sub_group=None):
@@ -174,6 +197,7 @@ class Context:
         bu = np.ascontiguousarray(sub_built, dtype=np.uint8)
         gr = np.ascontiguousarray(np.zeros(bu.size) if sub_group is None else sub_group, dtype=np.uint32)
         self._chk(lib().nnm_set_partition(self.h, _ptr(sb), _ptr(bu), _ptr(gr), bu.size))
+        self.n_act = int(sb[-1])
 
     def random_init(self, K, seed=1):
         st = Stats()
@@ -283,6 +307,45 @@ class Context:
         secs = C.c_double()
         self._chk(lib().nnm_mlp_project(self.h, _ptr(out), C.byref(secs)))
         return out, secs.value
+
+    # -------------------------------------------------------- DPG + search
+    def dpg_build(self, M, maxM, carry=False):
+        st = Stats()
+        self._chk(lib().nnm_dpg_build(self.h, M, maxM, 1 if carry else 0, C.byref(st)))
+        return st.as_dict()
+
+    def dpg_get(self, rows=None):
+        rows = self.n_act if rows is None else rows
+        offs = np.empty(rows + 1, np.uint32)
+        self._chk(lib().nnm_dpg_get(self.h, _ptr(offs), None, None))
+        adj = np.empty(int(offs[-1]), np.uint32)
+        dist = np.empty(int(offs[-1]), np.float32)
+        self._chk(lib().nnm_dpg_get(self.h, _ptr(offs), _ptr(adj), _ptr(dist)))
+        return offs, adj, dist
+
+    def search_set_layers(self, layers):
+        """layers: list of (offs, adj) CSR pairs, upper layers first."""
+        offs = [np.ascontiguousarray(o, dtype=np.uint32) for o, _ in layers]
+        adj = [np.ascontiguousarray(a, dtype=np.uint32) for _, a in layers]
+        rows = np.array([o.size - 1 for o in offs], np.uint32)
+        po = (_vp * len(offs))(*[o.ctypes.data for o in offs])
+        pa = (_vp * len(adj))(*[a.ctypes.data if a.size else offs[0].ctypes.data for a in adj])
+        self._chk(lib().nnm_search_set_layers(self.h, len(offs), _ptr(rows), po, pa))
+
+    def search_use_graph(self, source=SEARCH_SRC_DPG):
+        self._chk(lib().nnm_search_use_graph(self.h, source))
+
+    def search(self, Q, beam, k=10, width=1, prec=PREC_FP32, mode=SEARCH_LAYERED, entry=0, seeds=None, seed=1,
+               rerank=0):
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        nq = Q.shape[0]
+        p = SearchParams(mode=mode, beam=beam, k=k, width=width, prec=prec, rerank=rerank, entry=entry, seed=seed)
+        sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint32)
+        ids = np.empty((nq, k), np.uint32)
+        dist = np.empty((nq, k), np.float32)
+        st = SearchStats()
+        self._chk(lib().nnm_search(self.h, _ptr(Q), nq, C.byref(p), _ptr(sd), _ptr(ids), _ptr(dist), C.byref(st)))
+        return ids, dist, st.as_dict()
 
     def pair_dists(self, a, b):
         a = np.ascontiguousarray(a, dtype=np.uint32)

@@ -7,6 +7,8 @@ Every generator is deterministic in its seed (numpy PCG64).  The workloads:
   [0, 255]^d; x = round(clip(c + N(0, sigma^2), 0, 255)).  Every value is an
   integer <= 255, so squared L2 is exact in float32 for d <= 256 and the
   rows are exactly representable in bfloat16.
+* ``sift_like(..., nq=m)``: also m queries from the same mixture (same
+  centres and factors, independent draws).
 * ``sift_like(..., rank=r)``: the same mixture with SIFT-like low intrinsic
   dimension: x = round(clip(c + A_c z + e, 0, 255)) with z ~ N(0, I_r), a
   random d x r factor A_c per centre scaled so the total per-coordinate
@@ -22,13 +24,27 @@ import numpy as np
 SEED_BASE = 20261203
 
 
-def sift_like(n, d=128, centers=64, sigma=20.0, seed=SEED_BASE + 1, chunk=1 << 20, rank=None, noise=2.0):
+def sift_like(n, d=128, centers=64, sigma=20.0, seed=SEED_BASE + 1, chunk=1 << 20, rank=None, noise=2.0, nq=0):
+    """n rows of the mixture; with nq > 0 also nq query rows drawn from the
+    same mixture (same centres and factors) by an independent stream, and
+    (X, Q) is returned."""
     rng = np.random.default_rng(seed)
     C = rng.uniform(0.0, 255.0, size=(centers, d)).astype(np.float32)
     lab = rng.integers(0, centers, size=n)
+    A = None
     if rank is not None:
         scale = np.sqrt(max(sigma * sigma - noise * noise, 0.0) / rank)
         A = (rng.standard_normal(size=(centers, d, rank)) * scale).astype(np.float32)
+    X = _mixture_rows(rng, C, A, lab, d, sigma, noise, rank, chunk)
+    if nq:
+        qrng = np.random.default_rng([seed, 1])
+        qlab = qrng.integers(0, centers, size=nq)
+        return X, _mixture_rows(qrng, C, A, qlab, d, sigma, noise, rank, chunk)
+    return X
+
+
+def _mixture_rows(rng, C, A, lab, d, sigma, noise, rank, chunk):
+    n, centers = lab.size, C.shape[0]
     X = np.empty((n, d), np.float32)
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
@@ -53,18 +69,27 @@ def sift_like(n, d=128, centers=64, sigma=20.0, seed=SEED_BASE + 1, chunk=1 << 2
     return X
 
 
-def unit_mixture(n, d=96, centers=512, sigma=0.35, seed=SEED_BASE + 3, chunk=1 << 20):
+def unit_mixture(n, d=96, centers=512, sigma=0.35, seed=SEED_BASE + 3, chunk=1 << 20, nq=0):
+    """With nq > 0, also nq queries from the same mixture; returns (X, Q)."""
     rng = np.random.default_rng(seed)
     C = rng.standard_normal(size=(centers, d)).astype(np.float32)
     lab = rng.integers(0, centers, size=n)
-    X = np.empty((n, d), np.float32)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        blk = rng.standard_normal(size=(e - s, d), dtype=np.float32)
-        blk *= np.float32(sigma)
-        blk += C[lab[s:e]]
-        blk /= np.linalg.norm(blk, axis=1, keepdims=True)
-        X[s:e] = blk
+
+    def rows(r, lb):
+        out = np.empty((lb.size, d), np.float32)
+        for s in range(0, lb.size, chunk):
+            e = min(lb.size, s + chunk)
+            blk = r.standard_normal(size=(e - s, d), dtype=np.float32)
+            blk *= np.float32(sigma)
+            blk += C[lb[s:e]]
+            blk /= np.linalg.norm(blk, axis=1, keepdims=True)
+            out[s:e] = blk
+        return out
+
+    X = rows(rng, lab)
+    if nq:
+        qrng = np.random.default_rng([seed, 1])
+        return X, rows(qrng, qrng.integers(0, centers, size=nq))
     return X
 
 

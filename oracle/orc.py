@@ -67,6 +67,18 @@ def lib():
         L.orc_merge.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u8p,
                                 u32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float,
                                 C.c_uint32, C.c_int, u32p, f32p, C.POINTER(OrcStats)]
+        vp, u64p = C.c_void_p, C.POINTER(C.c_uint64)
+        L.orc_search_layered.restype = None
+        L.orc_search_layered.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(vp), C.POINTER(vp),
+                                         f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, f32p, u64p]
+        L.orc_search_flat.restype = None
+        L.orc_search_flat.argtypes = [f32p, C.c_uint32, C.c_uint32, u64p, u32p, f32p, C.c_uint32, u32p,
+                                      C.c_uint32, C.c_uint32, u32p, f32p, u64p]
+        for nm in ("orc_dpg_ref", "orc_dpg_par"):
+            f = getattr(L, nm)
+            f.restype = None
+            f.argtypes = [f32p, C.c_uint32, C.c_uint32, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                          vp, vp, vp, u64p, u32p, f32p]
         L.orc_recall.restype = C.c_double
         L.orc_recall.argtypes = [u32p, C.c_uint32, u32p, C.c_uint64, u32p, C.c_uint32,
                                  C.c_uint32, C.c_int]
@@ -177,3 +189,73 @@ def recall(ids, rows, gt, k=10, unique=True):
     gt = np.ascontiguousarray(gt, np.uint32)
     return lib().orc_recall(_p(ids, C.c_uint32), ids.shape[1], _p(rows, C.c_uint32), rows.size,
                             _p(gt, C.c_uint32), gt.shape[1], k, 1 if unique else 0)
+
+
+def _csr64(offs):
+    return np.ascontiguousarray(offs, np.uint64)
+
+
+def search_layered(X, layers, Q, entry, beam, k=10):
+    """LySearch::knnSearch restated: layers = [(offs, adj), ...], upper first.
+    Returns ids, dist (nq x k) and the number of distances computed."""
+    X = _f32(X)
+    Q = _f32(Q)
+    n, d = X.shape
+    offs = [_csr64(o) for o, _ in layers]
+    adj = [np.ascontiguousarray(a, np.uint32) if len(a) else np.zeros(1, np.uint32) for _, a in layers]
+    po = (C.c_void_p * len(offs))(*[o.ctypes.data for o in offs])
+    pa = (C.c_void_p * len(adj))(*[a.ctypes.data for a in adj])
+    nq = Q.shape[0]
+    ids = np.empty((nq, k), np.uint32)
+    dist = np.empty((nq, k), np.float32)
+    nc = C.c_uint64()
+    lib().orc_search_layered(_p(X, C.c_float), n, d, len(layers), po, pa, _p(Q, C.c_float), nq, entry, beam, k,
+                             _p(ids, C.c_uint32), _p(dist, C.c_float), C.byref(nc))
+    return ids, dist, nc.value
+
+
+def search_flat(X, offs, adj, Q, seeds, beam, k=10):
+    """LySearch::nndFlatSrch restated, seeds: nq x beam."""
+    X = _f32(X)
+    Q = _f32(Q)
+    n, d = X.shape
+    o = _csr64(offs)
+    a = np.ascontiguousarray(adj, np.uint32)
+    sd = np.ascontiguousarray(seeds, np.uint32)
+    nq = Q.shape[0]
+    ids = np.empty((nq, k), np.uint32)
+    dist = np.empty((nq, k), np.float32)
+    nc = C.c_uint64()
+    lib().orc_search_flat(_p(X, C.c_float), n, d, _p(o, C.c_uint64), _p(a, C.c_uint32), _p(Q, C.c_float), nq,
+                          _p(sd, C.c_uint32), beam, k, _p(ids, C.c_uint32), _p(dist, C.c_float), C.byref(nc))
+    return ids, dist, nc.value
+
+
+def dpg(X, knn, M, maxM, prev=None, variant="par"):
+    """DelaunayGraph restated on the k-NN lists knn (n x K).  variant "ref":
+    the reference's in-place pass; "par": the order-free pass the GPU runs.
+    prev: (offs, adj, dist) of the previous layer (carried lists) or None.
+    Returns CSR offs (uint32), adj, dist."""
+    X = _f32(X)
+    n, d = X.shape
+    knn = np.ascontiguousarray(knn, np.uint32)
+    K = knn.shape[1]
+    cap = max(M, maxM, 0 if prev is None else int(np.diff(prev[0]).max(initial=0)))
+    offs = np.empty(n + 1, np.uint64)
+    adj = np.empty(n * cap + 1, np.uint32)
+    dist = np.empty(n * cap + 1, np.float32)
+    keep = []
+    if prev is not None:
+        po = _csr64(prev[0])
+        pa = np.ascontiguousarray(prev[1], np.uint32)
+        pd = np.ascontiguousarray(prev[2], np.float32)
+        keep = [po, pa, pd]
+        pargs = (po.size - 1, po.ctypes.data, pa.ctypes.data if pa.size else None, pd.ctypes.data if pd.size else None)
+    else:
+        pargs = (0, None, None, None)
+    f = lib().orc_dpg_ref if variant == "ref" else lib().orc_dpg_par
+    f(_p(X, C.c_float), n, d, _p(knn, C.c_uint32), K, M, maxM, *pargs, _p(offs, C.c_uint64),
+      _p(adj, C.c_uint32), _p(dist, C.c_float))
+    del keep
+    m = int(offs[-1])
+    return offs.astype(np.uint32), adj[:m].copy(), dist[:m].copy()
