@@ -121,7 +121,19 @@ typedef struct nnm_stats {
                                  form); 8 per offer; 8 per vertex */
     uint64_t join_main_flops; /* its distance flops: 3d per FP32 distance
                                  (difference form), 2d per MFMA distance */
+    uint64_t near_ties;    /* k-th slot decisions re-made on FP64 distances
+                              (FP32 distances within rounding of each other) */
+    uint64_t tie_swaps;    /* ... where FP64 kept the other entry */
 } nnm_stats;
+
+/* FP64 error of the stored FP32 list distances (nnm_graph_dist_error). */
+typedef struct nnm_dist_error {
+    double max_abs;       /* max |d32 - d64| */
+    double max_rel;       /* max |d32 - d64| / d64 */
+    double mean_rel;      /* mean relative error over all entries */
+    uint64_t entries;     /* list entries checked (active rows x K) */
+    uint64_t order_flips; /* adjacent entries whose FP64 order disagrees */
+} nnm_dist_error;
 
 void nnm_default_params(nnm_params* p);
 
@@ -275,6 +287,12 @@ int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm_params* p,
                     const uint32_t* leaf_begin, uint32_t* ids, float* dist, nnm_stats* st, uint32_t* level_rows,
                     uint32_t* level_groups, double* level_secs, uint64_t* level_dists, uint32_t max_levels,
                     uint32_t* nlevels);
+
+/* Per-result FP64 error of the graph's FP32 distances: every list entry of
+ * the active rows is recomputed in FP64 on the GPU.  No reference
+ * counterpart (the reference stores FP32 distances only,
+ * src/pvetex.h:18-30); the chosen precision reports it. */
+int nnm_graph_dist_error(nnm_ctx* ctx, nnm_dist_error* out);
 
 /* ------------------------------------------------------------ DPG layers */
 

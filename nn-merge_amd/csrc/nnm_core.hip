@@ -1092,12 +1092,25 @@ __global__ void k_scan_apply(const uint32_t* __restrict__ in, uint64_t m, const 
 // UpdateKnnList's rule applied to the whole batch of the iteration.  Slots
 // are reset to empty.  With rerank, candidates are recomputed in exact FP32
 // before ranking.
+// FP64 squared L2 across the wave: lanes take elements lane, lane + 64, ...;
+// the partial sums are combined in a fixed tree (same result every call).
+__device__ inline double wave_l2_f64(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
+    double acc = 0.0;
+    for (uint32_t i = nnm_lane(); i < d; i += NNM_WAVE) {
+        const double t = (double)a[i] - (double)b[i];
+        acc = __builtin_fma(t, t, acc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, NNM_WAVE);
+    return acc;
+}
+
 __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X, uint32_t ld,
-                         int rerank, float rerank_tau, uint32_t* __restrict__ g_ids,
+                         int rerank, float rerank_tau, float tie_win, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
                          uint64_t* __restrict__ counters) {
     const uint32_t lane = nnm_lane();
-    uint64_t c_upd = 0, c_flags = 0, c_rr = 0;  // wave-uniform
+    uint64_t c_upd = 0, c_flags = 0, c_rr = 0, c_ties = 0, c_swaps = 0;  // wave-uniform
     NNM_WAVE_LOOP(v, n) {
         uint64_t pkey = NNM_EMPTY_KEY;
         uint32_t pid = 0xFFFFFFFFu, pf = 0;
@@ -1132,7 +1145,10 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
                 }
                 rr = nnm_wave_sum32(rr);
             }
-            valid = valid && (skey < kth_key);
+            // candidates within the FP32 error window of the k-th entry stay in
+            // play for the FP64 tie test below
+            const float kd_win = nnm_key_dist(kth_key) * tie_win;
+            valid = valid && (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd_win));
             const uint64_t vm = __ballot(valid);
             uint32_t pos_p = lane, pos_s = 0;
             // rank against the other valid candidates
@@ -1148,6 +1164,35 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
             for (uint32_t p = 0; p < K; ++p) {
                 const uint64_t q = __shfl(pkey, (int)p, NNM_WAVE);
                 pos_s += (q < skey) ? 1u : 0u;
+            }
+            if (tie_win > 1.f) {
+                // the entries ranked K-1 (last kept) and K (first dropped): when
+                // their FP32 distances are within rounding of each other the
+                // choice is re-made on FP64 distances (ties by id)
+                const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(valid && pos_s == K - 1);
+                const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(valid && pos_s == K);
+                if ((ap | as) && (bp | bs)) {
+                    const bool a_pool = ap != 0, b_pool = bp != 0;
+                    const int la = __ffsll((long long)(a_pool ? ap : as)) - 1;
+                    const int lb = __ffsll((long long)(b_pool ? bp : bs)) - 1;
+                    const uint64_t ka = __shfl(a_pool ? pkey : skey, la, NNM_WAVE);
+                    const uint64_t kb = __shfl(b_pool ? pkey : skey, lb, NNM_WAVE);
+                    if (nnm_key_dist(kb) <= nnm_key_dist(ka) * tie_win) {
+                        ++c_ties;
+                        const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
+                        const double ea = wave_l2_f64(X + v * ld, X + (uint64_t)ia * ld, ld);
+                        const double eb = wave_l2_f64(X + v * ld, X + (uint64_t)ib * ld, ld);
+                        if (eb < ea || (eb == ea && ib < ia)) {
+                            ++c_swaps;
+                            if ((int)lane == la) {
+                                if (a_pool) pos_p = K; else pos_s = K;
+                            }
+                            if ((int)lane == lb) {
+                                if (b_pool) pos_p = K - 1; else pos_s = K - 1;
+                            }
+                        }
+                    }
+                }
             }
             const bool ins = valid && pos_s < K;
             const bool keep = lane < K && pos_p < K;
@@ -1174,7 +1219,71 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
         if (c_upd) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)c_upd);
         if (c_flags) atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)c_flags);
         if (c_rr) atomicAdd((unsigned long long*)&counters[CNT_RERANKED], (unsigned long long)c_rr);
+        if (c_ties) atomicAdd((unsigned long long*)&counters[CNT_NEAR_TIES], (unsigned long long)c_ties);
+        if (c_swaps) atomicAdd((unsigned long long*)&counters[CNT_TIE_SWAPS], (unsigned long long)c_swaps);
     }
+}
+
+float nnm_near_tie_window(nnm_ctx* c, const nnm_params* p) {
+    (void)p;
+    if (c->data_exact_bf16) return 1.f;  // integer rows: every distance is exact
+    return 1.f + 2.f * (float)(c->d + 2) * 5.9604645e-8f;
+}
+
+// FP64 error of the stored FP32 list distances (per entry |d32 - d64|, and
+// adjacent entries whose FP64 order disagrees with the stored order).
+__global__ void k_dist_error(uint64_t n, uint32_t K, const float* __restrict__ X, uint32_t ld,
+                             const uint32_t* __restrict__ g_ids, const float* __restrict__ g_dist,
+                             unsigned long long* __restrict__ out) {
+    const uint32_t lane = nnm_lane();
+    double max_abs = 0.0, max_rel = 0.0, sum_rel = 0.0;
+    uint64_t flips = 0;
+    NNM_WAVE_LOOP(v, n) {
+        double prev = -1.0;
+        for (uint32_t j = 0; j < K; ++j) {
+            const uint32_t id = g_ids[v * K + j];
+            const double e = wave_l2_f64(X + v * ld, X + (uint64_t)id * ld, ld);
+            const double err = fabs((double)g_dist[v * K + j] - e);
+            max_abs = err > max_abs ? err : max_abs;
+            const double rel = e > 0.0 ? err / e : (err > 0.0 ? 1.0 : 0.0);
+            max_rel = rel > max_rel ? rel : max_rel;
+            sum_rel += rel;
+            if (e < prev) ++flips;
+            prev = e;
+        }
+    }
+    if (lane == 0) {
+        // non-negative doubles order like their bit patterns
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(max_abs));
+        atomicMax(&out[1], (unsigned long long)__double_as_longlong(max_rel));
+        atomicAdd(&out[2], (unsigned long long)(sum_rel * 1e12));
+        atomicAdd(&out[3], (unsigned long long)flips);
+    }
+}
+
+extern "C" int nnm_graph_dist_error(nnm_ctx* c, nnm_dist_error* e) {
+    if (!c || !e) return NNM_ERR_INVALID;
+    if (!c->X || !c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no data or graph");
+    unsigned long long* acc = nullptr;
+    NNM_TRY(nnm_alloc(c, (void**)&acc, 4 * 8));
+    int rc = fill_u64(c, reinterpret_cast<uint64_t*>(acc), 4, 0);
+    if (rc == NNM_OK) {
+        hipLaunchKernelGGL(k_dist_error, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
+                           c->ld, c->g_ids, c->g_dist, acc);
+        hipError_t he = hipGetLastError();
+        if (he != hipSuccess) rc = nnm_hip_fail(c, he, "k_dist_error launch");
+    }
+    uint64_t h[4] = {0, 0, 0, 0};
+    if (rc == NNM_OK) rc = nnm_d2h(c, h, acc, sizeof(h));
+    nnm_free(c, acc);
+    if (rc != NNM_OK) return rc;
+    const double ent = (double)c->n_act * c->K;
+    std::memcpy(&e->max_abs, &h[0], 8);
+    std::memcpy(&e->max_rel, &h[1], 8);
+    e->mean_rel = ent > 0 ? (double)h[2] * 1e-12 / ent : 0.0;
+    e->entries = (uint64_t)ent;
+    e->order_flips = h[3];
+    return NNM_OK;
 }
 
 // ================================================================ NN-Descent
@@ -1282,7 +1391,8 @@ static int update_lists(nnm_ctx* c, const nnm_params* p) {
     }
     if (p->rerank == NNM_RERANK_NEVER) rerank = 0;
     hipLaunchKernelGGL(k_update, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
-                       c->slots_C, c->X, c->ld, rerank, tau, c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
+                       c->slots_C, c->X, c->ld, rerank, tau, nnm_near_tie_window(c, p), c->g_ids, c->g_dist, c->g_flag,
+                       c->slots, c->counters);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
@@ -1501,6 +1611,8 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->long_offers = h[CNT_LONG_OFFERS];
         st->join_main_bytes = main_bytes;
         st->join_main_flops = main_flops;
+        st->near_ties = h[CNT_NEAR_TIES];
+        st->tie_swaps = h[CNT_TIE_SWAPS];
     }
     return NNM_OK;
 }

@@ -42,6 +42,8 @@ enum CounterIdx {
     CNT_LONG_DISTS = 11,    // share of DISTS / LIST_ENTRIES / OFFERS that the
     CNT_LONG_ENTRIES = 12,  // long-list join kernels account for (the rest is
     CNT_LONG_OFFERS = 13,   // the short-list kernel's)
+    CNT_NEAR_TIES = 14,     // k-th slot decisions re-made in FP64
+    CNT_TIE_SWAPS = 15,     // ... that changed which entry stays
     CNT_N = 16
 };
 
@@ -140,6 +142,11 @@ int nnm_h2d(nnm_ctx* c, void* dst, const void* src, size_t bytes);
 int nnm_d2h(nnm_ctx* c, void* dst, const void* src, size_t bytes);
 int nnm_h2d_rows(nnm_ctx* c, float* dst, uint32_t ld, const float* src, uint32_t d, uint64_t n);
 int nnm_ensure_bf16(nnm_ctx* c);
+// Offer window above the k-th distance: 1 + 2 * (d + 2) * 2^-24 (twice the
+// relative error bound of an FP32 squared distance of d terms), so that
+// candidates within rounding of the k-th entry reach the FP64 tie test; 1
+// where distances are exact (integer rows) and for learned estimates.
+float nnm_near_tie_window(nnm_ctx* c, const nnm_params* p);
 int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
 
 #define NNM_HIP(c, expr)                                  \

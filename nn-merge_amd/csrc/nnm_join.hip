@@ -52,6 +52,7 @@ struct JoinArgs {
     uint32_t rule;
     uint32_t salt;    // set inside each kernel from seed and the iteration counter
     uint64_t seed;
+    float kwin;       // offer window above the k-th distance (1 + near-tie width)
     uint32_t js;      // short-list window rows (<= the kernel's compiled window)
     uint32_t chunk;   // rows per work-queue grab
     const uint32_t* l_new;
@@ -164,13 +165,13 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
                 const uint32_t id = idx < nn ? Ln[idx] : Lo[idx - nn];
                 L.metaB[0][lane] = id;
                 L.metaB[1][lane] = a.label[id];
-                L.metaB[2][lane] = __float_as_uint(a.kth[id]);
+                L.metaB[2][lane] = __float_as_uint(a.kth[id] * a.kwin);
             }
             if (!diag && lane < na) {
                 const uint32_t id = Ln[a0 + lane];
                 L.metaA[0][lane] = id;
                 L.metaA[1][lane] = a.label[id];
-                L.metaA[2][lane] = __float_as_uint(a.kth[id]);
+                L.metaA[2][lane] = __float_as_uint(a.kth[id] * a.kwin);
             }
             const float* rA = diag ? rowsB : rowsA;
             const uint32_t(*mA)[JW] = diag ? L.metaB : L.metaA;
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
                     L.meta[0][lane] = wid;
                     L.meta[1][lane] = a.label[wid];
-                    L.meta[2][lane] = __float_as_uint(a.kth[wid]);
+                    L.meta[2][lane] = __float_as_uint(a.kth[wid] * a.kwin);
                 }
             }
             // rows -> LDS, one row per direct-to-LDS instruction
@@ -553,7 +554,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
                     L.meta[0][lane] = wid;
                     L.meta[1][lane] = a.label[wid];
-                    L.meta[2][lane] = __float_as_uint(a.kth[wid]);
+                    L.meta[2][lane] = __float_as_uint(a.kth[wid] * a.kwin);
                     L.meta[3][lane] = __float_as_uint(norms[wid]);
                 }
             }
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
             const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
             L.meta[0][j] = id;
             L.meta[1][j] = a.label[id];
-            L.meta[2][j] = __float_as_uint(a.kth[id]);
+            L.meta[2][j] = __float_as_uint(a.kth[id] * a.kwin);
             L.meta[3][j] = __float_as_uint(norms[id]);
         }
         __syncthreads();
@@ -856,6 +857,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
     a.salt = 0;
     a.seed = p->seed;
     a.chunk = chunk ? chunk : JOIN_CHUNK;
+    a.kwin = nnm_near_tie_window(c, p);
     a.l_new = c->l_new;
     a.cap_new = c->cap_new;
     a.l_old = c->l_old;

@@ -39,7 +39,8 @@ class Stats(C.Structure):
                 ("join_secs", C.c_double), ("learned_iters", C.c_uint32), ("plain_iters", C.c_uint32),
                 ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32),
                 ("long_dists", C.c_uint64), ("long_entries", C.c_uint64), ("long_offers", C.c_uint64),
-                ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64)]
+                ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64),
+                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -54,6 +55,14 @@ class SearchParams(C.Structure):
 class SearchStats(C.Structure):
     _fields_ = [("dists", C.c_uint64), ("reranked", C.c_uint64), ("hash_resets", C.c_uint64),
                 ("capped", C.c_uint64), ("steps", C.c_uint64), ("secs", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class DistError(C.Structure):
+    _fields_ = [("max_abs", C.c_double), ("max_rel", C.c_double), ("mean_rel", C.c_double),
+                ("entries", C.c_uint64), ("order_flips", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -101,6 +110,7 @@ _SIGS = {
                                   C.POINTER(Stats), _vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_uint32)]),
     "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
                                  C.POINTER(Stats), C.POINTER(C.c_double)]),
+    "nnm_graph_dist_error": (C.c_int, [_vp, C.POINTER(DistError)]),
     "nnm_dpg_build": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Stats)]),
     "nnm_dpg_get": (C.c_int, [_vp, _vp, _vp, _vp]),
     "nnm_search_set_layers": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
@@ -307,6 +317,11 @@ class Context:
         secs = C.c_double()
         self._chk(lib().nnm_mlp_project(self.h, _ptr(out), C.byref(secs)))
         return out, secs.value
+
+    def dist_error(self):
+        e = DistError()
+        self._chk(lib().nnm_graph_dist_error(self.h, C.byref(e)))
+        return e.as_dict()
 
     # -------------------------------------------------------- DPG + search
     def dpg_build(self, M, maxM, carry=False):

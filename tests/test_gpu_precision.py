@@ -82,3 +82,26 @@ def test_bf16_real_rows_rerank_exact(ctx):
     r32, r16 = nnm.recall(i32, gt, rows), nnm.recall(i16, gt, rows)
     print("fp32", r32, s32, "bf16", r16, s16)
     assert r16 >= r32 - 0.005
+
+
+def test_fp64_error_report_and_near_ties(ctx):
+    """nnm_graph_dist_error against an FP64 numpy recomputation; integer rows
+    have no error and no near-tie decisions, real rows stay within the FP32
+    accumulation bound (d + 2) * 2^-24 and resolve near-ties in FP64."""
+    Xi = nnm_data.sift_like(20000, 128, centers=64, seed=61)
+    _, _, _, st = _run(ctx, Xi, nnm.PREC_FP32)
+    e = ctx.dist_error()
+    assert e["entries"] == 20000 * 20
+    assert e["max_abs"] == 0.0 and e["order_flips"] == 0
+    assert st["near_ties"] == 0
+    X = nnm_data.unit_mixture(20000, 96, centers=256, seed=62)
+    for prec in (nnm.PREC_FP32, nnm.PREC_BF16):
+        ids, dist, _, st = _run(ctx, X, prec)
+        e = ctx.dist_error()
+        rows = np.arange(0, 20000, 37)
+        d64 = ((X[ids[rows]].astype(np.float64) - X[rows][:, None, :].astype(np.float64)) ** 2).sum(-1)
+        rel = np.abs(dist[rows].astype(np.float64) - d64) / d64
+        print("prec", prec, e, "sample max rel", rel.max(), "ties", st["near_ties"], st["tie_swaps"])
+        assert rel.max() <= e["max_rel"] * (1 + 1e-9)
+        assert e["max_rel"] <= (96 + 2) * 2.0 ** -24
+        assert st["tie_swaps"] <= st["near_ties"]
