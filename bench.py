@@ -173,6 +173,32 @@ def summarize(dirs, picks):
     print("\n".join(out))
 
 
+def run_search(nnm, ctx, Q, args):
+    """Batched search on the merged graph (LySearch on a DPG layer): the DPG
+    of the final k-NN lists (M = K/2, maxM = K, LyNNDescent::dpg on the base
+    layer), then the seeded flat search of every query at each beam, bf16 rows
+    with FP32 re-rank.  recall@10 against exact brute force of the queries."""
+    t0 = time.time()
+    gt, _ = ctx.bruteforce(10, Q=Q)
+    log(f"search ground truth for {Q.shape[0]} queries in {time.time() - t0:.1f}s")
+    st = ctx.dpg_build(args.K // 2, args.K)
+    ctx.search_use_graph(nnm.SEARCH_SRC_DPG)
+    qrows = np.arange(Q.shape[0], dtype=np.uint32)
+    res = {"queries": int(Q.shape[0]), "graph": "DPG of the merged k-NN graph", "dpg_secs": round(st["secs"], 4),
+           "mode": "flat (seeded)", "prec": "bf16 + FP32 re-rank", "width": args.search_width, "beams": []}
+    for beam in (32, 48, 64, 96, 128, 192, 256):
+        ctx.search(Q[:1000], beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, mode=nnm.SEARCH_FLAT, seed=3)
+        ids, _, ss = ctx.search(Q, beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, mode=nnm.SEARCH_FLAT,
+                                seed=3)
+        r = float(nnm.recall(ids, gt, qrows, 10))
+        row = {"beam": beam, "recall@10": round(r, 5), "qps": round(Q.shape[0] / ss["secs"], 1),
+               "kernel_secs": round(ss["secs"], 5), "dists_per_query": round(ss["dists"] / Q.shape[0], 1),
+               "hash_resets": int(ss["hash_resets"])}
+        res["beams"].append(row)
+        log(f"[search] {row}")
+    return res
+
+
 def copy_calibration():
     """1 GiB device-to-device copy (torch), three launches: the reference
     kernel for scaling FETCH_SIZE / WRITE_SIZE to bytes."""
@@ -203,13 +229,20 @@ def main():
     ap.add_argument("--prec", choices=["headline", "fp32", "bf16", "learned"], default="headline")
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--keep", type=float, default=0.98)
-    ap.add_argument("--plain-tail", type=int, default=1)
+    ap.add_argument("--plain-tail", type=int, default=3)
+    ap.add_argument("--no-renew", action="store_true", help="learned: keep the flags at the switch")
+    ap.add_argument("--learned-iters", type=int, default=8,
+                    help="learned: total iterations T of the learned run (learned phase + plain tail)")
     ap.add_argument("--graph", action="store_true", help="replay iterations from HIP graphs")
     ap.add_argument("--recall-rows", type=int, default=20000)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--copy-calib", action="store_true")
+    ap.add_argument("--search", action="store_true",
+                    help="also measure batched graph search on the merged graph (DPG layer, seeded flat search)")
+    ap.add_argument("--search-queries", type=int, default=100_000)
+    ap.add_argument("--search-width", type=int, default=1)
     ap.add_argument("--summarize", metavar="DIR", nargs="+")
     ap.add_argument("--pick", action="append", default=[], metavar="REGEX@a:b",
                     help="with --summarize: dispatches of a kernel, sliced by order")
@@ -234,8 +267,13 @@ def main():
     half = n // 2
     seed = 1
     t0 = time.time()
-    X = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4,
-                           rank=args.rank if args.rank > 0 else None)
+    Qs = None
+    if args.search:
+        X, Qs = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4,
+                                   rank=args.rank if args.rank > 0 else None, nq=args.search_queries)
+    else:
+        X = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4,
+                               rank=args.rank if args.rank > 0 else None)
     log(f"data {n}x{d} in {time.time() - t0:.1f}s")
 
     ctx = nnm.Context(0)
@@ -265,8 +303,10 @@ def main():
     flags = nnm.FLAG_AUTOTUNE | (nnm.FLAG_GRAPH if args.graph else 0)
 
     def run(name):
-        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=args.iters, prec=precs[name],
-                               seed=seed, plain_tail=args.plain_tail, flags=flags)
+        T = args.learned_iters if (name == "learned" and args.learned_iters) else args.iters
+        fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and not args.no_renew) else 0)
+        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name],
+                               seed=seed, plain_tail=args.plain_tail, flags=fl)
         train = None
         if name == "learned":
             ctx.graph_restore(recompute=True)
@@ -276,7 +316,8 @@ def main():
             log(f"mlp training: {train}")
 
         def step():
-            r = ctx.graph_restore(recompute=True)
+            # InitSGraph's recompute: FP32 rows for the plain run, bf16 rows (exact here) otherwise
+            r = ctx.graph_restore(recompute=1 if name == "fp32" else 2)
             i = ctx.merge_init(p)
             s = ctx.nndescent(p)
             ctx.merge_finish(True)
@@ -297,7 +338,8 @@ def main():
         for k, (r, i, s) in enumerate(recs):
             log(f"[{name}] step {k}: restore {r['secs']:.3f}s init {i['secs']:.3f}s descent {s['secs']:.3f}s "
                 f"join {s['join_secs']:.3f}s (short {s['join_main_secs']:.3f}s) iters {s['iters']} "
-                f"dists {s['dists']} exact {s['reranked']} offers {s['offers']}")
+                f"dists {s['dists']} exact {s['reranked']} offers {s['offers']} atomics {s['slot_atomics']} "
+                f"updates {s['updates']}")
         dists = float(np.mean([a["init_dists"] + b["init_dists"] + c["dists"] for a, b, c in recs]))
         res = {"merge_secs": round(ms * 1e-3, 4), "gdist_s": round(dists / (ms * 1e-3) / 1e9, 3),
                "dists_per_step": int(dists), "iters": recs[-1][2]["iters"],
@@ -343,6 +385,9 @@ def main():
         runs = {args.prec: run(args.prec)}
         chosen = args.prec
     ch = runs[chosen]
+    search = None
+    if args.search:
+        search = run_search(nnm, ctx, Qs, args)
     ctx.close()
 
     workload = (f"J-Merge {n}x{d} sift-like rank-{args.rank} ({half} built + {n - half} raw), K={K}, "
@@ -360,6 +405,8 @@ def main():
                               "outside the timed region"},
            "roofline": ch.get("roofline"), "runs": runs, "first_half_build": {k: bst[k] for k in
                                                                              ("iters", "secs", "dists")}}
+    if search is not None:
+        out["search"] = search
     if "fp32" in runs and chosen != "fp32":
         out["speedup_vs_plain_fp32"] = round(runs["fp32"]["merge_secs"] / ch["merge_secs"], 3)
     if not args.no_cpu:

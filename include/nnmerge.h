@@ -61,6 +61,9 @@ extern "C" {
 #define NNM_FLAG_AUTOTUNE 2u /* time the join launch shapes on the first
                                 iteration and keep the fastest (results do not
                                 depend on the shape) */
+#define NNM_FLAG_LEARNED_RENEW 4u /* NNM_PREC_LEARNED: when the lists leave the
+                                     learned space, every entry is marked new so
+                                     the plain join re-judges all of them */
 
 /* Re-rank policy for approximate precisions. */
 #define NNM_RERANK_AUTO 0   /* skip when the approximate form is exact */
@@ -124,6 +127,7 @@ typedef struct nnm_stats {
     uint64_t near_ties;    /* k-th slot decisions re-made on FP64 distances
                               (FP32 distances within rounding of each other) */
     uint64_t tie_swaps;    /* ... where FP64 kept the other entry */
+    uint64_t slot_atomics; /* slot atomics issued by the join (one per offer) */
 } nnm_stats;
 
 /* FP64 error of the stored FP32 list distances (nnm_graph_dist_error). */
@@ -186,8 +190,10 @@ int nnm_get_graph(nnm_ctx* ctx, uint32_t* ids, float* dist, uint8_t* flags);
 
 /* Device-side checkpoint of the lists: nnm_graph_save copies ids, dists and
  * flags into a second device buffer, nnm_graph_restore copies them back
- * (recompute = 1 also recomputes every distance, as InitSGraph does,
- * src/abstractmerge.cpp:55-76).  The reference's only checkpoint is the text
+ * (recompute = 1 also recomputes the distances of every row not in a raw
+ * subgraph from the FP32 rows, as InitSGraph does, src/abstractmerge.cpp:55-76;
+ * recompute = 2 the same from the bf16 rows where those are exact, which
+ * gives the same bits).  The reference's only checkpoint is the text
  * dump of AbstractMerge::saveKGraph (src/abstractmerge.cpp:278-291); this
  * keeps the copy in HBM so repeated merges start without a host transfer. */
 int nnm_graph_save(nnm_ctx* ctx);

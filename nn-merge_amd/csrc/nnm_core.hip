@@ -279,7 +279,9 @@ static void free_graph(nnm_ctx* c) {
 static void free_iter(nnm_ctx* c) {
     void** ps[] = {(void**)&c->l_new, (void**)&c->l_old, (void**)&c->n_new, (void**)&c->n_old,
                    (void**)&c->newmask, (void**)&c->rcnt, (void**)&c->roff, (void**)&c->rcur,
-                   (void**)&c->rbuf, (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->kth};
+                   (void**)&c->rbuf, (void**)&c->ocnt, (void**)&c->obuf, (void**)&c->ovf_bits,
+                   (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->kth,
+                   (void**)&c->vmeta};
     for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
     c->cap_new = c->cap_old = c->rev_cap = c->slots_C = 0;
 }
@@ -471,7 +473,8 @@ __device__ inline void wave_write_sorted_row(uint64_t key, uint32_t flag, uint32
 
 // K random distinct neighbours inside the row's group, self excluded.
 // Draws K+1 values like getRndSds(rng, K0+1, seeds, N) (src/kgraph.cpp:71).
-__device__ inline void wave_random_row(const float* __restrict__ X, uint32_t ld, uint64_t v, uint32_t gb,
+__device__ inline void wave_random_row(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
+                                       uint64_t v, uint32_t gb,
                                        uint32_t gsize, uint32_t K, uint64_t seed, uint32_t* __restrict__ ids,
                                        float* __restrict__ dist, uint8_t* __restrict__ flg, uint64_t* nd) {
     const uint32_t lane = nnm_lane();
@@ -493,7 +496,7 @@ __device__ inline void wave_random_row(const float* __restrict__ X, uint32_t ld,
     (void)take;
     uint32_t myid = (uint32_t)__shfl((int)id, (int)src_lane, NNM_WAVE);
     float dd = 0.f;
-    if (lane < K) dd = nnm_l2_rows(X + v * ld, X + (uint64_t)myid * ld, ld);
+    if (lane < K) dd = nnm_row_dist(X, Xb, ld, v, myid);
     uint64_t key = nnm_key(dd, myid);
     if (lane < K) *nd += 1;
     wave_write_sorted_row(key, 1u, K, ids, dist, flg);
@@ -508,26 +511,29 @@ __global__ void k_random_init(const float* __restrict__ X, uint32_t ld, uint64_t
     NNM_WAVE_LOOP(v, n) {
         const uint32_t g = sub_grp[vsub[v]];
         const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
-        wave_random_row(X, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd);
+        wave_random_row(X, nullptr, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd);
     }
     wave_count(counters + CNT_DISTS, nd);
 }
 
 // Upload fix-up: optional distance recompute, optional flags, sort rows.
-__global__ void k_set_graph(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K, int recompute,
-                            int have_flags, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+// skip_raw (optional): rows labelled raw are left as they are.
+__global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld, uint64_t n,
+                            uint32_t K, int recompute, int have_flags, const uint32_t* __restrict__ skip_raw,
+                            uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                             uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters,
                             uint32_t* __restrict__ bad) {
     const uint32_t lane = nnm_lane();
     uint64_t nd = 0;
     NNM_WAVE_LOOP(v, n) {
+        if (skip_raw && skip_raw[v] == NNM_RAW) continue;
         uint32_t id = 0;
         float dd = 0.f;
         uint32_t f = 1;
         if (lane < K) {
             id = g_ids[v * K + lane];
             if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
-            dd = recompute ? nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld) : g_dist[v * K + lane];
+            dd = recompute ? nnm_row_dist(X, Xb, ld, v, id) : g_dist[v * K + lane];
             f = have_flags ? g_flag[v * K + lane] : 1u;
             nd += recompute ? 1 : 0;
         }
@@ -536,8 +542,57 @@ __global__ void k_set_graph(const float* __restrict__ X, uint32_t ld, uint64_t n
     wave_count(counters + CNT_DISTS, nd);
 }
 
+
+// Learned phase entry: every list distance becomes the learned estimate
+// kscale * max(|p_v|^2 + |p_u|^2 - 2 p_v.p_u, 0) on the bf16 projections (the
+// learned join's key), rows re-sorted, flags kept.
+__global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restrict__ P, const float* __restrict__ pn,
+                                 uint32_t dout, float kscale, uint32_t* __restrict__ g_ids,
+                                 float* __restrict__ g_dist, uint8_t* __restrict__ g_flag) {
+    const uint32_t lane = nnm_lane();
+    NNM_WAVE_LOOP(v, n) {
+        uint32_t id = 0xFFFFFFFFu, f = 0;
+        float dd = 3.402823466e38f;
+        if (lane < K) {
+            id = g_ids[v * K + lane];
+            f = g_flag[v * K + lane];
+            const uint16_t* a = P + v * dout;
+            const uint16_t* b = P + (uint64_t)id * dout;
+            float dot = 0.f;
+            for (uint32_t k = 0; k < dout; ++k)
+                dot = __builtin_fmaf(__uint_as_float((uint32_t)a[k] << 16), __uint_as_float((uint32_t)b[k] << 16), dot);
+            dd = fmaxf((pn[v] + pn[id]) - 2.f * dot, 0.f) * kscale;
+        }
+        wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+    }
+}
+
+// Learned phase exit: exact FP32 distances for every list entry (from the
+// bf16 rows where those are exact), rows re-sorted, flags kept.  Counted as
+// exact distances outside the join.
+__global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ X, const uint16_t* __restrict__ Xb_exact,
+                              uint32_t ld, int renew, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                              uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters) {
+    const uint32_t lane = nnm_lane();
+    uint64_t nd = 0;
+    NNM_WAVE_LOOP(v, n) {
+        uint32_t id = 0xFFFFFFFFu, f = 0;
+        float dd = 3.402823466e38f;
+        if (lane < K) {
+            id = g_ids[v * K + lane];
+            f = renew ? 1u : g_flag[v * K + lane];
+            dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
+                          : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
+            ++nd;
+        }
+        wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+    }
+    wave_count(counters + CNT_RERANKED, nd);
+}
+
 // Merge set-up, one wave per row.
-__global__ void k_merge_init(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K, uint32_t cut,
+__global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld, uint64_t n,
+                             uint32_t K, uint32_t cut,
                              const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_begin,
                              const uint32_t* __restrict__ sub_grp, const uint8_t* __restrict__ sub_built,
                              const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
@@ -556,7 +611,7 @@ __global__ void k_merge_init(const float* __restrict__ X, uint32_t ld, uint64_t 
         uint8_t* flg = g_flag + v * K;
         if (!sub_built[s]) {
             if (lane == 0) has_cpn[v] = 0;
-            wave_random_row(X, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd);
+            wave_random_row(X, Xb, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd);
         } else {
             const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
             const uint32_t remain = K - cut;
@@ -599,7 +654,7 @@ __global__ void k_merge_init(const float* __restrict__ X, uint32_t ld, uint64_t 
                 uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
                 if (lane >= remain && lane < remain + fresh_n) {
                     pid = got;
-                    pd = nnm_l2_rows(X + (uint64_t)got * ld, X + v * ld, ld);
+                    pd = nnm_row_dist(X, Xb, ld, got, v);
                     pf = 1;
                     nd += 1;
                 }
@@ -733,8 +788,8 @@ extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist,
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
-                       K, dist ? 0 : 1, flags ? 1 : 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+    hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, nullptr, c->ld, c->n,
+                       K, dist ? 0 : 1, flags ? 1 : 0, nullptr, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
@@ -789,9 +844,16 @@ extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
     NNM_HIP(c, hipMemcpyAsync(c->g_dist, c->s_dist, nk * 4, hipMemcpyDeviceToDevice, c->stream));
     NNM_HIP(c, hipMemcpyAsync(c->g_flag, c->s_flag, nk, hipMemcpyDeviceToDevice, c->stream));
     if (recompute) {
+        // InitSGraph (src/abstractmerge.cpp:55-76) recomputes the built
+        // subgraphs' distances; raw rows are left to nnm_merge_init
+        const uint16_t* xb = nullptr;
+        if (recompute == 2 && c->data_exact_bf16) {
+            NNM_TRY(nnm_ensure_bf16(c));
+            xb = c->Xb;
+        }
         uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
-        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld,
-                           c->n_act, c->K, 1, 0, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld,
+                           c->n_act, c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
         NNM_LAUNCH_CHECK(c);
     }
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -856,9 +918,14 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
         NNM_TRY(alloc_n(c, &c->c_flag, c->n * cut));
         NNM_TRY(alloc_n(c, &c->has_cpn, c->n));
     }
+    const uint16_t* xb = nullptr;  // bf16 rows where exact, for the bf16 / learned precisions
+    if (p->prec != NNM_PREC_FP32 && c->data_exact_bf16) {
+        NNM_TRY(nnm_ensure_bf16(c));
+        xb = c->Xb;
+    }
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld, c->n_act,
+    hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld, c->n_act,
                        K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin, c->grp_end,
                        p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
                        c->c_flag, c->has_cpn, c->counters);
@@ -891,65 +958,116 @@ extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
 
 // k-th distance of every list, compact (n floats stay in the last-level cache
 // for the random lookups of the list build and the join).
-__global__ void k_kth(uint64_t n, uint32_t K, const float* __restrict__ g_dist, float* __restrict__ kth) {
+// k-th list distance per vertex, and the join's packed per-vertex record
+// {label, k-th distance x offer window, norm of the joined rows, 0}: one
+// 16-byte random read per candidate-list entry in the join.
+__global__ void k_kth(uint64_t n, uint32_t K, const float* __restrict__ g_dist, const uint32_t* __restrict__ label,
+                      const float* __restrict__ norms, float win, float* __restrict__ kth, uint4* __restrict__ vmeta) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) kth[v] = g_dist[v * K + K - 1];
+    if (v < n) {
+        const float k = g_dist[v * K + K - 1];
+        kth[v] = k;
+        vmeta[v] = make_uint4(label[v], __float_as_uint(k * win), norms ? __float_as_uint(norms[v]) : 0u, 0u);
+    }
 }
 
-// Forward lists, new-sample mask, reverse counts.  One thread per row.
-__global__ void k_lb_count(uint64_t n, uint32_t K, uint32_t S, const uint32_t* __restrict__ g_ids,
-                           const float* __restrict__ g_dist, const float* __restrict__ kthv,
-                           uint8_t* __restrict__ g_flag,
-                           uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
-                           uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
-                           uint32_t* __restrict__ newmask, uint32_t* __restrict__ rcnt) {
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const uint32_t* ids = g_ids + v * K;
-    const float* dd = g_dist + v * K;
-    uint8_t* fl = g_flag + v * K;
-    uint32_t nn = 0, no = 0, mask = 0;
-    for (uint32_t e = 0; e < K; ++e) {
-        const uint32_t u = ids[e];
-        const float de = dd[e];
-        const float kth = kthv[u];
-        if (fl[e]) {
-            if (nn >= S) continue;  // forward sampling: nearest S new entries
-            l_new[v * cap_new + nn++] = u;
-            mask |= 1u << e;
-            fl[e] = 0;
-            if (kth < de) atomicAdd(&rcnt[u], 1u);
-        } else {
-            l_old[v * cap_old + no++] = u;
-            if (kth < de) atomicAdd(&rcnt[n + u], 1u);
+// List build, pass 1 (src/abstractmerge.cpp:94-134): one lane per list
+// entry, G lanes per row.  Flagged entries become new samples (the first S
+// in list order, flag cleared), unflagged ones old; an entry (v -> u) whose
+// distance exceeds u's k-th distance also goes to u's reverse list (new or
+// old): an atomic counter gives its position in u's fixed bucket of NNM_RB
+// entries.  Entries past the bucket are collected by the overflow pass.
+template <uint32_t G>
+__global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32_t S, const uint32_t* __restrict__ g_ids,
+                                                  const float* __restrict__ g_dist, const float* __restrict__ kthv,
+                                                  uint8_t* __restrict__ g_flag, uint32_t* __restrict__ l_new,
+                                                  uint32_t cap_new, uint32_t* __restrict__ l_old, uint32_t cap_old,
+                                                  uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
+                                                  uint64_t* __restrict__ newmask, uint32_t* __restrict__ rcnt,
+                                                  uint32_t* __restrict__ rbuf) {
+    constexpr uint32_t GPW = NNM_WAVE / G;  // rows per wave
+    const uint32_t lane = nnm_lane(), grp = lane / G, gl = lane % G;
+    const uint64_t gmask = (G == NNM_WAVE) ? ~0ull : (((1ull << G) - 1) << (grp * G));
+    const uint64_t below = (1ull << lane) - 1;
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / NNM_WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / NNM_WAVE;
+    for (uint64_t v0 = wid * GPW; v0 < n; v0 += nw * GPW) {
+        const uint64_t v = v0 + grp;
+        const bool act = v < n && gl < K;
+        uint32_t u = 0;
+        float de = 0.f;
+        uint8_t f = 0;
+        if (act) {
+            u = g_ids[v * K + gl];
+            de = g_dist[v * K + gl];
+            f = g_flag[v * K + gl];
+        }
+        const uint64_t fm = nnm_ballot(act && f) & gmask;
+        const uint64_t om = nnm_ballot(act && !f) & gmask;
+        const uint32_t frank = (uint32_t)__popcll(fm & below);
+        const bool is_new = act && f && frank < S;
+        const bool is_old = act && !f;
+        if (is_new) {
+            l_new[v * cap_new + frank] = u;
+            g_flag[v * K + gl] = 0;
+        }
+        if (is_old) l_old[v * cap_old + (uint32_t)__popcll(om & below)] = u;
+        const uint64_t nm = nnm_ballot(is_new) & gmask;
+        if (gl == 0 && v < n) {
+            n_new[v] = min(S, (uint32_t)__popcll(fm));
+            n_old[v] = (uint32_t)__popcll(om);
+            newmask[v] = (G == NNM_WAVE) ? nm : (nm >> (grp * G));
+        }
+        if ((is_new || is_old) && kthv[u] < de) {
+            const uint64_t b = is_new ? (uint64_t)u : n + u;
+            const uint32_t pos = atomicAdd(&rcnt[b], 1u);
+            if (pos < NNM_RB) rbuf[b * NNM_RB + pos] = (uint32_t)v;
         }
     }
-    n_new[v] = nn;
-    n_old[v] = no;
-    newmask[v] = mask;
 }
 
-// Reverse entries into CSR buckets.
-__global__ void k_lb_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g_ids,
-                          const float* __restrict__ g_dist, const float* __restrict__ kthv,
-                          const uint8_t* __restrict__ g_flag,
-                          const uint32_t* __restrict__ newmask, const uint32_t* __restrict__ roff,
-                          uint32_t* __restrict__ rcur, uint32_t* __restrict__ rbuf) {
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const uint32_t* ids = g_ids + v * K;
-    const float* dd = g_dist + v * K;
-    const uint8_t* fl = g_flag + v * K;
-    const uint32_t mask = newmask[v];
-    for (uint32_t e = 0; e < K; ++e) {
-        const bool is_new = (mask >> e) & 1u;
-        const bool is_old = !is_new && fl[e] == 0;
+// Buckets that overflowed: their full size (scanned into overflow offsets)
+// and a bit in the overflow map.
+__global__ void k_lb_ovf_mark(uint64_t m, const uint32_t* __restrict__ rcnt, uint32_t* __restrict__ ocnt,
+                              uint32_t* __restrict__ ovf_bits) {
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < m; b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = rcnt[b];
+        const bool o = c > NNM_RB;
+        ocnt[b] = o ? c : 0u;
+        if (o) {
+            atomicOr(&ovf_bits[b >> 5], 1u << (b & 31));
+            atomicOr(&ovf_bits[(m + 31) / 32], 1u);  // any overflow at all
+        }
+    }
+}
+
+// List build, overflow pass: every reverse entry of an overflowed bucket
+// again, into the overflow CSR (the bucket holds only a prefix of them).
+template <uint32_t G>
+__global__ __launch_bounds__(256) void k_lb_ovf_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g_ids,
+                                                     const float* __restrict__ g_dist, const float* __restrict__ kthv,
+                                                     const uint8_t* __restrict__ g_flag,
+                                                     const uint64_t* __restrict__ newmask,
+                                                     const uint32_t* __restrict__ ovf_bits,
+                                                     const uint32_t* __restrict__ roff, uint32_t* __restrict__ rcur,
+                                                     uint32_t* __restrict__ obuf) {
+    constexpr uint32_t GPW = NNM_WAVE / G;
+    if (ovf_bits[(2 * n + 31) / 32] == 0u) return;  // no bucket overflowed
+    const uint32_t lane = nnm_lane(), grp = lane / G, gl = lane % G;
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / NNM_WAVE;
+    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / NNM_WAVE;
+    for (uint64_t v0 = wid * GPW; v0 < n; v0 += nw * GPW) {
+        const uint64_t v = v0 + grp;
+        if (v >= n || gl >= K) continue;
+        const bool is_new = (newmask[v] >> gl) & 1ull;
+        const bool is_old = !is_new && g_flag[v * K + gl] == 0;
         if (!is_new && !is_old) continue;
-        const uint32_t u = ids[e];
-        if (!(kthv[u] < dd[e])) continue;
-        const uint64_t b = is_new ? u : n + u;
-        uint32_t slot = atomicAdd(&rcur[b], 1u);
-        rbuf[roff[b] + slot] = (uint32_t)v;
+        const uint32_t u = g_ids[v * K + gl];
+        const uint64_t b = is_new ? (uint64_t)u : n + u;
+        if (!((ovf_bits[b >> 5] >> (b & 31)) & 1u)) continue;
+        if (!(kthv[u] < g_dist[v * K + gl])) continue;
+        const uint32_t pos = atomicAdd(&rcur[b], 1u);
+        obuf[roff[b] + pos] = (uint32_t)v;
     }
 }
 
@@ -995,7 +1113,8 @@ __device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ src,
 }
 
 __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ roff,
-                            const uint32_t* __restrict__ rbuf, uint32_t cap, uint64_t seed,
+                            const uint32_t* __restrict__ rbuf, const uint32_t* __restrict__ obuf, uint32_t cap,
+                            uint64_t seed,
                             uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
                             uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                             const uint64_t* __restrict__ counters) {
@@ -1004,10 +1123,11 @@ __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const
     const uint32_t salt = (uint32_t)(nnm_rand(seed, 0xC0FFEEull, counters[CNT_ITER]) >> 32);
     NNM_WAVE_LOOP(v, n) {
         const uint32_t fn = n_new[v], fo = n_old[v];
-        const uint32_t an = wave_append_reverse(rbuf + roff[v], rcnt[v], cap, nnm_hash32((uint32_t)v, salt),
-                                                l_new + v * cap_new + fn);
-        const uint32_t ao = wave_append_reverse(rbuf + roff[n + v], rcnt[n + v], cap,
-                                                nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u),
+        const uint32_t cn = rcnt[v], co = rcnt[n + v];
+        const uint32_t* sn = cn <= NNM_RB ? rbuf + v * NNM_RB : obuf + roff[v];
+        const uint32_t* so = co <= NNM_RB ? rbuf + (n + v) * NNM_RB : obuf + roff[n + v];
+        const uint32_t an = wave_append_reverse(sn, cn, cap, nnm_hash32((uint32_t)v, salt), l_new + v * cap_new + fn);
+        const uint32_t ao = wave_append_reverse(so, co, cap, nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u),
                                                 l_old + v * cap_old + fo);
         if (lane == 0) {
             n_new[v] = fn + an;
@@ -1105,11 +1225,31 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
     return acc;
 }
 
-__global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X, uint32_t ld,
-                         int rerank, float rerank_tau, float tie_win, uint32_t* __restrict__ g_ids,
+// Number of keys in s[0..cnt) below key (s sorted ascending, in LDS).
+__device__ inline uint32_t lds_rank(const uint64_t* s, uint32_t cnt, uint64_t key) {
+    uint32_t lo = 0, hi = cnt;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Merge each vertex's slots into its list (UpdateKnnList, src/pvetex.h:40-85,
+// for a batch): one wave per vertex.  The candidates that can enter (below
+// the k-th key, or within the FP32 rounding window of it) are de-duplicated
+// against the list, sorted by the wave, and merged with the sorted list by
+// rank (binary searches in LDS).  Slots are reset to empty.  With rerank,
+// lower-bound keys are recomputed exactly first.
+__global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X,
+                         const uint16_t* __restrict__ Xb_exact, uint32_t ld, int rerank, float rerank_tau,
+                         float tie_win, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
                          uint64_t* __restrict__ counters) {
-    const uint32_t lane = nnm_lane();
+    __shared__ uint64_t lds_pool[4][NNM_WAVE], lds_cand[4][NNM_WAVE];
+    const uint32_t lane = nnm_lane(), wib = threadIdx.x / NNM_WAVE;
+    uint64_t* LP = lds_pool[wib];
+    uint64_t* LC = lds_cand[wib];
     uint64_t c_upd = 0, c_flags = 0, c_rr = 0, c_ties = 0, c_swaps = 0;  // wave-uniform
     NNM_WAVE_LOOP(v, n) {
         uint64_t pkey = NNM_EMPTY_KEY;
@@ -1125,95 +1265,88 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
             if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
         }
         const uint64_t kth_key = __shfl(pkey, (int)(K - 1), NNM_WAVE);
-        bool valid = skey != NNM_EMPTY_KEY;
+        const float kd = nnm_key_dist(kth_key);
+        // cheap filter first: estimates against the margin, exact keys against
+        // the k-th key (or its rounding window)
+        bool valid = skey != NNM_EMPTY_KEY &&
+                     (rerank ? nnm_key_dist(skey) <= kd * rerank_tau
+                             : (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd * tie_win)));
+        if (!__ballot(valid)) {
+            c_flags += (uint32_t)__popcll(__ballot(lane < K && pf));
+            continue;
+        }
         const uint32_t sid = nnm_key_id(skey);
-        if (__ballot(valid)) {
-            for (uint32_t p = 0; p < K; ++p) {
-                uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
-                valid = valid && (q != sid);
+        for (uint32_t p = 0; p < K; ++p) {
+            // every lane takes part in the shuffle (a lane that skipped it
+            // would leave its value undefined for the others)
+            const uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
+            valid = valid && q != sid;
+        }
+        if (rerank) {
+            if (valid) {
+                const float ex = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
+                                          : nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
+                skey = nnm_key(ex, sid);
             }
-            uint32_t rr = 0;
-            if (rerank) {
-                // approximate keys carry a bounded error: re-rank every candidate
-                // that can still cross the k-th entry
-                const float kd = nnm_key_dist(kth_key);
-                valid = valid && (nnm_key_dist(skey) <= kd * rerank_tau);
-                if (valid) {
-                    float ex = nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
-                    skey = nnm_key(ex, sid);
-                    rr = 1;
-                }
-                rr = nnm_wave_sum32(rr);
-            }
-            // candidates within the FP32 error window of the k-th entry stay in
-            // play for the FP64 tie test below
-            const float kd_win = nnm_key_dist(kth_key) * tie_win;
-            valid = valid && (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd_win));
-            const uint64_t vm = __ballot(valid);
-            uint32_t pos_p = lane, pos_s = 0;
-            // rank against the other valid candidates
-            uint64_t mm = vm;
-            while (mm) {
-                const uint32_t e = (uint32_t)__ffsll((long long)mm) - 1;
-                mm &= mm - 1;
-                const uint64_t ek = __shfl(skey, (int)e, NNM_WAVE);
-                pos_p += (ek < pkey) ? 1u : 0u;
-                pos_s += (ek < skey) ? 1u : 0u;
-            }
-            // rank against the pool
-            for (uint32_t p = 0; p < K; ++p) {
-                const uint64_t q = __shfl(pkey, (int)p, NNM_WAVE);
-                pos_s += (q < skey) ? 1u : 0u;
-            }
-            if (tie_win > 1.f) {
-                // the entries ranked K-1 (last kept) and K (first dropped): when
-                // their FP32 distances are within rounding of each other the
-                // choice is re-made on FP64 distances (ties by id)
-                const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(valid && pos_s == K - 1);
-                const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(valid && pos_s == K);
-                if ((ap | as) && (bp | bs)) {
-                    const bool a_pool = ap != 0, b_pool = bp != 0;
-                    const int la = __ffsll((long long)(a_pool ? ap : as)) - 1;
-                    const int lb = __ffsll((long long)(b_pool ? bp : bs)) - 1;
-                    const uint64_t ka = __shfl(a_pool ? pkey : skey, la, NNM_WAVE);
-                    const uint64_t kb = __shfl(b_pool ? pkey : skey, lb, NNM_WAVE);
-                    if (nnm_key_dist(kb) <= nnm_key_dist(ka) * tie_win) {
-                        ++c_ties;
-                        const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
-                        const double ea = wave_l2_f64(X + v * ld, X + (uint64_t)ia * ld, ld);
-                        const double eb = wave_l2_f64(X + v * ld, X + (uint64_t)ib * ld, ld);
-                        if (eb < ea || (eb == ea && ib < ia)) {
-                            ++c_swaps;
-                            if ((int)lane == la) {
-                                if (a_pool) pos_p = K; else pos_s = K;
-                            }
-                            if ((int)lane == lb) {
-                                if (b_pool) pos_p = K - 1; else pos_s = K - 1;
-                            }
+            c_rr += (uint32_t)__popcll(__ballot(valid));
+            valid = valid && (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd * tie_win));
+        }
+        // sort the candidates; rank candidates and list entries against each other
+        const uint64_t ck = nnm_wave_sort64(valid ? skey : NNM_EMPTY_KEY);
+        const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+        LP[lane] = pkey;
+        LC[lane] = ck;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bool is_c = lane < nv;
+        uint32_t pos_p = lane + lds_rank(LC, nv, pkey);
+        uint32_t pos_s = lane + lds_rank(LP, K, ck);
+        if (tie_win > 1.f) {
+            // the entries ranked K-1 (last kept) and K (first dropped): when
+            // their FP32 distances are within rounding of each other the
+            // choice is re-made on FP64 distances (ties by id)
+            const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(is_c && pos_s == K - 1);
+            const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(is_c && pos_s == K);
+            if ((ap | as) && (bp | bs)) {
+                const bool a_pool = ap != 0, b_pool = bp != 0;
+                const int la = __ffsll((long long)(a_pool ? ap : as)) - 1;
+                const int lb = __ffsll((long long)(b_pool ? bp : bs)) - 1;
+                const uint64_t ka = __shfl(a_pool ? pkey : ck, la, NNM_WAVE);
+                const uint64_t kb = __shfl(b_pool ? pkey : ck, lb, NNM_WAVE);
+                if (nnm_key_dist(kb) <= nnm_key_dist(ka) * tie_win) {
+                    ++c_ties;
+                    const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
+                    const double ea = wave_l2_f64(X + v * ld, X + (uint64_t)ia * ld, ld);
+                    const double eb = wave_l2_f64(X + v * ld, X + (uint64_t)ib * ld, ld);
+                    if (eb < ea || (eb == ea && ib < ia)) {
+                        ++c_swaps;
+                        if ((int)lane == la) {
+                            if (a_pool) pos_p = K; else pos_s = K;
+                        }
+                        if ((int)lane == lb) {
+                            if (b_pool) pos_p = K - 1; else pos_s = K - 1;
                         }
                     }
                 }
             }
-            const bool ins = valid && pos_s < K;
-            const bool keep = lane < K && pos_p < K;
-            if (keep && pos_p != lane) {
-                g_ids[v * K + pos_p] = pid;
-                g_dist[v * K + pos_p] = nnm_key_dist(pkey);
-                g_flag[v * K + pos_p] = (uint8_t)pf;
-            }
-            if (ins) {
-                g_ids[v * K + pos_s] = sid;
-                g_dist[v * K + pos_s] = nnm_key_dist(skey);
-                g_flag[v * K + pos_s] = 1;
-            }
-            const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
-            const uint32_t n_flag = (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
-            c_upd += n_ins;
-            c_flags += n_flag;
-            c_rr += rr;
-        } else {
-            c_flags += (uint32_t)__popcll(__ballot(lane < K && pf));
         }
+        const bool ins = is_c && pos_s < K;
+        const bool keep = lane < K && pos_p < K;
+        if (keep && pos_p != lane) {
+            g_ids[v * K + pos_p] = pid;
+            g_dist[v * K + pos_p] = nnm_key_dist(pkey);
+            g_flag[v * K + pos_p] = (uint8_t)pf;
+        }
+        if (ins) {
+            g_ids[v * K + pos_s] = nnm_key_id(ck);
+            g_dist[v * K + pos_s] = nnm_key_dist(ck);
+            g_flag[v * K + pos_s] = 1;
+        }
+        const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
+        c_upd += n_ins;
+        c_flags += (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
+        __builtin_amdgcn_wave_barrier();  // LDS reused by the next vertex
     }
     if (lane == 0) {
         if (c_upd) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)c_upd);
@@ -1225,7 +1358,7 @@ __global__ void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __rest
 }
 
 float nnm_near_tie_window(nnm_ctx* c, const nnm_params* p) {
-    (void)p;
+    if (p->prec == NNM_PREC_LEARNED) return 1.f;  // learned estimates: no FP64 decision
     if (c->data_exact_bf16) return 1.f;  // integer rows: every distance is exact
     return 1.f + 2.f * (float)(c->d + 2) * 5.9604645e-8f;
 }
@@ -1311,10 +1444,14 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     NNM_TRY(alloc_n(c, &c->rcnt, 2 * n));
     NNM_TRY(alloc_n(c, &c->roff, 2 * n + 1));
     NNM_TRY(alloc_n(c, &c->rcur, 2 * n));
-    NNM_TRY(alloc_n(c, &c->rbuf, n * K));
+    NNM_TRY(alloc_n(c, &c->rbuf, 2 * n * NNM_RB));
+    NNM_TRY(alloc_n(c, &c->ocnt, 2 * n));
+    NNM_TRY(alloc_n(c, &c->obuf, n * K));
+    NNM_TRY(alloc_n(c, &c->ovf_bits, (2 * n + 31) / 32 + 1));
     NNM_TRY(alloc_n(c, &c->scan_tmp, (2 * n + SCAN_TILE - 1) / SCAN_TILE + 1));
     NNM_TRY(alloc_n(c, &c->slots, n * C));
     NNM_TRY(alloc_n(c, &c->kth, n));
+    NNM_TRY(alloc_n(c, &c->vmeta, n));
     NNM_TRY(fill_u64(c, c->slots, n * C, NNM_EMPTY_KEY));
     return NNM_OK;
 }
@@ -1331,7 +1468,8 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
-    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE)) return nnm_fail(c, NNM_ERR_INVALID, "unknown flags");
+    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE | NNM_FLAG_LEARNED_RENEW))
+        return nnm_fail(c, NNM_ERR_INVALID, "unknown flags");
     return NNM_OK;
 }
 
@@ -1355,26 +1493,38 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
     NNM_TRY(fill_u32(c, c->rcnt, 2 * n, 0));
     NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
-    hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->kth);
+    NNM_TRY(fill_u32(c, c->ovf_bits, (2 * n + 31) / 32 + 1, 0));
+    const float* nrm = p->prec == NNM_PREC_BF16 ? c->norms : (p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c) : nullptr);
+    hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->label,
+                       nrm, nnm_near_tie_window(c, p), c->kth, c->vmeta);
     NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_lb_count, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, S, c->g_ids,
-                       c->g_dist, c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
-                       c->newmask, c->rcnt);
+    const uint32_t lb_grid = wave_grid(c, (n + 1) / 2);
+    if (K <= 32) {
+        hipLaunchKernelGGL(k_lb_build<32>, dim3(lb_grid), dim3(256), 0, c->stream, n, K, S, c->g_ids, c->g_dist,
+                           c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
+                           c->newmask, c->rcnt, c->rbuf);
+    } else {
+        hipLaunchKernelGGL(k_lb_build<64>, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, S, c->g_ids,
+                           c->g_dist, c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,
+                           c->n_old, c->newmask, c->rcnt, c->rbuf);
+    }
     NNM_LAUNCH_CHECK(c);
     const uint64_t m = 2 * n;
-    const uint32_t ntiles = (uint32_t)((m + SCAN_TILE - 1) / SCAN_TILE);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(256), 0, c->stream, c->rcnt, m, c->scan_tmp);
+    hipLaunchKernelGGL(k_lb_ovf_mark, dim3(grid_for(m, 256, 8192)), dim3(256), 0, c->stream, m, c->rcnt, c->ocnt,
+                       c->ovf_bits);
     NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, c->scan_tmp, ntiles, c->roff + m);
-    NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(256), 0, c->stream, c->rcnt, m, c->scan_tmp, c->roff);
-    NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_lb_fill, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_ids,
-                       c->g_dist, c->kth, c->g_flag, c->newmask, c->roff, c->rcur, c->rbuf);
+    NNM_TRY(nnm_scan_u32(c, c->ocnt, m, c->roff, c->scan_tmp));
+    if (K <= 32) {
+        hipLaunchKernelGGL(k_lb_ovf_fill<32>, dim3(lb_grid), dim3(256), 0, c->stream, n, K, c->g_ids, c->g_dist,
+                           c->kth, c->g_flag, c->newmask, c->ovf_bits, c->roff, c->rcur, c->obuf);
+    } else {
+        hipLaunchKernelGGL(k_lb_ovf_fill<64>, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, c->g_ids,
+                           c->g_dist, c->kth, c->g_flag, c->newmask, c->ovf_bits, c->roff, c->rcur, c->obuf);
+    }
     NNM_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
-                       c->rbuf, c->rev_cap, p->seed, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
-                       c->counters);
+                       c->rbuf, c->obuf, c->rev_cap, p->seed, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,
+                       c->n_old, c->counters);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
@@ -1385,14 +1535,13 @@ static int update_lists(nnm_ctx* c, const nnm_params* p) {
     float tau = 1.f;
     if (p->prec == NNM_PREC_BF16) {
         rerank = (p->rerank == NNM_RERANK_ALWAYS) || (p->rerank == NNM_RERANK_AUTO && !c->data_exact_bf16);
-    } else if (p->prec == NNM_PREC_LEARNED) {
-        rerank = 1;  // projected distances are estimates: every offer is recomputed
-        tau = nnm_mlp_tau(c);
     }
-    if (p->rerank == NNM_RERANK_NEVER) rerank = 0;
+    // learned phase: lists and offers are both estimates of one metric, so the
+    // update merges them as they are (the exact distances come at the switch)
+    if (p->rerank == NNM_RERANK_NEVER || p->prec == NNM_PREC_LEARNED) rerank = 0;
     hipLaunchKernelGGL(k_update, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
-                       c->slots_C, c->X, c->ld, rerank, tau, nnm_near_tie_window(c, p), c->g_ids, c->g_dist, c->g_flag,
-                       c->slots, c->counters);
+                       c->slots_C, c->X, (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld, rerank, tau,
+                       nnm_near_tie_window(c, p), c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
@@ -1458,7 +1607,8 @@ static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) 
     c->tune_js[p->prec] = best_js;
     c->tune_chunk[p->prec] = best_ch;
     c->tune_ld[p->prec] = c->ld;
-    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS, CNT_LONG_DISTS, CNT_LONG_ENTRIES, CNT_LONG_OFFERS})
+    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS, CNT_LONG_DISTS, CNT_LONG_ENTRIES, CNT_LONG_OFFERS,
+                  CNT_ATOMICS})
         NNM_TRY(fill_u64(c, c->counters + i, 1, h[i]));
     NNM_TRY(reset_queues(c));
     NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
@@ -1516,16 +1666,38 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     const bool autotune = (p->flags & NNM_FLAG_AUTOTUNE) != 0;
     const bool graph = (p->flags & NNM_FLAG_GRAPH) != 0;
     IterGraphs graphs;
-    // Learned runs end on the plain join, which catches what the prefilter
-    // dropped: the last plain_tail iterations of a capped run, and every
-    // iteration after the learned phase meets the delta test.
+    // Learned runs: the learned phase runs NN-Descent on the learned
+    // estimates (the lists hold estimates, the join gathers 32-d projected
+    // rows), then every list distance becomes exact once and the plain BF16
+    // join finishes: the last plain_tail iterations of a capped run, and every
+    // iteration after the learned phase meets the delta test.  The plain join
+    // is the fallback and judges what the learned phase missed.
     bool tail = !learned;
+    bool estimates = false;  // the lists hold learned estimates
+    auto to_estimates = [&]() -> int {
+        hipLaunchKernelGGL(k_lists_estimate, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
+                           nnm_mlp_rows(c), nnm_mlp_norms(c), nnm_mlp_dout(c), nnm_mlp_scale(c), c->g_ids, c->g_dist,
+                           c->g_flag);
+        NNM_LAUNCH_CHECK(c);
+        estimates = true;
+        return NNM_OK;
+    };
+    auto to_exact = [&]() -> int {
+        hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
+                           (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld,
+                           (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, c->g_ids, c->g_dist, c->g_flag, c->counters);
+        NNM_LAUNCH_CHECK(c);
+        estimates = false;
+        return NNM_OK;
+    };
     for (;;) {
         const uint32_t iter = single ? only_iter : it;
         nnm_params q = *p;
         if (p->prec == NNM_PREC_LEARNED) {
             if (p->max_iters && iter + p->plain_tail >= p->max_iters) tail = true;
             q.prec = tail ? NNM_PREC_BF16 : NNM_PREC_LEARNED;
+            if (q.prec == NNM_PREC_LEARNED && !estimates) NNM_TRY(to_estimates());
+            if (q.prec != NNM_PREC_LEARNED && estimates) NNM_TRY(to_exact());
         }
         if (q.prec == NNM_PREC_LEARNED) ++n_learned; else ++n_plain;
         NNM_TRY(fill_u64(c, c->counters + CNT_ITER, 1, iter));
@@ -1584,6 +1756,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         if (single) break;
         if (p->max_iters && it >= p->max_iters) break;
     }
+    if (estimates) NNM_TRY(to_exact());  // a run that ended in the learned phase
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
     NNM_TRY(read_counters(c, h));
@@ -1612,6 +1785,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->join_main_bytes = main_bytes;
         st->join_main_flops = main_flops;
         st->near_ties = h[CNT_NEAR_TIES];
+        st->slot_atomics = h[CNT_ATOMICS];
         st->tie_swaps = h[CNT_TIE_SWAPS];
     }
     return NNM_OK;

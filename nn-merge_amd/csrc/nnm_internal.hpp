@@ -27,6 +27,10 @@
 // values; the low word breaks ties by id.  An empty slot holds all ones.
 #define NNM_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
 
+// Reverse-list bucket capacity per target (entries beyond it go to the
+// overflow CSR).
+#define NNM_RB 64u
+
 enum CounterIdx {
     CNT_DISTS = 0,
     CNT_UPDATES = 1,
@@ -44,7 +48,8 @@ enum CounterIdx {
     CNT_LONG_OFFERS = 13,   // the short-list kernel's)
     CNT_NEAR_TIES = 14,     // k-th slot decisions re-made in FP64
     CNT_TIE_SWAPS = 15,     // ... that changed which entry stays
-    CNT_N = 16
+    CNT_ATOMICS = 16,       // slot atomics issued (offers not already dominated)
+    CNT_N = 24
 };
 
 struct nnm_ctx {
@@ -100,15 +105,19 @@ struct nnm_ctx {
     uint32_t* l_old = nullptr;      // n x cap_old
     uint32_t* n_new = nullptr;      // n
     uint32_t* n_old = nullptr;      // n
-    uint32_t* newmask = nullptr;    // n
-    uint32_t* rcnt = nullptr;       // 2n counts (new, old)
-    uint32_t* roff = nullptr;       // 2n + 1 offsets
-    uint32_t* rcur = nullptr;       // 2n cursors
-    uint32_t* rbuf = nullptr;       // n*K reverse entries
+    uint64_t* newmask = nullptr;    // n: list entries taken as new samples
+    uint32_t* rcnt = nullptr;       // 2n reverse counts (new, old)
+    uint32_t* rbuf = nullptr;       // 2n x NNM_RB reverse buckets
+    uint32_t* ocnt = nullptr;       // 2n overflow sizes (count when above NNM_RB)
+    uint32_t* roff = nullptr;       // 2n + 1 overflow offsets
+    uint32_t* rcur = nullptr;       // 2n overflow cursors
+    uint32_t* obuf = nullptr;       // n*K overflow entries
+    uint32_t* ovf_bits = nullptr;   // 2n bits: bucket overflowed
     uint32_t* scan_tmp = nullptr;   // block sums
     uint64_t* slots = nullptr;      // n x slots_C
     uint64_t* counters = nullptr;   // CNT_N
     float* kth = nullptr;           // n: k-th distance of every list at iteration start
+    uint4* vmeta = nullptr;         // n: join record {label, kth x offer window, norm, 0}
     uint32_t* grp_flags = nullptr;  // ngrp: per-group new-flag counts
     uint8_t* grp_active = nullptr;  // ngrp
 
@@ -247,6 +256,34 @@ __device__ inline float nnm_l2_rows(const float* __restrict__ a, const float* __
         acc = __builtin_fmaf(t, t, acc);
     }
     return acc;
+}
+
+// Exact FP32 distance of rows a, b in the canonical order from bf16 rows:
+// used where every value is exact in bf16 (integer rows), where it equals
+// nnm_l2_rows on the FP32 rows bit for bit while reading half the bytes.
+__device__ inline float nnm_l2_rows_bf16(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, uint32_t ld) {
+    const uint4* a4 = reinterpret_cast<const uint4*>(a);
+    const uint4* b4 = reinterpret_cast<const uint4*>(b);
+    float acc = 0.f;
+    for (uint32_t i = 0; i < ld / 8; ++i) {
+        const uint4 x = a4[i], y = b4[i];
+        const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            float t = __uint_as_float(xs[h] << 16) - __uint_as_float(ys[h] << 16);
+            acc = __builtin_fmaf(t, t, acc);
+            t = __uint_as_float(xs[h] & 0xFFFF0000u) - __uint_as_float(ys[h] & 0xFFFF0000u);
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    return acc;
+}
+
+// Row distance from the bf16 rows when given (values exact in bf16), else
+// from the FP32 rows: the same bits either way.
+__device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
+                                     uint64_t a, uint64_t b) {
+    return Xb ? nnm_l2_rows_bf16(Xb + a * ld, Xb + b * ld, ld) : nnm_l2_rows(X + a * ld, X + b * ld, ld);
 }
 
 // Wave-wide bitonic sort of one 64-bit key per lane, ascending by lane.

@@ -52,7 +52,7 @@ struct JoinArgs {
     uint32_t rule;
     uint32_t salt;    // set inside each kernel from seed and the iteration counter
     uint64_t seed;
-    float kwin;       // offer window above the k-th distance (1 + near-tie width)
+    const uint4* vmeta;  // per vertex {label, k-th distance x offer window, row norm, 0}
     uint32_t js;      // short-list window rows (<= the kernel's compiled window)
     uint32_t chunk;   // rows per work-queue grab
     const uint32_t* l_new;
@@ -87,10 +87,15 @@ __device__ inline uint32_t join_salt(const JoinArgs& a) {
     return (uint32_t)nnm_rand(a.seed, 0xC0FFEEull, a.counters[CNT_ITER]);
 }
 
-__device__ inline void offer(uint64_t* __restrict__ slots, uint32_t C, uint32_t salt, uint32_t target,
-                             uint32_t id, float dd) {
+// Offer (id, dd) to target's slot: one fire-and-forget 64-bit atomicMin (no
+// value returned, so the wave does not wait on it).  Returns the count to add
+// to a lane's offer tally (1 for the offer, 1 << 32 for its slot atomic).
+__device__ inline uint64_t offer(uint64_t* __restrict__ slots, uint32_t C, uint32_t salt, uint32_t target,
+                                 uint32_t id, float dd) {
     const uint32_t s = nnm_hash32(id, salt) & (C - 1);
-    atomicMin((unsigned long long*)&slots[(uint64_t)target * C + s], (unsigned long long)nnm_key(dd, id));
+    atomicMin(reinterpret_cast<unsigned long long*>(&slots[(uint64_t)target * C + s]),
+              (unsigned long long)nnm_key(dd, id));
+    return 1ull | (1ull << 32);
 }
 
 // Epilogue of one 2x4 micro-tile: pair rule, distance count, offers.
@@ -114,12 +119,10 @@ __device__ inline void tile_epilogue(const JoinArgs& a, const float (&acc)[TM * 
             const float dd = acc[i * TN + j];
             ++my_dists;
             if (dd < __uint_as_float(mB[2][cj])) {
-                offer(a.slots, a.C, a.salt, bid, aid, dd);
-                ++my_offers;
+                my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
             }
             if (dd < athr) {
-                offer(a.slots, a.C, a.salt, aid, bid, dd);
-                ++my_offers;
+                my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
             }
         }
     }
@@ -163,15 +166,17 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
             if (lane < nb) {
                 const uint32_t idx = b0 + lane;
                 const uint32_t id = idx < nn ? Ln[idx] : Lo[idx - nn];
+                const uint4 mt = a.vmeta[id];
                 L.metaB[0][lane] = id;
-                L.metaB[1][lane] = a.label[id];
-                L.metaB[2][lane] = __float_as_uint(a.kth[id] * a.kwin);
+                L.metaB[1][lane] = mt.x;
+                L.metaB[2][lane] = mt.y;
             }
             if (!diag && lane < na) {
                 const uint32_t id = Ln[a0 + lane];
+                const uint4 mt = a.vmeta[id];
                 L.metaA[0][lane] = id;
-                L.metaA[1][lane] = a.label[id];
-                L.metaA[2][lane] = __float_as_uint(a.kth[id] * a.kwin);
+                L.metaA[1][lane] = mt.x;
+                L.metaA[2][lane] = mt.y;
             }
             const float* rA = diag ? rowsB : rowsA;
             const uint32_t(*mA)[JW] = diag ? L.metaB : L.metaA;
@@ -417,9 +422,10 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
                     const uint64_t v = base + vb;
                     const uint32_t e = lane - ob;
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
+                    const uint4 mt = a.vmeta[wid];
                     L.meta[0][lane] = wid;
-                    L.meta[1][lane] = a.label[wid];
-                    L.meta[2][lane] = __float_as_uint(a.kth[wid] * a.kwin);
+                    L.meta[1][lane] = mt.x;
+                    L.meta[2][lane] = mt.y;
                 }
             }
             // rows -> LDS, one row per direct-to-LDS instruction
@@ -445,7 +451,8 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
     if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
+        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
     }
 }
 
@@ -552,10 +559,11 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                     const uint64_t v = base + vb;
                     const uint32_t e = lane - ob;
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
+                    const uint4 mt = a.vmeta[wid];
                     L.meta[0][lane] = wid;
-                    L.meta[1][lane] = a.label[wid];
-                    L.meta[2][lane] = __float_as_uint(a.kth[wid] * a.kwin);
-                    L.meta[3][lane] = __float_as_uint(norms[wid]);
+                    L.meta[1][lane] = mt.x;
+                    L.meta[2][lane] = mt.y;
+                    L.meta[3][lane] = mt.z;
                 }
             }
             // rows -> LDS: lane l of a row instruction fetches dword 2u + (l & 1), u = (l/2) ^ (r & SWM)
@@ -615,12 +623,10 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                             const float dd = offer_key(an, bn, acc[q], mode, kscale);
                             ++my_dists;
                             if (dd < tau * bthr) {
-                                offer(a.slots, a.C, a.salt, bid, aid, dd);
-                                ++my_offers;
+                                my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
                             }
                             if (dd < tau * __uint_as_float(L.meta[2][ob + ri])) {
-                                offer(a.slots, a.C, a.salt, aid, bid, dd);
-                                ++my_offers;
+                                my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
                             }
                         }
                     }
@@ -637,7 +643,8 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
     if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
+        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
     }
 }
 
@@ -690,10 +697,11 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
         if (tid == 0) my_entries += m;
         for (uint32_t j = tid; j < m; j += blockDim.x) {
             const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
+            const uint4 mt = a.vmeta[id];
             L.meta[0][j] = id;
-            L.meta[1][j] = a.label[id];
-            L.meta[2][j] = __float_as_uint(a.kth[id] * a.kwin);
-            L.meta[3][j] = __float_as_uint(norms[id]);
+            L.meta[1][j] = mt.x;
+            L.meta[2][j] = mt.y;
+            L.meta[3][j] = mt.z;
         }
         __syncthreads();
         // each wave copies every 4th row
@@ -749,12 +757,10 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
                     const float dd = offer_key(an, bn, acc[q], mode, kscale);
                     ++my_dists;
                     if (dd < tau * bthr) {
-                        offer(a.slots, a.C, a.salt, bid, aid, dd);
-                        ++my_offers;
+                        my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
                     }
                     if (dd < tau * __uint_as_float(L.meta[2][ri])) {
-                        offer(a.slots, a.C, a.salt, aid, bid, dd);
-                        ++my_offers;
+                        my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
                     }
                 }
             }
@@ -770,10 +776,11 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
     if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
+        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
         atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
     }
 }
 
@@ -806,10 +813,11 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
     if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
+        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
         atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)my_dists);
         atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)my_offers);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
     }
 }
 
@@ -857,7 +865,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
     a.salt = 0;
     a.seed = p->seed;
     a.chunk = chunk ? chunk : JOIN_CHUNK;
-    a.kwin = nnm_near_tie_window(c, p);
+    a.vmeta = c->vmeta;
     a.l_new = c->l_new;
     a.cap_new = c->cap_new;
     a.l_old = c->l_old;
@@ -881,7 +889,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
             rl = nnm_mlp_dout(c);
             mode = KEY_LEARN;
             kscale = nnm_mlp_scale(c);
-            tau = nnm_mlp_tau(c);
+            tau = 1.f;  // the lists hold estimates in the same metric during the learned phase
             if (!rows) return nnm_fail(c, NNM_ERR_STATE, "learned join without a projection");
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");

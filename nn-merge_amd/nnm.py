@@ -20,7 +20,7 @@ RAW = 0xFFFFFFFF
 PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
 RULE_ALL, RULE_MERGE = 0, 1
 RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
-FLAG_GRAPH, FLAG_AUTOTUNE = 1, 2
+FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_LEARNED_RENEW = 1, 2, 4
 SEARCH_LAYERED, SEARCH_FLAT = 0, 1
 SEARCH_SRC_KNN, SEARCH_SRC_DPG = 0, 1
 
@@ -40,7 +40,7 @@ class Stats(C.Structure):
                 ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32),
                 ("long_dists", C.c_uint64), ("long_entries", C.c_uint64), ("long_offers", C.c_uint64),
                 ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64),
-                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64)]
+                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64), ("slot_atomics", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -236,8 +236,9 @@ class Context:
         self._chk(lib().nnm_graph_save(self.h))
 
     def graph_restore(self, recompute=True):
+        """recompute: False, True (FP32 rows) or 2 (bf16 rows where exact)."""
         st = Stats()
-        self._chk(lib().nnm_graph_restore(self.h, 1 if recompute else 0, C.byref(st)))
+        self._chk(lib().nnm_graph_restore(self.h, int(recompute), C.byref(st)))
         return st.as_dict()
 
     def nndescent(self, p):

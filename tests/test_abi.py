@@ -40,4 +40,4 @@ def test_params_struct_layout_matches_header():
     # nnm_params: 12 x 4-byte fields then a uint64 seed (offset 48), size 56
     assert ctypes.sizeof(nnm.Params) == 56
     assert nnm.Params.seed.offset == 48
-    assert ctypes.sizeof(nnm.Stats) == 8 + 6 * 8 + 3 * 8 + 8 + 16 + 32 + 16 + 16
+    assert ctypes.sizeof(nnm.Stats) == 8 + 6 * 8 + 3 * 8 + 8 + 16 + 32 + 16 + 16 + 8
