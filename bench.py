@@ -176,20 +176,28 @@ def summarize(dirs, picks):
 def run_search(nnm, ctx, Q, args):
     """Batched search on the merged graph (LySearch on a DPG layer): the DPG
     of the final k-NN lists (M = K/2, maxM = K, LyNNDescent::dpg on the base
-    layer), then the seeded flat search of every query at each beam, bf16 rows
-    with FP32 re-rank.  recall@10 against exact brute force of the queries."""
+    layer) under a complete graph on a row sample, then the layered search of
+    every query at each beam (LySearch::knnSearch), bf16 rows with FP32
+    re-rank.  recall@10 against exact brute force of the queries."""
     t0 = time.time()
     gt, _ = ctx.bruteforce(10, Q=Q)
     log(f"search ground truth for {Q.shape[0]} queries in {time.time() - t0:.1f}s")
     st = ctx.dpg_build(args.K // 2, args.K)
-    ctx.search_use_graph(nnm.SEARCH_SRC_DPG)
+    # layers: a complete graph on the first S0 rows (a random sample of the
+    # mixture) for the greedy descent, then the DPG base layer
+    S0 = args.search_top
+    base = ctx.dpg_get()
+    top_offs = (np.arange(S0 + 1, dtype=np.uint64) * (S0 - 1)).astype(np.uint32)
+    top_adj = np.tile(np.arange(S0, dtype=np.uint32), (S0, 1))[~np.eye(S0, dtype=bool)]
+    ctx.search_set_layers([(top_offs, top_adj), (base[0], base[1])])
+    del base
     qrows = np.arange(Q.shape[0], dtype=np.uint32)
-    res = {"queries": int(Q.shape[0]), "graph": "DPG of the merged k-NN graph", "dpg_secs": round(st["secs"], 4),
-           "mode": "flat (seeded)", "prec": "bf16 + FP32 re-rank", "width": args.search_width, "beams": []}
+    res = {"queries": int(Q.shape[0]), "graph": f"complete graph on {S0} rows over the DPG of the merged k-NN graph",
+           "dpg_secs": round(st["secs"], 4), "mode": "layered (greedy top, best-first base)",
+           "prec": "bf16 + FP32 re-rank", "width": args.search_width, "beams": []}
     for beam in (32, 48, 64, 96, 128, 192, 256):
-        ctx.search(Q[:1000], beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, mode=nnm.SEARCH_FLAT, seed=3)
-        ids, _, ss = ctx.search(Q, beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, mode=nnm.SEARCH_FLAT,
-                                seed=3)
+        ctx.search(Q[:1000], beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, entry=0)
+        ids, _, ss = ctx.search(Q, beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, entry=0)
         r = float(nnm.recall(ids, gt, qrows, 10))
         row = {"beam": beam, "recall@10": round(r, 5), "qps": round(Q.shape[0] / ss["secs"], 1),
                "kernel_secs": round(ss["secs"], 5), "dists_per_query": round(ss["dists"] / Q.shape[0], 1),
@@ -243,6 +251,7 @@ def main():
                     help="also measure batched graph search on the merged graph (DPG layer, seeded flat search)")
     ap.add_argument("--search-queries", type=int, default=100_000)
     ap.add_argument("--search-width", type=int, default=1)
+    ap.add_argument("--search-top", type=int, default=2048, help="rows of the complete top search layer")
     ap.add_argument("--summarize", metavar="DIR", nargs="+")
     ap.add_argument("--pick", action="append", default=[], metavar="REGEX@a:b",
                     help="with --summarize: dispatches of a kernel, sliced by order")
