@@ -106,9 +106,10 @@ def read_traffic(prec):
 
 
 def _slice(spec, n):
-    """Python slice 'a:b' (either side may be empty or negative) over n items."""
-    a, b = (spec.split(":") + [""])[:2]
-    return range(n)[slice(int(a) if a else None, int(b) if b else None)]
+    """Python slice 'a:b' or 'a:b:c' (parts may be empty or negative) over n items."""
+    parts = (spec.split(":") + ["", ""])[:3]
+    a, b, c = [int(x) if x else None for x in parts]
+    return range(n)[slice(a, b, c)]
 
 
 def summarize(dirs, picks):
