@@ -369,6 +369,37 @@ int nnm_search_use_graph(nnm_ctx* ctx, int source);
 int nnm_search(nnm_ctx* ctx, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* seeds,
                uint32_t* ids, float* dist, nnm_search_stats* st);
 
+/* ------------------------------------------------------------ file formats */
+/* Host-side readers and writers of the reference's files (no device work).
+ * Readers are two-phase (shape, then read into caller memory) and return
+ * NNM_ERR_INVALID for unreadable or malformed files. */
+
+/* Data matrix by extension: .txt "row col" then values
+ * (IODelegator::loadMatrix, src/iodelegator.cpp:186-220); .fvecs records
+ * [int32 d][d float32] (load_fvecs, :222-265); .bvecs records [int32 d][d
+ * uint8] as floats, at most 10M rows, empty records skipped (loadBvec10m,
+ * :319-385). */
+int nnm_io_matrix_shape(const char* path, uint64_t* rows, uint32_t* dim);
+int nnm_io_read_matrix(const char* path, float* out, uint64_t rows, uint32_t dim);
+int nnm_io_write_matrix(const char* path, const float* X, uint64_t rows, uint32_t dim);
+
+/* k-NN graph text "N K" then "i k id..." per row (IODelegator::loadKNNGraph,
+ * src/iodelegator.cpp:65-102; AbstractMerge::saveKGraph,
+ * src/abstractmerge.cpp:278-291).  nnm_io_read_knn keeps the first topk ids
+ * of each row. */
+int nnm_io_knn_shape(const char* path, uint64_t* rows, uint32_t* k);
+int nnm_io_read_knn(const char* path, uint32_t* ids, uint64_t rows, uint32_t topk);
+int nnm_io_write_knn(const char* path, const uint32_t* ids, uint64_t rows, uint32_t k);
+
+/* DPG / layered search graph text: blocks of "N" then "i deg id..." per row
+ * (LyNNDescent::saveDPGraph, src/lynndescent.cpp:625-638;
+ * IODelegator::loadDPGraph, src/iodelegator.cpp:1173-1217), upper layer
+ * first in a layered file (LySearch::loadGraphs, src/lysearch.cpp:112-186).
+ * rows / edges: per block; offs[g] holds rows[g] + 1 entries. */
+int nnm_io_layers_shape(const char* path, uint32_t max_layers, uint32_t* nlayers, uint32_t* rows, uint64_t* edges);
+int nnm_io_read_layers(const char* path, uint32_t nlayers, uint32_t* const* offs, uint32_t* const* adj);
+int nnm_io_write_layer(const char* path, int append, const uint32_t* offs, const uint32_t* adj, uint32_t rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,15 @@ _SIGS = {
     "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
                                  C.POINTER(Stats), C.POINTER(C.c_double)]),
     "nnm_graph_dist_error": (C.c_int, [_vp, C.POINTER(DistError)]),
+    "nnm_io_matrix_shape": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "nnm_io_read_matrix": (C.c_int, [C.c_char_p, _vp, C.c_uint64, C.c_uint32]),
+    "nnm_io_write_matrix": (C.c_int, [C.c_char_p, _vp, C.c_uint64, C.c_uint32]),
+    "nnm_io_knn_shape": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "nnm_io_read_knn": (C.c_int, [C.c_char_p, _vp, C.c_uint64, C.c_uint32]),
+    "nnm_io_write_knn": (C.c_int, [C.c_char_p, _vp, C.c_uint64, C.c_uint32]),
+    "nnm_io_layers_shape": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint32), _vp, _vp]),
+    "nnm_io_read_layers": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_vp), C.POINTER(_vp)]),
+    "nnm_io_write_layer": (C.c_int, [C.c_char_p, C.c_int, _vp, _vp, C.c_uint32]),
     "nnm_dpg_build": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(Stats)]),
     "nnm_dpg_get": (C.c_int, [_vp, _vp, _vp, _vp]),
     "nnm_search_set_layers": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
@@ -426,3 +435,62 @@ def recall(ids, gt, rows, k=10, unique=True):
             row = np.unique(row)
         hits += np.isin(row, G[i]).sum()
     return hits / (L.shape[0] * k)
+
+
+# ------------------------------------------------------------ file formats
+
+def _io_chk(rc, what):
+    if rc != NNM_OK:
+        raise NNMError(rc, what)
+
+
+def read_matrix(path, max_rows=None):
+    """.txt / .fvecs / .bvecs matrix as float32 (rows x dim)."""
+    r, d = C.c_uint64(), C.c_uint32()
+    _io_chk(lib().nnm_io_matrix_shape(path.encode(), C.byref(r), C.byref(d)), f"cannot read {path}")
+    rows = r.value if max_rows is None else min(r.value, max_rows)
+    X = np.empty((rows, d.value), np.float32)
+    _io_chk(lib().nnm_io_read_matrix(path.encode(), _ptr(X), rows, d.value), f"malformed {path}")
+    return X
+
+
+def write_matrix(path, X):
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    _io_chk(lib().nnm_io_write_matrix(path.encode(), _ptr(X), X.shape[0], X.shape[1]), f"cannot write {path}")
+
+
+def read_knn(path, topk=None):
+    r, k = C.c_uint64(), C.c_uint32()
+    _io_chk(lib().nnm_io_knn_shape(path.encode(), C.byref(r), C.byref(k)), f"cannot read {path}")
+    topk = k.value if topk is None else topk
+    ids = np.empty((r.value, topk), np.uint32)
+    _io_chk(lib().nnm_io_read_knn(path.encode(), _ptr(ids), r.value, topk), f"malformed {path}")
+    return ids
+
+
+def write_knn(path, ids):
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    _io_chk(lib().nnm_io_write_knn(path.encode(), _ptr(ids), ids.shape[0], ids.shape[1]), f"cannot write {path}")
+
+
+def read_layers(path, max_layers=8):
+    """DPG blocks of a (layered) graph file as [(offs, adj), ...]."""
+    nl = C.c_uint32()
+    rows = np.zeros(max_layers, np.uint32)
+    edges = np.zeros(max_layers, np.uint64)
+    _io_chk(lib().nnm_io_layers_shape(path.encode(), max_layers, C.byref(nl), _ptr(rows), _ptr(edges)),
+            f"cannot read {path}")
+    offs = [np.empty(int(rows[g]) + 1, np.uint32) for g in range(nl.value)]
+    adj = [np.empty(max(int(edges[g]), 1), np.uint32) for g in range(nl.value)]
+    po = (_vp * nl.value)(*[o.ctypes.data for o in offs])
+    pa = (_vp * nl.value)(*[a.ctypes.data for a in adj])
+    _io_chk(lib().nnm_io_read_layers(path.encode(), nl.value, po, pa), f"malformed {path}")
+    return [(offs[g], adj[g][:int(edges[g])]) for g in range(nl.value)]
+
+
+def write_layers(path, layers):
+    for g, (offs, adj) in enumerate(layers):
+        o = np.ascontiguousarray(offs, dtype=np.uint32)
+        a = np.ascontiguousarray(adj, dtype=np.uint32)
+        _io_chk(lib().nnm_io_write_layer(path.encode(), 1 if g else 0, _ptr(o), _ptr(a) if a.size else _ptr(o),
+                                         o.size - 1), f"cannot write {path}")

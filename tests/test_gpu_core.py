@@ -115,3 +115,32 @@ def test_bad_graph_rejected(ctx):
     with pytest.raises(nnm.NNMError) as e:
         ctx.set_graph(ids)  # row v < 10 contains itself
     assert e.value.code == nnm.NNM_ERR_INVALID
+
+
+def test_real_sift_rows_parity(ctx):
+    """Real SIFT descriptors (the reference's data/sift_query.txt, first 1000
+    rows): brute force bit-exact against the FP64 oracle, the reference's own
+    distance code bit-exact on pairs, NN-Descent recall within 0.005 of the
+    CPU restatement."""
+    import os
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sift_query_head1000.txt")
+    X = nnm.read_matrix(p)
+    ctx.set_data(X)
+    ids, dist = ctx.bruteforce(10)
+    oi, od = orc.bruteforce(X, 10)
+    assert np.array_equal(ids, oi)
+    assert np.array_equal(dist.astype(np.float64), od)  # integer rows: exact
+    a = np.arange(0, 1000, 7, dtype=np.uint32)
+    b = (a * 31 + 5) % 1000
+    pd = ctx.pair_dists(a, b)
+    ref = np.array([orc.l2_seq(X[i], X[j]) for i, j in zip(a, b)], np.float32)
+    assert np.array_equal(pd, ref)
+    ctx.random_init(10, seed=2)
+    ctx.nndescent(nnm.default_params(K=10, rule=nnm.RULE_ALL, seed=2, max_iters=20))
+    g, _, _ = ctx.get_graph()
+    rows = np.arange(1000, dtype=np.uint32)
+    r_gpu = nnm.recall(g, oi, rows, 10)
+    gi, _, _ = orc.kgraph(X, K=10, seed=2, max_iters=20)
+    r_cpu = orc.recall(gi, rows, oi, 10)
+    print("sift head recall gpu", r_gpu, "cpu", r_cpu)
+    assert r_gpu >= r_cpu - 0.005
