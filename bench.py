@@ -239,6 +239,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--keep", type=float, default=0.98)
     ap.add_argument("--plain-tail", type=int, default=3)
+    ap.add_argument("--learn-tau", type=float, default=0.0,
+                    help="learned: offer threshold of the learned phase (x estimated k-th; 0 = library default)")
     ap.add_argument("--no-renew", action="store_true", help="learned: keep the flags at the switch")
     ap.add_argument("--learned-iters", type=int, default=8,
                     help="learned: total iterations T of the learned run (learned phase + plain tail)")
@@ -317,6 +319,8 @@ def main():
         fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and not args.no_renew) else 0)
         p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name],
                                seed=seed, plain_tail=args.plain_tail, flags=fl)
+        if args.learn_tau > 0:
+            p.learn_tau = args.learn_tau
         train = None
         if name == "learned":
             ctx.graph_restore(recompute=True)
@@ -358,6 +362,7 @@ def main():
                "tile": [recs[-1][2]["tile_js"], recs[-1][2]["tile_chunk"]]}
         if train is not None:
             res["train"] = {k: round(v, 5) for k, v in train.items()}
+            res["learn_tau"] = round(float(p.learn_tau), 4)
         # dominant kernel: the packed short-list join, averaged per launch
         st = [c for _, _, c in recs]
         launches = sum(c["iters"] for c in st)

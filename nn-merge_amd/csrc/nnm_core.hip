@@ -207,6 +207,8 @@ extern "C" void nnm_default_params(nnm_params* p) {
     p->cut = 0;
     p->plain_tail = 1;
     p->flags = 0;
+    p->learn_tau = NNM_LEARN_TAU_DEFAULT;
+    p->reserved = 0;
     p->seed = 1;
 }
 
@@ -1225,128 +1227,150 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
     return acc;
 }
 
-// Number of keys in s[0..cnt) below key (s sorted ascending, in LDS).
-__device__ inline uint32_t lds_rank(const uint64_t* s, uint32_t cnt, uint64_t key) {
-    uint32_t lo = 0, hi = cnt;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+// 64-bit value of lane `src` (wave-uniform), broadcast through scalar registers.
+__device__ inline uint64_t wave_read64(uint64_t x, uint32_t src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Merge each vertex's slots into its list (UpdateKnnList, src/pvetex.h:40-85,
-// for a batch): one wave per vertex.  The candidates that can enter (below
-// the k-th key, or within the FP32 rounding window of it) are de-duplicated
-// against the list, sorted by the wave, and merged with the sorted list by
-// rank (binary searches in LDS).  Slots are reset to empty.  With rerank,
-// lower-bound keys are recomputed exactly first.
+// for a batch): one wave per vertex, list entry j and slot j in lane j.  The
+// candidates that can enter (below the k-th key, or within the FP32 rounding
+// window of it) are de-duplicated against the list.  Every key is then ranked
+// against the list and against the other candidates by loops over lanes read
+// through scalar registers; all keys are distinct (a list holds distinct ids,
+// an id always hashes to the same slot, candidates equal to a list id are
+// dropped), so the ranks give each kept entry its new position without a
+// sort.  Slots are reset to empty.  With rerank, lower-bound keys are
+// recomputed exactly first.  The next vertex's list and slots are loaded
+// while the current one is merged.
 __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X,
                          const uint16_t* __restrict__ Xb_exact, uint32_t ld, int rerank, float rerank_tau,
                          float tie_win, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
                          uint64_t* __restrict__ counters) {
-    __shared__ uint64_t lds_pool[4][NNM_WAVE], lds_cand[4][NNM_WAVE];
-    const uint32_t lane = nnm_lane(), wib = threadIdx.x / NNM_WAVE;
-    uint64_t* LP = lds_pool[wib];
-    uint64_t* LC = lds_cand[wib];
+    const uint32_t lane = nnm_lane();
+    const uint64_t W = (uint64_t)gridDim.x * (blockDim.x / NNM_WAVE);
     uint64_t c_upd = 0, c_flags = 0, c_rr = 0, c_ties = 0, c_swaps = 0;  // wave-uniform
-    NNM_WAVE_LOOP(v, n) {
-        uint64_t pkey = NNM_EMPTY_KEY;
-        uint32_t pid = 0xFFFFFFFFu, pf = 0;
+    auto fetch = [&](uint64_t u, uint32_t& id, float& d, uint32_t& f, uint64_t& sk) {
+        id = 0xFFFFFFFFu;
+        d = 0.f;
+        f = 0;
+        sk = NNM_EMPTY_KEY;
+        if (u >= n) return;
         if (lane < K) {
-            pid = g_ids[v * K + lane];
-            pkey = nnm_key(g_dist[v * K + lane], pid);
-            pf = g_flag[v * K + lane];
+            id = g_ids[u * K + lane];
+            d = g_dist[u * K + lane];
+            f = g_flag[u * K + lane];
         }
-        uint64_t skey = NNM_EMPTY_KEY;
-        if (lane < C) {
-            skey = slots[v * C + lane];
-            if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
-        }
-        const uint64_t kth_key = __shfl(pkey, (int)(K - 1), NNM_WAVE);
-        const float kd = nnm_key_dist(kth_key);
-        // cheap filter first: estimates against the margin, exact keys against
-        // the k-th key (or its rounding window)
-        bool valid = skey != NNM_EMPTY_KEY &&
-                     (rerank ? nnm_key_dist(skey) <= kd * rerank_tau
-                             : (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd * tie_win)));
-        if (!__ballot(valid)) {
-            c_flags += (uint32_t)__popcll(__ballot(lane < K && pf));
-            continue;
-        }
-        const uint32_t sid = nnm_key_id(skey);
-        for (uint32_t p = 0; p < K; ++p) {
-            // every lane takes part in the shuffle (a lane that skipped it
-            // would leave its value undefined for the others)
-            const uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
-            valid = valid && q != sid;
-        }
-        if (rerank) {
-            if (valid) {
-                const float ex = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
-                                          : nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
-                skey = nnm_key(ex, sid);
+        if (lane < C) sk = slots[u * C + lane];
+    };
+    uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    uint32_t pid, pf;
+    float pd;
+    uint64_t skey;
+    fetch(v, pid, pd, pf, skey);
+    for (; v < n; v += W) {
+        uint32_t nid, nf;
+        float nd;
+        uint64_t nskey;
+        fetch(v + W, nid, nd, nf, nskey);
+        if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
+        [&]() {
+            const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
+            const uint64_t kth_key = wave_read64(pkey, K - 1);
+            const float kd = nnm_key_dist(kth_key);
+            const uint32_t pflags = (uint32_t)__popcll(__ballot(lane < K && pf));
+            // cheap filter first: estimates against the margin, exact keys
+            // against the k-th key (or its rounding window)
+            bool valid = skey != NNM_EMPTY_KEY &&
+                         (rerank ? nnm_key_dist(skey) <= kd * rerank_tau
+                                 : (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd * tie_win)));
+            if (!__ballot(valid)) {
+                c_flags += pflags;
+                return;
             }
-            c_rr += (uint32_t)__popcll(__ballot(valid));
-            valid = valid && (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd * tie_win));
-        }
-        // sort the candidates; rank candidates and list entries against each other
-        const uint64_t ck = nnm_wave_sort64(valid ? skey : NNM_EMPTY_KEY);
-        const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
-        LP[lane] = pkey;
-        LC[lane] = ck;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const bool is_c = lane < nv;
-        uint32_t pos_p = lane + lds_rank(LC, nv, pkey);
-        uint32_t pos_s = lane + lds_rank(LP, K, ck);
-        if (tie_win > 1.f) {
-            // the entries ranked K-1 (last kept) and K (first dropped): when
-            // their FP32 distances are within rounding of each other the
-            // choice is re-made on FP64 distances (ties by id)
-            const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(is_c && pos_s == K - 1);
-            const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(is_c && pos_s == K);
-            if ((ap | as) && (bp | bs)) {
-                const bool a_pool = ap != 0, b_pool = bp != 0;
-                const int la = __ffsll((long long)(a_pool ? ap : as)) - 1;
-                const int lb = __ffsll((long long)(b_pool ? bp : bs)) - 1;
-                const uint64_t ka = __shfl(a_pool ? pkey : ck, la, NNM_WAVE);
-                const uint64_t kb = __shfl(b_pool ? pkey : ck, lb, NNM_WAVE);
-                if (nnm_key_dist(kb) <= nnm_key_dist(ka) * tie_win) {
-                    ++c_ties;
-                    const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
-                    const double ea = wave_l2_f64(X + v * ld, X + (uint64_t)ia * ld, ld);
-                    const double eb = wave_l2_f64(X + v * ld, X + (uint64_t)ib * ld, ld);
-                    if (eb < ea || (eb == ea && ib < ia)) {
-                        ++c_swaps;
-                        if ((int)lane == la) {
-                            if (a_pool) pos_p = K; else pos_s = K;
-                        }
-                        if ((int)lane == lb) {
-                            if (b_pool) pos_p = K - 1; else pos_s = K - 1;
+            const uint32_t sid = nnm_key_id(skey);
+            for (uint32_t p = 0; p < K; ++p) {
+                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p);
+                valid = valid && q != sid;
+            }
+            uint64_t ck = skey;
+            if (rerank) {
+                if (valid) {
+                    const float ex = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
+                                              : nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
+                    ck = nnm_key(ex, sid);
+                }
+                c_rr += (uint32_t)__popcll(__ballot(valid));
+                valid = valid && (ck < kth_key || (tie_win > 1.f && nnm_key_dist(ck) <= kd * tie_win));
+            }
+            const uint64_t vb = __ballot(valid);
+            if (!vb) {
+                c_flags += pflags;
+                return;
+            }
+            // ranks: a candidate's position is (list keys below it) + (candidate
+            // keys below it); a list entry's is its index + (candidate keys below it)
+            uint32_t below_list = 0, below_cand_c = 0, below_cand_p = 0;
+            for (uint32_t p = 0; p < K; ++p) below_list += wave_read64(pkey, p) < ck ? 1u : 0u;
+            for (uint64_t b = vb; b; b &= b - 1) {
+                const uint64_t ci = wave_read64(ck, (uint32_t)__builtin_ctzll(b));
+                below_cand_c += ci < ck ? 1u : 0u;
+                below_cand_p += ci < pkey ? 1u : 0u;
+            }
+            uint32_t pos_p = lane + below_cand_p;
+            uint32_t pos_s = below_list + below_cand_c;
+            if (tie_win > 1.f) {
+                // the entries ranked K-1 (last kept) and K (first dropped): when
+                // their FP32 distances are within rounding of each other the
+                // choice is re-made on FP64 distances (ties by id)
+                const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(valid && pos_s == K - 1);
+                const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(valid && pos_s == K);
+                if ((ap | as) && (bp | bs)) {
+                    const bool a_pool = ap != 0, b_pool = bp != 0;
+                    const uint32_t la = (uint32_t)__builtin_ctzll(a_pool ? ap : as);
+                    const uint32_t lb = (uint32_t)__builtin_ctzll(b_pool ? bp : bs);
+                    const uint64_t ka = wave_read64(a_pool ? pkey : ck, la);
+                    const uint64_t kb = wave_read64(b_pool ? pkey : ck, lb);
+                    if (nnm_key_dist(kb) <= nnm_key_dist(ka) * tie_win) {
+                        ++c_ties;
+                        const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
+                        const double ea = wave_l2_f64(X + v * ld, X + (uint64_t)ia * ld, ld);
+                        const double eb = wave_l2_f64(X + v * ld, X + (uint64_t)ib * ld, ld);
+                        if (eb < ea || (eb == ea && ib < ia)) {
+                            ++c_swaps;
+                            if (lane == la) {
+                                if (a_pool) pos_p = K; else pos_s = K;
+                            }
+                            if (lane == lb) {
+                                if (b_pool) pos_p = K - 1; else pos_s = K - 1;
+                            }
                         }
                     }
                 }
             }
-        }
-        const bool ins = is_c && pos_s < K;
-        const bool keep = lane < K && pos_p < K;
-        if (keep && pos_p != lane) {
-            g_ids[v * K + pos_p] = pid;
-            g_dist[v * K + pos_p] = nnm_key_dist(pkey);
-            g_flag[v * K + pos_p] = (uint8_t)pf;
-        }
-        if (ins) {
-            g_ids[v * K + pos_s] = nnm_key_id(ck);
-            g_dist[v * K + pos_s] = nnm_key_dist(ck);
-            g_flag[v * K + pos_s] = 1;
-        }
-        const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
-        c_upd += n_ins;
-        c_flags += (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
-        __builtin_amdgcn_wave_barrier();  // LDS reused by the next vertex
+            const bool ins = valid && pos_s < K;
+            const bool keep = lane < K && pos_p < K;
+            if (keep && pos_p != lane) {
+                g_ids[v * K + pos_p] = pid;
+                g_dist[v * K + pos_p] = pd;
+                g_flag[v * K + pos_p] = (uint8_t)pf;
+            }
+            if (ins) {
+                g_ids[v * K + pos_s] = nnm_key_id(ck);
+                g_dist[v * K + pos_s] = nnm_key_dist(ck);
+                g_flag[v * K + pos_s] = 1;
+            }
+            const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
+            c_upd += n_ins;
+            c_flags += (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
+        }();
+        pid = nid;
+        pd = nd;
+        pf = nf;
+        skey = nskey;
     }
     if (lane == 0) {
         if (c_upd) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)c_upd);

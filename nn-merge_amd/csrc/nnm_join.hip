@@ -495,8 +495,9 @@ __device__ inline float offer_key(float an, float bn, float g, int mode, float k
 }
 #define BJS 64         // bf16 window rows (one meta entry per lane)
 
+template <int U>
 struct Bf16Lds {
-    uint2 rows[(BJS + 16) * 32];   // (window + 16 slack rows) x 32 units of 8 bytes
+    uint2 rows[(BJS + 16) * U];    // (window + 16 slack rows) x U units of 8 bytes
     uint32_t meta[4][BJS];         // id, label, k-th distance bits, norm bits
 };
 
@@ -507,7 +508,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
     static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;  // dwords per row
-    __shared__ __align__(16) Bf16Lds L;
+    __shared__ __align__(16) Bf16Lds<U> L;
     const uint32_t js = min(a.js, (uint32_t)BJS);
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
@@ -655,8 +656,9 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
 // four waves.
 #define BLM 256
 
+template <int U>
 struct Bf16LongLds {
-    uint2 rows[(BLM + 16) * 32];
+    uint2 rows[(BLM + 16) * U];
     uint32_t meta[4][BLM];
 };
 
@@ -666,7 +668,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
     a.salt = join_salt(a);
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;
-    __shared__ __align__(16) Bf16LongLds L;
+    __shared__ __align__(16) Bf16LongLds<U> L;
     __shared__ uint64_t vshare;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
@@ -889,7 +891,9 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
             rl = nnm_mlp_dout(c);
             mode = KEY_LEARN;
             kscale = nnm_mlp_scale(c);
-            tau = 1.f;  // the lists hold estimates in the same metric during the learned phase
+            // the lists hold estimates in the same metric during the learned
+            // phase; pairs above learn_tau x the estimated k-th rarely enter
+            tau = p->learn_tau > 0.f ? p->learn_tau : NNM_LEARN_TAU_DEFAULT;
             if (!rows) return nnm_fail(c, NNM_ERR_STATE, "learned join without a projection");
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");

@@ -34,10 +34,11 @@ def test_default_params_are_reference_constants():
     p = nnm.default_params()
     assert p.K == 20 and p.rev_cap == 100 and abs(p.delta - 0.002) < 1e-9 and p.rho == 1.0
     assert p.max_iters == 0 and p.rule == nnm.RULE_ALL and p.prec == nnm.PREC_FP32
+    assert abs(p.learn_tau - 0.7) < 1e-6 and p.reserved == 0
 
 
 def test_params_struct_layout_matches_header():
-    # nnm_params: 12 x 4-byte fields then a uint64 seed (offset 48), size 56
-    assert ctypes.sizeof(nnm.Params) == 56
-    assert nnm.Params.seed.offset == 48
+    # nnm_params: 14 x 4-byte fields then a uint64 seed (offset 56), size 64
+    assert ctypes.sizeof(nnm.Params) == 64
+    assert nnm.Params.seed.offset == 56
     assert ctypes.sizeof(nnm.Stats) == 8 + 6 * 8 + 3 * 8 + 8 + 16 + 32 + 16 + 16 + 8
