@@ -462,8 +462,8 @@ __device__ inline void wave_write_sorted_row(uint64_t key, uint32_t flag, uint32
     uint32_t sid = nnm_key_id(s);
     uint32_t myf = 1;
     for (uint32_t p = 0; p < K; ++p) {
-        uint32_t pid = (uint32_t)__shfl(key, (int)p, NNM_WAVE);
-        uint32_t pf = __shfl(flag, (int)p, NNM_WAVE);
+        const uint32_t pid = wave_read32((uint32_t)key, p);
+        const uint32_t pf = wave_read32(flag, p);
         if (pid == sid) myf = pf;
     }
     if (lane < K) {
@@ -558,11 +558,19 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
         if (lane < K) {
             id = g_ids[v * K + lane];
             f = g_flag[v * K + lane];
-            const uint16_t* a = P + v * dout;
-            const uint16_t* b = P + (uint64_t)id * dout;
+            // rows of dout bf16 (a multiple of 8): 16-byte loads, products in k order
+            const uint4* a = reinterpret_cast<const uint4*>(P + v * dout);
+            const uint4* b = reinterpret_cast<const uint4*>(P + (uint64_t)id * dout);
             float dot = 0.f;
-            for (uint32_t k = 0; k < dout; ++k)
-                dot = __builtin_fmaf(__uint_as_float((uint32_t)a[k] << 16), __uint_as_float((uint32_t)b[k] << 16), dot);
+            for (uint32_t k = 0; k < dout / 8; ++k) {
+                const uint4 x = a[k], y = b[k];
+                const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    dot = __builtin_fmaf(__uint_as_float(xs[h] << 16), __uint_as_float(ys[h] << 16), dot);
+                    dot = __builtin_fmaf(__uint_as_float(xs[h] & 0xFFFF0000u), __uint_as_float(ys[h] & 0xFFFF0000u), dot);
+                }
+            }
             dd = fmaxf((pn[v] + pn[id]) - 2.f * dot, 0.f) * kscale;
         }
         wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
@@ -637,7 +645,7 @@ __global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __rest
                 cand = r < (sb - gb) ? gb + r : gb + r + ssize;
                 bool dup = false;
                 for (uint32_t p = 0; p < K; ++p) {
-                    uint32_t q = (uint32_t)__shfl((int)pid, (int)p, NNM_WAVE);
+                    const uint32_t q = wave_read32(pid, p);
                     dup |= (q == cand);
                 }
                 fresh = lane < draw && !dup;
@@ -690,7 +698,8 @@ __global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, 
         }
         if (dedup && lane >= K && lane < K + cut) {
             for (uint32_t p = 0; p < K; ++p) {
-                uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
+                // the list lanes are inactive here: read them through scalars
+                const uint32_t q = wave_read32(id, p);
                 if (q == id) key = NNM_EMPTY_KEY;
             }
         }
@@ -698,8 +707,8 @@ __global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, 
         uint32_t sid = nnm_key_id(s);
         uint32_t myf = 0;
         for (uint32_t p = 0; p < K + cut; ++p) {
-            uint32_t q = (uint32_t)__shfl((int)id, (int)p, NNM_WAVE);
-            uint32_t qf = __shfl(f, (int)p, NNM_WAVE);
+            const uint32_t q = wave_read32(id, p);
+            const uint32_t qf = wave_read32(f, p);
             if (q == sid) myf = qf;
         }
         if (lane < K) {
@@ -1227,12 +1236,9 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
     return acc;
 }
 
-// 64-bit value of lane `src` (wave-uniform), broadcast through scalar registers.
-__device__ inline uint64_t wave_read64(uint64_t x, uint32_t src) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)src);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)src);
-    return ((uint64_t)hi << 32) | lo;
-}
+// Candidate count above which the update sorts the candidates (a 21-step
+// bitonic network) instead of ranking them by a loop over the candidates.
+#define UPD_SORT_MIN 32u
 
 // Merge each vertex's slots into its list (UpdateKnnList, src/pvetex.h:40-85,
 // for a batch): one wave per vertex, list entry j and slot j in lane j.  The
@@ -1299,8 +1305,8 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             uint64_t ck = skey;
             if (rerank) {
                 if (valid) {
-                    const float ex = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
-                                              : nnm_l2_rows(X + v * ld, X + (uint64_t)sid * ld, ld);
+                    const float ex = Xb_exact ? nnm_l2_rows_bf16<1>(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
+                                              : nnm_l2_rows<1>(X + v * ld, X + (uint64_t)sid * ld, ld);
                     ck = nnm_key(ex, sid);
                 }
                 c_rr += (uint32_t)__popcll(__ballot(valid));
@@ -1314,11 +1320,26 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             // ranks: a candidate's position is (list keys below it) + (candidate
             // keys below it); a list entry's is its index + (candidate keys below it)
             uint32_t below_list = 0, below_cand_c = 0, below_cand_p = 0;
-            for (uint32_t p = 0; p < K; ++p) below_list += wave_read64(pkey, p) < ck ? 1u : 0u;
-            for (uint64_t b = vb; b; b &= b - 1) {
-                const uint64_t ci = wave_read64(ck, (uint32_t)__builtin_ctzll(b));
-                below_cand_c += ci < ck ? 1u : 0u;
-                below_cand_p += ci < pkey ? 1u : 0u;
+            const uint32_t nv = (uint32_t)__popcll(vb);
+            if (nv > UPD_SORT_MIN) {
+                // many candidates: sort them into lanes 0..nv-1 (a candidate's
+                // candidate rank is then its lane) and count per list entry
+                ck = nnm_wave_sort64(valid ? ck : NNM_EMPTY_KEY);
+                valid = lane < nv;
+                below_cand_c = lane;
+                for (uint32_t p = 0; p < K; ++p) {
+                    const uint64_t pk = wave_read64(pkey, p);
+                    below_list += pk < ck ? 1u : 0u;
+                    const uint32_t cnt = (uint32_t)__popcll(__ballot(valid && ck < pk));
+                    if (lane == p) below_cand_p = cnt;
+                }
+            } else {
+                for (uint32_t p = 0; p < K; ++p) below_list += wave_read64(pkey, p) < ck ? 1u : 0u;
+                for (uint64_t b = vb; b; b &= b - 1) {
+                    const uint64_t ci = wave_read64(ck, (uint32_t)__builtin_ctzll(b));
+                    below_cand_c += ci < ck ? 1u : 0u;
+                    below_cand_p += ci < pkey ? 1u : 0u;
+                }
             }
             uint32_t pos_p = lane + below_cand_p;
             uint32_t pos_s = below_list + below_cand_c;

@@ -237,15 +237,45 @@ __device__ inline uint32_t nnm_key_id(uint64_t k) { return (uint32_t)k; }
 
 __device__ inline uint32_t nnm_lane() { return __lane_id(); }
 
+// Value of lane `src` (wave-uniform) through a scalar register: reads the lane
+// whatever the exec mask, unlike a shuffle (whose inactive source lanes read 0).
+__device__ inline uint32_t wave_read32(uint32_t x, uint32_t src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)src);
+}
+__device__ inline uint64_t wave_read64(uint64_t x, uint32_t src) {
+    return ((uint64_t)wave_read32((uint32_t)(x >> 32), src) << 32) | wave_read32((uint32_t)x, src);
+}
+
 // The library's FP32 squared L2: elements in order 0..d-1, one accumulator,
 // each step acc = fma(a-b, a-b, acc).  Every kernel that produces a stored
 // distance uses this order, so one pair always gives the same bits.
+// B 16-byte loads of each row are issued before their (in-order) FMAs; any B
+// gives the same bits, a larger B hides more latency for more registers.
+template <int B = 8>
 __device__ inline float nnm_l2_rows(const float* __restrict__ a, const float* __restrict__ b,
                                     uint32_t d) {
     const float4* a4 = reinterpret_cast<const float4*>(a);
     const float4* b4 = reinterpret_cast<const float4*>(b);
     float acc = 0.f;
-    for (uint32_t i = 0; i < d / 4; ++i) {
+    const uint32_t n4 = d / 4;
+    uint32_t i = 0;
+    for (; i + B <= n4; i += B) {
+        float4 x[B], y[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            x[u] = a4[i + u];
+            y[u] = b4[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            float t;
+            t = x[u].x - y[u].x; acc = __builtin_fmaf(t, t, acc);
+            t = x[u].y - y[u].y; acc = __builtin_fmaf(t, t, acc);
+            t = x[u].z - y[u].z; acc = __builtin_fmaf(t, t, acc);
+            t = x[u].w - y[u].w; acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    for (; i < n4; ++i) {
         float4 x = a4[i], y = b4[i];
         float t;
         t = x.x - y.x; acc = __builtin_fmaf(t, t, acc);
@@ -253,8 +283,8 @@ __device__ inline float nnm_l2_rows(const float* __restrict__ a, const float* __
         t = x.z - y.z; acc = __builtin_fmaf(t, t, acc);
         t = x.w - y.w; acc = __builtin_fmaf(t, t, acc);
     }
-    for (uint32_t i = (d / 4) * 4; i < d; ++i) {
-        float t = a[i] - b[i];
+    for (uint32_t j = n4 * 4; j < d; ++j) {
+        float t = a[j] - b[j];
         acc = __builtin_fmaf(t, t, acc);
     }
     return acc;
@@ -263,12 +293,13 @@ __device__ inline float nnm_l2_rows(const float* __restrict__ a, const float* __
 // Exact FP32 distance of rows a, b in the canonical order from bf16 rows:
 // used where every value is exact in bf16 (integer rows), where it equals
 // nnm_l2_rows on the FP32 rows bit for bit while reading half the bytes.
+template <int B = 8>
 __device__ inline float nnm_l2_rows_bf16(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, uint32_t ld) {
     const uint4* a4 = reinterpret_cast<const uint4*>(a);
     const uint4* b4 = reinterpret_cast<const uint4*>(b);
     float acc = 0.f;
-    for (uint32_t i = 0; i < ld / 8; ++i) {
-        const uint4 x = a4[i], y = b4[i];
+    const uint32_t n8 = ld / 8;
+    auto step = [&](const uint4 x, const uint4 y) {
         const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
@@ -277,7 +308,19 @@ __device__ inline float nnm_l2_rows_bf16(const uint16_t* __restrict__ a, const u
             t = __uint_as_float(xs[h] & 0xFFFF0000u) - __uint_as_float(ys[h] & 0xFFFF0000u);
             acc = __builtin_fmaf(t, t, acc);
         }
+    };
+    uint32_t i = 0;
+    for (; i + B <= n8; i += B) {
+        uint4 x[B], y[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            x[u] = a4[i + u];
+            y[u] = b4[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) step(x[u], y[u]);
     }
+    for (; i < n8; ++i) step(a4[i], b4[i]);
     return acc;
 }
 

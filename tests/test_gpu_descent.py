@@ -180,3 +180,38 @@ def test_join_distances_exact(ctx, d):
     # integer rows: the canonical order is exact
     ex = ((X[v].astype(np.float64) - X[ids.ravel()].astype(np.float64)) ** 2).sum(1)
     assert np.array_equal(ex, dist.ravel().astype(np.float64))
+
+
+def test_merge_finish_matches_merge2one_with_dedup(ctx):
+    """merge2one (src/abstractmerge.cpp:232-277) with the documented de-dup:
+    the parked far entries of each built list (cut = K/2 of them) are merged
+    back with the final pool by (dist, id), parked ids already in the pool
+    dropped.  The expected rows are computed here from the pool read before
+    the finish and the parked entries read after set-up.  The descent joins
+    every pair, so pools re-find many parked ids and the de-dup path runs."""
+    X = nnm_data.sift_like(6000, 64, centers=8, seed=21)
+    K, cut = 16, 8
+    b, sub, _ = _build_halves(ctx, X, K, 5, 2)
+    ctx.set_partition(b, [1, 1], [0, 0])
+    ctx.set_graph(sub)
+    li, ld, _ = ctx.get_graph()  # sorted, exact distances
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, seed=8, max_iters=6, cut=cut)
+    ctx.merge_init(p)
+    # pairs inside a built subgraph are joined too, so pools can re-find
+    # parked ids (under the merge rule they never can)
+    ctx.nndescent(nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=8, max_iters=6, cut=cut))
+    pi, pd, _ = ctx.get_graph()
+    ctx.merge_finish(True)
+    gi, gd, _ = ctx.get_graph()
+    _invariants(gi, gd, K)
+    park_i, park_d = li[:, K - cut:], ld[:, K - cut:]
+    overlap = 0
+    for v in range(X.shape[0]):
+        keep = ~np.isin(park_i[v], pi[v])
+        overlap += int((~keep).sum())
+        ids = np.concatenate([pi[v], park_i[v][keep]])
+        ds = np.concatenate([pd[v], park_d[v][keep]])
+        o = np.lexsort((ids, ds))[:K]
+        assert np.array_equal(gi[v], ids[o]) and np.array_equal(gd[v], ds[o]), f"row {v}"
+    print("parked ids re-found by the pool:", overlap)
+    assert overlap > 1000
