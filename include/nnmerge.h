@@ -70,6 +70,18 @@ extern "C" {
 #define NNM_RERANK_ALWAYS 1
 #define NNM_RERANK_NEVER 2
 
+/* Distance metrics (SpaceInterface, src/abstractgraph.h:14-19; the spaces
+ * of src/space_lp.h:9-53 and the routines of src/distopt.cpp).  Each is
+ * evaluated in FP32 with the reference's operation order: one accumulator,
+ * elements 0..d-1, every multiply and add rounded separately (except the
+ * library's L2, which uses fused multiply-add, exact on integer rows). */
+#define NNM_METRIC_L2 0     /* squared L2: VMath::l2norm, src/vmath.cpp:56-88 */
+#define NNM_METRIC_L1 1     /* sum |a-b|: DistOpt::l1norm, src/distopt.cpp:82-110 */
+#define NNM_METRIC_COSINE 2 /* 2 - 2 a.b / (|a| |b|), 2 when a norm is 0:
+                               DistOpt::cosine, src/distopt.cpp:113-160 */
+#define NNM_METRIC_CHI2 3   /* sum (a-b)^2 / (a+b) over elements with a or b
+                               nonzero: DistOpt::KSquare, src/distopt.cpp:163-184 */
+
 typedef struct nnm_ctx nnm_ctx;
 
 /* NN-Descent parameters.  Defaults (nnm_default_params) equal the reference
@@ -161,6 +173,15 @@ uint64_t nnm_mem_used(const nnm_ctx* ctx);
  * Replaces the matrix every distance call indexes through DISTFUNC
  * (src/abstractgraph.h:11-12).  Resets graph and partition state. */
 int nnm_set_data(nnm_ctx* ctx, const float* X, uint64_t n, uint32_t d);
+
+/* Metric of every later distance (NNM_METRIC_*; the default is L2).  The
+ * choice of the space object in the reference (L2space / L1space / Ksquare,
+ * src/space_lp.h:9-53, passed as SpaceInterface to the graph classes).
+ * Metrics other than L2 run the FP32 join (NNM_PREC_FP32), random and merge
+ * set-up, list recompute, brute force and pair distances; the BF16 and
+ * learned joins, search, DPG and the FP64 error report return
+ * NNM_ERR_UNSUPPORTED for them. */
+int nnm_set_metric(nnm_ctx* ctx, uint32_t metric);
 
 /* Partition of the rows into nsub contiguous subgraphs: subgraph s covers
  * rows [sub_begin[s], sub_begin[s+1]) (sub_begin has nsub+1 entries, the

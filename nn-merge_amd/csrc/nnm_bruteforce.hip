@@ -9,8 +9,9 @@
 // top-KMAX list in registers; database rows stream through LDS in tiles of
 // BT rows, read by every lane of the block at the same address.  The
 // database is split into nsplit ranges over blockIdx.y; a second kernel
-// merges the per-range lists.  Distances use the library order (0..d-1,
-// fused multiply-add), so they match nnm_l2_rows bit for bit.
+// merges the per-range lists.  Distances use the library order (0..d-1; L2
+// with fused multiply-add, the other metrics with every operation rounded on
+// its own), so they match the list kernels bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,11 +42,12 @@ __device__ inline void bf_insert(float (&td)[KM], uint32_t (&ti)[KM], float d, u
     }
 }
 
-template <int LDMAX>
+template <int LDMAX, uint32_t M>
 __global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X, uint32_t ld, uint64_t n,
                                                     const float* __restrict__ Q, const uint32_t* __restrict__ rows,
                                                     uint64_t nq, int exclude_self, uint32_t nsplit,
-                                                    float* __restrict__ part_d, uint32_t* __restrict__ part_i) {
+                                                    float* __restrict__ part_d, uint32_t* __restrict__ part_i,
+                                                    const float* __restrict__ sumsq) {
     __shared__ __align__(16) float tile[BF_BT * LDMAX];
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t split = blockIdx.y;
@@ -74,6 +76,11 @@ __global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X,
             qv[c] = t.x; qv[c + 1] = t.y; qv[c + 2] = t.z; qv[c + 3] = t.w;
         }
     }
+    float qs = 0.f;  // cosine: the query's squared norm, in the metric's order
+    if constexpr (M == NNM_METRIC_COSINE) {
+#pragma unroll
+        for (int c = 0; c < LDMAX; ++c) qs = metric_step<M>(qs, qv[c], qv[c]);
+    }
     float td[BF_KMAX];
     uint32_t ti[BF_KMAX];
 #pragma unroll
@@ -97,14 +104,15 @@ __global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X,
             for (int c = 0; c < LDMAX; c += 4) {
                 if ((uint32_t)c < ld) {
                     const float4 y = *reinterpret_cast<const float4*>(tr + c);
-                    float t = qv[c] - y.x; acc = __builtin_fmaf(t, t, acc);
-                    t = qv[c + 1] - y.y; acc = __builtin_fmaf(t, t, acc);
-                    t = qv[c + 2] - y.z; acc = __builtin_fmaf(t, t, acc);
-                    t = qv[c + 3] - y.w; acc = __builtin_fmaf(t, t, acc);
+                    acc = metric_step<M>(acc, qv[c], y.x);
+                    acc = metric_step<M>(acc, qv[c + 1], y.y);
+                    acc = metric_step<M>(acc, qv[c + 2], y.z);
+                    acc = metric_step<M>(acc, qv[c + 3], y.w);
                 }
             }
             const uint32_t id = (uint32_t)(r0 + rr);
             if (id == self) continue;
+            if constexpr (M == NNM_METRIC_COSINE) acc = nnm_cos_finish(acc, qs, sumsq[id]);
             bf_insert(td, ti, acc, id);
         }
     }
@@ -178,15 +186,22 @@ extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* ro
         return rc;
     }
     dim3 grid((uint32_t)qblocks, nsplit);
-    if (c->ld <= 64)
-        hipLaunchKernelGGL(k_bruteforce<64>, grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,
-                           exclude_self, nsplit, pd, pi);
-    else if (c->ld <= 128)
-        hipLaunchKernelGGL(k_bruteforce<128>, grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,
-                           exclude_self, nsplit, pd, pi);
-    else
-        hipLaunchKernelGGL(k_bruteforce<256>, grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,
-                           exclude_self, nsplit, pd, pi);
+    if (c->metric == NNM_METRIC_COSINE && !c->sumsq) { cleanup(); return nnm_fail(c, NNM_ERR_STATE, "no row norms"); }
+#define NNM_BF(LDM, M)                                                                                         \
+    hipLaunchKernelGGL((k_bruteforce<LDM, M>), grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,   \
+                       exclude_self, nsplit, pd, pi, c->sumsq)
+#define NNM_BF_LD(M)                       \
+    if (c->ld <= 64) NNM_BF(64, M);        \
+    else if (c->ld <= 128) NNM_BF(128, M); \
+    else NNM_BF(256, M);
+    switch (c->metric) {
+        case NNM_METRIC_L1: NNM_BF_LD(NNM_METRIC_L1) break;
+        case NNM_METRIC_COSINE: NNM_BF_LD(NNM_METRIC_COSINE) break;
+        case NNM_METRIC_CHI2: NNM_BF_LD(NNM_METRIC_CHI2) break;
+        default: NNM_BF_LD(NNM_METRIC_L2) break;
+    }
+#undef NNM_BF_LD
+#undef NNM_BF
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { cleanup(); return nnm_hip_fail(c, e, "k_bruteforce launch"); }
     hipLaunchKernelGGL(k_bf_merge, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, c->stream, nq, nsplit, k, pd,

@@ -330,6 +330,34 @@ int nnm_ensure_bf16(nnm_ctx* c) {
     return NNM_OK;
 }
 
+// Squared row norms for the cosine metric, in the metric's own order.
+__global__ void k_sumsq(const float* __restrict__ X, uint32_t ld, uint64_t n, float* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = nnm_sumsq_rows(X + i * ld, ld);
+}
+
+static int ensure_sumsq(nnm_ctx* c) {
+    if (c->metric != NNM_METRIC_COSINE || !c->X || c->sumsq) return NNM_OK;
+    NNM_TRY(alloc_n(c, &c->sumsq, c->n));
+    hipLaunchKernelGGL(k_sumsq, dim3(grid_for(c->n, 256, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, c->n,
+                       c->sumsq);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+extern "C" int nnm_set_metric(nnm_ctx* c, uint32_t metric) {
+    if (!c) return NNM_ERR_INVALID;
+    if (metric > NNM_METRIC_CHI2) return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_metric: unknown metric");
+    NNM_HIP(c, hipSetDevice(c->device));
+    c->metric = metric;
+    return ensure_sumsq(c);
+}
+
+int nnm_require_l2(nnm_ctx* c, const char* what) {
+    if (c->metric == NNM_METRIC_L2) return NNM_OK;
+    return nnm_fail(c, NNM_ERR_UNSUPPORTED, std::string(what) + ": L2 metric only");
+}
+
 extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) {
     if (!c || !X || n < 2 || d == 0 || d > 1024) return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_data: bad arguments");
     if (n >= 0xFFFFFFFFull) return nnm_fail(c, NNM_ERR_INVALID, "nnm_set_data: n must be < 2^32-1");
@@ -344,11 +372,13 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
     nnm_free(c, c->X); c->X = nullptr;
     nnm_free(c, c->Xb); c->Xb = nullptr;
     nnm_free(c, c->norms); c->norms = nullptr;
+    nnm_free(c, c->sumsq); c->sumsq = nullptr;
     c->n = n;
     c->d = d;
     c->ld = (d + 31) / 32 * 32;
     NNM_TRY(alloc_n(c, &c->X, n * c->ld));
     NNM_TRY(nnm_h2d_rows(c, c->X, c->ld, X, d, n));
+    NNM_TRY(ensure_sumsq(c));
     // bf16 exactness: integers of magnitude <= 256 are exact in bf16, and
     // with d <= 128 every partial sum of the norm form stays below 2^24
     int exact = (d <= 128) ? 1 : 0;
@@ -478,7 +508,8 @@ __device__ inline void wave_write_sorted_row(uint64_t key, uint32_t flag, uint32
 __device__ inline void wave_random_row(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                        uint64_t v, uint32_t gb,
                                        uint32_t gsize, uint32_t K, uint64_t seed, uint32_t* __restrict__ ids,
-                                       float* __restrict__ dist, uint8_t* __restrict__ flg, uint64_t* nd) {
+                                       float* __restrict__ dist, uint8_t* __restrict__ flg, uint64_t* nd,
+                                       const nnm_metric_args& mt) {
     const uint32_t lane = nnm_lane();
     uint32_t r = nnm_wave_rnd_sds(seed, v, K + 1, gsize);
     uint32_t id = gb + r;
@@ -498,7 +529,7 @@ __device__ inline void wave_random_row(const float* __restrict__ X, const uint16
     (void)take;
     uint32_t myid = (uint32_t)__shfl((int)id, (int)src_lane, NNM_WAVE);
     float dd = 0.f;
-    if (lane < K) dd = nnm_row_dist(X, Xb, ld, v, myid);
+    if (lane < K) dd = nnm_row_dist(X, Xb, ld, v, myid, mt);
     uint64_t key = nnm_key(dd, myid);
     if (lane < K) *nd += 1;
     wave_write_sorted_row(key, 1u, K, ids, dist, flg);
@@ -508,12 +539,13 @@ __global__ void k_random_init(const float* __restrict__ X, uint32_t ld, uint64_t
                               const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_grp,
                               const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
                               uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
-                              uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters) {
+                              uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters, nnm_metric_args mt) {
     uint64_t nd = 0;
     NNM_WAVE_LOOP(v, n) {
         const uint32_t g = sub_grp[vsub[v]];
         const uint32_t gb = grp_begin[g], gsize = grp_end[g] - gb;
-        wave_random_row(X, nullptr, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd);
+        wave_random_row(X, nullptr, ld, v, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K, &nd,
+                        mt);
     }
     wave_count(counters + CNT_DISTS, nd);
 }
@@ -524,7 +556,7 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
                             uint32_t K, int recompute, int have_flags, const uint32_t* __restrict__ skip_raw,
                             uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                             uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters,
-                            uint32_t* __restrict__ bad) {
+                            uint32_t* __restrict__ bad, nnm_metric_args mt) {
     const uint32_t lane = nnm_lane();
     uint64_t nd = 0;
     NNM_WAVE_LOOP(v, n) {
@@ -535,7 +567,7 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
         if (lane < K) {
             id = g_ids[v * K + lane];
             if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
-            dd = recompute ? nnm_row_dist(X, Xb, ld, v, id) : g_dist[v * K + lane];
+            dd = recompute ? nnm_row_dist(X, Xb, ld, v, id, mt) : g_dist[v * K + lane];
             f = have_flags ? g_flag[v * K + lane] : 1u;
             nd += recompute ? 1 : 0;
         }
@@ -609,7 +641,8 @@ __global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __rest
                              uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                              uint8_t* __restrict__ g_flag, uint32_t* __restrict__ c_ids,
                              float* __restrict__ c_dist, uint8_t* __restrict__ c_flag,
-                             uint8_t* __restrict__ has_cpn, uint64_t* __restrict__ counters) {
+                             uint8_t* __restrict__ has_cpn, uint64_t* __restrict__ counters,
+                             nnm_metric_args mt) {
     const uint32_t lane = nnm_lane();
     uint64_t nd = 0;
     NNM_WAVE_LOOP(v, n) {
@@ -621,7 +654,7 @@ __global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __rest
         uint8_t* flg = g_flag + v * K;
         if (!sub_built[s]) {
             if (lane == 0) has_cpn[v] = 0;
-            wave_random_row(X, Xb, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd);
+            wave_random_row(X, Xb, ld, v, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd, mt);
         } else {
             const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
             const uint32_t remain = K - cut;
@@ -664,7 +697,7 @@ __global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __rest
                 uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
                 if (lane >= remain && lane < remain + fresh_n) {
                     pid = got;
-                    pd = nnm_row_dist(X, Xb, ld, got, v);
+                    pd = nnm_row_dist(X, Xb, ld, got, v, mt);
                     pf = 1;
                     nd += 1;
                 }
@@ -720,10 +753,11 @@ __global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, 
 }
 
 __global__ void k_pair_dists(const float* __restrict__ X, uint32_t ld, const uint32_t* __restrict__ a,
-                             const uint32_t* __restrict__ b, uint64_t m, float* __restrict__ out) {
+                             const uint32_t* __restrict__ b, uint64_t m, float* __restrict__ out,
+                             nnm_metric_args mt) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    out[i] = nnm_l2_rows(X + (uint64_t)a[i] * ld, X + (uint64_t)b[i] * ld, ld);
+    out[i] = nnm_row_dist(X, nullptr, ld, a[i], b[i], mt);
 }
 
 // ================================================================ graph API
@@ -772,7 +806,7 @@ extern "C" int nnm_graph_random_init(nnm_ctx* c, uint32_t K, uint64_t seed, nnm_
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_random_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld,
                        c->n_act, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
-                       c->g_flag, c->counters);
+                       c->g_flag, c->counters, metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
@@ -800,7 +834,8 @@ extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist,
     uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, nullptr, c->ld, c->n,
-                       K, dist ? 0 : 1, flags ? 1 : 0, nullptr, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+                       K, dist ? 0 : 1, flags ? 1 : 0, nullptr, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
+                       metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
@@ -858,13 +893,14 @@ extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
         // InitSGraph (src/abstractmerge.cpp:55-76) recomputes the built
         // subgraphs' distances; raw rows are left to nnm_merge_init
         const uint16_t* xb = nullptr;
-        if (recompute == 2 && c->data_exact_bf16) {
+        if (recompute == 2 && c->data_exact_bf16 && c->metric == NNM_METRIC_L2) {
             NNM_TRY(nnm_ensure_bf16(c));
             xb = c->Xb;
         }
         uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
         hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld,
-                           c->n_act, c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad);
+                           c->n_act, c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
+                           metric_args(c));
         NNM_LAUNCH_CHECK(c);
     }
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -894,7 +930,7 @@ extern "C" int nnm_pair_dists(nnm_ctx* c, const uint32_t* a, const uint32_t* b, 
     NNM_TRY(nnm_h2d(c, da, a, m * 4));
     NNM_TRY(nnm_h2d(c, db, b, m * 4));
     hipLaunchKernelGGL(k_pair_dists, dim3(grid_for(m, 256, 1u << 30)), dim3(256), 0, c->stream, c->X, c->ld, da,
-                       db, m, dout);
+                       db, m, dout, metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_TRY(nnm_d2h(c, out, dout, m * 4));
     NNM_HIP(c, hipStreamSynchronize(c->stream));
@@ -930,7 +966,7 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
         NNM_TRY(alloc_n(c, &c->has_cpn, c->n));
     }
     const uint16_t* xb = nullptr;  // bf16 rows where exact, for the bf16 / learned precisions
-    if (p->prec != NNM_PREC_FP32 && c->data_exact_bf16) {
+    if (p->prec != NNM_PREC_FP32 && c->data_exact_bf16 && c->metric == NNM_METRIC_L2) {
         NNM_TRY(nnm_ensure_bf16(c));
         xb = c->Xb;
     }
@@ -939,7 +975,7 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
     hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld, c->n_act,
                        K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin, c->grp_end,
                        p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
-                       c->c_flag, c->has_cpn, c->counters);
+                       c->c_flag, c->has_cpn, c->counters, metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
@@ -1404,6 +1440,7 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
 
 float nnm_near_tie_window(nnm_ctx* c, const nnm_params* p) {
     if (p->prec == NNM_PREC_LEARNED) return 1.f;  // learned estimates: no FP64 decision
+    if (c->metric != NNM_METRIC_L2) return 1.f;  // the FP64 re-decision computes L2
     if (c->data_exact_bf16) return 1.f;  // integer rows: every distance is exact
     return 1.f + 2.f * (float)(c->d + 2) * 5.9604645e-8f;
 }
@@ -1442,6 +1479,7 @@ __global__ void k_dist_error(uint64_t n, uint32_t K, const float* __restrict__ X
 extern "C" int nnm_graph_dist_error(nnm_ctx* c, nnm_dist_error* e) {
     if (!c || !e) return NNM_ERR_INVALID;
     if (!c->X || !c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no data or graph");
+    NNM_TRY(nnm_require_l2(c, "nnm_graph_dist_error"));
     unsigned long long* acc = nullptr;
     NNM_TRY(nnm_alloc(c, (void**)&acc, 4 * 8));
     int rc = fill_u64(c, reinterpret_cast<uint64_t*>(acc), 4, 0);
@@ -1539,7 +1577,11 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     NNM_TRY(fill_u32(c, c->rcnt, 2 * n, 0));
     NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
     NNM_TRY(fill_u32(c, c->ovf_bits, (2 * n + 31) / 32 + 1, 0));
-    const float* nrm = p->prec == NNM_PREC_BF16 ? c->norms : (p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c) : nullptr);
+    // row norms in the join's metadata: bf16 norms (norm form), learned
+    // projection norms, or the cosine metric's squared norms
+    const float* nrm = p->prec == NNM_PREC_BF16 ? c->norms
+                       : p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c)
+                       : c->metric == NNM_METRIC_COSINE ? c->sumsq : nullptr;
     hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->label,
                        nrm, nnm_near_tie_window(c, p), c->kth, c->vmeta);
     NNM_LAUNCH_CHECK(c);
@@ -1693,6 +1735,7 @@ static int capture_iteration(nnm_ctx* c, const nnm_params* p, uint32_t js, uint3
 
 static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool single, nnm_stats* st) {
     NNM_TRY(validate_params(c, p));
+    if (p->prec != NNM_PREC_FP32) NNM_TRY(nnm_require_l2(c, "BF16 and learned joins"));
     NNM_TRY(nnm_ensure_iter_buffers(c, p));
     // learned prefilter: needs a trained model; otherwise the plain BF16 join
     // runs every iteration (the fallback)

@@ -292,6 +292,7 @@ extern "C" int nnm_dpg_build(nnm_ctx* c, uint32_t M, uint32_t maxM, int carry, n
     if (!c) return NNM_ERR_INVALID;
     if (st) std::memset(st, 0, sizeof(*st));
     if (!c->X || !c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "nnm_dpg_build needs data and a k-NN graph");
+    NNM_TRY(nnm_require_l2(c, "nnm_dpg_build"));
     const uint32_t K = c->K, n = (uint32_t)c->n_act;
     if (K == 0 || K > DPG_MAXK) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "DPG supports K <= 64");
     if (M == 0 || maxM == 0 || M > DPG_MAXCAP || maxM > DPG_MAXCAP)

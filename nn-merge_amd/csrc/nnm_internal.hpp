@@ -71,6 +71,8 @@ struct nnm_ctx {
     uint16_t* Xb = nullptr;   // bf16 copy (lazy)
     float* norms = nullptr;   // squared norms of the bf16 rows (lazy)
     int data_exact_bf16 = 0;  // every value an integer in [-256, 256]
+    uint32_t metric = NNM_METRIC_L2;
+    float* sumsq = nullptr;   // cosine: squared row norms (with the data)
 
     // partition
     uint32_t nsub = 0, ngrp = 0;
@@ -158,6 +160,7 @@ int nnm_ensure_bf16(nnm_ctx* c);
 // candidates within rounding of the k-th entry reach the FP64 tie test; 1
 // where distances are exact (integer rows) and for learned estimates.
 float nnm_near_tie_window(nnm_ctx* c, const nnm_params* p);
+int nnm_require_l2(nnm_ctx* c, const char* what);
 int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
 
 #define NNM_HIP(c, expr)                                  \
@@ -329,6 +332,125 @@ __device__ inline float nnm_l2_rows_bf16(const uint16_t* __restrict__ a, const u
 __device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                      uint64_t a, uint64_t b) {
     return Xb ? nnm_l2_rows_bf16(Xb + a * ld, Xb + b * ld, ld) : nnm_l2_rows(X + a * ld, X + b * ld, ld);
+}
+
+// ---- the other metrics: DistOpt's operation order (src/distopt.cpp), one
+// accumulator over elements 0..d-1, every multiply, divide and add rounded on
+// its own (no contraction).  Rows are zero padded to d = ld; a padded element
+// adds an exact zero (L1, cosine) or is skipped (chi-square), so the padded
+// loops give the unpadded values bit for bit.
+
+// DistOpt::l1norm, src/distopt.cpp:82-110.
+__device__ inline float nnm_l1_rows(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
+#pragma clang fp contract(off)
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    float res = 0.f;
+    for (uint32_t i = 0; i < d / 4; ++i) {
+        const float4 x = a4[i], y = b4[i];
+        res = res + fabsf(x.x - y.x);
+        res = res + fabsf(x.y - y.y);
+        res = res + fabsf(x.z - y.z);
+        res = res + fabsf(x.w - y.w);
+    }
+    return res;
+}
+
+// DistOpt::KSquare, src/distopt.cpp:163-184: (a-b)^2 / (a+b) where a or b is
+// nonzero.
+__device__ inline float nnm_chi2_step(float w2, float x, float y) {
+#pragma clang fp contract(off)
+    const float w1 = x - y;
+    return (x != 0.f || y != 0.f) ? w2 + (w1 * w1) / (x + y) : w2;
+}
+__device__ inline float nnm_chi2_rows(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    float w2 = 0.f;
+    for (uint32_t i = 0; i < d / 4; ++i) {
+        const float4 x = a4[i], y = b4[i];
+        w2 = nnm_chi2_step(w2, x.x, y.x);
+        w2 = nnm_chi2_step(w2, x.y, y.y);
+        w2 = nnm_chi2_step(w2, x.z, y.z);
+        w2 = nnm_chi2_step(w2, x.w, y.w);
+    }
+    return w2;
+}
+
+// DistOpt::cosine, src/distopt.cpp:113-160, split in two: the dot product
+// res and the squared norms l1, l2 are three separate accumulators there, so
+// the norms are computed once per row (nnm_sumsq_rows) and combined with the
+// dot product by nnm_cos_finish.
+__device__ inline float nnm_dot_rows(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
+#pragma clang fp contract(off)
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    float res = 0.f;
+    for (uint32_t i = 0; i < d / 4; ++i) {
+        const float4 x = a4[i], y = b4[i];
+        res = res + x.x * y.x;
+        res = res + x.y * y.y;
+        res = res + x.z * y.z;
+        res = res + x.w * y.w;
+    }
+    return res;
+}
+__device__ inline float nnm_sumsq_rows(const float* __restrict__ a, uint32_t d) { return nnm_dot_rows(a, a, d); }
+__device__ inline float nnm_cos_finish(float dot, float l1, float l2) {
+#pragma clang fp contract(off)
+    const float r = (l1 != 0.f && l2 != 0.f) ? dot / (sqrtf(l1) * sqrtf(l2)) : 0.f;
+    return 2.f - 2.f * r;
+}
+
+// Metric of a distance call: which function, and for cosine the squared row
+// norms (sumsq, one per row).
+struct nnm_metric_args {
+    uint32_t metric;
+    const float* sumsq;
+};
+inline nnm_metric_args metric_args(const nnm_ctx* c) { return nnm_metric_args{c->metric, c->sumsq}; }
+
+__device__ inline float nnm_metric_rows(uint32_t metric, const float* __restrict__ a, const float* __restrict__ b,
+                                        uint32_t d, float sa, float sb) {
+    switch (metric) {
+        case NNM_METRIC_L1: return nnm_l1_rows(a, b, d);
+        case NNM_METRIC_COSINE: return nnm_cos_finish(nnm_dot_rows(a, b, d), sa, sb);
+        case NNM_METRIC_CHI2: return nnm_chi2_rows(a, b, d);
+        default: return nnm_l2_rows(a, b, d);
+    }
+}
+
+// One element of a distance accumulation under metric M (the library's L2
+// with fused multiply-add; the others with every operation rounded on its
+// own, in the reference's order).
+__device__ inline float step_l1(float acc, float x, float y) {
+#pragma clang fp contract(off)
+    return acc + fabsf(x - y);
+}
+__device__ inline float step_dot(float acc, float x, float y) {
+#pragma clang fp contract(off)
+    return acc + x * y;
+}
+template <uint32_t M>
+__device__ inline float metric_step(float acc, float x, float y) {
+    if constexpr (M == NNM_METRIC_L1) {
+        return step_l1(acc, x, y);
+    } else if constexpr (M == NNM_METRIC_COSINE) {
+        return step_dot(acc, x, y);
+    } else if constexpr (M == NNM_METRIC_CHI2) {
+        return nnm_chi2_step(acc, x, y);
+    } else {
+        const float t = x - y;
+        return __builtin_fmaf(t, t, acc);
+    }
+}
+
+// Distance of rows a, b of X under the call's metric (bf16 rows only for L2).
+__device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
+                                     uint64_t a, uint64_t b, const nnm_metric_args& m) {
+    if (m.metric == NNM_METRIC_L2) return nnm_row_dist(X, Xb, ld, a, b);
+    return nnm_metric_rows(m.metric, X + a * ld, X + b * ld, ld, m.sumsq ? m.sumsq[a] : 0.f,
+                           m.sumsq ? m.sumsq[b] : 0.f);
 }
 
 // Wave-wide bitonic sort of one 64-bit key per lane, ascending by lane.

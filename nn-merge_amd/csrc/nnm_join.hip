@@ -98,7 +98,10 @@ __device__ inline uint64_t offer(uint64_t* __restrict__ slots, uint32_t C, uint3
     return 1ull | (1ull << 32);
 }
 
-// Epilogue of one 2x4 micro-tile: pair rule, distance count, offers.
+// Epilogue of one 2x4 micro-tile: pair rule, distance count, offers.  Under
+// the cosine metric the accumulator is the dot product, finished here with
+// the rows' squared norms (vmeta .z).
+template <uint32_t M = NNM_METRIC_L2>
 __device__ inline void tile_epilogue(const JoinArgs& a, const float (&acc)[TM * TN], uint32_t rb, uint32_t cb,
                                      uint32_t na, uint32_t nb, bool diag, const uint32_t (*mA)[JW],
                                      const uint32_t (*mB)[JW], uint64_t& my_dists, uint64_t& my_offers) {
@@ -116,7 +119,9 @@ __device__ inline void tile_epilogue(const JoinArgs& a, const float (&acc)[TM * 
             const uint32_t bid = mB[0][cj], blab = mB[1][cj];
             if (aid == bid) continue;
             if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
-            const float dd = acc[i * TN + j];
+            float dd = acc[i * TN + j];
+            if constexpr (M == NNM_METRIC_COSINE)
+                dd = nnm_cos_finish(dd, __uint_as_float(a.vmeta[aid].z), __uint_as_float(a.vmeta[bid].z));
             ++my_dists;
             if (dd < __uint_as_float(mB[2][cj])) {
                 my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
@@ -150,6 +155,7 @@ __device__ inline uint32_t build_tiles(uint16_t* tiles, uint32_t na, uint32_t nb
 }
 
 // General path for one vertex: windows of 64, 32 dimensions per pass.
+template <uint32_t M = NNM_METRIC_L2>
 __device__ __attribute__((always_inline)) inline void join_vertex_windows(const JoinArgs& a, JoinLds& L, uint64_t vcur, uint32_t nn, uint32_t no,
                                     uint32_t lane, uint64_t& my_dists, uint64_t& my_offers) {
     float* rowsA = L.rows;
@@ -234,14 +240,10 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
                             for (int i = 0; i < TM; ++i)
 #pragma unroll
                                 for (int j = 0; j < TN; ++j) {
-                                    float t0 = x[i].x - y[j].x;
-                                    float r = __builtin_fmaf(t0, t0, acc[s][i * TN + j]);
-                                    t0 = x[i].y - y[j].y;
-                                    r = __builtin_fmaf(t0, t0, r);
-                                    t0 = x[i].z - y[j].z;
-                                    r = __builtin_fmaf(t0, t0, r);
-                                    t0 = x[i].w - y[j].w;
-                                    acc[s][i * TN + j] = __builtin_fmaf(t0, t0, r);
+                                    float r = metric_step<M>(acc[s][i * TN + j], x[i].x, y[j].x);
+                                    r = metric_step<M>(r, x[i].y, y[j].y);
+                                    r = metric_step<M>(r, x[i].z, y[j].z);
+                                    acc[s][i * TN + j] = metric_step<M>(r, x[i].w, y[j].w);
                                 }
                         }
                     }
@@ -251,7 +253,7 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
             for (int s = 0; s < MAXS; ++s) {
                 if ((uint32_t)s < my_n) {
                     const uint32_t t = L.tiles[lane + NNM_WAVE * s];
-                    tile_epilogue(a, acc[s], t / CBn, t % CBn, na, nb, diag, mA, L.metaB, my_dists, my_offers);
+                    tile_epilogue<M>(a, acc[s], t / CBn, t % CBn, na, nb, diag, mA, L.metaB, my_dists, my_offers);
                 }
             }
         }
@@ -787,6 +789,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
 }
 
 // Lists longer than min_m entries (every list when min_m = 0): windows of 64.
+template <uint32_t M>
 __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t min_m) {
     a.salt = join_salt(a);
     __shared__ __align__(16) JoinLds L;
@@ -803,7 +806,7 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
             const uint32_t nn = a.n_new[v], no = a.n_old[v];
             if (nn + no <= min_m) continue;
             if (lane == 0) my_entries += nn + no;
-            if (nn) join_vertex_windows(a, L, v, nn, no, lane, my_dists, my_offers);
+            if (nn) join_vertex_windows<M>(a, L, v, nn, no, lane, my_dists, my_offers);
         }
     }
 #pragma unroll
@@ -919,13 +922,33 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
         NNM_LAUNCH_CHECK(c);
         // lists beyond the long-list LDS window (rev_cap above 118) stay on FP32
         if (c->cap_new + c->cap_old > BLM) {
-            hipLaunchKernelGGL(k_join_f32_windows, dim3(join_grid(c, k_join_f32_windows)), dim3(64), 0, c->stream,
+            hipLaunchKernelGGL(k_join_f32_windows<NNM_METRIC_L2>, dim3(join_grid(c, k_join_f32_windows<NNM_METRIC_L2>)),
+                               dim3(64), 0, c->stream,
                                a, (uint32_t)BLM);
             NNM_LAUNCH_CHECK(c);
         }
         return NNM_OK;
     }
     if (p->prec != NNM_PREC_FP32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "join precision not built yet");
+    if (c->metric != NNM_METRIC_L2) {
+        // other metrics: the windows kernel joins every list (no packed fast path)
+        a.js = 0;
+        if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));
+#define NNM_WIN(M)                                                                                           \
+    case M:                                                                                                  \
+        hipLaunchKernelGGL(k_join_f32_windows<M>, dim3(join_grid(c, k_join_f32_windows<M>)), dim3(64), 0,    \
+                           c->stream, a, 0u);                                                                \
+        break;
+        switch (c->metric) {
+            NNM_WIN(NNM_METRIC_L1)
+            NNM_WIN(NNM_METRIC_COSINE)
+            NNM_WIN(NNM_METRIC_CHI2)
+            default: return nnm_fail(c, NNM_ERR_UNSUPPORTED, "join metric");
+        }
+#undef NNM_WIN
+        NNM_LAUNCH_CHECK(c);
+        return NNM_OK;
+    }
     // fast path for lists of at most js entries, windows kernel for the rest
     uint32_t js = 0;
     a.js = js_req ? min(js_req, fast_window(c->ld)) : fast_window(c->ld);
@@ -948,7 +971,8 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
 #undef NNM_FAST
     NNM_LAUNCH_CHECK(c);
     if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));
-    hipLaunchKernelGGL(k_join_f32_windows, dim3(join_grid(c, k_join_f32_windows)), dim3(64), 0, c->stream, a, js);
+    hipLaunchKernelGGL(k_join_f32_windows<NNM_METRIC_L2>, dim3(join_grid(c, k_join_f32_windows<NNM_METRIC_L2>)),
+                       dim3(64), 0, c->stream, a, js);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }

@@ -338,6 +338,7 @@ static int forward_backward(nnm_ctx* c, uint32_t B, bool bf16) {
 extern "C" int nnm_mlp_init(nnm_ctx* c, uint32_t hidden, uint32_t d_out, uint64_t seed) {
     if (!c) return NNM_ERR_INVALID;
     if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    NNM_TRY(nnm_require_l2(c, "nnm_mlp_init"));
     if (hidden == 0 || d_out == 0 || hidden % 16 || d_out % 16 || hidden > 1024 || d_out > 64)
         return nnm_fail(c, NNM_ERR_INVALID, "hidden and d_out must be multiples of 16 (d_out <= 64)");
     NNM_HIP(c, hipStreamSynchronize(c->stream));

@@ -450,6 +450,7 @@ extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_sea
                           uint32_t* ids, float* dist, nnm_search_stats* st) {
     if (!c) return NNM_ERR_INVALID;
     if (st) std::memset(st, 0, sizeof(*st));
+    NNM_TRY(nnm_require_l2(c, "nnm_search"));
     nnm_search_state* S = search_of(c);
     if (!S || S->nlayers == 0) return nnm_fail(c, NNM_ERR_STATE, "no search graph (nnm_search_set_layers)");
     if (!p || !Q || !ids || !dist) return nnm_fail(c, NNM_ERR_INVALID, "nnm_search: null argument");

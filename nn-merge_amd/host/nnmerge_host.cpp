@@ -1,6 +1,8 @@
 // Host classes over the C ABI; see nnmerge_host.hpp for the reference map.
 #include "nnmerge_host.hpp"
 
+#include <cmath>
+
 #include <algorithm>
 #include <cstring>
 #include <fstream>
@@ -19,11 +21,54 @@ float l2norm(const float* v1, const unsigned long s1, const float* v2, const uns
     return res;
 }
 
+float l1norm(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0) {
+    // DistOpt::l1norm, src/distopt.cpp:82-110
+    const float* a = v1 + s1 * d0;
+    const float* b = v2 + s2 * d0;
+    float res = 0.f;
+    for (unsigned i = 0; i < d0; ++i) res += std::fabs(a[i] - b[i]);
+    return res;
+}
+
+float ksquare(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0) {
+    // DistOpt::KSquare, src/distopt.cpp:163-184
+    const float* a = v1 + s1 * d0;
+    const float* b = v2 + s2 * d0;
+    float w2 = 0.f;
+    for (unsigned i = 0; i < d0; ++i) {
+        const float w1 = a[i] - b[i];
+        if (a[i] != 0.f || b[i] != 0.f) w2 += w1 * w1 / (a[i] + b[i]);
+    }
+    return w2;
+}
+
+float cosine(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0) {
+    // DistOpt::cosine, src/distopt.cpp:113-160
+    const float* a = v1 + s1 * d0;
+    const float* b = v2 + s2 * d0;
+    float res = 0.f, l1 = 0.f, l2 = 0.f;
+    for (unsigned i = 0; i < d0; ++i) {
+        l1 += a[i] * a[i];
+        l2 += b[i] * b[i];
+        res += a[i] * b[i];
+    }
+    res = (l1 != 0.f && l2 != 0.f) ? res / (std::sqrt(l1) * std::sqrt(l2)) : 0.f;
+    return 2.f - 2.f * res;
+}
+
 AbstractGraph::AbstractGraph(SpaceInterface<float>* space, int device) {
-    (void)space;  // the device path computes squared L2
     nnm_default_params(&params);
     int rc = nnm_create(&ctx_, device);
     if (rc != NNM_OK) throw Error(rc, "nnm_create failed");
+    if (space) {
+        rc = nnm_set_metric(ctx_, space->metric());
+        if (rc != NNM_OK) {
+            const std::string msg = nnm_last_error(ctx_);
+            nnm_destroy(ctx_);
+            ctx_ = nullptr;
+            throw Error(rc, msg);
+        }
+    }
 }
 
 AbstractGraph::~AbstractGraph() {

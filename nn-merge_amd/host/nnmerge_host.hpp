@@ -29,8 +29,9 @@
 namespace nnmerge {
 
 // DISTFUNC and SpaceInterface of src/abstractgraph.h:11-19, kept for CPU
-// callers that pass a distance to the host classes; the device path always
-// computes squared L2 (L2space, src/space_lp.h:9-23).
+// callers that pass a distance to the host classes.  A space also names its
+// device metric (NNM_METRIC_*), which the host classes hand to the library;
+// the reference's interface has only the function pointer.
 template <typename MTYPE>
 using DISTFUNC = MTYPE (*)(const float*, const unsigned long, const float*, const unsigned long, const unsigned int);
 
@@ -39,13 +40,41 @@ class SpaceInterface {
 public:
     virtual ~SpaceInterface() {}
     virtual DISTFUNC<MTYPE> get_dist_func() = 0;
+    virtual unsigned metric() const { return NNM_METRIC_L2; }
 };
 
+// CPU distances of the spaces, in the reference's operation order.
 float l2norm(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0);
+float l1norm(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0);
+float ksquare(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0);
+float cosine(const float* v1, const unsigned long s1, const float* v2, const unsigned long s2, const unsigned int d0);
 
+// L2space, src/space_lp.h:9-23.
 class L2space : public SpaceInterface<float> {
 public:
     DISTFUNC<float> get_dist_func() override { return l2norm; }
+};
+
+// L1space, src/space_lp.h:25-38.
+class L1space : public SpaceInterface<float> {
+public:
+    DISTFUNC<float> get_dist_func() override { return l1norm; }
+    unsigned metric() const override { return NNM_METRIC_L1; }
+};
+
+// Ksquare (chi-square), src/space_lp.h:40-51.
+class Ksquare : public SpaceInterface<float> {
+public:
+    DISTFUNC<float> get_dist_func() override { return ksquare; }
+    unsigned metric() const override { return NNM_METRIC_CHI2; }
+};
+
+// Cosine distance 2 - 2 cos (DistOpt::cosine, src/distopt.cpp:113-160; the
+// reference defines the routine but no space class for it).
+class CosineSpace : public SpaceInterface<float> {
+public:
+    DISTFUNC<float> get_dist_func() override { return cosine; }
+    unsigned metric() const override { return NNM_METRIC_COSINE; }
 };
 
 // Host view of one list entry (mergeDataType::Neighbor, src/pvetex.h:18-30).

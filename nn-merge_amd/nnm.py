@@ -19,6 +19,7 @@ NNM_ERR_UNSUPPORTED = -5
 RAW = 0xFFFFFFFF
 PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
 RULE_ALL, RULE_MERGE = 0, 1
+METRIC_L2, METRIC_L1, METRIC_COSINE, METRIC_CHI2 = 0, 1, 2, 3
 RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
 FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_LEARNED_RENEW = 1, 2, 4
 SEARCH_LAYERED, SEARCH_FLAT = 0, 1
@@ -86,6 +87,7 @@ _SIGS = {
     "nnm_set_mem_cap": (C.c_int, [_vp, C.c_uint64]),
     "nnm_mem_used": (C.c_uint64, [_vp]),
     "nnm_set_data": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32]),
+    "nnm_set_metric": (C.c_int, [_vp, C.c_uint32]),
     "nnm_set_partition": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32]),
     "nnm_graph_random_init": (C.c_int, [_vp, C.c_uint32, C.c_uint64, C.POINTER(Stats)]),
     "nnm_set_graph": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(Stats)]),
@@ -210,6 +212,11 @@ class Context:
         self.n, self.d = X.shape
         self.n_act = self.n
         self.K = 0
+
+    def set_metric(self, metric):
+        """NNM_METRIC_* for every later distance (L2 by default)."""
+        self._chk(lib().nnm_set_metric(self.h, int(metric)))
+        self.metric = int(metric)
 
     def set_partition(self, sub_begin, sub_built, sub_group=None):
         sb = np.ascontiguousarray(sub_begin, dtype=np.uint32)

@@ -85,6 +85,92 @@ double orc_l2_f64(const float* a, const float* b, uint32_t d) {
     return w;
 }
 
+// DistOpt::l1norm, src/distopt.cpp:82-110: sum of |a-b| in element order.
+float orc_l1(const float* a, const float* b, uint32_t d) {
+    float res = 0.0f;
+    for (uint32_t i = 0; i < d; ++i) {
+        float diff = a[i] - b[i];
+        res += std::fabs(diff);
+    }
+    return res;
+}
+
+// DistOpt::KSquare, src/distopt.cpp:163-184: (a-b)^2 / (a+b) summed over the
+// elements where a or b is nonzero.
+float orc_chi2(const float* a, const float* b, uint32_t d) {
+    float w2 = 0.0f;
+    for (uint32_t i = 0; i < d; ++i) {
+        float w1 = a[i] - b[i];
+        if (a[i] != 0 || b[i] != 0) w2 += w1 * w1 / (a[i] + b[i]);
+    }
+    return w2;
+}
+
+// DistOpt::cosine, src/distopt.cpp:113-160: three accumulators in element
+// order, res = res / (sqrt(l1) sqrt(l2)) (float square roots), 0 when a norm
+// is zero, then 2 - 2 res.
+float orc_cosine(const float* a, const float* b, uint32_t d) {
+    float res = 0.0f, l1 = 0.0f, l2 = 0.0f;
+    for (uint32_t i = 0; i < d; ++i) {
+        l1 += a[i] * a[i];
+        l2 += b[i] * b[i];
+        res += a[i] * b[i];
+    }
+    if (l1 != 0 && l2 != 0)
+        res = res / (std::sqrt(l1) * std::sqrt(l2));
+    else
+        res = 0;
+    return 2 - 2 * res;
+}
+
+// FP64 forms of the same metrics, for exact ground truth.
+double orc_l1_f64(const float* a, const float* b, uint32_t d) {
+    double w = 0.0;
+    for (uint32_t i = 0; i < d; ++i) w += std::fabs((double)a[i] - (double)b[i]);
+    return w;
+}
+double orc_chi2_f64(const float* a, const float* b, uint32_t d) {
+    double w = 0.0;
+    for (uint32_t i = 0; i < d; ++i) {
+        const double x = a[i], y = b[i], t = x - y;
+        if (x != 0 || y != 0) w += t * t / (x + y);
+    }
+    return w;
+}
+double orc_cosine_f64(const float* a, const float* b, uint32_t d) {
+    double r = 0.0, l1 = 0.0, l2 = 0.0;
+    for (uint32_t i = 0; i < d; ++i) {
+        l1 += (double)a[i] * a[i];
+        l2 += (double)b[i] * b[i];
+        r += (double)a[i] * b[i];
+    }
+    r = (l1 != 0 && l2 != 0) ? r / (std::sqrt(l1) * std::sqrt(l2)) : 0.0;
+    return 2.0 - 2.0 * r;
+}
+
+// Metric of the oracle's graph routines and brute force (0 L2, 1 L1,
+// 2 cosine, 3 chi-square): the SpaceInterface a graph class is built with
+// (src/abstractgraph.h:14-19, src/space_lp.h).
+static int g_metric = 0;
+void orc_set_metric(int m) { g_metric = m; }
+
+static float dist32(const float* a, const float* b, uint32_t d) {
+    switch (g_metric) {
+        case 1: return orc_l1(a, b, d);
+        case 2: return orc_cosine(a, b, d);
+        case 3: return orc_chi2(a, b, d);
+        default: return orc_l2_seq(a, b, d);
+    }
+}
+static double dist64(const float* a, const float* b, uint32_t d) {
+    switch (g_metric) {
+        case 1: return orc_l1_f64(a, b, d);
+        case 2: return orc_cosine_f64(a, b, d);
+        case 3: return orc_chi2_f64(a, b, d);
+        default: return orc_l2_f64(a, b, d);
+    }
+}
+
 // Brute-force k-NN in FP64 with ties broken by id (ascending).  For each of
 // the nq query rows (given as row indices into X, or as separate vectors Q
 // when Q != NULL) the k nearest rows of X are returned.  exclude_self drops
@@ -101,7 +187,7 @@ void orc_bruteforce_f64(const float* X, uint64_t n, uint32_t d,
         best.clear();
         for (uint64_t r = 0; r < n; ++r) {
             if (exclude_self && !Q && r == qrows[q]) continue;
-            double dd = orc_l2_f64(qv, X + r * d, d);
+            double dd = dist64(qv, X + r * d, d);
             std::pair<double, uint32_t> c(dd, (uint32_t)r);
             if (best.size() < k) {
                 best.push_back(c);
@@ -269,7 +355,7 @@ static void nn_descent(const float* X, uint32_t d, std::vector<OList>& g,
                     const OInd sj = j < ns ? L.newl[j] : L.oldl[j - ns];
                     if (si.id == sj.id) continue;
                     if (!rule_all && si.gid == sj.gid && si.gid != RAW_GID) continue;
-                    float dst = orc_l2_seq(X + (uint64_t)si.id * d, X + (uint64_t)sj.id * d, d);
+                    float dst = dist32(X + (uint64_t)si.id * d, X + (uint64_t)sj.id * d, d);
                     ++cc;
                     update_knn_list(g[si.id].pool.data(), K, ONb{sj.id, dst, sj.gid, 1});
                     update_knn_list(g[sj.id].pool.data(), K, ONb{si.id, dst, si.gid, 1});
@@ -314,7 +400,7 @@ static void random_init(const float* X, uint32_t d, std::vector<OList>& g,
             if (id == i) continue;
             ONb& nb = g[i].pool[L++];
             nb.id = id;
-            nb.dist = orc_l2_seq(X + (uint64_t)i * d, X + (uint64_t)id * d, d);
+            nb.dist = dist32(X + (uint64_t)i * d, X + (uint64_t)id * d, d);
             nb.flag = 1;
             nb.gd = gid[id];
             ++*nd;
@@ -384,7 +470,7 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
         for (uint32_t j = 0; j < K; ++j) {
             ONb& nb = g[v].pool[j];
             nb.id = init_ids[(uint64_t)v * K + j];
-            nb.dist = orc_l2_seq(X + (uint64_t)v * d, X + (uint64_t)nb.id * d, d);
+            nb.dist = dist32(X + (uint64_t)v * d, X + (uint64_t)nb.id * d, d);
             nb.flag = 1;
             nb.gd = gid[nb.id];
             ++st->init_dists;
@@ -416,7 +502,7 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
                 nb.id = fresh[j];
                 nb.gd = gid[nb.id];
                 nb.flag = 1;
-                nb.dist = orc_l2_seq(X + (uint64_t)nb.id * d, X + (uint64_t)v * d, d);
+                nb.dist = dist32(X + (uint64_t)nb.id * d, X + (uint64_t)v * d, d);
                 ++st->init_dists;
             }
             std::sort(g[v].pool.begin(), g[v].pool.end(), nb_less);
@@ -431,7 +517,7 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
                 nb.id = seeds[j];
                 nb.gd = gid[nb.id];
                 nb.flag = 1;
-                nb.dist = orc_l2_seq(X + (uint64_t)nb.id * d, X + (uint64_t)v * d, d);
+                nb.dist = dist32(X + (uint64_t)nb.id * d, X + (uint64_t)v * d, d);
                 ++st->init_dists;
             }
             std::sort(g[v].pool.begin(), g[v].pool.end(), nb_less);

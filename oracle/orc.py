@@ -53,6 +53,14 @@ def lib():
         L.orc_l2_simd4.argtypes = [f32p, f32p, C.c_uint32]
         L.orc_l2_f64.restype = C.c_double
         L.orc_l2_f64.argtypes = [f32p, f32p, C.c_uint32]
+        for nm in ("orc_l1", "orc_chi2", "orc_cosine"):
+            getattr(L, nm).restype = C.c_float
+            getattr(L, nm).argtypes = [f32p, f32p, C.c_uint32]
+        for nm in ("orc_l1_f64", "orc_chi2_f64", "orc_cosine_f64"):
+            getattr(L, nm).restype = C.c_double
+            getattr(L, nm).argtypes = [f32p, f32p, C.c_uint32]
+        L.orc_set_metric.restype = None
+        L.orc_set_metric.argtypes = [C.c_int]
         L.orc_bruteforce_f64.restype = None
         L.orc_bruteforce_f64.argtypes = [f32p, C.c_uint64, C.c_uint32, f32p, u32p, C.c_uint64,
                                          C.c_uint32, C.c_int, u32p, f64p]
@@ -96,6 +104,11 @@ def ref_lib():
         R.ref_l2norm.argtypes = [f32p, C.c_ulong, f32p, C.c_ulong, C.c_uint]
         R.ref_l2sqr_simd_ext.restype = C.c_float
         R.ref_l2sqr_simd_ext.argtypes = [f32p, C.c_uint, f32p, C.c_uint, C.c_uint]
+        for nm in ("ref_l1norm", "ref_ksquare"):
+            getattr(R, nm).restype = C.c_float
+            getattr(R, nm).argtypes = [f32p, C.c_ulong, f32p, C.c_ulong, C.c_uint]
+        R.ref_cosine.restype = C.c_float
+        R.ref_cosine.argtypes = [f32p, C.c_uint, f32p, C.c_uint, C.c_uint]
         _ref = R
     return _ref
 
@@ -119,8 +132,40 @@ def l2_f64(a, b):
     return lib().orc_l2_f64(_p(a, C.c_float), _p(b, C.c_float), a.size)
 
 
+METRIC_L2, METRIC_L1, METRIC_COSINE, METRIC_CHI2 = 0, 1, 2, 3
+_METRIC_FN = {METRIC_L2: "orc_l2_seq", METRIC_L1: "orc_l1", METRIC_COSINE: "orc_cosine", METRIC_CHI2: "orc_chi2"}
+
+
+def dist32(a, b, metric=METRIC_L2):
+    """The oracle's FP32 distance of two vectors under a metric."""
+    a, b = _f32(a), _f32(b)
+    return getattr(lib(), _METRIC_FN[metric])(_p(a, C.c_float), _p(b, C.c_float), a.size)
+
+
+def dist64(a, b, metric=METRIC_L2):
+    a, b = _f32(a), _f32(b)
+    fn = {METRIC_L2: "orc_l2_f64", METRIC_L1: "orc_l1_f64", METRIC_COSINE: "orc_cosine_f64",
+          METRIC_CHI2: "orc_chi2_f64"}[metric]
+    return getattr(lib(), fn)(_p(a, C.c_float), _p(b, C.c_float), a.size)
+
+
+class metric_scope:
+    """Sets the oracle's metric for the graph routines and brute force
+    inside a with-block, back to L2 afterwards."""
+
+    def __init__(self, metric):
+        self.metric = metric
+
+    def __enter__(self):
+        lib().orc_set_metric(int(self.metric))
+
+    def __exit__(self, *exc):
+        lib().orc_set_metric(METRIC_L2)
+
+
 def bruteforce(X, k, rows=None, Q=None, exclude_self=True):
-    """Exact FP64 k-NN of rows of X (or of query vectors Q) against X."""
+    """Exact FP64 k-NN of rows of X (or of query vectors Q) against X (under
+    the metric of an enclosing metric_scope)."""
     X = _f32(X)
     n, d = X.shape
     if Q is not None:
