@@ -320,6 +320,21 @@ int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm_params* p,
                     uint32_t* level_groups, double* level_secs, uint64_t* level_dists, uint32_t max_levels,
                     uint32_t* nlevels);
 
+/* LyNNDescent::buildhgraph with its search layers (src/lynndescent.cpp:735-831):
+ * the doubling schedule of nnm_host_hmerge, plus a DPG layer
+ * (LyNNDescent::dpg, src/lynndescent.cpp:502-540: occlusion bound K/2,
+ * reverse bound M on the base layer and M/2 above, lists carried from layer
+ * to layer) after the seed graph, after the levels that the reference's
+ * level[] table marks (3, 6 and 9) and after the last level.  dst_prefix !=
+ * NULL writes the layers to dst_prefix + ".txt" (upper layer first, each
+ * "N" then "i deg ids...") and the final k-NN graph to dst_prefix +
+ * "_baselayer.txt", as the reference does.  layer_rows gets each layer's row
+ * count (up to max_layers); st->secs excludes the layer builds, as the
+ * reference's construction time does, and dpg_secs reports them. */
+int nnm_host_hgraph(const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t M, const char* dst_prefix,
+                    uint32_t* ids, float* dist, nnm_stats* st, uint32_t* layer_rows, uint32_t max_layers,
+                    uint32_t* nlayers, double* dpg_secs);
+
 /* Per-result FP64 error of the graph's FP32 distances: every list entry of
  * the active rows is recomputed in FP64 on the GPU.  No reference
  * counterpart (the reference stores FP32 distances only,

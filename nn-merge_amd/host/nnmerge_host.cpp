@@ -237,15 +237,38 @@ void LyNNDescent::runLevel(const std::vector<uint32_t>& begin, const std::vector
     lastStats = st;
 }
 
-unsigned LyNNDescent::buildhgraph(float* mat, unsigned N, unsigned D, unsigned topk) {
+// LyNNDescent::dpg (src/lynndescent.cpp:502-540) on the current lists:
+// occlusion bound K0/2, reverse bound maxM, lists carried from the previous
+// layer; the layer is kept and, with a file, appended as saveDPGraph writes it.
+void LyNNDescent::emitLayer(unsigned maxM, bool carry, const char* layerFile) {
+    nnm_stats st{};
+    check(nnm_dpg_build(ctx_, K0 / 2, maxM, carry ? 1 : 0, &st));
+    dpgSecs += st.secs;
+    DPGLayer L;
+    L.rows = levels.empty() ? 0u : levels.back().rows;
+    L.offs.resize((size_t)L.rows + 1);
+    check(nnm_dpg_get(ctx_, L.offs.data(), nullptr, nullptr));
+    L.adj.resize(L.offs.back());
+    check(nnm_dpg_get(ctx_, L.offs.data(), L.adj.data(), nullptr));
+    if (layerFile)
+        check(nnm_io_write_layer(layerFile, layers.empty() ? 0 : 1, L.offs.data(), L.adj.data(), L.rows));
+    layers.push_back(std::move(L));
+}
+
+unsigned LyNNDescent::buildhgraph(float* mat, unsigned N, unsigned D, unsigned topk, unsigned M,
+                                  const char* dstFn) {
     K0 = topk;
     row = N;
     col = D;
     params.K = topk;
     levels.clear();
+    layers.clear();
     nCmps = 0;
     mergeSecs = 0.0;
+    dpgSecs = 0.0;
     if (N < N0 + 1) throw Error(NNM_ERR_INVALID, "buildhgraph needs more than N0 rows");
+    const std::string layerPath = dstFn ? std::string(dstFn) + ".txt" : std::string();
+    const char* layerFile = dstFn ? layerPath.c_str() : nullptr;
     check(nnm_set_data(ctx_, mat, N, D));
     // seed graph: plain NN-Descent over the first N0 rows (init + nnDescent(verse = 1))
     {
@@ -262,15 +285,28 @@ unsigned LyNNDescent::buildhgraph(float* mat, unsigned N, unsigned D, unsigned t
         nCmps += s0.init_dists + s1.dists;
         mergeSecs += s0.secs + s1.secs;
     }
-    // appdNwSmpl(K/2, smplNum) + nnDescent(0) + merge2one, smplNum doubling
+    // the layer base M: _M on the base layer (all N rows), _M / 2 above
+    auto layerM = [&](uint32_t rows) { return rows == N ? M : M / 2; };
+    if (M) emitLayer(layerM(N0), false, layerFile);
+    // appdNwSmpl(K/2, smplNum) + nnDescent(0) + merge2one, smplNum doubling;
+    // setflag() relabels the whole graph as one built subgraph (the next
+    // level's partition {built [0, cur), raw [cur, cur + b)})
+    static const unsigned char level[10] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1};
+    unsigned count_level = 1;
     uint32_t cur = N0, smpl = N0;
     while (cur < N) {
-        const uint32_t b = (N - cur > smpl) ? smpl : (N - cur);
+        const bool last = N - cur <= smpl;
+        const uint32_t b = last ? (N - cur) : smpl;
         runLevel({0u, cur, cur + b}, {1, 0}, {0, 0}, topk - topk / 2);
         cur += b;
         smpl = cur;
+        // the reference tests level[count_level] before count_level < 10
+        // (reading past the table from level 10 on); the bound comes first here
+        if (M && (last || (count_level < 10 && level[count_level]))) emitLayer(layerM(cur), true, layerFile);
+        ++count_level;
     }
     fetch();
+    if (dstFn) saveKGraph((std::string(dstFn) + "_baselayer.txt").c_str());
     return 0;
 }
 
@@ -353,6 +389,34 @@ extern "C" int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm
             if (level_secs) level_secs[i] = g.levels[i].secs;
             if (level_dists) level_dists[i] = g.levels[i].dists;
         }
+        return NNM_OK;
+    } catch (const nnmerge::Error& e) {
+        return e.code;
+    } catch (...) {
+        return NNM_ERR_HIP;
+    }
+}
+
+extern "C" int nnm_host_hgraph(const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t M,
+                               const char* dst_prefix, uint32_t* ids, float* dist, nnm_stats* st,
+                               uint32_t* layer_rows, uint32_t max_layers, uint32_t* nlayers, double* dpg_secs) {
+    if (!X || !p || !ids || n == 0 || n > 0xFFFFFFF0ull || M == 0) return NNM_ERR_INVALID;
+    try {
+        nnmerge::LyNNDescent g;
+        g.params = *p;
+        g.buildhgraph(const_cast<float*>(X), (unsigned)n, d, p->K, M, dst_prefix);
+        std::memcpy(ids, g.ids().data(), n * p->K * sizeof(uint32_t));
+        if (dist) std::memcpy(dist, g.dists().data(), n * p->K * sizeof(float));
+        if (st) {
+            *st = g.lastStats;
+            st->dists = g.nCmps;
+            st->secs = g.mergeSecs;
+        }
+        const uint32_t L = (uint32_t)g.layers.size();
+        if (nlayers) *nlayers = L;
+        for (uint32_t i = 0; i < L && i < max_layers; ++i)
+            if (layer_rows) layer_rows[i] = g.layers[i].rows;
+        if (dpg_secs) *dpg_secs = g.dpgSecs;
         return NNM_OK;
     } catch (const nnmerge::Error& e) {
         return e.code;

@@ -184,16 +184,32 @@ struct LevelStat {
     double secs;         // device time of the level's merge
 };
 
+// One DPG layer as CSR over rows [0, rows).
+struct DPGLayer {
+    uint32_t rows;
+    std::vector<uint32_t> offs;  // rows + 1
+    std::vector<uint32_t> adj;
+};
+
 class LyNNDescent : public AbstractGraph {
 public:
     static const unsigned N0 = 64;  // src/lynndescent.cpp:23
     using AbstractGraph::AbstractGraph;
     unsigned* buildKGraph(float* mat, const unsigned K, const unsigned N, const unsigned D) override;
-    unsigned buildhgraph(float* mat, unsigned N, unsigned D, unsigned topk);
+    // buildhgraph, src/lynndescent.cpp:735-831.  M > 0 also emits the DPG
+    // layers (LyNNDescent::dpg after the seed graph, after the levels the
+    // level[] table marks and after the last level) into `layers`, and with
+    // dstFn writes them to dstFn + ".txt" and the final k-NN graph to
+    // dstFn + "_baselayer.txt".
+    unsigned buildhgraph(float* mat, unsigned N, unsigned D, unsigned topk, unsigned M = 0,
+                         const char* dstFn = nullptr);
     unsigned buildLeafHierarchy(float* mat, unsigned N, unsigned D, unsigned K, const std::vector<uint32_t>& leaf_begin);
     std::vector<LevelStat> levels;
+    std::vector<DPGLayer> layers;  // emitted DPG layers, upper layer first
+    double dpgSecs = 0.0;          // device time of the layer builds (not in mergeSecs)
 
 private:
+    void emitLayer(unsigned maxM, bool carry, const char* layerFile);
     void runLevel(const std::vector<uint32_t>& begin, const std::vector<uint8_t>& built,
                   const std::vector<uint32_t>& group, unsigned cut);
 };

@@ -88,6 +88,8 @@ _SIGS = {
     "nnm_mem_used": (C.c_uint64, [_vp]),
     "nnm_set_data": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32]),
     "nnm_set_metric": (C.c_int, [_vp, C.c_uint32]),
+    "nnm_host_hgraph": (C.c_int, [_vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, C.c_char_p, _vp, _vp,
+                                  C.POINTER(Stats), _vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "nnm_set_partition": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32]),
     "nnm_graph_random_init": (C.c_int, [_vp, C.c_uint32, C.c_uint64, C.POINTER(Stats)]),
     "nnm_set_graph": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(Stats)]),
@@ -427,6 +429,25 @@ def host_hmerge(X, p, leaf_begin=None, max_levels=64):
     levels = [{"rows": int(rows[i]), "groups": int(groups[i]), "secs": float(secs[i]), "dists": int(dists[i])}
               for i in range(m)]
     return ids, dist, st.as_dict(), levels
+
+
+def host_hgraph(X, p, M, dst_prefix=None, max_layers=16):
+    """LyNNDescent::buildhgraph with its DPG search layers; returns ids,
+    dist, stats, the layers' row counts and the layers' device seconds."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    ids = np.empty((n, p.K), np.uint32)
+    dist = np.empty((n, p.K), np.float32)
+    st = Stats()
+    rows = np.zeros(max_layers, np.uint32)
+    L = C.c_uint32()
+    ds = C.c_double()
+    pre = None if dst_prefix is None else str(dst_prefix).encode()
+    rc = lib().nnm_host_hgraph(_ptr(X), n, d, C.byref(p), M, pre, _ptr(ids), _ptr(dist), C.byref(st), _ptr(rows),
+                               max_layers, C.byref(L), C.byref(ds))
+    if rc != NNM_OK:
+        raise NNMError(rc, "nnm_host_hgraph failed")
+    return ids, dist, st.as_dict(), [int(r) for r in rows[:min(L.value, max_layers)]], ds.value
 
 
 def recall(ids, gt, rows, k=10, unique=True):
