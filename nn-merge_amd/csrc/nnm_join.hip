@@ -499,7 +499,7 @@ __device__ inline float offer_key(float an, float bn, float g, int mode, float k
 
 template <int U>
 struct Bf16Lds {
-    uint2 rows[(BJS + 16) * U];    // (window + 16 slack rows) x U units of 8 bytes
+    uint2 rows[BJS * U];           // window rows x U units of 8 bytes
     uint32_t meta[4][BJS];         // id, label, k-th distance bits, norm bits
 };
 
@@ -590,7 +590,10 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                 const uint32_t RB = (nb_ + 15) / 16, CB = (mb + 15) / 16;
                 for (uint32_t rb = 0; rb < RB; ++rb) {
                     for (uint32_t cb = rb; cb < CB; ++cb) {
-                        const uint32_t ra = ob + rb * 16 + lr, rc = ob + cb * 16 + lr;
+                        // rows past the window read its last row: their
+                        // results are masked below (ri >= nb_ or cj >= mb)
+                        const uint32_t ra = min(ob + rb * 16 + lr, (uint32_t)BJS - 1);
+                        const uint32_t rc = min(ob + cb * 16 + lr, (uint32_t)BJS - 1);
                         nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                         for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
