@@ -424,20 +424,24 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
                     const uint64_t v = base + vb;
                     const uint32_t e = lane - ob;
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
-                    const uint4 mt = a.vmeta[wid];
-                    L.meta[0][lane] = wid;
-                    L.meta[1][lane] = mt.x;
-                    L.meta[2][lane] = mt.y;
                 }
             }
+            // the rows' metadata is fetched with the rows (one wait for both)
+            uint4 mt = make_uint4(0u, 0u, 0u, 0u);
+            if (lane < rows) mt = a.vmeta[wid];
             // rows -> LDS, one row per direct-to-LDS instruction
             for (uint32_t r = 0; r < rows; ++r) {
-                const uint32_t id = (uint32_t)__shfl((int)wid, (int)r, NNM_WAVE);
+                const uint32_t id = wave_read32(wid, r);
                 if (lane < (uint32_t)CPR) {
                     const float* g = a.X + (uint64_t)id * LD + 4 * lane;
                     __builtin_amdgcn_global_load_lds(
                         (const void*)g, (__attribute__((address_space(3))) void*)(L.rows + r * PF), 16, 0, 0);
                 }
+            }
+            if (lane < rows) {
+                L.meta[0][lane] = wid;
+                L.meta[1][lane] = mt.x;
+                L.meta[2][lane] = mt.y;
             }
             __builtin_amdgcn_s_waitcnt(0);
             wave_lds_sync();
@@ -500,7 +504,8 @@ __device__ inline float offer_key(float an, float bn, float g, int mode, float k
 template <int U>
 struct Bf16Lds {
     uint2 rows[BJS * U];           // window rows x U units of 8 bytes
-    uint32_t meta[4][BJS];         // id, label, k-th distance bits, norm bits
+    uint4 rec[BJS];                // the rows' join records {label, k-th bits, norm bits, 0}
+    uint32_t id[BJS];
 };
 
 template <int U>  // 8-byte units per bf16 row: ld = 4 * U
@@ -562,16 +567,25 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                     const uint64_t v = base + vb;
                     const uint32_t e = lane - ob;
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
-                    const uint4 mt = a.vmeta[wid];
-                    L.meta[0][lane] = wid;
-                    L.meta[1][lane] = mt.x;
-                    L.meta[2][lane] = mt.y;
-                    L.meta[3][lane] = mt.z;
                 }
+            }
+            // ids and join records -> LDS ahead of the rows (an LDS store after
+            // a direct-to-LDS load would wait for it): the records and the
+            // rows are in flight together, one wait for both
+            // The row loop reads the ids from a VALU copy made in the same
+            // block: a register written by a memory load and read after the
+            // block would make the compiler wait for every outstanding load
+            // (the records) first.
+            uint32_t wcp = 0;
+            if (lane < rows) {
+                L.id[lane] = wid;
+                __builtin_amdgcn_global_load_lds((const void*)(a.vmeta + wid),
+                                                 (__attribute__((address_space(3))) void*)L.rec, 16, 0, 0);
+                asm volatile("v_mov_b32 %0, %1" : "=v"(wcp) : "v"(wid));
             }
             // rows -> LDS: lane l of a row instruction fetches dword 2u + (l & 1), u = (l/2) ^ (r & SWM)
             for (uint32_t r = 0; r < rows; ++r) {
-                const uint32_t id = (uint32_t)__shfl((int)wid, (int)r, NNM_WAVE);
+                const uint32_t id = wave_read32(wcp, r);
                 if (lane < DW) {
                     const uint32_t u = (lane >> 1) ^ (r & SWM);
                     const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
@@ -613,25 +627,27 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                         uint32_t bid = 0, blab = 0;
                         float bthr = 0.f, bn = 0.f;
                         if (colok) {
-                            bid = L.meta[0][ob + cj];
-                            blab = L.meta[1][ob + cj];
-                            bthr = __uint_as_float(L.meta[2][ob + cj]);
-                            bn = __uint_as_float(L.meta[3][ob + cj]);
+                            const uint4 br = L.rec[ob + cj];
+                            bid = L.id[ob + cj];
+                            blab = br.x;
+                            bthr = __uint_as_float(br.y);
+                            bn = __uint_as_float(br.z);
                         }
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const uint32_t ri = rb * 16 + 4 * lg + q;
                             if (!colok || ri >= nb_ || cj <= ri) continue;
-                            const uint32_t aid = L.meta[0][ob + ri], alab = L.meta[1][ob + ri];
+                            const uint4 ar = L.rec[ob + ri];
+                            const uint32_t aid = L.id[ob + ri], alab = ar.x;
                             if (aid == bid) continue;
                             if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
-                            const float an = __uint_as_float(L.meta[3][ob + ri]);
+                            const float an = __uint_as_float(ar.z);
                             const float dd = offer_key(an, bn, acc[q], mode, kscale);
                             ++my_dists;
                             if (dd < tau * bthr) {
                                 my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
                             }
-                            if (dd < tau * __uint_as_float(L.meta[2][ob + ri])) {
+                            if (dd < tau * __uint_as_float(ar.y)) {
                                 my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
                             }
                         }
