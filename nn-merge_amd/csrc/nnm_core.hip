@@ -1038,38 +1038,65 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
     const uint64_t below = (1ull << lane) - 1;
     const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / NNM_WAVE;
     const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / NNM_WAVE;
-    for (uint64_t v0 = wid * GPW; v0 < n; v0 += nw * GPW) {
-        const uint64_t v = v0 + grp;
-        const bool act = v < n && gl < K;
-        uint32_t u = 0;
-        float de = 0.f;
-        uint8_t f = 0;
-        if (act) {
-            u = g_ids[v * K + gl];
-            de = g_dist[v * K + gl];
-            f = g_flag[v * K + gl];
+    // Two row groups per step (v0 and v0 + nw * GPW), their loads, k-th
+    // gathers and bucket atomics issued together: the step waits for each
+    // round trip once for both.
+    constexpr int R = 2;
+    for (uint64_t v0 = wid * GPW; v0 < n; v0 += (uint64_t)R * nw * GPW) {
+        uint64_t v[R];
+        bool act[R];
+        uint32_t u[R];
+        float de[R];
+        uint8_t f[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            v[k] = v0 + (uint64_t)k * nw * GPW + grp;
+            act[k] = v[k] < n && gl < K;
+            u[k] = 0;
+            de[k] = 0.f;
+            f[k] = 0;
+            if (act[k]) {
+                u[k] = g_ids[v[k] * K + gl];
+                de[k] = g_dist[v[k] * K + gl];
+                f[k] = g_flag[v[k] * K + gl];
+            }
         }
-        const uint64_t fm = nnm_ballot(act && f) & gmask;
-        const uint64_t om = nnm_ballot(act && !f) & gmask;
-        const uint32_t frank = (uint32_t)__popcll(fm & below);
-        const bool is_new = act && f && frank < S;
-        const bool is_old = act && !f;
-        if (is_new) {
-            l_new[v * cap_new + frank] = u;
-            g_flag[v * K + gl] = 0;
+        bool rev[R];
+        float ku[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint64_t fm = nnm_ballot(act[k] && f[k]) & gmask;
+            const uint64_t om = nnm_ballot(act[k] && !f[k]) & gmask;
+            const uint32_t frank = (uint32_t)__popcll(fm & below);
+            const bool is_new = act[k] && f[k] && frank < S;
+            const bool is_old = act[k] && !f[k];
+            if (is_new) {
+                l_new[v[k] * cap_new + frank] = u[k];
+                g_flag[v[k] * K + gl] = 0;
+            }
+            if (is_old) l_old[v[k] * cap_old + (uint32_t)__popcll(om & below)] = u[k];
+            const uint64_t nm = nnm_ballot(is_new) & gmask;
+            if (gl == 0 && v[k] < n) {
+                n_new[v[k]] = min(S, (uint32_t)__popcll(fm));
+                n_old[v[k]] = (uint32_t)__popcll(om);
+                newmask[v[k]] = (G == NNM_WAVE) ? nm : (nm >> (grp * G));
+            }
+            // bucket of the reverse entry: new entries' targets first, old second
+            rev[k] = is_new || is_old;
+            f[k] = is_new ? 1 : 0;
+            ku[k] = rev[k] ? kthv[u[k]] : 0.f;
         }
-        if (is_old) l_old[v * cap_old + (uint32_t)__popcll(om & below)] = u;
-        const uint64_t nm = nnm_ballot(is_new) & gmask;
-        if (gl == 0 && v < n) {
-            n_new[v] = min(S, (uint32_t)__popcll(fm));
-            n_old[v] = (uint32_t)__popcll(om);
-            newmask[v] = (G == NNM_WAVE) ? nm : (nm >> (grp * G));
+        uint32_t pos[R];
+        uint64_t b[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            rev[k] = rev[k] && ku[k] < de[k];
+            b[k] = f[k] ? (uint64_t)u[k] : n + u[k];
+            pos[k] = rev[k] ? atomicAdd(&rcnt[b[k]], 1u) : 0u;
         }
-        if ((is_new || is_old) && kthv[u] < de) {
-            const uint64_t b = is_new ? (uint64_t)u : n + u;
-            const uint32_t pos = atomicAdd(&rcnt[b], 1u);
-            if (pos < NNM_RB) rbuf[b * NNM_RB + pos] = (uint32_t)v;
-        }
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (rev[k] && pos[k] < NNM_RB) rbuf[b[k] * NNM_RB + pos[k]] = (uint32_t)v[k];
     }
 }
 
