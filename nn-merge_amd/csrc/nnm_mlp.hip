@@ -193,6 +193,8 @@ __global__ void k_mlp_pairs(uint32_t B, uint64_t n, uint32_t K, uint32_t ld, con
 
 // Projection of every row: 16 rows per wave, hidden layer in LDS (bf16),
 // output rows in bf16 plus their squared norms (of the bf16 values).
+#define KMAX 256  // longest row (ld) the projection takes; nnm_mlp_init checks it
+
 template <uint32_t HID, uint32_t DOUT>
 __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X, uint64_t n, uint32_t ld,
                                                      float alpha, const float* __restrict__ W1,
@@ -206,18 +208,34 @@ __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X
     for (uint64_t blk = (uint64_t)blockIdx.x * 4 + wv; blk < nblk; blk += (uint64_t)gridDim.x * 4) {
         const uint64_t r0 = blk * 16;
         const uint64_t myrow = min(r0 + lr, n - 1);
-        // layer 1: H[16 x HID] = relu(a X W1^T + b1)
+        // layer 1: H[16 x HID] = relu(a X W1^T + b1).  The row's operands
+        // (bf16 of alpha x, elements k0 + 4 lg .. + 3 of each 16-chunk) are
+        // loaded once; W1 comes by 16-byte loads (L2-resident).
+        mlp_bf16x4 xa[KMAX / 16];
+#pragma unroll
+        for (uint32_t c = 0; c < KMAX / 16; ++c) {
+            if (c * 16 < ld) {
+                const float4 x4 = *reinterpret_cast<const float4*>(X + myrow * ld + c * 16 + 4 * lg);
+                xa[c][0] = mlp_to_bf16(alpha * x4.x);
+                xa[c][1] = mlp_to_bf16(alpha * x4.y);
+                xa[c][2] = mlp_to_bf16(alpha * x4.z);
+                xa[c][3] = mlp_to_bf16(alpha * x4.w);
+            }
+        }
         for (uint32_t n0 = 0; n0 < HID; n0 += 16) {
             mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (uint32_t k0 = 0; k0 < ld; k0 += 16) {
-                mlp_bf16x4 a, b;
+            const float* w1r = W1 + (uint64_t)(n0 + lr) * ld + 4 * lg;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t k = k0 + 4 * lg + q;
-                    a[q] = mlp_to_bf16(alpha * X[myrow * ld + k]);
-                    b[q] = mlp_to_bf16(W1[(uint64_t)(n0 + lr) * ld + k]);
+            for (uint32_t c = 0; c < KMAX / 16; ++c) {
+                if (c * 16 < ld) {
+                    const float4 w4 = *reinterpret_cast<const float4*>(w1r + c * 16);
+                    mlp_bf16x4 b;
+                    b[0] = mlp_to_bf16(w4.x);
+                    b[1] = mlp_to_bf16(w4.y);
+                    b[2] = mlp_to_bf16(w4.z);
+                    b[3] = mlp_to_bf16(w4.w);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[c], b, acc, 0, 0, 0);
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) hsh[wv][4 * lg + q][n0 + lr] = mlp_to_bf16(fmaxf(acc[q] + b1[n0 + lr], 0.f));
@@ -232,12 +250,13 @@ __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X
             mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             for (uint32_t k0 = 0; k0 < HID; k0 += 16) {
                 mlp_bf16x4 a, b;
+                const float4 w4 = *reinterpret_cast<const float4*>(W2 + (uint64_t)(t * 16 + lr) * HID + k0 + 4 * lg);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t k = k0 + 4 * lg + q;
-                    a[q] = hsh[wv][lr][k];
-                    b[q] = mlp_to_bf16(W2[(uint64_t)(t * 16 + lr) * HID + k]);
-                }
+                for (int q = 0; q < 4; ++q) a[q] = hsh[wv][lr][k0 + 4 * lg + q];
+                b[0] = mlp_to_bf16(w4.x);
+                b[1] = mlp_to_bf16(w4.y);
+                b[2] = mlp_to_bf16(w4.z);
+                b[3] = mlp_to_bf16(w4.w);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
             }
 #pragma unroll
@@ -339,6 +358,7 @@ extern "C" int nnm_mlp_init(nnm_ctx* c, uint32_t hidden, uint32_t d_out, uint64_
     if (!c) return NNM_ERR_INVALID;
     if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
     NNM_TRY(nnm_require_l2(c, "nnm_mlp_init"));
+    if (c->ld > KMAX) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "the learned projection takes rows of at most 256 values");
     if (hidden == 0 || d_out == 0 || hidden % 16 || d_out % 16 || hidden > 1024 || d_out > 64)
         return nnm_fail(c, NNM_ERR_INVALID, "hidden and d_out must be multiples of 16 (d_out <= 64)");
     NNM_HIP(c, hipStreamSynchronize(c->stream));
