@@ -453,39 +453,80 @@ __device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t
                            m.sumsq ? m.sumsq[b] : 0.f);
 }
 
+// Value of lane (lane ^ J) for J in {1, 2, 4, 8, 16, 32}, without the LDS
+// crossbar: quad permutes for 1 and 2, row shifts for 4 and 8 (a lane takes
+// lane + J or lane - J inside its 16-lane row), lane swaps across rows for
+// 16 and halves for 32 (v_permlane16_swap / v_permlane32_swap).
+template <uint32_t J>
+__device__ inline uint32_t wave_xor32(uint32_t x) {
+    const uint32_t lane = nnm_lane();
+    if constexpr (J == 1) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (J == 2) {
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (J == 4) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x104, 0xF, 0xF, false);  // row_shl:4
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+        return (lane & 4u) ? dn : up;
+    } else if constexpr (J == 8) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x108, 0xF, 0xF, false);  // row_shl:8
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+        return (lane & 8u) ? dn : up;
+    } else if constexpr (J == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (lane & 16u) ? (uint32_t)r[0] : (uint32_t)r[1];
+    } else {
+        static_assert(J == 32, "lane xor distance");
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (lane & 32u) ? (uint32_t)r[0] : (uint32_t)r[1];
+    }
+}
+template <uint32_t J>
+__device__ inline uint64_t wave_xor64(uint64_t x) {
+    return ((uint64_t)wave_xor32<J>((uint32_t)(x >> 32)) << 32) | wave_xor32<J>((uint32_t)x);
+}
+
+// One compare-exchange step of the bitonic network (block size KB, distance J).
+template <uint32_t KB, uint32_t J, typename T>
+__device__ inline T bitonic_step(T key, T other) {
+    const uint32_t lane = nnm_lane();
+    const bool up = ((lane & KB) == 0);
+    const bool lower = ((lane & J) == 0);
+    const T mn = key < other ? key : other;
+    const T mx = key < other ? other : key;
+    return (lower == up) ? mn : mx;
+}
+
+template <uint32_t KB, uint32_t J>
+__device__ inline uint64_t bitonic_pass64(uint64_t key) {
+    key = bitonic_step<KB, J>(key, wave_xor64<J>(key));
+    if constexpr (J > 1) return bitonic_pass64<KB, J / 2>(key);
+    else return key;
+}
+template <uint32_t KB, uint32_t J>
+__device__ inline uint32_t bitonic_pass32(uint32_t key) {
+    key = bitonic_step<KB, J>(key, wave_xor32<J>(key));
+    if constexpr (J > 1) return bitonic_pass32<KB, J / 2>(key);
+    else return key;
+}
+
 // Wave-wide bitonic sort of one 64-bit key per lane, ascending by lane.
 __device__ inline uint64_t nnm_wave_sort64(uint64_t key) {
-    const uint32_t lane = nnm_lane();
-#pragma unroll
-    for (uint32_t k = 2; k <= NNM_WAVE; k <<= 1) {
-#pragma unroll
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            uint64_t other = __shfl_xor(key, (int)j, NNM_WAVE);
-            bool up = ((lane & k) == 0);
-            bool lower = ((lane & j) == 0);
-            uint64_t mn = key < other ? key : other;
-            uint64_t mx = key < other ? other : key;
-            key = (lower == up) ? mn : mx;
-        }
-    }
-    return key;
+    key = bitonic_pass64<2, 1>(key);
+    key = bitonic_pass64<4, 2>(key);
+    key = bitonic_pass64<8, 4>(key);
+    key = bitonic_pass64<16, 8>(key);
+    key = bitonic_pass64<32, 16>(key);
+    return bitonic_pass64<64, 32>(key);
 }
 
 __device__ inline uint32_t nnm_wave_sort32(uint32_t key) {
-    const uint32_t lane = nnm_lane();
-#pragma unroll
-    for (uint32_t k = 2; k <= NNM_WAVE; k <<= 1) {
-#pragma unroll
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            uint32_t other = __shfl_xor(key, (int)j, NNM_WAVE);
-            bool up = ((lane & k) == 0);
-            bool lower = ((lane & j) == 0);
-            uint32_t mn = key < other ? key : other;
-            uint32_t mx = key < other ? other : key;
-            key = (lower == up) ? mn : mx;
-        }
-    }
-    return key;
+    key = bitonic_pass32<2, 1>(key);
+    key = bitonic_pass32<4, 2>(key);
+    key = bitonic_pass32<8, 4>(key);
+    key = bitonic_pass32<16, 8>(key);
+    key = bitonic_pass32<32, 16>(key);
+    return bitonic_pass32<64, 32>(key);
 }
 
 __device__ inline uint32_t nnm_wave_sum32(uint32_t v) {
