@@ -239,6 +239,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--keep", type=float, default=0.98)
     ap.add_argument("--plain-tail", type=int, default=3)
+    ap.add_argument("--slots", type=int, default=64,
+                    help="update slots per vertex for every run (the library default is 128: fewer slots lose "
+                         "a few more colliding candidates per iteration and run faster)")
     ap.add_argument("--learn-tau", type=float, default=0.0,
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = library default)")
     ap.add_argument("--no-renew", action="store_true", help="learned: keep the flags at the switch")
@@ -321,6 +324,7 @@ def main():
                                seed=seed, plain_tail=args.plain_tail, flags=fl)
         if args.learn_tau > 0:
             p.learn_tau = args.learn_tau
+        p.slots = args.slots
         train = None
         if name == "learned":
             ctx.graph_restore(recompute=True)
@@ -418,7 +422,7 @@ def main():
            "dtype": "f32" if chosen == "fp32" else "bf16",
            "data": "synthetic (seeded generator nnm_data.sift_like)",
            "config": {"workload": workload, "n": n, "d": d, "K": K, "rho": args.rho, "max_iters": args.iters,
-                      "path": chosen, "parallelism": "single GPU"},
+                      "slots": args.slots, "path": chosen, "parallelism": "single GPU"},
            "merge_secs": ch["merge_secs"], "recall@10": ch.get("recall@10"),
            "upload": {"bytes": int(X.nbytes), "secs": round(upload_secs, 4),
                       "note": "nnm_set_data host -> HBM (pinned staging, with the bf16-exactness scan); "

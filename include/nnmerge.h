@@ -98,7 +98,9 @@ typedef struct nnm_params {
     uint32_t prec;      /* NNM_PREC_* */
     uint32_t rerank;    /* NNM_RERANK_* */
     uint32_t slots;     /* update slots per vertex per iteration (power of
-                           two, 16..64); 0 = 64 */
+                           two, 16..128); 0 = 128.  An offer that loses its
+                           slot to a better one is dropped for the
+                           iteration, so fewer slots lose more candidates */
     uint32_t cut;       /* merge set-up: far entries replaced by cross
                            samples; 0 = K/2 (cutSize, src/smerge.cpp:99) */
     uint32_t plain_tail;/* NNM_PREC_LEARNED with max_iters: the last

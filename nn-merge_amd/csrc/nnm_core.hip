@@ -203,7 +203,7 @@ extern "C" void nnm_default_params(nnm_params* p) {
     p->rule = NNM_RULE_ALL;
     p->prec = NNM_PREC_FP32;
     p->rerank = NNM_RERANK_AUTO;
-    p->slots = 64;
+    p->slots = 128;
     p->cut = 0;
     p->plain_tail = 1;
     p->flags = 0;
@@ -1322,30 +1322,34 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
     const uint32_t lane = nnm_lane();
     const uint64_t W = (uint64_t)gridDim.x * (blockDim.x / NNM_WAVE);
     uint64_t c_upd = 0, c_flags = 0, c_rr = 0, c_ties = 0, c_swaps = 0;  // wave-uniform
-    auto fetch = [&](uint64_t u, uint32_t& id, float& d, uint32_t& f, uint64_t& sk) {
+    // slots j and 64 + j of a vertex are read by lane j (C <= 128)
+    auto fetch = [&](uint64_t u, uint32_t& id, float& d, uint32_t& f, uint64_t& sk0, uint64_t& sk1) {
         id = 0xFFFFFFFFu;
         d = 0.f;
         f = 0;
-        sk = NNM_EMPTY_KEY;
+        sk0 = NNM_EMPTY_KEY;
+        sk1 = NNM_EMPTY_KEY;
         if (u >= n) return;
         if (lane < K) {
             id = g_ids[u * K + lane];
             d = g_dist[u * K + lane];
             f = g_flag[u * K + lane];
         }
-        if (lane < C) sk = slots[u * C + lane];
+        if (lane < C) sk0 = slots[u * C + lane];
+        if (lane + NNM_WAVE < C) sk1 = slots[u * C + NNM_WAVE + lane];
     };
     uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
     uint32_t pid, pf;
     float pd;
-    uint64_t skey;
-    fetch(v, pid, pd, pf, skey);
+    uint64_t skey0, skey1;
+    fetch(v, pid, pd, pf, skey0, skey1);
     for (; v < n; v += W) {
         uint32_t nid, nf;
         float nd;
-        uint64_t nskey;
-        fetch(v + W, nid, nd, nf, nskey);
-        if (skey != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
+        uint64_t nsk0, nsk1;
+        fetch(v + W, nid, nd, nf, nsk0, nsk1);
+        if (skey0 != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
+        if (skey1 != NNM_EMPTY_KEY) slots[v * C + NNM_WAVE + lane] = NNM_EMPTY_KEY;
         [&]() {
             const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
             const uint64_t kth_key = wave_read64(pkey, K - 1);
@@ -1353,27 +1357,67 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             const uint32_t pflags = (uint32_t)__popcll(__ballot(lane < K && pf));
             // cheap filter first: estimates against the margin, exact keys
             // against the k-th key (or its rounding window)
-            bool valid = skey != NNM_EMPTY_KEY &&
-                         (rerank ? nnm_key_dist(skey) <= kd * rerank_tau
-                                 : (skey < kth_key || (tie_win > 1.f && nnm_key_dist(skey) <= kd * tie_win)));
-            if (!__ballot(valid)) {
+            auto pass = [&](uint64_t sk) {
+                return sk != NNM_EMPTY_KEY &&
+                       (rerank ? nnm_key_dist(sk) <= kd * rerank_tau
+                               : (sk < kth_key || (tie_win > 1.f && nnm_key_dist(sk) <= kd * tie_win)));
+            };
+            bool valid0 = pass(skey0), valid1 = pass(skey1);
+            if (!__ballot(valid0 || valid1)) {
                 c_flags += pflags;
                 return;
             }
-            const uint32_t sid = nnm_key_id(skey);
+            const uint32_t sid0 = nnm_key_id(skey0), sid1 = nnm_key_id(skey1);
             for (uint32_t p = 0; p < K; ++p) {
                 const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p);
-                valid = valid && q != sid;
+                valid0 = valid0 && q != sid0;
+                valid1 = valid1 && q != sid1;
             }
-            uint64_t ck = skey;
+            uint64_t ck0 = skey0, ck1 = skey1;
             if (rerank) {
-                if (valid) {
-                    const float ex = Xb_exact ? nnm_l2_rows_bf16<1>(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
-                                              : nnm_l2_rows<1>(X + v * ld, X + (uint64_t)sid * ld, ld);
-                    ck = nnm_key(ex, sid);
+                auto exact = [&](uint32_t sid) {
+                    return Xb_exact ? nnm_l2_rows_bf16<1>(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
+                                    : nnm_l2_rows<1>(X + v * ld, X + (uint64_t)sid * ld, ld);
+                };
+                if (valid0) ck0 = nnm_key(exact(sid0), sid0);
+                if (valid1) ck1 = nnm_key(exact(sid1), sid1);
+                c_rr += (uint32_t)__popcll(__ballot(valid0)) + (uint32_t)__popcll(__ballot(valid1));
+                valid0 = valid0 && (ck0 < kth_key || (tie_win > 1.f && nnm_key_dist(ck0) <= kd * tie_win));
+                valid1 = valid1 && (ck1 < kth_key || (tie_win > 1.f && nnm_key_dist(ck1) <= kd * tie_win));
+            }
+            // one candidate set, at most one candidate per lane
+            uint64_t ck;
+            bool valid;
+            {
+                const uint64_t vb0 = __ballot(valid0), vb1 = __ballot(valid1);
+                if (!vb1) {
+                    ck = ck0;
+                    valid = valid0;
+                } else if (!vb0) {
+                    ck = ck1;
+                    valid = valid1;
+                } else {
+                    const uint32_t n0 = (uint32_t)__popcll(vb0), n1 = (uint32_t)__popcll(vb1);
+                    if (n0 + n1 <= NNM_WAVE) {
+                        // lane j < n0 takes the j-th candidate of the low slots,
+                        // lane n0 + j the j-th of the high slots
+                        const bool from0 = lane < n0;
+                        const uint32_t src = nth_set_bit(from0 ? vb0 : vb1, from0 ? lane : lane - n0);
+                        const uint64_t a0 = shfl64(valid0 ? ck0 : NNM_EMPTY_KEY, src);
+                        const uint64_t a1 = shfl64(valid1 ? ck1 : NNM_EMPTY_KEY, src);
+                        valid = lane < n0 + n1;
+                        ck = valid ? (from0 ? a0 : a1) : NNM_EMPTY_KEY;
+                    } else {
+                        // more than 64 candidates: keep the 64 smallest keys (a
+                        // list takes at most K <= 32 of them), the minima of the
+                        // low set ascending against the high set descending
+                        const uint64_t s0 = nnm_wave_sort64(valid0 ? ck0 : NNM_EMPTY_KEY);
+                        const uint64_t s1 = nnm_wave_sort64(valid1 ? ck1 : NNM_EMPTY_KEY);
+                        const uint64_t r1 = shfl64(s1, NNM_WAVE - 1 - lane);
+                        ck = s0 < r1 ? s0 : r1;
+                        valid = ck != NNM_EMPTY_KEY;
+                    }
                 }
-                c_rr += (uint32_t)__popcll(__ballot(valid));
-                valid = valid && (ck < kth_key || (tie_win > 1.f && nnm_key_dist(ck) <= kd * tie_win));
             }
             const uint64_t vb = __ballot(valid);
             if (!vb) {
@@ -1454,7 +1498,8 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
         pid = nid;
         pd = nd;
         pf = nf;
-        skey = nskey;
+        skey0 = nsk0;
+        skey1 = nsk1;
     }
     if (lane == 0) {
         if (c_upd) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)c_upd);
@@ -1537,7 +1582,7 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
     const uint32_t R = p->rev_cap;
     const uint32_t cap_new = S + R, cap_old = K + R;
-    const uint32_t C = p->slots ? p->slots : 64;
+    const uint32_t C = p->slots ? p->slots : 128;
     if (c->l_new && c->cap_new == cap_new && c->cap_old == cap_old && c->slots_C == C) return NNM_OK;
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     free_iter(c);
@@ -1573,8 +1618,8 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->K != c->K) return nnm_fail(c, NNM_ERR_INVALID, "params K differs from the graph's K");
     if (!(p->rho > 0.f && p->rho <= 1.f)) return nnm_fail(c, NNM_ERR_INVALID, "rho must be in (0, 1]");
     if (p->rev_cap == 0 || p->rev_cap > 4096) return nnm_fail(c, NNM_ERR_INVALID, "rev_cap must be in [1, 4096]");
-    const uint32_t C = p->slots ? p->slots : 64;
-    if (C < 16 || C > 64 || (C & (C - 1))) return nnm_fail(c, NNM_ERR_INVALID, "slots must be 16, 32 or 64");
+    const uint32_t C = p->slots ? p->slots : 128;
+    if (C < 16 || C > 128 || (C & (C - 1))) return nnm_fail(c, NNM_ERR_INVALID, "slots must be 16, 32, 64 or 128");
     if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");

@@ -249,6 +249,32 @@ __device__ inline uint64_t wave_read64(uint64_t x, uint32_t src) {
     return ((uint64_t)wave_read32((uint32_t)(x >> 32), src) << 32) | wave_read32((uint32_t)x, src);
 }
 
+// 64-bit value of lane src (per lane; the wave must be converged).
+__device__ inline uint64_t shfl64(uint64_t x, uint32_t src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Position of the j-th set bit (from 0) of m; some position in [0, 63] when
+// m has j bits or fewer.
+__device__ inline uint32_t nth_set_bit(uint64_t m, uint32_t j) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t w = 32; w > 0; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(low);
+        if (j >= c) {
+            j -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
 // The library's FP32 squared L2: elements in order 0..d-1, one accumulator,
 // each step acc = fma(a-b, a-b, acc).  Every kernel that produces a stored
 // distance uses this order, so one pair always gives the same bits.
