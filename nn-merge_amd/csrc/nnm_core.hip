@@ -1307,8 +1307,8 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
 // for a batch): one wave per vertex, list entry j and slot j in lane j.  The
 // candidates that can enter (below the k-th key, or within the FP32 rounding
 // window of it) are de-duplicated against the list.  Every key is then ranked
-// against the list and against the other candidates by loops over lanes read
-// through scalar registers; all keys are distinct (a list holds distinct ids,
+// against the list and against the other candidates by one loop over the
+// candidates read through scalar registers; all keys are distinct (a list holds distinct ids,
 // an id always hashes to the same slot, candidates equal to a list id are
 // dropped), so the ranks give each kept entry its new position without a
 // sort.  Slots are reset to empty.  With rerank, lower-bound keys are
@@ -1367,14 +1367,16 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
                 c_flags += pflags;
                 return;
             }
-            const uint32_t sid0 = nnm_key_id(skey0), sid1 = nnm_key_id(skey1);
-            for (uint32_t p = 0; p < K; ++p) {
-                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p);
-                valid0 = valid0 && q != sid0;
-                valid1 = valid1 && q != sid1;
-            }
             uint64_t ck0 = skey0, ck1 = skey1;
             if (rerank) {
+                // de-dup before the exact recomputation (no distance for an id
+                // already listed)
+                const uint32_t sid0 = nnm_key_id(skey0), sid1 = nnm_key_id(skey1);
+                for (uint32_t p = 0; p < K; ++p) {
+                    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p);
+                    valid0 = valid0 && q != sid0;
+                    valid1 = valid1 && q != sid1;
+                }
                 auto exact = [&](uint32_t sid) {
                     return Xb_exact ? nnm_l2_rows_bf16<1>(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
                                     : nnm_l2_rows<1>(X + v * ld, X + (uint64_t)sid * ld, ld);
@@ -1419,7 +1421,28 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
                     }
                 }
             }
-            const uint64_t vb = __ballot(valid);
+            uint64_t vb = __ballot(valid);
+            const uint64_t lmask = K < NNM_WAVE ? (1ull << K) - 1 : ~0ull;  // list lanes
+            if (!rerank && vb) {
+                // de-dup against the list after the compaction (at most K of the
+                // kept keys repeat a list id, so the K best distinct candidates
+                // are still there): a loop over the fewer of candidates and
+                // list entries
+                if ((uint32_t)__popcll(vb) <= K) {
+                    const uint32_t cid = nnm_key_id(ck);
+                    for (uint64_t b = vb; b; b &= b - 1) {
+                        const uint32_t cl = (uint32_t)__builtin_ctzll(b);
+                        const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)cid, (int)cl);
+                        if (__ballot(pid == q) & lmask) vb &= ~(1ull << cl);
+                    }
+                    valid = (vb >> lane) & 1ull;
+                } else {
+                    const uint32_t cid = nnm_key_id(ck);
+                    for (uint32_t p = 0; p < K; ++p)
+                        valid = valid && (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p) != cid;
+                    vb = __ballot(valid);
+                }
+            }
             if (!vb) {
                 c_flags += pflags;
                 return;
@@ -1441,11 +1464,17 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
                     if (lane == p) below_cand_p = cnt;
                 }
             } else {
-                for (uint32_t p = 0; p < K; ++p) below_list += wave_read64(pkey, p) < ck ? 1u : 0u;
+                // one pass over the candidates: list and candidate keys are
+                // distinct, so the list keys below candidate c are the K list
+                // lanes minus those whose key c is below
                 for (uint64_t b = vb; b; b &= b - 1) {
-                    const uint64_t ci = wave_read64(ck, (uint32_t)__builtin_ctzll(b));
+                    const uint32_t cl = (uint32_t)__builtin_ctzll(b);
+                    const uint64_t ci = wave_read64(ck, cl);
+                    const bool before_p = ci < pkey;
                     below_cand_c += ci < ck ? 1u : 0u;
-                    below_cand_p += ci < pkey ? 1u : 0u;
+                    below_cand_p += before_p ? 1u : 0u;
+                    const uint32_t nb = K - (uint32_t)__popcll(__ballot(before_p) & lmask);
+                    if (lane == cl) below_list = nb;
                 }
             }
             uint32_t pos_p = lane + below_cand_p;
