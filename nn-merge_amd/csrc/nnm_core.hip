@@ -503,6 +503,50 @@ __device__ inline void wave_write_sorted_row(uint64_t key, uint32_t flag, uint32
     }
 }
 
+// Two rows per wave (K <= 32): half h = lane / 32 holds row 2w + h, entry j
+// in lane 32h + j.  Each half is sorted on its own by a bitonic network that
+// moves a 32-bit payload (the entry's flag) with its key; the last stage
+// uses block size 64 so that both halves come out ascending.
+template <uint32_t KB, uint32_t J>
+__device__ inline void bitonic_pass_kv(uint64_t& key, uint32_t& val) {
+    const uint64_t ok = wave_xor64<J>(key);
+    const uint32_t ov = wave_xor32<J>(val);
+    const uint32_t lane = nnm_lane();
+    const bool want_min = ((lane & KB) == 0) == ((lane & J) == 0);
+    const bool take = want_min ? ok < key : key < ok;
+    key = take ? ok : key;
+    val = take ? ov : val;
+    if constexpr (J > 1) bitonic_pass_kv<KB, J / 2>(key, val);
+}
+
+__device__ inline void half_sort_kv(uint64_t& key, uint32_t& val) {
+    bitonic_pass_kv<2, 1>(key, val);
+    bitonic_pass_kv<4, 2>(key, val);
+    bitonic_pass_kv<8, 4>(key, val);
+    bitonic_pass_kv<16, 8>(key, val);
+    bitonic_pass_kv<64, 16>(key, val);
+}
+
+#define NNM_HALF_LOOP(v0, n)                                                                             \
+    for (uint64_t v0 = 2 * ((uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE); v0 < (n); \
+         v0 += 2 * (uint64_t)gridDim.x * (blockDim.x / NNM_WAVE))
+
+// Write the half's row (entries in half-lanes [0, K)) sorted by key; rows
+// with ok false (past the end, or skipped) are left alone.
+__device__ inline void half_write_sorted_row(uint64_t key, uint32_t flag, bool ok, uint32_t K,
+                                             uint32_t* __restrict__ ids, float* __restrict__ dist,
+                                             uint8_t* __restrict__ flg) {
+    const uint32_t hl = nnm_lane() & 31u;
+    uint64_t s = hl < K ? key : NNM_EMPTY_KEY;
+    uint32_t f = flag;
+    half_sort_kv(s, f);
+    if (ok && hl < K) {
+        ids[hl] = nnm_key_id(s);
+        dist[hl] = nnm_key_dist(s);
+        flg[hl] = (uint8_t)f;
+    }
+}
+
 // K random distinct neighbours inside the row's group, self excluded.
 // Draws K+1 values like getRndSds(rng, K0+1, seeds, N) (src/kgraph.cpp:71).
 __device__ inline void wave_random_row(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
@@ -557,21 +601,23 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
                             uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                             uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters,
                             uint32_t* __restrict__ bad, nnm_metric_args mt) {
-    const uint32_t lane = nnm_lane();
+    const uint32_t hl = nnm_lane() & 31u;
     uint64_t nd = 0;
-    NNM_WAVE_LOOP(v, n) {
-        if (skip_raw && skip_raw[v] == NNM_RAW) continue;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (nnm_lane() >> 5);
+        const bool ok = v < n && !(skip_raw && skip_raw[v] == NNM_RAW);
+        if (!__ballot(ok)) continue;
         uint32_t id = 0;
         float dd = 0.f;
         uint32_t f = 1;
-        if (lane < K) {
-            id = g_ids[v * K + lane];
+        if (ok && hl < K) {
+            id = g_ids[v * K + hl];
             if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
-            dd = recompute ? nnm_row_dist(X, Xb, ld, v, id, mt) : g_dist[v * K + lane];
-            f = have_flags ? g_flag[v * K + lane] : 1u;
+            dd = recompute ? nnm_row_dist(X, Xb, ld, v, id, mt) : g_dist[v * K + hl];
+            f = have_flags ? g_flag[v * K + hl] : 1u;
             nd += recompute ? 1 : 0;
         }
-        wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+        half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
     wave_count(counters + CNT_DISTS, nd);
 }
@@ -583,13 +629,15 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
 __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restrict__ P, const float* __restrict__ pn,
                                  uint32_t dout, float kscale, uint32_t* __restrict__ g_ids,
                                  float* __restrict__ g_dist, uint8_t* __restrict__ g_flag) {
-    const uint32_t lane = nnm_lane();
-    NNM_WAVE_LOOP(v, n) {
+    const uint32_t hl = nnm_lane() & 31u;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (nnm_lane() >> 5);
+        const bool ok = v < n;
         uint32_t id = 0xFFFFFFFFu, f = 0;
         float dd = 3.402823466e38f;
-        if (lane < K) {
-            id = g_ids[v * K + lane];
-            f = g_flag[v * K + lane];
+        if (ok && hl < K) {
+            id = g_ids[v * K + hl];
+            f = g_flag[v * K + hl];
             // rows of dout bf16 (a multiple of 8): 16-byte loads, products in k order
             const uint4* a = reinterpret_cast<const uint4*>(P + v * dout);
             const uint4* b = reinterpret_cast<const uint4*>(P + (uint64_t)id * dout);
@@ -605,7 +653,7 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
             }
             dd = fmaxf((pn[v] + pn[id]) - 2.f * dot, 0.f) * kscale;
         }
-        wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+        half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
 }
 
@@ -615,19 +663,21 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
 __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ X, const uint16_t* __restrict__ Xb_exact,
                               uint32_t ld, int renew, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
                               uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters) {
-    const uint32_t lane = nnm_lane();
+    const uint32_t hl = nnm_lane() & 31u;
     uint64_t nd = 0;
-    NNM_WAVE_LOOP(v, n) {
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (nnm_lane() >> 5);
+        const bool ok = v < n;
         uint32_t id = 0xFFFFFFFFu, f = 0;
         float dd = 3.402823466e38f;
-        if (lane < K) {
-            id = g_ids[v * K + lane];
-            f = renew ? 1u : g_flag[v * K + lane];
+        if (ok && hl < K) {
+            id = g_ids[v * K + hl];
+            f = renew ? 1u : g_flag[v * K + hl];
             dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
                           : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
             ++nd;
         }
-        wave_write_sorted_row(nnm_key(dd, id), f, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+        half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
     wave_count(counters + CNT_RERANKED, nd);
 }
@@ -833,7 +883,7 @@ extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist,
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n)), dim3(256), 0, c->stream, c->X, nullptr, c->ld, c->n,
+    hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, (c->n + 1) / 2)), dim3(256), 0, c->stream, c->X, nullptr, c->ld, c->n,
                        K, dist ? 0 : 1, flags ? 1 : 0, nullptr, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
                        metric_args(c));
     NNM_LAUNCH_CHECK(c);
@@ -898,7 +948,7 @@ extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
             xb = c->Xb;
         }
         uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
-        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld,
+        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X, xb, c->ld,
                            c->n_act, c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
                            metric_args(c));
         NNM_LAUNCH_CHECK(c);
@@ -1864,7 +1914,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     bool tail = !learned;
     bool estimates = false;  // the lists hold learned estimates
     auto to_estimates = [&]() -> int {
-        hipLaunchKernelGGL(k_lists_estimate, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
+        hipLaunchKernelGGL(k_lists_estimate, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K,
                            nnm_mlp_rows(c), nnm_mlp_norms(c), nnm_mlp_dout(c), nnm_mlp_scale(c), c->g_ids, c->g_dist,
                            c->g_flag);
         NNM_LAUNCH_CHECK(c);
@@ -1872,7 +1922,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         return NNM_OK;
     };
     auto to_exact = [&]() -> int {
-        hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
+        hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
                            (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld,
                            (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, c->g_ids, c->g_dist, c->g_flag, c->counters);
         NNM_LAUNCH_CHECK(c);
