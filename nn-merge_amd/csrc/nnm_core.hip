@@ -579,6 +579,73 @@ __device__ inline void wave_random_row(const float* __restrict__ X, const uint16
     wave_write_sorted_row(key, 1u, K, ids, dist, flg);
 }
 
+// nnm_wave_rnd_sds for each 32-lane half on its own (k <= 32; seed, stream,
+// k and M may differ between the halves): the same values in half-lanes
+// [0, k) as the full-wave draw gives in lanes [0, k).
+__device__ inline uint32_t half_rnd_sds(uint64_t seed, uint64_t stream, uint32_t k, uint32_t M) {
+    const uint32_t hl = nnm_lane() & 31u;
+    uint32_t r = 0xFFFFFFFFu;
+    if (hl < k && M > k) r = (uint32_t)(nnm_rand(seed, stream, hl) % (uint64_t)(M - k));
+    r = bitonic_pass32<2, 1>(r);
+    r = bitonic_pass32<4, 2>(r);
+    r = bitonic_pass32<8, 4>(r);
+    r = bitonic_pass32<16, 8>(r);
+    r = bitonic_pass32<64, 16>(r);  // both halves ascending
+    // bump inside the half: a'[i] = i + max_{j<=i}(a[j] - j)
+    int32_t t = hl < k ? (int32_t)r - (int32_t)hl : -0x7FFFFFFF;
+#pragma unroll
+    for (uint32_t o = 1; o < 32; o <<= 1) {
+        const int32_t u = __shfl_up(t, o, NNM_WAVE);
+        if (hl >= o) t = t > u ? t : u;
+    }
+    const uint32_t off = (uint32_t)(nnm_rand(seed, stream, 0xFFFFu) % M);
+    if (hl >= k) return M;
+    if (M == k) return hl;
+    return (uint32_t)(((uint64_t)(t + (int32_t)hl) + off) % M);
+}
+
+// Bits of the lane's half of a wave mask, in the low 32 bits.
+__device__ inline uint64_t half_bits(uint64_t m) { return (nnm_lane() & 32u) ? m >> 32 : m & 0xFFFFFFFFull; }
+
+// wave_random_row for the half's row (K + 1 <= 32); ok false: no row.
+__device__ inline void half_random_row(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
+                                       uint64_t v, bool ok, uint32_t gb, uint32_t gsize, uint32_t K, uint64_t seed,
+                                       uint32_t* __restrict__ ids, float* __restrict__ dist,
+                                       uint8_t* __restrict__ flg, uint64_t* nd, const nnm_metric_args& mt) {
+    const uint32_t hl = nnm_lane() & 31u;
+    const uint32_t r = half_rnd_sds(seed, v, K + 1, gsize);
+    const uint32_t id = gb + r;
+    const uint64_t m = half_bits(nnm_ballot(hl < K + 1 && id != (uint32_t)v));
+    // half-lane t < K takes the id of the half-lane holding the t-th kept draw
+    const uint32_t src = (nnm_lane() & 32u) + nth_set_bit(m, hl < K ? hl : 0u);
+    const uint32_t myid = (uint32_t)__shfl((int)id, (int)src, NNM_WAVE);
+    float dd = 0.f;
+    if (ok && hl < K) {
+        dd = nnm_row_dist(X, Xb, ld, v, myid, mt);
+        *nd += 1;
+    }
+    half_write_sorted_row(nnm_key(dd, myid), 1u, ok, K, ids, dist, flg);
+}
+
+// Random initial lists, two rows per wave (K <= 31).
+__global__ void k_random_init_half(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K,
+                                   const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_grp,
+                                   const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
+                                   uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                                   uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters,
+                                   nnm_metric_args mt) {
+    uint64_t nd = 0;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (nnm_lane() >> 5);
+        const bool ok = v < n;
+        const uint32_t g = ok ? sub_grp[vsub[v]] : 0u;
+        const uint32_t gb = ok ? grp_begin[g] : 0u, gsize = ok ? grp_end[g] - gb : K + 1;
+        half_random_row(X, nullptr, ld, v, ok, gb, gsize, K, seed, g_ids + v * K, g_dist + v * K, g_flag + v * K,
+                        &nd, mt);
+    }
+    wave_count(counters + CNT_DISTS, nd);
+}
+
 __global__ void k_random_init(const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K,
                               const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_grp,
                               const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
@@ -759,6 +826,76 @@ __global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __rest
     wave_count(counters + CNT_DISTS, nd);
 }
 
+// k_merge_init with two rows per wave (K <= 31, so K + 1 draws fit a half):
+// the same lists, parked entries and distance count.
+__global__ void k_merge_init_half(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
+                                  uint64_t n, uint32_t K, uint32_t cut,
+                                  const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_begin,
+                                  const uint32_t* __restrict__ sub_grp, const uint8_t* __restrict__ sub_built,
+                                  const uint32_t* __restrict__ grp_begin, const uint32_t* __restrict__ grp_end,
+                                  uint64_t seed, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                                  uint8_t* __restrict__ g_flag, uint32_t* __restrict__ c_ids,
+                                  float* __restrict__ c_dist, uint8_t* __restrict__ c_flag,
+                                  uint8_t* __restrict__ has_cpn, uint64_t* __restrict__ counters,
+                                  nnm_metric_args mt) {
+    const uint32_t lane = nnm_lane(), hl = lane & 31u, hbase = lane & 32u;
+    uint64_t nd = 0;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (lane >> 5);
+        const bool ok = v < n;
+        const uint32_t s = ok ? vsub[v] : 0u;
+        const uint32_t g = ok ? sub_grp[s] : 0u;
+        const uint32_t gb = ok ? grp_begin[g] : 0u, gsize = ok ? grp_end[g] - gb : K + 1;
+        const bool built = ok && sub_built[s];
+        uint32_t* ids = g_ids + v * K;
+        float* dist = g_dist + v * K;
+        uint8_t* flg = g_flag + v * K;
+        // the two halves may take different branches: every cross-lane step
+        // below reads lanes of its own half only
+        if (!built) {
+            if (ok && hl == 0) has_cpn[v] = 0;
+            half_random_row(X, Xb, ld, v, ok, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd, mt);
+        } else {
+            const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
+            const uint32_t remain = K - cut;
+            uint32_t pid = 0, pf = 0;
+            float pd = 0.f;
+            if (hl < K) { pid = ids[hl]; pd = dist[hl]; pf = flg[hl]; }
+            // park the far entries (cpnGraph)
+            if (hl >= remain && hl < K) {
+                c_ids[v * cut + (hl - remain)] = pid;
+                c_dist[v * cut + (hl - remain)] = pd;
+                c_flag[v * cut + (hl - remain)] = (uint8_t)pf;
+            }
+            if (hl == 0) has_cpn[v] = 1;
+            const uint32_t others = gsize - ssize;
+            const uint32_t draw = 2 * cut < others ? 2 * cut : others;
+            const uint32_t r = half_rnd_sds(seed, v, draw, others > 0 ? others : 1u);
+            const uint32_t cand = r < (sb - gb) ? gb + r : gb + r + ssize;
+            bool dup = false;
+            for (uint32_t p = 0; p < K; ++p) {
+                const uint32_t q0 = wave_read32(pid, p), q1 = wave_read32(pid, 32 + p);
+                dup |= (hbase ? q1 : q0) == cand;
+            }
+            const bool fresh = hl < draw && !dup;
+            const uint64_t m = half_bits(nnm_ballot(fresh));
+            const uint32_t fresh_n = (uint32_t)__popcll(m) < cut ? (uint32_t)__popcll(m) : cut;
+            // fresh candidate of rank j to half-lane remain + j
+            const bool takes = hl >= remain && hl < remain + fresh_n;
+            const uint32_t src = hbase + nth_set_bit(m, takes ? hl - remain : 0u);
+            const uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
+            if (takes) {
+                pid = got;
+                pd = nnm_row_dist(X, Xb, ld, got, v, mt);
+                pf = 1;
+                nd += 1;
+            }
+            half_write_sorted_row(nnm_key(pd, pid), pf, true, K, ids, dist, flg);
+        }
+    }
+    wave_count(counters + CNT_DISTS, nd);
+}
+
 // merge2one: pool (K, sorted) + parked (cut, sorted) -> first K by (dist, id).
 __global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, const uint8_t* __restrict__ has_cpn,
                                uint32_t* __restrict__ g_ids, float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
@@ -854,9 +991,14 @@ extern "C" int nnm_graph_random_init(nnm_ctx* c, uint32_t K, uint64_t seed, nnm_
     NNM_TRY(ensure_graph(c, K));
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_random_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld,
-                       c->n_act, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
-                       c->g_flag, c->counters, metric_args(c));
+    if (K + 1 <= 32)  // two rows per wave
+        hipLaunchKernelGGL(k_random_init_half, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X,
+                           c->ld, c->n_act, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids,
+                           c->g_dist, c->g_flag, c->counters, metric_args(c));
+    else
+        hipLaunchKernelGGL(k_random_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, c->ld,
+                           c->n_act, K, c->vsub, c->sub_grp, c->grp_begin, c->grp_end, seed, c->g_ids, c->g_dist,
+                           c->g_flag, c->counters, metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
@@ -1022,10 +1164,16 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
     }
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld, c->n_act,
-                       K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin, c->grp_end,
-                       p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
-                       c->c_flag, c->has_cpn, c->counters, metric_args(c));
+    if (K + 1 <= 32)  // two rows per wave
+        hipLaunchKernelGGL(k_merge_init_half, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X,
+                           xb, c->ld, c->n_act, K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built,
+                           c->grp_begin, c->grp_end, p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist,
+                           c->g_flag, c->c_ids, c->c_dist, c->c_flag, c->has_cpn, c->counters, metric_args(c));
+    else
+        hipLaunchKernelGGL(k_merge_init, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->X, xb, c->ld,
+                           c->n_act, K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built, c->grp_begin,
+                           c->grp_end, p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist, c->g_flag,
+                           c->c_ids, c->c_dist, c->c_flag, c->has_cpn, c->counters, metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];

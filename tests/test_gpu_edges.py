@@ -150,6 +150,55 @@ def test_many_ragged_partitions(ctx):
     assert r_gpu >= r_cpu - TOL
 
 
+@pytest.mark.parametrize("K", [31, 32])
+def test_merge_init_rows_both_widths(ctx, K):
+    """J-Merge set-up at K = 31 (two rows per wave, K + 1 draws fill a half)
+    and K = 32 (one row per wave).  Built rows keep their K - cut nearest
+    entries and take the rest from the raw half when fresh ones exist; raw
+    rows draw K distinct others; flags of new entries set; distances exact.
+    The merge then reaches the oracle's recall."""
+    n, half = 3000, 1500
+    X = nnm_data.sift_like(n, 64, centers=16, sigma=20.0, seed=95 + K)
+    ctx.set_data(X[:half])
+    ctx.random_init(K, seed=3)
+    ctx.nndescent(nnm.default_params(K=K, seed=3, max_iters=12))
+    g0, d0, _ = ctx.get_graph()
+    init = ((np.arange(n)[:, None] + 1 + np.arange(K)[None, :]) % n).astype(np.uint32)
+    init[:half] = g0
+    ctx.set_data(X)
+    ctx.set_partition([0, half, n], [1, 0], [0, 0])
+    ctx.set_graph(init)
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, seed=6, max_iters=12)
+    ctx.merge_init(p)
+    ids, dist, flags = ctx.get_graph()
+    _invariants(ids, dist, n)
+    cut = K // 2
+    remain = K - cut
+    for v in range(0, half, 7):
+        kept = set(g0[v, :remain].tolist())
+        row = set(ids[v].tolist())
+        assert kept <= row, f"built row {v} lost a near entry"
+        extra = row - kept
+        far = set(g0[v, remain:].tolist())
+        assert all(u >= half or u in far for u in extra), f"built row {v}: entry from nowhere"
+        assert any(u >= half for u in extra), f"built row {v}: no fresh entry"
+    assert flags[half:].all()
+    v = np.repeat(np.arange(0, n, 11), K)
+    e = ids[::11].ravel()
+    exact = np.array([orc.l2_seq(X[i], X[j]) for i, j in zip(v, e)], np.float32)
+    assert np.array_equal(dist[::11].ravel(), exact)
+    ctx.nndescent(p)
+    ctx.merge_finish(True)
+    ids, dist, _ = ctx.get_graph()
+    _invariants(ids, dist, n)
+    rows = np.arange(0, n, 3, dtype=np.uint32)
+    gt, _ = orc.bruteforce(X, 10, rows=rows)
+    oi, _, _ = orc.merge(X, [0, half, n], [1, 0], init, K=K, seed=6, max_iters=12)
+    r_gpu, r_cpu = nnm.recall(ids, gt, rows, 10), orc.recall(oi, rows, gt, 10)
+    print("K", K, "merge gpu", r_gpu, "cpu", r_cpu)
+    assert r_gpu >= r_cpu - TOL
+
+
 def test_refused_inputs(ctx):
     """Inputs the host must refuse before launching anything."""
     X = nnm_data.sift_like(200, 16, centers=4, seed=91)
