@@ -281,7 +281,7 @@ static void free_graph(nnm_ctx* c) {
 static void free_iter(nnm_ctx* c) {
     void** ps[] = {(void**)&c->l_new, (void**)&c->l_old, (void**)&c->n_new, (void**)&c->n_old,
                    (void**)&c->newmask, (void**)&c->rcnt, (void**)&c->roff, (void**)&c->rcur,
-                   (void**)&c->rbuf, (void**)&c->ocnt, (void**)&c->obuf, (void**)&c->ovf_bits,
+                   (void**)&c->rbuf, (void**)&c->ocnt, (void**)&c->obuf, (void**)&c->spill,
                    (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->kth,
                    (void**)&c->vmeta};
     for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
@@ -1229,7 +1229,7 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
                                                   uint32_t cap_new, uint32_t* __restrict__ l_old, uint32_t cap_old,
                                                   uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                                                   uint64_t* __restrict__ newmask, uint32_t* __restrict__ rcnt,
-                                                  uint32_t* __restrict__ rbuf) {
+                                                  uint32_t* __restrict__ rbuf, uint64_t* __restrict__ spill) {
     constexpr uint32_t GPW = NNM_WAVE / G;  // rows per wave
     const uint32_t lane = nnm_lane(), grp = lane / G, gl = lane % G;
     const uint64_t gmask = (G == NNM_WAVE) ? ~0ull : (((1ull << G) - 1) << (grp * G));
@@ -1293,64 +1293,57 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
             pos[k] = rev[k] ? atomicAdd(&rcnt[b[k]], 1u) : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < R; ++k)
+        for (int k = 0; k < R; ++k) {
             if (rev[k] && pos[k] < NNM_RB) rbuf[b[k] * NNM_RB + pos[k]] = (uint32_t)v[k];
-    }
-}
-
-// Buckets that overflowed: their full size (scanned into overflow offsets)
-// and a bit in the overflow map.
-__global__ void k_lb_ovf_mark(uint64_t m, const uint32_t* __restrict__ rcnt, uint32_t* __restrict__ ocnt,
-                              uint32_t* __restrict__ ovf_bits) {
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < m; b += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t c = rcnt[b];
-        const bool o = c > NNM_RB;
-        ocnt[b] = o ? c : 0u;
-        if (o) {
-            atomicOr(&ovf_bits[b >> 5], 1u << (b & 31));
-            atomicOr(&ovf_bits[(m + 31) / 32], 1u);  // any overflow at all
+            // entries past their bucket: a bit per list entry, one word per row
+            const uint64_t sm = nnm_ballot(rev[k] && pos[k] >= NNM_RB) & gmask;
+            if (gl == 0 && v[k] < n) spill[v[k]] = (G == NNM_WAVE) ? sm : (sm >> (grp * G));
         }
     }
 }
 
-// List build, overflow pass: every reverse entry of an overflowed bucket
-// again, into the overflow CSR (the bucket holds only a prefix of them).
-template <uint32_t G>
-__global__ __launch_bounds__(256) void k_lb_ovf_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g_ids,
-                                                     const float* __restrict__ g_dist, const float* __restrict__ kthv,
-                                                     const uint8_t* __restrict__ g_flag,
-                                                     const uint64_t* __restrict__ newmask,
-                                                     const uint32_t* __restrict__ ovf_bits,
-                                                     const uint32_t* __restrict__ roff, uint32_t* __restrict__ rcur,
-                                                     uint32_t* __restrict__ obuf) {
-    constexpr uint32_t GPW = NNM_WAVE / G;
-    if (ovf_bits[(2 * n + 31) / 32] == 0u) return;  // no bucket overflowed
-    const uint32_t lane = nnm_lane(), grp = lane / G, gl = lane % G;
-    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / NNM_WAVE;
-    const uint64_t nw = (uint64_t)gridDim.x * blockDim.x / NNM_WAVE;
-    for (uint64_t v0 = wid * GPW; v0 < n; v0 += nw * GPW) {
-        const uint64_t v = v0 + grp;
-        if (v >= n || gl >= K) continue;
-        const bool is_new = (newmask[v] >> gl) & 1ull;
-        const bool is_old = !is_new && g_flag[v * K + gl] == 0;
-        if (!is_new && !is_old) continue;
-        const uint32_t u = g_ids[v * K + gl];
-        const uint64_t b = is_new ? (uint64_t)u : n + u;
-        if (!((ovf_bits[b >> 5] >> (b & 31)) & 1u)) continue;
-        if (!(kthv[u] < g_dist[v * K + gl])) continue;
-        const uint32_t pos = atomicAdd(&rcur[b], 1u);
-        obuf[roff[b] + pos] = (uint32_t)v;
+// Entries past each full bucket: their count (scanned into offsets into
+// the overflow entries).
+__global__ void k_lb_ovf_mark(uint64_t m, const uint32_t* __restrict__ rcnt, uint32_t* __restrict__ ocnt) {
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < m; b += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = rcnt[b];
+        ocnt[b] = c > NNM_RB ? c - NNM_RB : 0u;
     }
 }
 
-// Append the reverse entries to the candidate lists.  More than cap entries:
-// keep the cap entries with the smallest random key (a seeded, order-free
-// stand-in for random_shuffle + resize, src/abstractmerge.cpp:149-159).
-__device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ src, uint32_t cnt, uint32_t cap,
-                                               uint32_t salt, uint32_t* __restrict__ dst) {
+// List build, overflow pass: the entries past their bucket (marked per row
+// by the build) grouped by bucket.  Order inside a bucket is free: the
+// reverse cap picks by seeded key.
+__global__ void k_lb_spill_fill(uint64_t n, uint32_t K, const uint32_t* __restrict__ g_ids,
+                                const uint64_t* __restrict__ newmask, const uint64_t* __restrict__ spill,
+                                const uint32_t* __restrict__ roff, uint32_t* __restrict__ rcur,
+                                uint32_t* __restrict__ obuf) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t m = spill[v];
+        if (!m) continue;
+        const uint64_t nm = newmask[v];
+        for (; m; m &= m - 1) {
+            const uint32_t gl = (uint32_t)__builtin_ctzll(m);
+            const uint32_t u = g_ids[v * K + gl];
+            const uint64_t b = ((nm >> gl) & 1ull) ? (uint64_t)u : n + u;
+            const uint32_t pos = atomicAdd(&rcur[b], 1u);
+            obuf[roff[b] + pos] = (uint32_t)v;
+        }
+    }
+}
+
+// Append the reverse entries to the candidate lists.  A bucket's entries
+// are its first NNM_RB in rbuf, then the rest in the overflow entries.  More
+// than cap entries: keep the cap entries with the smallest random key (a
+// seeded, order-free stand-in for random_shuffle + resize,
+// src/abstractmerge.cpp:149-159).
+__device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ head, const uint32_t* __restrict__ rest,
+                                               uint32_t cnt, uint32_t cap, uint32_t salt,
+                                               uint32_t* __restrict__ dst) {
     const uint32_t lane = nnm_lane();
+    auto at = [&](uint32_t i) { return i < NNM_RB ? head[i] : rest[i - NNM_RB]; };
     if (cnt <= cap) {
-        for (uint32_t i = lane; i < cnt; i += NNM_WAVE) dst[i] = src[i];
+        for (uint32_t i = lane; i < cnt; i += NNM_WAVE) dst[i] = at(i);
         return cnt;
     }
     // find the cap-th smallest 64-bit key (hash << 32 | id), bit by bit
@@ -1359,7 +1352,7 @@ __device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ src,
         const uint64_t trial = res + (1ull << b);
         uint32_t below = 0;
         for (uint32_t i = lane; i < cnt; i += NNM_WAVE) {
-            const uint32_t id = src[i];
+            const uint32_t id = at(i);
             const uint64_t key = ((uint64_t)nnm_hash32(id, salt) << 32) | id;
             below += key < trial ? 1u : 0u;
         }
@@ -1373,7 +1366,7 @@ __device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ src,
         bool keep = false;
         uint32_t id = 0;
         if (i < cnt) {
-            id = src[i];
+            id = at(i);
             const uint64_t key = ((uint64_t)nnm_hash32(id, salt) << 32) | id;
             keep = key <= res;
         }
@@ -1396,11 +1389,12 @@ __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const
     NNM_WAVE_LOOP(v, n) {
         const uint32_t fn = n_new[v], fo = n_old[v];
         const uint32_t cn = rcnt[v], co = rcnt[n + v];
-        const uint32_t* sn = cn <= NNM_RB ? rbuf + v * NNM_RB : obuf + roff[v];
-        const uint32_t* so = co <= NNM_RB ? rbuf + (n + v) * NNM_RB : obuf + roff[n + v];
-        const uint32_t an = wave_append_reverse(sn, cn, cap, nnm_hash32((uint32_t)v, salt), l_new + v * cap_new + fn);
-        const uint32_t ao = wave_append_reverse(so, co, cap, nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u),
-                                                l_old + v * cap_old + fo);
+        const uint32_t* rn = cn > NNM_RB ? obuf + roff[v] : nullptr;
+        const uint32_t* ro = co > NNM_RB ? obuf + roff[n + v] : nullptr;
+        const uint32_t an = wave_append_reverse(rbuf + v * NNM_RB, rn, cn, cap, nnm_hash32((uint32_t)v, salt),
+                                                l_new + v * cap_new + fn);
+        const uint32_t ao = wave_append_reverse(rbuf + (n + v) * NNM_RB, ro, co, cap,
+                                                nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u), l_old + v * cap_old + fo);
         if (lane == 0) {
             n_new[v] = fn + an;
             n_old[v] = fo + ao;
@@ -1829,7 +1823,7 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     NNM_TRY(alloc_n(c, &c->rbuf, 2 * n * NNM_RB));
     NNM_TRY(alloc_n(c, &c->ocnt, 2 * n));
     NNM_TRY(alloc_n(c, &c->obuf, n * K));
-    NNM_TRY(alloc_n(c, &c->ovf_bits, (2 * n + 31) / 32 + 1));
+    NNM_TRY(alloc_n(c, &c->spill, n));
     NNM_TRY(alloc_n(c, &c->scan_tmp, (2 * n + SCAN_TILE - 1) / SCAN_TILE + 1));
     NNM_TRY(alloc_n(c, &c->slots, n * C));
     NNM_TRY(alloc_n(c, &c->kth, n));
@@ -1875,7 +1869,6 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
     NNM_TRY(fill_u32(c, c->rcnt, 2 * n, 0));
     NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
-    NNM_TRY(fill_u32(c, c->ovf_bits, (2 * n + 31) / 32 + 1, 0));
     // row norms in the join's metadata: bf16 norms (norm form), learned
     // projection norms, or the cosine metric's squared norms
     const float* nrm = p->prec == NNM_PREC_BF16 ? c->norms
@@ -1888,25 +1881,19 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     if (K <= 32) {
         hipLaunchKernelGGL(k_lb_build<32>, dim3(lb_grid), dim3(256), 0, c->stream, n, K, S, c->g_ids, c->g_dist,
                            c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
-                           c->newmask, c->rcnt, c->rbuf);
+                           c->newmask, c->rcnt, c->rbuf, c->spill);
     } else {
         hipLaunchKernelGGL(k_lb_build<64>, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, S, c->g_ids,
                            c->g_dist, c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,
-                           c->n_old, c->newmask, c->rcnt, c->rbuf);
+                           c->n_old, c->newmask, c->rcnt, c->rbuf, c->spill);
     }
     NNM_LAUNCH_CHECK(c);
     const uint64_t m = 2 * n;
-    hipLaunchKernelGGL(k_lb_ovf_mark, dim3(grid_for(m, 256, 8192)), dim3(256), 0, c->stream, m, c->rcnt, c->ocnt,
-                       c->ovf_bits);
+    hipLaunchKernelGGL(k_lb_ovf_mark, dim3(grid_for(m, 256, 8192)), dim3(256), 0, c->stream, m, c->rcnt, c->ocnt);
     NNM_LAUNCH_CHECK(c);
     NNM_TRY(nnm_scan_u32(c, c->ocnt, m, c->roff, c->scan_tmp));
-    if (K <= 32) {
-        hipLaunchKernelGGL(k_lb_ovf_fill<32>, dim3(lb_grid), dim3(256), 0, c->stream, n, K, c->g_ids, c->g_dist,
-                           c->kth, c->g_flag, c->newmask, c->ovf_bits, c->roff, c->rcur, c->obuf);
-    } else {
-        hipLaunchKernelGGL(k_lb_ovf_fill<64>, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, c->g_ids,
-                           c->g_dist, c->kth, c->g_flag, c->newmask, c->ovf_bits, c->roff, c->rcur, c->obuf);
-    }
+    hipLaunchKernelGGL(k_lb_spill_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, n, K, c->g_ids,
+                       c->newmask, c->spill, c->roff, c->rcur, c->obuf);
     NNM_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
                        c->rbuf, c->obuf, c->rev_cap, p->seed, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,

@@ -112,11 +112,11 @@ struct nnm_ctx {
     uint64_t* newmask = nullptr;    // n: list entries taken as new samples
     uint32_t* rcnt = nullptr;       // 2n reverse counts (new, old)
     uint32_t* rbuf = nullptr;       // 2n x NNM_RB reverse buckets
-    uint32_t* ocnt = nullptr;       // 2n overflow sizes (count when above NNM_RB)
+    uint32_t* ocnt = nullptr;       // 2n overflow sizes (count past NNM_RB)
     uint32_t* roff = nullptr;       // 2n + 1 overflow offsets
     uint32_t* rcur = nullptr;       // 2n overflow cursors
-    uint32_t* obuf = nullptr;       // n*K overflow entries
-    uint32_t* ovf_bits = nullptr;   // 2n bits: bucket overflowed
+    uint32_t* obuf = nullptr;       // n*K overflow entries, grouped by bucket
+    uint64_t* spill = nullptr;      // n: per row, the list entries past a full bucket (bit j: entry j)
     uint32_t* scan_tmp = nullptr;   // block sums
     uint64_t* slots = nullptr;      // n x slots_C
     uint64_t* counters = nullptr;   // CNT_N

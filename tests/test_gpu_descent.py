@@ -162,6 +162,37 @@ def test_single_iteration_counts(ctx):
     assert st["iters"] == 1
 
 
+@pytest.mark.parametrize("cap", [1000, 100])
+def test_hub_bucket_overflow_counts(ctx, cap):
+    """A hub: every row lists row 0, whose reverse bucket then holds far more
+    entries than the 64 a bucket keeps (the rest go through the overflow
+    pass).  The joined list sizes and pair counts equal the closed form: all
+    reverse entries under a large cap, exactly cap of them under cap 100."""
+    n, K = 600, 10
+    X = nnm_data.sift_like(n, 32, centers=4, seed=61)
+    gi, _ = orc.bruteforce(X, K)
+    ids = gi.astype(np.uint32).copy()
+    for v in range(1, n):
+        if 0 not in ids[v]:
+            ids[v, -1] = 0
+    ctx.set_data(X)
+    ctx.set_graph(ids)
+    ids, dist, _ = ctx.get_graph()
+    kth = dist[:, -1]
+    rev = np.zeros(n, np.int64)
+    for v in range(n):
+        for e in range(K):
+            u = ids[v, e]
+            if kth[u] < dist[v, e]:
+                rev[u] += 1
+    assert rev[0] > 64 + 100
+    p = nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=2, rev_cap=cap)
+    st = ctx.local_join(p, 0)
+    n_new = K + np.minimum(rev, cap)
+    assert st["list_entries"] == int(n_new.sum())
+    assert st["dists"] == int((n_new * (n_new - 1) // 2).sum())
+
+
 @pytest.mark.parametrize("d", [128, 96, 64, 200])
 def test_join_distances_exact(ctx, d):
     """Every distance the join writes into a list equals the library's
