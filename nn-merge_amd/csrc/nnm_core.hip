@@ -939,6 +939,45 @@ __global__ void k_merge_finish(uint64_t n, uint32_t K, uint32_t cut, int dedup, 
     }
 }
 
+// k_merge_finish with dedup and two rows per wave (K + cut <= 32): parked
+// entries repeating a pool id are dropped first, so every kept key is
+// distinct and each flag can travel with its key through the sort.
+__global__ void k_merge_finish_half(uint64_t n, uint32_t K, uint32_t cut, const uint8_t* __restrict__ has_cpn,
+                                    uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
+                                    uint8_t* __restrict__ g_flag, const uint32_t* __restrict__ c_ids,
+                                    const float* __restrict__ c_dist, const uint8_t* __restrict__ c_flag) {
+    const uint32_t lane = nnm_lane(), hl = lane & 31u, hbase = lane & 32u;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (lane >> 5);
+        const bool ok = v < n && has_cpn[v];
+        if (!__ballot(ok)) continue;
+        uint64_t key = NNM_EMPTY_KEY;
+        uint32_t f = 0, id = 0xFFFFFFFFu;
+        const bool parked = hl >= K && hl < K + cut;
+        if (ok && hl < K) {
+            id = g_ids[v * K + hl];
+            key = nnm_key(g_dist[v * K + hl], id);
+            f = g_flag[v * K + hl];
+        } else if (ok && parked) {
+            id = c_ids[v * cut + hl - K];
+            key = nnm_key(c_dist[v * cut + hl - K], id);
+            f = c_flag[v * cut + hl - K];
+        }
+        bool dup = false;
+        for (uint32_t p = 0; p < K; ++p) {
+            const uint32_t q0 = wave_read32(id, p), q1 = wave_read32(id, 32 + p);
+            dup |= (hbase ? q1 : q0) == id;
+        }
+        if (parked && dup) key = NNM_EMPTY_KEY;
+        half_sort_kv(key, f);
+        if (ok && hl < K) {
+            g_ids[v * K + hl] = nnm_key_id(key);
+            g_dist[v * K + hl] = nnm_key_dist(key);
+            g_flag[v * K + hl] = (uint8_t)f;
+        }
+    }
+}
+
 __global__ void k_pair_dists(const float* __restrict__ X, uint32_t ld, const uint32_t* __restrict__ a,
                              const uint32_t* __restrict__ b, uint64_t m, float* __restrict__ out,
                              nnm_metric_args mt) {
@@ -1191,8 +1230,14 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
 extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
     if (!c) return NNM_ERR_INVALID;
     if (!c->cpn_live) return nnm_fail(c, NNM_ERR_STATE, "nnm_merge_finish: no merge in progress");
-    hipLaunchKernelGGL(k_merge_finish, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
-                       c->cut, dedup, c->has_cpn, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist, c->c_flag);
+    if (dedup && c->K + c->cut <= 32)  // two rows per wave
+        hipLaunchKernelGGL(k_merge_finish_half, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream,
+                           c->n_act, c->K, c->cut, c->has_cpn, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
+                           c->c_flag);
+    else
+        hipLaunchKernelGGL(k_merge_finish, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
+                           c->cut, dedup, c->has_cpn, c->g_ids, c->g_dist, c->g_flag, c->c_ids, c->c_dist,
+                           c->c_flag);
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     c->cpn_live = 0;
@@ -1377,27 +1422,53 @@ __device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ head
     return out;
 }
 
+// Two rows per wave: a half copies its row's reverse lists when both fit
+// the cap; a row with a list above the cap is then taken by the whole wave
+// (the seeded selection).
 __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ roff,
                             const uint32_t* __restrict__ rbuf, const uint32_t* __restrict__ obuf, uint32_t cap,
                             uint64_t seed,
                             uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
                             uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                             const uint64_t* __restrict__ counters) {
-    const uint32_t lane = nnm_lane();
+    const uint32_t lane = nnm_lane(), hl = lane & 31u;
     // sampling salt of this iteration (the iteration number lives on the device)
     const uint32_t salt = (uint32_t)(nnm_rand(seed, 0xC0FFEEull, counters[CNT_ITER]) >> 32);
-    NNM_WAVE_LOOP(v, n) {
-        const uint32_t fn = n_new[v], fo = n_old[v];
-        const uint32_t cn = rcnt[v], co = rcnt[n + v];
-        const uint32_t* rn = cn > NNM_RB ? obuf + roff[v] : nullptr;
-        const uint32_t* ro = co > NNM_RB ? obuf + roff[n + v] : nullptr;
-        const uint32_t an = wave_append_reverse(rbuf + v * NNM_RB, rn, cn, cap, nnm_hash32((uint32_t)v, salt),
-                                                l_new + v * cap_new + fn);
-        const uint32_t ao = wave_append_reverse(rbuf + (n + v) * NNM_RB, ro, co, cap,
-                                                nnm_hash32((uint32_t)v, salt ^ 0xA511E9B3u), l_old + v * cap_old + fo);
-        if (lane == 0) {
-            n_new[v] = fn + an;
-            n_old[v] = fo + ao;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (lane >> 5);
+        const bool ok = v < n;
+        const uint32_t cn = ok ? rcnt[v] : 0u, co = ok ? rcnt[n + v] : 0u;
+        const bool big = cn > cap || co > cap;
+        if (ok && !big) {
+            const uint32_t fn = n_new[v], fo = n_old[v];
+            const uint32_t* hn = rbuf + v * NNM_RB;
+            const uint32_t* ho = rbuf + (n + v) * NNM_RB;
+            const uint32_t* rn = cn > NNM_RB ? obuf + roff[v] : nullptr;
+            const uint32_t* ro = co > NNM_RB ? obuf + roff[n + v] : nullptr;
+            uint32_t* dn = l_new + v * cap_new + fn;
+            uint32_t* dold = l_old + v * cap_old + fo;
+            for (uint32_t i = hl; i < cn; i += 32) dn[i] = i < NNM_RB ? hn[i] : rn[i - NNM_RB];
+            for (uint32_t i = hl; i < co; i += 32) dold[i] = i < NNM_RB ? ho[i] : ro[i - NNM_RB];
+            if (hl == 0) {
+                n_new[v] = fn + cn;
+                n_old[v] = fo + co;
+            }
+        }
+        for (uint64_t bb = nnm_ballot(ok && big && hl == 0); bb; bb &= bb - 1) {
+            const uint64_t w = v0 + ((uint32_t)__builtin_ctzll(bb) >> 5);
+            const uint32_t fn = n_new[w], fo = n_old[w];
+            const uint32_t wn = rcnt[w], wo = rcnt[n + w];
+            const uint32_t* rn = wn > NNM_RB ? obuf + roff[w] : nullptr;
+            const uint32_t* ro = wo > NNM_RB ? obuf + roff[n + w] : nullptr;
+            const uint32_t an = wave_append_reverse(rbuf + w * NNM_RB, rn, wn, cap, nnm_hash32((uint32_t)w, salt),
+                                                    l_new + w * cap_new + fn);
+            const uint32_t ao = wave_append_reverse(rbuf + (n + w) * NNM_RB, ro, wo, cap,
+                                                    nnm_hash32((uint32_t)w, salt ^ 0xA511E9B3u),
+                                                    l_old + w * cap_old + fo);
+            if (lane == 0) {
+                n_new[w] = fn + an;
+                n_old[w] = fo + ao;
+            }
         }
     }
 }
@@ -1895,7 +1966,7 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     hipLaunchKernelGGL(k_lb_spill_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, n, K, c->g_ids,
                        c->newmask, c->spill, c->roff, c->rcur, c->obuf);
     NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
+    hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, (n + 1) / 2)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
                        c->rbuf, c->obuf, c->rev_cap, p->seed, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,
                        c->n_old, c->counters);
     NNM_LAUNCH_CHECK(c);
