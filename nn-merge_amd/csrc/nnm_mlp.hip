@@ -37,7 +37,9 @@ struct nnm_mlp_state {
     float *xs = nullptr, *pre = nullptr, *h = nullptr, *z = nullptr, *dz = nullptr, *dh = nullptr;
     float *t = nullptr, *s = nullptr, *loss = nullptr;
     uint32_t* pairs = nullptr;
-    // projection of every row (bf16) and its squared norms
+    // bf16 (RNE) copies of W1 and W2 for the projection, then the projection
+    // of every row (bf16) and its squared norms
+    uint16_t* Wb = nullptr;
     uint16_t* P = nullptr;
     float* pnorm = nullptr;
     uint64_t projected_n = 0;
@@ -195,10 +197,17 @@ __global__ void k_mlp_pairs(uint32_t B, uint64_t n, uint32_t K, uint32_t ld, con
 // output rows in bf16 plus their squared norms (of the bf16 values).
 #define KMAX 256  // longest row (ld) the projection takes; nnm_mlp_init checks it
 
-template <uint32_t HID, uint32_t DOUT>
+// W1 and W2 to bf16 once per projection (the MFMA operands; same rounding
+// as an on-the-fly conversion).
+__global__ void k_mlp_pack(const float* __restrict__ W, uint64_t m, uint16_t* __restrict__ Wb) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        Wb[i] = (uint16_t)mlp_to_bf16(W[i]);
+}
+
+template <uint32_t HID, uint32_t DOUT, uint32_t NC>
 __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X, uint64_t n, uint32_t ld,
-                                                     float alpha, const float* __restrict__ W1,
-                                                     const float* __restrict__ b1, const float* __restrict__ W2,
+                                                     float alpha, const uint16_t* __restrict__ W1b,
+                                                     const float* __restrict__ b1, const uint16_t* __restrict__ W2b,
                                                      const float* __restrict__ b2, uint16_t* __restrict__ P,
                                                      float* __restrict__ pnorm) {
     __shared__ short hsh[4][16][HID + 4];
@@ -211,10 +220,14 @@ __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X
         // layer 1: H[16 x HID] = relu(a X W1^T + b1).  The row's operands
         // (bf16 of alpha x, elements k0 + 4 lg .. + 3 of each 16-chunk) are
         // loaded once; W1 comes by 16-byte loads (L2-resident).
-        mlp_bf16x4 xa[KMAX / 16];
+        // NC chunks of 16 elements per row: a compile-time count lets every
+        // load of a layer issue before its first MFMA; NC = 0 takes the
+        // count from ld (a branch per chunk).
+        constexpr uint32_t CM = NC ? NC : KMAX / 16;
+        mlp_bf16x4 xa[CM];
 #pragma unroll
-        for (uint32_t c = 0; c < KMAX / 16; ++c) {
-            if (c * 16 < ld) {
+        for (uint32_t c = 0; c < CM; ++c) {
+            if (NC || c * 16 < ld) {
                 const float4 x4 = *reinterpret_cast<const float4*>(X + myrow * ld + c * 16 + 4 * lg);
                 xa[c][0] = mlp_to_bf16(alpha * x4.x);
                 xa[c][1] = mlp_to_bf16(alpha * x4.y);
@@ -224,16 +237,16 @@ __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X
         }
         for (uint32_t n0 = 0; n0 < HID; n0 += 16) {
             mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            const float* w1r = W1 + (uint64_t)(n0 + lr) * ld + 4 * lg;
+            const uint16_t* w1r = W1b + (uint64_t)(n0 + lr) * ld + 4 * lg;
 #pragma unroll
-            for (uint32_t c = 0; c < KMAX / 16; ++c) {
-                if (c * 16 < ld) {
-                    const float4 w4 = *reinterpret_cast<const float4*>(w1r + c * 16);
+            for (uint32_t c = 0; c < CM; ++c) {
+                if (NC || c * 16 < ld) {
+                    const uint2 w = *reinterpret_cast<const uint2*>(w1r + c * 16);
                     mlp_bf16x4 b;
-                    b[0] = mlp_to_bf16(w4.x);
-                    b[1] = mlp_to_bf16(w4.y);
-                    b[2] = mlp_to_bf16(w4.z);
-                    b[3] = mlp_to_bf16(w4.w);
+                    b[0] = (short)(w.x & 0xFFFFu);
+                    b[1] = (short)(w.x >> 16);
+                    b[2] = (short)(w.y & 0xFFFFu);
+                    b[3] = (short)(w.y >> 16);
                     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[c], b, acc, 0, 0, 0);
                 }
             }
@@ -250,13 +263,13 @@ __global__ __launch_bounds__(256) void k_mlp_project(const float* __restrict__ X
             mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             for (uint32_t k0 = 0; k0 < HID; k0 += 16) {
                 mlp_bf16x4 a, b;
-                const float4 w4 = *reinterpret_cast<const float4*>(W2 + (uint64_t)(t * 16 + lr) * HID + k0 + 4 * lg);
+                const uint2 w = *reinterpret_cast<const uint2*>(W2b + (uint64_t)(t * 16 + lr) * HID + k0 + 4 * lg);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) a[q] = hsh[wv][lr][k0 + 4 * lg + q];
-                b[0] = mlp_to_bf16(w4.x);
-                b[1] = mlp_to_bf16(w4.y);
-                b[2] = mlp_to_bf16(w4.z);
-                b[3] = mlp_to_bf16(w4.w);
+                b[0] = (short)(w.x & 0xFFFFu);
+                b[1] = (short)(w.x >> 16);
+                b[2] = (short)(w.y & 0xFFFFu);
+                b[3] = (short)(w.y >> 16);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
             }
 #pragma unroll
@@ -294,7 +307,7 @@ void nnm_mlp_free(nnm_ctx* c) {
     nnm_mlp_state* M = mlp_of(c);
     if (!M) return;
     void* ps[] = {M->W1, M->m, M->v, M->g, M->xs, M->pre, M->h, M->z, M->dz, M->dh, M->t, M->s, M->loss,
-                  M->pairs, M->P, M->pnorm};
+                  M->pairs, M->Wb, M->P, M->pnorm};
     for (void* p : ps) nnm_free(c, p);
     delete M;
     c->mlp = nullptr;
@@ -562,17 +575,27 @@ int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs) {
     M->pnorm = nullptr;
     NNM_TRY_MLP(mlp_alloc(c, &M->P, (size_t)c->n * M->dout));
     NNM_TRY_MLP(mlp_alloc(c, &M->pnorm, c->n));
+    const uint64_t w1n = (uint64_t)M->hid * M->din, w2n = (uint64_t)M->dout * M->hid;
+    if (!M->Wb) NNM_TRY_MLP(mlp_alloc(c, &M->Wb, w1n + w2n));
+    const uint16_t* W1b = M->Wb;
+    const uint16_t* W2b = M->Wb + w1n;
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, c->stream, M->W1, w1n, M->Wb);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(16), dim3(256), 0, c->stream, M->W2, w2n, M->Wb + w1n);
+    NNM_LAUNCH_CHECK(c);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((c->n + 63) / 64, (uint64_t)c->num_cus * 16);
-    if (M->hid == 256 && M->dout == 32)
-        hipLaunchKernelGGL((k_mlp_project<256, 32>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha,
-                           M->W1, M->b1, M->W2, M->b2, M->P, M->pnorm);
+#define NNM_PROJECT(H, D, NC)                                                                                   \
+    hipLaunchKernelGGL((k_mlp_project<H, D, NC>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha, \
+                       W1b, M->b1, W2b, M->b2, M->P, M->pnorm)
+    if (M->hid == 256 && M->dout == 32 && c->ld == 128)
+        NNM_PROJECT(256, 32, 8);
+    else if (M->hid == 256 && M->dout == 32)
+        NNM_PROJECT(256, 32, 0);
     else if (M->hid == 128 && M->dout == 32)
-        hipLaunchKernelGGL((k_mlp_project<128, 32>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha,
-                           M->W1, M->b1, M->W2, M->b2, M->P, M->pnorm);
+        NNM_PROJECT(128, 32, 0);
     else if (M->hid == 256 && M->dout == 16)
-        hipLaunchKernelGGL((k_mlp_project<256, 16>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha,
-                           M->W1, M->b1, M->W2, M->b2, M->P, M->pnorm);
+        NNM_PROJECT(256, 16, 0);
+#undef NNM_PROJECT
     else
         return nnm_fail(c, NNM_ERR_UNSUPPORTED, "projection kernel built for (hidden, d_out) in {(256,32),(128,32),(256,16)}");
     NNM_LAUNCH_CHECK(c);
