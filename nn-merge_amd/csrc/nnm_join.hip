@@ -583,14 +583,25 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                                                  (__attribute__((address_space(3))) void*)L.rec, 16, 0, 0);
                 asm volatile("v_mov_b32 %0, %1" : "=v"(wcp) : "v"(wid));
             }
-            // rows -> LDS: lane l of a row instruction fetches dword 2u + (l & 1), u = (l/2) ^ (r & SWM)
-            for (uint32_t r = 0; r < rows; ++r) {
-                const uint32_t id = wave_read32(wcp, r);
-                if (lane < DW) {
-                    const uint32_t u = (lane >> 1) ^ (r & SWM);
-                    const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
+            // rows -> LDS: lane l of a row fetches dword 2u + (l & 1), u = (l/2) ^ (r & SWM).
+            // Rows shorter than 64 dwords go RPI to an instruction (lanes
+            // [DW g, DW g + DW) take row r0 + g, which lands at the group's
+            // LDS base + DW g dwords: the next row).
+            constexpr uint32_t RPI = DW < NNM_WAVE ? NNM_WAVE / DW : 1u;
+            const uint32_t rg = lane / DW, rl = lane % DW;
+            for (uint32_t r0 = 0; r0 < rows; r0 += RPI) {
+                uint32_t id = wave_read32(wcp, r0);
+#pragma unroll
+                for (uint32_t k = 1; k < RPI; ++k) {
+                    const uint32_t t = wave_read32(wcp, r0 + k);  // lanes past the window hold 0
+                    if (rg == k) id = t;
+                }
+                const uint32_t r = r0 + rg;
+                if (rg < RPI && r < rows) {
+                    const uint32_t u = (rl >> 1) ^ (r & SWM);
+                    const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (rl & 1);
                     __builtin_amdgcn_global_load_lds((const void*)g,
-                                                     (__attribute__((address_space(3))) void*)(L.rows + r * U), 4, 0, 0);
+                                                     (__attribute__((address_space(3))) void*)(L.rows + r0 * U), 4, 0, 0);
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);
