@@ -2,10 +2,12 @@
 // (d_in -> hidden -> d_out, FP32 master weights, BF16 MFMA or FP32 GEMMs),
 // trained so that |f(x) - f(y)|^2 approximates a^2 |x - y|^2 on pairs that
 // NN-Descent itself produces (list neighbours and two-hop candidates).  The
-// projected rows feed the same join kernels as the BF16 path; every
-// candidate they offer is recomputed in exact FP32 before it can enter a
-// list.  The reference has no learned component; the pair loss, gradients
-// and optimizer are restated in FP64 by the CPU oracle for the tests.
+// projected rows (32-d bf16) feed the BF16 join kernels during the learned
+// phase, whose lists hold the estimates kscale |f(x) - f(y)|^2; afterwards
+// every list distance is recomputed exactly once and the plain BF16 join
+// finishes the merge.  The reference has no learned component; the pair loss,
+// gradients and optimizer are restated in FP64 by the CPU oracle for the
+// tests.
 //
 // Loss over a batch of B pairs (a_i, b_i) with exact squared distance t_i and
 // anchor k-th distance s_i (both scaled by a^2):

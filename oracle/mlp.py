@@ -1,6 +1,6 @@
 """FP64 restatement of the learned projection used to prune join candidates.
 
-TEST INFRASTRUCTURE ONLY (the checker for the library's MLP kernels).  The
+TEST INFRASTRUCTURE ONLY (it checks the library's MLP kernels).  The
 reference has no learned component; this is the closed-form mathematics of
 the model and loss the library trains:
     f(x) = W2 relu(W1 x + b1) + b2

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, as the checker.  The product library never
+CPU-baseline leg of bench.py, to check results.  The product library never
 imports it.
 """
 import ctypes as C

@@ -1,5 +1,5 @@
-"""Pins the CPU oracle (oracle/oracle.cpp) before it is trusted as the
-checker: against the reference's own distance routines compiled from its
+"""Pins the CPU oracle (oracle/oracle.cpp) before the tests trust it:
+against the reference's own distance routines compiled from its
 sources (oracle/_ref), the DistOpt::test vectors (src/distopt.cpp:208-215),
 FP64 closed forms, brute force on tiny inputs and list invariants."""
 import ctypes as C
