@@ -1577,8 +1577,12 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
 // sort.  Slots are reset to empty.  With rerank, lower-bound keys are
 // recomputed exactly first.  The next vertex's list and slots are loaded
 // while the current one is merged.
+// Compiled per configuration, so the common ones carry no dead state:
+// TWO = two slot sets per lane (C > 64), RR = exact re-rank of lower-bound
+// keys, TIE = FP64 near-tie decision at the k-th slot (tie_win > 1).
+template <bool TWO, bool RR, bool TIE>
 __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X,
-                         const uint16_t* __restrict__ Xb_exact, uint32_t ld, int rerank, float rerank_tau,
+                         const uint16_t* __restrict__ Xb_exact, uint32_t ld, float rerank_tau,
                          float tie_win, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
                          uint64_t* __restrict__ counters) {
@@ -1599,7 +1603,9 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             f = g_flag[u * K + lane];
         }
         if (lane < C) sk0 = slots[u * C + lane];
-        if (lane + NNM_WAVE < C) sk1 = slots[u * C + NNM_WAVE + lane];
+        if constexpr (TWO) {
+            if (lane + NNM_WAVE < C) sk1 = slots[u * C + NNM_WAVE + lane];
+        }
     };
     uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
     uint32_t pid, pf;
@@ -1612,7 +1618,9 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
         uint64_t nsk0, nsk1;
         fetch(v + W, nid, nd, nf, nsk0, nsk1);
         if (skey0 != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
-        if (skey1 != NNM_EMPTY_KEY) slots[v * C + NNM_WAVE + lane] = NNM_EMPTY_KEY;
+        if constexpr (TWO) {
+            if (skey1 != NNM_EMPTY_KEY) slots[v * C + NNM_WAVE + lane] = NNM_EMPTY_KEY;
+        }
         [&]() {
             const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
             const uint64_t kth_key = wave_read64(pkey, K - 1);
@@ -1621,17 +1629,17 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             // cheap filter first: estimates against the margin, exact keys
             // against the k-th key (or its rounding window)
             auto pass = [&](uint64_t sk) {
-                return sk != NNM_EMPTY_KEY &&
-                       (rerank ? nnm_key_dist(sk) <= kd * rerank_tau
-                               : (sk < kth_key || (tie_win > 1.f && nnm_key_dist(sk) <= kd * tie_win)));
+                if constexpr (RR) return sk != NNM_EMPTY_KEY && nnm_key_dist(sk) <= kd * rerank_tau;
+                else if constexpr (TIE) return sk != NNM_EMPTY_KEY && (sk < kth_key || nnm_key_dist(sk) <= kd * tie_win);
+                else return sk < kth_key;  // the empty key is above every key
             };
-            bool valid0 = pass(skey0), valid1 = pass(skey1);
+            bool valid0 = pass(skey0), valid1 = TWO && pass(skey1);
             if (!__ballot(valid0 || valid1)) {
                 c_flags += pflags;
                 return;
             }
             uint64_t ck0 = skey0, ck1 = skey1;
-            if (rerank) {
+            if constexpr (RR) {
                 // de-dup before the exact recomputation (no distance for an id
                 // already listed)
                 const uint32_t sid0 = nnm_key_id(skey0), sid1 = nnm_key_id(skey1);
@@ -1647,15 +1655,15 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
                 if (valid0) ck0 = nnm_key(exact(sid0), sid0);
                 if (valid1) ck1 = nnm_key(exact(sid1), sid1);
                 c_rr += (uint32_t)__popcll(__ballot(valid0)) + (uint32_t)__popcll(__ballot(valid1));
-                valid0 = valid0 && (ck0 < kth_key || (tie_win > 1.f && nnm_key_dist(ck0) <= kd * tie_win));
-                valid1 = valid1 && (ck1 < kth_key || (tie_win > 1.f && nnm_key_dist(ck1) <= kd * tie_win));
+                valid0 = valid0 && (ck0 < kth_key || (TIE && nnm_key_dist(ck0) <= kd * tie_win));
+                valid1 = valid1 && (ck1 < kth_key || (TIE && nnm_key_dist(ck1) <= kd * tie_win));
             }
             // one candidate set, at most one candidate per lane
             uint64_t ck;
             bool valid;
             {
-                const uint64_t vb0 = __ballot(valid0), vb1 = __ballot(valid1);
-                if (!vb1) {
+                const uint64_t vb0 = __ballot(valid0), vb1 = TWO ? __ballot(valid1) : 0ull;
+                if (!TWO || !vb1) {
                     ck = ck0;
                     valid = valid0;
                 } else if (!vb0) {
@@ -1686,7 +1694,7 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             }
             uint64_t vb = __ballot(valid);
             const uint64_t lmask = K < NNM_WAVE ? (1ull << K) - 1 : ~0ull;  // list lanes
-            if (!rerank && vb) {
+            if (!RR && vb) {
                 // de-dup against the list after the compaction (at most K of the
                 // kept keys repeat a list id, so the K best distinct candidates
                 // are still there): a loop over the fewer of candidates and
@@ -1742,7 +1750,7 @@ __global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t
             }
             uint32_t pos_p = lane + below_cand_p;
             uint32_t pos_s = below_list + below_cand_c;
-            if (tie_win > 1.f) {
+            if constexpr (TIE) {
                 // the entries ranked K-1 (last kept) and K (first dropped): when
                 // their FP32 distances are within rounding of each other the
                 // choice is re-made on FP64 distances (ties by id)
@@ -1983,9 +1991,24 @@ static int update_lists(nnm_ctx* c, const nnm_params* p) {
     // learned phase: lists and offers are both estimates of one metric, so the
     // update merges them as they are (the exact distances come at the switch)
     if (p->rerank == NNM_RERANK_NEVER || p->prec == NNM_PREC_LEARNED) rerank = 0;
-    hipLaunchKernelGGL(k_update, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K,
-                       c->slots_C, c->X, (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld, rerank, tau,
-                       nnm_near_tie_window(c, p), c->g_ids, c->g_dist, c->g_flag, c->slots, c->counters);
+    const float tie_win = nnm_near_tie_window(c, p);
+    const bool two = c->slots_C > NNM_WAVE, tie = tie_win > 1.f;
+    const unsigned cfg = (two ? 4u : 0u) | (rerank ? 2u : 0u) | (tie ? 1u : 0u);
+    void (*kern)(uint64_t, uint32_t, uint32_t, const float*, const uint16_t*, uint32_t, float, float, uint32_t*, float*,
+                 uint8_t*, uint64_t*, uint64_t*) = nullptr;
+    switch (cfg) {
+        case 0: kern = k_update<false, false, false>; break;
+        case 1: kern = k_update<false, false, true>; break;
+        case 2: kern = k_update<false, true, false>; break;
+        case 3: kern = k_update<false, true, true>; break;
+        case 4: kern = k_update<true, false, false>; break;
+        case 5: kern = k_update<true, false, true>; break;
+        case 6: kern = k_update<true, true, false>; break;
+        default: kern = k_update<true, true, true>; break;
+    }
+    hipLaunchKernelGGL(kern, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K, c->slots_C, c->X,
+                       (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld, tau, tie_win, c->g_ids, c->g_dist,
+                       c->g_flag, c->slots, c->counters);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
