@@ -244,7 +244,9 @@ def main():
                          "a few more colliding candidates per iteration and run faster)")
     ap.add_argument("--learn-tau", type=float, default=0.0,
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = library default)")
-    ap.add_argument("--no-renew", action="store_true", help="learned: keep the flags at the switch")
+    ap.add_argument("--renew", action="store_true",
+                    help="learned: mark every list entry new when the lists leave the learned space (default: "
+                         "keep the flags; measured slower and lower in recall on the 5M J-Merge)")
     ap.add_argument("--learned-iters", type=int, default=8,
                     help="learned: total iterations T of the learned run (learned phase + plain tail)")
     ap.add_argument("--graph", action="store_true", help="replay iterations from HIP graphs")
@@ -319,7 +321,7 @@ def main():
 
     def run(name):
         T = args.learned_iters if (name == "learned" and args.learned_iters) else args.iters
-        fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and not args.no_renew) else 0)
+        fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and args.renew) else 0)
         p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name],
                                seed=seed, plain_tail=args.plain_tail, flags=fl)
         if args.learn_tau > 0:

@@ -109,7 +109,7 @@ typedef struct nnm_params {
     float learn_tau;    /* NNM_PREC_LEARNED: the learned-phase join offers a
                            pair to a target when its estimate is below
                            learn_tau x the target's estimated k-th distance
-                           (0 = 0.7); the plain tail uses the exact k-th */
+                           (0 = 0.6); the plain tail uses the exact k-th */
     uint32_t reserved;  /* 0 */
     uint64_t seed;      /* every random choice derives from this seed */
 } nnm_params;

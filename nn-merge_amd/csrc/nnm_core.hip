@@ -1581,7 +1581,7 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
 // TWO = two slot sets per lane (C > 64), RR = exact re-rank of lower-bound
 // keys, TIE = FP64 near-tie decision at the k-th slot (tie_win > 1).
 template <bool TWO, bool RR, bool TIE>
-__global__ __launch_bounds__(256) void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X,
                          const uint16_t* __restrict__ Xb_exact, uint32_t ld, float rerank_tau,
                          float tie_win, uint32_t* __restrict__ g_ids,
                          float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,

@@ -23,7 +23,7 @@
 #define NNM_WAVE 64
 #define NNM_MAX_K 64
 // Learned-phase offer threshold (fraction of the target's estimated k-th distance).
-#define NNM_LEARN_TAU_DEFAULT 0.7f
+#define NNM_LEARN_TAU_DEFAULT 0.6f
 
 // Slot keys: squared distances are >= 0, so their IEEE bits order like the
 // values; the low word breaks ties by id.  An empty slot holds all ones.
