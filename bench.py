@@ -397,6 +397,11 @@ def main():
             if tr is not None:
                 res["roofline"]["traffic"] = int(tr)
                 res["roofline"]["traffic_source"] = src
+        # per-result error of the merged lists' FP32 distances against FP64
+        # (every entry recomputed on the GPU; outside the timed region)
+        e = ctx.dist_error()
+        res["fp64_error"] = {"max_rel": e["max_rel"], "mean_rel": e["mean_rel"], "max_abs": e["max_abs"],
+                             "entries": int(e["entries"]), "order_flips": int(e["order_flips"])}
         if gt is not None:
             ids, _, _ = ctx.get_graph()
             res["recall@10"] = round(float(nnm.recall(ids, gt, rows, 10)), 5)
@@ -424,8 +429,11 @@ def main():
            "dtype": "f32" if chosen == "fp32" else "bf16",
            "data": "synthetic (seeded generator nnm_data.sift_like)",
            "config": {"workload": workload, "n": n, "d": d, "K": K, "rho": args.rho, "max_iters": args.iters,
-                      "slots": args.slots, "path": chosen, "parallelism": "single GPU"},
-           "merge_secs": ch["merge_secs"], "recall@10": ch.get("recall@10"),
+                      "slots": args.slots, "path": chosen, "parallelism": "single GPU",
+                      "learned": {"T": args.learned_iters, "plain_tail": args.plain_tail,
+                                  "learn_tau": args.learn_tau if args.learn_tau > 0 else "library default (0.6)",
+                                  "renew": bool(args.renew)}},
+           "merge_secs": ch["merge_secs"], "recall@10": ch.get("recall@10"), "fp64_error": ch.get("fp64_error"),
            "upload": {"bytes": int(X.nbytes), "secs": round(upload_secs, 4),
                       "note": "nnm_set_data host -> HBM (pinned staging, with the bf16-exactness scan); "
                               "outside the timed region"},

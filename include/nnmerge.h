@@ -276,9 +276,11 @@ int nnm_pair_dists(nnm_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t 
  * |f(x) - f(y)|^2 ~ a^2 |x - y|^2 on pairs taken from the current lists (list
  * neighbours and two-hop candidates, the pairs the local join evaluates).
  * nnm_mlp_init: He-initialised model, input scale a from the data.
- * nnm_mlp_train: `steps` Adam steps on `batch` fresh pairs each; afterwards the
- *   prefilter margin tau is the `keep` quantile of p / kth over true
- *   improvements of a fresh sample.
+ * nnm_mlp_train: `steps` Adam steps on `batch` fresh pairs each; afterwards
+ *   tau (read back by nnm_mlp_get_weights) is the `keep` quantile of p / kth
+ *   over true improvements of a fresh sample, a diagnostic of how well the
+ *   model ranks near the k-th distance.  The learned join's offer threshold
+ *   is nnm_params.learn_tau, not tau.
  * nnm_mlp_loss_grad: loss and gradients of one batch given by the caller
  *   (rows already scaled; bf16 = 0 uses FP32 GEMMs), for checking.
  * nnm_mlp_project: projection of every row (used by the learned join);

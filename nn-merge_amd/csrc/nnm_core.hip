@@ -579,6 +579,24 @@ __device__ inline void wave_random_row(const float* __restrict__ X, const uint16
     wave_write_sorted_row(key, 1u, K, ids, dist, flg);
 }
 
+// Distance of the list set-up kernels.  Where the metric is L2 the choice
+// of rows is made at compile time, so those kernels carry no other metric's
+// code or registers; each mode calls what nnm_row_dist would, so the bits
+// are the same.  DM_ANY: the run-time metric (FP32 rows, or bf16 rows when
+// given); DM_L2_BF16: L2 on bf16 rows (exact there); DM_L2_F32: L2 on FP32
+// rows.
+enum : int { DM_ANY = 0, DM_L2_BF16 = 1, DM_L2_F32 = 2 };
+template <int DM>
+__device__ inline float setup_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
+                                   uint64_t a, uint64_t b, const nnm_metric_args& mt) {
+    if constexpr (DM == DM_L2_BF16) return nnm_l2_rows_bf16(Xb + a * ld, Xb + b * ld, ld);
+    else if constexpr (DM == DM_L2_F32) return nnm_l2_rows(X + a * ld, X + b * ld, ld);
+    else return nnm_row_dist(X, Xb, ld, a, b, mt);
+}
+static int setup_dm(const nnm_ctx* c, const uint16_t* xb) {
+    return c->metric != NNM_METRIC_L2 ? DM_ANY : xb ? DM_L2_BF16 : DM_L2_F32;
+}
+
 // nnm_wave_rnd_sds for each 32-lane half on its own (k <= 32; seed, stream,
 // k and M may differ between the halves): the same values in half-lanes
 // [0, k) as the full-wave draw gives in lanes [0, k).
@@ -608,6 +626,7 @@ __device__ inline uint32_t half_rnd_sds(uint64_t seed, uint64_t stream, uint32_t
 __device__ inline uint64_t half_bits(uint64_t m) { return (nnm_lane() & 32u) ? m >> 32 : m & 0xFFFFFFFFull; }
 
 // wave_random_row for the half's row (K + 1 <= 32); ok false: no row.
+template <int DM = DM_ANY>
 __device__ inline void half_random_row(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                        uint64_t v, bool ok, uint32_t gb, uint32_t gsize, uint32_t K, uint64_t seed,
                                        uint32_t* __restrict__ ids, float* __restrict__ dist,
@@ -621,7 +640,7 @@ __device__ inline void half_random_row(const float* __restrict__ X, const uint16
     const uint32_t myid = (uint32_t)__shfl((int)id, (int)src, NNM_WAVE);
     float dd = 0.f;
     if (ok && hl < K) {
-        dd = nnm_row_dist(X, Xb, ld, v, myid, mt);
+        dd = setup_dist<DM>(X, Xb, ld, v, myid, mt);
         *nd += 1;
     }
     half_write_sorted_row(nnm_key(dd, myid), 1u, ok, K, ids, dist, flg);
@@ -663,6 +682,7 @@ __global__ void k_random_init(const float* __restrict__ X, uint32_t ld, uint64_t
 
 // Upload fix-up: optional distance recompute, optional flags, sort rows.
 // skip_raw (optional): rows labelled raw are left as they are.
+template <int DM>
 __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld, uint64_t n,
                             uint32_t K, int recompute, int have_flags, const uint32_t* __restrict__ skip_raw,
                             uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
@@ -680,7 +700,7 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
         if (ok && hl < K) {
             id = g_ids[v * K + hl];
             if (id >= n || id == (uint32_t)v) { atomicAdd(bad, 1u); id = (uint32_t)((v + 1) % n); }
-            dd = recompute ? nnm_row_dist(X, Xb, ld, v, id, mt) : g_dist[v * K + hl];
+            dd = recompute ? setup_dist<DM>(X, Xb, ld, v, id, mt) : g_dist[v * K + hl];
             f = have_flags ? g_flag[v * K + hl] : 1u;
             nd += recompute ? 1 : 0;
         }
@@ -692,7 +712,10 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
 
 // Learned phase entry: every list distance becomes the learned estimate
 // kscale * max(|p_v|^2 + |p_u|^2 - 2 p_v.p_u, 0) on the bf16 projections (the
-// learned join's key), rows re-sorted, flags kept.
+// learned join's key), rows re-sorted, flags kept.  NC > 0: rows of NC
+// 16-byte chunks (dout = 8 NC), every chunk loaded before the products;
+// NC = 0: any dout (a multiple of 8).  The products run in k order either way.
+template <uint32_t NC>
 __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restrict__ P, const float* __restrict__ pn,
                                  uint32_t dout, float kscale, uint32_t* __restrict__ g_ids,
                                  float* __restrict__ g_dist, uint8_t* __restrict__ g_flag) {
@@ -705,18 +728,29 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
         if (ok && hl < K) {
             id = g_ids[v * K + hl];
             f = g_flag[v * K + hl];
-            // rows of dout bf16 (a multiple of 8): 16-byte loads, products in k order
             const uint4* a = reinterpret_cast<const uint4*>(P + v * dout);
             const uint4* b = reinterpret_cast<const uint4*>(P + (uint64_t)id * dout);
             float dot = 0.f;
-            for (uint32_t k = 0; k < dout / 8; ++k) {
-                const uint4 x = a[k], y = b[k];
+            auto step = [&](const uint4 x, const uint4 y) {
                 const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
                     dot = __builtin_fmaf(__uint_as_float(xs[h] << 16), __uint_as_float(ys[h] << 16), dot);
-                    dot = __builtin_fmaf(__uint_as_float(xs[h] & 0xFFFF0000u), __uint_as_float(ys[h] & 0xFFFF0000u), dot);
+                    dot = __builtin_fmaf(__uint_as_float(xs[h] & 0xFFFF0000u), __uint_as_float(ys[h] & 0xFFFF0000u),
+                                         dot);
                 }
+            };
+            if constexpr (NC > 0) {
+                uint4 x[NC], y[NC];
+#pragma unroll
+                for (uint32_t k = 0; k < NC; ++k) {
+                    x[k] = a[k];
+                    y[k] = b[k];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < NC; ++k) step(x[k], y[k]);
+            } else {
+                for (uint32_t k = 0; k < dout / 8; ++k) step(a[k], b[k]);
             }
             dd = fmaxf((pn[v] + pn[id]) - 2.f * dot, 0.f) * kscale;
         }
@@ -828,6 +862,7 @@ __global__ void k_merge_init(const float* __restrict__ X, const uint16_t* __rest
 
 // k_merge_init with two rows per wave (K <= 31, so K + 1 draws fit a half):
 // the same lists, parked entries and distance count.
+template <int DM>
 __global__ void k_merge_init_half(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                   uint64_t n, uint32_t K, uint32_t cut,
                                   const uint32_t* __restrict__ vsub, const uint32_t* __restrict__ sub_begin,
@@ -854,7 +889,7 @@ __global__ void k_merge_init_half(const float* __restrict__ X, const uint16_t* _
         // below reads lanes of its own half only
         if (!built) {
             if (ok && hl == 0) has_cpn[v] = 0;
-            half_random_row(X, Xb, ld, v, ok, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd, mt);
+            half_random_row<DM>(X, Xb, ld, v, ok, gb, gsize, K, seed ^ 0x5A5A5A5Aull, ids, dist, flg, &nd, mt);
         } else {
             const uint32_t sb = sub_begin[s], ssize = sub_begin[s + 1] - sb;
             const uint32_t remain = K - cut;
@@ -886,7 +921,7 @@ __global__ void k_merge_init_half(const float* __restrict__ X, const uint16_t* _
             const uint32_t got = (uint32_t)__shfl((int)cand, (int)src, NNM_WAVE);
             if (takes) {
                 pid = got;
-                pd = nnm_row_dist(X, Xb, ld, got, v, mt);
+                pd = setup_dist<DM>(X, Xb, ld, got, v, mt);
                 pf = 1;
                 nd += 1;
             }
@@ -1064,8 +1099,9 @@ extern "C" int nnm_set_graph(nnm_ctx* c, const uint32_t* ids, const float* dist,
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, (c->n + 1) / 2)), dim3(256), 0, c->stream, c->X, nullptr, c->ld, c->n,
-                       K, dist ? 0 : 1, flags ? 1 : 0, nullptr, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
+    hipLaunchKernelGGL(setup_dm(c, nullptr) == DM_ANY ? k_set_graph<DM_ANY> : k_set_graph<DM_L2_F32>,
+                       dim3(wave_grid(c, (c->n + 1) / 2)), dim3(256), 0, c->stream, c->X, nullptr, c->ld, c->n, K,
+                       dist ? 0 : 1, flags ? 1 : 0, nullptr, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
                        metric_args(c));
     NNM_LAUNCH_CHECK(c);
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -1129,9 +1165,11 @@ extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
             xb = c->Xb;
         }
         uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
-        hipLaunchKernelGGL(k_set_graph, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X, xb, c->ld,
-                           c->n_act, c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad,
-                           metric_args(c));
+        const int dm = setup_dm(c, xb);
+        hipLaunchKernelGGL(dm == DM_L2_BF16 ? k_set_graph<DM_L2_BF16>
+                                            : dm == DM_L2_F32 ? k_set_graph<DM_L2_F32> : k_set_graph<DM_ANY>,
+                           dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X, xb, c->ld, c->n_act,
+                           c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad, metric_args(c));
         NNM_LAUNCH_CHECK(c);
     }
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -1203,8 +1241,12 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
     }
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    const int dm = setup_dm(c, xb);
     if (K + 1 <= 32)  // two rows per wave
-        hipLaunchKernelGGL(k_merge_init_half, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X,
+        hipLaunchKernelGGL(dm == DM_L2_BF16  ? k_merge_init_half<DM_L2_BF16>
+                           : dm == DM_L2_F32 ? k_merge_init_half<DM_L2_F32>
+                                             : k_merge_init_half<DM_ANY>,
+                           dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X,
                            xb, c->ld, c->n_act, K, cut, c->vsub, c->sub_begin, c->sub_grp, c->sub_built,
                            c->grp_begin, c->grp_end, p->seed * 0x9E3779B97F4A7C15ull + 17, c->g_ids, c->g_dist,
                            c->g_flag, c->c_ids, c->c_dist, c->c_flag, c->has_cpn, c->counters, metric_args(c));
@@ -2143,9 +2185,10 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     bool tail = !learned;
     bool estimates = false;  // the lists hold learned estimates
     auto to_estimates = [&]() -> int {
-        hipLaunchKernelGGL(k_lists_estimate, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K,
-                           nnm_mlp_rows(c), nnm_mlp_norms(c), nnm_mlp_dout(c), nnm_mlp_scale(c), c->g_ids, c->g_dist,
-                           c->g_flag);
+        const uint32_t dout = nnm_mlp_dout(c);
+        hipLaunchKernelGGL(dout == 32 ? k_lists_estimate<4> : k_lists_estimate<0>, dim3(wave_grid(c, (c->n_act + 1) / 2)),
+                           dim3(256), 0, c->stream, c->n_act, c->K, nnm_mlp_rows(c), nnm_mlp_norms(c), dout,
+                           nnm_mlp_scale(c), c->g_ids, c->g_dist, c->g_flag);
         NNM_LAUNCH_CHECK(c);
         estimates = true;
         return NNM_OK;

@@ -204,7 +204,6 @@ const uint16_t* nnm_mlp_rows(nnm_ctx* c);
 const float* nnm_mlp_norms(nnm_ctx* c);
 uint32_t nnm_mlp_dout(nnm_ctx* c);
 float nnm_mlp_scale(nnm_ctx* c);
-float nnm_mlp_tau(nnm_ctx* c);
 int nnm_mlp_trained(nnm_ctx* c);
 
 // ---------------------------------------------------------- device helpers

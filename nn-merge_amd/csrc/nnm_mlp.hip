@@ -27,7 +27,7 @@ typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 struct nnm_mlp_state {
     uint32_t din = 0, hid = 0, dout = 0;
     float alpha = 1.f;  // input scale
-    float tau = 1.f;    // prefilter margin on projected distances
+    float tau = 1.f;    // keep-quantile of p / kth on true improvements (diagnostic)
     int trained = 0;
     // master weights and Adam moments (FP32), gradients
     float *W1 = nullptr, *b1 = nullptr, *W2 = nullptr, *b2 = nullptr;
@@ -492,9 +492,10 @@ extern "C" int nnm_mlp_loss_grad(nnm_ctx* c, const float* xa, const float* xb, c
     return NNM_OK;
 }
 
-// Prefilter margin: the tau-quantile rule.  On fresh pairs, find the smallest
-// tau such that a fraction `keep` of the pairs with t < s (true improvements
-// near the boundary) has p < tau * s... computed on the host from a sample.
+// Ranking diagnostic of the trained model: on fresh pairs, the smallest tau
+// such that a fraction `keep` of the pairs with t < s (true improvements near
+// the k-th distance) has p < tau * s, computed on the host from a sample.  It
+// is reported with the weights; the learned join's threshold is learn_tau.
 static int calibrate_tau(nnm_ctx* c, uint32_t B, float keep, uint64_t seed) {
     nnm_mlp_state* M = mlp_of(c);
     NNM_TRY_MLP(ensure_batch(c, B));
@@ -570,7 +571,10 @@ int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs) {
     nnm_mlp_state* M = mlp_of(c);
     if (!M) return nnm_fail(c, NNM_ERR_STATE, "learned join needs a model (nnm_mlp_init / nnm_mlp_train)");
     if (M->din != c->ld) return nnm_fail(c, NNM_ERR_STATE, "model was built for another row length");
-    if (M->projected_n == c->n && M->P) return NNM_OK;
+    if (M->projected_n == c->n && M->P) {
+        if (secs) *secs = 0.0;  // already current: no work
+        return NNM_OK;
+    }
     nnm_free(c, M->P);
     nnm_free(c, M->pnorm);
     M->P = nullptr;
@@ -638,5 +642,4 @@ const uint16_t* nnm_mlp_rows(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->P : nul
 const float* nnm_mlp_norms(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->pnorm : nullptr; }
 uint32_t nnm_mlp_dout(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->dout : 0; }
 float nnm_mlp_scale(nnm_ctx* c) { return mlp_of(c) ? 1.f / (mlp_of(c)->alpha * mlp_of(c)->alpha) : 1.f; }
-float nnm_mlp_tau(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->tau : 1.f; }
 int nnm_mlp_trained(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->trained : 0; }
