@@ -16,6 +16,12 @@
 //   * Layers are CSR (offs[rows + 1], adj[]) instead of the reference's
 //     "augmented" arrays (deg, ids...) + knnLocs (src/lysearch.cpp:112-186);
 //     the content is the same.
+//
+// Parity unpinned: the order-free DPG reverse pass (orc_dpg_par) differs from
+// the reference's in-place pass (orc_dpg_ref) only where a row's list gained
+// reverse entries before its own turn; it is checked against orc_dpg_ref by
+// edge overlap (>= 0.99, measured 1.0), not bit for bit, and the reference's
+// own DPG code cannot be built here (it needs Boost).
 
 #include <algorithm>
 #include <cstdint>
