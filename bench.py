@@ -244,6 +244,8 @@ def main():
                          "a few more colliding candidates per iteration and run faster)")
     ap.add_argument("--learn-tau", type=float, default=0.0,
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = library default)")
+    ap.add_argument("--fp8", action="store_true",
+                    help="learned: run the learned phase on FP8 (OCP e4m3) rows of the projection")
     ap.add_argument("--renew", action="store_true",
                     help="learned: mark every list entry new when the lists leave the learned space (default: "
                          "keep the flags; measured slower and lower in recall on the 5M J-Merge)")
@@ -322,6 +324,7 @@ def main():
     def run(name):
         T = args.learned_iters if (name == "learned" and args.learned_iters) else args.iters
         fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and args.renew) else 0)
+        fl |= nnm.FLAG_LEARNED_FP8 if (name == "learned" and args.fp8) else 0
         p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name],
                                seed=seed, plain_tail=args.plain_tail, flags=fl)
         if args.learn_tau > 0:
@@ -393,7 +396,7 @@ def main():
                 "compute_achieved_tflops": round(tfs, 2), "compute_peak_tflops": peak_c,
                 "compute_unit": "fp32 vector ALU" if name == "fp32" else "bf16 MFMA",
                 "compute_frac": round(tfs / peak_c, 4)}
-            tr, src = read_traffic(name)
+            tr, src = read_traffic(name + ("_fp8" if (name == "learned" and args.fp8) else ""))
             if tr is not None:
                 res["roofline"]["traffic"] = int(tr)
                 res["roofline"]["traffic_source"] = src
@@ -432,7 +435,7 @@ def main():
                       "slots": args.slots, "path": chosen, "parallelism": "single GPU",
                       "learned": {"T": args.learned_iters, "plain_tail": args.plain_tail,
                                   "learn_tau": args.learn_tau if args.learn_tau > 0 else "library default (0.6)",
-                                  "renew": bool(args.renew)}},
+                                  "renew": bool(args.renew), "fp8_rows": bool(args.fp8)}},
            "merge_secs": ch["merge_secs"], "recall@10": ch.get("recall@10"), "fp64_error": ch.get("fp64_error"),
            "upload": {"bytes": int(X.nbytes), "secs": round(upload_secs, 4),
                       "note": "nnm_set_data host -> HBM (pinned staging, with the bf16-exactness scan); "

@@ -64,6 +64,11 @@ extern "C" {
 #define NNM_FLAG_LEARNED_RENEW 4u /* NNM_PREC_LEARNED: when the lists leave the
                                      learned space, every entry is marked new so
                                      the plain join re-judges all of them */
+#define NNM_FLAG_LEARNED_FP8 8u /* NNM_PREC_LEARNED: the learned phase works on
+                                   FP8 (OCP e4m3) rows of the projection,
+                                   scaled by 448 / max |p|, on the 16x16x32
+                                   fp8 MFMA (d_out = 32); estimates and the
+                                   plain tail are unchanged otherwise */
 
 /* Re-rank policy for approximate precisions. */
 #define NNM_RERANK_AUTO 0   /* skip when the approximate form is exact */
@@ -284,7 +289,10 @@ int nnm_pair_dists(nnm_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t 
  * nnm_mlp_loss_grad: loss and gradients of one batch given by the caller
  *   (rows already scaled; bf16 = 0 uses FP32 GEMMs), for checking.
  * nnm_mlp_project: projection of every row (used by the learned join);
- *   out (n x d_out floats) may be NULL. */
+ *   out (n x d_out floats) may be NULL.
+ * nnm_mlp_project_fp8: the FP8 rows of NNM_FLAG_LEARNED_FP8 (made from the
+ *   current projection when missing): out (n x d_out bytes, OCP e4m3 of
+ *   clamp(s p, +-448), round to nearest even) may be NULL; *scale = s. */
 int nnm_mlp_init(nnm_ctx* ctx, uint32_t hidden, uint32_t d_out, uint64_t seed);
 int nnm_mlp_set_weights(nnm_ctx* ctx, const float* W1, const float* b1, const float* W2, const float* b2,
                         float alpha);
@@ -294,6 +302,7 @@ int nnm_mlp_loss_grad(nnm_ctx* ctx, const float* xa, const float* xb, const floa
 int nnm_mlp_train(nnm_ctx* ctx, uint32_t steps, uint32_t batch, float lr, float keep, uint64_t seed,
                   float* loss_first, float* loss_last, double* secs);
 int nnm_mlp_project(nnm_ctx* ctx, float* out, double* secs);
+int nnm_mlp_project_fp8(nnm_ctx* ctx, uint8_t* out, float* scale);
 
 /* --- host-class drivers ----------------------------------------------- */
 /* Runs one host class end to end (nn-merge_amd/host): kind 0 = KGraph

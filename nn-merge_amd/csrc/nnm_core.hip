@@ -758,6 +758,44 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
     }
 }
 
+// Learned phase entry with FP8 rows: every list distance becomes the FP8
+// estimate kscale8 * max(|q_v|^2 + |q_u|^2 - 2 q_v.q_u, 0) on the e4m3 rows
+// of d_out = 32 (the FP8 learned join's key), rows re-sorted, flags kept.
+__device__ inline float e4m3_value(uint32_t b) {
+    const uint32_t e = (b >> 3) & 15u, m = b & 7u;
+    const float v = e ? __uint_as_float(((e + 120u) << 23) | (m << 20)) : (float)m * 0.001953125f;
+    return (b & 0x80u) ? -v : v;
+}
+__global__ void k_lists_estimate_fp8(uint64_t n, uint32_t K, const uint8_t* __restrict__ P8,
+                                     const float* __restrict__ pn8, float kscale, uint32_t* __restrict__ g_ids,
+                                     float* __restrict__ g_dist, uint8_t* __restrict__ g_flag) {
+    const uint32_t hl = nnm_lane() & 31u;
+    NNM_HALF_LOOP(v0, n) {
+        const uint64_t v = v0 + (nnm_lane() >> 5);
+        const bool ok = v < n;
+        uint32_t id = 0xFFFFFFFFu, f = 0;
+        float dd = 3.402823466e38f;
+        if (ok && hl < K) {
+            id = g_ids[v * K + hl];
+            f = g_flag[v * K + hl];
+            const uint4* a = reinterpret_cast<const uint4*>(P8 + v * 32);
+            const uint4* b = reinterpret_cast<const uint4*>(P8 + (uint64_t)id * 32);
+            const uint4 x0 = a[0], x1 = a[1], y0 = b[0], y1 = b[1];
+            const uint32_t xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            const uint32_t ys[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+            float dot = 0.f;
+#pragma unroll
+            for (int w = 0; w < 8; ++w)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    dot = __builtin_fmaf(e4m3_value((xs[w] >> (8 * j)) & 0xFFu), e4m3_value((ys[w] >> (8 * j)) & 0xFFu),
+                                         dot);
+            dd = fmaxf((pn8[v] + pn8[id]) - 2.f * dot, 0.f) * kscale;
+        }
+        half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
+    }
+}
+
 // Learned phase exit: exact FP32 distances for every list entry (from the
 // bf16 rows where those are exact), rows re-sorted, flags kept.  Counted as
 // exact distances outside the join.
@@ -1965,7 +2003,7 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
-    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE | NNM_FLAG_LEARNED_RENEW))
+    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE | NNM_FLAG_LEARNED_RENEW | NNM_FLAG_LEARNED_FP8))
         return nnm_fail(c, NNM_ERR_INVALID, "unknown flags");
     return NNM_OK;
 }
@@ -1992,7 +2030,9 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     NNM_TRY(fill_u32(c, c->rcur, 2 * n, 0));
     // row norms in the join's metadata: bf16 norms (norm form), learned
     // projection norms, or the cosine metric's squared norms
+    const bool fp8 = p->prec == NNM_PREC_LEARNED && (p->flags & NNM_FLAG_LEARNED_FP8);
     const float* nrm = p->prec == NNM_PREC_BF16 ? c->norms
+                       : fp8 ? nnm_mlp_norms8(c)
                        : p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c)
                        : c->metric == NNM_METRIC_COSINE ? c->sumsq : nullptr;
     hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->label,
@@ -2163,6 +2203,11 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     // runs every iteration (the fallback)
     bool learned = p->prec == NNM_PREC_LEARNED && nnm_mlp_trained(c);
     if (learned) NNM_TRY(nnm_mlp_ensure_projection(c, nullptr));
+    const bool fp8 = learned && (p->flags & NNM_FLAG_LEARNED_FP8);
+    if (fp8) {
+        if (nnm_mlp_dout(c) != 32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "FP8 learned join built for d_out = 32");
+        NNM_TRY(nnm_mlp_ensure_fp8(c));
+    }
     if (p->prec != NNM_PREC_FP32) NNM_TRY(nnm_ensure_bf16(c));
     stats_reset(st);
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
@@ -2186,9 +2231,15 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     bool estimates = false;  // the lists hold learned estimates
     auto to_estimates = [&]() -> int {
         const uint32_t dout = nnm_mlp_dout(c);
-        hipLaunchKernelGGL(dout == 32 ? k_lists_estimate<4> : k_lists_estimate<0>, dim3(wave_grid(c, (c->n_act + 1) / 2)),
-                           dim3(256), 0, c->stream, c->n_act, c->K, nnm_mlp_rows(c), nnm_mlp_norms(c), dout,
-                           nnm_mlp_scale(c), c->g_ids, c->g_dist, c->g_flag);
+        if (fp8)
+            hipLaunchKernelGGL(k_lists_estimate_fp8, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream,
+                               c->n_act, c->K, nnm_mlp_rows8(c), nnm_mlp_norms8(c), nnm_mlp_scale8(c), c->g_ids,
+                               c->g_dist, c->g_flag);
+        else
+            hipLaunchKernelGGL(dout == 32 ? k_lists_estimate<4> : k_lists_estimate<0>,
+                               dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K,
+                               nnm_mlp_rows(c), nnm_mlp_norms(c), dout, nnm_mlp_scale(c), c->g_ids, c->g_dist,
+                               c->g_flag);
         NNM_LAUNCH_CHECK(c);
         estimates = true;
         return NNM_OK;
@@ -2245,7 +2296,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
                 row_bytes = (uint64_t)c->d * 4 + 12;
                 flops_per_dist = 3ull * c->d;
             } else if (q.prec == NNM_PREC_LEARNED) {
-                row_bytes = (uint64_t)nnm_mlp_dout(c) * 2 + 16;
+                row_bytes = (uint64_t)nnm_mlp_dout(c) * (fp8 ? 1 : 2) + 16;
                 flops_per_dist = 2ull * nnm_mlp_dout(c);
             } else {
                 row_bytes = (uint64_t)c->d * 2 + 16;

@@ -205,6 +205,11 @@ const float* nnm_mlp_norms(nnm_ctx* c);
 uint32_t nnm_mlp_dout(nnm_ctx* c);
 float nnm_mlp_scale(nnm_ctx* c);
 int nnm_mlp_trained(nnm_ctx* c);
+// FP8 (OCP e4m3) rows of the projection for the learned join's FP8 prefilter
+int nnm_mlp_ensure_fp8(nnm_ctx* c);
+const uint8_t* nnm_mlp_rows8(nnm_ctx* c);
+const float* nnm_mlp_norms8(nnm_ctx* c);
+float nnm_mlp_scale8(nnm_ctx* c);
 
 // ---------------------------------------------------------- device helpers
 

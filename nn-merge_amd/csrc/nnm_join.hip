@@ -501,6 +501,27 @@ __device__ inline float offer_key(float an, float bn, float g, int mode, float k
 }
 #define BJS 64         // bf16 window rows (one meta entry per lane)
 
+// One MFMA step of a 16x16 Gram block from one 8-byte unit per lane of each
+// operand: four bf16 values (16x16x16 bf16 MFMA) or, for F8, eight OCP e4m3
+// values (16x16x32 fp8 MFMA).  Lane (r, g) gives unit g of its block row r
+// for A and of its block column r for B, so both operands take the same k
+// positions and the sum over k is the dot product of the two rows.
+template <bool F8>
+__device__ inline nnm_f32x4 gram_step(const uint2 av, const uint2 bv, nnm_f32x4 acc) {
+    if constexpr (F8) {
+        const long A = (long)(((uint64_t)av.y << 32) | av.x);
+        const long B = (long)(((uint64_t)bv.y << 32) | bv.x);
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(A, B, acc, 0, 0, 0);
+    } else {
+        nnm_bf16x4 A, B;
+        A[0] = (short)(av.x & 0xFFFFu); A[1] = (short)(av.x >> 16);
+        A[2] = (short)(av.y & 0xFFFFu); A[3] = (short)(av.y >> 16);
+        B[0] = (short)(bv.x & 0xFFFFu); B[1] = (short)(bv.x >> 16);
+        B[2] = (short)(bv.y & 0xFFFFu); B[3] = (short)(bv.y >> 16);
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc, 0, 0, 0);
+    }
+}
+
 template <int U>
 struct Bf16Lds {
     uint2 rows[BJS * U];           // window rows x U units of 8 bytes
@@ -508,7 +529,8 @@ struct Bf16Lds {
     uint32_t id[BJS];
 };
 
-template <int U>  // 8-byte units per bf16 row: ld = 4 * U
+// U: 8-byte units per row (bf16 rows of 4U elements; F8: e4m3 rows of 8U)
+template <int U, bool F8 = false>
 __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb,
                                                   const float* __restrict__ norms, int mode, float kscale, float tau) {
     a.salt = join_salt(a);
@@ -625,12 +647,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                             const uint32_t u = ks * 4 + lg;
                             const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
                             const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
-                            nnm_bf16x4 A, B;
-                            A[0] = (short)(av.x & 0xFFFFu); A[1] = (short)(av.x >> 16);
-                            A[2] = (short)(av.y & 0xFFFFu); A[3] = (short)(av.y >> 16);
-                            B[0] = (short)(bv.x & 0xFFFFu); B[1] = (short)(bv.x >> 16);
-                            B[2] = (short)(bv.y & 0xFFFFu); B[3] = (short)(bv.y >> 16);
-                            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc, 0, 0, 0);
+                            acc = gram_step<F8>(av, bv, acc);
                         }
                         // lane holds G[4 lg + q][lr] of the block
                         const uint32_t cj = cb * 16 + lr;
@@ -694,7 +711,7 @@ struct Bf16LongLds {
     uint32_t meta[4][BLM];
 };
 
-template <int U>
+template <int U, bool F8 = false>
 __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb,
                                                         const float* __restrict__ norms, int mode, float kscale, float tau) {
     a.salt = join_salt(a);
@@ -763,12 +780,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
                     const uint32_t u = ks * 4 + lg;
                     const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
                     const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
-                    nnm_bf16x4 A, B;
-                    A[0] = (short)(av.x & 0xFFFFu); A[1] = (short)(av.x >> 16);
-                    A[2] = (short)(av.y & 0xFFFFu); A[3] = (short)(av.y >> 16);
-                    B[0] = (short)(bv.x & 0xFFFFu); B[1] = (short)(bv.x >> 16);
-                    B[2] = (short)(bv.y & 0xFFFFu); B[3] = (short)(bv.y >> 16);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B, acc, 0, 0, 0);
+                    acc = gram_step<F8>(av, bv, acc);
                 }
                 const uint32_t cj = cb * 16 + lr;
                 const bool colok = cj < m;
@@ -928,6 +940,30 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
             // phase; pairs above learn_tau x the estimated k-th rarely enter
             tau = p->learn_tau > 0.f ? p->learn_tau : NNM_LEARN_TAU_DEFAULT;
             if (!rows) return nnm_fail(c, NNM_ERR_STATE, "learned join without a projection");
+            if (p->flags & NNM_FLAG_LEARNED_FP8) {
+                // FP8 prefilter: e4m3 rows of the projection (32 bytes at d_out 32)
+                if (rl != 32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "FP8 learned join built for d_out = 32");
+                const uint16_t* rows8 = reinterpret_cast<const uint16_t*>(nnm_mlp_rows8(c));
+                if (!rows8) return nnm_fail(c, NNM_ERR_STATE, "FP8 learned join without FP8 rows");
+                a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;
+                const float* nrm8 = nnm_mlp_norms8(c);
+                const float ks8 = nnm_mlp_scale8(c);
+                hipLaunchKernelGGL((k_join_bf16<4, true>), dim3(join_grid(c, k_join_bf16<4, true>)), dim3(64), 0,
+                                   c->stream, a, rows8, nrm8, mode, ks8, tau);
+                NNM_LAUNCH_CHECK(c);
+                if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));
+                hipLaunchKernelGGL((k_join_bf16_long<4, true>), dim3(join_grid256(c, k_join_bf16_long<4, true>)),
+                                   dim3(256), 0, c->stream, a, rows8, nrm8, mode, ks8, tau);
+                NNM_LAUNCH_CHECK(c);
+                // lists beyond the long-list LDS window, as on the bf16 path
+                if (c->cap_new + c->cap_old > BLM) {
+                    hipLaunchKernelGGL(k_join_f32_windows<NNM_METRIC_L2>,
+                                       dim3(join_grid(c, k_join_f32_windows<NNM_METRIC_L2>)), dim3(64), 0, c->stream,
+                                       a, (uint32_t)BLM);
+                    NNM_LAUNCH_CHECK(c);
+                }
+                return NNM_OK;
+            }
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
         a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;

@@ -45,6 +45,15 @@ struct nnm_mlp_state {
     uint16_t* P = nullptr;
     float* pnorm = nullptr;
     uint64_t projected_n = 0;
+    uint64_t proj_gen = 0;  // counts computed projections
+    // FP8 copy of the projection (OCP e4m3, scaled by s8 = 448 / max |P|),
+    // the squared norms of the encoded rows, and the projection it encodes
+    uint8_t* P8 = nullptr;
+    float* pnorm8 = nullptr;
+    uint32_t* amax = nullptr;  // max |P| bits (one word)
+    float s8 = 1.f;
+    uint64_t fp8_rows = 0;  // rows P8 holds room for
+    uint64_t fp8_gen = 0;   // proj_gen of the projection P8 encodes (0: none)
 };
 
 static nnm_mlp_state* mlp_of(nnm_ctx* c) { return reinterpret_cast<nnm_mlp_state*>(c->mlp); }
@@ -309,7 +318,7 @@ void nnm_mlp_free(nnm_ctx* c) {
     nnm_mlp_state* M = mlp_of(c);
     if (!M) return;
     void* ps[] = {M->W1, M->m, M->v, M->g, M->xs, M->pre, M->h, M->z, M->dz, M->dh, M->t, M->s, M->loss,
-                  M->pairs, M->Wb, M->P, M->pnorm};
+                  M->pairs, M->Wb, M->P, M->pnorm, M->P8, M->pnorm8, M->amax};
     for (void* p : ps) nnm_free(c, p);
     delete M;
     c->mlp = nullptr;
@@ -611,12 +620,111 @@ int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs) {
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     if (secs) *secs = ms * 1e-3;
     M->projected_n = c->n;
+    ++M->proj_gen;
     return NNM_OK;
 }
 
 void nnm_mlp_invalidate_projection(nnm_ctx* c) {
     nnm_mlp_state* M = mlp_of(c);
-    if (M) M->projected_n = 0;
+    if (M) {
+        M->projected_n = 0;
+        M->fp8_gen = 0;
+    }
+}
+
+// ---- FP8 projection rows (the learned join's FP8 prefilter)
+
+// max |P| over the projection: bf16 magnitudes order like their bits, so an
+// atomicMax on the bits (sign cleared) finds it.
+__global__ void k_bf16_absmax(const uint16_t* __restrict__ P, uint64_t m, uint32_t* __restrict__ out) {
+    uint32_t mx = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        mx = max(mx, (uint32_t)(P[i] & 0x7FFFu));
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+    if ((threadIdx.x & 63u) == 0 && mx) atomicMax(out, mx);
+}
+
+// OCP e4m3 (e4m3fn) value of a byte: bias 7, no infinities (the encoder
+// below never produces the NaN codes).
+__device__ inline float fp8_e4m3_value(uint32_t b) {
+    const uint32_t e = (b >> 3) & 15u, m = b & 7u;
+    const float v = e ? __uint_as_float(((e + 120u) << 23) | (m << 20)) : (float)m * 0.001953125f;  // m 2^-9
+    return (b & 0x80u) ? -v : v;
+}
+
+// Row r: x = clamp(s P[r], +-448) rounded to e4m3 by the hardware convert
+// (round to nearest even; the clamp keeps it off the NaN code), and the
+// squared norm of the encoded row in element order (FMA), the norm form's
+// |q|^2.  One thread per row, rows of dout (a multiple of 4) elements.
+__global__ void k_mlp_to_fp8(const uint16_t* __restrict__ P, uint64_t n, uint32_t dout,
+                             const uint32_t* __restrict__ amax, uint8_t* __restrict__ P8,
+                             float* __restrict__ pnorm8) {
+    const float mx = __uint_as_float(*amax << 16);
+    const float s = mx > 0.f ? 448.f / mx : 1.f;
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint16_t* src = P + r * dout;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(P8 + r * dout);
+        float nrm = 0.f;
+        for (uint32_t k = 0; k < dout; k += 4) {
+            float x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                x[j] = fminf(fmaxf(__uint_as_float((uint32_t)src[k + j] << 16) * s, -448.f), 448.f);
+            int w = __builtin_amdgcn_cvt_pk_fp8_f32(x[0], x[1], 0, false);
+            w = __builtin_amdgcn_cvt_pk_fp8_f32(x[2], x[3], w, true);
+            dst[k / 4] = (uint32_t)w;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float q = fp8_e4m3_value(((uint32_t)w >> (8 * j)) & 0xFFu);
+                nrm = __builtin_fmaf(q, q, nrm);
+            }
+        }
+        pnorm8[r] = nrm;
+    }
+}
+
+int nnm_mlp_ensure_fp8(nnm_ctx* c) {
+    nnm_mlp_state* M = mlp_of(c);
+    NNM_TRY_MLP(nnm_mlp_ensure_projection(c, nullptr));
+    if (M->dout % 4) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "FP8 rows need d_out a multiple of 4");
+    if (M->P8 && M->fp8_gen == M->proj_gen) return NNM_OK;
+    if (!M->P8 || M->fp8_rows != c->n) {
+        nnm_free(c, M->P8);
+        nnm_free(c, M->pnorm8);
+        M->P8 = nullptr;
+        M->pnorm8 = nullptr;
+        M->fp8_rows = 0;
+        NNM_TRY_MLP(mlp_alloc(c, &M->P8, (size_t)c->n * M->dout));
+        NNM_TRY_MLP(mlp_alloc(c, &M->pnorm8, c->n));
+        M->fp8_rows = c->n;
+    }
+    if (!M->amax) NNM_TRY_MLP(mlp_alloc(c, &M->amax, 1));
+    NNM_TRY_MLP(nnm_fill_u32(c, M->amax, 1, 0));
+    const uint64_t m = (uint64_t)c->n * M->dout;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((m + 255) / 256, (uint64_t)c->num_cus * 8);
+    hipLaunchKernelGGL(k_bf16_absmax, dim3(grid), dim3(256), 0, c->stream, M->P, m, M->amax);
+    NNM_LAUNCH_CHECK(c);
+    const uint32_t rgrid = (uint32_t)std::min<uint64_t>((c->n + 255) / 256, (uint64_t)c->num_cus * 16);
+    hipLaunchKernelGGL(k_mlp_to_fp8, dim3(rgrid), dim3(256), 0, c->stream, M->P, c->n, M->dout, M->amax, M->P8,
+                       M->pnorm8);
+    NNM_LAUNCH_CHECK(c);
+    uint32_t h = 0;
+    NNM_TRY_MLP(nnm_d2h(c, &h, M->amax, 4));
+    const uint32_t u = h << 16;
+    float mx;
+    std::memcpy(&mx, &u, 4);
+    M->s8 = mx > 0.f ? 448.f / mx : 1.f;
+    M->fp8_gen = M->proj_gen;
+    return NNM_OK;
+}
+
+extern "C" int nnm_mlp_project_fp8(nnm_ctx* c, uint8_t* out, float* scale) {
+    if (!c || !mlp_of(c)) return nnm_fail(c, NNM_ERR_STATE, "no model");
+    NNM_TRY_MLP(nnm_mlp_ensure_fp8(c));
+    nnm_mlp_state* M = mlp_of(c);
+    if (out) NNM_TRY_MLP(nnm_d2h(c, out, M->P8, (size_t)c->n * M->dout));
+    if (scale) *scale = M->s8;
+    return NNM_OK;
 }
 
 extern "C" int nnm_mlp_project(nnm_ctx* c, float* out, double* secs) {
@@ -642,4 +750,10 @@ const uint16_t* nnm_mlp_rows(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->P : nul
 const float* nnm_mlp_norms(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->pnorm : nullptr; }
 uint32_t nnm_mlp_dout(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->dout : 0; }
 float nnm_mlp_scale(nnm_ctx* c) { return mlp_of(c) ? 1.f / (mlp_of(c)->alpha * mlp_of(c)->alpha) : 1.f; }
+const uint8_t* nnm_mlp_rows8(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->P8 : nullptr; }
+const float* nnm_mlp_norms8(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->pnorm8 : nullptr; }
+// estimate scale of the FP8 rows: distances of s8-scaled rows back to a^2 units
+float nnm_mlp_scale8(nnm_ctx* c) {
+    return mlp_of(c) ? 1.f / (mlp_of(c)->alpha * mlp_of(c)->alpha * mlp_of(c)->s8 * mlp_of(c)->s8) : 1.f;
+}
 int nnm_mlp_trained(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->trained : 0; }

@@ -21,7 +21,7 @@ PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
 RULE_ALL, RULE_MERGE = 0, 1
 METRIC_L2, METRIC_L1, METRIC_COSINE, METRIC_CHI2 = 0, 1, 2, 3
 RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
-FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_LEARNED_RENEW = 1, 2, 4
+FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_LEARNED_RENEW, FLAG_LEARNED_FP8 = 1, 2, 4, 8
 SEARCH_LAYERED, SEARCH_FLAT = 0, 1
 SEARCH_SRC_KNN, SEARCH_SRC_DPG = 0, 1
 
@@ -110,6 +110,7 @@ _SIGS = {
     "nnm_mlp_train": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint64,
                                 C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_double)]),
     "nnm_mlp_project": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
+    "nnm_mlp_project_fp8": (C.c_int, [_vp, _vp, C.POINTER(C.c_float)]),
     "nnm_host_hmerge": (C.c_int, [_vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp, _vp,
                                   C.POINTER(Stats), _vp, _vp, _vp, _vp, C.c_uint32, C.POINTER(C.c_uint32)]),
     "nnm_host_build": (C.c_int, [C.c_int, _vp, C.c_uint64, C.c_uint32, C.POINTER(Params), C.c_uint32, _vp, _vp,
@@ -336,6 +337,14 @@ class Context:
         secs = C.c_double()
         self._chk(lib().nnm_mlp_project(self.h, _ptr(out), C.byref(secs)))
         return out, secs.value
+
+    def mlp_project_fp8(self, want=True):
+        """FP8 rows of the projection (n x d_out uint8, OCP e4m3) and their scale."""
+        hid, dout = self.mlp_shape
+        out = np.empty((self.n, dout), np.uint8) if want else None
+        sc = C.c_float()
+        self._chk(lib().nnm_mlp_project_fp8(self.h, _ptr(out), C.byref(sc)))
+        return out, sc.value
 
     def dist_error(self):
         e = DistError()

@@ -74,13 +74,10 @@ def test_mlp_training_reduces_loss(ctx):
     assert 1.0 <= tau < 10.0
 
 
-def test_learned_join_ablation(ctx):
-    """J-Merge of 100k low-rank sift-like rows: plain BF16 join vs the learned
-    phase with a trained network vs with the untrained (random) network.
-    The learned run must keep recall within 0.01 of the plain join (its
-    fallback for the last iteration); the trained network must not do worse
-    than the untrained one.  Exact distances outside the join: one per list
-    entry when the lists leave the learned space."""
+@pytest.fixture(scope="module")
+def jm100k(ctx):
+    """J-Merge input of 100k low-rank sift-like rows: the built half's
+    NN-Descent lists plus K placeholder neighbours for the raw half."""
     n, K = 100000, 20
     half = n // 2
     X = nnm_data.sift_like(n, 128, centers=256, seed=nnm_data.SEED_BASE + 4, rank=12)
@@ -91,30 +88,51 @@ def test_learned_join_ablation(ctx):
     ctx.random_init(K, seed=3)
     ctx.nndescent(nnm.default_params(K=K, seed=3))
     g0, _, _ = ctx.get_graph()
-    ctx.set_data(X)
-    ctx.set_partition([0, half, n], [1, 0], [0, 0])
     init = np.vstack([g0, ((np.arange(half, n)[:, None] + 1 + np.arange(K)[None, :]) % n).astype(np.uint32)])
+    return X, rows, gt, init
+
+
+def _jmerge(ctx, jm, prec, flags):
+    X, rows, gt, init = jm
+    n, K = init.shape
+    half = n // 2
+    ctx.set_partition([0, half, n], [1, 0], [0, 0])
+    ctx.set_graph(init)
+    # learned runs: 5 learned iterations, then exact lists and 3 plain iterations
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=0.5, max_iters=8 if prec == nnm.PREC_LEARNED else 10,
+                           prec=prec, seed=5, plain_tail=3, flags=flags)
+    ctx.merge_init(p)
+    st = ctx.nndescent(p)
+    ctx.merge_finish(True)
+    ids, _, _ = ctx.get_graph()
+    return nnm.recall(ids, gt, rows), st
+
+
+def test_learned_join_ablation(ctx, jm100k):
+    """J-Merge of 100k low-rank sift-like rows: plain BF16 join vs the learned
+    phase with a trained network vs with the untrained (random) network.
+    The learned run must keep recall within 0.01 of the plain join (its
+    fallback for the last iteration); the trained network must not do worse
+    than the untrained one.  Exact distances outside the join: one per list
+    entry when the lists leave the learned space."""
+    X, rows, gt, init = jm100k
+    n, K = init.shape
+    ctx.set_data(X)
     res = {}
 
     def merge(prec):
-        ctx.set_graph(init)
-        # learned runs: 5 learned iterations, then every entry renewed and 3
-        # plain iterations
-        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=0.5, max_iters=8 if prec == nnm.PREC_LEARNED else 10,
-                               prec=prec, seed=5, plain_tail=3, flags=nnm.FLAG_LEARNED_RENEW)
-        ctx.merge_init(p)
-        st = ctx.nndescent(p)
-        ctx.merge_finish(True)
-        ids, _, _ = ctx.get_graph()
-        return nnm.recall(ids, gt, rows), st
+        # every entry renewed at the switch
+        return _jmerge(ctx, jm100k, prec, nnm.FLAG_LEARNED_RENEW)
 
     res["plain"] = merge(nnm.PREC_BF16)
     # network trained on pairs of the built half (the plain NN-Descent's own lists)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
     ctx.set_graph(init)
     ctx.mlp_init(256, 32, seed=7)
     res["untrained_tau"] = None
     ctx.mlp_train(steps=1, batch=1024, lr=0.0, keep=0.98, seed=8)  # calibrates tau only
     res["untrained"] = merge(nnm.PREC_LEARNED)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
     ctx.set_graph(init)
     ctx.mlp_init(256, 32, seed=7)
     tst = ctx.mlp_train(steps=600, batch=4096, lr=2e-3, keep=0.98, seed=9)
@@ -131,3 +149,66 @@ def test_learned_join_ablation(ctx):
     assert r_tr >= r_plain - 0.01
     assert r_tr >= r_un - 0.002
     assert s_tr["reranked"] == n * K  # one exact recompute per list entry at the switch
+
+
+_E4M3 = np.array([(2.0 ** ((c >> 3) - 7)) * (1 + (c & 7) / 8.0) if (c >> 3) else (c & 7) * 2.0 ** -9
+                  for c in range(127)])  # positive OCP e4m3 values, codes 0..126 ascending
+
+
+def _e4m3_value(b):
+    b = b.astype(np.int64)
+    v = _E4M3[np.minimum(b & 0x7F, 126)]
+    return np.where(b & 0x80, -v, v)
+
+
+def _e4m3_encode(x):
+    """Round to nearest e4m3 (ties to the even code), |x| <= 448."""
+    ax = np.abs(x.astype(np.float64))
+    hi = np.clip(np.searchsorted(_E4M3, ax), 1, 126)
+    lo = hi - 1
+    dl, dh = ax - _E4M3[lo], _E4M3[hi] - ax
+    code = np.where(dl < dh, lo, np.where(dh < dl, hi, np.where(lo % 2 == 0, lo, hi)))
+    return (code | np.where(x < 0, 0x80, 0)).astype(np.uint8)
+
+
+def test_fp8_rows_encode_the_projection(ctx):
+    """FP8 rows of the learned phase: OCP e4m3 of clamp(s p, +-448), rounded
+    to nearest even, s = 448 / max |p|, against a table encoder."""
+    X = nnm_data.sift_like(6000, 128, centers=32, seed=73, rank=12)
+    ctx.set_data(X)
+    ctx.random_init(16, seed=2)
+    ctx.nndescent(nnm.default_params(K=16, seed=2, max_iters=5))
+    ctx.mlp_init(256, 32, seed=3)
+    ctx.mlp_train(steps=50, batch=2048, lr=2e-3, keep=0.98, seed=4)
+    P, _ = ctx.mlp_project(want=True)  # the bf16 projection, as floats
+    P8, s = ctx.mlp_project_fp8()
+    assert s == np.float32(448.0) / np.float32(np.abs(P).max())
+    x = np.clip(P.astype(np.float32) * np.float32(s), np.float32(-448), np.float32(448))
+    ref = _e4m3_encode(x)
+    assert np.array_equal(_e4m3_value(P8), _e4m3_value(ref)), "FP8 rows differ from round-to-nearest-even e4m3"
+    # a second projection (the model trained on) re-encodes
+    ctx.mlp_train(steps=5, batch=2048, lr=2e-3, keep=0.98, seed=5)
+    P2, _ = ctx.mlp_project(want=True)
+    P82, s2 = ctx.mlp_project_fp8()
+    x2 = np.clip(P2.astype(np.float32) * np.float32(s2), np.float32(-448), np.float32(448))
+    assert np.array_equal(_e4m3_value(P82), _e4m3_value(_e4m3_encode(x2)))
+
+
+def test_learned_fp8_prefilter(ctx, jm100k):
+    """The learned phase on FP8 rows (16x16x32 fp8 MFMA) against the same
+    phase on bf16 rows, both with a trained network and the plain BF16
+    tail: the FP8 prefilter may lose a little recall, the tail judges it."""
+    X, rows, gt, init = jm100k
+    n, K = init.shape
+    ctx.set_data(X)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
+    ctx.set_graph(init)
+    ctx.mlp_init(256, 32, seed=7)
+    ctx.mlp_train(steps=600, batch=4096, lr=2e-3, keep=0.98, seed=9)
+    r_bf, s_bf = _jmerge(ctx, jm100k, nnm.PREC_LEARNED, 0)
+    r_f8, s_f8 = _jmerge(ctx, jm100k, nnm.PREC_LEARNED, nnm.FLAG_LEARNED_FP8)
+    r_pl, _ = _jmerge(ctx, jm100k, nnm.PREC_BF16, 0)
+    print(f"\nlearned bf16 rows {r_bf:.4f} ({s_bf['secs']:.4f} s), fp8 rows {r_f8:.4f} ({s_f8['secs']:.4f} s), "
+          f"plain {r_pl:.4f}; offers bf16 {s_bf['offers']} fp8 {s_f8['offers']}")
+    assert s_f8["learned_iters"] == 5 and s_f8["plain_iters"] == 3
+    assert r_f8 >= r_bf - 0.03
