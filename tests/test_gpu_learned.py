@@ -212,3 +212,23 @@ def test_learned_fp8_prefilter(ctx, jm100k):
           f"plain {r_pl:.4f}; offers bf16 {s_bf['offers']} fp8 {s_f8['offers']}")
     assert s_f8["learned_iters"] == 5 and s_f8["plain_iters"] == 3
     assert r_f8 >= r_bf - 0.03
+
+
+def test_learned_fp8_rejects_other_widths(ctx):
+    """FP8 rows are built for d_out = 32: another width is refused with
+    NNM_ERR_UNSUPPORTED before any launch, and the context stays usable."""
+    X = nnm_data.sift_like(4000, 128, centers=16, seed=75, rank=12)
+    n, K = X.shape[0], 16
+    ctx.set_data(X)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
+    ctx.random_init(K, seed=2)
+    ctx.mlp_init(256, 16, seed=3)
+    ctx.mlp_train(steps=5, batch=1024, lr=2e-3, keep=0.98, seed=4)
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, max_iters=4, prec=nnm.PREC_LEARNED, plain_tail=1,
+                           flags=nnm.FLAG_LEARNED_FP8)
+    with pytest.raises(nnm.NNMError) as e:
+        ctx.nndescent(p)
+    assert e.value.code == nnm.NNM_ERR_UNSUPPORTED
+    p.flags = 0
+    st = ctx.nndescent(p)  # the bf16 learned phase still runs
+    assert st["learned_iters"] >= 1
