@@ -246,9 +246,9 @@ def main():
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = library default)")
     ap.add_argument("--fp8", action="store_true",
                     help="learned: run the learned phase on FP8 (OCP e4m3) rows of the projection")
-    ap.add_argument("--renew", action="store_true",
-                    help="learned: mark every list entry new when the lists leave the learned space (default: "
-                         "keep the flags; measured slower and lower in recall on the 5M J-Merge)")
+    ap.add_argument("--switch", choices=["found", "keep", "renew"], default="found",
+                    help="learned: flags when the lists leave the learned space: mark new the entries the "
+                         "learned phase found (default), keep every flag, or mark every entry new")
     ap.add_argument("--learned-iters", type=int, default=8,
                     help="learned: total iterations T of the learned run (learned phase + plain tail)")
     ap.add_argument("--graph", action="store_true", help="replay iterations from HIP graphs")
@@ -323,8 +323,9 @@ def main():
 
     def run(name):
         T = args.learned_iters if (name == "learned" and args.learned_iters) else args.iters
-        fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and args.renew) else 0)
+        fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and args.switch == "renew") else 0)
         fl |= nnm.FLAG_LEARNED_FP8 if (name == "learned" and args.fp8) else 0
+        fl |= nnm.FLAG_LEARNED_MARK_FOUND if (name == "learned" and args.switch == "found") else 0
         p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name],
                                seed=seed, plain_tail=args.plain_tail, flags=fl)
         if args.learn_tau > 0:
@@ -435,7 +436,7 @@ def main():
                       "slots": args.slots, "path": chosen, "parallelism": "single GPU",
                       "learned": {"T": args.learned_iters, "plain_tail": args.plain_tail,
                                   "learn_tau": args.learn_tau if args.learn_tau > 0 else "library default (0.6)",
-                                  "renew": bool(args.renew), "fp8_rows": bool(args.fp8)}},
+                                  "switch": args.switch, "fp8_rows": bool(args.fp8)}},
            "merge_secs": ch["merge_secs"], "recall@10": ch.get("recall@10"), "fp64_error": ch.get("fp64_error"),
            "upload": {"bytes": int(X.nbytes), "secs": round(upload_secs, 4),
                       "note": "nnm_set_data host -> HBM (pinned staging, with the bf16-exactness scan); "

@@ -69,6 +69,12 @@ extern "C" {
                                    scaled by 448 / max |p|, on the 16x16x32
                                    fp8 MFMA (d_out = 32); estimates and the
                                    plain tail are unchanged otherwise */
+#define NNM_FLAG_LEARNED_MARK_FOUND 16u /* NNM_PREC_LEARNED: when the lists
+                                   leave the learned space, the entries the
+                                   learned phase found (ids not in the row when
+                                   it started) are marked new, so the plain
+                                   join re-judges them; the others keep their
+                                   flags (RENEW marks every entry) */
 
 /* Re-rank policy for approximate precisions. */
 #define NNM_RERANK_AUTO 0   /* skip when the approximate form is exact */

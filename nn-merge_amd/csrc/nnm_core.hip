@@ -273,6 +273,8 @@ static void free_graph(nnm_ctx* c) {
     c->g_ids = nullptr; c->g_dist = nullptr; c->g_flag = nullptr;
     nnm_free(c, c->s_ids); nnm_free(c, c->s_dist); nnm_free(c, c->s_flag);
     c->s_ids = nullptr; c->s_dist = nullptr; c->s_flag = nullptr;
+    nnm_free(c, c->p_ids);
+    c->p_ids = nullptr;
     nnm_free(c, c->c_ids); nnm_free(c, c->c_dist); nnm_free(c, c->c_flag); nnm_free(c, c->has_cpn);
     c->c_ids = nullptr; c->c_dist = nullptr; c->c_flag = nullptr; c->has_cpn = nullptr;
     c->K = 0; c->cut = 0; c->cpn_live = 0;
@@ -799,22 +801,33 @@ __global__ void k_lists_estimate_fp8(uint64_t n, uint32_t K, const uint8_t* __re
 // Learned phase exit: exact FP32 distances for every list entry (from the
 // bf16 rows where those are exact), rows re-sorted, flags kept.  Counted as
 // exact distances outside the join.
+// prior (optional): the lists when the learned phase started; an entry whose
+// id is not in its row's prior list was found by the learned phase and is
+// marked new.
 __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ X, const uint16_t* __restrict__ Xb_exact,
-                              uint32_t ld, int renew, uint32_t* __restrict__ g_ids, float* __restrict__ g_dist,
-                              uint8_t* __restrict__ g_flag, uint64_t* __restrict__ counters) {
-    const uint32_t hl = nnm_lane() & 31u;
+                              uint32_t ld, int renew, const uint32_t* __restrict__ prior, uint32_t* __restrict__ g_ids,
+                              float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
+                              uint64_t* __restrict__ counters) {
+    const uint32_t hl = nnm_lane() & 31u, hbase = nnm_lane() & 32u;
     uint64_t nd = 0;
     NNM_HALF_LOOP(v0, n) {
         const uint64_t v = v0 + (nnm_lane() >> 5);
         const bool ok = v < n;
-        uint32_t id = 0xFFFFFFFFu, f = 0;
+        uint32_t id = 0xFFFFFFFFu, f = 0, pr = 0xFFFFFFFFu;
         float dd = 3.402823466e38f;
         if (ok && hl < K) {
             id = g_ids[v * K + hl];
             f = renew ? 1u : g_flag[v * K + hl];
+            if (prior) pr = prior[v * K + hl];
             dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
                           : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
             ++nd;
+        }
+        if (prior) {
+            // membership in the half's prior row (every lane takes part in the shuffles)
+            bool found = false;
+            for (uint32_t p = 0; p < K; ++p) found |= (uint32_t)__shfl((int)pr, (int)(hbase + p), NNM_WAVE) == id;
+            if (!found) f = 1u;
         }
         half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
@@ -2003,7 +2016,8 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
-    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE | NNM_FLAG_LEARNED_RENEW | NNM_FLAG_LEARNED_FP8))
+    if (p->flags & ~(NNM_FLAG_GRAPH | NNM_FLAG_AUTOTUNE | NNM_FLAG_LEARNED_RENEW | NNM_FLAG_LEARNED_FP8 |
+                     NNM_FLAG_LEARNED_MARK_FOUND))
         return nnm_fail(c, NNM_ERR_INVALID, "unknown flags");
     return NNM_OK;
 }
@@ -2204,6 +2218,8 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     bool learned = p->prec == NNM_PREC_LEARNED && nnm_mlp_trained(c);
     if (learned) NNM_TRY(nnm_mlp_ensure_projection(c, nullptr));
     const bool fp8 = learned && (p->flags & NNM_FLAG_LEARNED_FP8);
+    const bool mark_found = learned && (p->flags & NNM_FLAG_LEARNED_MARK_FOUND) && !(p->flags & NNM_FLAG_LEARNED_RENEW);
+    if (mark_found && !c->p_ids) NNM_TRY(alloc_n(c, &c->p_ids, c->n * c->K));
     if (fp8) {
         if (nnm_mlp_dout(c) != 32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "FP8 learned join built for d_out = 32");
         NNM_TRY(nnm_mlp_ensure_fp8(c));
@@ -2230,6 +2246,10 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     bool tail = !learned;
     bool estimates = false;  // the lists hold learned estimates
     auto to_estimates = [&]() -> int {
+        // the lists as the learned phase finds them, for marking its finds later
+        if (mark_found)
+            NNM_HIP(c, hipMemcpyAsync(c->p_ids, c->g_ids, (size_t)c->n_act * c->K * sizeof(uint32_t),
+                                      hipMemcpyDeviceToDevice, c->stream));
         const uint32_t dout = nnm_mlp_dout(c);
         if (fp8)
             hipLaunchKernelGGL(k_lists_estimate_fp8, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream,
@@ -2247,7 +2267,8 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     auto to_exact = [&]() -> int {
         hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
                            (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld,
-                           (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, c->g_ids, c->g_dist, c->g_flag, c->counters);
+                           (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, mark_found ? c->p_ids : nullptr, c->g_ids,
+                           c->g_dist, c->g_flag, c->counters);
         NNM_LAUNCH_CHECK(c);
         estimates = false;
         return NNM_OK;

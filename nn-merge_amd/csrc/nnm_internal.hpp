@@ -94,6 +94,7 @@ struct nnm_ctx {
     uint32_t* s_ids = nullptr;      // checkpoint copy
     float* s_dist = nullptr;
     uint8_t* s_flag = nullptr;
+    uint32_t* p_ids = nullptr;      // lists at the learned phase's start (NNM_FLAG_LEARNED_MARK_FOUND)
 
     // parked far entries of a merge (cpnGraph)
     uint32_t cut = 0;

@@ -21,7 +21,7 @@ PREC_FP32, PREC_BF16, PREC_LEARNED = 0, 1, 2
 RULE_ALL, RULE_MERGE = 0, 1
 METRIC_L2, METRIC_L1, METRIC_COSINE, METRIC_CHI2 = 0, 1, 2, 3
 RERANK_AUTO, RERANK_ALWAYS, RERANK_NEVER = 0, 1, 2
-FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_LEARNED_RENEW, FLAG_LEARNED_FP8 = 1, 2, 4, 8
+FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_LEARNED_RENEW, FLAG_LEARNED_FP8, FLAG_LEARNED_MARK_FOUND = 1, 2, 4, 8, 16
 SEARCH_LAYERED, SEARCH_FLAT = 0, 1
 SEARCH_SRC_KNN, SEARCH_SRC_DPG = 0, 1
 

@@ -232,3 +232,36 @@ def test_learned_fp8_rejects_other_widths(ctx):
     p.flags = 0
     st = ctx.nndescent(p)  # the bf16 learned phase still runs
     assert st["learned_iters"] >= 1
+
+
+def test_learned_switch_marks_found_entries(ctx, jm100k):
+    """NNM_FLAG_LEARNED_MARK_FOUND: when the lists leave the learned space,
+    every entry whose id was not in its row when the learned phase started
+    is flagged new; with kept flags some of those are not (so the check
+    is not vacuous).  A run of learned iterations only ends with that
+    switch."""
+    X, rows, gt, init = jm100k
+    n, K = init.shape
+    ctx.set_data(X)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
+    ctx.set_graph(init)
+    ctx.mlp_init(256, 32, seed=7)
+    ctx.mlp_train(steps=200, batch=4096, lr=2e-3, keep=0.98, seed=9)
+    out = {}
+    for name, flags in (("kept", 0), ("found", nnm.FLAG_LEARNED_MARK_FOUND)):
+        ctx.set_graph(init)
+        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=0.5, max_iters=5, prec=nnm.PREC_LEARNED, seed=5,
+                               plain_tail=0, flags=flags)
+        ctx.merge_init(p)
+        prior, _, _ = ctx.get_graph()
+        st = ctx.nndescent(p)
+        ids, dist, fl = ctx.get_graph()
+        assert st["learned_iters"] >= 1 and st["plain_iters"] == 0
+        in_prior = (ids[:, :, None] == prior[:, None, :]).any(-1)
+        out[name] = (in_prior, fl)
+    in_prior, fl = out["found"]
+    assert (fl[~in_prior] == 1).all(), "an entry the learned phase found is not flagged new"
+    in_prior_k, fl_k = out["kept"]
+    assert (fl_k[~in_prior_k] == 0).any(), "kept flags already mark every found entry: the check is vacuous"
+    print(f"\nfound entries: {(~in_prior).mean():.3f} of all; with kept flags {(fl_k[~in_prior_k] == 0).mean():.3f} "
+          f"of them were not new")
