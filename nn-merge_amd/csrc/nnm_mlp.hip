@@ -448,6 +448,7 @@ extern "C" int nnm_mlp_set_weights(nnm_ctx* c, const float* W1, const float* b1,
     NNM_TRY_MLP(nnm_h2d(c, M->W2, W2, (size_t)M->dout * M->hid * 4));
     NNM_TRY_MLP(nnm_h2d(c, M->b2, b2, (size_t)M->dout * 4));
     if (alpha > 0.f) M->alpha = alpha;
+    nnm_mlp_invalidate_projection(c);  // the cached rows came from the old weights
     return NNM_OK;
 }
 
@@ -572,6 +573,7 @@ extern "C" int nnm_mlp_train(nnm_ctx* c, uint32_t steps, uint32_t batch, float l
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     NNM_TRY_MLP(calibrate_tau(c, 8192, keep, nnm_rand(seed, 0xCA11B, 0)));
     M->trained = 1;
+    nnm_mlp_invalidate_projection(c);  // the cached rows came from the old weights
     if (loss_first) *loss_first = lf;
     if (loss_last) *loss_last = ll;
     if (secs) *secs = ms * 1e-3;

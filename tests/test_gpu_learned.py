@@ -265,3 +265,24 @@ def test_learned_switch_marks_found_entries(ctx, jm100k):
     assert (fl_k[~in_prior_k] == 0).any(), "kept flags already mark every found entry: the check is vacuous"
     print(f"\nfound entries: {(~in_prior).mean():.3f} of all; with kept flags {(fl_k[~in_prior_k] == 0).mean():.3f} "
           f"of them were not new")
+
+
+def test_projection_follows_weight_changes(ctx):
+    """New weights (training or nnm_mlp_set_weights) invalidate the cached
+    projection and its FP8 rows: the next use projects again, without an
+    explicit nnm_mlp_project."""
+    X = nnm_data.sift_like(3000, 128, centers=16, seed=77, rank=12)
+    ctx.set_data(X)
+    ctx.random_init(16, seed=2)
+    ctx.mlp_init(256, 32, seed=3)
+    P8a, sa = ctx.mlp_project_fp8()  # projects on first use
+    W1, b1, W2, b2, alpha, _ = ctx.mlp_weights()
+    ctx.mlp_set_weights(W1, b1, -W2, -b2)
+    P8b, sb = ctx.mlp_project_fp8()  # no explicit mlp_project in between
+    assert sb == sa
+    assert np.array_equal(_e4m3_value(P8b), -_e4m3_value(P8a)), "FP8 rows not rebuilt after new weights"
+    ctx.mlp_train(steps=5, batch=1024, lr=2e-3, keep=0.98, seed=4)
+    P8c, sc = ctx.mlp_project_fp8()
+    P, _ = ctx.mlp_project(want=True)
+    x = np.clip(P.astype(np.float32) * np.float32(sc), np.float32(-448), np.float32(448))
+    assert np.array_equal(_e4m3_value(P8c), _e4m3_value(_e4m3_encode(x))), "FP8 rows not rebuilt after training"
