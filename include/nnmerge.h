@@ -158,6 +158,8 @@ typedef struct nnm_stats {
                               (FP32 distances within rounding of each other) */
     uint64_t tie_swaps;    /* ... where FP64 kept the other entry */
     uint64_t slot_atomics; /* slot atomics issued by the join (one per offer) */
+    uint64_t learned_dists;/* part of dists joined on learned estimates
+                              (projected distances, not exact ones) */
 } nnm_stats;
 
 /* FP64 error of the stored FP32 list distances (nnm_graph_dist_error). */

@@ -2232,6 +2232,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     uint32_t it = 0, n_learned = 0, n_plain = 0, n_graph = 0;
     uint32_t used_js = 0, used_chunk = 0;
     uint64_t prev_se = 0, prev_sd = 0, prev_so = 0, main_bytes = 0, main_flops = 0;
+    uint64_t prev_dall = 0, learned_dists = 0;
     int converged = 0;
     double last_delta = 1.0;
     const bool autotune = (p->flags & NNM_FLAG_AUTOTUNE) != 0;
@@ -2312,6 +2313,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
             prev_se += se;
             prev_sd += sd;
             prev_so += so;
+            // pairs joined on learned estimates (projected distances)
+            if (q.prec == NNM_PREC_LEARNED) learned_dists += h[CNT_DISTS] - prev_dall;
+            prev_dall = h[CNT_DISTS];
             uint64_t row_bytes, flops_per_dist;
             if (q.prec == NNM_PREC_FP32) {
                 row_bytes = (uint64_t)c->d * 4 + 12;
@@ -2369,6 +2373,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->join_main_flops = main_flops;
         st->near_ties = h[CNT_NEAR_TIES];
         st->slot_atomics = h[CNT_ATOMICS];
+        st->learned_dists = learned_dists;
         st->tie_swaps = h[CNT_TIE_SWAPS];
     }
     return NNM_OK;

@@ -41,7 +41,8 @@ class Stats(C.Structure):
                 ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32),
                 ("long_dists", C.c_uint64), ("long_entries", C.c_uint64), ("long_offers", C.c_uint64),
                 ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64),
-                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64), ("slot_atomics", C.c_uint64)]
+                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64), ("slot_atomics", C.c_uint64),
+                ("learned_dists", C.c_uint64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
