@@ -239,9 +239,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--keep", type=float, default=0.98)
     ap.add_argument("--plain-tail", type=int, default=3)
-    ap.add_argument("--slots", type=int, default=64,
-                    help="update slots per vertex for every run (the library default is 128: fewer slots lose "
-                         "a few more colliding candidates per iteration and run faster)")
+    ap.add_argument("--offer-cap", type=int, default=0,
+                    help="offer-log records per row (0 = library default); a full log costs redo rounds, "
+                         "never offers")
     ap.add_argument("--learn-tau", type=float, default=0.0,
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = library default)")
     ap.add_argument("--fp8", action="store_true",
@@ -330,7 +330,7 @@ def main():
                                seed=seed, plain_tail=args.plain_tail, flags=fl)
         if args.learn_tau > 0:
             p.learn_tau = args.learn_tau
-        p.slots = args.slots
+        p.offer_cap = args.offer_cap
         train = None
         if name == "learned":
             ctx.graph_restore(recompute=True)
@@ -367,8 +367,8 @@ def main():
         for k, (r, i, s) in enumerate(recs):
             log(f"[{name}] step {k}: restore {r['secs']:.3f}s init {i['secs']:.3f}s descent {s['secs']:.3f}s "
                 f"join {s['join_secs']:.3f}s (short {s['join_main_secs']:.3f}s) iters {s['iters']} "
-                f"dists {s['dists']} exact {s['reranked']} offers {s['offers']} atomics {s['slot_atomics']} "
-                f"updates {s['updates']}")
+                f"deliver {s['deliver_secs']:.3f}s dists {s['dists']} exact {s['reranked']} offers {s['offers']} "
+                f"redo {s['redo_rows']}/{s['redo_rounds']} updates {s['updates']}")
         dists = float(np.mean([a["init_dists"] + b["init_dists"] + c["dists"] for a, b, c in recs]))
         # what the count holds: every pair the join resolves (its exact distance
         # on the plain paths; its 32-d learned estimate in learned iterations)
@@ -443,7 +443,7 @@ def main():
            "dtype": "f32" if chosen == "fp32" else "bf16",
            "data": "synthetic (seeded generator nnm_data.sift_like)",
            "config": {"workload": workload, "n": n, "d": d, "K": K, "rho": args.rho, "max_iters": args.iters,
-                      "slots": args.slots, "path": chosen, "parallelism": "single GPU",
+                      "offer_cap": args.offer_cap, "path": chosen, "parallelism": "single GPU",
                       "learned": {"T": args.learned_iters, "plain_tail": args.plain_tail,
                                   "learn_tau": args.learn_tau if args.learn_tau > 0 else "library default (0.6)",
                                   "switch": args.switch, "fp8_rows": bool(args.fp8)}},

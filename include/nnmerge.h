@@ -108,10 +108,12 @@ typedef struct nnm_params {
     uint32_t rule;      /* NNM_RULE_* */
     uint32_t prec;      /* NNM_PREC_* */
     uint32_t rerank;    /* NNM_RERANK_* */
-    uint32_t slots;     /* update slots per vertex per iteration (power of
-                           two, 16..128); 0 = 128.  An offer that loses its
-                           slot to a better one is dropped for the
-                           iteration, so fewer slots lose more candidates */
+    uint32_t offer_cap; /* offer-log capacity: records per row the join may
+                           write per delivery round; 0 = 256 per row plus
+                           one 4096-record chunk per join wave.  Every
+                           offer reaches its target's list: when the log
+                           fills, the vertices it could not take are joined
+                           again in another round (slower, same result) */
     uint32_t cut;       /* merge set-up: far entries replaced by cross
                            samples; 0 = K/2 (cutSize, src/smerge.cpp:99) */
     uint32_t plain_tail;/* NNM_PREC_LEARNED with max_iters: the last
@@ -135,7 +137,8 @@ typedef struct nnm_stats {
     uint64_t updates;      /* list entries replaced */
     uint64_t reranked;     /* candidates recomputed in exact FP32 */
     uint64_t list_entries; /* candidate-list entries joined */
-    uint64_t offers;       /* update offers that passed the k-th prefilter */
+    uint64_t offers;       /* offers that passed the k-th prefilter (each is
+                              written to the offer log and delivered) */
     double last_delta;     /* mean fraction of new entries after the loop */
     double secs;           /* device time of the call */
     double join_secs;      /* device time spent in the join kernels */
@@ -157,9 +160,14 @@ typedef struct nnm_stats {
     uint64_t near_ties;    /* k-th slot decisions re-made on FP64 distances
                               (FP32 distances within rounding of each other) */
     uint64_t tie_swaps;    /* ... where FP64 kept the other entry */
-    uint64_t slot_atomics; /* slot atomics issued by the join (one per offer) */
+    uint64_t redo_rows;    /* vertices joined again because the offer log
+                              was full (0 when offer_cap suffices) */
     uint64_t learned_dists;/* part of dists joined on learned estimates
                               (projected distances, not exact ones) */
+    uint32_t redo_rounds;  /* extra join rounds those vertices took */
+    uint32_t pad2_;
+    double deliver_secs;   /* device time of offer delivery (partition passes
+                              and list owners; eager iterations) */
 } nnm_stats;
 
 /* FP64 error of the stored FP32 list distances (nnm_graph_dist_error). */

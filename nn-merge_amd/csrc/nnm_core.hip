@@ -203,7 +203,7 @@ extern "C" void nnm_default_params(nnm_params* p) {
     p->rule = NNM_RULE_ALL;
     p->prec = NNM_PREC_FP32;
     p->rerank = NNM_RERANK_AUTO;
-    p->slots = 128;
+    p->offer_cap = 0;
     p->cut = 0;
     p->plain_tail = 1;
     p->flags = 0;
@@ -228,6 +228,7 @@ extern "C" int nnm_create(nnm_ctx** out, int device) {
     hipEventCreate(&c->evj0);
     hipEventCreate(&c->evj1);
     hipEventCreate(&c->evjm);
+    hipEventCreate(&c->evd1);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     if (alloc_n(c, &c->counters, CNT_N) != NNM_OK) { delete c; return NNM_ERR_NOMEM; }
@@ -250,6 +251,7 @@ extern "C" int nnm_destroy(nnm_ctx* c) {
     hipEventDestroy(c->evj0);
     hipEventDestroy(c->evj1);
     hipEventDestroy(c->evjm);
+    hipEventDestroy(c->evd1);
     hipStreamDestroy(c->stream);
     delete c;
     return NNM_OK;
@@ -284,10 +286,10 @@ static void free_iter(nnm_ctx* c) {
     void** ps[] = {(void**)&c->l_new, (void**)&c->l_old, (void**)&c->n_new, (void**)&c->n_old,
                    (void**)&c->newmask, (void**)&c->rcnt, (void**)&c->roff, (void**)&c->rcur,
                    (void**)&c->rbuf, (void**)&c->ocnt, (void**)&c->obuf, (void**)&c->spill,
-                   (void**)&c->scan_tmp, (void**)&c->slots, (void**)&c->kth,
-                   (void**)&c->vmeta};
+                   (void**)&c->scan_tmp, (void**)&c->kth, (void**)&c->vmeta};
     for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
-    c->cap_new = c->cap_old = c->rev_cap = c->slots_C = 0;
+    nnm_offer_free(c);
+    c->cap_new = c->cap_old = c->rev_cap = 0;
 }
 
 static void free_partition(nnm_ctx* c) {
@@ -1634,14 +1636,8 @@ __global__ void k_scan_apply(const uint32_t* __restrict__ in, uint64_t m, const 
     }
 }
 
-// ================================================================ update (merge slots into lists)
+// ================================================================ FP64 error report
 
-// One wave per row: pool (lanes < K) and slot candidates (lanes < C) merged
-// by rank into the sorted list.  A slot candidate enters when it is not
-// already in the list and its (dist, id) key is below the k-th entry; that is
-// UpdateKnnList's rule applied to the whole batch of the iteration.  Slots
-// are reset to empty.  With rerank, candidates are recomputed in exact FP32
-// before ranking.
 // FP64 squared L2 across the wave: lanes take elements lane, lane + 64, ...;
 // the partial sums are combined in a fixed tree (same result every call).
 __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
@@ -1653,254 +1649,6 @@ __device__ inline double wave_l2_f64(const float* __restrict__ a, const float* _
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, NNM_WAVE);
     return acc;
-}
-
-// Candidate count above which the update sorts the candidates (a 21-step
-// bitonic network) instead of ranking them by a loop over the candidates.
-#define UPD_SORT_MIN 32u
-
-// Merge each vertex's slots into its list (UpdateKnnList, src/pvetex.h:40-85,
-// for a batch): one wave per vertex, list entry j and slot j in lane j.  The
-// candidates that can enter (below the k-th key, or within the FP32 rounding
-// window of it) are de-duplicated against the list.  Every key is then ranked
-// against the list and against the other candidates by one loop over the
-// candidates read through scalar registers; all keys are distinct (a list holds distinct ids,
-// an id always hashes to the same slot, candidates equal to a list id are
-// dropped), so the ranks give each kept entry its new position without a
-// sort.  Slots are reset to empty.  With rerank, lower-bound keys are
-// recomputed exactly first.  The next vertex's list and slots are loaded
-// while the current one is merged.
-// Compiled per configuration, so the common ones carry no dead state:
-// TWO = two slot sets per lane (C > 64), RR = exact re-rank of lower-bound
-// keys, TIE = FP64 near-tie decision at the k-th slot (tie_win > 1).
-template <bool TWO, bool RR, bool TIE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_update(uint64_t n, uint32_t K, uint32_t C, const float* __restrict__ X,
-                         const uint16_t* __restrict__ Xb_exact, uint32_t ld, float rerank_tau,
-                         float tie_win, uint32_t* __restrict__ g_ids,
-                         float* __restrict__ g_dist, uint8_t* __restrict__ g_flag, uint64_t* __restrict__ slots,
-                         uint64_t* __restrict__ counters) {
-    const uint32_t lane = nnm_lane();
-    const uint64_t W = (uint64_t)gridDim.x * (blockDim.x / NNM_WAVE);
-    uint64_t c_upd = 0, c_flags = 0, c_rr = 0, c_ties = 0, c_swaps = 0;  // wave-uniform
-    // slots j and 64 + j of a vertex are read by lane j (C <= 128)
-    auto fetch = [&](uint64_t u, uint32_t& id, float& d, uint32_t& f, uint64_t& sk0, uint64_t& sk1) {
-        id = 0xFFFFFFFFu;
-        d = 0.f;
-        f = 0;
-        sk0 = NNM_EMPTY_KEY;
-        sk1 = NNM_EMPTY_KEY;
-        if (u >= n) return;
-        if (lane < K) {
-            id = g_ids[u * K + lane];
-            d = g_dist[u * K + lane];
-            f = g_flag[u * K + lane];
-        }
-        if (lane < C) sk0 = slots[u * C + lane];
-        if constexpr (TWO) {
-            if (lane + NNM_WAVE < C) sk1 = slots[u * C + NNM_WAVE + lane];
-        }
-    };
-    uint64_t v = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
-    uint32_t pid, pf;
-    float pd;
-    uint64_t skey0, skey1;
-    fetch(v, pid, pd, pf, skey0, skey1);
-    for (; v < n; v += W) {
-        uint32_t nid, nf;
-        float nd;
-        uint64_t nsk0, nsk1;
-        fetch(v + W, nid, nd, nf, nsk0, nsk1);
-        if (skey0 != NNM_EMPTY_KEY) slots[v * C + lane] = NNM_EMPTY_KEY;
-        if constexpr (TWO) {
-            if (skey1 != NNM_EMPTY_KEY) slots[v * C + NNM_WAVE + lane] = NNM_EMPTY_KEY;
-        }
-        [&]() {
-            const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
-            const uint64_t kth_key = wave_read64(pkey, K - 1);
-            const float kd = nnm_key_dist(kth_key);
-            const uint32_t pflags = (uint32_t)__popcll(__ballot(lane < K && pf));
-            // cheap filter first: estimates against the margin, exact keys
-            // against the k-th key (or its rounding window)
-            auto pass = [&](uint64_t sk) {
-                if constexpr (RR) return sk != NNM_EMPTY_KEY && nnm_key_dist(sk) <= kd * rerank_tau;
-                else if constexpr (TIE) return sk != NNM_EMPTY_KEY && (sk < kth_key || nnm_key_dist(sk) <= kd * tie_win);
-                else return sk < kth_key;  // the empty key is above every key
-            };
-            bool valid0 = pass(skey0), valid1 = TWO && pass(skey1);
-            if (!__ballot(valid0 || valid1)) {
-                c_flags += pflags;
-                return;
-            }
-            uint64_t ck0 = skey0, ck1 = skey1;
-            if constexpr (RR) {
-                // de-dup before the exact recomputation (no distance for an id
-                // already listed)
-                const uint32_t sid0 = nnm_key_id(skey0), sid1 = nnm_key_id(skey1);
-                for (uint32_t p = 0; p < K; ++p) {
-                    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p);
-                    valid0 = valid0 && q != sid0;
-                    valid1 = valid1 && q != sid1;
-                }
-                auto exact = [&](uint32_t sid) {
-                    return Xb_exact ? nnm_l2_rows_bf16<1>(Xb_exact + v * ld, Xb_exact + (uint64_t)sid * ld, ld)
-                                    : nnm_l2_rows<1>(X + v * ld, X + (uint64_t)sid * ld, ld);
-                };
-                if (valid0) ck0 = nnm_key(exact(sid0), sid0);
-                if (valid1) ck1 = nnm_key(exact(sid1), sid1);
-                c_rr += (uint32_t)__popcll(__ballot(valid0)) + (uint32_t)__popcll(__ballot(valid1));
-                valid0 = valid0 && (ck0 < kth_key || (TIE && nnm_key_dist(ck0) <= kd * tie_win));
-                valid1 = valid1 && (ck1 < kth_key || (TIE && nnm_key_dist(ck1) <= kd * tie_win));
-            }
-            // one candidate set, at most one candidate per lane
-            uint64_t ck;
-            bool valid;
-            {
-                const uint64_t vb0 = __ballot(valid0), vb1 = TWO ? __ballot(valid1) : 0ull;
-                if (!TWO || !vb1) {
-                    ck = ck0;
-                    valid = valid0;
-                } else if (!vb0) {
-                    ck = ck1;
-                    valid = valid1;
-                } else {
-                    const uint32_t n0 = (uint32_t)__popcll(vb0), n1 = (uint32_t)__popcll(vb1);
-                    if (n0 + n1 <= NNM_WAVE) {
-                        // lane j < n0 takes the j-th candidate of the low slots,
-                        // lane n0 + j the j-th of the high slots
-                        const bool from0 = lane < n0;
-                        const uint32_t src = nth_set_bit(from0 ? vb0 : vb1, from0 ? lane : lane - n0);
-                        const uint64_t a0 = shfl64(valid0 ? ck0 : NNM_EMPTY_KEY, src);
-                        const uint64_t a1 = shfl64(valid1 ? ck1 : NNM_EMPTY_KEY, src);
-                        valid = lane < n0 + n1;
-                        ck = valid ? (from0 ? a0 : a1) : NNM_EMPTY_KEY;
-                    } else {
-                        // more than 64 candidates: keep the 64 smallest keys (a
-                        // list takes at most K <= 32 of them), the minima of the
-                        // low set ascending against the high set descending
-                        const uint64_t s0 = nnm_wave_sort64(valid0 ? ck0 : NNM_EMPTY_KEY);
-                        const uint64_t s1 = nnm_wave_sort64(valid1 ? ck1 : NNM_EMPTY_KEY);
-                        const uint64_t r1 = shfl64(s1, NNM_WAVE - 1 - lane);
-                        ck = s0 < r1 ? s0 : r1;
-                        valid = ck != NNM_EMPTY_KEY;
-                    }
-                }
-            }
-            uint64_t vb = __ballot(valid);
-            const uint64_t lmask = K < NNM_WAVE ? (1ull << K) - 1 : ~0ull;  // list lanes
-            if (!RR && vb) {
-                // de-dup against the list after the compaction (at most K of the
-                // kept keys repeat a list id, so the K best distinct candidates
-                // are still there): a loop over the fewer of candidates and
-                // list entries
-                if ((uint32_t)__popcll(vb) <= K) {
-                    const uint32_t cid = nnm_key_id(ck);
-                    for (uint64_t b = vb; b; b &= b - 1) {
-                        const uint32_t cl = (uint32_t)__builtin_ctzll(b);
-                        const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)cid, (int)cl);
-                        if (__ballot(pid == q) & lmask) vb &= ~(1ull << cl);
-                    }
-                    valid = (vb >> lane) & 1ull;
-                } else {
-                    const uint32_t cid = nnm_key_id(ck);
-                    for (uint32_t p = 0; p < K; ++p)
-                        valid = valid && (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p) != cid;
-                    vb = __ballot(valid);
-                }
-            }
-            if (!vb) {
-                c_flags += pflags;
-                return;
-            }
-            // ranks: a candidate's position is (list keys below it) + (candidate
-            // keys below it); a list entry's is its index + (candidate keys below it)
-            uint32_t below_list = 0, below_cand_c = 0, below_cand_p = 0;
-            const uint32_t nv = (uint32_t)__popcll(vb);
-            if (nv > UPD_SORT_MIN) {
-                // many candidates: sort them into lanes 0..nv-1 (a candidate's
-                // candidate rank is then its lane) and count per list entry
-                ck = nnm_wave_sort64(valid ? ck : NNM_EMPTY_KEY);
-                valid = lane < nv;
-                below_cand_c = lane;
-                for (uint32_t p = 0; p < K; ++p) {
-                    const uint64_t pk = wave_read64(pkey, p);
-                    below_list += pk < ck ? 1u : 0u;
-                    const uint32_t cnt = (uint32_t)__popcll(__ballot(valid && ck < pk));
-                    if (lane == p) below_cand_p = cnt;
-                }
-            } else {
-                // one pass over the candidates: list and candidate keys are
-                // distinct, so the list keys below candidate c are the K list
-                // lanes minus those whose key c is below
-                for (uint64_t b = vb; b; b &= b - 1) {
-                    const uint32_t cl = (uint32_t)__builtin_ctzll(b);
-                    const uint64_t ci = wave_read64(ck, cl);
-                    const bool before_p = ci < pkey;
-                    below_cand_c += ci < ck ? 1u : 0u;
-                    below_cand_p += before_p ? 1u : 0u;
-                    const uint32_t nb = K - (uint32_t)__popcll(__ballot(before_p) & lmask);
-                    if (lane == cl) below_list = nb;
-                }
-            }
-            uint32_t pos_p = lane + below_cand_p;
-            uint32_t pos_s = below_list + below_cand_c;
-            if constexpr (TIE) {
-                // the entries ranked K-1 (last kept) and K (first dropped): when
-                // their FP32 distances are within rounding of each other the
-                // choice is re-made on FP64 distances (ties by id)
-                const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(valid && pos_s == K - 1);
-                const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(valid && pos_s == K);
-                if ((ap | as) && (bp | bs)) {
-                    const bool a_pool = ap != 0, b_pool = bp != 0;
-                    const uint32_t la = (uint32_t)__builtin_ctzll(a_pool ? ap : as);
-                    const uint32_t lb = (uint32_t)__builtin_ctzll(b_pool ? bp : bs);
-                    const uint64_t ka = wave_read64(a_pool ? pkey : ck, la);
-                    const uint64_t kb = wave_read64(b_pool ? pkey : ck, lb);
-                    if (nnm_key_dist(kb) <= nnm_key_dist(ka) * tie_win) {
-                        ++c_ties;
-                        const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
-                        const double ea = wave_l2_f64(X + v * ld, X + (uint64_t)ia * ld, ld);
-                        const double eb = wave_l2_f64(X + v * ld, X + (uint64_t)ib * ld, ld);
-                        if (eb < ea || (eb == ea && ib < ia)) {
-                            ++c_swaps;
-                            if (lane == la) {
-                                if (a_pool) pos_p = K; else pos_s = K;
-                            }
-                            if (lane == lb) {
-                                if (b_pool) pos_p = K - 1; else pos_s = K - 1;
-                            }
-                        }
-                    }
-                }
-            }
-            const bool ins = valid && pos_s < K;
-            const bool keep = lane < K && pos_p < K;
-            if (keep && pos_p != lane) {
-                g_ids[v * K + pos_p] = pid;
-                g_dist[v * K + pos_p] = pd;
-                g_flag[v * K + pos_p] = (uint8_t)pf;
-            }
-            if (ins) {
-                g_ids[v * K + pos_s] = nnm_key_id(ck);
-                g_dist[v * K + pos_s] = nnm_key_dist(ck);
-                g_flag[v * K + pos_s] = 1;
-            }
-            const uint32_t n_ins = (uint32_t)__popcll(__ballot(ins));
-            c_upd += n_ins;
-            c_flags += (uint32_t)__popcll(__ballot(keep && pf)) + n_ins;
-        }();
-        pid = nid;
-        pd = nd;
-        pf = nf;
-        skey0 = nsk0;
-        skey1 = nsk1;
-    }
-    if (lane == 0) {
-        if (c_upd) atomicAdd((unsigned long long*)&counters[CNT_UPDATES], (unsigned long long)c_upd);
-        if (c_flags) atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)c_flags);
-        if (c_rr) atomicAdd((unsigned long long*)&counters[CNT_RERANKED], (unsigned long long)c_rr);
-        if (c_ties) atomicAdd((unsigned long long*)&counters[CNT_NEAR_TIES], (unsigned long long)c_ties);
-        if (c_swaps) atomicAdd((unsigned long long*)&counters[CNT_TIE_SWAPS], (unsigned long long)c_swaps);
-    }
 }
 
 float nnm_near_tie_window(nnm_ctx* c, const nnm_params* p) {
@@ -1975,14 +1723,12 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
     const uint32_t R = p->rev_cap;
     const uint32_t cap_new = S + R, cap_old = K + R;
-    const uint32_t C = p->slots ? p->slots : 128;
-    if (c->l_new && c->cap_new == cap_new && c->cap_old == cap_old && c->slots_C == C) return NNM_OK;
+    if (c->l_new && c->cap_new == cap_new && c->cap_old == cap_old) return nnm_offer_buffers(c, p);
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     free_iter(c);
     c->cap_new = cap_new;
     c->cap_old = cap_old;
     c->rev_cap = R;
-    c->slots_C = C;
     const uint64_t n = c->n;
     NNM_TRY(alloc_n(c, &c->l_new, n * cap_new));
     NNM_TRY(alloc_n(c, &c->l_old, n * cap_old));
@@ -1997,11 +1743,9 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     NNM_TRY(alloc_n(c, &c->obuf, n * K));
     NNM_TRY(alloc_n(c, &c->spill, n));
     NNM_TRY(alloc_n(c, &c->scan_tmp, (2 * n + SCAN_TILE - 1) / SCAN_TILE + 1));
-    NNM_TRY(alloc_n(c, &c->slots, n * C));
     NNM_TRY(alloc_n(c, &c->kth, n));
     NNM_TRY(alloc_n(c, &c->vmeta, n));
-    NNM_TRY(fill_u64(c, c->slots, n * C, NNM_EMPTY_KEY));
-    return NNM_OK;
+    return nnm_offer_buffers(c, p);
 }
 
 static int validate_params(nnm_ctx* c, const nnm_params* p) {
@@ -2011,8 +1755,8 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (p->K != c->K) return nnm_fail(c, NNM_ERR_INVALID, "params K differs from the graph's K");
     if (!(p->rho > 0.f && p->rho <= 1.f)) return nnm_fail(c, NNM_ERR_INVALID, "rho must be in (0, 1]");
     if (p->rev_cap == 0 || p->rev_cap > 4096) return nnm_fail(c, NNM_ERR_INVALID, "rev_cap must be in [1, 4096]");
-    const uint32_t C = p->slots ? p->slots : 128;
-    if (C < 16 || C > 128 || (C & (C - 1))) return nnm_fail(c, NNM_ERR_INVALID, "slots must be 16, 32, 64 or 128");
+    if (p->offer_cap != 0 && (p->offer_cap < 4 || p->offer_cap > 65536))
+        return nnm_fail(c, NNM_ERR_INVALID, "offer_cap must be 0 (default) or in [4, 65536]");
     if (p->rule > NNM_RULE_MERGE) return nnm_fail(c, NNM_ERR_INVALID, "unknown rule");
     if (p->prec > NNM_PREC_LEARNED) return nnm_fail(c, NNM_ERR_INVALID, "unknown precision");
     if (p->rerank > NNM_RERANK_NEVER) return nnm_fail(c, NNM_ERR_INVALID, "unknown rerank policy");
@@ -2077,73 +1821,72 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     return NNM_OK;
 }
 
-// Merge the slots into the lists (with the re-rank the precision asks for).
-static int update_lists(nnm_ctx* c, const nnm_params* p) {
-    int rerank = 0;
-    float tau = 1.f;
-    if (p->prec == NNM_PREC_BF16) {
-        rerank = (p->rerank == NNM_RERANK_ALWAYS) || (p->rerank == NNM_RERANK_AUTO && !c->data_exact_bf16);
-    }
-    // learned phase: lists and offers are both estimates of one metric, so the
-    // update merges them as they are (the exact distances come at the switch)
-    if (p->rerank == NNM_RERANK_NEVER || p->prec == NNM_PREC_LEARNED) rerank = 0;
-    const float tie_win = nnm_near_tie_window(c, p);
-    const bool two = c->slots_C > NNM_WAVE, tie = tie_win > 1.f;
-    const unsigned cfg = (two ? 4u : 0u) | (rerank ? 2u : 0u) | (tie ? 1u : 0u);
-    void (*kern)(uint64_t, uint32_t, uint32_t, const float*, const uint16_t*, uint32_t, float, float, uint32_t*, float*,
-                 uint8_t*, uint64_t*, uint64_t*) = nullptr;
-    switch (cfg) {
-        case 0: kern = k_update<false, false, false>; break;
-        case 1: kern = k_update<false, false, true>; break;
-        case 2: kern = k_update<false, true, false>; break;
-        case 3: kern = k_update<false, true, true>; break;
-        case 4: kern = k_update<true, false, false>; break;
-        case 5: kern = k_update<true, false, true>; break;
-        case 6: kern = k_update<true, true, false>; break;
-        default: kern = k_update<true, true, true>; break;
-    }
-    hipLaunchKernelGGL(kern, dim3(wave_grid(c, c->n_act)), dim3(256), 0, c->stream, c->n_act, c->K, c->slots_C, c->X,
-                       (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld, tau, tie_win, c->g_ids, c->g_dist,
-                       c->g_flag, c->slots, c->counters);
-    NNM_LAUNCH_CHECK(c);
-    return NNM_OK;
-}
-
 static int reset_queues(nnm_ctx* c) {
     NNM_TRY(fill_u64(c, c->counters + CNT_WORK, 1, 0));
     NNM_TRY(fill_u64(c, c->counters + CNT_WORK2, 2, 0));
     return NNM_OK;
 }
 
-// join_ms[0] += whole join, join_ms[1] += its short-list kernel (evj0..evjm)
+// join_ms[0] += whole join, join_ms[1] += its short-list kernel (evj0..evjm),
+// join_ms[2] += offer delivery (evj1..evd1)
 static int add_join_times(nnm_ctx* c, double* join_ms) {
-    NNM_HIP(c, hipEventSynchronize(c->evj1));
-    float ms = 0.f, mm = 0.f;
+    NNM_HIP(c, hipEventSynchronize(c->evd1));
+    float ms = 0.f, mm = 0.f, md = 0.f;
     hipEventElapsedTime(&ms, c->evj0, c->evj1);
     hipEventElapsedTime(&mm, c->evj0, c->evjm);
+    hipEventElapsedTime(&md, c->evj1, c->evd1);
     join_ms[0] += ms;
     join_ms[1] += mm;
+    join_ms[2] += md;
     return NNM_OK;
 }
 
-// One iteration; leaves the flag count in counters[CNT_FLAGS].  The
+// One iteration; leaves the flag count in counters[CNT_FLAGS] (exact unless
+// the join left vertices for a redo round, counters[CNT_REDO] > 0).  The
 // iteration number is read from counters[CNT_ITER], which the caller sets.
 // join_ms == nullptr: no events or waits, so the sequence can be captured.
 static int run_iteration(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, double* join_ms) {
     NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
     NNM_TRY(reset_queues(c));
+    NNM_TRY(nnm_offer_reset(c));
     NNM_TRY(build_lists(c, p));
     if (join_ms) NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
     NNM_TRY(nnm_launch_join(c, p, js, chunk, join_ms ? c->evjm : nullptr));
     if (join_ms) NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
-    NNM_TRY(update_lists(c, p));
-    if (join_ms) NNM_TRY(add_join_times(c, join_ms));
+    NNM_TRY(nnm_offer_deliver(c, p));
+    if (join_ms) {
+        NNM_HIP(c, hipEventRecord(c->evd1, c->stream));
+        NNM_TRY(add_join_times(c, join_ms));
+    }
+    return NNM_OK;
+}
+
+// Redo rounds: join again the vertices whose offers did not fit the log
+// (counters[CNT_REDO] after a round), deliver, until none is left.  The
+// owners of the round that leaves none count the iteration's updates and
+// flags.
+static int redo_rounds(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, uint64_t nwork,
+                       uint32_t* rounds) {
+    uint32_t round = 1;
+    while (nwork) {
+        NNM_TRY(reset_queues(c));
+        NNM_TRY(nnm_offer_reset(c));
+        NNM_TRY(nnm_launch_join(c, p, js, chunk, nullptr, round, nwork));
+        NNM_TRY(nnm_offer_deliver(c, p));
+        uint64_t left = 0;
+        NNM_TRY(nnm_d2h(c, &left, c->counters + CNT_REDO, sizeof(left)));
+        ++*rounds;
+        if (left >= nwork)
+            return nnm_fail(c, NNM_ERR_NOMEM, "offer log too small for the offers of single vertices; raise offer_cap");
+        nwork = left;
+        ++round;
+    }
     return NNM_OK;
 }
 
 // One iteration that times every join launch shape on the lists it builds
-// and keeps the fastest for this precision.  The join only lowers slot keys
-// (atomicMin), so repeated launches leave the slots as one launch does; the
+// and keeps the fastest for this precision.  Every trial starts from an
+// empty offer log, so the last launch leaves the log one launch writes; the
 // work counters are reset before and the tallies restored after each trial.
 static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) {
     NNM_TRY(fill_u64(c, c->counters + CNT_FLAGS, 1, 0));
@@ -2158,6 +1901,7 @@ static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) 
     for (int rep = 0; rep < 2; ++rep) {
         for (const auto& cand : space) {
             NNM_TRY(reset_queues(c));
+            NNM_TRY(nnm_offer_reset(c));
             NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
             NNM_TRY(nnm_launch_join(c, p, cand.first, cand.second));
             NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
@@ -2170,14 +1914,15 @@ static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) 
     c->tune_js[p->prec] = best_js;
     c->tune_chunk[p->prec] = best_ch;
     c->tune_ld[p->prec] = c->ld;
-    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS, CNT_LONG_DISTS, CNT_LONG_ENTRIES, CNT_LONG_OFFERS,
-                  CNT_ATOMICS})
+    for (int i : {CNT_DISTS, CNT_LIST_ENTRIES, CNT_OFFERS, CNT_LONG_DISTS, CNT_LONG_ENTRIES, CNT_LONG_OFFERS})
         NNM_TRY(fill_u64(c, c->counters + i, 1, h[i]));
     NNM_TRY(reset_queues(c));
+    NNM_TRY(nnm_offer_reset(c));
     NNM_HIP(c, hipEventRecord(c->evj0, c->stream));
     NNM_TRY(nnm_launch_join(c, p, best_js, best_ch, c->evjm));
     NNM_HIP(c, hipEventRecord(c->evj1, c->stream));
-    NNM_TRY(update_lists(c, p));
+    NNM_TRY(nnm_offer_deliver(c, p));
+    NNM_HIP(c, hipEventRecord(c->evd1, c->stream));
     return add_join_times(c, join_ms);
 }
 
@@ -2213,6 +1958,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     NNM_TRY(validate_params(c, p));
     if (p->prec != NNM_PREC_FP32) NNM_TRY(nnm_require_l2(c, "BF16 and learned joins"));
     NNM_TRY(nnm_ensure_iter_buffers(c, p));
+    if (p->prec == NNM_PREC_LEARNED && c->cap_new + c->cap_old > nnm_join_learned_max_list())
+        return nnm_fail(c, NNM_ERR_UNSUPPORTED,
+                        "learned join: candidate lists (round(rho K) + K + 2 rev_cap) longer than 256 entries");
     // learned prefilter: needs a trained model; otherwise the plain BF16 join
     // runs every iteration (the fallback)
     bool learned = p->prec == NNM_PREC_LEARNED && nnm_mlp_trained(c);
@@ -2228,7 +1976,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     stats_reset(st);
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    double join_ms[2] = {0.0, 0.0};
+    double join_ms[3] = {0.0, 0.0, 0.0};
+    uint32_t redo_rounds_run = 0;
+    uint64_t redo_rows = 0;
     uint32_t it = 0, n_learned = 0, n_plain = 0, n_graph = 0;
     uint32_t used_js = 0, used_chunk = 0;
     uint64_t prev_se = 0, prev_sd = 0, prev_so = 0, main_bytes = 0, main_flops = 0;
@@ -2302,11 +2052,17 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         used_chunk = autotune ? c->tune_chunk[q.prec] : chunk;
         uint64_t h[CNT_N];
         NNM_TRY(read_counters(c, h));
+        if (h[CNT_REDO]) {
+            // the offer log filled up: the left vertices join again
+            redo_rows += h[CNT_REDO];
+            NNM_TRY(redo_rounds(c, &q, used_js, used_chunk, h[CNT_REDO], &redo_rounds_run));
+            NNM_TRY(read_counters(c, h));
+        }
         {
             // algorithmic traffic and work of the short-list join kernel in
             // this iteration: per list entry one row, its id, label and
             // k-th distance (and norm for the norm form); per offer one
-            // 8-byte slot update; per vertex the two list sizes
+            // 12-byte log record; per vertex the two list sizes
             const uint64_t se = (h[CNT_LIST_ENTRIES] - h[CNT_LONG_ENTRIES]) - prev_se;
             const uint64_t sd = (h[CNT_DISTS] - h[CNT_LONG_DISTS]) - prev_sd;
             const uint64_t so = (h[CNT_OFFERS] - h[CNT_LONG_OFFERS]) - prev_so;
@@ -2327,7 +2083,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
                 row_bytes = (uint64_t)c->d * 2 + 16;
                 flops_per_dist = 2ull * c->d;
             }
-            main_bytes += se * row_bytes + so * 8 + c->n_act * 8;
+            main_bytes += se * row_bytes + so * 12 + c->n_act * 8;
             main_flops += sd * flops_per_dist;
         }
         ++it;
@@ -2372,7 +2128,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         st->join_main_bytes = main_bytes;
         st->join_main_flops = main_flops;
         st->near_ties = h[CNT_NEAR_TIES];
-        st->slot_atomics = h[CNT_ATOMICS];
+        st->redo_rows = redo_rows;
+        st->redo_rounds = redo_rounds_run;
+        st->deliver_secs = join_ms[2] * 1e-3;
         st->learned_dists = learned_dists;
         st->tie_swaps = h[CNT_TIE_SWAPS];
     }

@@ -8,7 +8,9 @@
 //   g_flag n x K   uint8   1 = entry is new (not yet joined)
 //   label  n       uint32  subgraph label: s for rows of built subgraph s,
 //                          NNM_RAW for raw rows
-//   slots  n x C   uint64  per-iteration update slots, key = dist bits << 32 | id
+//   offer log      records {dist bits, id, target} the join writes in order,
+//                  grouped by target by partition passes before the owners
+//                  merge them into their lists (nnm_offers.hip)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,9 +27,17 @@
 // Learned-phase offer threshold (fraction of the target's estimated k-th distance).
 #define NNM_LEARN_TAU_DEFAULT 0.6f
 
-// Slot keys: squared distances are >= 0, so their IEEE bits order like the
-// values; the low word breaks ties by id.  An empty slot holds all ones.
+// Candidate keys: squared distances are >= 0, so their IEEE bits order like
+// the values; the low word breaks ties by id.  All ones marks no candidate.
 #define NNM_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+// Offer log: records of 3 words {dist bits, id, target}, written by each join
+// wave into chunks of NNM_OCH records it reserves from a shared counter.
+#define NNM_OCH 4096u
+// Owner bins: the partition passes group offers by target >> NNM_OWN_BITS.
+#define NNM_OWN_BITS 6u
+// Default offer-log capacity per active row (nnm_params.offer_cap = 0).
+#define NNM_OFFER_CAP_DEFAULT 256u
 
 // Reverse-list bucket capacity per target (entries beyond it go to the
 // overflow CSR).
@@ -41,7 +51,7 @@ enum CounterIdx {
     CNT_LIST_ENTRIES = 4,
     CNT_WORK = 5,      // work-queue cursor of the persistent join
     CNT_TOTAL_REV = 6,
-    CNT_OFFERS = 7,    // slot offers issued by the join
+    CNT_OFFERS = 7,    // offers the join wrote to the offer log
     CNT_WORK2 = 8,     // work-queue cursor of the long-list join
     CNT_WORK3 = 9,     // work-queue cursor of the window join
     CNT_ITER = 10,     // iteration number (salts derive from it on the device)
@@ -50,14 +60,16 @@ enum CounterIdx {
     CNT_LONG_OFFERS = 13,   // the short-list kernel's)
     CNT_NEAR_TIES = 14,     // k-th slot decisions re-made in FP64
     CNT_TIE_SWAPS = 15,     // ... that changed which entry stays
-    CNT_ATOMICS = 16,       // slot atomics issued (offers not already dominated)
+    CNT_OCHUNKS = 16,       // offer-log chunks reserved in this round
+    CNT_REDO = 17,          // vertices whose offers did not fit the log (joined again)
+    CNT_ROUNDS = 18,        // extra join rounds run for such vertices
     CNT_N = 24
 };
 
 struct nnm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evj0 = nullptr, evj1 = nullptr, evjm = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evj0 = nullptr, evj1 = nullptr, evjm = nullptr, evd1 = nullptr;
     std::string err;
     uint64_t mem_cap = 0;
     uint64_t mem_used = 0;
@@ -105,7 +117,7 @@ struct nnm_ctx {
     int cpn_live = 0;
 
     // iteration scratch
-    uint32_t cap_new = 0, cap_old = 0, rev_cap = 0, slots_C = 0;
+    uint32_t cap_new = 0, cap_old = 0, rev_cap = 0;
     uint32_t* l_new = nullptr;      // n x cap_new
     uint32_t* l_old = nullptr;      // n x cap_old
     uint32_t* n_new = nullptr;      // n
@@ -119,7 +131,14 @@ struct nnm_ctx {
     uint32_t* obuf = nullptr;       // n*K overflow entries, grouped by bucket
     uint64_t* spill = nullptr;      // n: per row, the list entries past a full bucket (bit j: entry j)
     uint32_t* scan_tmp = nullptr;   // block sums
-    uint64_t* slots = nullptr;      // n x slots_C
+    // offer log and its partition (nnm_offers.hip)
+    uint32_t* orec[2] = {nullptr, nullptr};  // 2 x (ochunks x NNM_OCH) records, 3 words each
+    uint32_t* ochunk_used = nullptr;  // records written per log chunk
+    uint32_t ochunks = 0;             // log capacity in chunks
+    uint32_t ocap_row = 0;            // capacity per row it was sized for
+    uint32_t* opart_cnt = nullptr;    // per pass: bin counts (largest pass)
+    uint32_t* opart_cur[3] = {nullptr, nullptr, nullptr};  // per pass: bin cursors (bins + 1)
+    uint32_t* work[2] = {nullptr, nullptr};  // join work lists for redo rounds (n each)
     uint64_t* counters = nullptr;   // CNT_N
     float* kth = nullptr;           // n: k-th distance of every list at iteration start
     uint4* vmeta = nullptr;         // n: join record {label, kth x offer window, norm, 0}
@@ -184,7 +203,12 @@ int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p);
 #define NNM_TRY_MLP NNM_TRY
 
 // Kernel launchers shared between translation units.
-int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, hipEvent_t mid = nullptr);
+// round 0 joins every active row; round r > 0 the nwork vertices of the
+// redo list round r - 1 left (work[(r - 1) & 1]).
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, hipEvent_t mid = nullptr,
+                    uint32_t round = 0, uint64_t nwork = 0);
+// Longest candidate list (new + old) the learned join handles.
+uint32_t nnm_join_learned_max_list();
 void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32_t, uint32_t>>* out);
 uint32_t nnm_join_default_js(nnm_ctx* c, uint32_t prec);
 uint32_t nnm_join_default_chunk();
@@ -597,3 +621,91 @@ __device__ inline uint32_t nnm_wave_rnd_sds(uint64_t seed, uint64_t stream, uint
     uint64_t v = (uint64_t)(t + (int32_t)lane) + off;
     return (uint32_t)(v % M);
 }
+
+// ---------------------------------------------------------- offer log
+//
+// The local join hands every offer (target, id, dist) to the target's owner
+// through the offer log: each wave writes its offers in order into a log
+// chunk of NNM_OCH records that it reserved with one atomic on a shared
+// counter, so a wave store covers consecutive records and no offer ever
+// meets another in a hash slot.  Partition passes then group the records by
+// target and the owners merge them (nnm_offers.hip).  When the log has no
+// chunk left the wave records the vertices it could not finish in the redo
+// list; they are joined again in a later round (their earlier offers then
+// arrive twice, which the owners' de-dup by id absorbs).
+struct OfferSink {
+    uint32_t* rec;          // ochunks x NNM_OCH records of 3 words
+    uint32_t* chunk_used;   // records written per chunk
+    uint64_t* counters;     // CNT_OCHUNKS, CNT_REDO
+    uint32_t* redo;         // vertices to join again
+    uint32_t nchunks;
+};
+
+// Per-wave writer state (wave-uniform).  A writer without a chunk counts its
+// chunk as full, so its first offers reserve one.
+struct OfferWriter {
+    uint32_t chunk = 0xFFFFFFFFu;
+    uint32_t used = NNM_OCH;
+    bool full = false;      // the log ran out of chunks for this wave
+};
+
+// Append the offers of the lanes with p set: record {dist, id, target}.
+__device__ inline void offer_emit(const OfferSink& s, OfferWriter& w, bool p, uint32_t target, uint32_t id,
+                                  float dist) {
+    const uint64_t m = __ballot(p);
+    if (!m || w.full) return;
+    const uint32_t k = (uint32_t)__popcll(m);
+    if (w.used + k > NNM_OCH) {
+        const uint32_t lane = __lane_id();
+        if (w.chunk != 0xFFFFFFFFu && lane == 0) s.chunk_used[w.chunk] = w.used;
+        uint32_t ch = 0;
+        if (lane == 0) ch = (uint32_t)atomicAdd((unsigned long long*)&s.counters[CNT_OCHUNKS], 1ull);
+        ch = (uint32_t)__builtin_amdgcn_readfirstlane((int)ch);
+        if (ch >= s.nchunks) {
+            w.full = true;
+            w.chunk = 0xFFFFFFFFu;
+            w.used = NNM_OCH;
+            return;
+        }
+        w.chunk = ch;
+        w.used = 0;
+    }
+    if (p) {
+        const uint64_t below = __lane_id() ? (m & (~0ull >> (64 - __lane_id()))) : 0ull;
+        const uint64_t pos = (uint64_t)w.chunk * NNM_OCH + w.used + (uint32_t)__popcll(below);
+        uint32_t* r = s.rec + pos * 3;
+        r[0] = __float_as_uint(dist);
+        r[1] = id;
+        r[2] = target;
+    }
+    w.used += k;
+}
+
+// Close the wave's chunk (its record count becomes visible to the passes).
+__device__ inline void offer_close(const OfferSink& s, OfferWriter& w) {
+    if (w.chunk != 0xFFFFFFFFu && __lane_id() == 0) s.chunk_used[w.chunk] = w.used;
+    w.chunk = 0xFFFFFFFFu;
+    w.used = NNM_OCH;
+}
+
+// Record vertices whose offers did not all reach the log (lanes with p set).
+__device__ inline void offer_redo(const OfferSink& s, bool p, uint32_t v) {
+    const uint64_t m = __ballot(p);
+    if (!m) return;
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = (uint32_t)atomicAdd((unsigned long long*)&s.counters[CNT_REDO], (unsigned long long)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if (p) {
+        const uint64_t below = __lane_id() ? (m & (~0ull >> (64 - __lane_id()))) : 0ull;
+        s.redo[base + (uint32_t)__popcll(below)] = v;
+    }
+}
+
+// Offer log and delivery (nnm_offers.hip).
+int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p);
+void nnm_offer_free(nnm_ctx* c);
+OfferSink nnm_offer_sink(nnm_ctx* c, uint32_t round);
+// Before a join round: no chunk reserved, empty redo list.
+int nnm_offer_reset(nnm_ctx* c);
+// Group the round's offers by target and merge them into the lists.
+int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p);

@@ -23,11 +23,15 @@
 //
 // A pair that passes the rule is counted as one distance (nCmps) and offered
 // to both ends.  An offer (target t, id u, dist) whose distance is below t's
-// k-th distance at the start of the iteration goes into slot
-// hash(u) mod C of t by a 64-bit atomicMin of (dist bits << 32 | id): the
-// slot keeps the best offer that hashes there, whatever the order of the
-// atomics, so the result does not depend on scheduling.  k_update then merges
-// each vertex's slots into its list (per-vertex ownership, no list locks).
+// k-th distance at the start of the iteration is appended to the offer log
+// (offer_emit: the wave's offers of one step are consecutive records of its
+// log chunk).  No offer is dropped: the partition passes and owners of
+// nnm_offers.hip merge every one of them into its target's list
+// (UpdateKnnList, src/pvetex.h:40-85, for the batch of the iteration), so
+// the result does not depend on scheduling and the lists have one writer.
+//
+// Work source: the rows [0, n) in order, or in a redo round the vertices a
+// previous round could not finish (JoinArgs.work).
 #include <hip/hip_runtime.h>
 
 #include "nnm_internal.hpp"
@@ -46,12 +50,10 @@
 struct JoinArgs {
     const float* X;
     uint32_t ld;
-    uint64_t n;
+    uint64_t n;       // work items: rows [0, n), or work[0, n)
+    const uint32_t* work;  // redo round: the vertices to join (nullptr: every row)
     uint32_t K;
-    uint32_t C;
     uint32_t rule;
-    uint32_t salt;    // set inside each kernel from seed and the iteration counter
-    uint64_t seed;
     const uint4* vmeta;  // per vertex {label, k-th distance x offer window, row norm, 0}
     uint32_t js;      // short-list window rows (<= the kernel's compiled window)
     uint32_t chunk;   // rows per work-queue grab
@@ -64,9 +66,14 @@ struct JoinArgs {
     const uint32_t* label;
     const float* g_dist;
     const float* kth;
-    uint64_t* slots;
+    OfferSink sink;
     uint64_t* counters;
 };
+
+// Vertex of work item i.
+__device__ inline uint32_t work_vertex(const JoinArgs& a, uint64_t i) {
+    return a.work ? a.work[i] : (uint32_t)i;
+}
 
 struct JoinLds {
     float rows[ROW_LDS_BYTES / 4];
@@ -81,54 +88,59 @@ __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Slot salt of the iteration: derived on the device from the seed and the
-// iteration counter, so a captured iteration replays with fresh salts.
-__device__ inline uint32_t join_salt(const JoinArgs& a) {
-    return (uint32_t)nnm_rand(a.seed, 0xC0FFEEull, a.counters[CNT_ITER]);
-}
-
-// Offer (id, dd) to target's slot: one fire-and-forget 64-bit atomicMin (no
-// value returned, so the wave does not wait on it).  Returns the count to add
-// to a lane's offer tally (1 for the offer, 1 << 32 for its slot atomic).
-__device__ inline uint64_t offer(uint64_t* __restrict__ slots, uint32_t C, uint32_t salt, uint32_t target,
-                                 uint32_t id, float dd) {
-    const uint32_t s = nnm_hash32(id, salt) & (C - 1);
-    atomicMin(reinterpret_cast<unsigned long long*>(&slots[(uint64_t)target * C + s]),
-              (unsigned long long)nnm_key(dd, id));
-    return 1ull | (1ull << 32);
+// Adds a wave's tallies (lanes' distances and offers, lane 0's list entries)
+// to the counters; long = the long-list kernels, whose share is also kept apart.
+__device__ inline void join_tally(const JoinArgs& a, uint64_t dists, uint64_t entries, uint64_t offers, bool lng) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        dists += __shfl_xor(dists, o, NNM_WAVE);
+        entries += __shfl_xor(entries, o, NNM_WAVE);
+        offers += __shfl_xor(offers, o, NNM_WAVE);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)dists);
+        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)entries);
+        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)offers);
+        if (lng) {
+            atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)dists);
+            atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)entries);
+            atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)offers);
+        }
+    }
 }
 
 // Epilogue of one 2x4 micro-tile: pair rule, distance count, offers.  Under
 // the cosine metric the accumulator is the dot product, finished here with
-// the rows' squared norms (vmeta .z).
+// the rows' squared norms (vmeta .z).  Called by the whole wave (the offers
+// are appended by wave-wide steps); lanes with on = false take no pair.
 template <uint32_t M = NNM_METRIC_L2>
-__device__ inline void tile_epilogue(const JoinArgs& a, const float (&acc)[TM * TN], uint32_t rb, uint32_t cb,
-                                     uint32_t na, uint32_t nb, bool diag, const uint32_t (*mA)[JW],
-                                     const uint32_t (*mB)[JW], uint64_t& my_dists, uint64_t& my_offers) {
+__device__ inline void tile_epilogue(const JoinArgs& a, OfferWriter& w, bool on, const float (&acc)[TM * TN],
+                                     uint32_t rb, uint32_t cb, uint32_t na, uint32_t nb, bool diag,
+                                     const uint32_t (*mA)[JW], const uint32_t (*mB)[JW], uint64_t& my_dists,
+                                     uint64_t& my_offers) {
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
         const uint32_t ri = rb * TM + i;
-        if (ri >= na) continue;
         const uint32_t aid = mA[0][ri], alab = mA[1][ri];
         const float athr = __uint_as_float(mA[2][ri]);
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             const uint32_t cj = cb * TN + j;
-            if (cj >= nb) continue;
-            if (diag && cj <= ri) continue;
             const uint32_t bid = mB[0][cj], blab = mB[1][cj];
-            if (aid == bid) continue;
-            if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
+            bool ok = on && ri < na && cj < nb && !(diag && cj <= ri) && aid != bid;
+            ok = ok && !(a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW);
             float dd = acc[i * TN + j];
-            if constexpr (M == NNM_METRIC_COSINE)
-                dd = nnm_cos_finish(dd, __uint_as_float(a.vmeta[aid].z), __uint_as_float(a.vmeta[bid].z));
-            ++my_dists;
-            if (dd < __uint_as_float(mB[2][cj])) {
-                my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
+            if constexpr (M == NNM_METRIC_COSINE) {
+                const float sa = ok ? __uint_as_float(a.vmeta[aid].z) : 1.f;
+                const float sb = ok ? __uint_as_float(a.vmeta[bid].z) : 1.f;
+                dd = nnm_cos_finish(dd, sa, sb);
             }
-            if (dd < athr) {
-                my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
-            }
+            my_dists += ok ? 1u : 0u;
+            const bool pb = ok && dd < __uint_as_float(mB[2][cj]);
+            const bool pa = ok && dd < athr;
+            my_offers += (pb ? 1u : 0u) + (pa ? 1u : 0u);
+            offer_emit(a.sink, w, pb, bid, aid, dd);
+            offer_emit(a.sink, w, pa, aid, bid, dd);
         }
     }
 }
@@ -156,8 +168,10 @@ __device__ inline uint32_t build_tiles(uint16_t* tiles, uint32_t na, uint32_t nb
 
 // General path for one vertex: windows of 64, 32 dimensions per pass.
 template <uint32_t M = NNM_METRIC_L2>
-__device__ __attribute__((always_inline)) inline void join_vertex_windows(const JoinArgs& a, JoinLds& L, uint64_t vcur, uint32_t nn, uint32_t no,
-                                    uint32_t lane, uint64_t& my_dists, uint64_t& my_offers) {
+__device__ __attribute__((always_inline)) inline void join_vertex_windows(const JoinArgs& a, JoinLds& L, OfferWriter& w,
+                                                                         uint64_t vcur, uint32_t nn, uint32_t no,
+                                                                         uint32_t lane, uint64_t& my_dists,
+                                                                         uint64_t& my_offers) {
     float* rowsA = L.rows;
     float* rowsB = L.rows + JW * PITCH;
     const uint32_t m = nn + no;
@@ -166,6 +180,7 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
     for (uint32_t a0 = 0; a0 < nn; a0 += JW) {
         const uint32_t na = min(JW, nn - a0);
         for (uint32_t b0 = a0; b0 < m; b0 += JW) {
+            if (w.full) return;  // the vertex is joined again in a redo round
             const uint32_t nb = min(JW, m - b0);
             const bool diag = (b0 == a0);
             wave_lds_sync();  // earlier readers of meta/tiles are done
@@ -251,10 +266,10 @@ __device__ __attribute__((always_inline)) inline void join_vertex_windows(const 
             }
 #pragma unroll
             for (int s = 0; s < MAXS; ++s) {
-                if ((uint32_t)s < my_n) {
-                    const uint32_t t = L.tiles[lane + NNM_WAVE * s];
-                    tile_epilogue<M>(a, acc[s], t / CBn, t % CBn, na, nb, diag, mA, L.metaB, my_dists, my_offers);
-                }
+                const bool on = (uint32_t)s < my_n;
+                const uint32_t t = on ? L.tiles[lane + NNM_WAVE * s] : 0u;
+                tile_epilogue<M>(a, w, on, acc[s], t / CBn, t % CBn, na, nb, diag, mA, L.metaB, my_dists,
+                                 my_offers);
             }
         }
     }
@@ -278,7 +293,7 @@ struct FastLds {
 // Joins the list heads of `nb_v` vertices packed in the window: window rows
 // [off[b], off[b] + m[b]) hold vertex b's list, new entries first.
 template <int CPR, uint32_t JS>
-__device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, uint32_t lane, uint32_t nbv,
+__device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, OfferWriter& w, uint32_t lane, uint32_t nbv,
                                    const uint32_t (&off)[FB], const uint32_t (&nnv)[FB], const uint32_t (&mv)[FB],
                                    uint64_t& my_dists, uint64_t& my_offers) {
     constexpr uint32_t PF = 4 * CPR + 4;  // LDS pitch in floats
@@ -307,6 +322,7 @@ __device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, uint32_t l
     ntile = min(ntile, (uint32_t)(sizeof(L.tiles) / sizeof(L.tiles[0])));
     wave_lds_sync();
     for (uint32_t t0 = 0; t0 < ntile; t0 += NNM_WAVE * RND) {
+        if (w.full) return;  // the batch is joined again in a redo round
         float acc[RND][TM * TN];
         uint32_t code[RND];
         bool on[RND];
@@ -348,9 +364,8 @@ __device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, uint32_t l
         }
 #pragma unroll
         for (int s = 0; s < RND; ++s) {
-            if (!on[s]) continue;
             const uint32_t b = code[s] >> 12, rb = (code[s] >> 6) & 63u, cb = code[s] & 63u;
-            tile_epilogue(a, acc[s], rb, cb, nnv[b], mv[b], true,
+            tile_epilogue(a, w, on[s], acc[s], rb, cb, nnv[b], mv[b], true,
                           reinterpret_cast<const uint32_t(*)[JW]>(&L.meta[0][off[b]]),
                           reinterpret_cast<const uint32_t(*)[JW]>(&L.meta[0][off[b]]), my_dists, my_offers);
         }
@@ -359,7 +374,6 @@ __device__ inline void join_packed(const JoinArgs& a, FastLds<JS>& L, uint32_t l
 
 template <int CPR>  // 16-byte chunks per row: ld = 4 * CPR
 __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
-    a.salt = join_salt(a);
     constexpr uint32_t LD = 4 * CPR;
     constexpr uint32_t PF = LD + 4;
     constexpr uint32_t JS_ROWS = ROW_LDS_BYTES / (PF * 4);
@@ -370,6 +384,7 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
 
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
+    OfferWriter w;
 
     for (;;) {
         uint64_t base = 0;
@@ -377,16 +392,17 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
         const uint32_t cnt = (uint32_t)min((uint64_t)a.chunk, a.n - base);
-        // list sizes of the chunk: lane i < cnt holds vertex base + i
-        uint32_t cn = 0, co = 0;
+        // list sizes of the chunk: lane i < cnt holds work item base + i
+        uint32_t vtx = 0, cn = 0, co = 0;
         if (lane < cnt) {
-            cn = a.n_new[base + lane];
-            co = a.n_old[base + lane];
+            vtx = work_vertex(a, base + lane);
+            cn = a.n_new[vtx];
+            co = a.n_old[vtx];
         }
         uint32_t i = 0;
         while (i < cnt) {
             // greedy packing of consecutive short lists
-            uint32_t nbv = 0, rows = 0;
+            uint32_t nbv = 0, rows = 0, be = 0;
             uint32_t off[FB], nnv[FB], mv[FB], vid[FB];
 #pragma unroll
             for (int b = 0; b < FB; ++b) { off[b] = 0; nnv[b] = 0; mv[b] = 0; vid[b] = 0; }
@@ -405,9 +421,20 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
                 rows += m;
                 ++nbv;
                 ++i;
-                if (lane == 0) my_entries += m;
+                be += m;
             }
             if (nbv == 0) continue;
+            // vertex of packed list b in lane b (for the redo list)
+            uint32_t bvtx = 0;
+#pragma unroll
+            for (int q = 0; q < FB; ++q) {
+                const uint32_t t = (uint32_t)__shfl((int)vtx, (int)vid[q], NNM_WAVE);
+                if (lane == (uint32_t)q) bvtx = t;
+            }
+            if (w.full) {  // the log is out of chunks: join the batch in a redo round
+                offer_redo(a.sink, lane < nbv, bvtx);
+                continue;
+            }
             // window ids: lane j < rows holds window row j
             wave_lds_sync();  // previous batch's readers are done with LDS
             uint32_t wid = 0;
@@ -420,8 +447,8 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
 #pragma unroll
                 for (int q = 0; q < FB; ++q)
                     if ((uint32_t)q == b) { ob = off[q]; nb_ = nnv[q]; vb = vid[q]; }
+                const uint64_t v = (uint32_t)__shfl((int)vtx, (int)vb, NNM_WAVE);
                 if (lane < rows) {
-                    const uint64_t v = base + vb;
                     const uint32_t e = lane - ob;
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
                 }
@@ -445,21 +472,19 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
             }
             __builtin_amdgcn_s_waitcnt(0);
             wave_lds_sync();
-            join_packed<CPR, JS>(a, L, lane, nbv, off, nnv, mv, my_dists, my_offers);
+            uint64_t bd = 0, bo = 0;
+            join_packed<CPR, JS>(a, L, w, lane, nbv, off, nnv, mv, bd, bo);
+            if (w.full) {
+                offer_redo(a.sink, lane < nbv, bvtx);
+            } else {
+                my_dists += bd;
+                my_offers += bo;
+                if (lane == 0) my_entries += be;
+            }
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        my_dists += __shfl_xor(my_dists, o, NNM_WAVE);
-        my_entries += __shfl_xor(my_entries, o, NNM_WAVE);
-        my_offers += __shfl_xor(my_offers, o, NNM_WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
-        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
-    }
+    offer_close(a.sink, w);
+    join_tally(a, my_dists, my_entries, my_offers, false);
 }
 
 // ---------------------------------------------------------------- BF16 MFMA join
@@ -533,7 +558,6 @@ struct Bf16Lds {
 template <int U, bool F8 = false>
 __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb,
                                                   const float* __restrict__ norms, int mode, float kscale, float tau) {
-    a.salt = join_salt(a);
     static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;  // dwords per row
@@ -542,6 +566,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
     const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
+    OfferWriter w;
 
     for (;;) {
         uint64_t base = 0;
@@ -549,14 +574,15 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
         const uint32_t cnt = (uint32_t)min((uint64_t)a.chunk, a.n - base);
-        uint32_t cn = 0, co = 0;
+        uint32_t vtx = 0, cn = 0, co = 0;
         if (lane < cnt) {
-            cn = a.n_new[base + lane];
-            co = a.n_old[base + lane];
+            vtx = work_vertex(a, base + lane);
+            cn = a.n_new[vtx];
+            co = a.n_old[vtx];
         }
         uint32_t i = 0;
         while (i < cnt) {
-            uint32_t nbv = 0, rows = 0;
+            uint32_t nbv = 0, rows = 0, be = 0;
             uint32_t off[BFB], nnv[BFB], mv[BFB], vid[BFB];
 #pragma unroll
             for (int b = 0; b < BFB; ++b) { off[b] = 0; nnv[b] = 0; mv[b] = 0; vid[b] = 0; }
@@ -575,9 +601,20 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                 rows += m;
                 ++nbv;
                 ++i;
-                if (lane == 0) my_entries += m;
+                be += m;
             }
             if (nbv == 0) continue;
+            // vertex of packed list b in lane b (for the redo list)
+            uint32_t bvtx = 0;
+#pragma unroll
+            for (int q = 0; q < BFB; ++q) {
+                const uint32_t t = (uint32_t)__shfl((int)vtx, (int)vid[q], NNM_WAVE);
+                if (lane == (uint32_t)q) bvtx = t;
+            }
+            if (w.full) {  // the log is out of chunks: join the batch in a redo round
+                offer_redo(a.sink, lane < nbv, bvtx);
+                continue;
+            }
             wave_lds_sync();
             uint32_t wid = 0;
             {
@@ -585,8 +622,8 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
 #pragma unroll
                 for (int q = 0; q < BFB; ++q)
                     if ((uint32_t)q < nbv && lane >= off[q]) { ob = off[q]; nb_ = nnv[q]; vb = vid[q]; }
+                const uint64_t v = (uint32_t)__shfl((int)vtx, (int)vb, NNM_WAVE);
                 if (lane < rows) {
-                    const uint64_t v = base + vb;
                     const uint32_t e = lane - ob;
                     wid = e < nb_ ? a.l_new[v * a.cap_new + e] : a.l_old[v * a.cap_old + (e - nb_)];
                 }
@@ -629,7 +666,8 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
             __builtin_amdgcn_s_waitcnt(0);
             wave_lds_sync();
             const uint32_t lr = lane & 15u, lg = lane >> 4;
-            for (uint32_t b = 0; b < nbv; ++b) {
+            uint64_t bd = 0, bo = 0;
+            for (uint32_t b = 0; b < nbv && !w.full; ++b) {
                 uint32_t ob = 0, nb_ = 0, mb = 0;
 #pragma unroll
                 for (int q = 0; q < BFB; ++q)
@@ -664,38 +702,34 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const uint32_t ri = rb * 16 + 4 * lg + q;
-                            if (!colok || ri >= nb_ || cj <= ri) continue;
-                            const uint4 ar = L.rec[ob + ri];
-                            const uint32_t aid = L.id[ob + ri], alab = ar.x;
-                            if (aid == bid) continue;
-                            if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
+                            const uint32_t ia = min(ob + ri, (uint32_t)BJS - 1);
+                            const uint4 ar = L.rec[ia];
+                            const uint32_t aid = L.id[ia], alab = ar.x;
+                            bool ok = colok && ri < nb_ && cj > ri && aid != bid;
+                            ok = ok && !(a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW);
                             const float an = __uint_as_float(ar.z);
                             const float dd = offer_key(an, bn, acc[q], mode, kscale);
-                            ++my_dists;
-                            if (dd < tau * bthr) {
-                                my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
-                            }
-                            if (dd < tau * __uint_as_float(ar.y)) {
-                                my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
-                            }
+                            bd += ok ? 1u : 0u;
+                            const bool pb = ok && dd < tau * bthr;
+                            const bool pa = ok && dd < tau * __uint_as_float(ar.y);
+                            bo += (pb ? 1u : 0u) + (pa ? 1u : 0u);
+                            offer_emit(a.sink, w, pb, bid, aid, dd);
+                            offer_emit(a.sink, w, pa, aid, bid, dd);
                         }
                     }
                 }
             }
+            if (w.full) {
+                offer_redo(a.sink, lane < nbv, bvtx);
+            } else {
+                my_dists += bd;
+                my_offers += bo;
+                if (lane == 0) my_entries += be;
+            }
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        my_dists += __shfl_xor(my_dists, o, NNM_WAVE);
-        my_entries += __shfl_xor(my_entries, o, NNM_WAVE);
-        my_offers += __shfl_xor(my_offers, o, NNM_WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
-        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
-    }
+    offer_close(a.sink, w);
+    join_tally(a, my_dists, my_entries, my_offers, false);
 }
 
 // Long lists (BJS < m <= BLM) on BF16: one workgroup of 4 waves per vertex.
@@ -714,38 +748,48 @@ struct Bf16LongLds {
 template <int U, bool F8 = false>
 __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb,
                                                         const float* __restrict__ norms, int mode, float kscale, float tau) {
-    a.salt = join_salt(a);
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;
     __shared__ __align__(16) Bf16LongLds<U> L;
     __shared__ uint64_t vshare;
+    __shared__ uint32_t wfull;  // a wave of the block ran out of log chunks
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
     const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
+    OfferWriter w;
     uint64_t cur = 0, end = 0;
+    if (tid == 0) wfull = 0;
     for (;;) {
-        // block-wide queue over vertices: thread 0 takes chunks of rows
+        // block-wide queue over work items: thread 0 takes chunks of them
         if (tid == 0) {
-            uint64_t v = cur;
+            uint64_t i = cur, vx = ~0ull;
             for (;;) {
-                if (v >= end) {
-                    v = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(a.chunk * 16));
-                    end = min(a.n, v + a.chunk * 16);
-                    if (v >= a.n) break;
+                if (i >= end) {
+                    i = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(a.chunk * 16));
+                    end = min(a.n, i + a.chunk * 16);
+                    if (i >= a.n) break;
                 }
-                const uint32_t m = a.n_new[v] + a.n_old[v];
-                if (m > min(a.js, (uint32_t)BJS) && m <= BLM) break;
-                ++v;
+                const uint32_t vi = work_vertex(a, i);
+                const uint32_t m = a.n_new[vi] + a.n_old[vi];
+                if (m > min(a.js, (uint32_t)BJS) && m <= BLM) {
+                    vx = vi;
+                    break;
+                }
+                ++i;
             }
-            vshare = v;
-            cur = v + 1;
+            vshare = vx;
+            cur = i + 1;
         }
         __syncthreads();
         const uint64_t v = vshare;
+        const bool skip = wfull != 0;
         __syncthreads();
-        if (v >= a.n) break;
+        if (v == ~0ull) break;
+        if (skip) {  // the log is out of chunks: join the vertex in a redo round
+            if (wave == 0) offer_redo(a.sink, lane == 0, (uint32_t)v);
+            continue;
+        }
         const uint32_t nn = a.n_new[v], no = a.n_old[v], m = nn + no;
-        if (tid == 0) my_entries += m;
         for (uint32_t j = tid; j < m; j += blockDim.x) {
             const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
             const uint4 mt = a.vmeta[id];
@@ -770,9 +814,10 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
         const uint32_t lr = lane & 15u, lg = lane >> 4;
         const uint32_t RB = (nn + 15) / 16, CB = (m + 15) / 16;
         uint32_t blk = 0;
+        uint64_t bd = 0, bo = 0;
         for (uint32_t rb = 0; rb < RB; ++rb) {
             for (uint32_t cb = rb; cb < CB; ++cb, ++blk) {
-                if ((blk & 3u) != wave) continue;
+                if ((blk & 3u) != wave || w.full) continue;
                 const uint32_t ra = rb * 16 + lr, rc = cb * 16 + lr;
                 nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -795,48 +840,41 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const uint32_t ri = rb * 16 + 4 * lg + q;
-                    if (!colok || ri >= nn || cj <= ri) continue;
                     const uint32_t aid = L.meta[0][ri], alab = L.meta[1][ri];
-                    if (aid == bid) continue;
-                    if (a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW) continue;
+                    bool ok = colok && ri < nn && cj > ri && aid != bid;
+                    ok = ok && !(a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW);
                     const float an = __uint_as_float(L.meta[3][ri]);
                     const float dd = offer_key(an, bn, acc[q], mode, kscale);
-                    ++my_dists;
-                    if (dd < tau * bthr) {
-                        my_offers += offer(a.slots, a.C, a.salt, bid, aid, dd);
-                    }
-                    if (dd < tau * __uint_as_float(L.meta[2][ri])) {
-                        my_offers += offer(a.slots, a.C, a.salt, aid, bid, dd);
-                    }
+                    bd += ok ? 1u : 0u;
+                    const bool pb = ok && dd < tau * bthr;
+                    const bool pa = ok && dd < tau * __uint_as_float(L.meta[2][ri]);
+                    bo += (pb ? 1u : 0u) + (pa ? 1u : 0u);
+                    offer_emit(a.sink, w, pb, bid, aid, dd);
+                    offer_emit(a.sink, w, pa, aid, bid, dd);
                 }
             }
         }
-        __syncthreads();  // LDS reused by the next vertex
+        if (w.full && lane == 0) wfull = 1;
+        __syncthreads();  // LDS reused by the next vertex; every wave's offers issued
+        if (wfull) {
+            if (wave == 0) offer_redo(a.sink, lane == 0, (uint32_t)v);
+        } else {
+            my_dists += bd;
+            my_offers += bo;
+            if (tid == 0) my_entries += m;
+        }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        my_dists += __shfl_xor(my_dists, o, NNM_WAVE);
-        my_entries += __shfl_xor(my_entries, o, NNM_WAVE);
-        my_offers += __shfl_xor(my_offers, o, NNM_WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
-        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)my_dists);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
-    }
+    offer_close(a.sink, w);
+    join_tally(a, my_dists, my_entries, my_offers, true);
 }
 
 // Lists longer than min_m entries (every list when min_m = 0): windows of 64.
 template <uint32_t M>
 __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t min_m) {
-    a.salt = join_salt(a);
     __shared__ __align__(16) JoinLds L;
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
+    OfferWriter w;
     for (;;) {
         uint64_t base = 0;
         if (lane == 0)
@@ -844,28 +882,31 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
         const uint64_t end = min(a.n, base + JOIN_CHUNK * 16);
-        for (uint64_t v = base; v < end; ++v) {
+        for (uint64_t i = base; i < end; ++i) {
+            const uint32_t v = work_vertex(a, i);
             const uint32_t nn = a.n_new[v], no = a.n_old[v];
             if (nn + no <= min_m) continue;
-            if (lane == 0) my_entries += nn + no;
-            if (nn) join_vertex_windows<M>(a, L, v, nn, no, lane, my_dists, my_offers);
+            if (!nn) {
+                if (lane == 0) my_entries += no;
+                continue;
+            }
+            if (w.full) {  // the log is out of chunks: join the vertex in a redo round
+                offer_redo(a.sink, lane == 0, v);
+                continue;
+            }
+            uint64_t bd = 0, bo = 0;
+            join_vertex_windows<M>(a, L, w, v, nn, no, lane, bd, bo);
+            if (w.full) {
+                offer_redo(a.sink, lane == 0, v);
+            } else {
+                my_dists += bd;
+                my_offers += bo;
+                if (lane == 0) my_entries += nn + no;
+            }
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        my_dists += __shfl_xor(my_dists, o, NNM_WAVE);
-        my_entries += __shfl_xor(my_entries, o, NNM_WAVE);
-        my_offers += __shfl_xor(my_offers, o, NNM_WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd((unsigned long long*)&a.counters[CNT_DISTS], (unsigned long long)my_dists);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LIST_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
-        atomicAdd((unsigned long long*)&a.counters[CNT_ATOMICS], (unsigned long long)(my_offers >> 32));
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_DISTS], (unsigned long long)my_dists);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_ENTRIES], (unsigned long long)my_entries);
-        atomicAdd((unsigned long long*)&a.counters[CNT_LONG_OFFERS], (unsigned long long)(my_offers & 0xFFFFFFFFull));
-    }
+    offer_close(a.sink, w);
+    join_tally(a, my_dists, my_entries, my_offers, true);
 }
 
 static uint32_t fast_window(uint32_t ld) {
@@ -901,16 +942,18 @@ void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32
             if (js >= 8) out->emplace_back(js, ch);
 }
 
-int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t chunk, hipEvent_t mid) {
+uint32_t nnm_join_learned_max_list() { return BLM; }
+
+int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t chunk, hipEvent_t mid, uint32_t round,
+                    uint64_t nwork) {
     JoinArgs a;
     a.X = c->X;
     a.ld = c->ld;
-    a.n = c->n_act;
+    // round 0 joins every active row; a redo round the previous round's redo list
+    a.n = round ? nwork : c->n_act;
+    a.work = round ? c->work[(round - 1) & 1] : nullptr;
     a.K = c->K;
-    a.C = c->slots_C;
     a.rule = p->rule;
-    a.salt = 0;
-    a.seed = p->seed;
     a.chunk = chunk ? chunk : JOIN_CHUNK;
     a.vmeta = c->vmeta;
     a.l_new = c->l_new;
@@ -922,7 +965,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
     a.label = c->label;
     a.g_dist = c->g_dist;
     a.kth = c->kth;
-    a.slots = c->slots;
+    a.sink = nnm_offer_sink(c, round);
     a.counters = c->counters;
     if (p->prec == NNM_PREC_BF16 || p->prec == NNM_PREC_LEARNED) {
         const uint16_t* rows = c->Xb;
@@ -955,13 +998,9 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
                 hipLaunchKernelGGL((k_join_bf16_long<4, true>), dim3(join_grid256(c, k_join_bf16_long<4, true>)),
                                    dim3(256), 0, c->stream, a, rows8, nrm8, mode, ks8, tau);
                 NNM_LAUNCH_CHECK(c);
-                // lists beyond the long-list LDS window, as on the bf16 path
-                if (c->cap_new + c->cap_old > BLM) {
-                    hipLaunchKernelGGL(k_join_f32_windows<NNM_METRIC_L2>,
-                                       dim3(join_grid(c, k_join_f32_windows<NNM_METRIC_L2>)), dim3(64), 0, c->stream,
-                                       a, (uint32_t)BLM);
-                    NNM_LAUNCH_CHECK(c);
-                }
+                // lists longer than BLM are refused before the learned phase
+                // (their FP32 join would offer exact distances against
+                // estimated k-th thresholds)
                 return NNM_OK;
             }
         }
@@ -986,8 +1025,10 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
         }
 #undef NNM_BF
         NNM_LAUNCH_CHECK(c);
-        // lists beyond the long-list LDS window (rev_cap above 118) stay on FP32
-        if (c->cap_new + c->cap_old > BLM) {
+        // lists beyond the long-list LDS window (rev_cap above 118) stay on
+        // FP32 (plain BF16 joins only: a learned run with such lists is
+        // refused before it starts)
+        if (c->cap_new + c->cap_old > BLM && p->prec != NNM_PREC_LEARNED) {
             hipLaunchKernelGGL(k_join_f32_windows<NNM_METRIC_L2>, dim3(join_grid(c, k_join_f32_windows<NNM_METRIC_L2>)),
                                dim3(64), 0, c->stream,
                                a, (uint32_t)BLM);

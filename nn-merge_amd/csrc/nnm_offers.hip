@@ -1,0 +1,616 @@
+// Offer delivery: every offer of the local join reaches its target's list.
+//
+// The reference inserts each accepted pair into both lists at once
+// (NNList::parallel_try_insert -> UpdateKnnList, src/pvetex.h:40-85, 99-109);
+// no candidate is lost.  Here one iteration's offers form a batch:
+//
+//   join      each wave appends its offers {dist, id, target} to log chunks
+//             it reserved (nnm_internal.hpp, offer_emit)
+//   passes    one or two partition passes group the records by target:
+//             a pass counts the records of each bin (per 4096-record tile an
+//             LDS histogram, one atomic per nonempty bin), scans the counts,
+//             then scatters each tile's records to their bins (LDS ranks, one
+//             cursor atomic per nonempty bin of the tile); the last pass's
+//             bins are blocks of 64 targets
+//   owners    one workgroup per 64-target bin sorts the bin's records by
+//             target in LDS and one wave per target merges the target's
+//             candidates into its list: de-dup by id (candidates and list),
+//             candidates below the k-th key enter at their rank by (dist,
+//             id) -- UpdateKnnList's rule for the whole batch.  Only the
+//             owner writes a list.
+//
+// The merged list of a target is the K smallest distinct (dist, id) keys of
+// its list and every offer, whatever the order in which the offers were
+// written, partitioned or split into chunks.  A vertex whose offers did not
+// fit the log is joined again in a redo round (its earlier offers then
+// arrive twice, and the de-dup by id absorbs them).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "nnm_internal.hpp"
+
+#define PT_THREADS 256u
+#define PT_PER 16u                         // records per thread in a tile
+#define PT_TILE (PT_THREADS * PT_PER)      // records per tile
+#define PT_BINS 2048u                      // LDS bins of a tile (relative to its smallest digit)
+#define OWN_T (1u << NNM_OWN_BITS)         // targets per owner bin
+#define OWN_CH 4096u                       // records an owner sorts in LDS at a time
+#define NO_DIGIT 0xFFFFFFFFu
+
+static_assert(PT_TILE == NNM_OCH, "first-pass tiles are log chunks");
+static_assert(OWN_T == 64, "owner bins of 64 targets (one LDS counter per lane)");
+
+// ------------------------------------------------------------------ passes
+
+struct PassIn {
+    const uint32_t* rec;         // records, 3 words each
+    const uint32_t* chunk_used;  // first pass: records per log chunk (tile t = chunk t)
+    const uint64_t* counters;    // first pass: CNT_OCHUNKS
+    uint32_t nchunks;            // log capacity in chunks (= tiles of any pass)
+    const uint32_t* total;       // later passes: record count of the input
+    uint32_t nbins;              // bins of the pass (records of other digits are not moved)
+};
+
+// Records [*b, *b + *m) of tile t (m = 0: no tile).
+__device__ inline void tile_range(const PassIn& in, uint64_t t, uint64_t* b, uint32_t* m) {
+    *b = t * PT_TILE;
+    *m = 0;
+    if (in.chunk_used) {
+        const uint64_t used = min(in.counters[CNT_OCHUNKS], (uint64_t)in.nchunks);
+        if (t < used) *m = in.chunk_used[t];
+    } else {
+        const uint64_t tot = *in.total;
+        if (*b < tot) *m = (uint32_t)min((uint64_t)PT_TILE, tot - *b);
+    }
+}
+
+// Smallest digit of the tile into *dmin_s (LDS; preset to NO_DIGIT).
+__device__ inline uint32_t block_min_digit(const uint32_t (&d)[PT_PER], uint32_t* dmin_s) {
+    uint32_t lm = NO_DIGIT;
+#pragma unroll
+    for (uint32_t k = 0; k < PT_PER; ++k) lm = min(lm, d[k]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lm = min(lm, (uint32_t)__shfl_xor((int)lm, o, NNM_WAVE));
+    if (__lane_id() == 0) atomicMin(dmin_s, lm);
+    __syncthreads();
+    return *dmin_s;
+}
+
+// Bin counts of one pass: cnt[target >> shift] += records.
+__global__ __launch_bounds__(PT_THREADS) void k_part_count(PassIn in, uint32_t shift, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t hist[PT_BINS];
+    __shared__ uint32_t dmin_s;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t t = blockIdx.x; t < in.nchunks; t += gridDim.x) {
+        uint64_t b;
+        uint32_t m;
+        tile_range(in, t, &b, &m);
+        if (m == 0) continue;  // uniform over the block
+        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
+        if (tid == 0) dmin_s = NO_DIGIT;
+        __syncthreads();
+        uint32_t d[PT_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            const uint32_t j = tid + k * PT_THREADS;
+            d[k] = j < m ? in.rec[(b + j) * 3 + 2] >> shift : NO_DIGIT;
+            if (d[k] >= in.nbins) d[k] = NO_DIGIT;
+        }
+        const uint32_t dmin = block_min_digit(d, &dmin_s);
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            if (d[k] == NO_DIGIT) continue;
+            const uint32_t r = d[k] - dmin;
+            if (r < PT_BINS) atomicAdd(&hist[r], 1u);
+            else atomicAdd(&cnt[d[k]], 1u);  // digits far above the tile's smallest
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS)
+            if (hist[i]) atomicAdd(&cnt[dmin + i], hist[i]);
+        __syncthreads();  // hist and dmin_s are reset for the next tile
+    }
+}
+
+// Scatter of one pass: every record to out[cur[digit]++] (cursors start at
+// the scanned bin offsets).
+__global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassIn in, uint32_t shift, uint32_t* __restrict__ cur,
+                                                             uint32_t* __restrict__ out) {
+    __shared__ uint32_t hist[PT_BINS];
+    __shared__ uint32_t basev[PT_BINS];
+    __shared__ uint32_t dmin_s;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t t = blockIdx.x; t < in.nchunks; t += gridDim.x) {
+        uint64_t b;
+        uint32_t m;
+        tile_range(in, t, &b, &m);
+        if (m == 0) continue;
+        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
+        if (tid == 0) dmin_s = NO_DIGIT;
+        __syncthreads();
+        uint32_t r0[PT_PER], r1[PT_PER], r2[PT_PER], d[PT_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            const uint32_t j = tid + k * PT_THREADS;
+            r0[k] = r1[k] = r2[k] = 0;
+            d[k] = NO_DIGIT;
+            if (j < m) {
+                const uint32_t* r = in.rec + (b + j) * 3;
+                r0[k] = r[0];
+                r1[k] = r[1];
+                r2[k] = r[2];
+                d[k] = r2[k] >> shift;
+                if (d[k] >= in.nbins) d[k] = NO_DIGIT;
+            }
+        }
+        const uint32_t dmin = block_min_digit(d, &dmin_s);
+        uint32_t pos[PT_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            pos[k] = 0;
+            if (d[k] == NO_DIGIT) continue;
+            const uint32_t r = d[k] - dmin;
+            pos[k] = r < PT_BINS ? atomicAdd(&hist[r], 1u) : atomicAdd(&cur[d[k]], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS)
+            if (hist[i]) basev[i] = atomicAdd(&cur[dmin + i], hist[i]);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            if (d[k] == NO_DIGIT) continue;
+            const uint32_t r = d[k] - dmin;
+            const uint64_t p = r < PT_BINS ? (uint64_t)basev[r] + pos[k] : (uint64_t)pos[k];
+            uint32_t* o = out + p * 3;
+            o[0] = r0[k];
+            o[1] = r1[k];
+            o[2] = r2[k];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ owners
+
+struct OwnArgs {
+    uint64_t n;                 // active rows
+    uint32_t K;
+    const float* X;
+    const uint16_t* Xb_exact;   // bf16 rows where they are exact (same bits as X)
+    uint32_t ld;
+    float rerank_tau;
+    float tie_win;
+    uint32_t* g_ids;
+    float* g_dist;
+    uint8_t* g_flag;
+    const uint32_t* rec;        // records grouped by owner bin
+    const uint32_t* bin_end;    // bin b holds records [bin_end[b - 1], bin_end[b])
+    uint32_t nbins;
+    uint64_t* counters;
+};
+
+// FP64 squared L2 across the wave: lanes take elements lane, lane + 64, ...;
+// the partial sums are combined in a fixed tree (same result every call).
+__device__ inline double own_l2_f64(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
+    double acc = 0.0;
+    for (uint32_t i = nnm_lane(); i < d; i += NNM_WAVE) {
+        const double t = (double)a[i] - (double)b[i];
+        acc = __builtin_fma(t, t, acc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, NNM_WAVE);
+    return acc;
+}
+
+struct OwnTally {
+    uint64_t upd = 0, flags = 0, rr = 0, ties = 0, swaps = 0;  // wave-uniform
+};
+
+// Candidate count above which the ranks come from a sort of the candidates
+// instead of a loop over them.
+#define OWN_SORT_MIN 32u
+
+// Merge `len` candidate keys (keys[0..len), any order, repeats allowed) into
+// the list of row v: UpdateKnnList (src/pvetex.h:40-85) for the batch, 64
+// candidates at a time.  A candidate enters when its id is not listed and
+// its (dist, id) key ranks among the K smallest; with RR its lower-bound key
+// is first recomputed exactly; with TIE the entries ranked K-1 and K are
+// re-decided on FP64 distances when their FP32 ones are within rounding.
+// `perm` is 64 words of LDS owned by the wave.  A target whose candidates
+// span several chunks of its bin (or several redo rounds) is merged once per
+// chunk; entries that entered in an earlier merge carry flag 2 until the
+// last chunk of the iteration's last round (last = true), which counts the
+// entries that entered (the iteration's updates) and the new flags, and
+// stores every flag as 0 or 1.
+template <bool RR, bool TIE>
+__device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* keys, uint32_t len, bool last,
+                                 uint32_t* perm, OwnTally& t) {
+    const uint32_t lane = nnm_lane();
+    const uint32_t K = a.K;
+    const uint64_t lmask = K < NNM_WAVE ? (1ull << K) - 1 : ~0ull;  // list lanes
+    uint32_t pid = 0xFFFFFFFFu, pf = 0;
+    float pd = 0.f;
+    uint32_t fresh = 0;  // the entry entered in this iteration's delivery
+    if (lane < K) {
+        pid = a.g_ids[v * K + lane];
+        pd = a.g_dist[v * K + lane];
+        pf = a.g_flag[v * K + lane];
+        fresh = pf == 2 ? 1u : 0u;
+        pf = pf ? 1u : 0u;
+    }
+    bool changed = last && __ballot(fresh) != 0;  // flags 2 are stored as 1 on the last chunk
+    for (uint32_t p0 = 0; p0 < len; p0 += NNM_WAVE) {
+        uint64_t ck = p0 + lane < len ? keys[p0 + lane] : NNM_EMPTY_KEY;
+        const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
+        const uint64_t kth_key = wave_read64(pkey, K - 1);
+        const float kd = nnm_key_dist(kth_key);
+        // cheap filter first: lower bounds against the margin, exact keys
+        // against the k-th key (or its rounding window)
+        bool valid;
+        if constexpr (RR) valid = ck != NNM_EMPTY_KEY && nnm_key_dist(ck) <= kd * a.rerank_tau;
+        else if constexpr (TIE) valid = ck != NNM_EMPTY_KEY && (ck < kth_key || nnm_key_dist(ck) <= kd * a.tie_win);
+        else valid = ck < kth_key;  // the empty key is above every key
+        if (!__ballot(valid)) continue;
+        // de-dup by id: sort by (id, dist) and keep the first of each id (its
+        // smallest key); the valid candidates land in lanes 0..nv-1
+        {
+            uint64_t sk = valid ? (((uint64_t)nnm_key_id(ck) << 32) | (ck >> 32)) : NNM_EMPTY_KEY;
+            sk = nnm_wave_sort64(sk);
+            const uint64_t prev = shfl64(sk, lane ? lane - 1 : 0);
+            valid = sk != NNM_EMPTY_KEY && (lane == 0 || (prev >> 32) != (sk >> 32));
+            ck = valid ? ((sk << 32) | (sk >> 32)) : NNM_EMPTY_KEY;
+        }
+        const uint32_t cid = nnm_key_id(ck);
+        uint64_t vb = __ballot(valid);
+        // de-dup against the list: a loop over the fewer of candidates and
+        // list entries
+        if ((uint32_t)__popcll(vb) <= K) {
+            for (uint64_t b = vb; b; b &= b - 1) {
+                const uint32_t cl = (uint32_t)__builtin_ctzll(b);
+                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)cid, (int)cl);
+                if (__ballot(pid == q) & lmask) vb &= ~(1ull << cl);
+            }
+            valid = (vb >> lane) & 1ull;
+        } else {
+            for (uint32_t p = 0; p < K; ++p)
+                valid = valid && (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p) != cid;
+            vb = __ballot(valid);
+        }
+        if constexpr (RR) {
+            // exact FP32 distances of the survivors, then the exact filter
+            if (valid) {
+                const float e = a.Xb_exact
+                                    ? nnm_l2_rows_bf16<1>(a.Xb_exact + v * a.ld, a.Xb_exact + (uint64_t)cid * a.ld, a.ld)
+                                    : nnm_l2_rows<1>(a.X + v * a.ld, a.X + (uint64_t)cid * a.ld, a.ld);
+                ck = nnm_key(e, cid);
+            }
+            t.rr += (uint32_t)__popcll(vb);
+            valid = valid && (ck < kth_key || (TIE && nnm_key_dist(ck) <= kd * a.tie_win));
+            vb = __ballot(valid);
+        }
+        if (!vb) continue;
+        // ranks: a candidate's position is (list keys below it) + (candidate
+        // keys below it); a list entry's is its index + (candidate keys below
+        // it).  Keys are distinct (distinct ids).
+        uint32_t below_list = 0, below_cand_c = 0, below_cand_p = 0;
+        const uint32_t nv = (uint32_t)__popcll(vb);
+        if (nv > OWN_SORT_MIN) {
+            ck = nnm_wave_sort64(valid ? ck : NNM_EMPTY_KEY);
+            valid = lane < nv;
+            below_cand_c = lane;
+            for (uint32_t p = 0; p < K; ++p) {
+                const uint64_t pk = wave_read64(pkey, p);
+                below_list += pk < ck ? 1u : 0u;
+                const uint32_t cnt = (uint32_t)__popcll(__ballot(valid && ck < pk));
+                if (lane == p) below_cand_p = cnt;
+            }
+        } else {
+            for (uint64_t b = vb; b; b &= b - 1) {
+                const uint32_t cl = (uint32_t)__builtin_ctzll(b);
+                const uint64_t ci = wave_read64(ck, cl);
+                const bool before_p = ci < pkey;
+                below_cand_c += ci < ck ? 1u : 0u;
+                below_cand_p += before_p ? 1u : 0u;
+                const uint32_t nb = K - (uint32_t)__popcll(__ballot(before_p) & lmask);
+                if (lane == cl) below_list = nb;
+            }
+        }
+        uint32_t pos_p = lane + below_cand_p;
+        uint32_t pos_s = below_list + below_cand_c;
+        if constexpr (TIE) {
+            const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(valid && pos_s == K - 1);
+            const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(valid && pos_s == K);
+            if ((ap | as) && (bp | bs)) {
+                const bool a_pool = ap != 0, b_pool = bp != 0;
+                const uint32_t la = (uint32_t)__builtin_ctzll(a_pool ? ap : as);
+                const uint32_t lb = (uint32_t)__builtin_ctzll(b_pool ? bp : bs);
+                const uint64_t ka = wave_read64(a_pool ? pkey : ck, la);
+                const uint64_t kb = wave_read64(b_pool ? pkey : ck, lb);
+                if (nnm_key_dist(kb) <= nnm_key_dist(ka) * a.tie_win) {
+                    ++t.ties;
+                    const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
+                    const double ea = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ia * a.ld, a.ld);
+                    const double eb = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ib * a.ld, a.ld);
+                    if (eb < ea || (eb == ea && ib < ia)) {
+                        ++t.swaps;
+                        if (lane == la) {
+                            if (a_pool) pos_p = K; else pos_s = K;
+                        }
+                        if (lane == lb) {
+                            if (b_pool) pos_p = K - 1; else pos_s = K - 1;
+                        }
+                    }
+                }
+            }
+        }
+        const bool ins = valid && pos_s < K;
+        const bool keep = lane < K && pos_p < K;
+        if (!__ballot(ins)) continue;
+        changed = true;
+        // new list in registers: position q takes list lane perm[q] (< 64) or
+        // candidate lane perm[q] - 64
+        if (keep) perm[pos_p] = lane;
+        if (ins) perm[pos_s] = NNM_WAVE + lane;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t src = lane < K ? perm[lane] : lane;
+        const uint32_t sl = src & (NNM_WAVE - 1);
+        const uint32_t lid = (uint32_t)__shfl((int)pid, (int)sl, NNM_WAVE);
+        const float ldist = __shfl(pd, (int)sl, NNM_WAVE);
+        const uint32_t lflag = (uint32_t)__shfl((int)pf, (int)sl, NNM_WAVE);
+        const uint32_t lfresh = (uint32_t)__shfl((int)fresh, (int)sl, NNM_WAVE);
+        const uint64_t cnew = shfl64(ck, sl);
+        __builtin_amdgcn_wave_barrier();  // perm is read before the next piece writes it
+        if (lane < K) {
+            if (src >= NNM_WAVE) {
+                pid = nnm_key_id(cnew);
+                pd = nnm_key_dist(cnew);
+                pf = 1;
+                fresh = 1;
+            } else {
+                pid = lid;
+                pd = ldist;
+                pf = lflag;
+                fresh = lfresh;
+            }
+        }
+    }
+    if (changed && lane < K) {
+        a.g_ids[v * K + lane] = pid;
+        a.g_dist[v * K + lane] = pd;
+        a.g_flag[v * K + lane] = (uint8_t)(!last && fresh ? 2u : pf);
+    }
+    if (last) {
+        // entries replaced: the final list's ids that were not listed when
+        // the delivery started (the same count in whatever order the
+        // candidates came), and the new flags for the delta test
+        t.upd += (uint32_t)__popcll(__ballot(lane < K && fresh));
+        t.flags += (uint32_t)__popcll(__ballot(lane < K && pf));
+    }
+}
+
+// One workgroup per bin of 64 targets (persistent over the bins): the bin's
+// records in chunks of OWN_CH, each counting-sorted by target in LDS, then
+// one wave per target merges its run.  Every target of a bin is visited on
+// its last chunk to count its new flags (the delta test).
+template <bool RR, bool TIE>
+__global__ __launch_bounds__(256) void k_own(OwnArgs a) {
+    __shared__ uint64_t skey[OWN_CH];
+    __shared__ uint32_t hcnt[OWN_T];
+    __shared__ uint32_t hoff[OWN_T];
+    __shared__ uint32_t perm[4][NNM_WAVE];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    OwnTally t;
+    // vertices left for a redo round: this round is not the iteration's last
+    const bool final_round = a.counters[CNT_REDO] == 0;
+    for (uint32_t bin = blockIdx.x; bin < a.nbins; bin += gridDim.x) {
+        const uint32_t s = bin ? a.bin_end[bin - 1] : 0u, e = a.bin_end[bin];
+        const uint64_t t0 = (uint64_t)bin << NNM_OWN_BITS;
+        uint32_t c0 = s;
+        for (;;) {
+            const uint32_t ce = min(e, c0 + OWN_CH), m = ce - c0;
+            const bool last_chunk = ce >= e;
+            const bool last = last_chunk && final_round;
+            if (tid < OWN_T) hcnt[tid] = 0;
+            __syncthreads();
+            uint64_t key[OWN_CH / 256];
+            uint32_t tl[OWN_CH / 256], rk[OWN_CH / 256];
+#pragma unroll
+            for (uint32_t k = 0; k < OWN_CH / 256; ++k) {
+                const uint32_t j = tid + 256 * k;
+                tl[k] = OWN_T;
+                rk[k] = 0;
+                key[k] = NNM_EMPTY_KEY;
+                if (j < m) {
+                    const uint32_t* r = a.rec + ((uint64_t)c0 + j) * 3;
+                    key[k] = ((uint64_t)r[0] << 32) | r[1];
+                    const uint64_t tg = r[2];
+                    // the partition puts only this bin's targets here
+                    tl[k] = tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
+                    if (tl[k] < OWN_T) rk[k] = atomicAdd(&hcnt[tl[k]], 1u);
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {
+                // exclusive scan of the 64 counts
+                const uint32_t c = hcnt[lane];
+                uint32_t x = c;
+#pragma unroll
+                for (int o = 1; o < NNM_WAVE; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
+                    if (lane >= (uint32_t)o) x += y;
+                }
+                hoff[lane] = x - c;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t k = 0; k < OWN_CH / 256; ++k)
+                if (tl[k] < OWN_T) skey[hoff[tl[k]] + rk[k]] = key[k];
+            __syncthreads();
+            for (uint32_t q = wave; q < OWN_T; q += 4) {
+                const uint64_t v = t0 + q;
+                if (v >= a.n) break;
+                const uint32_t len = hcnt[q];
+                if (len == 0 && !last) continue;
+                own_merge<RR, TIE>(a, v, skey + hoff[q], len, last, perm[wave], t);
+            }
+            __syncthreads();  // the chunk's LDS is reused
+            if (last_chunk) break;
+            c0 = ce;
+        }
+    }
+    if (lane == 0) {
+        if (t.upd) atomicAdd((unsigned long long*)&a.counters[CNT_UPDATES], (unsigned long long)t.upd);
+        if (t.flags) atomicAdd((unsigned long long*)&a.counters[CNT_FLAGS], (unsigned long long)t.flags);
+        if (t.rr) atomicAdd((unsigned long long*)&a.counters[CNT_RERANKED], (unsigned long long)t.rr);
+        if (t.ties) atomicAdd((unsigned long long*)&a.counters[CNT_NEAR_TIES], (unsigned long long)t.ties);
+        if (t.swaps) atomicAdd((unsigned long long*)&a.counters[CNT_TIE_SWAPS], (unsigned long long)t.swaps);
+    }
+}
+
+// ------------------------------------------------------------------ host
+
+namespace {
+
+struct PassPlan {
+    uint32_t npass = 1;
+    uint32_t shift[3] = {0, 0, 0};
+    uint32_t bins[3] = {0, 0, 0};
+};
+
+uint32_t ceil_log2(uint64_t x) {
+    uint32_t b = 0;
+    while ((1ull << b) < x) ++b;
+    return b;
+}
+
+// Digits of every pass: the last pass's bins are the owner bins (target >>
+// NNM_OWN_BITS); the bits above them are split into passes of at most 11
+// bits, so a tile's digits fit the LDS histogram.
+PassPlan plan_passes(uint64_t n) {
+    PassPlan p;
+    const uint32_t bits = ceil_log2(std::max<uint64_t>(n, 2));
+    const uint32_t r = bits > NNM_OWN_BITS ? bits - NNM_OWN_BITS : 0u;
+    p.npass = std::max<uint32_t>(1, (r + 10) / 11);
+    const uint32_t per = p.npass ? (r + p.npass - 1) / p.npass : 0u;
+    for (uint32_t k = 0; k < p.npass; ++k) {
+        p.shift[k] = NNM_OWN_BITS + per * (p.npass - 1 - k);
+        p.bins[k] = (uint32_t)(((n - 1) >> p.shift[k]) + 1);
+    }
+    return p;
+}
+
+int fill32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) { return nnm_fill_u32(c, p, n, v); }
+
+}  // namespace
+
+void nnm_offer_free(nnm_ctx* c) {
+    void** ps[] = {(void**)&c->orec[0], (void**)&c->orec[1], (void**)&c->ochunk_used, (void**)&c->opart_cnt,
+                   (void**)&c->opart_cur[0], (void**)&c->opart_cur[1], (void**)&c->opart_cur[2],
+                   (void**)&c->work[0], (void**)&c->work[1]};
+    for (void** p : ps) {
+        nnm_free(c, *p);
+        *p = nullptr;
+    }
+    c->ochunks = 0;
+    c->ocap_row = 0;
+}
+
+int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
+    const uint32_t cap_row = p->offer_cap ? p->offer_cap : 0u;
+    if (c->orec[0] && c->ocap_row == cap_row) return NNM_OK;
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
+    nnm_offer_free(c);
+    const uint64_t n = c->n;
+    // the default: a chunk per 4096 records at 256 records per row, plus room
+    // for the partly filled last chunk of every join wave (at most 32 waves
+    // per CU over the three join kernels); an explicit offer_cap sizes the
+    // log to exactly offer_cap records per row
+    uint64_t chunks = std::max<uint64_t>(1, (n * (uint64_t)cap_row + NNM_OCH - 1) / NNM_OCH);
+    if (!p->offer_cap) chunks += (uint64_t)c->num_cus * 32;
+    chunks = std::min<uint64_t>(chunks, (0xFFFFFFFFull / NNM_OCH) - 1);
+    c->ochunks = (uint32_t)chunks;
+    c->ocap_row = cap_row;  // 0: the default sizing
+    const size_t rec_bytes = (size_t)chunks * NNM_OCH * 3 * sizeof(uint32_t);
+    NNM_TRY(nnm_alloc(c, (void**)&c->orec[0], rec_bytes));
+    NNM_TRY(nnm_alloc(c, (void**)&c->orec[1], rec_bytes));
+    NNM_TRY(nnm_alloc(c, (void**)&c->ochunk_used, (size_t)chunks * sizeof(uint32_t)));
+    // bins of any pass over any active prefix: at most (n >> NNM_OWN_BITS) + 1
+    const uint64_t maxbins = (n >> NNM_OWN_BITS) + 2;
+    NNM_TRY(nnm_alloc(c, (void**)&c->opart_cnt, maxbins * sizeof(uint32_t)));
+    for (int k = 0; k < 3; ++k) NNM_TRY(nnm_alloc(c, (void**)&c->opart_cur[k], (maxbins + 1) * sizeof(uint32_t)));
+    NNM_TRY(nnm_alloc(c, (void**)&c->work[0], n * sizeof(uint32_t)));
+    NNM_TRY(nnm_alloc(c, (void**)&c->work[1], n * sizeof(uint32_t)));
+    return NNM_OK;
+}
+
+OfferSink nnm_offer_sink(nnm_ctx* c, uint32_t round) {
+    OfferSink s;
+    s.rec = c->orec[0];
+    s.chunk_used = c->ochunk_used;
+    s.counters = c->counters;
+    s.redo = c->work[round & 1];
+    s.nchunks = c->ochunks;
+    return s;
+}
+
+int nnm_offer_reset(nnm_ctx* c) { return nnm_fill_u64(c, c->counters + CNT_OCHUNKS, 2, 0); }
+
+static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus * 4; }
+
+int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
+    const PassPlan pl = plan_passes(c->n_act);
+    PassIn in;
+    in.rec = c->orec[0];
+    in.chunk_used = c->ochunk_used;
+    in.counters = c->counters;
+    in.nchunks = c->ochunks;
+    in.total = nullptr;
+    for (uint32_t k = 0; k < pl.npass; ++k) {
+        uint32_t* out = c->orec[(k + 1) & 1];
+        const uint32_t bins = pl.bins[k];
+        in.nbins = bins;
+        NNM_TRY(fill32(c, c->opart_cnt, bins, 0));
+        hipLaunchKernelGGL(k_part_count, dim3(pass_grid(c)), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k],
+                           c->opart_cnt);
+        NNM_LAUNCH_CHECK(c);
+        NNM_TRY(nnm_scan_u32(c, c->opart_cnt, bins, c->opart_cur[k], c->scan_tmp));
+        hipLaunchKernelGGL(k_part_scatter, dim3(pass_grid(c)), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k],
+                           c->opart_cur[k], out);
+        NNM_LAUNCH_CHECK(c);
+        in.rec = out;
+        in.chunk_used = nullptr;
+        in.total = c->opart_cur[k] + bins;  // the scan's total
+    }
+    OwnArgs a;
+    a.n = c->n_act;
+    a.K = c->K;
+    a.X = c->X;
+    a.Xb_exact = (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr;
+    a.ld = c->ld;
+    a.rerank_tau = 1.f;
+    a.tie_win = nnm_near_tie_window(c, p);
+    a.g_ids = c->g_ids;
+    a.g_dist = c->g_dist;
+    a.g_flag = c->g_flag;
+    a.rec = in.rec;
+    a.bin_end = c->opart_cur[pl.npass - 1];
+    a.nbins = pl.bins[pl.npass - 1];
+    a.counters = c->counters;
+    int rerank = 0;
+    if (p->prec == NNM_PREC_BF16)
+        rerank = (p->rerank == NNM_RERANK_ALWAYS) || (p->rerank == NNM_RERANK_AUTO && !c->data_exact_bf16);
+    // learned phase: lists and offers are both estimates of one metric, so the
+    // owners merge them as they are (the exact distances come at the switch)
+    if (p->rerank == NNM_RERANK_NEVER || p->prec == NNM_PREC_LEARNED) rerank = 0;
+    const bool tie = a.tie_win > 1.f;
+    void (*kern)(OwnArgs) = rerank ? (tie ? k_own<true, true> : k_own<true, false>)
+                                   : (tie ? k_own<false, true> : k_own<false, false>);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 2;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(a.nbins, (uint32_t)c->num_cus * per_cu));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, a);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}

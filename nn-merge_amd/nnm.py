@@ -29,7 +29,7 @@ SEARCH_SRC_KNN, SEARCH_SRC_DPG = 0, 1
 class Params(C.Structure):
     _fields_ = [("K", C.c_uint32), ("rho", C.c_float), ("rev_cap", C.c_uint32), ("max_iters", C.c_uint32),
                 ("delta", C.c_float), ("rule", C.c_uint32), ("prec", C.c_uint32), ("rerank", C.c_uint32),
-                ("slots", C.c_uint32), ("cut", C.c_uint32), ("plain_tail", C.c_uint32), ("flags", C.c_uint32),
+                ("offer_cap", C.c_uint32), ("cut", C.c_uint32), ("plain_tail", C.c_uint32), ("flags", C.c_uint32),
                 ("learn_tau", C.c_float), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
 
 
@@ -41,8 +41,9 @@ class Stats(C.Structure):
                 ("tile_js", C.c_uint32), ("tile_chunk", C.c_uint32), ("graph_iters", C.c_uint32), ("pad_", C.c_uint32),
                 ("long_dists", C.c_uint64), ("long_entries", C.c_uint64), ("long_offers", C.c_uint64),
                 ("join_main_secs", C.c_double), ("join_main_bytes", C.c_uint64), ("join_main_flops", C.c_uint64),
-                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64), ("slot_atomics", C.c_uint64),
-                ("learned_dists", C.c_uint64)]
+                ("near_ties", C.c_uint64), ("tie_swaps", C.c_uint64), ("redo_rows", C.c_uint64),
+                ("learned_dists", C.c_uint64), ("redo_rounds", C.c_uint32), ("pad2_", C.c_uint32),
+                ("deliver_secs", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

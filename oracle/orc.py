@@ -94,6 +94,12 @@ def lib():
     return _lib
 
 
+def ref_built():
+    """True when the reference's distance routines are built (oracle/_ref);
+    checks the file only, without loading it."""
+    return os.path.exists(REF_LIB_PATH)
+
+
 def ref_lib():
     """The reference's own distance routines, or None when not built."""
     global _ref

@@ -41,5 +41,7 @@ def test_params_struct_layout_matches_header():
     # nnm_params: 14 x 4-byte fields then a uint64 seed (offset 56), size 64
     assert ctypes.sizeof(nnm.Params) == 64
     assert nnm.Params.seed.offset == 56
-    assert ctypes.sizeof(nnm.Stats) == 8 + 6 * 8 + 3 * 8 + 8 + 16 + 32 + 16 + 16 + 8 + 8  # ... slot_atomics, learned_dists
+    # ... redo_rows, learned_dists, redo_rounds + pad, deliver_secs
+    assert ctypes.sizeof(nnm.Stats) == 8 + 6 * 8 + 3 * 8 + 8 + 16 + 32 + 16 + 16 + 8 + 8 + 8 + 8
     assert nnm.Stats.learned_dists.offset == 176
+    assert nnm.Stats.deliver_secs.offset == 192

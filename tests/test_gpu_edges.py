@@ -96,11 +96,10 @@ def test_duplicate_rows(ctx):
 def test_reverse_cap_extremes(ctx, rev_cap):
     """rev_cap 1 (almost no reverse sampling) and 300 (lists longer than the
     long-list window, joined by the windows kernel in FP32).  The reference
-    always samples 100; at cap 1 few paths lead back to a candidate that
-    lost its update slot, and the bound is 0.01 instead of 0.005."""
+    always samples 100."""
     X = nnm_data.sift_like(8000, 64, centers=16, sigma=20.0, seed=71)
     K = 16
-    tol = 0.01 if rev_cap == 1 else TOL
+    tol = TOL
     rows = np.arange(0, 8000, 8, dtype=np.uint32)
     gt, _ = orc.bruteforce(X, 10, rows=rows)
     oi, _, _ = orc.kgraph(X, K=K, rev_cap=rev_cap, seed=4)
@@ -212,7 +211,7 @@ def test_refused_inputs(ctx):
         ctx.random_init(10, seed=1)  # a group of 10 rows cannot hold K + 1 = 11
     ctx.set_partition([0, 200], [1], [0])
     ctx.random_init(8, seed=1)
-    for bad in (dict(rev_cap=0), dict(slots=48), dict(rho=0.0), dict(rho=1.5), dict(rule=7), dict(prec=9)):
+    for bad in (dict(rev_cap=0), dict(offer_cap=2), dict(rho=0.0), dict(rho=1.5), dict(rule=7), dict(prec=9)):
         with pytest.raises(nnm.NNMError):
             ctx.nndescent(nnm.default_params(K=8, max_iters=2, **bad))
     with pytest.raises(nnm.NNMError):

@@ -27,7 +27,7 @@ def test_distopt_known_answer():
     assert abs(exact - 19.44) < 1e-5  # 1+1+1+1+1+14.44
 
 
-@pytest.mark.skipif(orc.ref_lib() is None, reason="reference sources not built (oracle/_ref)")
+@pytest.mark.skipif(not orc.ref_built(), reason="reference sources not built (oracle/_ref)")
 def test_restatement_bit_equal_to_reference_distances():
     R = orc.ref_lib()
     rng = np.random.default_rng(0)
@@ -39,7 +39,7 @@ def test_restatement_bit_equal_to_reference_distances():
             assert np.float32(orc.l2_simd4(a, b)) == np.float32(R.ref_l2sqr_simd_ext(_fp(a), 0, _fp(b), 0, d))
 
 
-@pytest.mark.skipif(orc.ref_lib() is None, reason="reference sources not built (oracle/_ref)")
+@pytest.mark.skipif(not orc.ref_built(), reason="reference sources not built (oracle/_ref)")
 def test_reference_row_indexing():
     R = orc.ref_lib()
     X = nnm_data.sift_like(10, 16, centers=2, seed=1)
