@@ -35,7 +35,7 @@
 // wave into chunks of NNM_OCH records it reserves from a shared counter.
 #define NNM_OCH 4096u
 // Owner bins: the partition passes group offers by target >> NNM_OWN_BITS.
-#define NNM_OWN_BITS 6u
+#define NNM_OWN_BITS 5u
 // Default offer-log capacity per active row (nnm_params.offer_cap = 0).
 #define NNM_OFFER_CAP_DEFAULT 256u
 

@@ -39,7 +39,7 @@
 #define NO_DIGIT 0xFFFFFFFFu
 
 static_assert(PT_TILE == NNM_OCH, "first-pass tiles are log chunks");
-static_assert(OWN_T == 64, "owner bins of 64 targets (one LDS counter per lane)");
+static_assert(OWN_T <= 64, "owner bins of at most 64 targets (one LDS counter per lane)");
 
 // ------------------------------------------------------------------ passes
 
@@ -202,6 +202,15 @@ __device__ inline double own_l2_f64(const float* __restrict__ a, const float* __
     return acc;
 }
 
+// Orders the wave's LDS accesses across lanes: the stores before it are
+// seen by the loads after it, and no load before it is moved past it (a
+// store of another lane after it cannot clobber what an earlier load reads).
+__device__ inline void own_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct OwnTally {
     uint64_t upd = 0, flags = 0, rr = 0, ties = 0, swaps = 0;  // wave-uniform
 };
@@ -222,7 +231,11 @@ struct OwnTally {
 // last chunk of the iteration's last round (last = true), which counts the
 // entries that entered (the iteration's updates) and the new flags, and
 // stores every flag as 0 or 1.
-template <bool RR, bool TIE>
+// IDD: one id may come with different keys (lower bounds from the bf16 join
+// and exact distances from the FP32 long-list join, without re-rank), so
+// the candidates are de-duplicated by id first; otherwise a pair has the
+// same key whichever window offered it and equal keys are repeats.
+template <bool RR, bool TIE, bool IDD>
 __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* keys, uint32_t len, bool last,
                                  uint32_t* perm, OwnTally& t) {
     const uint32_t lane = nnm_lane();
@@ -251,9 +264,9 @@ __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* k
         else if constexpr (TIE) valid = ck != NNM_EMPTY_KEY && (ck < kth_key || nnm_key_dist(ck) <= kd * a.tie_win);
         else valid = ck < kth_key;  // the empty key is above every key
         if (!__ballot(valid)) continue;
-        // de-dup by id: sort by (id, dist) and keep the first of each id (its
-        // smallest key); the valid candidates land in lanes 0..nv-1
-        {
+        if constexpr (IDD) {
+            // de-dup by id: sort by (id, dist) and keep the first of each id
+            // (its smallest key)
             uint64_t sk = valid ? (((uint64_t)nnm_key_id(ck) << 32) | (ck >> 32)) : NNM_EMPTY_KEY;
             sk = nnm_wave_sort64(sk);
             const uint64_t prev = shfl64(sk, lane ? lane - 1 : 0);
@@ -289,91 +302,81 @@ __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* k
             vb = __ballot(valid);
         }
         if (!vb) continue;
-        // ranks: a candidate's position is (list keys below it) + (candidate
-        // keys below it); a list entry's is its index + (candidate keys below
-        // it).  Keys are distinct (distinct ids).
-        uint32_t below_list = 0, below_cand_c = 0, below_cand_p = 0;
-        const uint32_t nv = (uint32_t)__popcll(vb);
-        if (nv > OWN_SORT_MIN) {
-            ck = nnm_wave_sort64(valid ? ck : NNM_EMPTY_KEY);
-            valid = lane < nv;
-            below_cand_c = lane;
-            for (uint32_t p = 0; p < K; ++p) {
-                const uint64_t pk = wave_read64(pkey, p);
-                below_list += pk < ck ? 1u : 0u;
-                const uint32_t cnt = (uint32_t)__popcll(__ballot(valid && ck < pk));
-                if (lane == p) below_cand_p = cnt;
+        // sorted by key, the repeats of one pair (equal keys) are adjacent:
+        // keep the first of each and close the gaps
+        ck = nnm_wave_sort64(valid ? ck : NNM_EMPTY_KEY);
+        {
+            const uint64_t prev = shfl64(ck, lane ? lane - 1 : 0);
+            const bool keepc = ck != NNM_EMPTY_KEY && (lane == 0 || prev != ck);
+            const uint64_t m = __ballot(keepc);
+            const uint32_t nc = (uint32_t)__popcll(m);
+            if (m != (nc == NNM_WAVE ? ~0ull : (1ull << nc) - 1)) {
+                own_wave_sync();  // earlier readers of perm are done
+                if (keepc) perm[nnm_mask_rank(m)] = lane;
+                own_wave_sync();
+                ck = shfl64(ck, lane < nc ? perm[lane] : lane);
             }
-        } else {
-            for (uint64_t b = vb; b; b &= b - 1) {
-                const uint32_t cl = (uint32_t)__builtin_ctzll(b);
-                const uint64_t ci = wave_read64(ck, cl);
-                const bool before_p = ci < pkey;
-                below_cand_c += ci < ck ? 1u : 0u;
-                below_cand_p += before_p ? 1u : 0u;
-                const uint32_t nb = K - (uint32_t)__popcll(__ballot(before_p) & lmask);
-                if (lane == cl) below_list = nb;
-            }
+            if (lane >= nc) ck = NNM_EMPTY_KEY;  // also repeats trailing the kept keys
         }
-        uint32_t pos_p = lane + below_cand_p;
-        uint32_t pos_s = below_list + below_cand_c;
+        // The candidates are now distinct from the list and from each other,
+        // so every key is distinct.  The first 64 - K candidates ascending
+        // and the list descending form one bitonic sequence of 64 keys; six
+        // half-cleaner steps sort it, and the first K lanes are the new list
+        // (at most K candidates can enter, and 64 - K >= K for K <= 32).  A
+        // payload word carries each entry's flag and whether it entered in
+        // this delivery.
+        const uint32_t L = NNM_WAVE - K;
+        const uint32_t src = lane >= L ? NNM_WAVE - 1 - lane : lane;
+        const uint64_t lk = shfl64(pkey, src);
+        const uint32_t lpay = (uint32_t)__shfl((int)(pf | (fresh << 1)), (int)src, NNM_WAVE);
+        uint64_t sk = lane >= L ? lk : ck;
+        uint32_t pay = lane >= L ? lpay : 3u;  // a candidate enters new (flag 1) and fresh
+        auto half_clean = [&](auto J) {
+            constexpr uint32_t j = decltype(J)::value;
+            const uint64_t ok_ = wave_xor64<j>(sk);
+            const uint32_t op = wave_xor32<j>(pay);
+            const bool lower = (lane & j) == 0;
+            const bool take = lower ? ok_ < sk : ok_ > sk;
+            sk = take ? ok_ : sk;
+            pay = take ? op : pay;
+        };
+        half_clean(std::integral_constant<uint32_t, 32>());
+        half_clean(std::integral_constant<uint32_t, 16>());
+        half_clean(std::integral_constant<uint32_t, 8>());
+        half_clean(std::integral_constant<uint32_t, 4>());
+        half_clean(std::integral_constant<uint32_t, 2>());
+        half_clean(std::integral_constant<uint32_t, 1>());
         if constexpr (TIE) {
-            const uint64_t ap = __ballot(lane < K && pos_p == K - 1), as = __ballot(valid && pos_s == K - 1);
-            const uint64_t bp = __ballot(lane < K && pos_p == K), bs = __ballot(valid && pos_s == K);
-            if ((ap | as) && (bp | bs)) {
-                const bool a_pool = ap != 0, b_pool = bp != 0;
-                const uint32_t la = (uint32_t)__builtin_ctzll(a_pool ? ap : as);
-                const uint32_t lb = (uint32_t)__builtin_ctzll(b_pool ? bp : bs);
-                const uint64_t ka = wave_read64(a_pool ? pkey : ck, la);
-                const uint64_t kb = wave_read64(b_pool ? pkey : ck, lb);
-                if (nnm_key_dist(kb) <= nnm_key_dist(ka) * a.tie_win) {
-                    ++t.ties;
-                    const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
-                    const double ea = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ia * a.ld, a.ld);
-                    const double eb = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ib * a.ld, a.ld);
-                    if (eb < ea || (eb == ea && ib < ia)) {
-                        ++t.swaps;
-                        if (lane == la) {
-                            if (a_pool) pos_p = K; else pos_s = K;
-                        }
-                        if (lane == lb) {
-                            if (b_pool) pos_p = K - 1; else pos_s = K - 1;
-                        }
+            // the entries ranked K-1 (last kept) and K (first dropped): when
+            // their FP32 distances are within rounding of each other the
+            // choice is re-made on FP64 distances (ties by id)
+            const uint64_t ka = wave_read64(sk, K - 1), kb = wave_read64(sk, K);
+            if (kb != NNM_EMPTY_KEY && nnm_key_dist(kb) <= nnm_key_dist(ka) * a.tie_win) {
+                ++t.ties;
+                const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
+                const double ea = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ia * a.ld, a.ld);
+                const double eb = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ib * a.ld, a.ld);
+                if (eb < ea || (eb == ea && ib < ia)) {
+                    ++t.swaps;
+                    const uint32_t pa_ = wave_read32(pay, K - 1), pb_ = wave_read32(pay, K);
+                    if (lane == K - 1) {
+                        sk = kb;
+                        pay = pb_;
+                    }
+                    if (lane == K) {
+                        sk = ka;
+                        pay = pa_;
                     }
                 }
             }
         }
-        const bool ins = valid && pos_s < K;
-        const bool keep = lane < K && pos_p < K;
-        if (!__ballot(ins)) continue;
+        if (!(__ballot(lane < K && sk != pkey))) continue;  // no candidate entered
         changed = true;
-        // new list in registers: position q takes list lane perm[q] (< 64) or
-        // candidate lane perm[q] - 64
-        if (keep) perm[pos_p] = lane;
-        if (ins) perm[pos_s] = NNM_WAVE + lane;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t src = lane < K ? perm[lane] : lane;
-        const uint32_t sl = src & (NNM_WAVE - 1);
-        const uint32_t lid = (uint32_t)__shfl((int)pid, (int)sl, NNM_WAVE);
-        const float ldist = __shfl(pd, (int)sl, NNM_WAVE);
-        const uint32_t lflag = (uint32_t)__shfl((int)pf, (int)sl, NNM_WAVE);
-        const uint32_t lfresh = (uint32_t)__shfl((int)fresh, (int)sl, NNM_WAVE);
-        const uint64_t cnew = shfl64(ck, sl);
-        __builtin_amdgcn_wave_barrier();  // perm is read before the next piece writes it
         if (lane < K) {
-            if (src >= NNM_WAVE) {
-                pid = nnm_key_id(cnew);
-                pd = nnm_key_dist(cnew);
-                pf = 1;
-                fresh = 1;
-            } else {
-                pid = lid;
-                pd = ldist;
-                pf = lflag;
-                fresh = lfresh;
-            }
+            pid = nnm_key_id(sk);
+            pd = nnm_key_dist(sk);
+            pf = pay & 1u;
+            fresh = (pay >> 1) & 1u;
         }
     }
     if (changed && lane < K) {
@@ -394,7 +397,7 @@ __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* k
 // records in chunks of OWN_CH, each counting-sorted by target in LDS, then
 // one wave per target merges its run.  Every target of a bin is visited on
 // its last chunk to count its new flags (the delta test).
-template <bool RR, bool TIE>
+template <bool RR, bool TIE, bool IDD>
 __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
     __shared__ uint64_t skey[OWN_CH];
     __shared__ uint32_t hcnt[OWN_T];
@@ -433,15 +436,15 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
             }
             __syncthreads();
             if (wave == 0) {
-                // exclusive scan of the 64 counts
-                const uint32_t c = hcnt[lane];
+                // exclusive scan of the bin's counts
+                const uint32_t c = lane < OWN_T ? hcnt[lane] : 0u;
                 uint32_t x = c;
 #pragma unroll
                 for (int o = 1; o < NNM_WAVE; o <<= 1) {
                     const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
                     if (lane >= (uint32_t)o) x += y;
                 }
-                hoff[lane] = x - c;
+                if (lane < OWN_T) hoff[lane] = x - c;
             }
             __syncthreads();
 #pragma unroll
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
                 if (v >= a.n) break;
                 const uint32_t len = hcnt[q];
                 if (len == 0 && !last) continue;
-                own_merge<RR, TIE>(a, v, skey + hoff[q], len, last, perm[wave], t);
+                own_merge<RR, TIE, IDD>(a, v, skey + hoff[q], len, last, perm[wave], t);
             }
             __syncthreads();  // the chunk's LDS is reused
             if (last_chunk) break;
@@ -527,8 +530,9 @@ int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
     // for the partly filled last chunk of every join wave (at most 32 waves
     // per CU over the three join kernels); an explicit offer_cap sizes the
     // log to exactly offer_cap records per row
-    uint64_t chunks = std::max<uint64_t>(1, (n * (uint64_t)cap_row + NNM_OCH - 1) / NNM_OCH);
-    if (!p->offer_cap) chunks += (uint64_t)c->num_cus * 32;
+    const uint64_t per_row = cap_row ? cap_row : NNM_OFFER_CAP_DEFAULT;
+    uint64_t chunks = std::max<uint64_t>(1, (n * per_row + NNM_OCH - 1) / NNM_OCH);
+    if (!cap_row) chunks += (uint64_t)c->num_cus * 32;
     chunks = std::min<uint64_t>(chunks, (0xFFFFFFFFull / NNM_OCH) - 1);
     c->ochunks = (uint32_t)chunks;
     c->ocap_row = cap_row;  // 0: the default sizing
@@ -605,8 +609,13 @@ int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
     // owners merge them as they are (the exact distances come at the switch)
     if (p->rerank == NNM_RERANK_NEVER || p->prec == NNM_PREC_LEARNED) rerank = 0;
     const bool tie = a.tie_win > 1.f;
-    void (*kern)(OwnArgs) = rerank ? (tie ? k_own<true, true> : k_own<true, false>)
-                                   : (tie ? k_own<false, true> : k_own<false, false>);
+    // lower-bound keys kept as they are next to exact keys of the FP32
+    // long-list join: one id can carry two keys
+    const bool idd = !rerank && p->prec == NNM_PREC_BF16 && !c->data_exact_bf16 &&
+                     c->cap_new + c->cap_old > nnm_join_learned_max_list();
+    void (*kern)(OwnArgs) = rerank ? (tie ? k_own<true, true, false> : k_own<true, false, false>)
+                            : idd  ? (tie ? k_own<false, true, true> : k_own<false, false, true>)
+                                   : (tie ? k_own<false, true, false> : k_own<false, false, false>);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 2;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(a.nbins, (uint32_t)c->num_cus * per_cu));
