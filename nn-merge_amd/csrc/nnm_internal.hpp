@@ -8,7 +8,7 @@
 //   g_flag n x K   uint8   1 = entry is new (not yet joined)
 //   label  n       uint32  subgraph label: s for rows of built subgraph s,
 //                          NNM_RAW for raw rows
-//   offer log      records {dist bits, id, target} the join writes in order,
+//   offer log      records {key, target} the join writes in order,
 //                  grouped by target by partition passes before the owners
 //                  merge them into their lists (nnm_offers.hip)
 #pragma once
@@ -31,8 +31,9 @@
 // the values; the low word breaks ties by id.  All ones marks no candidate.
 #define NNM_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
 
-// Offer log: records of 3 words {dist bits, id, target}, written by each join
-// wave into chunks of NNM_OCH records it reserves from a shared counter.
+// Offer log: records {key = dist bits << 32 | id, target} in two arrays,
+// written by each join wave into chunks of NNM_OCH records it reserves from
+// a shared counter.
 #define NNM_OCH 4096u
 // Owner bins: the partition passes group offers by target >> NNM_OWN_BITS.
 #define NNM_OWN_BITS 5u
@@ -132,13 +133,17 @@ struct nnm_ctx {
     uint64_t* spill = nullptr;      // n: per row, the list entries past a full bucket (bit j: entry j)
     uint32_t* scan_tmp = nullptr;   // block sums
     // offer log and its partition (nnm_offers.hip)
-    uint32_t* orec[2] = {nullptr, nullptr};  // 2 x (ochunks x NNM_OCH) records, 3 words each
+    uint64_t* okey[2] = {nullptr, nullptr};  // 2 x (ochunks x NNM_OCH) record keys (dist bits << 32 | id)
+    uint32_t* otgt[2] = {nullptr, nullptr};  // ... and their targets
     uint32_t* ochunk_used = nullptr;  // records written per log chunk
     uint32_t ochunks = 0;             // log capacity in chunks
     uint32_t ocap_row = 0;            // capacity per row it was sized for
     uint32_t* opart_cnt = nullptr;    // per pass: bin counts (largest pass)
     uint32_t* opart_cur[3] = {nullptr, nullptr, nullptr};  // per pass: bin cursors (bins + 1)
     uint32_t* work[2] = {nullptr, nullptr};  // join work lists for redo rounds (n each)
+    uint32_t* opart_blk = nullptr;    // first pass: per-block bin counts (bins x grid)
+    uint32_t* opart_boff = nullptr;   // ... their exclusive scan
+    uint32_t* opart_tmp = nullptr;    // scan scratch of the passes
     uint64_t* counters = nullptr;   // CNT_N
     float* kth = nullptr;           // n: k-th distance of every list at iteration start
     uint4* vmeta = nullptr;         // n: join record {label, kth x offer window, norm, 0}
@@ -634,7 +639,8 @@ __device__ inline uint32_t nnm_wave_rnd_sds(uint64_t seed, uint64_t stream, uint
 // list; they are joined again in a later round (their earlier offers then
 // arrive twice, which the owners' de-dup by id absorbs).
 struct OfferSink {
-    uint32_t* rec;          // ochunks x NNM_OCH records of 3 words
+    uint64_t* key;          // ochunks x NNM_OCH record keys
+    uint32_t* tgt;          // ... and targets
     uint32_t* chunk_used;   // records written per chunk
     uint64_t* counters;     // CNT_OCHUNKS, CNT_REDO
     uint32_t* redo;         // vertices to join again
@@ -649,7 +655,7 @@ struct OfferWriter {
     bool full = false;      // the log ran out of chunks for this wave
 };
 
-// Append the offers of the lanes with p set: record {dist, id, target}.
+// Append the offers of the lanes with p set: record {key(dist, id), target}.
 __device__ inline void offer_emit(const OfferSink& s, OfferWriter& w, bool p, uint32_t target, uint32_t id,
                                   float dist) {
     const uint64_t m = __ballot(p);
@@ -673,10 +679,8 @@ __device__ inline void offer_emit(const OfferSink& s, OfferWriter& w, bool p, ui
     if (p) {
         const uint64_t below = __lane_id() ? (m & (~0ull >> (64 - __lane_id()))) : 0ull;
         const uint64_t pos = (uint64_t)w.chunk * NNM_OCH + w.used + (uint32_t)__popcll(below);
-        uint32_t* r = s.rec + pos * 3;
-        r[0] = __float_as_uint(dist);
-        r[1] = id;
-        r[2] = target;
+        s.key[pos] = ((uint64_t)__float_as_uint(dist) << 32) | id;
+        s.tgt[pos] = target;
     }
     w.used += k;
 }

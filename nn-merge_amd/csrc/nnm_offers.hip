@@ -31,23 +31,26 @@
 #include "nnm_internal.hpp"
 
 #define PT_THREADS 256u
-#define PT_PER 16u                         // records per thread in a tile
-#define PT_TILE (PT_THREADS * PT_PER)      // records per tile
-#define PT_BINS 2048u                      // LDS bins of a tile (relative to its smallest digit)
+#define PT_PER 8u                          // records per thread in a tile
+#define PT_TILE (PT_THREADS * PT_PER)      // records per tile (half a log chunk)
+#define PT_BINS 1024u                      // LDS bins of a tile (relative to its smallest digit)
+#define PT_MAXBITS 10u                     // digit bits of one pass
 #define OWN_T (1u << NNM_OWN_BITS)         // targets per owner bin
 #define OWN_CH 4096u                       // records an owner sorts in LDS at a time
 #define NO_DIGIT 0xFFFFFFFFu
 
-static_assert(PT_TILE == NNM_OCH, "first-pass tiles are log chunks");
+static_assert(2 * PT_TILE == NNM_OCH, "first-pass tiles are halves of log chunks");
+static_assert(PT_BINS == 4 * PT_THREADS, "four bins per thread in the LDS scan");
 static_assert(OWN_T <= 64, "owner bins of at most 64 targets (one LDS counter per lane)");
 
 // ------------------------------------------------------------------ passes
 
 struct PassIn {
-    const uint32_t* rec;         // records, 3 words each
-    const uint32_t* chunk_used;  // first pass: records per log chunk (tile t = chunk t)
+    const uint64_t* key;         // record keys
+    const uint32_t* tgt;         // record targets
+    const uint32_t* chunk_used;  // first pass: records per log chunk (tiles 2c, 2c + 1 = chunk c)
     const uint64_t* counters;    // first pass: CNT_OCHUNKS
-    uint32_t nchunks;            // log capacity in chunks (= tiles of any pass)
+    uint32_t nchunks;            // log capacity in chunks (2 tiles each, in any pass)
     const uint32_t* total;       // later passes: record count of the input
     uint32_t nbins;              // bins of the pass (records of other digits are not moved)
 };
@@ -58,11 +61,22 @@ __device__ inline void tile_range(const PassIn& in, uint64_t t, uint64_t* b, uin
     *m = 0;
     if (in.chunk_used) {
         const uint64_t used = min(in.counters[CNT_OCHUNKS], (uint64_t)in.nchunks);
-        if (t < used) *m = in.chunk_used[t];
+        const uint64_t ch = t >> 1;
+        if (ch < used) {
+            const uint32_t u = in.chunk_used[ch], h = (uint32_t)(t & 1u) * PT_TILE;
+            *m = u > h ? min(u - h, PT_TILE) : 0u;
+        }
     } else {
         const uint64_t tot = *in.total;
         if (*b < tot) *m = (uint32_t)min((uint64_t)PT_TILE, tot - *b);
     }
+}
+
+// Tiles of the pass input: two per reserved log chunk (first pass) or the
+// record count in tiles.
+__device__ inline uint64_t tile_count(const PassIn& in) {
+    if (in.chunk_used) return 2 * min(in.counters[CNT_OCHUNKS], (uint64_t)in.nchunks);
+    return ((uint64_t)*in.total + PT_TILE - 1) / PT_TILE;
 }
 
 // Smallest digit of the tile into *dmin_s (LDS; preset to NO_DIGIT).
@@ -77,12 +91,24 @@ __device__ inline uint32_t block_min_digit(const uint32_t (&d)[PT_PER], uint32_t
     return *dmin_s;
 }
 
-// Bin counts of one pass: cnt[target >> shift] += records.
+// Digits of thread tid's records of a tile (NO_DIGIT past the tile or the bins).
+__device__ inline void tile_digits(const PassIn& in, uint64_t b, uint32_t m, uint32_t shift, uint32_t tid,
+                                   uint32_t (&d)[PT_PER]) {
+#pragma unroll
+    for (uint32_t k = 0; k < PT_PER; ++k) {
+        const uint32_t j = tid + k * PT_THREADS;
+        d[k] = j < m ? in.tgt[b + j] >> shift : NO_DIGIT;
+        if (d[k] >= in.nbins) d[k] = NO_DIGIT;
+    }
+}
+
+// Bin counts of one pass: cnt[target >> shift] += records (reads the targets only).
 __global__ __launch_bounds__(PT_THREADS) void k_part_count(PassIn in, uint32_t shift, uint32_t* __restrict__ cnt) {
     __shared__ uint32_t hist[PT_BINS];
     __shared__ uint32_t dmin_s;
     const uint32_t tid = threadIdx.x;
-    for (uint64_t t = blockIdx.x; t < in.nchunks; t += gridDim.x) {
+    const uint64_t nt = tile_count(in);
+    for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
         uint64_t b;
         uint32_t m;
         tile_range(in, t, &b, &m);
@@ -91,12 +117,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(PassIn in, uint32_t s
         if (tid == 0) dmin_s = NO_DIGIT;
         __syncthreads();
         uint32_t d[PT_PER];
-#pragma unroll
-        for (uint32_t k = 0; k < PT_PER; ++k) {
-            const uint32_t j = tid + k * PT_THREADS;
-            d[k] = j < m ? in.rec[(b + j) * 3 + 2] >> shift : NO_DIGIT;
-            if (d[k] >= in.nbins) d[k] = NO_DIGIT;
-        }
+        tile_digits(in, b, m, shift, tid, d);
         const uint32_t dmin = block_min_digit(d, &dmin_s);
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
@@ -112,15 +133,21 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_count(PassIn in, uint32_t s
     }
 }
 
-// Scatter of one pass: every record to out[cur[digit]++] (cursors start at
-// the scanned bin offsets).
+// Scatter of one pass: every record to its bin at cur[digit]++ (cursors start
+// at the scanned bin offsets).  The tile is first ordered by bin in LDS, so
+// the writes go out as runs of consecutive records per bin, one cursor atomic
+// per nonempty bin.
 __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassIn in, uint32_t shift, uint32_t* __restrict__ cur,
-                                                             uint32_t* __restrict__ out) {
-    __shared__ uint32_t hist[PT_BINS];
-    __shared__ uint32_t basev[PT_BINS];
+                                                             uint64_t* __restrict__ okey, uint32_t* __restrict__ otgt) {
+    __shared__ uint64_t lkey[PT_TILE];
+    __shared__ uint32_t ltgt[PT_TILE];
+    __shared__ uint32_t hist[PT_BINS];   // counts, then their exclusive prefix
+    __shared__ uint32_t basev[PT_BINS];  // global position of each bin's run
+    __shared__ uint32_t wsum[PT_THREADS / NNM_WAVE];
     __shared__ uint32_t dmin_s;
-    const uint32_t tid = threadIdx.x;
-    for (uint64_t t = blockIdx.x; t < in.nchunks; t += gridDim.x) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint64_t nt = tile_count(in);
+    for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
         uint64_t b;
         uint32_t m;
         tile_range(in, t, &b, &m);
@@ -128,44 +155,199 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassIn in, uint32_t
         for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
         if (tid == 0) dmin_s = NO_DIGIT;
         __syncthreads();
-        uint32_t r0[PT_PER], r1[PT_PER], r2[PT_PER], d[PT_PER];
+        uint32_t d[PT_PER], tg[PT_PER], rk[PT_PER];
+        uint64_t ky[PT_PER];
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
             const uint32_t j = tid + k * PT_THREADS;
-            r0[k] = r1[k] = r2[k] = 0;
+            tg[k] = 0;
+            ky[k] = 0;
             d[k] = NO_DIGIT;
             if (j < m) {
-                const uint32_t* r = in.rec + (b + j) * 3;
-                r0[k] = r[0];
-                r1[k] = r[1];
-                r2[k] = r[2];
-                d[k] = r2[k] >> shift;
+                tg[k] = in.tgt[b + j];
+                ky[k] = in.key[b + j];
+                d[k] = tg[k] >> shift;
                 if (d[k] >= in.nbins) d[k] = NO_DIGIT;
             }
         }
         const uint32_t dmin = block_min_digit(d, &dmin_s);
-        uint32_t pos[PT_PER];
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
-            pos[k] = 0;
+            rk[k] = 0;
             if (d[k] == NO_DIGIT) continue;
             const uint32_t r = d[k] - dmin;
-            pos[k] = r < PT_BINS ? atomicAdd(&hist[r], 1u) : atomicAdd(&cur[d[k]], 1u);
+            if (r < PT_BINS) {
+                rk[k] = atomicAdd(&hist[r], 1u);
+            } else {
+                // digits far above the tile's smallest: written one by one
+                const uint32_t p = atomicAdd(&cur[d[k]], 1u);
+                okey[p] = ky[k];
+                otgt[p] = tg[k];
+                d[k] = NO_DIGIT;
+            }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS)
-            if (hist[i]) basev[i] = atomicAdd(&cur[dmin + i], hist[i]);
+        // global runs of the nonempty bins, and the exclusive prefix of the
+        // counts (thread tid owns bins 4 tid .. 4 tid + 3)
+        uint32_t c4[4], s4 = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t i = 4 * tid + q;
+            c4[q] = hist[i];
+            if (c4[q]) basev[i] = atomicAdd(&cur[dmin + i], c4[q]);
+            s4 += c4[q];
+        }
+        uint32_t x = s4;
+#pragma unroll
+        for (int o = 1; o < NNM_WAVE; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == NNM_WAVE - 1) wsum[wave] = x;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (uint32_t w = 0; w < wave; ++w) wbase += wsum[w];
+        uint32_t run = wbase + x - s4;
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            hist[4 * tid + q] = run;
+            run += c4[q];
+        }
+        const uint32_t nl = wsum[0] + wsum[1] + wsum[2] + wsum[3];  // records staged in LDS
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
             if (d[k] == NO_DIGIT) continue;
-            const uint32_t r = d[k] - dmin;
-            const uint64_t p = r < PT_BINS ? (uint64_t)basev[r] + pos[k] : (uint64_t)pos[k];
-            uint32_t* o = out + p * 3;
-            o[0] = r0[k];
-            o[1] = r1[k];
-            o[2] = r2[k];
+            const uint32_t lp = hist[d[k] - dmin] + rk[k];
+            lkey[lp] = ky[k];
+            ltgt[lp] = tg[k];
         }
+        __syncthreads();
+        for (uint32_t lp = tid; lp < nl; lp += PT_THREADS) {
+            const uint32_t tgv = ltgt[lp];
+            const uint32_t r = (tgv >> shift) - dmin;
+            const uint32_t p = basev[r] + (lp - hist[r]);
+            okey[p] = lkey[lp];
+            otgt[p] = tgv;
+        }
+        __syncthreads();
+    }
+}
+
+// Exclusive prefix of the 1024 LDS counts hist[] into hist[] (thread tid owns
+// bins 4 tid .. 4 tid + 3); returns the total.  c4 gets the thread's counts.
+__device__ inline uint32_t block_prefix_bins(uint32_t* hist, uint32_t* wsum, uint32_t (&c4)[4]) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t s4 = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        c4[q] = hist[4 * tid + q];
+        s4 += c4[q];
+    }
+    uint32_t x = s4;
+#pragma unroll
+    for (int o = 1; o < NNM_WAVE; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == NNM_WAVE - 1) wsum[wave] = x;
+    __syncthreads();
+    uint32_t run = x - s4;
+    for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        hist[4 * tid + q] = run;
+        run += c4[q];
+    }
+    const uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return tot;
+}
+
+// First pass (at most PT_BINS bins): each block counts the records of the
+// tiles it will scatter, per bin, over all its tiles; bcnt[bin * grid +
+// block] (bin-major, so that one scan gives every block the start of its run
+// in every bin: no cursor atomics, which would all meet on a few hundred
+// addresses).
+__global__ __launch_bounds__(PT_THREADS) void k_part0_count(PassIn in, uint32_t shift, uint32_t* __restrict__ bcnt) {
+    __shared__ uint32_t hist[PT_BINS];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
+    __syncthreads();
+    const uint64_t nt = tile_count(in);
+    for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        uint64_t b;
+        uint32_t m;
+        tile_range(in, t, &b, &m);
+        uint32_t d[PT_PER];
+        tile_digits(in, b, m, shift, tid, d);
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k)
+            if (d[k] != NO_DIGIT) atomicAdd(&hist[d[k]], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < in.nbins; i += PT_THREADS) bcnt[(uint64_t)i * gridDim.x + blockIdx.x] = hist[i];
+}
+
+// First-pass scatter: the same blocks over the same tiles; block runs start
+// at the scanned per-block counts boff[bin * grid + block].  Each tile is
+// ordered by bin in LDS and written out in runs.
+__global__ __launch_bounds__(PT_THREADS) void k_part0_scatter(PassIn in, uint32_t shift,
+                                                              const uint32_t* __restrict__ boff,
+                                                              uint64_t* __restrict__ okey, uint32_t* __restrict__ otgt) {
+    __shared__ uint64_t lkey[PT_TILE];
+    __shared__ uint32_t ltgt[PT_TILE];
+    __shared__ uint32_t hist[PT_BINS];   // per tile: counts, then their exclusive prefix
+    __shared__ uint32_t curs[PT_BINS];   // the block's next global position in every bin
+    __shared__ uint32_t wsum[PT_THREADS / NNM_WAVE];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < in.nbins; i += PT_THREADS) curs[i] = boff[(uint64_t)i * gridDim.x + blockIdx.x];
+    const uint64_t nt = tile_count(in);
+    for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        uint64_t b;
+        uint32_t m;
+        tile_range(in, t, &b, &m);
+        if (m == 0) continue;
+        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
+        __syncthreads();
+        uint32_t d[PT_PER], tg[PT_PER], rk[PT_PER];
+        uint64_t ky[PT_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            const uint32_t j = tid + k * PT_THREADS;
+            tg[k] = 0;
+            ky[k] = 0;
+            d[k] = NO_DIGIT;
+            rk[k] = 0;
+            if (j < m) {
+                tg[k] = in.tgt[b + j];
+                ky[k] = in.key[b + j];
+                d[k] = tg[k] >> shift;
+                if (d[k] >= in.nbins) d[k] = NO_DIGIT;
+            }
+            if (d[k] != NO_DIGIT) rk[k] = atomicAdd(&hist[d[k]], 1u);
+        }
+        __syncthreads();
+        uint32_t c4[4];
+        const uint32_t nl = block_prefix_bins(hist, wsum, c4);
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            if (d[k] == NO_DIGIT) continue;
+            const uint32_t lp = hist[d[k]] + rk[k];
+            lkey[lp] = ky[k];
+            ltgt[lp] = tg[k];
+        }
+        __syncthreads();
+        for (uint32_t lp = tid; lp < nl; lp += PT_THREADS) {
+            const uint32_t tgv = ltgt[lp];
+            const uint32_t r = tgv >> shift;
+            const uint32_t p = curs[r] + (lp - hist[r]);
+            okey[p] = lkey[lp];
+            otgt[p] = tgv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) curs[4 * tid + q] += c4[q];
         __syncthreads();
     }
 }
@@ -183,8 +365,10 @@ struct OwnArgs {
     uint32_t* g_ids;
     float* g_dist;
     uint8_t* g_flag;
-    const uint32_t* rec;        // records grouped by owner bin
-    const uint32_t* bin_end;    // bin b holds records [bin_end[b - 1], bin_end[b])
+    const uint64_t* key;        // records grouped by owner bin: keys
+    const uint32_t* tgt;        // ... and targets
+    const uint32_t* bin_end;    // bin b holds records [bin_end[(b - 1) * stride], bin_end[b * stride]) (from 0 for b = 0)
+    uint32_t bin_stride;
     uint32_t nbins;
     uint64_t* counters;
 };
@@ -408,7 +592,8 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
     // vertices left for a redo round: this round is not the iteration's last
     const bool final_round = a.counters[CNT_REDO] == 0;
     for (uint32_t bin = blockIdx.x; bin < a.nbins; bin += gridDim.x) {
-        const uint32_t s = bin ? a.bin_end[bin - 1] : 0u, e = a.bin_end[bin];
+        const uint32_t s = bin ? a.bin_end[(uint64_t)(bin - 1) * a.bin_stride] : 0u;
+        const uint32_t e = a.bin_end[(uint64_t)bin * a.bin_stride];
         const uint64_t t0 = (uint64_t)bin << NNM_OWN_BITS;
         uint32_t c0 = s;
         for (;;) {
@@ -426,9 +611,8 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
                 rk[k] = 0;
                 key[k] = NNM_EMPTY_KEY;
                 if (j < m) {
-                    const uint32_t* r = a.rec + ((uint64_t)c0 + j) * 3;
-                    key[k] = ((uint64_t)r[0] << 32) | r[1];
-                    const uint64_t tg = r[2];
+                    key[k] = a.key[(uint64_t)c0 + j];
+                    const uint64_t tg = a.tgt[(uint64_t)c0 + j];
                     // the partition puts only this bin's targets here
                     tl[k] = tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
                     if (tl[k] < OWN_T) rk[k] = atomicAdd(&hcnt[tl[k]], 1u);
@@ -489,13 +673,13 @@ uint32_t ceil_log2(uint64_t x) {
 }
 
 // Digits of every pass: the last pass's bins are the owner bins (target >>
-// NNM_OWN_BITS); the bits above them are split into passes of at most 11
-// bits, so a tile's digits fit the LDS histogram.
+// NNM_OWN_BITS); the bits above them are split into passes of at most
+// PT_MAXBITS bits, so a tile's digits fit the LDS histogram.
 PassPlan plan_passes(uint64_t n) {
     PassPlan p;
     const uint32_t bits = ceil_log2(std::max<uint64_t>(n, 2));
     const uint32_t r = bits > NNM_OWN_BITS ? bits - NNM_OWN_BITS : 0u;
-    p.npass = std::max<uint32_t>(1, (r + 10) / 11);
+    p.npass = std::max<uint32_t>(1, (r + PT_MAXBITS - 1) / PT_MAXBITS);
     const uint32_t per = p.npass ? (r + p.npass - 1) / p.npass : 0u;
     for (uint32_t k = 0; k < p.npass; ++k) {
         p.shift[k] = NNM_OWN_BITS + per * (p.npass - 1 - k);
@@ -508,10 +692,15 @@ int fill32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) { return nnm_fill_u3
 
 }  // namespace
 
+static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus * 4; }
+static uint64_t pass0_slots(nnm_ctx* c) { return (uint64_t)PT_BINS * pass_grid(c) + 1; }
+
 void nnm_offer_free(nnm_ctx* c) {
-    void** ps[] = {(void**)&c->orec[0], (void**)&c->orec[1], (void**)&c->ochunk_used, (void**)&c->opart_cnt,
+    void** ps[] = {(void**)&c->okey[0], (void**)&c->okey[1], (void**)&c->otgt[0], (void**)&c->otgt[1],
+                   (void**)&c->ochunk_used, (void**)&c->opart_cnt,
                    (void**)&c->opart_cur[0], (void**)&c->opart_cur[1], (void**)&c->opart_cur[2],
-                   (void**)&c->work[0], (void**)&c->work[1]};
+                   (void**)&c->work[0], (void**)&c->work[1], (void**)&c->opart_blk, (void**)&c->opart_boff,
+                   (void**)&c->opart_tmp};
     for (void** p : ps) {
         nnm_free(c, *p);
         *p = nullptr;
@@ -522,7 +711,7 @@ void nnm_offer_free(nnm_ctx* c) {
 
 int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
     const uint32_t cap_row = p->offer_cap ? p->offer_cap : 0u;
-    if (c->orec[0] && c->ocap_row == cap_row) return NNM_OK;
+    if (c->okey[0] && c->ocap_row == cap_row) return NNM_OK;
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     nnm_offer_free(c);
     const uint64_t n = c->n;
@@ -536,9 +725,11 @@ int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
     chunks = std::min<uint64_t>(chunks, (0xFFFFFFFFull / NNM_OCH) - 1);
     c->ochunks = (uint32_t)chunks;
     c->ocap_row = cap_row;  // 0: the default sizing
-    const size_t rec_bytes = (size_t)chunks * NNM_OCH * 3 * sizeof(uint32_t);
-    NNM_TRY(nnm_alloc(c, (void**)&c->orec[0], rec_bytes));
-    NNM_TRY(nnm_alloc(c, (void**)&c->orec[1], rec_bytes));
+    const size_t recs = (size_t)chunks * NNM_OCH;
+    for (int k = 0; k < 2; ++k) {
+        NNM_TRY(nnm_alloc(c, (void**)&c->okey[k], recs * sizeof(uint64_t)));
+        NNM_TRY(nnm_alloc(c, (void**)&c->otgt[k], recs * sizeof(uint32_t)));
+    }
     NNM_TRY(nnm_alloc(c, (void**)&c->ochunk_used, (size_t)chunks * sizeof(uint32_t)));
     // bins of any pass over any active prefix: at most (n >> NNM_OWN_BITS) + 1
     const uint64_t maxbins = (n >> NNM_OWN_BITS) + 2;
@@ -546,12 +737,18 @@ int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
     for (int k = 0; k < 3; ++k) NNM_TRY(nnm_alloc(c, (void**)&c->opart_cur[k], (maxbins + 1) * sizeof(uint32_t)));
     NNM_TRY(nnm_alloc(c, (void**)&c->work[0], n * sizeof(uint32_t)));
     NNM_TRY(nnm_alloc(c, (void**)&c->work[1], n * sizeof(uint32_t)));
+    // first pass: per-block counts and their scan; scan scratch for every pass
+    NNM_TRY(nnm_alloc(c, (void**)&c->opart_blk, pass0_slots(c) * sizeof(uint32_t)));
+    NNM_TRY(nnm_alloc(c, (void**)&c->opart_boff, (pass0_slots(c) + 1) * sizeof(uint32_t)));
+    const uint64_t scan_m = std::max<uint64_t>(pass0_slots(c), maxbins);
+    NNM_TRY(nnm_alloc(c, (void**)&c->opart_tmp, ((scan_m + 2047) / 2048 + 2) * sizeof(uint32_t)));
     return NNM_OK;
 }
 
 OfferSink nnm_offer_sink(nnm_ctx* c, uint32_t round) {
     OfferSink s;
-    s.rec = c->orec[0];
+    s.key = c->okey[0];
+    s.tgt = c->otgt[0];
     s.chunk_used = c->ochunk_used;
     s.counters = c->counters;
     s.redo = c->work[round & 1];
@@ -561,31 +758,49 @@ OfferSink nnm_offer_sink(nnm_ctx* c, uint32_t round) {
 
 int nnm_offer_reset(nnm_ctx* c) { return nnm_fill_u64(c, c->counters + CNT_OCHUNKS, 2, 0); }
 
-static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus * 4; }
 
 int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
     const PassPlan pl = plan_passes(c->n_act);
     PassIn in;
-    in.rec = c->orec[0];
+    in.key = c->okey[0];
+    in.tgt = c->otgt[0];
     in.chunk_used = c->ochunk_used;
     in.counters = c->counters;
     in.nchunks = c->ochunks;
     in.total = nullptr;
+    const uint32_t G = pass_grid(c);
+    const uint32_t* bin_end = nullptr;
+    uint32_t bin_stride = 1;
     for (uint32_t k = 0; k < pl.npass; ++k) {
-        uint32_t* out = c->orec[(k + 1) & 1];
+        const int o = (k + 1) & 1;
         const uint32_t bins = pl.bins[k];
         in.nbins = bins;
-        NNM_TRY(fill32(c, c->opart_cnt, bins, 0));
-        hipLaunchKernelGGL(k_part_count, dim3(pass_grid(c)), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k],
-                           c->opart_cnt);
-        NNM_LAUNCH_CHECK(c);
-        NNM_TRY(nnm_scan_u32(c, c->opart_cnt, bins, c->opart_cur[k], c->scan_tmp));
-        hipLaunchKernelGGL(k_part_scatter, dim3(pass_grid(c)), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k],
-                           c->opart_cur[k], out);
-        NNM_LAUNCH_CHECK(c);
-        in.rec = out;
+        if (k == 0) {
+            // per-block counts, one scan, no cursor atomics
+            hipLaunchKernelGGL(k_part0_count, dim3(G), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k], c->opart_blk);
+            NNM_LAUNCH_CHECK(c);
+            NNM_TRY(nnm_scan_u32(c, c->opart_blk, (uint64_t)bins * G, c->opart_boff, c->opart_tmp));
+            hipLaunchKernelGGL(k_part0_scatter, dim3(G), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k], c->opart_boff,
+                               c->okey[o], c->otgt[o]);
+            NNM_LAUNCH_CHECK(c);
+            in.total = c->opart_boff + (uint64_t)bins * G;  // the scan's total
+            bin_end = c->opart_boff + G;  // bin r ends where bin r + 1 starts
+            bin_stride = G;
+        } else {
+            NNM_TRY(fill32(c, c->opart_cnt, bins, 0));
+            hipLaunchKernelGGL(k_part_count, dim3(G), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k], c->opart_cnt);
+            NNM_LAUNCH_CHECK(c);
+            NNM_TRY(nnm_scan_u32(c, c->opart_cnt, bins, c->opart_cur[k], c->opart_tmp));
+            hipLaunchKernelGGL(k_part_scatter, dim3(G), dim3(PT_THREADS), 0, c->stream, in, pl.shift[k],
+                               c->opart_cur[k], c->okey[o], c->otgt[o]);
+            NNM_LAUNCH_CHECK(c);
+            in.total = c->opart_cur[k] + bins;  // the scan's total
+            bin_end = c->opart_cur[k];  // the cursors end at their bins' ends
+            bin_stride = 1;
+        }
+        in.key = c->okey[o];
+        in.tgt = c->otgt[o];
         in.chunk_used = nullptr;
-        in.total = c->opart_cur[k] + bins;  // the scan's total
     }
     OwnArgs a;
     a.n = c->n_act;
@@ -598,8 +813,10 @@ int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
     a.g_ids = c->g_ids;
     a.g_dist = c->g_dist;
     a.g_flag = c->g_flag;
-    a.rec = in.rec;
-    a.bin_end = c->opart_cur[pl.npass - 1];
+    a.key = in.key;
+    a.tgt = in.tgt;
+    a.bin_end = bin_end;
+    a.bin_stride = bin_stride;
     a.nbins = pl.bins[pl.npass - 1];
     a.counters = c->counters;
     int rerank = 0;
