@@ -403,22 +403,65 @@ struct OwnTally {
 // instead of a loop over them.
 #define OWN_SORT_MIN 32u
 
+// Bitonic sort of the keys in lanes [0, S) ascending (S = 2^lg, lg <= 6;
+// the other lanes hold the empty key and stay out of the way).
+__device__ inline uint64_t own_sort_prefix(uint64_t key, uint32_t lg) {
+    if (lg >= 1) key = bitonic_pass64<2, 1>(key);
+    if (lg >= 2) key = bitonic_pass64<4, 2>(key);
+    if (lg >= 3) key = bitonic_pass64<8, 4>(key);
+    if (lg >= 4) key = bitonic_pass64<16, 8>(key);
+    if (lg >= 5) key = bitonic_pass64<32, 16>(key);
+    if (lg >= 6) key = bitonic_pass64<64, 32>(key);
+    return key;
+}
+
+// Keys of the lanes with keep set moved to lanes 0..popcount - 1 in lane
+// order; the empty key above them.  perm: the wave's 64 words of LDS.
+__device__ inline uint64_t own_compact(uint64_t key, bool keep, uint32_t* perm) {
+    const uint32_t lane = nnm_lane();
+    const uint64_t m = __ballot(keep);
+    const uint32_t nc = (uint32_t)__popcll(m);
+    if (m != (nc == NNM_WAVE ? ~0ull : (1ull << nc) - 1)) {
+        own_wave_sync();  // earlier readers of perm are done
+        if (keep) perm[nnm_mask_rank(m)] = lane;
+        own_wave_sync();
+        key = shfl64(key, lane < nc ? perm[lane] : lane);
+    }
+    return lane < nc ? key : NNM_EMPTY_KEY;
+}
+
+__device__ inline void own_cswap(uint64_t& x, uint64_t& y) {
+    const uint64_t lo = x < y ? x : y, hi = x < y ? y : x;
+    x = lo;
+    y = hi;
+}
+
 // Merge `len` candidate keys (keys[0..len), any order, repeats allowed) into
-// the list of row v: UpdateKnnList (src/pvetex.h:40-85) for the batch, 64
-// candidates at a time.  A candidate enters when its id is not listed and
-// its (dist, id) key ranks among the K smallest; with RR its lower-bound key
-// is first recomputed exactly; with TIE the entries ranked K-1 and K are
-// re-decided on FP64 distances when their FP32 ones are within rounding.
+// the list of row v: UpdateKnnList (src/pvetex.h:40-85) for the batch.  A
+// candidate enters when its id is not listed and its (dist, id) key ranks
+// among the K smallest; with RR its lower-bound key is first recomputed
+// exactly; with TIE the entries ranked K-1 and K are re-decided on FP64
+// distances when their FP32 ones are within rounding.
+//
+// The candidates come 256 at a time, four per lane, sorted inside each lane;
+// round r merges every lane's r-th smallest.  After round 0 the list's k-th
+// key has dropped to the best of the lanes' minima, so later rounds keep few
+// candidates (and none at all once a round keeps none: each lane's keys
+// only grow).  A round: filter by the k-th key, de-dup against the list by
+// id, sort the survivors (a network sized to their count), drop repeats of
+// one pair (equal keys), then merge with the list by a bitonic merge.
+//
 // `perm` is 64 words of LDS owned by the wave.  A target whose candidates
 // span several chunks of its bin (or several redo rounds) is merged once per
 // chunk; entries that entered in an earlier merge carry flag 2 until the
 // last chunk of the iteration's last round (last = true), which counts the
 // entries that entered (the iteration's updates) and the new flags, and
 // stores every flag as 0 or 1.
+//
 // IDD: one id may come with different keys (lower bounds from the bf16 join
 // and exact distances from the FP32 long-list join, without re-rank), so
-// the candidates are de-duplicated by id first; otherwise a pair has the
-// same key whichever window offered it and equal keys are repeats.
+// the candidates of a round are de-duplicated by id first; otherwise a pair
+// has the same key whichever window offered it and equal keys are repeats.
 template <bool RR, bool TIE, bool IDD>
 __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* keys, uint32_t len, bool last,
                                  uint32_t* perm, OwnTally& t) {
@@ -436,131 +479,139 @@ __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* k
         pf = pf ? 1u : 0u;
     }
     bool changed = last && __ballot(fresh) != 0;  // flags 2 are stored as 1 on the last chunk
-    for (uint32_t p0 = 0; p0 < len; p0 += NNM_WAVE) {
-        uint64_t ck = p0 + lane < len ? keys[p0 + lane] : NNM_EMPTY_KEY;
-        const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
-        const uint64_t kth_key = wave_read64(pkey, K - 1);
-        const float kd = nnm_key_dist(kth_key);
-        // cheap filter first: lower bounds against the margin, exact keys
-        // against the k-th key (or its rounding window)
-        bool valid;
-        if constexpr (RR) valid = ck != NNM_EMPTY_KEY && nnm_key_dist(ck) <= kd * a.rerank_tau;
-        else if constexpr (TIE) valid = ck != NNM_EMPTY_KEY && (ck < kth_key || nnm_key_dist(ck) <= kd * a.tie_win);
-        else valid = ck < kth_key;  // the empty key is above every key
-        if (!__ballot(valid)) continue;
-        if constexpr (IDD) {
-            // de-dup by id: sort by (id, dist) and keep the first of each id
-            // (its smallest key)
-            uint64_t sk = valid ? (((uint64_t)nnm_key_id(ck) << 32) | (ck >> 32)) : NNM_EMPTY_KEY;
-            sk = nnm_wave_sort64(sk);
-            const uint64_t prev = shfl64(sk, lane ? lane - 1 : 0);
-            valid = sk != NNM_EMPTY_KEY && (lane == 0 || (prev >> 32) != (sk >> 32));
-            ck = valid ? ((sk << 32) | (sk >> 32)) : NNM_EMPTY_KEY;
+    for (uint32_t g0 = 0; g0 < len; g0 += 4 * NNM_WAVE) {
+        uint64_t kk[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint32_t i = g0 + lane + NNM_WAVE * r;
+            kk[r] = i < len ? keys[i] : NNM_EMPTY_KEY;
         }
-        const uint32_t cid = nnm_key_id(ck);
-        uint64_t vb = __ballot(valid);
-        // de-dup against the list: a loop over the fewer of candidates and
-        // list entries
-        if ((uint32_t)__popcll(vb) <= K) {
-            for (uint64_t b = vb; b; b &= b - 1) {
-                const uint32_t cl = (uint32_t)__builtin_ctzll(b);
-                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)cid, (int)cl);
-                if (__ballot(pid == q) & lmask) vb &= ~(1ull << cl);
+        own_cswap(kk[0], kk[1]);
+        own_cswap(kk[2], kk[3]);
+        own_cswap(kk[0], kk[2]);
+        own_cswap(kk[1], kk[3]);
+        own_cswap(kk[1], kk[2]);
+        const uint32_t rounds = min(4u, (len - g0 + NNM_WAVE - 1) / NNM_WAVE);
+        for (uint32_t r = 0; r < rounds; ++r) {
+            uint64_t ck = r == 0 ? kk[0] : r == 1 ? kk[1] : r == 2 ? kk[2] : kk[3];
+            const uint64_t pkey = lane < K ? nnm_key(pd, pid) : NNM_EMPTY_KEY;
+            const uint64_t kth_key = wave_read64(pkey, K - 1);
+            const float kd = nnm_key_dist(kth_key);
+            // cheap filter first: lower bounds against the margin, exact keys
+            // against the k-th key (or its rounding window)
+            bool valid;
+            if constexpr (RR) valid = ck != NNM_EMPTY_KEY && nnm_key_dist(ck) <= kd * a.rerank_tau;
+            else if constexpr (TIE) valid = ck != NNM_EMPTY_KEY && (ck < kth_key || nnm_key_dist(ck) <= kd * a.tie_win);
+            else valid = ck < kth_key;  // the empty key is above every key
+            if (!__ballot(valid)) break;  // the later rounds' keys are larger still
+            if constexpr (IDD) {
+                // de-dup by id: sort by (id, dist) and keep the first of each id
+                // (its smallest key)
+                uint64_t sk = valid ? (((uint64_t)nnm_key_id(ck) << 32) | (ck >> 32)) : NNM_EMPTY_KEY;
+                sk = nnm_wave_sort64(sk);
+                const uint64_t prev = shfl64(sk, lane ? lane - 1 : 0);
+                valid = sk != NNM_EMPTY_KEY && (lane == 0 || (prev >> 32) != (sk >> 32));
+                ck = valid ? ((sk << 32) | (sk >> 32)) : NNM_EMPTY_KEY;
             }
-            valid = (vb >> lane) & 1ull;
-        } else {
-            for (uint32_t p = 0; p < K; ++p)
-                valid = valid && (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p) != cid;
-            vb = __ballot(valid);
-        }
-        if constexpr (RR) {
-            // exact FP32 distances of the survivors, then the exact filter
-            if (valid) {
-                const float e = a.Xb_exact
-                                    ? nnm_l2_rows_bf16<1>(a.Xb_exact + v * a.ld, a.Xb_exact + (uint64_t)cid * a.ld, a.ld)
-                                    : nnm_l2_rows<1>(a.X + v * a.ld, a.X + (uint64_t)cid * a.ld, a.ld);
-                ck = nnm_key(e, cid);
+            const uint32_t cid = nnm_key_id(ck);
+            uint64_t vb = __ballot(valid);
+            // de-dup against the list: a loop over the fewer of candidates and
+            // list entries
+            if ((uint32_t)__popcll(vb) <= K) {
+                for (uint64_t b = vb; b; b &= b - 1) {
+                    const uint32_t cl = (uint32_t)__builtin_ctzll(b);
+                    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)cid, (int)cl);
+                    if (__ballot(pid == q) & lmask) vb &= ~(1ull << cl);
+                }
+                valid = (vb >> lane) & 1ull;
+            } else {
+                for (uint32_t p = 0; p < K; ++p)
+                    valid = valid && (uint32_t)__builtin_amdgcn_readlane((int)pid, (int)p) != cid;
+                vb = __ballot(valid);
             }
-            t.rr += (uint32_t)__popcll(vb);
-            valid = valid && (ck < kth_key || (TIE && nnm_key_dist(ck) <= kd * a.tie_win));
-            vb = __ballot(valid);
-        }
-        if (!vb) continue;
-        // sorted by key, the repeats of one pair (equal keys) are adjacent:
-        // keep the first of each and close the gaps
-        ck = nnm_wave_sort64(valid ? ck : NNM_EMPTY_KEY);
-        {
-            const uint64_t prev = shfl64(ck, lane ? lane - 1 : 0);
-            const bool keepc = ck != NNM_EMPTY_KEY && (lane == 0 || prev != ck);
-            const uint64_t m = __ballot(keepc);
-            const uint32_t nc = (uint32_t)__popcll(m);
-            if (m != (nc == NNM_WAVE ? ~0ull : (1ull << nc) - 1)) {
-                own_wave_sync();  // earlier readers of perm are done
-                if (keepc) perm[nnm_mask_rank(m)] = lane;
-                own_wave_sync();
-                ck = shfl64(ck, lane < nc ? perm[lane] : lane);
+            if constexpr (RR) {
+                // exact FP32 distances of the survivors, then the exact filter
+                if (valid) {
+                    const float e = a.Xb_exact
+                                        ? nnm_l2_rows_bf16<1>(a.Xb_exact + v * a.ld, a.Xb_exact + (uint64_t)cid * a.ld, a.ld)
+                                        : nnm_l2_rows<1>(a.X + v * a.ld, a.X + (uint64_t)cid * a.ld, a.ld);
+                    ck = nnm_key(e, cid);
+                }
+                t.rr += (uint32_t)__popcll(vb);
+                valid = valid && (ck < kth_key || (TIE && nnm_key_dist(ck) <= kd * a.tie_win));
+                vb = __ballot(valid);
             }
-            if (lane >= nc) ck = NNM_EMPTY_KEY;  // also repeats trailing the kept keys
-        }
-        // The candidates are now distinct from the list and from each other,
-        // so every key is distinct.  The first 64 - K candidates ascending
-        // and the list descending form one bitonic sequence of 64 keys; six
-        // half-cleaner steps sort it, and the first K lanes are the new list
-        // (at most K candidates can enter, and 64 - K >= K for K <= 32).  A
-        // payload word carries each entry's flag and whether it entered in
-        // this delivery.
-        const uint32_t L = NNM_WAVE - K;
-        const uint32_t src = lane >= L ? NNM_WAVE - 1 - lane : lane;
-        const uint64_t lk = shfl64(pkey, src);
-        const uint32_t lpay = (uint32_t)__shfl((int)(pf | (fresh << 1)), (int)src, NNM_WAVE);
-        uint64_t sk = lane >= L ? lk : ck;
-        uint32_t pay = lane >= L ? lpay : 3u;  // a candidate enters new (flag 1) and fresh
-        auto half_clean = [&](auto J) {
-            constexpr uint32_t j = decltype(J)::value;
-            const uint64_t ok_ = wave_xor64<j>(sk);
-            const uint32_t op = wave_xor32<j>(pay);
-            const bool lower = (lane & j) == 0;
-            const bool take = lower ? ok_ < sk : ok_ > sk;
-            sk = take ? ok_ : sk;
-            pay = take ? op : pay;
-        };
-        half_clean(std::integral_constant<uint32_t, 32>());
-        half_clean(std::integral_constant<uint32_t, 16>());
-        half_clean(std::integral_constant<uint32_t, 8>());
-        half_clean(std::integral_constant<uint32_t, 4>());
-        half_clean(std::integral_constant<uint32_t, 2>());
-        half_clean(std::integral_constant<uint32_t, 1>());
-        if constexpr (TIE) {
-            // the entries ranked K-1 (last kept) and K (first dropped): when
-            // their FP32 distances are within rounding of each other the
-            // choice is re-made on FP64 distances (ties by id)
-            const uint64_t ka = wave_read64(sk, K - 1), kb = wave_read64(sk, K);
-            if (kb != NNM_EMPTY_KEY && nnm_key_dist(kb) <= nnm_key_dist(ka) * a.tie_win) {
-                ++t.ties;
-                const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
-                const double ea = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ia * a.ld, a.ld);
-                const double eb = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ib * a.ld, a.ld);
-                if (eb < ea || (eb == ea && ib < ia)) {
-                    ++t.swaps;
-                    const uint32_t pa_ = wave_read32(pay, K - 1), pb_ = wave_read32(pay, K);
-                    if (lane == K - 1) {
-                        sk = kb;
-                        pay = pb_;
-                    }
-                    if (lane == K) {
-                        sk = ka;
-                        pay = pa_;
+            if (!vb) continue;
+            // survivors to lanes 0..nv-1, sorted by a network sized to them;
+            // the repeats of one pair (equal keys) are then adjacent: keep the
+            // first of each
+            const uint32_t nv = (uint32_t)__popcll(vb);
+            ck = own_compact(ck, valid, perm);
+            ck = own_sort_prefix(ck, nv > 1 ? 32u - (uint32_t)__builtin_clz(nv - 1) : 0u);
+            {
+                const uint64_t prev = shfl64(ck, lane ? lane - 1 : 0);
+                ck = own_compact(ck, ck != NNM_EMPTY_KEY && (lane == 0 || prev != ck), perm);
+            }
+            // The candidates are now distinct from the list and from each
+            // other, so every key is distinct.  The first 64 - K candidates
+            // ascending and the list descending form one bitonic sequence of
+            // 64 keys; six half-cleaner steps sort it, and the first K lanes
+            // are the new list (at most K candidates can enter, and 64 - K >= K
+            // for K <= 32).  A payload word carries each entry's flag and
+            // whether it entered in this delivery.
+            const uint32_t L = NNM_WAVE - K;
+            const uint32_t src = lane >= L ? NNM_WAVE - 1 - lane : lane;
+            const uint64_t lk = shfl64(pkey, src);
+            const uint32_t lpay = (uint32_t)__shfl((int)(pf | (fresh << 1)), (int)src, NNM_WAVE);
+            uint64_t sk = lane >= L ? lk : ck;
+            uint32_t pay = lane >= L ? lpay : 3u;  // a candidate enters new (flag 1) and fresh
+            auto half_clean = [&](auto J) {
+                constexpr uint32_t j = decltype(J)::value;
+                const uint64_t ok_ = wave_xor64<j>(sk);
+                const uint32_t op = wave_xor32<j>(pay);
+                const bool lower = (lane & j) == 0;
+                const bool take = lower ? ok_ < sk : ok_ > sk;
+                sk = take ? ok_ : sk;
+                pay = take ? op : pay;
+            };
+            half_clean(std::integral_constant<uint32_t, 32>());
+            half_clean(std::integral_constant<uint32_t, 16>());
+            half_clean(std::integral_constant<uint32_t, 8>());
+            half_clean(std::integral_constant<uint32_t, 4>());
+            half_clean(std::integral_constant<uint32_t, 2>());
+            half_clean(std::integral_constant<uint32_t, 1>());
+            if constexpr (TIE) {
+                // the entries ranked K-1 (last kept) and K (first dropped): when
+                // their FP32 distances are within rounding of each other the
+                // choice is re-made on FP64 distances (ties by id)
+                const uint64_t ka = wave_read64(sk, K - 1), kb = wave_read64(sk, K);
+                if (kb != NNM_EMPTY_KEY && nnm_key_dist(kb) <= nnm_key_dist(ka) * a.tie_win) {
+                    ++t.ties;
+                    const uint32_t ia = nnm_key_id(ka), ib = nnm_key_id(kb);
+                    const double ea = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ia * a.ld, a.ld);
+                    const double eb = own_l2_f64(a.X + v * a.ld, a.X + (uint64_t)ib * a.ld, a.ld);
+                    if (eb < ea || (eb == ea && ib < ia)) {
+                        ++t.swaps;
+                        const uint32_t pa_ = wave_read32(pay, K - 1), pb_ = wave_read32(pay, K);
+                        if (lane == K - 1) {
+                            sk = kb;
+                            pay = pb_;
+                        }
+                        if (lane == K) {
+                            sk = ka;
+                            pay = pa_;
+                        }
                     }
                 }
             }
-        }
-        if (!(__ballot(lane < K && sk != pkey))) continue;  // no candidate entered
-        changed = true;
-        if (lane < K) {
-            pid = nnm_key_id(sk);
-            pd = nnm_key_dist(sk);
-            pf = pay & 1u;
-            fresh = (pay >> 1) & 1u;
+            if (!(__ballot(lane < K && sk != pkey))) continue;  // no candidate entered
+            changed = true;
+            if (lane < K) {
+                pid = nnm_key_id(sk);
+                pd = nnm_key_dist(sk);
+                pf = pay & 1u;
+                fresh = (pay >> 1) & 1u;
+            }
         }
     }
     if (changed && lane < K) {
