@@ -462,22 +462,35 @@ __device__ inline void own_cswap(uint64_t& x, uint64_t& y) {
 // and exact distances from the FP32 long-list join, without re-rank), so
 // the candidates of a round are de-duplicated by id first; otherwise a pair
 // has the same key whichever window offered it and equal keys are repeats.
+// A target's list in registers: entry j in lane j.
+struct OwnList {
+    uint32_t id = 0xFFFFFFFFu;
+    float dist = 0.f;
+    uint32_t flag = 0;
+};
+
+__device__ inline OwnList own_load(const OwnArgs& a, uint64_t v) {
+    OwnList l;
+    const uint32_t lane = nnm_lane();
+    if (v < a.n && lane < a.K) {
+        l.id = a.g_ids[v * a.K + lane];
+        l.dist = a.g_dist[v * a.K + lane];
+        l.flag = a.g_flag[v * a.K + lane];
+    }
+    return l;
+}
+
 template <bool RR, bool TIE, bool IDD>
-__device__ inline void own_merge(const OwnArgs& a, uint64_t v, const uint64_t* keys, uint32_t len, bool last,
-                                 uint32_t* perm, OwnTally& t) {
+__device__ inline void own_merge(const OwnArgs& a, uint64_t v, const OwnList& in, const uint64_t* keys, uint32_t len,
+                                 bool last, uint32_t* perm, OwnTally& t) {
     const uint32_t lane = nnm_lane();
     const uint32_t K = a.K;
     const uint64_t lmask = K < NNM_WAVE ? (1ull << K) - 1 : ~0ull;  // list lanes
-    uint32_t pid = 0xFFFFFFFFu, pf = 0;
-    float pd = 0.f;
-    uint32_t fresh = 0;  // the entry entered in this iteration's delivery
-    if (lane < K) {
-        pid = a.g_ids[v * K + lane];
-        pd = a.g_dist[v * K + lane];
-        pf = a.g_flag[v * K + lane];
-        fresh = pf == 2 ? 1u : 0u;
-        pf = pf ? 1u : 0u;
-    }
+    uint32_t pid = in.id, pf = in.flag;
+    float pd = in.dist;
+    const uint32_t fresh0 = lane < K && pf == 2 ? 1u : 0u;
+    uint32_t fresh = fresh0;  // the entry entered in this iteration's delivery
+    pf = pf ? 1u : 0u;
     bool changed = last && __ballot(fresh) != 0;  // flags 2 are stored as 1 on the last chunk
     for (uint32_t g0 = 0; g0 < len; g0 += 4 * NNM_WAVE) {
         uint64_t kk[4];
@@ -686,12 +699,19 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
             for (uint32_t k = 0; k < OWN_CH / 256; ++k)
                 if (tl[k] < OWN_T) skey[hoff[tl[k]] + rk[k]] = key[k];
             __syncthreads();
-            for (uint32_t q = wave; q < OWN_T; q += 4) {
-                const uint64_t v = t0 + q;
-                if (v >= a.n) break;
-                const uint32_t len = hcnt[q];
-                if (len == 0 && !last) continue;
-                own_merge<RR, TIE, IDD>(a, v, skey + hoff[q], len, last, perm[wave], t);
+            // the wave's targets q = wave, wave + 4, ...; the next one's list
+            // is loaded while the current one merges
+            auto need = [&](uint32_t q) { return t0 + q < a.n && (hcnt[q] != 0 || last); };
+            uint32_t q = wave;
+            while (q < OWN_T && !need(q)) q += 4;
+            OwnList cur = q < OWN_T ? own_load(a, t0 + q) : OwnList();
+            while (q < OWN_T) {
+                uint32_t qn = q + 4;
+                while (qn < OWN_T && !need(qn)) qn += 4;
+                const OwnList nxt = qn < OWN_T ? own_load(a, t0 + qn) : OwnList();
+                own_merge<RR, TIE, IDD>(a, t0 + q, cur, skey + hoff[q], hcnt[q], last, perm[wave], t);
+                cur = nxt;
+                q = qn;
             }
             __syncthreads();  // the chunk's LDS is reused
             if (last_chunk) break;
