@@ -71,9 +71,9 @@ def build_first_half(nnm, ctx, X, half, K, rho, seed):
     return g0, st
 
 
-def cpu_baseline(nnm, X, half, K, rho, T, sample, seed):
+def cpu_baseline(nnm, X, half, K, rho, T, sample, seed, threads=0):
     """The oracle restatement of the reference J-Merge on `sample` rows (half
-    from each side of the workload), one thread, timed by the oracle."""
+    from each side of the workload), timed by the oracle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     s2 = sample // 2
@@ -231,14 +231,20 @@ def main():
     ap.add_argument("--n", type=int, default=5_000_000)
     ap.add_argument("--d", type=int, default=128)
     ap.add_argument("--centers", type=int, default=256)
-    ap.add_argument("--rank", type=int, default=12)
+    ap.add_argument("--rank", type=int, default=12,
+                    help="per-centre rank-r spread plus noise 2 (SIFT-like low intrinsic dimension); 0: the "
+                         "isotropic mixture, where NN-Descent levels off below recall 0.8 at K = 20")
     ap.add_argument("--K", type=int, default=20)
     ap.add_argument("--rho", type=float, default=0.5)
-    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--max-iters", type=int, default=12, help="largest iteration cap T the sweep tries")
+    ap.add_argument("--fp32-iters", type=int, default=0, help="plain runs: fixed T (0 = smallest T reaching the target)")
+    ap.add_argument("--learned-iters", type=int, default=0,
+                    help="learned: fixed T (0 = sweep T and the plain tail, keep the fastest reaching the target)")
+    ap.add_argument("--plain-tail", type=int, default=3, help="learned with fixed T: plain iterations at the end")
     ap.add_argument("--prec", choices=["headline", "fp32", "bf16", "learned"], default="headline")
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--keep", type=float, default=0.98)
-    ap.add_argument("--plain-tail", type=int, default=3)
     ap.add_argument("--offer-cap", type=int, default=0,
                     help="offer-log records per row (0 = library default); a full log costs redo rounds, "
                          "never offers")
@@ -249,16 +255,14 @@ def main():
     ap.add_argument("--switch", choices=["found", "keep", "renew"], default="found",
                     help="learned: flags when the lists leave the learned space: mark new the entries the "
                          "learned phase found (default), keep every flag, or mark every entry new")
-    ap.add_argument("--learned-iters", type=int, default=8,
-                    help="learned: total iterations T of the learned run (learned phase + plain tail)")
-    ap.add_argument("--graph", action="store_true", help="replay iterations from HIP graphs")
+    ap.add_argument("--no-graph", action="store_true", help="timed runs launch every iteration eagerly")
     ap.add_argument("--recall-rows", type=int, default=20000)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = every host core)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--copy-calib", action="store_true")
     ap.add_argument("--search", action="store_true",
-                    help="also measure batched graph search on the merged graph (DPG layer, seeded flat search)")
+                    help="also measure batched graph search on the merged graph (DPG layer, layered search)")
     ap.add_argument("--search-queries", type=int, default=100_000)
     ap.add_argument("--search-width", type=int, default=1)
     ap.add_argument("--search-top", type=int, default=2048, help="rows of the complete top search layer")
@@ -285,25 +289,23 @@ def main():
     K, n, d = args.K, args.n, args.d
     half = n // 2
     seed = 1
+    rank = args.rank if args.rank > 0 else None
     t0 = time.time()
     Qs = None
     if args.search:
         X, Qs = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4,
-                                   rank=args.rank if args.rank > 0 else None, nq=args.search_queries)
+                                   rank=rank, nq=args.search_queries)
     else:
-        X = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4,
-                               rank=args.rank if args.rank > 0 else None)
+        X = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4, rank=rank)
     log(f"data {n}x{d} in {time.time() - t0:.1f}s")
 
     ctx = nnm.Context(0)
-    gt = rows = None
-    if not args.no_recall:
-        t0 = time.time()
-        rng = np.random.default_rng(7)
-        rows = np.sort(rng.choice(n, args.recall_rows, replace=False)).astype(np.uint32)
-        ctx.set_data(X)
-        gt, _ = ctx.bruteforce(10, rows=rows)
-        log(f"ground truth for {args.recall_rows} rows in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    rng = np.random.default_rng(7)
+    rows = np.sort(rng.choice(n, args.recall_rows, replace=False)).astype(np.uint32)
+    ctx.set_data(X)
+    gt, _ = ctx.bruteforce(10, rows=rows)
+    log(f"ground truth for {args.recall_rows} rows in {time.time() - t0:.1f}s")
     t0 = time.time()
     g0, bst = build_first_half(nnm, ctx, X, half, K, args.rho, seed + 10)
     log(f"first-half build {time.time() - t0:.1f}s: iters {bst['iters']} secs {bst['secs']:.3f}")
@@ -319,18 +321,74 @@ def main():
     del init
 
     precs = {"fp32": nnm.PREC_FP32, "bf16": nnm.PREC_BF16, "learned": nnm.PREC_LEARNED}
-    flags = nnm.FLAG_AUTOTUNE | (nnm.FLAG_GRAPH if args.graph else 0)
+    graph_flag = 0 if args.no_graph else nnm.FLAG_GRAPH
 
-    def run(name):
-        T = args.learned_iters if (name == "learned" and args.learned_iters) else args.iters
-        fl = flags | (nnm.FLAG_LEARNED_RENEW if (name == "learned" and args.switch == "renew") else 0)
-        fl |= nnm.FLAG_LEARNED_FP8 if (name == "learned" and args.fp8) else 0
-        fl |= nnm.FLAG_LEARNED_MARK_FOUND if (name == "learned" and args.switch == "found") else 0
-        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name],
-                               seed=seed, plain_tail=args.plain_tail, flags=fl)
+    def params(name, T, tail, graph):
+        fl = nnm.FLAG_AUTOTUNE | (graph_flag if graph else 0)
+        if name == "learned":
+            fl |= {"renew": nnm.FLAG_LEARNED_RENEW, "found": nnm.FLAG_LEARNED_MARK_FOUND, "keep": 0}[args.switch]
+            fl |= nnm.FLAG_LEARNED_FP8 if args.fp8 else 0
+        p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=args.rho, max_iters=T, prec=precs[name], seed=seed,
+                               plain_tail=tail, flags=fl, offer_cap=args.offer_cap)
         if args.learn_tau > 0:
             p.learn_tau = args.learn_tau
-        p.offer_cap = args.offer_cap
+        return p
+
+    def step(name, p):
+        # the learned run projects every row through the network inside the
+        # timed region (the projection is part of each merge's cost; only the
+        # training is left out)
+        if name == "learned":
+            ctx.mlp_project()
+        # InitSGraph's recompute: FP32 rows for the plain run, bf16 rows (exact here) otherwise
+        r = ctx.graph_restore(recompute=1 if name == "fp32" else 2)
+        i = ctx.merge_init(p)
+        s = ctx.nndescent(p)
+        ctx.merge_finish(True)
+        return r, i, s
+
+    def recall_now():
+        ids, _, _ = ctx.get_graph()
+        return float(nnm.recall(ids, gt, rows, 10)), float(nnm.recall(ids, gt, rows, 10, unique=False))
+
+    sweep_log = []
+
+    def choose(name):
+        """Smallest-time setting of the path whose merged graph reaches the
+        target recall: plain paths the smallest T; the learned path the
+        fastest (T, plain tail) among the smallest T of each tail."""
+        if name != "learned":
+            if args.fp32_iters:
+                return args.fp32_iters, 1
+            for T in range(2, args.max_iters + 1):
+                step(name, params(name, T, 1, False))
+                rec = recall_now()[0]
+                sweep_log.append({"path": name, "T": T, "recall@10": round(rec, 5)})
+                log(f"[sweep {name}] T {T}: recall {rec:.4f}")
+                if rec >= args.target_recall:
+                    return T, 1
+            return args.max_iters, 1
+        if args.learned_iters:
+            return args.learned_iters, args.plain_tail
+        best = None
+        for tail in (2, 3):
+            for T in range(tail + 2, args.max_iters + 1):
+                step(name, params(name, T, tail, False))  # warm the tuned shapes
+                t0 = time.perf_counter()
+                step(name, params(name, T, tail, False))
+                torch.cuda.synchronize()
+                secs = time.perf_counter() - t0
+                rec = recall_now()[0]
+                sweep_log.append({"path": name, "T": T, "plain_tail": tail, "recall@10": round(rec, 5),
+                                  "merge_secs_eager": round(secs, 4)})
+                log(f"[sweep learned] T {T} tail {tail}: recall {rec:.4f} secs {secs:.3f}")
+                if rec >= args.target_recall:
+                    if best is None or secs < best[0]:
+                        best = (secs, T, tail)
+                    break
+        return (best[1], best[2]) if best else (args.max_iters, 3)
+
+    def run(name):
         train = None
         if name == "learned":
             ctx.graph_restore(recompute=True)
@@ -338,72 +396,66 @@ def main():
             train = ctx.mlp_train(steps=args.train_steps, batch=4096, lr=2e-3, keep=args.keep, seed=9)
             train["tau"] = float(ctx.mlp_weights()[5])
             log(f"mlp training: {train}")
-
-        def step():
-            # the learned run projects every row through the network inside
-            # the timed region (the projection is part of each merge's cost;
-            # only the training is left out)
-            if name == "learned":
-                ctx.mlp_project()
-            # InitSGraph's recompute: FP32 rows for the plain run, bf16 rows (exact here) otherwise
-            r = ctx.graph_restore(recompute=1 if name == "fp32" else 2)
-            i = ctx.merge_init(p)
-            s = ctx.nndescent(p)
-            ctx.merge_finish(True)
-            return r, i, s
-
+        T, tail = choose(name)
+        p = params(name, T, tail, True)
         for w in range(args.warmup):
-            r, i, s = step()
-            log(f"[{name}] warmup {w}: descent {s['secs']:.3f}s iters {s['iters']} tile {s['tile_js']}/"
-                f"{s['tile_chunk']}")
+            r, i, s = step(name, p)
+            log(f"[{name}] warmup {w}: descent {s['secs']:.3f}s iters {s['iters']} graph {s['graph_iters']}")
         recs = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            recs.append(step())
+            recs.append(step(name, p))
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) / args.steps * 1e3
-        for k, (r, i, s) in enumerate(recs):
+        rec, rec_ref = recall_now()
+        for k, (r, i, s) in enumerate(recs[:3]):
             log(f"[{name}] step {k}: restore {r['secs']:.3f}s init {i['secs']:.3f}s descent {s['secs']:.3f}s "
-                f"join {s['join_secs']:.3f}s (short {s['join_main_secs']:.3f}s) iters {s['iters']} "
-                f"deliver {s['deliver_secs']:.3f}s dists {s['dists']} exact {s['reranked']} offers {s['offers']} "
-                f"redo {s['redo_rows']}/{s['redo_rounds']} updates {s['updates']}")
-        dists = float(np.mean([a["init_dists"] + b["init_dists"] + c["dists"] for a, b, c in recs]))
-        # what the count holds: every pair the join resolves (its exact distance
-        # on the plain paths; its 32-d learned estimate in learned iterations)
-        # plus the set-up's exact distances; the learned run's exact and
-        # projected parts are reported beside it
-        projected = float(np.mean([c["learned_dists"] for _, _, c in recs]))
+                f"iters {s['iters']} dists {s['dists']} offers {s['offers']} redo {s['redo_rows']}/{s['redo_rounds']} "
+                f"updates {s['updates']}")
+        # one eager step for the kernel timings (HIP events on the library's
+        # stream around the join and the delivery; graph replays carry none)
+        er, ei, es = step(name, params(name, T, tail, False))
+        log(f"[{name}] eager: descent {es['secs']:.3f}s join {es['join_secs']:.3f}s (short {es['join_main_secs']:.3f}s) "
+            f"deliver {es['deliver_secs']:.3f}s")
+        # distances: exact ones (the reference's nCmps: join distances on the
+        # plain paths, set-up distances, the learned run's exact list
+        # distances at the switch) and the learned phase's projected 32-d
+        # estimates, counted apart
         exact = float(np.mean([a["init_dists"] + b["init_dists"] + c["dists"] - c["learned_dists"] + c["reranked"]
                                for a, b, c in recs]))
-        res = {"merge_secs": round(ms * 1e-3, 4), "gdist_s": round(dists / (ms * 1e-3) / 1e9, 3),
-               "gdist_s_counts": "pairs resolved per second (join pairs + set-up distances)",
-               "dists_per_step": int(dists), "exact_dists_per_step": int(exact),
-               "projected_dists_per_step": int(projected),
-               "exact_gdist_s": round(exact / (ms * 1e-3) / 1e9, 3), "iters": recs[-1][2]["iters"],
-               "exact_rerank_per_step": int(np.mean([c["reranked"] for _, _, c in recs])),
+        projected = float(np.mean([c["learned_dists"] for _, _, c in recs]))
+        pairs = float(np.mean([a["init_dists"] + b["init_dists"] + c["dists"] for a, b, c in recs]))
+        secs = ms * 1e-3
+        res = {"merge_secs": round(secs, 4), "T": T, "recall@10": round(rec, 5), "recall@10_refcount": round(rec_ref, 5),
+               "exact_gdist_s": round(exact / secs / 1e9, 3), "exact_dists_per_step": int(exact),
+               "pairs_resolved_gpair_s": round(pairs / secs / 1e9, 3), "projected_dists_per_step": int(projected),
+               "graph_iters": recs[-1][2]["graph_iters"], "iters": recs[-1][2]["iters"],
                "learned_iters": recs[-1][2]["learned_iters"], "plain_iters": recs[-1][2]["plain_iters"],
+               "offers_per_step": int(recs[-1][2]["offers"]), "redo_rows": int(recs[-1][2]["redo_rows"]),
+               "eager": {"merge_descent_secs": round(es["secs"], 4), "join_secs": round(es["join_secs"], 4),
+                         "join_short_secs": round(es["join_main_secs"], 4),
+                         "deliver_secs": round(es["deliver_secs"], 4)},
                "tile": [recs[-1][2]["tile_js"], recs[-1][2]["tile_chunk"]]}
-        if train is not None:
-            res["train"] = {k: round(v, 5) for k, v in train.items()}
+        if name == "learned":
+            res["plain_tail"] = tail
             res["learn_tau"] = round(float(p.learn_tau), 4)
-        # dominant kernel: the packed short-list join, averaged per launch
-        st = [c for _, _, c in recs]
-        launches = sum(c["iters"] for c in st)
-        secs = sum(c["join_main_secs"] for c in st)
-        byts = sum(c["join_main_bytes"] for c in st)
-        flops = sum(c["join_main_flops"] for c in st)
-        if secs > 0:
-            gbs = byts / secs / 1e9
-            tfs = flops / secs / 1e12
+            res["train"] = {k: round(v, 5) for k, v in train.items()}
+        # dominant kernel: the packed short-list join of the eager step,
+        # averaged per launch
+        launches = es["iters"]
+        if es["join_main_secs"] > 0 and launches:
+            gbs = es["join_main_bytes"] / es["join_main_secs"] / 1e9
+            tfs = es["join_main_flops"] / es["join_main_secs"] / 1e12
             peak_c = PEAK["fp32"] if name == "fp32" else PEAK["bf16"]
             res["roofline"] = {
                 "bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
                 "kernel": MAIN_KERNEL[name] + "...> (packed short-list join)",
-                "launch_ms": round(secs / launches * 1e3, 3),
-                "alg_bytes_per_launch": int(byts / launches), "alg_flops_per_launch": int(flops / launches),
+                "launch_ms": round(es["join_main_secs"] / launches * 1e3, 3),
+                "alg_bytes_per_launch": int(es["join_main_bytes"] / launches),
+                "alg_flops_per_launch": int(es["join_main_flops"] / launches),
                 "compute_achieved_tflops": round(tfs, 2), "compute_peak_tflops": peak_c,
                 "compute_unit": "fp32 vector ALU" if name == "fp32" else "bf16 MFMA",
                 "compute_frac": round(tfs / peak_c, 4)}
@@ -416,10 +468,6 @@ def main():
         e = ctx.dist_error()
         res["fp64_error"] = {"max_rel": e["max_rel"], "mean_rel": e["mean_rel"], "max_abs": e["max_abs"],
                              "entries": int(e["entries"]), "order_flips": int(e["order_flips"])}
-        if gt is not None:
-            ids, _, _ = ctx.get_graph()
-            res["recall@10"] = round(float(nnm.recall(ids, gt, rows, 10)), 5)
-            res["recall@10_refcount"] = round(float(nnm.recall(ids, gt, rows, 10, unique=False)), 5)
         log(f"[{name}] {json.dumps(res)}")
         return res
 
@@ -435,33 +483,46 @@ def main():
         search = run_search(nnm, ctx, Qs, args)
     ctx.close()
 
-    workload = (f"J-Merge {n}x{d} sift-like rank-{args.rank} ({half} built + {n - half} raw), K={K}, "
-                f"rho={args.rho}, T={args.iters}")
-    out = {"metric": METRIC, "value": ch["gdist_s"], "unit": "Gdist/s", "n_gpus": 1, "steps": args.steps,
+    gen = f"rank-{args.rank} spread" if rank else "isotropic"
+    workload = (f"J-Merge {n}x{d} integer mixture ({args.centers} centres, sigma 20, {gen}; {half} built + "
+                f"{n - half} raw), K={K}, rho={args.rho}, each path at its smallest T reaching recall@10 >= "
+                f"{args.target_recall}")
+    out = {"metric": METRIC, "value": ch["exact_gdist_s"], "unit": "Gdist/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(ch["merge_secs"] * 1e3, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f32" if chosen == "fp32" else "bf16",
            "data": "synthetic (seeded generator nnm_data.sift_like)",
-           "config": {"workload": workload, "n": n, "d": d, "K": K, "rho": args.rho, "max_iters": args.iters,
+           "config": {"workload": workload, "n": n, "d": d, "K": K, "rho": args.rho, "centers": args.centers,
+                      "rank": args.rank, "target_recall": args.target_recall, "T": ch["T"],
                       "offer_cap": args.offer_cap, "path": chosen, "parallelism": "single GPU",
-                      "learned": {"T": args.learned_iters, "plain_tail": args.plain_tail,
+                      "hip_graph": not args.no_graph,
+                      "learned": {"plain_tail": runs.get("learned", {}).get("plain_tail"),
                                   "learn_tau": args.learn_tau if args.learn_tau > 0 else "library default (0.6)",
                                   "switch": args.switch, "fp8_rows": bool(args.fp8)}},
-           "value_counts": "pairs resolved per second: join pairs (exact distances, or 32-d learned estimates in "
-                           "the learned phase) plus set-up distances; exact-only rate in runs.*.exact_gdist_s",
+           "value_counts": "exact distances per second of the chosen form (join distances of the plain iterations, "
+                           "set-up distances, exact list distances at the switch); the learned phase's projected "
+                           "32-d estimates are counted apart (runs.*.projected_dists_per_step, pairs_resolved_gpair_s)",
            "merge_secs": ch["merge_secs"], "recall@10": ch.get("recall@10"), "fp64_error": ch.get("fp64_error"),
            "upload": {"bytes": int(X.nbytes), "secs": round(upload_secs, 4),
                       "note": "nnm_set_data host -> HBM (pinned staging, with the bf16-exactness scan); "
                               "outside the timed region"},
-           "roofline": ch.get("roofline"), "runs": runs, "first_half_build": {k: bst[k] for k in
-                                                                             ("iters", "secs", "dists")}}
+           "roofline": ch.get("roofline"), "runs": runs, "sweep": sweep_log,
+           "first_half_build": {k: bst[k] for k in ("iters", "secs", "dists")}}
     if search is not None:
         out["search"] = search
     if "fp32" in runs and chosen != "fp32":
-        out["speedup_vs_plain_fp32"] = round(runs["fp32"]["merge_secs"] / ch["merge_secs"], 3)
+        f = runs["fp32"]["merge_secs"]
+        out["speedup_vs_plain_fp32"] = round(f / ch["merge_secs"], 3)
+        if chosen == "learned":
+            tr = runs["learned"]["train"]["secs"]
+            out["training"] = {"secs": tr, "per_merge_with_training_secs": round(ch["merge_secs"] + tr, 4),
+                               "speedup_with_training": round(f / (ch["merge_secs"] + tr), 3),
+                               "merges_to_amortise": round(tr / (f - ch["merge_secs"]), 2)
+                               if f > ch["merge_secs"] else None}
     if not args.no_cpu:
         t0 = time.time()
-        out["cpu_baseline"] = cpu_baseline(nnm, X, half, K, args.rho, args.iters, args.cpu_sample, seed)
+        out["cpu_baseline"] = cpu_baseline(nnm, X, half, K, args.rho, runs["fp32"]["T"] if "fp32" in runs else 10,
+                                           args.cpu_sample, seed, args.cpu_threads)
         log(f"cpu baseline {time.time() - t0:.1f}s")
     print(json.dumps(out), flush=True)
 
