@@ -286,6 +286,20 @@ int nnm_merge_finish(nnm_ctx* ctx, int dedup);
 int nnm_bruteforce_knn(nnm_ctx* ctx, const float* Q, const uint32_t* rows, uint64_t nq,
                        uint32_t k, int exclude_self, uint32_t* ids, float* dist);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid allocation): nnm_bruteforce_knn with every buffer on the device and
+ * no allocation of any kind.  The nq query rows are row numbers of the
+ * context's data (rows_dev, device memory; a row past the data gets ids
+ * 0xFFFFFFFF and infinite distances), excluding each query's own row when
+ * exclude_self is set; the nq x k results go to ids_dev / dist_dev (device
+ * memory).  The per-range partial lists live in a fixed 16 MiB device array
+ * per GPU, so nq is at most 65536 (larger requests return NNM_ERR_INVALID).
+ * Calls that overlap on one GPU race on that array.  Returns after the
+ * results are written. */
+int nnm_bruteforce_knn_noalloc(nnm_ctx* ctx, const uint32_t* rows_dev, uint64_t nq, uint32_t k, int exclude_self,
+                               uint32_t* ids_dev, float* dist_dev);
+
 /* Squared L2 between row pairs (a[i], b[i]) in the library's FP32 order:
  * the DISTFUNC call of src/abstractgraph.h:11-12 batched. */
 int nnm_pair_dists(nnm_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t m, float* out);

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X,
 
     float qv[LDMAX];
     uint32_t self = 0xFFFFFFFFu;
+    bool rowok = active;  // a query row past the data gets an empty list
     {
         const float* src = nullptr;
         if (active) {
@@ -65,14 +66,15 @@ __global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X,
                 src = Q + q * ld;
             } else {
                 const uint32_t r = rows[q];
-                src = X + (uint64_t)r * ld;
+                rowok = r < n;
+                src = X + (uint64_t)(rowok ? r : 0u) * ld;
                 if (exclude_self) self = r;
             }
         }
 #pragma unroll
         for (int c = 0; c < LDMAX; c += 4) {
             float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (active && (uint32_t)c < ld) t = *reinterpret_cast<const float4*>(src + c);
+            if (rowok && (uint32_t)c < ld) t = *reinterpret_cast<const float4*>(src + c);
             qv[c] = t.x; qv[c + 1] = t.y; qv[c + 2] = t.z; qv[c + 3] = t.w;
         }
     }
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(256) void k_bruteforce(const float* __restrict__ X,
                 *reinterpret_cast<const float4*>(X + (r0 + rr) * ld + cc * 4);
         }
         __syncthreads();
-        if (!active) continue;
+        if (!rowok) continue;
         for (uint32_t rr = 0; rr < nt; ++rr) {
             const float* tr = &tile[rr * LDMAX];
             float acc = 0.f;
@@ -145,6 +147,45 @@ __global__ void k_bf_merge(uint64_t nq, uint32_t nsplit, uint32_t k, const float
         if ((uint32_t)i < k) { out_d[q * k + i] = td[i]; out_i[q * k + i] = ti[i]; }
 }
 
+// Split count of a brute-force launch: enough (query block, range) blocks to
+// fill the CUs a few times, at most 512 ranges, at most one per tile.
+static uint32_t bf_splits(nnm_ctx* c, uint64_t nq) {
+    const uint64_t qblocks = (nq + 255) / 256;
+    const uint64_t want = (uint64_t)c->num_cus * 4;
+    uint32_t nsplit = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want / qblocks, 512));
+    return (uint32_t)std::min<uint64_t>(nsplit, (c->n + BF_BT - 1) / BF_BT);
+}
+
+// Launch of the range kernel and the merge over caller-provided buffers.
+static int bf_launch(nnm_ctx* c, const float* dQ, const uint32_t* dR, uint64_t nq, uint32_t k, int exclude_self,
+                     uint32_t nsplit, float* pd, uint32_t* pi, float* od, uint32_t* oi) {
+    const uint64_t qblocks = (nq + 255) / 256;
+    dim3 grid((uint32_t)qblocks, nsplit);
+    if (c->metric == NNM_METRIC_COSINE && !c->sumsq) return nnm_fail(c, NNM_ERR_STATE, "no row norms");
+#define NNM_BF(LDM, M)                                                                                         \
+    hipLaunchKernelGGL((k_bruteforce<LDM, M>), grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,   \
+                       exclude_self, nsplit, pd, pi, c->sumsq)
+#define NNM_BF_LD(M)                       \
+    if (c->ld <= 64) NNM_BF(64, M);        \
+    else if (c->ld <= 128) NNM_BF(128, M); \
+    else NNM_BF(256, M);
+    switch (c->metric) {
+        case NNM_METRIC_L1: NNM_BF_LD(NNM_METRIC_L1) break;
+        case NNM_METRIC_COSINE: NNM_BF_LD(NNM_METRIC_COSINE) break;
+        case NNM_METRIC_CHI2: NNM_BF_LD(NNM_METRIC_CHI2) break;
+        default: NNM_BF_LD(NNM_METRIC_L2) break;
+    }
+#undef NNM_BF_LD
+#undef NNM_BF
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return nnm_hip_fail(c, e, "k_bruteforce launch");
+    hipLaunchKernelGGL(k_bf_merge, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, c->stream, nq, nsplit, k, pd,
+                       pi, od, oi);
+    e = hipGetLastError();
+    if (e != hipSuccess) return nnm_hip_fail(c, e, "k_bf_merge launch");
+    return NNM_OK;
+}
+
 extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* rows, uint64_t nq, uint32_t k,
                                   int exclude_self, uint32_t* ids, float* dist) {
     if (!c || !ids || !dist || nq == 0) return nnm_fail(c, NNM_ERR_INVALID, "nnm_bruteforce_knn: bad arguments");
@@ -156,10 +197,7 @@ extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* ro
         for (uint64_t i = 0; i < nq; ++i)
             if (rows[i] >= c->n) return nnm_fail(c, NNM_ERR_INVALID, "query row out of range");
     NNM_HIP(c, hipSetDevice(c->device));
-    const uint64_t qblocks = (nq + 255) / 256;
-    uint64_t want = (uint64_t)c->num_cus * 4;
-    uint32_t nsplit = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want / qblocks, 512));
-    nsplit = (uint32_t)std::min<uint64_t>(nsplit, (c->n + BF_BT - 1) / BF_BT);
+    const uint32_t nsplit = bf_splits(c, nq);
     float* dQ = nullptr;
     uint32_t* dR = nullptr;
     float *pd = nullptr, *od = nullptr;
@@ -185,32 +223,57 @@ extern "C" int nnm_bruteforce_knn(nnm_ctx* c, const float* Q, const uint32_t* ro
         cleanup();
         return rc;
     }
-    dim3 grid((uint32_t)qblocks, nsplit);
-    if (c->metric == NNM_METRIC_COSINE && !c->sumsq) { cleanup(); return nnm_fail(c, NNM_ERR_STATE, "no row norms"); }
-#define NNM_BF(LDM, M)                                                                                         \
-    hipLaunchKernelGGL((k_bruteforce<LDM, M>), grid, dim3(256), 0, c->stream, c->X, c->ld, c->n, dQ, dR, nq,   \
-                       exclude_self, nsplit, pd, pi, c->sumsq)
-#define NNM_BF_LD(M)                       \
-    if (c->ld <= 64) NNM_BF(64, M);        \
-    else if (c->ld <= 128) NNM_BF(128, M); \
-    else NNM_BF(256, M);
-    switch (c->metric) {
-        case NNM_METRIC_L1: NNM_BF_LD(NNM_METRIC_L1) break;
-        case NNM_METRIC_COSINE: NNM_BF_LD(NNM_METRIC_COSINE) break;
-        case NNM_METRIC_CHI2: NNM_BF_LD(NNM_METRIC_CHI2) break;
-        default: NNM_BF_LD(NNM_METRIC_L2) break;
-    }
-#undef NNM_BF_LD
-#undef NNM_BF
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) { cleanup(); return nnm_hip_fail(c, e, "k_bruteforce launch"); }
-    hipLaunchKernelGGL(k_bf_merge, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, c->stream, nq, nsplit, k, pd,
-                       pi, od, oi);
-    e = hipGetLastError();
-    if (e != hipSuccess) { cleanup(); return nnm_hip_fail(c, e, "k_bf_merge launch"); }
+    rc = bf_launch(c, dQ, dR, nq, k, exclude_self, nsplit, pd, pi, od, oi);
+    if (rc != NNM_OK) { cleanup(); return rc; }
     rc = nnm_d2h(c, ids, oi, nq * k * 4);
     if (rc == NNM_OK) rc = nnm_d2h(c, dist, od, nq * k * 4);
     cleanup();
     if (rc != NNM_OK) return rc;
+    return NNM_OK;
+}
+
+// ------------------------------------------------------------ no allocation
+//
+// nnm_bruteforce_knn_noalloc keeps the per-range partial lists in this
+// file-scope device array instead of allocations.  Host code uses the array
+// only through its device address, taken once per device with that device
+// current (nnm_bf_scratch_init, called by nnm_create); the name itself is a
+// host stand-in, not a device address.  Every range kernel writes all its
+// partial entries before the merge kernel reads them, so nothing in the
+// array has to start at zero.
+static __device__ __align__(256) uint8_t scratch[NNM_NOALLOC_SCRATCH_BYTES];
+
+#define NNM_MAX_DEVICES 64
+static void* g_scratch_ptr[NNM_MAX_DEVICES];
+
+int nnm_bf_scratch_init(nnm_ctx* c) {
+    if (c->device < 0 || c->device >= NNM_MAX_DEVICES) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "device index");
+    if (g_scratch_ptr[c->device]) return NNM_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return nnm_hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[c->device] = p;
+    return NNM_OK;
+}
+
+extern "C" int nnm_bruteforce_knn_noalloc(nnm_ctx* c, const uint32_t* rows_dev, uint64_t nq, uint32_t k,
+                                          int exclude_self, uint32_t* ids_dev, float* dist_dev) {
+    if (!c || !rows_dev || !ids_dev || !dist_dev || nq == 0)
+        return nnm_fail(c, NNM_ERR_INVALID, "nnm_bruteforce_knn_noalloc: bad arguments");
+    if (!c->X) return nnm_fail(c, NNM_ERR_STATE, "no data");
+    if (k == 0 || k > BF_KMAX) return nnm_fail(c, NNM_ERR_INVALID, "k must be in [1, 32]");
+    if (c->ld > 256) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "brute force supports d <= 256");
+    void* base = c->device >= 0 && c->device < NNM_MAX_DEVICES ? g_scratch_ptr[c->device] : nullptr;
+    if (!base) return nnm_fail(c, NNM_ERR_STATE, "no scratch address for this device (context not created)");
+    const uint64_t per_split = nq * BF_KMAX * (sizeof(float) + sizeof(uint32_t));
+    if (per_split > NNM_NOALLOC_SCRATCH_BYTES)
+        return nnm_fail(c, NNM_ERR_INVALID, "nnm_bruteforce_knn_noalloc: nq above the scratch limit (65536 queries)");
+    uint32_t nsplit = bf_splits(c, nq);
+    nsplit = (uint32_t)std::min<uint64_t>(nsplit, NNM_NOALLOC_SCRATCH_BYTES / per_split);
+    NNM_HIP(c, hipSetDevice(c->device));
+    float* pd = reinterpret_cast<float*>(base);
+    uint32_t* pi = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(base) + nq * nsplit * BF_KMAX * sizeof(float));
+    NNM_TRY(bf_launch(c, nullptr, rows_dev, nq, k, exclude_self, nsplit, pd, pi, dist_dev, ids_dev));
+    NNM_HIP(c, hipStreamSynchronize(c->stream));
     return NNM_OK;
 }

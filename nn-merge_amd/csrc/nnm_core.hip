@@ -232,6 +232,8 @@ extern "C" int nnm_create(nnm_ctx** out, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     if (alloc_n(c, &c->counters, CNT_N) != NNM_OK) { delete c; return NNM_ERR_NOMEM; }
+    // the allocation-free brute force's scratch address on this device
+    if (nnm_bf_scratch_init(c) != NNM_OK) { nnm_destroy(c); return NNM_ERR_HIP; }
     *out = c;
     return NNM_OK;
 }

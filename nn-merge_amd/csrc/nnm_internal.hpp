@@ -705,6 +705,11 @@ __device__ inline void offer_redo(const OfferSink& s, bool p, uint32_t v) {
     }
 }
 
+// Device scratch of nnm_bruteforce_knn_noalloc (bytes, one array per GPU)
+// and the set-up of its per-device address (nnm_bruteforce.hip).
+#define NNM_NOALLOC_SCRATCH_BYTES (16u << 20)
+int nnm_bf_scratch_init(nnm_ctx* c);
+
 // Offer log and delivery (nnm_offers.hip).
 int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p);
 void nnm_offer_free(nnm_ctx* c);

@@ -102,6 +102,7 @@ _SIGS = {
     "nnm_merge_finish": (C.c_int, [_vp, C.c_int]),
     "nnm_bruteforce_knn": (C.c_int, [_vp, _vp, _vp, C.c_uint64, C.c_uint32, C.c_int, _vp, _vp]),
     "nnm_pair_dists": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp]),
+    "nnm_bruteforce_knn_noalloc": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_int, _vp, _vp]),
     "nnm_graph_save": (C.c_int, [_vp]),
     "nnm_graph_restore": (C.c_int, [_vp, C.c_int, C.POINTER(Stats)]),
     "nnm_mlp_init": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint64]),
@@ -293,6 +294,12 @@ class Context:
         self._chk(lib().nnm_bruteforce_knn(self.h, _ptr(Q), _ptr(r), nq, k, 1 if exclude_self else 0,
                                            _ptr(ids), _ptr(dist)))
         return ids, dist
+
+    def bruteforce_noalloc(self, k, rows_dev, nq, ids_dev, dist_dev, exclude_self=True):
+        """nnm_bruteforce_knn_noalloc on device buffers given by address
+        (rows_dev: nq uint32 row numbers; ids_dev / dist_dev: nq x k)."""
+        self._chk(lib().nnm_bruteforce_knn_noalloc(self.h, C.c_void_p(rows_dev), nq, k, 1 if exclude_self else 0,
+                                                   C.c_void_p(ids_dev), C.c_void_p(dist_dev)))
 
     # ---- learned projection
     def mlp_init(self, hidden=256, d_out=32, seed=1):
