@@ -24,11 +24,19 @@
 //     drops repeated ids (the product's default), `dedup = 0` keeps the
 //     reference's duplicates.
 //
-// Parity unpinned: the delta termination test (the mean fraction of new list
-// entries against 0.002, src/abstractmerge.cpp:171-180) uses a Boost
-// accumulator in the reference, which is not vendored; it is restated here
-// as the plain arithmetic mean, and no fixture of the reference pins it.
-//
+// Pins (tests/test_oracle.py).  The delta test (the mean fraction of new
+// list entries against 0.002, src/abstractmerge.cpp:171-180) uses a Boost
+// accumulator in the reference, which is not vendored; it is restated as the
+// plain arithmetic mean and pinned by closed forms: on a complete graph
+// with forward sampling rho, delta after one iteration is (K - S) / K,
+// equals the flagged entries over N K, and the loop ends when it falls
+// strictly below the threshold.  The join's pair rules (KGraph, S-Merge,
+// J-Merge) are pinned by closed-form one-iteration pair counts, the reverse
+// lists by a hand-derived four-point case, merge2one by brute force over
+// pool and parked entries (both dedup settings), getRndSds against numpy's
+// MT19937 stream over many (seed, k, N), distances bit for bit against the
+// reference's own routines built from its sources.
+
 // Build: see oracle/Makefile (g++ -O2 -std=c++11 -ffp-contract=off, like the
 // reference's src/Makefile:33 flags, contraction off so every float op rounds
 // as the reference's non-FMA x86-64 code does).
@@ -439,6 +447,84 @@ void orc_kgraph(const float* X, uint32_t n, uint32_t d, uint32_t K, float rho,
         }
 }
 
+// merge2one, src/abstractmerge.cpp:232-277: the first K entries of the
+// two-way merge of the pool (K entries, sorted) and the parked entries c
+// (sorted); the pool wins distance ties.  dedup drops an id already taken.
+static void merge2one(std::vector<ONb>& pool, const std::vector<ONb>& c, uint32_t K, int dedup) {
+    std::vector<ONb> tmp(K);
+    uint32_t a = 0, b = 0, k = 0;
+    while (k < K && (a < K || b < c.size())) {
+        const float da = a < K ? pool[a].dist : INFINITY;
+        const float db = b < c.size() ? c[b].dist : INFINITY;
+        ONb pick;
+        if (a < K && da <= db) pick = pool[a++];     // pool wins ties
+        else pick = c[b++];
+        if (dedup) {
+            bool dup = false;
+            for (uint32_t t = 0; t < k; ++t)
+                if (tmp[t].id == pick.id) { dup = true; break; }
+            if (dup) continue;
+        }
+        tmp[k++] = pick;
+    }
+    for (uint32_t j = 0; j < k; ++j) pool[j] = tmp[j];
+}
+
+// merge2one over arrays: pool n x K and parked entries n x cut (ids, dists,
+// each row sorted by distance; a parked id of 0xFFFFFFFF ends its row).
+void orc_merge2one(uint32_t n, uint32_t K, uint32_t* pool_ids, float* pool_dist, uint32_t cut,
+                   const uint32_t* cpn_ids, const float* cpn_dist, int dedup) {
+    std::vector<ONb> pool(K), c;
+    for (uint32_t v = 0; v < n; ++v) {
+        for (uint32_t j = 0; j < K; ++j)
+            pool[j] = ONb{pool_ids[(uint64_t)v * K + j], pool_dist[(uint64_t)v * K + j], 0, 0};
+        c.clear();
+        for (uint32_t j = 0; j < cut && cpn_ids[(uint64_t)v * cut + j] != 0xFFFFFFFFu; ++j)
+            c.push_back(ONb{cpn_ids[(uint64_t)v * cut + j], cpn_dist[(uint64_t)v * cut + j], 0, 0});
+        merge2one(pool, c, K, dedup);
+        for (uint32_t j = 0; j < K; ++j) {
+            pool_ids[(uint64_t)v * K + j] = pool[j].id;
+            pool_dist[(uint64_t)v * K + j] = pool[j].dist;
+        }
+    }
+}
+
+// The NN-Descent loop (nn_descent) from lists given by the caller: ids n x K
+// (each row's entries distinct, self excluded), flags n x K, and the
+// subgraph label of every row (gid, 0xFFFFFFFF for raw rows); distances are
+// computed here and each row is sorted by distance first.  rule_all = 1 is
+// KGraph::join's rule, 0 the merge rule.  The lists after the loop come
+// back in ids / dist / flags.
+void orc_nndescent_lists(const float* X, uint32_t n, uint32_t d, uint32_t K, uint32_t* ids, float* dist,
+                         uint8_t* flags, const uint32_t* gid_of, int rule_all, float rho, uint32_t rev_cap,
+                         uint32_t max_iters, float delta, uint32_t seed, orc_stats* st) {
+    std::memset(st, 0, sizeof(*st));
+    OParams p{K, rho, rev_cap, max_iters, delta, seed};
+    std::mt19937 rng(seed);
+    std::vector<OList> g(n);
+    std::vector<uint32_t> gid(gid_of, gid_of + n);
+    for (uint32_t v = 0; v < n; ++v) {
+        g[v].pool.resize(K);
+        for (uint32_t j = 0; j < K; ++j) {
+            ONb& nb = g[v].pool[j];
+            nb.id = ids[(uint64_t)v * K + j];
+            nb.dist = dist32(X + (uint64_t)v * d, X + (uint64_t)nb.id * d, d);
+            nb.flag = flags[(uint64_t)v * K + j];
+            nb.gd = gid[nb.id];
+        }
+        std::stable_sort(g[v].pool.begin(), g[v].pool.end(), nb_less);
+    }
+    double t0 = now_s();
+    nn_descent(X, d, g, gid, rule_all, p, st, rng);
+    st->secs = now_s() - t0;
+    for (uint32_t v = 0; v < n; ++v)
+        for (uint32_t j = 0; j < K; ++j) {
+            ids[(uint64_t)v * K + j] = g[v].pool[j].id;
+            dist[(uint64_t)v * K + j] = g[v].pool[j].dist;
+            flags[(uint64_t)v * K + j] = g[v].pool[j].flag;
+        }
+}
+
 // Partitioned merge: AbstractMerge::InitSGraph (src/abstractmerge.cpp:55-76),
 // Smerge::InitialGraph (src/smerge.cpp:94-209) / Jmerge::InitialGraph
 // (src/jmerge.cpp:83-176), AbstractMerge::nnDescent with the merge pair rule,
@@ -529,28 +615,8 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
         }
     }
     nn_descent(X, d, g, gid, 0, p, st, rng);
-    // merge2one, src/abstractmerge.cpp:232-277.
-    std::vector<ONb> tmp(K);
-    for (uint32_t v = 0; v < n; ++v) {
-        if (cpn[v].empty()) continue;
-        const std::vector<ONb>& c = cpn[v];
-        uint32_t a = 0, b = 0, k = 0;
-        while (k < K && (a < K || b < c.size())) {
-            const float da = a < K ? g[v].pool[a].dist : INFINITY;
-            const float db = b < c.size() ? c[b].dist : INFINITY;
-            ONb pick;
-            if (a < K && da <= db) pick = g[v].pool[a++];     // pool wins ties
-            else pick = c[b++];
-            if (dedup) {
-                bool dup = false;
-                for (uint32_t t = 0; t < k; ++t)
-                    if (tmp[t].id == pick.id) { dup = true; break; }
-                if (dup) continue;
-            }
-            tmp[k++] = pick;
-        }
-        for (uint32_t j = 0; j < K; ++j) g[v].pool[j] = tmp[j];
-    }
+    for (uint32_t v = 0; v < n; ++v)
+        if (!cpn[v].empty()) merge2one(g[v].pool, cpn[v], K, dedup);
     st->secs = now_s() - t0;
     for (uint32_t v = 0; v < n; ++v)
         for (uint32_t j = 0; j < K; ++j) {

@@ -75,6 +75,12 @@ def lib():
         L.orc_merge.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u8p,
                                 u32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float,
                                 C.c_uint32, C.c_int, u32p, f32p, C.POINTER(OrcStats)]
+        L.orc_merge2one.restype = None
+        L.orc_merge2one.argtypes = [C.c_uint32, C.c_uint32, u32p, f32p, C.c_uint32, u32p, f32p, C.c_int]
+        L.orc_nndescent_lists.restype = None
+        L.orc_nndescent_lists.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p, f32p, u8p, u32p,
+                                          C.c_int, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                                          C.POINTER(OrcStats)]
         vp, u64p = C.c_void_p, C.POINTER(C.c_uint64)
         L.orc_search_layered.restype = None
         L.orc_search_layered.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(vp), C.POINTER(vp),
@@ -232,6 +238,37 @@ def merge(X, sub_begin, sub_built, init_ids, K=20, cut=None, rho=1.0, rev_cap=10
                     _p(ii, C.c_uint32), cut, rho, rev_cap, max_iters, delta, seed,
                     1 if dedup else 0, _p(ids, C.c_uint32), _p(dist, C.c_float), C.byref(st))
     return ids, dist, st.as_dict()
+
+
+def merge2one(pool_ids, pool_dist, cpn_ids, cpn_dist, dedup=True):
+    """merge2one per row: the first K of the two-way merge of the pool rows
+    (n x K) and the parked rows (n x cut, an id of RAW ends a row)."""
+    ids = np.ascontiguousarray(pool_ids, np.uint32).copy()
+    dist = np.ascontiguousarray(pool_dist, np.float32).copy()
+    ci = np.ascontiguousarray(cpn_ids, np.uint32)
+    cd = np.ascontiguousarray(cpn_dist, np.float32)
+    n, K = ids.shape
+    lib().orc_merge2one(n, K, _p(ids, C.c_uint32), _p(dist, C.c_float), ci.shape[1],
+                        _p(ci, C.c_uint32), _p(cd, C.c_float), 1 if dedup else 0)
+    return ids, dist
+
+
+def nndescent_lists(X, ids, flags, gid_of, rule_all=False, rho=1.0, rev_cap=100, max_iters=0,
+                    delta=0.002, seed=1):
+    """The NN-Descent loop from given lists (n x K ids and flags) and row
+    labels (RAW for raw rows); returns ids, dists, flags, stats."""
+    X = _f32(X)
+    n, d = X.shape
+    ids = np.ascontiguousarray(ids, np.uint32).copy()
+    K = ids.shape[1]
+    flags = np.ascontiguousarray(flags, np.uint8).copy()
+    gid = np.ascontiguousarray(gid_of, np.uint32)
+    dist = np.empty((n, K), np.float32)
+    st = OrcStats()
+    lib().orc_nndescent_lists(_p(X, C.c_float), n, d, K, _p(ids, C.c_uint32), _p(dist, C.c_float),
+                              _p(flags, C.c_uint8), _p(gid, C.c_uint32), 1 if rule_all else 0, rho,
+                              rev_cap, max_iters, delta, seed, C.byref(st))
+    return ids, dist, flags, st.as_dict()
 
 
 def recall(ids, rows, gt, k=10, unique=True):

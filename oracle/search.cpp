@@ -17,11 +17,13 @@
 //     "augmented" arrays (deg, ids...) + knnLocs (src/lysearch.cpp:112-186);
 //     the content is the same.
 //
-// Parity unpinned: the order-free DPG reverse pass (orc_dpg_par) differs from
-// the reference's in-place pass (orc_dpg_ref) only where a row's list gained
-// reverse entries before its own turn; it is checked against orc_dpg_ref by
-// edge overlap (>= 0.99, measured 1.0), not bit for bit, and the reference's
-// own DPG code cannot be built here (it needs Boost).
+// Pins (tests/test_oracle_search.py): the reference's own DPG code cannot be
+// built here (it needs Boost), so orc_dpg_ref is pinned by closed forms:
+// collinear rows give the path graph, a centre with axis rows gives a star
+// whose centre keeps its maxM nearest, and a four-row case with ties,
+// derived by hand, shows the in-place reverse pass dropping the one edge
+// the order-free pass (orc_dpg_par, what the GPU builds) adds.  On sampled
+// data the two passes share at least 99% of edges (measured 100%).
 
 #include <algorithm>
 #include <cstdint>

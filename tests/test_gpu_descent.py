@@ -246,12 +246,24 @@ def test_join_distances_exact(ctx, d):
     assert np.array_equal(ex, dist.ravel().astype(np.float64))
 
 
-def test_merge_finish_matches_merge2one_with_dedup(ctx):
-    """merge2one (src/abstractmerge.cpp:232-277) with the documented de-dup:
-    the parked far entries of each built list (cut = K/2 of them) are merged
-    back with the final pool by (dist, id), parked ids already in the pool
-    dropped.  The expected rows are computed here from the pool read before
-    the finish and the parked entries read after set-up.  The descent joins
+def _same_up_to_ties(gi, gd, oi, od):
+    """Equal distance rows, and equal ids wherever the distance is below the
+    row's last one (entries tied at the cut may differ: the library orders
+    ties by id, merge2one's two-way merge puts the pool first)."""
+    assert np.array_equal(gd, od)
+    for v in range(gi.shape[0]):
+        m = od[v] < od[v, -1]
+        assert sorted(gi[v][m].tolist()) == sorted(oi[v][m].tolist()), f"row {v}"
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+def test_merge_finish_matches_merge2one(ctx, dedup):
+    """merge2one (src/abstractmerge.cpp:232-277), with and without the
+    documented de-dup: the parked far entries of each built list (cut = K/2
+    of them) are merged back with the final pool by (dist, id), parked ids
+    already in the pool dropped under de-dup.  The expected rows come from
+    the oracle's merge2one on the pool read before the finish and the parked
+    entries read at set-up, and from a restatement here.  The descent joins
     every pair, so pools re-find many parked ids and the de-dup path runs."""
     X = nnm_data.sift_like(6000, 64, centers=8, seed=21)
     K, cut = 16, 8
@@ -265,17 +277,101 @@ def test_merge_finish_matches_merge2one_with_dedup(ctx):
     # parked ids (under the merge rule they never can)
     ctx.nndescent(nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=8, max_iters=6, cut=cut))
     pi, pd, _ = ctx.get_graph()
-    ctx.merge_finish(True)
+    ctx.merge_finish(dedup)
     gi, gd, _ = ctx.get_graph()
-    _invariants(gi, gd, K)
     park_i, park_d = li[:, K - cut:], ld[:, K - cut:]
+    oi, od = orc.merge2one(pi, pd, park_i, park_d, dedup)
+    _same_up_to_ties(gi, gd, oi, od)
+    if dedup:
+        _invariants(gi, gd, K)
     overlap = 0
     for v in range(X.shape[0]):
-        keep = ~np.isin(park_i[v], pi[v])
-        overlap += int((~keep).sum())
+        keep = ~np.isin(park_i[v], pi[v]) if dedup else np.ones(cut, bool)
+        overlap += int(np.isin(park_i[v], pi[v]).sum())
         ids = np.concatenate([pi[v], park_i[v][keep]])
         ds = np.concatenate([pd[v], park_d[v][keep]])
         o = np.lexsort((ids, ds))[:K]
         assert np.array_equal(gi[v], ids[o]) and np.array_equal(gd[v], ds[o]), f"row {v}"
     print("parked ids re-found by the pool:", overlap)
     assert overlap > 1000
+    if not dedup:
+        assert any(len(set(r.tolist())) < K for r in gi), "no repeated id without de-dup"
+
+
+def _complete(K, d=16, seed=0):
+    """n = K + 1 random points, every list holding all other rows."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((K + 1, d)).astype(np.float32)
+    ids = np.array([[u for u in range(K + 1) if u != v] for v in range(K + 1)], np.uint32)
+    return X, ids
+
+
+@pytest.mark.parametrize("K,a", [(20, 9), (16, 1), (12, 12)])
+def test_complete_graph_pair_counts_match_oracle(ctx, K, a):
+    """Complete graph on K + 1 rows, all entries new, rho = 1: no reverse
+    entries, so one iteration joins C(K, 2) pairs per row under KGraph's
+    rule, a*b*(a+b-2) under the merge rule with two built subgraphs of a and
+    b rows (S-Merge), and a(C(K,2)-C(a-1,2)) + b(C(K,2)-C(a,2)) with one
+    built subgraph and raw rows (J-Merge); the oracle gives the same counts."""
+    X, ids = _complete(K, seed=K + a)
+    n, b = K + 1, K + 1 - a
+    c2 = lambda m: m * (m - 1) // 2
+    ones = np.ones_like(ids, np.uint8)
+    RAW = orc.RAW
+    cases = [(nnm.RULE_ALL, [0, n], [0], [RAW] * n, n * c2(K)),
+             (nnm.RULE_MERGE, [0, a, n], [1, 1], [0] * a + [1] * b, a * b * (a + b - 2)),
+             (nnm.RULE_MERGE, [0, a, n], [1, 0], [0] * a + [RAW] * b,
+              a * (c2(K) - c2(a - 1)) + b * (c2(K) - c2(a)))]
+    ctx.set_data(X)
+    for rule, sb, built, gid, want in cases:
+        ctx.set_partition(sb, built, [0] * len(built))
+        ctx.set_graph(ids, flags=ones)
+        st = ctx.local_join(nnm.default_params(K=K, rule=rule, seed=1), 0)
+        _, _, _, ost = orc.nndescent_lists(X, ids, ones, gid, rule_all=rule == nnm.RULE_ALL, max_iters=1)
+        assert st["dists"] == want == ost["dists"], (rule, built, st["dists"], want, ost["dists"])
+
+
+@pytest.mark.parametrize("K,rho", [(20, 0.5), (16, 0.25)])
+def test_delta_closed_form_matches_oracle(ctx, K, rho):
+    """Forward sampling takes round(rho K) new entries a row; on the complete
+    graph nothing changes, so after one iteration delta = (K - S) / K, the
+    reported delta equals the flags left in the lists over N K, and the run
+    stops after ceil(K / S) iterations with C(K, 2) pairs a row in all, as
+    in the oracle."""
+    X, ids = _complete(K, seed=3)
+    n = K + 1
+    S = int(round(rho * K))
+    ones = np.ones_like(ids, np.uint8)
+    ctx.set_data(X)
+    ctx.set_partition([0, n], [0], [0])
+    ctx.set_graph(ids, flags=ones)
+    st = ctx.nndescent(nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=1, rho=rho, max_iters=1))
+    _, _, f = ctx.get_graph()
+    _, _, of, ost = orc.nndescent_lists(X, ids, ones, [orc.RAW] * n, rule_all=True, rho=rho, max_iters=1)
+    assert st["iters"] == 1 and st["converged"] == 0
+    assert st["last_delta"] == f.sum() / (n * K)
+    assert abs(st["last_delta"] - (K - S) / K) < 1e-12
+    assert abs(st["last_delta"] - ost["last_delta"]) < 1e-7
+    assert st["dists"] == ost["dists"] == n * S * (S - 1) // 2
+    assert (f.sum(1) == of.sum(1)).all()
+    ctx.set_graph(ids, flags=ones)
+    st = ctx.nndescent(nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=1, rho=rho))
+    _, _, _, ost = orc.nndescent_lists(X, ids, ones, [orc.RAW] * n, rule_all=True, rho=rho)
+    assert st["iters"] == ost["iters"] == -(-K // S)
+    assert st["converged"] == ost["converged"] == 1
+    assert st["dists"] == ost["dists"] == n * K * (K - 1) // 2
+    assert st["last_delta"] == 0.0
+
+
+def test_last_delta_is_flag_fraction(ctx, data20k):
+    """On real data the reported delta after a capped run is the fraction of
+    entries still flagged new in the lists the run leaves."""
+    X, _, _ = data20k
+    K = 20
+    ctx.set_data(X)
+    ctx.random_init(K, seed=12)
+    for iters in (2, 1):
+        st = ctx.nndescent(nnm.default_params(K=K, rule=nnm.RULE_ALL, seed=12, max_iters=iters))
+        _, _, f = ctx.get_graph()
+        assert st["last_delta"] == f.astype(np.int64).sum() / (X.shape[0] * K)
+        assert 0.0 < st["last_delta"] < 1.0

@@ -73,9 +73,23 @@ def test_dpg_matches_oracle(data, layers):
         fo, fa, _ = orc.dpg(X[:rows], knn, 10, maxM, variant="ref")
         e1 = {(i, int(j)) for i in range(rows) for j in a[o[i]:o[i + 1]]}
         e2 = {(i, int(j)) for i in range(rows) for j in fa[fo[i]:fo[i + 1]]}
-        print("dpg rows", rows, "edges", len(e1), "overlap with reference pass", len(e1 & e2) / len(e1 | e2),
-              st["secs"])
+        print("dpg rows", rows, "edges", len(e1), "only in the GPU layer", len(e1 - e2),
+              "only in the reference pass", len(e2 - e1), st["secs"])
         assert len(e1 & e2) / len(e1 | e2) >= 0.99
+
+
+def test_dpg_in_place_tie_case():
+    """Four rows on which the reference's in-place reverse pass and the
+    order-free pass differ (derived by hand in the oracle's tests): the GPU
+    layer is the order-free one, one edge (3 -> 2) more than the reference."""
+    X = np.array([[23, 28], [8, 22], [12, 20], [4, 9]], np.float32)
+    knn, _ = orc.bruteforce(X, 3)
+    with nnm.Context(0) as c:
+        c.set_data(X)
+        c.set_graph(knn.astype(np.uint32))
+        c.dpg_build(2, 2)
+        o, a, _ = c.dpg_get()
+    assert [a[o[i]:o[i + 1]].tolist() for i in range(4)] == [[2], [2, 3], [1, 0], [1, 2]]
 
 
 def test_dpg_carry_matches_oracle(ctx, data):

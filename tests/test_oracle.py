@@ -3,7 +3,6 @@ against the reference's own distance routines compiled from its
 sources (oracle/_ref), the DistOpt::test vectors (src/distopt.cpp:208-215),
 FP64 closed forms, brute force on tiny inputs and list invariants."""
 import ctypes as C
-import itertools
 
 import numpy as np
 import pytest
@@ -105,11 +104,38 @@ def test_get_rnd_sds_distinct_and_in_range():
             assert (s < N).all()
 
 
-def test_get_rnd_sds_first_draw_is_mt19937():
-    # getRndSds draws rng() % (N - k) for each slot, sorts, bumps and adds
-    # rng() % N; std::mt19937 with seed 5489 outputs 3499211612 first
-    s = orc.get_rnd_sds(5489, 1, 1000)
-    assert s[0] == (3499211612 % 999 + 581869302 % 1000) % 1000
+def _mt_stream(seed):
+    # numpy's MT19937 with the legacy integer seeding is std::mt19937(seed)
+    # (init_genrand); random_raw yields its 32-bit outputs in order
+    bg = np.random.MT19937()
+    bg._legacy_seeding(seed)
+    while True:
+        for x in bg.random_raw(64):
+            yield int(x)
+
+
+def _rnd_sds_numpy(seed, k, N):
+    # getRndSds as written at src/abstractgraph.cpp:7-35, on numpy's stream
+    if N == k:
+        return list(range(k))
+    g = _mt_stream(seed)
+    a = sorted(next(g) % (N - k) for _ in range(k))
+    for i in range(1, k):
+        if a[i] <= a[i - 1]:
+            a[i] = a[i - 1] + 1
+    off = next(g) % N
+    return [(x + off) % N for x in a]
+
+
+def test_get_rnd_sds_against_numpy_mt19937():
+    # std::mt19937 with seed 5489 outputs 3499211612 first (C++ standard)
+    assert next(_mt_stream(5489)) == 3499211612
+    rng = np.random.default_rng(5)
+    for _ in range(400):
+        seed = int(rng.integers(0, 2**32))
+        N = int(rng.integers(2, 100000))
+        k = int(rng.integers(1, min(N, 70) + 1))
+        assert orc.get_rnd_sds(seed, k, N).tolist() == _rnd_sds_numpy(seed, k, N), (seed, k, N)
 
 
 def test_kgraph_small_exact():
@@ -141,16 +167,140 @@ def test_kgraph_and_merges_reach_high_recall():
     assert sst["dists"] < st["dists"]
 
 
-def test_merge_pair_rule_closed_form():
-    # with every row new and no reverse entries beyond forward ones, the
-    # merge rule keeps exactly the cross-subgraph pairs; checked on a
-    # 2-subgraph toy by counting against an enumeration of the same lists
-    n_new = 6
-    labels = [0, 0, 1, 1, 1, 0xFFFFFFFF]
-    kept = sum(1 for i, j in itertools.combinations(range(n_new), 2)
-               if not (labels[i] == labels[j] and labels[i] != 0xFFFFFFFF))
-    # pairs: (0,1) same built -> skip; (2,3),(2,4),(3,4) skip; raw pairs kept
-    assert kept == 15 - 1 - 3
+def _complete(K, d=8, seed=0):
+    """n = K + 1 random points, every list holding all other rows."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((K + 1, d)).astype(np.float32)
+    ids = np.array([[u for u in range(K + 1) if u != v] for v in range(K + 1)], np.uint32)
+    return X, ids
+
+
+def _c2(m):
+    return m * (m - 1) // 2
+
+
+@pytest.mark.parametrize("K,a", [(20, 9), (16, 1), (12, 12)])
+def test_one_iteration_pair_counts_closed_form(K, a):
+    # complete graph on n = K + 1 rows, every entry new, rho = 1: no reverse
+    # entry exists (each row's k-th distance reaches every row), so row v
+    # joins its K entries pairwise.  KGraph's rule keeps all C(K, 2) pairs;
+    # the merge rule drops the pairs inside one built subgraph.  Every
+    # candidate is already listed, so the iteration changes nothing and
+    # the run stops after it with delta 0.
+    X, ids = _complete(K, seed=K + a)
+    n, b = K + 1, K + 1 - a
+    flags = np.ones_like(ids, np.uint8)
+    RAW = orc.RAW
+    _, _, f, st = orc.nndescent_lists(X, ids, flags, [RAW] * n, rule_all=True)
+    assert st["dists"] == n * _c2(K) and st["iters"] == 1 and st["converged"] == 1
+    assert st["last_delta"] == 0.0 and not f.any()
+    # S-Merge: two built subgraphs [0, a) and [a, n): a*b*(a+b-2) pairs
+    gid = [0] * a + [1] * b
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=False)
+    assert st["dists"] == a * b * (a + b - 2)
+    # J-Merge: one built subgraph [0, a), raw rows after it
+    gid = [0] * a + [RAW] * b
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=False)
+    assert st["dists"] == a * (_c2(K) - _c2(a - 1)) + b * (_c2(K) - _c2(a))
+    # no built rows: the merge rule equals KGraph's
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, [RAW] * n, rule_all=False)
+    assert st["dists"] == n * _c2(K)
+
+
+@pytest.mark.parametrize("K,rho", [(20, 0.5), (16, 0.25), (10, 0.7)])
+def test_delta_and_stop_closed_form(K, rho):
+    # forward sampling takes round(rho K) new entries per row; the others
+    # keep their flag and sit out the iteration, so after one iteration
+    # delta = (K - S) / K exactly; the run goes on until every entry has been
+    # sampled, and the pair count over all iterations adds up to the one
+    # of a single full iteration, C(K, 2) per row
+    X, ids = _complete(K, seed=3)
+    n = K + 1
+    S = int(round(rho * K))
+    flags = np.ones_like(ids, np.uint8)
+    gid = [orc.RAW] * n
+    _, _, f, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=True, rho=rho, max_iters=1)
+    assert st["iters"] == 1 and st["converged"] == 0
+    assert st["dists"] == n * _c2(S)
+    assert abs(st["last_delta"] - (K - S) / K) < 1e-7
+    assert (f.sum(1) == K - S).all()
+    assert st["last_delta"] == pytest.approx(f.sum() / (n * K), abs=1e-7)
+    # the stop rule: delta below the threshold ends the loop, not at it
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=True, rho=rho,
+                                      delta=(K - S) / K + 1e-6)
+    assert st["iters"] == 1 and st["converged"] == 1
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=True, rho=rho)
+    iters = -(-K // S)
+    assert st["iters"] == iters and st["converged"] == 1 and st["last_delta"] == 0.0
+    assert st["dists"] == n * _c2(K)
+
+
+def test_reverse_entries_closed_form():
+    # four points on a line at 0, 1, 3, 7 with K = 1: rows 2 and 3 list a
+    # nearer row whose own k-th distance is smaller, so they enter that
+    # row's reverse list (maxDst < nb.dist, src/abstractmerge.cpp:94-104);
+    # rows 1 and 2 then each join one pair: (0, 2) and (1, 3)
+    X = np.array([[0.0], [1.0], [3.0], [7.0]], np.float32)
+    ids = np.array([[1], [0], [1], [2]], np.uint32)
+    flags = np.ones_like(ids, np.uint8)
+    gid = [orc.RAW] * 4
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=True)
+    assert st["dists"] == 2 and st["converged"] == 1
+    _, _, _, st = orc.nndescent_lists(X, ids, flags, gid, rule_all=True, rev_cap=0)
+    assert st["dists"] == 0
+    # old entries join new ones: with row 3's entry already old, row 2's new
+    # list is [1] and its old list [3] (through reverse), one pair again
+    f2 = flags.copy()
+    f2[3, 0] = 0
+    _, _, _, st = orc.nndescent_lists(X, ids, f2, gid, rule_all=True)
+    assert st["dists"] == 2
+
+
+def _merge2one_brute(pi, pd, ci, cd, dedup):
+    # a global stable order of all entries: distance, then the pool before
+    # the parked entries, then position; the first K (distinct ids when
+    # de-duplicating) are the merge's output
+    K = len(pi)
+    allv = [(float(pd[j]), 0, j, int(pi[j])) for j in range(K)]
+    allv += [(float(cd[j]), 1, j, int(ci[j])) for j in range(len(ci))]
+    allv.sort()
+    out, seen = [], set()
+    for dd, _, _, i in allv:
+        if dedup and i in seen:
+            continue
+        seen.add(i)
+        out.append((i, dd))
+        if len(out) == K:
+            break
+    return [o[0] for o in out], [o[1] for o in out]
+
+
+@pytest.mark.parametrize("dedup", [True, False])
+def test_merge2one_against_brute_force(dedup):
+    rng = np.random.default_rng(11 + dedup)
+    n, K, cut = 400, 12, 6
+    pi = np.empty((n, K), np.uint32)
+    pd = np.empty((n, K), np.float32)
+    ci = np.full((n, cut), orc.RAW, np.uint32)
+    cd = np.zeros((n, cut), np.float32)
+    dist_of = rng.integers(0, 20, 60).astype(np.float32)  # a fixed distance per id: ties, repeats
+    for v in range(n):
+        ids = rng.choice(60, K, replace=False)
+        o = np.argsort(dist_of[ids], kind="stable")
+        pi[v], pd[v] = ids[o], dist_of[ids[o]]
+        m = int(rng.integers(0, cut + 1))
+        c = rng.choice(60, m, replace=False)  # may repeat pool ids
+        o = np.argsort(dist_of[c], kind="stable")
+        ci[v, :m], cd[v, :m] = c[o], dist_of[c[o]]
+    gi, gd = orc.merge2one(pi, pd, ci, cd, dedup)
+    dup_rows = 0
+    for v in range(n):
+        m = int((ci[v] != orc.RAW).sum())
+        ei, ed = _merge2one_brute(pi[v], pd[v], ci[v, :m], cd[v, :m], dedup)
+        assert gi[v].tolist() == ei and gd[v].tolist() == ed, f"row {v}"
+        dup_rows += len(set(gi[v].tolist())) < K
+    # without de-dup a parked id already in the pool can come back twice
+    assert (dup_rows > 0) == (not dedup)
 
 
 def test_recall_semantics():

@@ -44,6 +44,50 @@ def test_dpg_of_collinear_points_is_the_path():
                 assert dj == np.float32((X[i, 0] - X[j, 0]) ** 2)
 
 
+def test_dpg_star_keeps_the_nearest_axes():
+    # a centre row and axis rows r_j e_j: an axis row keeps only the centre
+    # (every other axis row is farther from it than the centre is), the
+    # centre keeps its M nearest axis rows (no axis row occludes another:
+    # r_i^2 + r_j^2 > r_j^2), and the reverse pass offers every axis row to
+    # the centre, which ends with its maxM nearest, replacing its farthest
+    # entry whenever a nearer offer comes in (src/lynndescent.cpp:685-705)
+    m, M, maxM = 9, 3, 5
+    rng = np.random.default_rng(2)
+    r = rng.permutation(np.arange(3, 3 + m)).astype(np.float32)
+    order = rng.permutation(m + 1)  # the centre's row index is order[0]
+    X = np.zeros((m + 1, m), np.float32)
+    for j in range(m):
+        X[order[j + 1], j] = r[j]
+    c = int(order[0])
+    axis_by_r = [int(order[j + 1]) for j in np.argsort(r)]
+    for variant in ("ref", "par"):
+        offs, adj, dist = orc.dpg(X, _knn(X, m), M, maxM, variant=variant)
+        assert adj[offs[c]:offs[c + 1]].tolist() == axis_by_r[:maxM], variant
+        assert dist[offs[c]:offs[c + 1]].tolist() == sorted(float(x * x) for x in r)[:maxM]
+        for a in axis_by_r:
+            assert adj[offs[a]:offs[a + 1]].tolist() == [c], (variant, a)
+
+
+def test_dpg_in_place_pass_closed_form():
+    # four rows where the reference's in-place reverse pass and the
+    # order-free pass differ (ties included).  Squared distances:
+    # d01 = 261, d02 = 185, d03 = 722, d12 = 20, d13 = 185, d23 = 185.
+    # Phase 1 (M = 2): row 0 keeps [2] (1 and 3 are occluded by 2); row 1
+    # keeps [2, 3] (d23 = d13 is not "closer"); row 2 keeps 1, then 0, then
+    # 3 (equal to the last distance) replaces 0: [1, 3]; row 3 keeps [1].
+    # Reverse pass (maxM = 2): row 0 offers itself to row 2, whose full list
+    # has no entry closer to row 0 than d02, so 0 replaces 3: [1, 0].  In the
+    # reference row 2 then walks its current list [1, 0] and never offers
+    # itself to row 3; the order-free pass walks row 2's phase-1 list [1, 3]
+    # and row 3 (with room) takes 2 at the tied distance 185: [1, 2].
+    X = np.array([[23, 28], [8, 22], [12, 20], [4, 9]], np.float32)
+    knn = _knn(X, 3)
+    want = {"ref": [[2], [2, 3], [1, 0], [1]], "par": [[2], [2, 3], [1, 0], [1, 2]]}
+    for variant, rows in want.items():
+        offs, adj, _ = orc.dpg(X, knn, 2, 2, variant=variant)
+        assert [adj[offs[i]:offs[i + 1]].tolist() for i in range(4)] == rows, variant
+
+
 def test_dpg_bounds_and_reverse_rule():
     X = nnm_data.sift_like(1500, 16, centers=6, seed=8)
     knn = _knn(X, 10)
