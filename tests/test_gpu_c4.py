@@ -1,0 +1,144 @@
+"""The learned-pruning J-Merge at full size: 5M x 128 integer rows (the
+sift-like low-rank mixture), 2.5M rows arriving with their NN-Descent
+graph, 2.5M raw, K = 20, rho = 0.5; the learned phase on bf16 rows and on
+FP8 rows, then plain BF16 iterations (src/jmerge.cpp:83-203 with the
+learned prefilter in front of the join).
+
+Size-independent properties: every list distance equals its FP64 value (the
+rows are integers, so FP32 is exact), rows are sorted, distinct and free of
+self loops, the switch recomputes every list entry exactly once (N K), the
+phase split is the configured one, and recall@10 against the GPU brute
+force (itself bit-exact against the FP64 oracle) reaches 0.95.  The gain:
+at recall within 0.005 of the plain BF16 merge, the learned merge computes
+fewer exact distances and takes less time, and a trained network beats the
+untrained one by a recall margin."""
+import time
+
+import numpy as np
+import pytest
+
+import nnm
+import nnm_data
+
+pytestmark = pytest.mark.gpu
+
+N, D, K, RHO = 5_000_000, 128, 20, 0.5
+
+
+@pytest.fixture(scope="module")
+def c4():
+    t0 = time.time()
+    X = nnm_data.sift_like(N, D, centers=256, sigma=20.0, seed=nnm_data.SEED_BASE + 4, rank=12)
+    half = N // 2
+    rows = np.sort(np.random.default_rng(7).choice(N, 4000, replace=False)).astype(np.uint32)
+    with nnm.Context(0) as ctx:
+        ctx.set_data(X)
+        gt, _ = ctx.bruteforce(10, rows=rows)
+        ctx.set_data(X[:half])
+        ctx.random_init(K, seed=11)
+        ctx.nndescent(nnm.default_params(K=K, rho=RHO, seed=11))
+        g0, _, _ = ctx.get_graph()
+        ctx.set_data(X)
+        ctx.set_partition([0, half, N], [1, 0], [0, 0])
+        init = np.empty((N, K), np.uint32)
+        init[:half] = g0
+        init[half:] = ((np.arange(half, N)[:, None] + 1 + np.arange(K)[None, :]) % N).astype(np.uint32)
+        ctx.set_graph(init)
+        ctx.graph_save()
+        del init, g0
+        print(f"\nC4 set-up {time.time() - t0:.1f}s")
+        yield ctx, rows, gt
+
+
+def _train(ctx, steps):
+    ctx.graph_restore(recompute=True)
+    ctx.mlp_init(256, 32, seed=7)
+    return ctx.mlp_train(steps=steps, batch=4096, lr=2e-3, keep=0.98, seed=9)
+
+
+def _merge(ctx, prec, T, tail=0, flags=0):
+    """One merge from the saved lists, as the benchmark's step runs it;
+    returns stats, the wall time of the step and the exact distances it
+    computed (set-up, join outside the learned phase, switch recompute)."""
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=RHO, max_iters=T, prec=prec, seed=1, plain_tail=tail,
+                           flags=flags)
+    t0 = time.perf_counter()
+    if prec == nnm.PREC_LEARNED:
+        ctx.mlp_project()
+    r = ctx.graph_restore(recompute=1 if prec == nnm.PREC_FP32 else 2)
+    i = ctx.merge_init(p)
+    s = ctx.nndescent(p)
+    ctx.merge_finish(True)
+    secs = time.perf_counter() - t0
+    exact = r["init_dists"] + i["init_dists"] + s["dists"] - s["learned_dists"] + s["reranked"]
+    return s, secs, exact
+
+
+def _recall(ctx, rows, gt):
+    ids, _, _ = ctx.get_graph()
+    return float(nnm.recall(ids, gt, rows, 10)), ids
+
+
+def _properties(ctx, ids):
+    _, dist, _ = ctx.get_graph()
+    s = np.sort(ids, axis=1)
+    assert (s[:, 1:] != s[:, :-1]).all(), "duplicate ids"
+    del s
+    assert (ids != np.arange(N, dtype=np.uint32)[:, None]).all(), "self loop"
+    assert (dist[:, 1:] >= dist[:, :-1]).all(), "rows not sorted"
+    e = ctx.dist_error()
+    assert e["entries"] == N * K
+    assert e["max_abs"] == 0.0 and e["order_flips"] == 0, e
+
+
+LEARNED_FLAGS = nnm.FLAG_LEARNED_MARK_FOUND
+
+
+@pytest.mark.parametrize("fp8,T,tail", [(False, 8, 2), (True, 8, 3)], ids=["bf16_rows", "fp8_rows"])
+def test_c4_learned_merge_properties(c4, fp8, T, tail):
+    """bf16 rows at the benchmark's setting (8 iterations, the last 2 plain);
+    FP8 rows, whose estimates are noisier, with 3 plain iterations."""
+    ctx, rows, gt = c4
+    tr = _train(ctx, 600)
+    flags = LEARNED_FLAGS | (nnm.FLAG_LEARNED_FP8 if fp8 else 0)
+    st, secs, exact = _merge(ctx, nnm.PREC_LEARNED, T, tail, flags)
+    rec, ids = _recall(ctx, rows, gt)
+    print(f"\nlearned {'fp8' if fp8 else 'bf16'} rows: recall {rec:.4f} wall {secs:.3f}s descent {st['secs']:.3f}s "
+          f"exact {exact} learned {st['learned_dists']} offers {st['offers']} train {tr}")
+    assert st["learned_iters"] == T - tail and st["plain_iters"] == tail
+    assert st["reranked"] == N * K  # the switch recomputes every list entry once
+    assert st["redo_rows"] == 0 or st["redo_rounds"] > 0
+    _properties(ctx, ids)
+    assert rec >= 0.95
+
+
+def test_c4_learned_gain(c4):
+    """At recall within 0.005 of the plain BF16 merge, the learned merge
+    computes fewer exact distances in less time (8 iterations, the last 3
+    plain); with the untrained network (only its threshold calibrated) the
+    same schedule loses at least 0.01 of recall."""
+    ctx, rows, gt = c4
+    res = {}
+    _merge(ctx, nnm.PREC_BF16, 7)  # warm
+    s, secs, exact = _merge(ctx, nnm.PREC_BF16, 7)
+    res["plain"] = (_recall(ctx, rows, gt)[0], secs, exact, s["secs"])
+    _train(ctx, 600)
+    T, tail = 8, 3
+    _merge(ctx, nnm.PREC_LEARNED, T, tail, LEARNED_FLAGS)  # warm
+    s, secs, exact = _merge(ctx, nnm.PREC_LEARNED, T, tail, LEARNED_FLAGS)
+    res["trained"] = (_recall(ctx, rows, gt)[0], secs, exact, s["secs"])
+    ctx.graph_restore(recompute=True)
+    ctx.mlp_init(256, 32, seed=7)  # untrained weights: one step at lr 0 calibrates tau only
+    ctx.mlp_train(steps=1, batch=4096, lr=0.0, keep=0.98, seed=8)
+    s, secs, exact = _merge(ctx, nnm.PREC_LEARNED, T, tail, LEARNED_FLAGS)
+    res["untrained"] = (_recall(ctx, rows, gt)[0], secs, exact, s["secs"])
+    for k, (r, w, e, d) in res.items():
+        print(f"\n{k}: recall {r:.4f} wall {w:.3f}s descent {d:.3f}s exact dists {e}")
+    r_p, w_p, e_p, _ = res["plain"]
+    r_t, w_t, e_t, _ = res["trained"]
+    r_u, _, _, _ = res["untrained"]
+    assert r_p >= 0.95
+    assert r_t >= r_p - 0.005
+    assert e_t < 0.6 * e_p
+    assert w_t < w_p
+    assert r_u <= r_t - 0.01

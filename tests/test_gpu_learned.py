@@ -110,18 +110,18 @@ def _jmerge(ctx, jm, prec, flags):
 
 def test_learned_join_ablation(ctx, jm100k):
     """J-Merge of 100k low-rank sift-like rows: plain BF16 join vs the learned
-    phase with a trained network vs with the untrained (random) network.
-    The learned run must keep recall within 0.01 of the plain join (its
-    fallback for the last iteration); the trained network must not do worse
-    than the untrained one.  Exact distances outside the join: one per list
-    entry when the lists leave the learned space."""
+    phase with a trained network vs with the untrained (random) network,
+    each list entry renewed at the switch.  The learned run keeps recall
+    within 0.005 of the plain join and computes fewer exact distances (one
+    per list entry at the switch plus the plain tail's joins); the trained
+    network beats the untrained one by 0.01 of recall.  Device time at this
+    size is launch-bound; the time gain is asserted on the 5M merge."""
     X, rows, gt, init = jm100k
     n, K = init.shape
     ctx.set_data(X)
     res = {}
 
     def merge(prec):
-        # every entry renewed at the switch
         return _jmerge(ctx, jm100k, prec, nnm.FLAG_LEARNED_RENEW)
 
     res["plain"] = merge(nnm.PREC_BF16)
@@ -129,7 +129,6 @@ def test_learned_join_ablation(ctx, jm100k):
     ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
     ctx.set_graph(init)
     ctx.mlp_init(256, 32, seed=7)
-    res["untrained_tau"] = None
     ctx.mlp_train(steps=1, batch=1024, lr=0.0, keep=0.98, seed=8)  # calibrates tau only
     res["untrained"] = merge(nnm.PREC_LEARNED)
     ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
@@ -137,18 +136,65 @@ def test_learned_join_ablation(ctx, jm100k):
     ctx.mlp_init(256, 32, seed=7)
     tst = ctx.mlp_train(steps=600, batch=4096, lr=2e-3, keep=0.98, seed=9)
     res["trained"] = merge(nnm.PREC_LEARNED)
+
+    def exact(st):
+        return st["dists"] - st["learned_dists"] + st["reranked"]
+
     for k in ("plain", "untrained", "trained"):
         r, st = res[k]
-        print(k, round(r, 4), "iters", st["iters"], "learned", st["learned_iters"], "exact", st["reranked"],
-              "pairs", st["dists"], "secs", round(st["secs"], 4))
+        print(k, round(r, 4), "iters", st["iters"], "learned", st["learned_iters"], "exact dists", exact(st),
+              "switch", st["reranked"], "pairs", st["dists"], "secs", round(st["secs"], 4))
     print("train", tst)
     r_plain, s_plain = res["plain"]
     r_tr, s_tr = res["trained"]
     r_un, s_un = res["untrained"]
     assert s_tr["learned_iters"] > 0 and s_tr["plain_iters"] >= 1
-    assert r_tr >= r_plain - 0.01
-    assert r_tr >= r_un - 0.002
     assert s_tr["reranked"] == n * K  # one exact recompute per list entry at the switch
+    assert r_tr >= r_plain - 0.005
+    assert exact(s_tr) < 0.7 * exact(s_plain)
+    assert r_tr >= r_un + 0.01
+
+
+def test_learned_merge_matches_oracle(ctx, jm100k):
+    """The learned J-Merge (trained network, 5 learned iterations, then 3
+    plain BF16 ones) against the oracle's J-Merge of the same lists and the
+    same iteration cap: recall@10 within 0.005."""
+    X, rows, gt, init = jm100k
+    n, K = init.shape
+    ctx.set_data(X)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
+    ctx.set_graph(init)
+    ctx.mlp_init(256, 32, seed=7)
+    ctx.mlp_train(steps=600, batch=4096, lr=2e-3, keep=0.98, seed=9)
+    r_l, st = _jmerge(ctx, jm100k, nnm.PREC_LEARNED, nnm.FLAG_LEARNED_MARK_FOUND)
+    oi, _, ost = orc.merge(X, [0, n // 2, n], [1, 0], init, K=K, rho=0.5, max_iters=8, seed=5)
+    r_o = orc.recall(oi, rows, gt, 10)
+    print(f"\nlearned {r_l:.4f} (iters {st['iters']}, learned {st['learned_iters']}) oracle {r_o:.4f} "
+          f"(iters {ost['iters']}, {ost['secs']:.1f}s)")
+    assert st["learned_iters"] == 5 and st["plain_iters"] == 3
+    assert r_l >= r_o - 0.005
+
+
+def test_learned_refuses_long_candidate_lists(ctx):
+    """The learned join handles candidate lists of up to 256 entries
+    (round(rho K) + K + 2 rev_cap): K = 32 at rho = 1 with rev_cap 100 asks
+    for 264 and is refused with NNM_ERR_UNSUPPORTED before any launch;
+    rev_cap 96 (256 entries) runs."""
+    X = nnm_data.sift_like(6000, 128, centers=16, seed=77, rank=12)
+    n, K = X.shape[0], 32
+    ctx.set_data(X)
+    ctx.set_partition([0, n // 2, n], [1, 0], [0, 0])
+    ctx.random_init(K, seed=2)
+    ctx.mlp_init(256, 32, seed=3)
+    ctx.mlp_train(steps=20, batch=1024, lr=2e-3, keep=0.98, seed=4)
+    p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, rho=1.0, rev_cap=100, max_iters=3, prec=nnm.PREC_LEARNED,
+                           plain_tail=1)
+    with pytest.raises(nnm.NNMError) as e:
+        ctx.nndescent(p)
+    assert e.value.code == nnm.NNM_ERR_UNSUPPORTED
+    p.rev_cap = 96
+    st = ctx.nndescent(p)
+    assert st["learned_iters"] == 2 and st["plain_iters"] == 1
 
 
 _E4M3 = np.array([(2.0 ** ((c >> 3) - 7)) * (1 + (c & 7) / 8.0) if (c >> 3) else (c & 7) * 2.0 ** -9
