@@ -49,6 +49,8 @@
 #include <random>
 #include <vector>
 #include <chrono>
+#include <mutex>
+#include <omp.h>
 
 extern "C" {
 
@@ -166,6 +168,7 @@ double orc_cosine_f64(const float* a, const float* b, uint32_t d) {
 // (src/abstractgraph.h:14-19, src/space_lp.h).
 static int g_metric = 0;
 void orc_set_metric(int m) { g_metric = m; }
+
 
 static float dist32(const float* a, const float* b, uint32_t d) {
     switch (g_metric) {
@@ -395,6 +398,108 @@ static void nn_descent(const float* X, uint32_t d, std::vector<OList>& g,
     st->iters = it;
 }
 
+// nn_descent on every core: the reference's parallel form, which its
+// sources carry as commented OpenMP pragmas (schedule(dynamic, 100) over
+// the rows in Jmerge::localJoin, src/jmerge.cpp:185, and
+// Smerge::localJoin, src/smerge.cpp:220) with one lock per list
+// (NNList::parallel_try_insert, src/pvetex.h:99-109, and the LockGuard
+// around the reverse pushes, src/abstractmerge.cpp:103-104).  The same
+// algorithm as nn_descent; the reverse lists are shuffled with a generator
+// per row and iteration (seed, iteration, row), and the insertion order
+// of concurrent offers is the scheduler's, so results differ from the
+// sequential loop's in their random choices only.
+static void nn_descent_mt(const float* X, uint32_t d, std::vector<OList>& g,
+                          const std::vector<uint32_t>& gid, int rule_all,
+                          const OParams& p, orc_stats* st) {
+    const uint32_t N = (uint32_t)g.size();
+    const uint32_t K = p.K;
+    uint32_t S = (uint32_t)std::lround((double)p.rho * K);
+    if (S < 1) S = 1;
+    if (S > K) S = K;
+    std::vector<std::mutex> lock(N);
+    uint32_t it = 0;
+    st->converged = 0;
+    for (;;) {
+#pragma omp parallel for schedule(dynamic, 256)
+        for (uint32_t v = 0; v < N; ++v) {
+            OList& L = g[v];
+            uint32_t taken = 0;
+            for (uint32_t e = 0; e < K; ++e) {
+                ONb& nb = L.pool[e];
+                OList& o = g[nb.id];
+                // pools change only in the join: no lock for reading them here
+                const float maxDst = o.pool[K - 1].dist;
+                if (nb.flag) {
+                    if (taken >= S) continue;
+                    ++taken;
+                    L.newl.push_back(OInd{nb.id, nb.gd});
+                    if (maxDst < nb.dist) {
+                        std::lock_guard<std::mutex> gd(lock[nb.id]);
+                        o.rnew.push_back(OInd{v, gid[v]});
+                    }
+                    nb.flag = 0;
+                } else {
+                    L.oldl.push_back(OInd{nb.id, nb.gd});
+                    if (maxDst < nb.dist) {
+                        std::lock_guard<std::mutex> gd(lock[nb.id]);
+                        o.rold.push_back(OInd{v, gid[v]});
+                    }
+                }
+            }
+        }
+#pragma omp parallel for schedule(dynamic, 256)
+        for (uint32_t v = 0; v < N; ++v) {
+            OList& L = g[v];
+            std::mt19937 rng(p.seed * 0x9E3779B9u ^ (it * 0x85EBCA6Bu) ^ (v * 0xC2B2AE35u));
+            std::shuffle(L.rnew.begin(), L.rnew.end(), rng);
+            if (L.rnew.size() > p.rev_cap) L.rnew.resize(p.rev_cap);
+            std::shuffle(L.rold.begin(), L.rold.end(), rng);
+            if (L.rold.size() > p.rev_cap) L.rold.resize(p.rev_cap);
+            L.newl.insert(L.newl.end(), L.rnew.begin(), L.rnew.end());
+            L.oldl.insert(L.oldl.end(), L.rold.begin(), L.rold.end());
+        }
+        uint64_t cc = 0;
+#pragma omp parallel for schedule(dynamic, 100) reduction(+ : cc)
+        for (uint32_t v = 0; v < N; ++v) {
+            const OList& L = g[v];
+            const size_t ns = L.newl.size(), os = L.oldl.size();
+            for (size_t i = 0; i < ns; ++i) {
+                const OInd si = L.newl[i];
+                for (size_t j = i + 1; j < ns + os; ++j) {
+                    const OInd sj = j < ns ? L.newl[j] : L.oldl[j - ns];
+                    if (si.id == sj.id) continue;
+                    if (!rule_all && si.gid == sj.gid && si.gid != RAW_GID) continue;
+                    float dst = dist32(X + (uint64_t)si.id * d, X + (uint64_t)sj.id * d, d);
+                    ++cc;
+                    {
+                        std::lock_guard<std::mutex> gd(lock[si.id]);
+                        update_knn_list(g[si.id].pool.data(), K, ONb{sj.id, dst, sj.gid, 1});
+                    }
+                    {
+                        std::lock_guard<std::mutex> gd(lock[sj.id]);
+                        update_knn_list(g[sj.id].pool.data(), K, ONb{si.id, dst, si.gid, 1});
+                    }
+                }
+            }
+        }
+        st->dists += cc;
+        ++it;
+        double acc = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : acc)
+        for (uint32_t v = 0; v < N; ++v) {
+            uint32_t c = 0;
+            for (uint32_t e = 0; e < K; ++e) c += g[v].pool[e].flag ? 1 : 0;
+            acc += (double)((float)c / (float)K);
+            g[v].newl.clear(); g[v].oldl.clear(); g[v].rnew.clear(); g[v].rold.clear();
+        }
+        double mean = N ? acc / N : 0.0;
+        st->last_delta = mean;
+        if (mean < p.delta) { st->converged = 1; break; }
+        if (p.max_iters && it >= p.max_iters) break;
+    }
+    st->iters = it;
+}
+
 static bool nb_less(const ONb& a, const ONb& b) { return a.dist < b.dist; }   // Neighbor::ComNeighbor
 
 // KGraph::InitialGraph, src/kgraph.cpp:56-96: K random distinct neighbours
@@ -536,11 +641,11 @@ void orc_nndescent_lists(const float* X, uint32_t n, uint32_t d, uint32_t K, uin
 // built subgraph, its K neighbour ids (global, sorted by distance); rows of
 // raw subgraphs are ignored.  cut is the far part replaced by cross samples
 // (cutSize = K/2 in src/smerge.cpp:99 and K*3/6 in src/jmerge.cpp:87).
-void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
-               uint32_t nsub, const uint32_t* sub_begin, const uint8_t* sub_built,
-               const uint32_t* init_ids, uint32_t cut, float rho, uint32_t rev_cap,
-               uint32_t max_iters, float delta, uint32_t seed, int dedup,
-               uint32_t* out_ids, float* out_dist, orc_stats* st) {
+static void merge_impl(const float* X, uint32_t n, uint32_t d, uint32_t K,
+                       uint32_t nsub, const uint32_t* sub_begin, const uint8_t* sub_built,
+                       const uint32_t* init_ids, uint32_t cut, float rho, uint32_t rev_cap,
+                       uint32_t max_iters, float delta, uint32_t seed, int dedup,
+                       uint32_t* out_ids, float* out_dist, orc_stats* st, bool mt) {
     std::memset(st, 0, sizeof(*st));
     OParams p{K, rho, rev_cap, max_iters, delta, seed};
     std::mt19937 rng(seed);
@@ -555,6 +660,8 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
     const uint32_t remain = K - cut;
     double t0 = now_s();
     // InitSGraph: recompute distances of the given lists, all flagged new.
+    uint64_t nd = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nd) if (mt)
     for (uint32_t v = 0; v < n; ++v) {
         if (!sub_built[sub[v]]) continue;
         g[v].pool.resize(K);
@@ -564,19 +671,18 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
             nb.dist = dist32(X + (uint64_t)v * d, X + (uint64_t)nb.id * d, d);
             nb.flag = 1;
             nb.gd = gid[nb.id];
-            ++st->init_dists;
+            ++nd;
         }
     }
-    std::vector<uint32_t> seeds(2 * K + 2);
-    std::vector<uint32_t> fresh;
-    for (uint32_t v = 0; v < n; ++v) {
+    auto setup_row = [&](uint32_t v, std::mt19937& rng, std::vector<uint32_t>& seeds,
+                         std::vector<uint32_t>& fresh, uint64_t& nd) {
         const uint32_t s = sub[v];
         if (sub_built[s]) {
             // Cross samples from the other subgraphs (src/smerge.cpp:96-142).
             const uint32_t own_b = sub_begin[s], own_n = sub_begin[s + 1] - sub_begin[s];
             const uint32_t others = n - own_n;
             cpn[v].assign(g[v].pool.begin() + remain, g[v].pool.end());
-            if (others == 0) continue;
+            if (others == 0) return;
             uint32_t draw = 2 * cut < others ? 2 * cut : others;
             get_rnd_sds(rng, draw, seeds.data(), others);
             fresh.clear();
@@ -594,7 +700,7 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
                 nb.gd = gid[nb.id];
                 nb.flag = 1;
                 nb.dist = dist32(X + (uint64_t)nb.id * d, X + (uint64_t)v * d, d);
-                ++st->init_dists;
+                ++nd;
             }
             std::sort(g[v].pool.begin(), g[v].pool.end(), nb_less);
         } else {
@@ -609,12 +715,27 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
                 nb.gd = gid[nb.id];
                 nb.flag = 1;
                 nb.dist = dist32(X + (uint64_t)nb.id * d, X + (uint64_t)v * d, d);
-                ++st->init_dists;
+                ++nd;
             }
             std::sort(g[v].pool.begin(), g[v].pool.end(), nb_less);
         }
+    };
+    if (!mt) {
+        std::vector<uint32_t> seeds(2 * K + 2), fresh;
+        for (uint32_t v = 0; v < n; ++v) setup_row(v, rng, seeds, fresh, nd);
+    } else {
+        // one generator per row: the draws no longer depend on row order
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nd)
+        for (uint32_t v = 0; v < n; ++v) {
+            std::mt19937 r(seed * 0x9E3779B9u ^ (v * 0xC2B2AE35u) ^ 0x5EEDu);
+            std::vector<uint32_t> seeds(2 * K + 2), fresh;
+            setup_row(v, r, seeds, fresh, nd);
+        }
     }
-    nn_descent(X, d, g, gid, 0, p, st, rng);
+    st->init_dists = nd;
+    if (mt) nn_descent_mt(X, d, g, gid, 0, p, st);
+    else nn_descent(X, d, g, gid, 0, p, st, rng);
+#pragma omp parallel for schedule(static) if (mt)
     for (uint32_t v = 0; v < n; ++v)
         if (!cpn[v].empty()) merge2one(g[v].pool, cpn[v], K, dedup);
     st->secs = now_s() - t0;
@@ -623,6 +744,29 @@ void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
             out_ids[(uint64_t)v * K + j] = g[v].pool[j].id;
             out_dist[(uint64_t)v * K + j] = g[v].pool[j].dist;
         }
+}
+
+void orc_merge(const float* X, uint32_t n, uint32_t d, uint32_t K,
+               uint32_t nsub, const uint32_t* sub_begin, const uint8_t* sub_built,
+               const uint32_t* init_ids, uint32_t cut, float rho, uint32_t rev_cap,
+               uint32_t max_iters, float delta, uint32_t seed, int dedup,
+               uint32_t* out_ids, float* out_dist, orc_stats* st) {
+    merge_impl(X, n, d, K, nsub, sub_begin, sub_built, init_ids, cut, rho, rev_cap, max_iters, delta, seed, dedup,
+               out_ids, out_dist, st, false);
+}
+
+// orc_merge on `threads` host threads (0 = OpenMP's default); returns the
+// number of threads used.
+int orc_merge_mt(const float* X, uint32_t n, uint32_t d, uint32_t K,
+                 uint32_t nsub, const uint32_t* sub_begin, const uint8_t* sub_built,
+                 const uint32_t* init_ids, uint32_t cut, float rho, uint32_t rev_cap,
+                 uint32_t max_iters, float delta, uint32_t seed, int dedup,
+                 uint32_t* out_ids, float* out_dist, orc_stats* st, int threads) {
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+    omp_set_num_threads(nt);
+    merge_impl(X, n, d, K, nsub, sub_begin, sub_built, init_ids, cut, rho, rev_cap, max_iters, delta, seed, dedup,
+               out_ids, out_dist, st, true);
+    return nt;
 }
 
 // Evaluator::getKGraphRecall, src/evaluator.cpp:258-295 with intersect

@@ -75,6 +75,8 @@ def lib():
         L.orc_merge.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u8p,
                                 u32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float,
                                 C.c_uint32, C.c_int, u32p, f32p, C.POINTER(OrcStats)]
+        L.orc_merge_mt.restype = C.c_int
+        L.orc_merge_mt.argtypes = L.orc_merge.argtypes + [C.c_int]
         L.orc_merge2one.restype = None
         L.orc_merge2one.argtypes = [C.c_uint32, C.c_uint32, u32p, f32p, C.c_uint32, u32p, f32p, C.c_int]
         L.orc_nndescent_lists.restype = None
@@ -269,6 +271,28 @@ def nndescent_lists(X, ids, flags, gid_of, rule_all=False, rho=1.0, rev_cap=100,
                               _p(flags, C.c_uint8), _p(gid, C.c_uint32), 1 if rule_all else 0, rho,
                               rev_cap, max_iters, delta, seed, C.byref(st))
     return ids, dist, flags, st.as_dict()
+
+
+def merge_mt(X, sub_begin, sub_built, init_ids, K=20, cut=None, rho=1.0, rev_cap=100,
+             max_iters=0, delta=0.002, seed=1, dedup=True, threads=0):
+    """merge on every host core (threads = 0) or on `threads` threads: the
+    reference's OpenMP form, one lock per list.  Returns ids, dists, stats
+    (with "threads")."""
+    X = _f32(X)
+    n, d = X.shape
+    sb = np.ascontiguousarray(sub_begin, np.uint32)
+    bu = np.ascontiguousarray(sub_built, np.uint8)
+    ii = np.ascontiguousarray(init_ids, np.uint32)
+    cut = K // 2 if cut is None else cut
+    ids = np.empty((n, K), np.uint32)
+    dist = np.empty((n, K), np.float32)
+    st = OrcStats()
+    nt = lib().orc_merge_mt(_p(X, C.c_float), n, d, K, bu.size, _p(sb, C.c_uint32), _p(bu, C.c_uint8),
+                            _p(ii, C.c_uint32), cut, rho, rev_cap, max_iters, delta, seed,
+                            1 if dedup else 0, _p(ids, C.c_uint32), _p(dist, C.c_float), C.byref(st), threads)
+    out = st.as_dict()
+    out["threads"] = nt
+    return ids, dist, out
 
 
 def recall(ids, rows, gt, k=10, unique=True):

@@ -351,3 +351,24 @@ def test_mlp_oracle_loss_closed_form():
     assert mlp.loss(W1, b1, W2, b2, xa, xb, t, np.ones(2)) == 0.0
     L = mlp.loss(W1, b1, W2, b2, xa, xb, 2 * t, np.zeros(2))  # p = t/2 of target: r = -1/2
     assert abs(L - 0.25) < 1e-15
+
+
+def test_merge_mt_agrees_with_sequential_merge():
+    # the all-core form (the CPU baseline) runs the same algorithm with other
+    # random draws: equal list invariants, recall within 0.005, pair counts
+    # within 2%, the same set-up distance count
+    X = nnm_data.sift_like(6000, 64, centers=16, seed=12)
+    rows = np.arange(0, 6000, 5, dtype=np.uint32)
+    gt, _ = orc.bruteforce(X, 10, rows=rows)
+    h = 3000
+    g0, _, _ = orc.kgraph(X[:h], K=16, seed=3)
+    init = np.vstack([g0, np.zeros((h, 16), np.uint32)])
+    si, sd, sst = orc.merge(X, [0, h, 6000], [1, 0], init, K=16, seed=5)
+    mi, md, mst = orc.merge_mt(X, [0, h, 6000], [1, 0], init, K=16, seed=5, threads=4)
+    assert mst["threads"] == 4
+    assert (np.diff(md.astype(np.float64), axis=1) >= 0).all()
+    assert all(len(set(r.tolist())) == 16 for r in mi)
+    rs, rm = orc.recall(si, rows, gt, 10), orc.recall(mi, rows, gt, 10)
+    assert abs(rs - rm) < 0.005 and rm > 0.97
+    assert abs(mst["dists"] / sst["dists"] - 1) < 0.02
+    assert mst["init_dists"] == sst["init_dists"]
