@@ -378,6 +378,19 @@ int nnm_host_hgraph(const float* X, uint64_t n, uint32_t d, const nnm_params* p,
                     uint32_t* ids, float* dist, nnm_stats* st, uint32_t* layer_rows, uint32_t max_layers,
                     uint32_t* nlayers, double* dpg_secs);
 
+/* LySearch through its host class: LySearch(refer, graph) loads the
+ * reference set and the layered graph file (src/lysearch.cpp:32-39,
+ * :112-186), then knnSearch(query, dst, grd, record) (src/lysearch.cpp:363-437)
+ * runs the reference's beam list {60, 50, ..., 2} from entry row `entry` of
+ * layer 0 as one batched launch, writes one record line per beam
+ * ("topk\trecall@1\tsecs\tnCmps/nQry", stopping after the first beam with
+ * recall 1) and leaves the last beam's top-1 results in dst.  prec:
+ * NNM_PREC_FP32 or NNM_PREC_BF16.  The records also come back in topk /
+ * recall / secs / cmps (up to max_records; nrecords = lines written). */
+int nnm_host_lysearch(const char* refer, const char* graph, const char* query, const char* dst, const char* grd,
+                      const char* record, uint32_t entry, uint32_t prec, uint32_t max_records, uint32_t* nrecords,
+                      uint32_t* topk, double* recall, double* secs, uint64_t* cmps);
+
 /* Per-result FP64 error of the graph's FP32 distances: every list entry of
  * the active rows is recomputed in FP64 on the GPU.  No reference
  * counterpart (the reference stores FP32 distances only,
@@ -431,7 +444,9 @@ typedef struct nnm_search_stats {
     uint64_t hash_resets; /* visited-hash refills (revisited rows are recomputed) */
     uint64_t capped;      /* queries stopped by the step cap (an error) */
     uint64_t steps;       /* list entries expanded, all queries */
-    double secs;          /* search kernel time */
+    double secs;          /* search kernel time (a batched launch: this beam's share, by wave time) */
+    double wave_secs;     /* summed per-query wave time on the device wall clock */
+    double launch_secs;   /* time of the whole launch (all beams of a batched launch) */
 } nnm_search_stats;
 
 /* Search layers as CSR (the reference's augKnnGraph + knnLocs,
@@ -452,6 +467,16 @@ int nnm_search_use_graph(nnm_ctx* ctx, int source);
  * (dist, id) into ids / dist (nq x k); missing results are 0xFFFFFFFF. */
 int nnm_search(nnm_ctx* ctx, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* seeds,
                uint32_t* ids, float* dist, nnm_search_stats* st);
+
+/* Every beam of a beam list in one launch: the reference's beam sweep in
+ * LySearch::knnSearch (for topk in Ks1, src/lysearch.cpp:380-384), one
+ * search per (beam, query) from the same entry row.  nbeams in [1, 32];
+ * p->beam is ignored and p->k must not exceed the smallest beam; no explicit
+ * seeds (flat mode draws them from p->seed).  ids / dist: nbeams x nq x k in
+ * the order of `beams`; st: nbeams records, the same as nnm_search gives for
+ * each beam alone, with secs the beam's share of the launch. */
+int nnm_search_beams(nnm_ctx* ctx, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* beams,
+                     uint32_t nbeams, uint32_t* ids, float* dist, nnm_search_stats* st);
 
 /* ------------------------------------------------------------ file formats */
 /* Host-side readers and writers of the reference's files (no device work).

@@ -20,11 +20,18 @@
 //
 // Results: the first k list entries; in bf16 mode the first `rerank`
 // entries are recomputed in FP32 and re-sorted first.
+//
+// Several beams in one launch: the queue holds (beam, query) items, the
+// largest beam first so the long searches start early; each wave keeps its
+// counters per beam and flushes them when its beam changes, and measures
+// every item on the constant-rate wall clock, so the host can give each
+// beam its share of the launch time.
 
 #include "nnm_internal.hpp"
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 namespace {
 
@@ -33,6 +40,7 @@ constexpr uint32_t SR_HS = 4096;   // visited hash slots per wave
 constexpr uint32_t SR_MAXLD = 256; // longest row (floats)
 constexpr uint32_t SR_LAYERS = 8;
 constexpr uint32_t SR_MAXW = 4;
+constexpr uint32_t SR_MAXB = 32;   // beams per launch
 constexpr uint64_t SR_NONE = ~0ull;
 
 struct SearchLayer {
@@ -53,14 +61,18 @@ struct SearchArgs {
     const uint32_t* seeds;  // flat mode: nq x beam, or null (drawn from seed)
     uint64_t seed;
     uint32_t flat;
-    uint32_t beam, k, width, rerank, entry;
+    uint32_t k, width, rerank, entry;
     uint32_t max_steps;
+    uint32_t nbeams;
+    uint32_t beams[SR_MAXB];  // launch order: largest first
+    uint32_t slot[SR_MAXB];   // output block (caller's order) of each launch-order beam
     uint32_t* out_ids;
     float* out_d;
-    unsigned long long* counters;  // SC_*
+    unsigned long long* counters;  // SC_N per launch-order beam; the queue in beam 0's block
 };
 
-enum { SC_DISTS = 0, SC_RERANKED = 1, SC_RESETS = 2, SC_QUEUE = 3, SC_CAPPED = 4, SC_STEPS = 5, SC_N = 8 };
+enum { SC_DISTS = 0, SC_RERANKED = 1, SC_RESETS = 2, SC_QUEUE = 3, SC_CAPPED = 4, SC_STEPS = 5, SC_TICKS = 6,
+       SC_N = 8 };
 
 struct SearchLds {
     float q[SR_MAXLD];
@@ -173,13 +185,41 @@ template <bool BF16>
 __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
     __shared__ __align__(16) SearchLds L;
     const uint32_t lane = threadIdx.x;
-    unsigned long long my_d = 0, my_rr = 0, my_resets = 0, my_caps = 0, my_steps = 0;
-    const uint32_t lmax = a.beam;
+    unsigned long long my_d = 0, my_rr = 0, my_resets = 0, my_caps = 0, my_steps = 0, my_ticks = 0;
+    const uint64_t items = (uint64_t)a.nq * a.nbeams;
+    uint32_t cur_b = SR_MAXB;
+    // the wave's counters into beam b's block (lane sums for the per-lane ones)
+    auto flush = [&](uint32_t b) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            my_d += __shfl_xor(my_d, o, NNM_WAVE);
+            my_rr += __shfl_xor(my_rr, o, NNM_WAVE);
+        }
+        if (lane == 0) {
+            unsigned long long* cb = a.counters + (uint64_t)b * SC_N;
+            atomicAdd(&cb[SC_DISTS], my_d);
+            atomicAdd(&cb[SC_RERANKED], my_rr);
+            atomicAdd(&cb[SC_RESETS], my_resets);
+            atomicAdd(&cb[SC_CAPPED], my_caps);
+            atomicAdd(&cb[SC_STEPS], my_steps);
+            atomicAdd(&cb[SC_TICKS], my_ticks);
+        }
+        my_d = my_rr = my_resets = my_caps = my_steps = my_ticks = 0;
+    };
     for (;;) {
-        uint32_t qi = 0;
-        if (lane == 0) qi = (uint32_t)atomicAdd(&a.counters[SC_QUEUE], 1ull);
-        qi = __shfl(qi, 0, NNM_WAVE);
-        if (qi >= a.nq) break;
+        uint64_t item = 0;
+        if (lane == 0) item = atomicAdd(&a.counters[SC_QUEUE], 1ull);
+        item = __shfl(item, 0, NNM_WAVE);
+        if (item >= items) break;
+        const uint32_t bi = (uint32_t)(item / a.nq);
+        const uint32_t qi = (uint32_t)(item - (uint64_t)bi * a.nq);
+        if (bi != cur_b) {
+            if (cur_b < SR_MAXB) flush(cur_b);
+            cur_b = bi;
+        }
+        const uint32_t beam = a.beams[bi];
+        const uint32_t lmax = beam;
+        const uint64_t t_start = wall_clock64();
         for (uint32_t i = lane; i < a.ld; i += NNM_WAVE) L.q[i] = a.Q[(uint64_t)qi * a.ld + i];
         for (uint32_t i = lane; i < SR_HS; i += NNM_WAVE) L.hash[i] = 0u;
         wave_sync_lds();
@@ -220,11 +260,11 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             nins = 1;
         } else {
             // seeded start (src/lysearch.cpp:458-471): every seed enters the list
-            for (uint32_t j0 = 0; j0 < a.beam; j0 += NNM_WAVE) {
+            for (uint32_t j0 = 0; j0 < beam; j0 += NNM_WAVE) {
                 uint64_t kk = SR_NONE;
                 bool ins = false;
-                if (j0 + lane < a.beam) {
-                    const uint32_t id = a.seeds ? a.seeds[(uint64_t)qi * a.beam + j0 + lane]
+                if (j0 + lane < beam) {
+                    const uint32_t id = a.seeds ? a.seeds[(uint64_t)qi * beam + j0 + lane]
                                                 : (uint32_t)(nnm_rand(a.seed, qi, j0 + lane) % a.n);
                     ins = hash_insert(L.hash, id);
                     if (ins) {
@@ -330,7 +370,8 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
         }
         my_steps += lane == 0 ? steps : 0;
         // results
-        const uint32_t R = BF16 ? min(bs, max(a.k, min(a.rerank, (uint32_t)NNM_WAVE))) : min(bs, a.k);
+        const uint32_t rr = a.rerank ? a.rerank : beam;
+        const uint32_t R = BF16 ? min(bs, max(a.k, min(rr, (uint32_t)NNM_WAVE))) : min(bs, a.k);
         uint64_t kk = SR_NONE;
         if (lane < R) {
             const uint32_t id = nnm_key_id(L.beam[cb][lane]);
@@ -342,24 +383,15 @@ __global__ __launch_bounds__(64) void k_search(SearchArgs a) {
             }
         }
         if (BF16) kk = nnm_wave_sort64(kk);
+        const uint64_t ob = ((uint64_t)a.slot[bi] * a.nq + qi) * a.k;
         if (lane < a.k) {
-            a.out_ids[(uint64_t)qi * a.k + lane] = kk == SR_NONE ? 0xFFFFFFFFu : nnm_key_id(kk);
-            a.out_d[(uint64_t)qi * a.k + lane] = kk == SR_NONE ? 3.402823466e38f : nnm_key_dist(kk);
+            a.out_ids[ob + lane] = kk == SR_NONE ? 0xFFFFFFFFu : nnm_key_id(kk);
+            a.out_d[ob + lane] = kk == SR_NONE ? 3.402823466e38f : nnm_key_dist(kk);
         }
         wave_sync_lds();
+        my_ticks += lane == 0 ? wall_clock64() - t_start : 0;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        my_d += __shfl_xor(my_d, o, NNM_WAVE);
-        my_rr += __shfl_xor(my_rr, o, NNM_WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd(&a.counters[SC_DISTS], my_d);
-        atomicAdd(&a.counters[SC_RERANKED], my_rr);
-        atomicAdd(&a.counters[SC_RESETS], my_resets);
-        atomicAdd(&a.counters[SC_CAPPED], my_caps);
-        atomicAdd(&a.counters[SC_STEPS], my_steps);
-    }
+    if (cur_b < SR_MAXB) flush(cur_b);
 }
 
 }  // namespace
@@ -446,17 +478,24 @@ int nnm_search_adopt_layer(nnm_ctx* c, uint32_t rows, uint32_t* offs, uint32_t* 
     return NNM_OK;
 }
 
-extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* seeds,
-                          uint32_t* ids, float* dist, nnm_search_stats* st) {
+// One launch over nbeams beams (nbeams >= 1); st holds nbeams records.
+static int search_impl(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* beams,
+                       uint32_t nbeams, const uint32_t* seeds, uint32_t* ids, float* dist, nnm_search_stats* st) {
     if (!c) return NNM_ERR_INVALID;
-    if (st) std::memset(st, 0, sizeof(*st));
+    if (st && beams && nbeams <= SR_MAXB) std::memset(st, 0, sizeof(*st) * nbeams);
     NNM_TRY(nnm_require_l2(c, "nnm_search"));
     nnm_search_state* S = search_of(c);
     if (!S || S->nlayers == 0) return nnm_fail(c, NNM_ERR_STATE, "no search graph (nnm_search_set_layers)");
-    if (!p || !Q || !ids || !dist) return nnm_fail(c, NNM_ERR_INVALID, "nnm_search: null argument");
+    if (!p || !Q || !ids || !dist || !beams) return nnm_fail(c, NNM_ERR_INVALID, "nnm_search: null argument");
+    if (nbeams == 0 || nbeams > SR_MAXB) return nnm_fail(c, NNM_ERR_INVALID, "1..32 beams per launch");
     if (nq == 0) return NNM_OK;
-    if (p->beam == 0 || p->beam > SR_L) return nnm_fail(c, NNM_ERR_INVALID, "beam must be in [1, 256]");
-    if (p->k == 0 || p->k > p->beam || p->k > NNM_WAVE) return nnm_fail(c, NNM_ERR_INVALID, "k must be in [1, min(beam, 64)]");
+    uint32_t min_beam = SR_L;
+    for (uint32_t b = 0; b < nbeams; ++b) {
+        if (beams[b] == 0 || beams[b] > SR_L) return nnm_fail(c, NNM_ERR_INVALID, "beam must be in [1, 256]");
+        min_beam = std::min(min_beam, beams[b]);
+    }
+    if (p->k == 0 || p->k > min_beam || p->k > NNM_WAVE)
+        return nnm_fail(c, NNM_ERR_INVALID, "k must be in [1, min(beam, 64)]");
     if (p->width == 0 || p->width > SR_MAXW) return nnm_fail(c, NNM_ERR_INVALID, "width must be in [1, 4]");
     if (p->prec != NNM_PREC_FP32 && p->prec != NNM_PREC_BF16)
         return nnm_fail(c, NNM_ERR_INVALID, "search precision must be FP32 or BF16");
@@ -467,11 +506,13 @@ extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_sea
     const uint32_t base_rows = S->rows[S->nlayers - 1];
     if (seeds) {
         if (!flat) return nnm_fail(c, NNM_ERR_INVALID, "seeds are for the flat mode");
-        for (uint64_t i = 0; i < (uint64_t)nq * p->beam; ++i)
+        if (nbeams != 1) return nnm_fail(c, NNM_ERR_INVALID, "explicit seeds take one beam per launch");
+        for (uint64_t i = 0; i < (uint64_t)nq * beams[0]; ++i)
             if (seeds[i] >= base_rows) return nnm_fail(c, NNM_ERR_INVALID, "seed outside the base layer");
     }
     NNM_HIP(c, hipSetDevice(c->device));
     if (p->prec == NNM_PREC_BF16) NNM_TRY(nnm_ensure_bf16(c));
+    const uint64_t nout = (uint64_t)nbeams * nq * p->k;
     float* dQ = nullptr;
     uint32_t *dS = nullptr, *dI = nullptr;
     float* dD = nullptr;
@@ -481,16 +522,15 @@ extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_sea
         nnm_free(c, dQ); nnm_free(c, dS); nnm_free(c, dI); nnm_free(c, dD); nnm_free(c, cnt);
     };
     if ((rc = nnm_alloc(c, (void**)&dQ, (uint64_t)nq * c->ld * 4)) ||
-        (rc = nnm_alloc(c, (void**)&dI, (uint64_t)nq * p->k * 4)) ||
-        (rc = nnm_alloc(c, (void**)&dD, (uint64_t)nq * p->k * 4)) ||
-        (rc = nnm_alloc(c, (void**)&cnt, SC_N * 8)) ||
-        (seeds && (rc = nnm_alloc(c, (void**)&dS, (uint64_t)nq * p->beam * 4)))) {
+        (rc = nnm_alloc(c, (void**)&dI, nout * 4)) || (rc = nnm_alloc(c, (void**)&dD, nout * 4)) ||
+        (rc = nnm_alloc(c, (void**)&cnt, (uint64_t)SC_N * nbeams * 8)) ||
+        (seeds && (rc = nnm_alloc(c, (void**)&dS, (uint64_t)nq * beams[0] * 4)))) {
         cleanup();
         return rc;
     }
     if ((rc = nnm_h2d_rows(c, dQ, c->ld, Q, c->d, nq)) ||
-        (seeds && (rc = nnm_h2d(c, dS, seeds, (uint64_t)nq * p->beam * 4))) ||
-        (rc = nnm_fill_u64(c, reinterpret_cast<uint64_t*>(cnt), SC_N, 0))) {
+        (seeds && (rc = nnm_h2d(c, dS, seeds, (uint64_t)nq * beams[0] * 4))) ||
+        (rc = nnm_fill_u64(c, reinterpret_cast<uint64_t*>(cnt), (uint64_t)SC_N * nbeams, 0))) {
         cleanup();
         return rc;
     }
@@ -510,12 +550,20 @@ extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_sea
     a.seeds = dS;
     a.seed = p->seed;
     a.flat = flat ? 1 : 0;
-    a.beam = p->beam;
     a.k = p->k;
     a.width = p->width;
-    a.rerank = p->rerank ? p->rerank : p->beam;
+    a.rerank = p->rerank;
     a.entry = p->entry;
     a.max_steps = 1u << 22;
+    // launch order: largest beam first (stable for equal beams)
+    uint32_t order[SR_MAXB];
+    for (uint32_t b = 0; b < nbeams; ++b) order[b] = b;
+    std::stable_sort(order, order + nbeams, [&](uint32_t x, uint32_t y) { return beams[x] > beams[y]; });
+    a.nbeams = nbeams;
+    for (uint32_t b = 0; b < nbeams; ++b) {
+        a.beams[b] = beams[order[b]];
+        a.slot[b] = order[b];
+    }
     a.out_ids = dI;
     a.out_d = dD;
     a.counters = cnt;
@@ -525,7 +573,7 @@ extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_sea
             hipSuccess ||
         per_cu < 1)
         per_cu = 4;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cus * per_cu, nq);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)c->num_cus * per_cu, (uint64_t)nq * nbeams);
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (bf16)
         hipLaunchKernelGGL(k_search<true>, dim3(grid), dim3(64), 0, c->stream, a);
@@ -537,23 +585,50 @@ extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_sea
         return nnm_hip_fail(c, e, "k_search launch");
     }
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
-    uint64_t h[SC_N];
-    if ((rc = nnm_d2h(c, ids, dI, (uint64_t)nq * p->k * 4)) || (rc = nnm_d2h(c, dist, dD, (uint64_t)nq * p->k * 4)) ||
-        (rc = nnm_d2h(c, h, cnt, sizeof(h)))) {
+    std::vector<uint64_t> h((size_t)SC_N * nbeams);
+    if ((rc = nnm_d2h(c, ids, dI, nout * 4)) || (rc = nnm_d2h(c, dist, dD, nout * 4)) ||
+        (rc = nnm_d2h(c, h.data(), cnt, h.size() * 8))) {
         cleanup();
         return rc;
     }
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     cleanup();
-    if (st) {
-        st->dists = h[SC_DISTS];
-        st->reranked = h[SC_RERANKED];
-        st->hash_resets = h[SC_RESETS];
-        st->capped = h[SC_CAPPED];
-        st->steps = h[SC_STEPS];
-        st->secs = ms * 1e-3;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0)
+        khz = 100000;
+    uint64_t ticks = 0, capped = 0;
+    for (uint32_t b = 0; b < nbeams; ++b) {
+        ticks += h[(size_t)b * SC_N + SC_TICKS];
+        capped += h[(size_t)b * SC_N + SC_CAPPED];
     }
-    if (h[SC_CAPPED]) return nnm_fail(c, NNM_ERR_STATE, "search step cap reached");
+    if (st)
+        for (uint32_t b = 0; b < nbeams; ++b) {
+            const uint64_t* hb = h.data() + (size_t)b * SC_N;
+            nnm_search_stats& o = st[order[b]];
+            o.dists = hb[SC_DISTS];
+            o.reranked = hb[SC_RERANKED];
+            o.hash_resets = hb[SC_RESETS];
+            o.capped = hb[SC_CAPPED];
+            o.steps = hb[SC_STEPS];
+            o.wave_secs = (double)hb[SC_TICKS] / ((double)khz * 1e3);
+            o.launch_secs = ms * 1e-3;
+            o.secs = nbeams == 1 ? ms * 1e-3 : ms * 1e-3 * (ticks ? (double)hb[SC_TICKS] / (double)ticks : 0.0);
+        }
+    if (capped) return nnm_fail(c, NNM_ERR_STATE, "search step cap reached");
     return NNM_OK;
+}
+
+extern "C" int nnm_search(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_search_params* p, const uint32_t* seeds,
+                          uint32_t* ids, float* dist, nnm_search_stats* st) {
+    if (!c) return NNM_ERR_INVALID;
+    if (!p) return nnm_fail(c, NNM_ERR_INVALID, "nnm_search: null argument");
+    const uint32_t beam = p->beam;
+    return search_impl(c, Q, nq, p, &beam, 1, seeds, ids, dist, st);
+}
+
+extern "C" int nnm_search_beams(nnm_ctx* c, const float* Q, uint32_t nq, const nnm_search_params* p,
+                                const uint32_t* beams, uint32_t nbeams, uint32_t* ids, float* dist,
+                                nnm_search_stats* st) {
+    return search_impl(c, Q, nq, p, beams, nbeams, nullptr, ids, dist, st);
 }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <map>
 
 namespace nnmerge {
 
@@ -359,6 +360,166 @@ unsigned* LyNNDescent::buildKGraph(float* mat, const unsigned K, const unsigned 
     return exportIds();
 }
 
+const std::vector<unsigned> LySearch::Ks1 = {60, 50, 40, 36, 32, 28, 24, 20, 18, 15, 14, 12, 10, 8, 6, 4, 2};
+
+void LySearch::check(int rc) const {
+    if (rc != NNM_OK) throw Error(rc, ctx_ ? nnm_last_error(ctx_) : "nnmerge error");
+}
+
+void LySearch::init(SpaceInterface<float>* space, int device) {
+    int rc = nnm_create(&ctx_, device);
+    if (rc != NNM_OK) throw Error(rc, "nnm_create failed");
+    if (space) check(nnm_set_metric(ctx_, space->metric()));
+}
+
+void LySearch::setLayers(const std::vector<DPGLayer>& layers) {
+    if (layers.empty()) throw Error(NNM_ERR_INVALID, "LySearch: no layers");
+    std::vector<uint32_t> rows;
+    std::vector<const uint32_t*> offs, adj;
+    static const uint32_t none = 0;
+    for (const DPGLayer& L : layers) {
+        if (L.offs.size() != (size_t)L.rows + 1) throw Error(NNM_ERR_INVALID, "LySearch: layer offsets");
+        rows.push_back(L.rows);
+        offs.push_back(L.offs.data());
+        adj.push_back(L.adj.empty() ? &none : L.adj.data());
+    }
+    check(nnm_search_set_layers(ctx_, (uint32_t)layers.size(), rows.data(), offs.data(), adj.data()));
+}
+
+LySearch::LySearch(const char* Rsrc, const char* Gsrc, SpaceInterface<float>* space, int device) {
+    init(space, device);
+    try {
+        uint64_t n = 0;
+        uint32_t d = 0;
+        check(nnm_io_matrix_shape(Rsrc, &n, &d));
+        if (n == 0 || n > 0xFFFFFFF0ull) throw Error(NNM_ERR_INVALID, "LySearch: reference set size");
+        std::vector<float> X(n * d);
+        check(nnm_io_read_matrix(Rsrc, X.data(), n, d));
+        check(nnm_set_data(ctx_, X.data(), n, d));
+        row = (unsigned)n;
+        col = d;
+        uint32_t nl = 0, lr[8];
+        uint64_t le[8];
+        check(nnm_io_layers_shape(Gsrc, 8, &nl, lr, le));
+        if (nl == 0) throw Error(NNM_ERR_INVALID, "LySearch: no layer in the graph file");
+        std::vector<DPGLayer> layers(nl);
+        std::vector<uint32_t*> po(nl), pa(nl);
+        for (uint32_t g = 0; g < nl; ++g) {
+            layers[g].rows = lr[g];
+            layers[g].offs.resize((size_t)lr[g] + 1);
+            layers[g].adj.resize(le[g]);
+            po[g] = layers[g].offs.data();
+            pa[g] = layers[g].adj.data();
+        }
+        check(nnm_io_read_layers(Gsrc, nl, po.data(), pa.data()));
+        setLayers(layers);
+    } catch (...) {
+        nnm_destroy(ctx_);
+        ctx_ = nullptr;
+        throw;
+    }
+}
+
+LySearch::LySearch(const float* mat, unsigned N, unsigned D, const std::vector<DPGLayer>& layers,
+                   SpaceInterface<float>* space, int device) {
+    init(space, device);
+    try {
+        check(nnm_set_data(ctx_, mat, N, D));
+        row = N;
+        col = D;
+        setLayers(layers);
+    } catch (...) {
+        nnm_destroy(ctx_);
+        ctx_ = nullptr;
+        throw;
+    }
+}
+
+LySearch::~LySearch() {
+    if (ctx_) nnm_destroy(ctx_);
+}
+
+std::vector<SearchRecord> LySearch::knnSearch(const float* queries, unsigned nQry, const std::vector<unsigned>& beams,
+                                              unsigned k, std::vector<uint32_t>* ids, std::vector<float>* dists) {
+    if (beams.empty()) throw Error(NNM_ERR_INVALID, "LySearch::knnSearch: no beam");
+    records.clear();
+    stats.clear();
+    std::vector<SearchRecord> out;
+    // the launch takes at most 32 beams: longer lists run in chunks of 32
+    std::vector<uint32_t> I;
+    std::vector<float> Dd;
+    if (ids) ids->assign((size_t)beams.size() * nQry * k, 0xFFFFFFFFu);
+    if (dists) dists->assign((size_t)beams.size() * nQry * k, 0.f);
+    nnm_search_params p{};
+    p.mode = NNM_SEARCH_LAYERED;
+    p.k = k;
+    p.width = width;
+    p.prec = prec;
+    p.entry = entry;
+    for (size_t b0 = 0; b0 < beams.size(); b0 += 32) {
+        const size_t nb = std::min<size_t>(32, beams.size() - b0);
+        std::vector<uint32_t> bl(beams.begin() + b0, beams.begin() + b0 + nb);
+        I.resize(nb * nQry * k);
+        Dd.resize(nb * nQry * k);
+        std::vector<nnm_search_stats> st(nb);
+        check(nnm_search_beams(ctx_, queries, nQry, &p, bl.data(), (uint32_t)nb, I.data(), Dd.data(), st.data()));
+        if (ids) std::copy(I.begin(), I.end(), ids->begin() + b0 * nQry * k);
+        if (dists) std::copy(Dd.begin(), Dd.end(), dists->begin() + b0 * nQry * k);
+        for (size_t b = 0; b < nb; ++b) {
+            stats.push_back(st[b]);
+            out.push_back(SearchRecord{bl[b], 0.0, st[b].secs, nQry ? (unsigned long)(st[b].dists / nQry) : 0ul});
+        }
+    }
+    records = out;
+    return out;
+}
+
+int LySearch::knnSearch(const char* qryFn, const char* dstFn, const char* grdFn, const char* record) {
+    uint64_t nq = 0;
+    uint32_t qd = 0;
+    check(nnm_io_matrix_shape(qryFn, &nq, &qd));
+    if (qd != col) throw Error(NNM_ERR_INVALID, "LySearch::knnSearch: query dimension differs from the reference set");
+    if (nq == 0 || nq > 0xFFFFFFF0ull) throw Error(NNM_ERR_INVALID, "LySearch::knnSearch: query count");
+    std::vector<float> Q(nq * qd);
+    check(nnm_io_read_matrix(qryFn, Q.data(), nq, qd));
+    // ground truth: "query nn" pairs; the first pair of a query counts, and
+    // every query named counts in the total (a std::map insert, as
+    // Evaluator::load_NNmap does)
+    std::map<uint64_t, uint64_t> nnmap;
+    {
+        std::ifstream g(grdFn);
+        if (!g) throw Error(NNM_ERR_INVALID, "LySearch::knnSearch: cannot read the ground truth");
+        uint64_t key = 0, val = 0;
+        while (g >> key >> val) nnmap.insert(std::make_pair(key, val));
+    }
+    std::vector<uint32_t> ids;
+    std::vector<float> dd;
+    std::vector<SearchRecord> rec = knnSearch(Q.data(), (unsigned)nq, Ks1, 1, &ids, &dd);
+    const std::vector<nnm_search_stats> all = stats;
+    std::ofstream out(record);
+    if (!out) throw Error(NNM_ERR_INVALID, "LySearch::knnSearch: cannot write the record");
+    size_t last = 0;
+    for (size_t b = 0; b < rec.size(); ++b) {
+        uint64_t hits = 0;
+        const uint32_t* r = ids.data() + b * nq;
+        for (uint64_t q = 0; q < nq; ++q) {
+            auto it = nnmap.find(q);
+            if (it != nnmap.end() && it->second == r[q]) ++hits;
+        }
+        rec[b].recall = nnmap.empty() ? 0.0 : (double)hits / (double)nnmap.size();
+        out << rec[b].topk << "\t" << rec[b].recall << "\t" << rec[b].secs << "\t" << rec[b].cmpsPerQuery << "\n";
+        last = b;
+        if (rec[b].recall == 1.0) break;
+    }
+    records.assign(rec.begin(), rec.begin() + last + 1);
+    stats.assign(all.begin(), all.begin() + last + 1);
+    // the result file after the sweep holds the last evaluated beam's top-1
+    std::ofstream dst(dstFn);
+    if (!dst) throw Error(NNM_ERR_INVALID, "LySearch::knnSearch: cannot write the results");
+    for (uint64_t q = 0; q < nq; ++q) dst << q << "\t" << ids[last * nq + q] << "\t" << dd[last * nq + q] << "\n";
+    return 0;
+}
+
 }  // namespace nnmerge
 
 extern "C" int nnm_host_hmerge(const float* X, uint64_t n, uint32_t d, const nnm_params* p, uint32_t nleaves,
@@ -417,6 +578,32 @@ extern "C" int nnm_host_hgraph(const float* X, uint64_t n, uint32_t d, const nnm
         for (uint32_t i = 0; i < L && i < max_layers; ++i)
             if (layer_rows) layer_rows[i] = g.layers[i].rows;
         if (dpg_secs) *dpg_secs = g.dpgSecs;
+        return NNM_OK;
+    } catch (const nnmerge::Error& e) {
+        return e.code;
+    } catch (...) {
+        return NNM_ERR_HIP;
+    }
+}
+
+extern "C" int nnm_host_lysearch(const char* refer, const char* graph, const char* query, const char* dst,
+                                 const char* grd, const char* record, uint32_t entry, uint32_t prec,
+                                 uint32_t max_records, uint32_t* nrecords, uint32_t* topk, double* recall, double* secs,
+                                 uint64_t* cmps) {
+    if (!refer || !graph || !query || !dst || !grd || !record) return NNM_ERR_INVALID;
+    try {
+        nnmerge::LySearch s(refer, graph);
+        s.entry = entry;
+        s.prec = prec;
+        s.knnSearch(query, dst, grd, record);
+        const uint32_t R = (uint32_t)s.records.size();
+        if (nrecords) *nrecords = R;
+        for (uint32_t i = 0; i < R && i < max_records; ++i) {
+            if (topk) topk[i] = s.records[i].topk;
+            if (recall) recall[i] = s.records[i].recall;
+            if (secs) secs[i] = s.records[i].secs;
+            if (cmps) cmps[i] = s.records[i].cmpsPerQuery;
+        }
         return NNM_OK;
     } catch (const nnmerge::Error& e) {
         return e.code;

@@ -9,6 +9,8 @@
 //   AbstractMerge::merge2one               src/abstractmerge.cpp:232-277
 //   saveKGraph (text "N K" / "i K ids")    src/abstractmerge.cpp:278-291
 //   getNeighbors -> the Neighbor list view src/pvetex.h:18-30
+//   LySearch(Rsrc, Gsrc, space)            src/lysearch.cpp:32-39
+//   LySearch::knnSearch(qry, dst, grd, rec) src/lysearch.cpp:363-437
 //
 // PMerge generalises the two-way merges to P contiguous partitions merged
 // jointly (pairs inside one built partition skipped), all partition builds
@@ -212,6 +214,67 @@ private:
     void emitLayer(unsigned maxM, bool carry, const char* layerFile);
     void runLevel(const std::vector<uint32_t>& begin, const std::vector<uint8_t>& built,
                   const std::vector<uint32_t>& group, unsigned cut);
+};
+
+// One beam's line of the search record (src/lysearch.cpp:427-429): beam,
+// recall@1 against the ground-truth file, seconds, distances per query.
+struct SearchRecord {
+    unsigned topk;
+    double recall;
+    double secs;
+    unsigned long cmpsPerQuery;  // nCmps / nQry, integer division as in the reference
+};
+
+// LySearch (src/lysearch.h:23-93): layered graph search over a reference
+// set.  The reference walks its beam list one beam after another, each a
+// loop over the queries; here the whole list is one batched launch
+// (nnm_search_beams: one wavefront per (beam, query)), and the files and
+// records are produced afterwards in the reference's order.
+class LySearch {
+public:
+    // the reference's beam list Ks1 (src/lysearch.cpp:380)
+    static const std::vector<unsigned> Ks1;
+
+    // Reference set (.txt / .fvecs / .bvecs) and layered graph file (upper
+    // layer first, LySearch::loadGraphs src/lysearch.cpp:112-186).
+    LySearch(const char* Rsrc, const char* Gsrc, SpaceInterface<float>* space = nullptr, int device = 0);
+    // In memory: row-major N x D rows and CSR layers (upper layer first).
+    LySearch(const float* mat, unsigned N, unsigned D, const std::vector<DPGLayer>& layers,
+             SpaceInterface<float>* space = nullptr, int device = 0);
+    ~LySearch();
+    LySearch(const LySearch&) = delete;
+    LySearch& operator=(const LySearch&) = delete;
+
+    // src/lysearch.cpp:363-437: queries from qryFn, the Ks1 sweep; dstFn
+    // ends with the top-1 result per query of the last beam evaluated
+    // ("i\tid\tdist" lines, printTops :684-691), record gets one line per
+    // beam ("topk\trecall\tsecs\tnCmps/nQry"), recall@1 against grdFn
+    // ("query nn" pairs, Evaluator::load_NNmap src/evaluator.cpp:488-507;
+    // Evaluator::NNRecall :199-256); the sweep stops after the first beam
+    // whose recall is 1.
+    int knnSearch(const char* qryFn, const char* dstFn, const char* grdFn, const char* record);
+    // Batched search of nQry queries at every beam of `beams` in one launch;
+    // k results per query (k <= the smallest beam), ids / dists as
+    // beams x nQry x k.  Records carry recall 0 (no ground truth here).
+    std::vector<SearchRecord> knnSearch(const float* queries, unsigned nQry, const std::vector<unsigned>& beams,
+                                        unsigned k, std::vector<uint32_t>* ids = nullptr,
+                                        std::vector<float>* dists = nullptr);
+
+    unsigned entry = 0;             // entry row of layer 0 (the reference draws rand() % rows, :378)
+    unsigned prec = NNM_PREC_FP32;  // NNM_PREC_BF16: bf16 rows with FP32 re-rank of the results
+    unsigned width = 1;             // list entries expanded per step (1 = the reference's order)
+    std::vector<SearchRecord> records;   // of the last knnSearch
+    std::vector<nnm_search_stats> stats;  // of the last knnSearch, per beam
+    unsigned rows() const { return row; }
+    unsigned dim() const { return col; }
+
+private:
+    void check(int rc) const;
+    void init(SpaceInterface<float>* space, int device);
+    void setLayers(const std::vector<DPGLayer>& layers);
+
+    nnm_ctx* ctx_ = nullptr;
+    unsigned row = 0, col = 0;
 };
 
 }  // namespace nnmerge

@@ -57,7 +57,8 @@ class SearchParams(C.Structure):
 
 class SearchStats(C.Structure):
     _fields_ = [("dists", C.c_uint64), ("reranked", C.c_uint64), ("hash_resets", C.c_uint64),
-                ("capped", C.c_uint64), ("steps", C.c_uint64), ("secs", C.c_double)]
+                ("capped", C.c_uint64), ("steps", C.c_uint64), ("secs", C.c_double), ("wave_secs", C.c_double),
+                ("launch_secs", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -134,6 +135,10 @@ _SIGS = {
     "nnm_search_use_graph": (C.c_int, [_vp, C.c_int]),
     "nnm_search": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(SearchParams), _vp, _vp, _vp,
                              C.POINTER(SearchStats)]),
+    "nnm_search_beams": (C.c_int, [_vp, _vp, C.c_uint32, C.POINTER(SearchParams), _vp, C.c_uint32, _vp, _vp,
+                                   _vp]),
+    "nnm_host_lysearch": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                    C.c_uint32, C.c_uint32, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
 }
 
 
@@ -399,6 +404,20 @@ class Context:
         self._chk(lib().nnm_search(self.h, _ptr(Q), nq, C.byref(p), _ptr(sd), _ptr(ids), _ptr(dist), C.byref(st)))
         return ids, dist, st.as_dict()
 
+    def search_beams(self, Q, beams, k=10, width=1, prec=PREC_FP32, mode=SEARCH_LAYERED, entry=0, seed=1, rerank=0):
+        """Every beam of `beams` in one launch: ids / dists (nbeams x nq x k)
+        and one stats dict per beam."""
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        nq = Q.shape[0]
+        bl = np.ascontiguousarray(beams, dtype=np.uint32)
+        p = SearchParams(mode=mode, beam=0, k=k, width=width, prec=prec, rerank=rerank, entry=entry, seed=seed)
+        ids = np.empty((bl.size, nq, k), np.uint32)
+        dist = np.empty((bl.size, nq, k), np.float32)
+        st = (SearchStats * bl.size)()
+        self._chk(lib().nnm_search_beams(self.h, _ptr(Q), nq, C.byref(p), _ptr(bl), bl.size, _ptr(ids), _ptr(dist),
+                                         C.cast(st, C.c_void_p)))
+        return ids, dist, [s.as_dict() for s in st]
+
     def pair_dists(self, a, b):
         a = np.ascontiguousarray(a, dtype=np.uint32)
         b = np.ascontiguousarray(b, dtype=np.uint32)
@@ -408,6 +427,22 @@ class Context:
 
 
 HOST_KGRAPH, HOST_SMERGE, HOST_JMERGE, HOST_PMERGE = 0, 1, 2, 3
+
+
+def host_lysearch(refer, graph, query, dst, grd, record, entry=0, prec=PREC_FP32):
+    """LySearch(refer, graph).knnSearch(query, dst, grd, record) through the
+    host class; returns the records [(topk, recall, secs, cmps_per_query)]."""
+    n = 32
+    topk = np.zeros(n, np.uint32)
+    rec = np.zeros(n, np.float64)
+    secs = np.zeros(n, np.float64)
+    cmps = np.zeros(n, np.uint64)
+    nr = C.c_uint32()
+    rc = lib().nnm_host_lysearch(*(str(x).encode() for x in (refer, graph, query, dst, grd, record)), entry, prec,
+                                 n, C.byref(nr), _ptr(topk), _ptr(rec), _ptr(secs), _ptr(cmps))
+    if rc != NNM_OK:
+        raise NNMError(rc, "nnm_host_lysearch failed")
+    return [(int(topk[i]), float(rec[i]), float(secs[i]), int(cmps[i])) for i in range(min(nr.value, n))]
 
 
 def host_build(kind, X, p, parts=0):

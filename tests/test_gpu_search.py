@@ -205,3 +205,63 @@ def test_search_rejects_bad_input(ctx, data, layers):
     bad = [(np.array([0, 1], np.uint32), np.array([5], np.uint32))]
     with pytest.raises(nnm.NNMError):
         ctx.search_set_layers(bad)  # neighbour outside the layer
+
+
+@pytest.mark.parametrize("prec", [nnm.PREC_FP32, nnm.PREC_BF16])
+def test_beams_in_one_launch_equal_single_beam_searches(ctx, data, layers, prec):
+    """nnm_search_beams: every beam of the list in one launch gives, beam for
+    beam, the results and counts of a launch of that beam alone (layered
+    mode from one entry row, and the flat mode with drawn seeds); each beam
+    gets its share of the launch time."""
+    X, Q = data
+    beams = [64, 16, 256, 32, 128, 48]
+    _set_layers(ctx, X, layers)
+    for mode in (nnm.SEARCH_LAYERED, nnm.SEARCH_FLAT):
+        bi, bd, bst = ctx.search_beams(Q, beams, k=10, prec=prec, mode=mode, entry=7, seed=3)
+        for j, beam in enumerate(beams):
+            ids, dist, st = ctx.search(Q, beam, k=10, prec=prec, mode=mode, entry=7, seed=3)
+            assert np.array_equal(bi[j], ids), (mode, beam)
+            assert np.array_equal(bd[j].view(np.uint32), dist.view(np.uint32)), (mode, beam)
+            for f in ("dists", "reranked", "hash_resets", "steps", "capped"):
+                assert bst[j][f] == st[f], (mode, beam, f)
+        assert abs(sum(s["secs"] for s in bst) - bst[0]["launch_secs"]) < 1e-6
+        assert all(s["wave_secs"] > 0 for s in bst)
+        print("mode", mode, [(b, round(s["secs"] * 1e3, 3)) for b, s in zip(beams, bst)])
+
+
+def test_lysearch_host_class_files(ctx, data, layers, tmp_path):
+    """LySearch(refer, graph).knnSearch(query, dst, grd, record) through the
+    host class: the reference's beam list in one launch, one record line per
+    beam with recall@1 against the "query nn" ground truth and distances per
+    query, stopping after the first beam of recall 1; dst holds the last
+    beam's top-1.  Every line equals a single-beam search of that beam."""
+    X, Q = data
+    refer, graph, query = tmp_path / "refer.txt", tmp_path / "graph.txt", tmp_path / "query.txt"
+    dst, grd, record = tmp_path / "dst.txt", tmp_path / "grd.txt", tmp_path / "record.txt"
+    nnm.write_matrix(str(refer), X)
+    nnm.write_matrix(str(query), Q)
+    lay = [(o, a) for _, _, (o, a, _), _ in layers]
+    nnm.write_layers(str(graph), lay)
+    gt, _ = orc.bruteforce(X, 1, Q=Q)
+    with open(grd, "w") as f:
+        for q in range(Q.shape[0]):
+            f.write(f"{q} {gt[q, 0]}\n")
+    recs = nnm.host_lysearch(refer, graph, query, dst, grd, record, entry=7)
+    ks1 = [60, 50, 40, 36, 32, 28, 24, 20, 18, 15, 14, 12, 10, 8, 6, 4, 2]
+    assert [r[0] for r in recs] == ks1[:len(recs)]
+    assert len(recs) == len(ks1) or recs[-1][1] == 1.0
+    assert all(r[1] < 1.0 for r in recs[:-1])
+    lines = [ln.split("\t") for ln in open(record).read().splitlines()]
+    assert [int(ln[0]) for ln in lines] == [r[0] for r in recs]
+    ctx.set_data(X)
+    ctx.search_set_layers(lay)
+    for topk, rec, secs, cmps in recs:
+        ids, dist, st = ctx.search(Q, topk, k=1, entry=7)
+        assert rec == float(np.mean(ids[:, 0] == gt[:, 0]))
+        assert cmps == st["dists"] // Q.shape[0]
+        assert secs > 0
+    out = [ln.split("\t") for ln in open(dst).read().splitlines()]
+    assert [int(o[1]) for o in out] == ids[:, 0].tolist()
+    # distances as the reference prints them (default stream precision)
+    assert np.allclose([float(o[2]) for o in out], dist[:, 0], rtol=1e-5, atol=0)
+    print("records", recs)
