@@ -389,11 +389,16 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
     // with d <= 128 every partial sum of the norm form stays below 2^24
     int exact = (d <= 128) ? 1 : 0;
     const uint64_t total = n * (uint64_t)d;
-    for (uint64_t i = 0; i < total && exact; ++i) {
-        float v = X[i];
-        if (!(v >= -256.f && v <= 256.f) || v != std::nearbyint(v)) exact = 0;
+    float vmin = X[0], vmax = X[0];
+    for (uint64_t i = 0; i < total; ++i) {
+        const float v = X[i];
+        vmin = v < vmin ? v : vmin;
+        vmax = v > vmax ? v : vmax;
+        if (exact && (!(v >= -256.f && v <= 256.f) || v != std::nearbyint(v))) exact = 0;
     }
     c->data_exact_bf16 = exact;
+    c->data_min = vmin;
+    c->data_max = vmax;
     // default partition: one built subgraph, one group
     uint32_t sb[2] = {0u, (uint32_t)n};
     uint8_t bu[1] = {1};

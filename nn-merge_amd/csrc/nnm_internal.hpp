@@ -84,6 +84,7 @@ struct nnm_ctx {
     uint16_t* Xb = nullptr;   // bf16 copy (lazy)
     float* norms = nullptr;   // squared norms of the bf16 rows (lazy)
     int data_exact_bf16 = 0;  // every value an integer in [-256, 256]
+    float data_min = 0.f, data_max = 0.f;  // value range of the rows (with the exactness scan)
     uint32_t metric = NNM_METRIC_L2;
     float* sumsq = nullptr;   // cosine: squared row norms (with the data)
 

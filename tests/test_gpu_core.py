@@ -58,6 +58,32 @@ def test_bruteforce_query_vectors(ctx):
     assert np.array_equal(gd.astype(np.float64), od)
 
 
+@pytest.mark.parametrize("d,vmax", [(20, 255), (64, 3), (96, 255), (128, 255), (128, 1)])
+def test_bruteforce_mfma_form_bit_exact(ctx, d, vmax):
+    """The MFMA form (integer rows: bf16 operands, FP32 norm form, exact)
+    against the FP64 oracle, ids and distances bit for bit with ties broken
+    by id: row widths of one to four 32-element steps, tiny value ranges
+    (mostly ties), lists of 16 and 32, query counts around the 256-query
+    block, query rows (self excluded) and query vectors."""
+    rng = np.random.default_rng(d * 7 + vmax)
+    n = 2100
+    X = rng.integers(0, vmax + 1, size=(n, d)).astype(np.float32)
+    ctx.set_data(X)
+    for k in (1, 10, 16, 17, 32):
+        for nq in (1, 255, 257):
+            rows = rng.choice(n, nq, replace=False).astype(np.uint32)
+            gi, gd = ctx.bruteforce(k, rows=rows)
+            oi, od = orc.bruteforce(X, k, rows=rows)
+            assert np.array_equal(gi, oi), (k, nq)
+            assert np.array_equal(gd.astype(np.float64), od), (k, nq)
+    Q = rng.integers(0, vmax + 1, size=(300, d)).astype(np.float32)
+    for k in (10, 32):
+        gi, gd = ctx.bruteforce(k, Q=Q)
+        oi, od = orc.bruteforce(X, k, Q=Q)
+        assert np.array_equal(gi, oi), k
+        assert np.array_equal(gd.astype(np.float64), od), k
+
+
 def test_pair_dists_match_reference_order_on_integers(ctx):
     X = nnm_data.sift_like(500, 128, centers=8, seed=15)
     ctx.set_data(X)
