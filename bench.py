@@ -191,12 +191,16 @@ def summarize(dirs, picks):
     print("\n".join(out))
 
 
+SEARCH_BEAMS = (32, 48, 64, 96, 128, 192, 256)
+
+
 def run_search(nnm, ctx, Q, args):
     """Batched search on the merged graph (LySearch on a DPG layer): the DPG
     of the final k-NN lists (M = K/2, maxM = K, LyNNDescent::dpg on the base
     layer) under a complete graph on a row sample, then the layered search of
-    every query at each beam (LySearch::knnSearch), bf16 rows with FP32
-    re-rank.  recall@10 against exact brute force of the queries."""
+    every query at every beam in one launch (LySearch::knnSearch's beam
+    sweep), bf16 rows with FP32 re-rank.  recall@10 against exact brute force
+    of the queries; each beam's time is its share of the launch (wave time)."""
     t0 = time.time()
     gt, _ = ctx.bruteforce(10, Q=Q)
     log(f"search ground truth for {Q.shape[0]} queries in {time.time() - t0:.1f}s")
@@ -210,15 +214,19 @@ def run_search(nnm, ctx, Q, args):
     ctx.search_set_layers([(top_offs, top_adj), (base[0], base[1])])
     del base
     qrows = np.arange(Q.shape[0], dtype=np.uint32)
+    ctx.search_beams(Q[:2000], SEARCH_BEAMS, k=10, width=args.search_width, prec=nnm.PREC_BF16, entry=0)  # warm
+    ids, _, sts = ctx.search_beams(Q, SEARCH_BEAMS, k=10, width=args.search_width, prec=nnm.PREC_BF16, entry=0)
+    launch = sts[0]["launch_secs"]
     res = {"queries": int(Q.shape[0]), "graph": f"complete graph on {S0} rows over the DPG of the merged k-NN graph",
            "dpg_secs": round(st["secs"], 4), "mode": "layered (greedy top, best-first base)",
-           "prec": "bf16 + FP32 re-rank", "width": args.search_width, "beams": []}
-    for beam in (32, 48, 64, 96, 128, 192, 256):
-        ctx.search(Q[:1000], beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, entry=0)
-        ids, _, ss = ctx.search(Q, beam, k=10, width=args.search_width, prec=nnm.PREC_BF16, entry=0)
-        r = float(nnm.recall(ids, gt, qrows, 10))
+           "prec": "bf16 + FP32 re-rank", "width": args.search_width, "beams_in_one_launch": list(SEARCH_BEAMS),
+           "launch_secs": round(launch, 5), "queries_per_sec_all_beams": round(len(SEARCH_BEAMS) * Q.shape[0] / launch, 1),
+           "beams": []}
+    for j, beam in enumerate(SEARCH_BEAMS):
+        r = float(nnm.recall(ids[j], gt, qrows, 10))
+        ss = sts[j]
         row = {"beam": beam, "recall@10": round(r, 5), "qps": round(Q.shape[0] / ss["secs"], 1),
-               "kernel_secs": round(ss["secs"], 5), "dists_per_query": round(ss["dists"] / Q.shape[0], 1),
+               "kernel_secs_share": round(ss["secs"], 5), "dists_per_query": round(ss["dists"] / Q.shape[0], 1),
                "hash_resets": int(ss["hash_resets"])}
         res["beams"].append(row)
         log(f"[search] {row}")
@@ -278,8 +286,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0 = every host core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true")
-    ap.add_argument("--search", action="store_true",
-                    help="also measure batched graph search on the merged graph (DPG layer, layered search)")
+    ap.add_argument("--no-search", action="store_true",
+                    help="skip the batched graph search on the merged graph (DPG layer, layered search)")
     ap.add_argument("--search-queries", type=int, default=100_000)
     ap.add_argument("--search-width", type=int, default=1)
     ap.add_argument("--search-top", type=int, default=2048, help="rows of the complete top search layer")
@@ -309,7 +317,7 @@ def main():
     rank = args.rank if args.rank > 0 else None
     t0 = time.time()
     Qs = None
-    if args.search:
+    if not args.no_search:
         X, Qs = nnm_data.sift_like(n, d, centers=args.centers, sigma=20.0, seed=nnm_data.SEED_BASE + 4,
                                    rank=rank, nq=args.search_queries)
     else:
@@ -496,7 +504,7 @@ def main():
         chosen = args.prec
     ch = runs[chosen]
     search = None
-    if args.search:
+    if not args.no_search:
         search = run_search(nnm, ctx, Qs, args)
     ctx.close()
 
