@@ -1376,7 +1376,8 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
                                                   uint32_t cap_new, uint32_t* __restrict__ l_old, uint32_t cap_old,
                                                   uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                                                   uint64_t* __restrict__ newmask, uint32_t* __restrict__ rcnt,
-                                                  uint32_t* __restrict__ rbuf, uint64_t* __restrict__ spill) {
+                                                  uint32_t* __restrict__ rbuf, uint64_t* __restrict__ spill,
+                                                  uint64_t* __restrict__ counters) {
     constexpr uint32_t GPW = NNM_WAVE / G;  // rows per wave
     const uint32_t lane = nnm_lane(), grp = lane / G, gl = lane % G;
     const uint64_t gmask = (G == NNM_WAVE) ? ~0ull : (((1ull << G) - 1) << (grp * G));
@@ -1387,6 +1388,7 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
     // gathers and bucket atomics issued together: the step waits for each
     // round trip once for both.
     constexpr int R = 2;
+    uint64_t kept_new = 0;  // new flags left in the lists (the delta test's base)
     for (uint64_t v0 = wid * GPW; v0 < n; v0 += (uint64_t)R * nw * GPW) {
         uint64_t v[R];
         bool act[R];
@@ -1419,6 +1421,11 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
                 l_new[v[k] * cap_new + frank] = u[k];
                 g_flag[v[k] * K + gl] = 0;
             }
+            // new entries past the sample stay new; the mark of an entry
+            // that entered in the last delivery (2) is stored as 1
+            const bool stays = act[k] && f[k] && !is_new;
+            if (stays && f[k] == 2) g_flag[v[k] * K + gl] = 1;
+            kept_new += (uint32_t)__popcll(nnm_ballot(stays));
             if (is_old) l_old[v[k] * cap_old + (uint32_t)__popcll(om & below)] = u[k];
             const uint64_t nm = nnm_ballot(is_new) & gmask;
             if (gl == 0 && v[k] < n) {
@@ -1447,6 +1454,7 @@ __global__ __launch_bounds__(256) void k_lb_build(uint64_t n, uint32_t K, uint32
             if (gl == 0 && v[k] < n) spill[v[k]] = (G == NNM_WAVE) ? sm : (sm >> (grp * G));
         }
     }
+    if (nnm_lane() == 0 && kept_new) atomicAdd((unsigned long long*)&counters[CNT_FLAGS], (unsigned long long)kept_new);
 }
 
 // Entries past each full bucket: their count (scanned into offsets into
@@ -1807,11 +1815,11 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     if (K <= 32) {
         hipLaunchKernelGGL(k_lb_build<32>, dim3(lb_grid), dim3(256), 0, c->stream, n, K, S, c->g_ids, c->g_dist,
                            c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new, c->n_old,
-                           c->newmask, c->rcnt, c->rbuf, c->spill);
+                           c->newmask, c->rcnt, c->rbuf, c->spill, c->counters);
     } else {
         hipLaunchKernelGGL(k_lb_build<64>, dim3(wave_grid(c, n)), dim3(256), 0, c->stream, n, K, S, c->g_ids,
                            c->g_dist, c->kth, c->g_flag, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,
-                           c->n_old, c->newmask, c->rcnt, c->rbuf, c->spill);
+                           c->n_old, c->newmask, c->rcnt, c->rbuf, c->spill, c->counters);
     }
     NNM_LAUNCH_CHECK(c);
     const uint64_t m = 2 * n;
@@ -1931,6 +1939,13 @@ static int autotune_iteration(nnm_ctx* c, const nnm_params* p, double* join_ms) 
     NNM_TRY(nnm_offer_deliver(c, p));
     NNM_HIP(c, hipEventRecord(c->evd1, c->stream));
     return add_join_times(c, join_ms);
+}
+
+// The marks of entries that entered in the last delivery (flag 2) stored as
+// plain new flags (1), for readers after the loop.
+__global__ void k_flag_norm(uint64_t m, uint8_t* __restrict__ g_flag) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        if (g_flag[i] == 2) g_flag[i] = 1;
 }
 
 // Instantiated iteration graphs of one descent call (one per join precision).
@@ -2107,6 +2122,11 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         if (p->max_iters && it >= p->max_iters) break;
     }
     if (estimates) NNM_TRY(to_exact());  // a run that ended in the learned phase
+    {
+        const uint64_t m = c->n_act * c->K;
+        hipLaunchKernelGGL(k_flag_norm, dim3(grid_for(m, 256, 8192)), dim3(256), 0, c->stream, m, c->g_flag);
+        NNM_LAUNCH_CHECK(c);
+    }
     NNM_HIP(c, hipEventRecord(c->ev1, c->stream));
     uint64_t h[CNT_N];
     NNM_TRY(read_counters(c, h));

@@ -453,10 +453,13 @@ __device__ inline void own_cswap(uint64_t& x, uint64_t& y) {
 //
 // `perm` is 64 words of LDS owned by the wave.  A target whose candidates
 // span several chunks of its bin (or several redo rounds) is merged once per
-// chunk; entries that entered in an earlier merge carry flag 2 until the
-// last chunk of the iteration's last round (last = true), which counts the
-// entries that entered (the iteration's updates) and the new flags, and
-// stores every flag as 0 or 1.
+// chunk.  Entries that entered in this iteration's delivery are stored with
+// flag 2 (new, and entered now); the next list build stores them as 1.  Each
+// merge adds its change of the row's new flags and of its entered entries to
+// the tallies: summed over the delivery, the entries that entered and
+// stayed (the iteration's updates, the same count in whatever order the
+// candidates came) and the change of the new flags since the list build,
+// which counted the rest (the delta test).
 //
 // IDD: one id may come with different keys (lower bounds from the bf16 join
 // and exact distances from the FP32 long-list join, without re-rank), so
@@ -482,7 +485,7 @@ __device__ inline OwnList own_load(const OwnArgs& a, uint64_t v) {
 
 template <bool RR, bool TIE, bool IDD>
 __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const OwnList& in, const uint64_t* keys, uint32_t len,
-                                 bool last, uint32_t* perm, OwnTally& t) {
+                                 uint32_t* perm, OwnTally& t) {
     const uint32_t lane = nnm_lane();
     const uint32_t K = a.K;
     const uint64_t lmask = K < NNM_WAVE ? (1ull << K) - 1 : ~0ull;  // list lanes
@@ -491,7 +494,8 @@ __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const OwnList& in
     const uint32_t fresh0 = lane < K && pf == 2 ? 1u : 0u;
     uint32_t fresh = fresh0;  // the entry entered in this iteration's delivery
     pf = pf ? 1u : 0u;
-    bool changed = last && __ballot(fresh) != 0;  // flags 2 are stored as 1 on the last chunk
+    const uint32_t f0 = (uint32_t)__popcll(__ballot(lane < K && pf)), e0 = (uint32_t)__popcll(__ballot(fresh));
+    bool changed = false;
     for (uint32_t g0 = 0; g0 < len; g0 += 4 * NNM_WAVE) {
         uint64_t kk[4];
 #pragma unroll
@@ -627,24 +631,22 @@ __device__ inline void own_merge(const OwnArgs& a, uint64_t v, const OwnList& in
             }
         }
     }
-    if (changed && lane < K) {
-        a.g_ids[v * K + lane] = pid;
-        a.g_dist[v * K + lane] = pd;
-        a.g_flag[v * K + lane] = (uint8_t)(!last && fresh ? 2u : pf);
-    }
-    if (last) {
-        // entries replaced: the final list's ids that were not listed when
-        // the delivery started (the same count in whatever order the
-        // candidates came), and the new flags for the delta test
-        t.upd += (uint32_t)__popcll(__ballot(lane < K && fresh));
-        t.flags += (uint32_t)__popcll(__ballot(lane < K && pf));
+    if (changed) {
+        if (lane < K) {
+            a.g_ids[v * K + lane] = pid;
+            a.g_dist[v * K + lane] = pd;
+            a.g_flag[v * K + lane] = (uint8_t)(fresh ? 2u : pf);
+        }
+        // wrap-around sums: the changes are added modulo 2^64
+        t.upd += (uint64_t)__popcll(__ballot(lane < K && fresh)) - e0;
+        t.flags += (uint64_t)__popcll(__ballot(lane < K && pf)) - f0;
     }
 }
 
-// One workgroup per bin of 64 targets (persistent over the bins): the bin's
+// One workgroup per bin of 32 targets (persistent over the bins): the bin's
 // records in chunks of OWN_CH, each counting-sorted by target in LDS, then
-// one wave per target merges its run.  Every target of a bin is visited on
-// its last chunk to count its new flags (the delta test).
+// one wave per target with records merges its run.  Targets without records
+// are not visited (the list build counted their new flags).
 template <bool RR, bool TIE, bool IDD>
 __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
     __shared__ uint64_t skey[OWN_CH];
@@ -653,8 +655,6 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
     __shared__ uint32_t perm[4][NNM_WAVE];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     OwnTally t;
-    // vertices left for a redo round: this round is not the iteration's last
-    const bool final_round = a.counters[CNT_REDO] == 0;
     for (uint32_t bin = blockIdx.x; bin < a.nbins; bin += gridDim.x) {
         const uint32_t s = bin ? a.bin_end[(uint64_t)(bin - 1) * a.bin_stride] : 0u;
         const uint32_t e = a.bin_end[(uint64_t)bin * a.bin_stride];
@@ -663,7 +663,6 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
         for (;;) {
             const uint32_t ce = min(e, c0 + OWN_CH), m = ce - c0;
             const bool last_chunk = ce >= e;
-            const bool last = last_chunk && final_round;
             if (tid < OWN_T) hcnt[tid] = 0;
             __syncthreads();
             uint64_t key[OWN_CH / 256];
@@ -701,7 +700,7 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
             __syncthreads();
             // the wave's targets q = wave, wave + 4, ...; the next one's list
             // is loaded while the current one merges
-            auto need = [&](uint32_t q) { return t0 + q < a.n && (hcnt[q] != 0 || last); };
+            auto need = [&](uint32_t q) { return t0 + q < a.n && hcnt[q] != 0; };
             uint32_t q = wave;
             while (q < OWN_T && !need(q)) q += 4;
             OwnList cur = q < OWN_T ? own_load(a, t0 + q) : OwnList();
@@ -709,7 +708,7 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
                 uint32_t qn = q + 4;
                 while (qn < OWN_T && !need(qn)) qn += 4;
                 const OwnList nxt = qn < OWN_T ? own_load(a, t0 + qn) : OwnList();
-                own_merge<RR, TIE, IDD>(a, t0 + q, cur, skey + hoff[q], hcnt[q], last, perm[wave], t);
+                own_merge<RR, TIE, IDD>(a, t0 + q, cur, skey + hoff[q], hcnt[q], perm[wave], t);
                 cur = nxt;
                 q = qn;
             }
