@@ -135,7 +135,8 @@ def summarize(dirs, picks):
     picks: 'REGEX@a:b' items naming dispatches of one kernel (ordered by
     dispatch id, sliced a:b); their union is the kernel instance summarised:
     average duration from a kernel trace, average FETCH_SIZE / WRITE_SIZE
-    from counter runs, scaled to bytes by the 1 GiB copy of --copy-calib.
+    from counter runs, scaled to bytes by the 1 GiB copy of --copy-calib,
+    and the per-dispatch average of any other counter or derived metric.
     """
     out = []
     sel_counters, sel_durs = {}, []
@@ -181,6 +182,10 @@ def summarize(dirs, picks):
             out.append(f"avg_duration_ms: {sum(sel_durs) / len(sel_durs):.4f}  (dispatches: {len(sel_durs)})")
         total = 0.0
         for cn, (v, c, fac) in sorted(sel_counters.items()):
+            if cn not in ("FETCH_SIZE", "WRITE_SIZE"):
+                # other counters and derived metrics: the average per dispatch
+                out.append(f"{cn}: {v:.6g} per dispatch (dispatches: {c})")
+                continue
             fs = f"{fac:.4f}" if fac else "n/a"
             b = v * 1024 * (fac or 1.0)
             total += b
