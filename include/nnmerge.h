@@ -157,7 +157,7 @@ typedef struct nnm_stats {
                                  form); 8 per offer; 8 per vertex */
     uint64_t join_main_flops; /* its distance flops: 3d per FP32 distance
                                  (difference form), 2d per MFMA distance */
-    uint64_t near_ties;    /* k-th slot decisions re-made on FP64 distances
+    uint64_t near_ties;    /* k-th entry decisions re-made on FP64 distances
                               (FP32 distances within rounding of each other) */
     uint64_t tie_swaps;    /* ... where FP64 kept the other entry */
     uint64_t redo_rows;    /* vertices joined again because the offer log

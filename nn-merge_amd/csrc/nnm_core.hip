@@ -7,10 +7,10 @@
 //                     Jmerge::InitialGraph        src/jmerge.cpp:83-176
 //                     LyNNDescent::appdNwSmpl     src/lynndescent.cpp:354-452
 //   k_merge_finish    AbstractMerge::merge2one    src/abstractmerge.cpp:232-277
-//   k_lb_count/fill   list build of nnDescent     src/abstractmerge.cpp:86-137
+//   k_lb_build        list build of nnDescent     src/abstractmerge.cpp:86-137
+//                     (+ the new flags it leaves, EvaluateDelta src/abstractmerge.cpp:194-202)
 //   k_lb_select       reverse sampling            src/abstractmerge.cpp:139-166
-//   k_update          UpdateKnnList per pair      src/pvetex.h:40-85, 99-109
-//                     + EvaluateDelta             src/abstractmerge.cpp:194-202
+//   (nnm_offers.hip)  UpdateKnnList per pair      src/pvetex.h:40-85, 99-109
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

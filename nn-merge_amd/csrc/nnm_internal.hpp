@@ -59,7 +59,7 @@ enum CounterIdx {
     CNT_LONG_DISTS = 11,    // share of DISTS / LIST_ENTRIES / OFFERS that the
     CNT_LONG_ENTRIES = 12,  // long-list join kernels account for (the rest is
     CNT_LONG_OFFERS = 13,   // the short-list kernel's)
-    CNT_NEAR_TIES = 14,     // k-th slot decisions re-made in FP64
+    CNT_NEAR_TIES = 14,     // k-th entry decisions re-made in FP64
     CNT_TIE_SWAPS = 15,     // ... that changed which entry stays
     CNT_OCHUNKS = 16,       // offer-log chunks reserved in this round
     CNT_REDO = 17,          // vertices whose offers did not fit the log (joined again)
@@ -634,7 +634,7 @@ __device__ inline uint32_t nnm_wave_rnd_sds(uint64_t seed, uint64_t stream, uint
 // through the offer log: each wave writes its offers in order into a log
 // chunk of NNM_OCH records that it reserved with one atomic on a shared
 // counter, so a wave store covers consecutive records and no offer ever
-// meets another in a hash slot.  Partition passes then group the records by
+// overwrites another.  Partition passes then group the records by
 // target and the owners merge them (nnm_offers.hip).  When the log has no
 // chunk left the wave records the vertices it could not finish in the redo
 // list; they are joined again in a later round (their earlier offers then

@@ -6,14 +6,16 @@
 //
 //   join      each wave appends its offers {dist, id, target} to log chunks
 //             it reserved (nnm_internal.hpp, offer_emit)
-//   passes    one or two partition passes group the records by target:
-//             a pass counts the records of each bin (per 4096-record tile an
-//             LDS histogram, one atomic per nonempty bin), scans the counts,
-//             then scatters each tile's records to their bins (LDS ranks, one
-//             cursor atomic per nonempty bin of the tile); the last pass's
-//             bins are blocks of 64 targets
-//   owners    one workgroup per 64-target bin sorts the bin's records by
-//             target in LDS and one wave per target merges the target's
+//   passes    one or two partition passes group the records by target.
+//             The first pass counts each block's records per bin into a
+//             bin-major table, scans it and scatters every tile through LDS
+//             (ranks in LDS, runs written out; no atomics); a second pass, when
+//             the targets need more digits, counts per-tile LDS histograms
+//             (one atomic per nonempty bin) and scatters with one cursor
+//             atomic per nonempty bin of a tile.  The last pass's bins are
+//             blocks of 32 targets
+//   owners    one workgroup per 32-target bin sorts the bin's records by
+//             target in LDS and one wave per target with records merges its
 //             candidates into its list: de-dup by id (candidates and list),
 //             candidates below the k-th key enter at their rank by (dist,
 //             id) -- UpdateKnnList's rule for the whole batch.  Only the
