@@ -39,6 +39,9 @@ struct nnm_mlp_state {
     float *xs = nullptr, *pre = nullptr, *h = nullptr, *z = nullptr, *dz = nullptr, *dh = nullptr;
     float *t = nullptr, *s = nullptr, *loss = nullptr;
     uint32_t* pairs = nullptr;
+    // gradient parts of the fused step: one nparam slice per block of 32 pairs
+    float* part = nullptr;
+    uint64_t part_cap = 0;
     // bf16 (RNE) copies of W1 and W2 for the projection, then the projection
     // of every row (bf16) and its squared norms
     uint16_t* Wb = nullptr;
@@ -204,6 +207,264 @@ __global__ void k_mlp_pairs(uint32_t B, uint64_t n, uint32_t K, uint32_t ld, con
     s[i] = alpha * alpha * g_dist[(uint64_t)a * K + K - 1];
 }
 
+// ------------------------------------------------------------ fused step
+//
+// One training step in two launches instead of eleven.  k_mlp_step: a block
+// of 256 threads takes 32 pairs (64 rows): gathers them (as k_mlp_pairs), or
+// reads caller rows; forward (pre = x W1^T + b1, h = relu, z = h W2^T + b2)
+// on 16x16x16 bf16 MFMA with the operands in LDS and the bf16 weights in
+// global memory; the pair loss and dz; backward (dh = dz W2 masked by relu',
+// then the block's gradient parts dz^T h, colsum dz, dh^T x, colsum dh), each
+// part written to the block's slot of `part`.  k_mlp_reduce sums the parts
+// over the blocks in a fixed order and either stores the gradient or takes an
+// Adam step and repacks the bf16 weights for the next step.  Operands are
+// rounded to bf16 exactly as k_gemm<true> rounds them; column sums and the
+// loss are FP32, in a fixed order (deterministic).
+#define MS_ROWS 64
+#define MS_DMAX 256
+#define MS_HMAX 256
+#define MS_OMAX 64
+
+struct MlpStepLds {
+    uint16_t xb[MS_ROWS * (MS_DMAX + 4)];
+    uint16_t hb[MS_ROWS * (MS_HMAX + 4)];
+    uint16_t db[MS_ROWS * (MS_HMAX + 4)];
+    float z[MS_ROWS * (MS_OMAX + 1)];
+    float dz[MS_ROWS * (MS_OMAX + 1)];
+    uint32_t row_id[MS_ROWS];
+};
+
+__device__ inline uint16_t ms_bf(float x) { return (uint16_t)mlp_to_bf16(x); }
+
+// part: per block nparam floats [gW1 (H x D) | gb1 (H) | gW2 (O x H) | gb2 (O)]
+__global__ __launch_bounds__(256) void k_mlp_step(uint32_t B, uint32_t D, uint32_t H, uint32_t O,
+                                                  const uint16_t* __restrict__ W1b, const float* __restrict__ b1,
+                                                  const uint16_t* __restrict__ W2b, const float* __restrict__ b2,
+                                                  // pair source: gather (X != nullptr) or rows xs (2B x D)
+                                                  const float* __restrict__ X, uint32_t ld, uint64_t n, uint32_t K,
+                                                  const uint32_t* __restrict__ ids, const float* __restrict__ g_dist,
+                                                  const uint32_t* __restrict__ label, float alpha, uint64_t seed,
+                                                  const float* __restrict__ xs, float* __restrict__ t_io,
+                                                  float* __restrict__ s_io, float* __restrict__ loss,
+                                                  float* __restrict__ part, uint64_t nparam) {
+    __shared__ __align__(16) MlpStepLds L;
+    __shared__ float ts[MS_ROWS / 2], ss[MS_ROWS / 2];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const uint32_t lr = lane & 15u, lg = lane >> 4;
+    const uint32_t PD = D + 4, PH = H + 4, PO = O + 1;
+    const uint32_t pair0 = blockIdx.x * (MS_ROWS / 2);
+    // ---- pairs: ids, t, s (k_mlp_pairs' draws)
+    if (tid < MS_ROWS / 2) {
+        const uint32_t i = pair0 + tid;
+        float t = 0.f, sv = 0.f;
+        uint32_t a = 0, b = 0;
+        if (i < B) {
+            if (X) {
+                const uint64_t r1 = nnm_rand(seed, i, 1), r2 = nnm_rand(seed, i, 2);
+                a = (uint32_t)(nnm_rand(seed, i, 0) % n);
+                for (uint32_t tr = 3; tr < 19 && label[a] == NNM_RAW; ++tr) a = (uint32_t)(nnm_rand(seed, i, tr) % n);
+                b = ids[(uint64_t)a * K + (uint32_t)(r1 % K)];
+                if (i & 1u) {
+                    const uint32_t cc = ids[(uint64_t)b * K + (uint32_t)(r2 % K)];
+                    if (cc != a) b = cc;
+                }
+                t = alpha * alpha * nnm_l2_rows(X + (uint64_t)a * ld, X + (uint64_t)b * ld, ld);
+                sv = alpha * alpha * g_dist[(uint64_t)a * K + K - 1];
+                t_io[i] = t;
+                s_io[i] = sv;
+            } else {
+                t = t_io[i];
+                sv = s_io[i];
+            }
+        }
+        L.row_id[2 * tid] = a;
+        L.row_id[2 * tid + 1] = b;
+        ts[tid] = t;
+        ss[tid] = sv;
+    }
+    __syncthreads();
+    // ---- rows -> LDS (bf16 of alpha x, zero rows past the batch)
+    for (uint32_t e = tid; e < MS_ROWS * D; e += 256) {
+        const uint32_t r = e / D, k = e % D;
+        const uint32_t i = pair0 + r / 2;
+        float v = 0.f;
+        if (i < B) v = X ? alpha * X[(uint64_t)L.row_id[r] * ld + k] : xs[(uint64_t)(2 * pair0 + r) * D + k];
+        L.xb[r * PD + k] = ms_bf(v);
+    }
+    __syncthreads();
+    // ---- pre = x W1^T + b1, h = relu(pre): wave w, column tiles w, w + 4, ...
+    for (uint32_t ct = wv; ct < H / 16; ct += 4) {
+        const uint32_t h0 = ct * 16;
+        mlp_f32x4 acc[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) acc[rt] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k0 = 0; k0 < D; k0 += 16) {
+            const uint2 bw = *reinterpret_cast<const uint2*>(W1b + (uint64_t)(h0 + lr) * D + k0 + 4 * lg);
+            const mlp_bf16x4 bo = __builtin_bit_cast(mlp_bf16x4, bw);
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt) {
+                const uint2 aw = *reinterpret_cast<const uint2*>(&L.xb[(rt * 16 + lr) * PD + k0 + 4 * lg]);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mlp_bf16x4, aw), bo, acc[rt],
+                                                                    0, 0, 0);
+            }
+        }
+        const float bb = b1[h0 + lr];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                L.hb[(rt * 16 + 4 * lg + q) * PH + h0 + lr] = ms_bf(fmaxf(acc[rt][q] + bb, 0.f));
+    }
+    __syncthreads();
+    // ---- z = h W2^T + b2: wave w, row tile w, every column tile
+    for (uint32_t ot = 0; ot < O / 16; ++ot) {
+        const uint32_t o0 = ot * 16;
+        mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k0 = 0; k0 < H; k0 += 16) {
+            const uint2 aw = *reinterpret_cast<const uint2*>(&L.hb[(wv * 16 + lr) * PH + k0 + 4 * lg]);
+            const uint2 bw = *reinterpret_cast<const uint2*>(W2b + (uint64_t)(o0 + lr) * H + k0 + 4 * lg);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mlp_bf16x4, aw),
+                                                            __builtin_bit_cast(mlp_bf16x4, bw), acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) L.z[(wv * 16 + 4 * lg + q) * PO + o0 + lr] = acc[q] + b2[o0 + lr];
+    }
+    __syncthreads();
+    // ---- pair loss and dz (k_pair_loss' arithmetic)
+    if (tid < MS_ROWS / 2) {
+        const uint32_t i = pair0 + tid;
+        const float* za = &L.z[(2 * tid) * PO];
+        const float* zb = za + PO;
+        float* da = &L.dz[(2 * tid) * PO];
+        float* db = da + PO;
+        if (i < B) {
+            float pp = 0.f;
+            for (uint32_t j = 0; j < O; ++j) {
+                const float e = za[j] - zb[j];
+                pp = __builtin_fmaf(e, e, pp);
+            }
+            const float den = ts[tid] + ss[tid];
+            const float r = (pp - ts[tid]) / den;
+            loss[i] = r * r;
+            const float gp = 2.f * r / den / (float)B;
+            for (uint32_t j = 0; j < O; ++j) {
+                const float e = 2.f * gp * (za[j] - zb[j]);
+                da[j] = e;
+                db[j] = -e;
+            }
+        } else {
+            for (uint32_t j = 0; j < O; ++j) { da[j] = 0.f; db[j] = 0.f; }
+        }
+    }
+    __syncthreads();
+    float* pb = part + (uint64_t)blockIdx.x * nparam;
+    float* pW1 = pb;
+    float* pb1 = pW1 + (uint64_t)H * D;
+    float* pW2 = pb1 + H;
+    float* pb2 = pW2 + (uint64_t)O * H;
+    // colsum dz -> gb2 part (rows in order)
+    if (tid < O) {
+        float acc = 0.f;
+        for (uint32_t r = 0; r < MS_ROWS; ++r) acc += L.dz[r * PO + tid];
+        pb2[tid] = acc;
+    }
+    // ---- dh = dz W2, masked by relu' (h > 0); colsum dh -> gb1 part.  Wave w
+    // holds whole columns (tiles w, w + 4, ...), so its column sums are
+    // complete: over its four row tiles, then over the lane groups.
+    for (uint32_t ct = wv; ct < H / 16; ct += 4) {
+        const uint32_t h0 = ct * 16;
+        mlp_f32x4 acc[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) acc[rt] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k0 = 0; k0 < O; k0 += 16) {
+            mlp_bf16x4 bo;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bo[q] = (short)W2b[(uint64_t)(k0 + 4 * lg + q) * H + h0 + lr];
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt) {
+                mlp_bf16x4 ao;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) ao[q] = mlp_to_bf16(L.dz[(rt * 16 + lr) * PO + k0 + 4 * lg + q]);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ao, bo, acc[rt], 0, 0, 0);
+            }
+        }
+        float cs = 0.f;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t r = rt * 16 + 4 * lg + q;
+                const float dh = (int16_t)L.hb[r * PH + h0 + lr] > 0 ? acc[rt][q] : 0.f;  // h > 0
+                cs += dh;
+                L.db[r * PH + h0 + lr] = ms_bf(dh);
+            }
+        cs += __shfl_xor(cs, 16, NNM_WAVE);
+        cs += __shfl_xor(cs, 32, NNM_WAVE);
+        if (lg == 0) pb1[h0 + lr] = cs;
+    }
+    __syncthreads();
+    // ---- gW2 part = dz^T h (O x H, k over the 64 rows)
+    for (uint32_t tt = wv; tt < (O / 16) * (H / 16); tt += 4) {
+        const uint32_t o0 = (tt / (H / 16)) * 16, h0 = (tt % (H / 16)) * 16;
+        mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k0 = 0; k0 < MS_ROWS; k0 += 16) {
+            mlp_bf16x4 ao, bo;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t r = k0 + 4 * lg + q;
+                ao[q] = mlp_to_bf16(L.dz[r * PO + o0 + lr]);
+                bo[q] = (short)L.hb[r * PH + h0 + lr];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ao, bo, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pW2[(uint64_t)(o0 + 4 * lg + q) * H + h0 + lr] = acc[q];
+    }
+    // ---- gW1 part = dh^T x (H x D, k over the 64 rows)
+    for (uint32_t tt = wv; tt < (H / 16) * (D / 16); tt += 4) {
+        const uint32_t h0 = (tt / (D / 16)) * 16, d0 = (tt % (D / 16)) * 16;
+        mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k0 = 0; k0 < MS_ROWS; k0 += 16) {
+            mlp_bf16x4 ao, bo;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t r = k0 + 4 * lg + q;
+                ao[q] = (short)L.db[r * PH + h0 + lr];
+                bo[q] = (short)L.xb[r * PD + d0 + lr];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ao, bo, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pW1[(uint64_t)(h0 + 4 * lg + q) * D + d0 + lr] = acc[q];
+    }
+}
+
+// Sum of the blocks' gradient parts (block order) per parameter; adam = 0:
+// store it in g; otherwise an Adam step on w, m, v and the bf16 copies of
+// W1 and W2 (Wb: W1 then W2) for the next step.
+__global__ void k_mlp_reduce(uint64_t nparam, uint32_t nblk, const float* __restrict__ part, float* __restrict__ g,
+                             int adam, float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, float lr,
+                             float b1, float b2, float c1, float c2, uint64_t w1n, uint64_t b1n, uint64_t w2n,
+                             uint16_t* __restrict__ Wb) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nparam) return;
+    float gi = 0.f;
+    for (uint32_t b = 0; b < nblk; ++b) gi += part[(uint64_t)b * nparam + i];
+    if (!adam) {
+        g[i] = gi;
+        return;
+    }
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    w[i] = wi;
+    // bf16 copies: W1 at Wb[0, w1n), W2 after it; the biases have none
+    if (i < w1n) Wb[i] = ms_bf(wi);
+    else if (i >= w1n + b1n && i < w1n + b1n + w2n) Wb[w1n + (i - w1n - b1n)] = ms_bf(wi);
+}
+
 // Projection of every row: 16 rows per wave, hidden layer in LDS (bf16),
 // output rows in bf16 plus their squared norms (of the bf16 values).
 #define KMAX 256  // longest row (ld) the projection takes; nnm_mlp_init checks it
@@ -311,6 +572,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // ---------------------------------------------------------------- host side
 
+static uint32_t grid_for_mlp(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + 255) / 256); }
+
 template <typename T>
 static int mlp_alloc(nnm_ctx* c, T** p, size_t n) {
     return nnm_alloc(c, reinterpret_cast<void**>(p), n * sizeof(T));
@@ -320,7 +583,7 @@ void nnm_mlp_free(nnm_ctx* c) {
     nnm_mlp_state* M = mlp_of(c);
     if (!M) return;
     void* ps[] = {M->W1, M->m, M->v, M->g, M->xs, M->pre, M->h, M->z, M->dz, M->dh, M->t, M->s, M->loss,
-                  M->pairs, M->Wb, M->P, M->pnorm, M->P8, M->pnorm8, M->amax};
+                  M->pairs, M->Wb, M->P, M->pnorm, M->P8, M->pnorm8, M->amax, M->part};
     for (void* p : ps) nnm_free(c, p);
     delete M;
     c->mlp = nullptr;
@@ -376,6 +639,57 @@ static int forward_backward(nnm_ctx* c, uint32_t B, bool bf16) {
     hipLaunchKernelGGL(k_relu_grad, dim3((R * H + 255) / 256), dim3(256), 0, c->stream, M->pre, M->dh, (uint64_t)R * H);
     NNM_TRY_MLP(gemm(c, bf16, H, D, R, M->dh, 1, H, M->xs, D, 1, nullptr, gW1, D));
     hipLaunchKernelGGL(k_colsum, dim3((H + 63) / 64), dim3(64, 16), 0, c->stream, M->dh, R, H, gb1);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
+// The fused step takes rows of at most 256 values, 256 hidden units and 64
+// outputs (the benchmark's 128 -> 256 -> 32 network).
+static bool fused_ok(const nnm_mlp_state* M) {
+    return M->din <= MS_DMAX && M->hid <= MS_HMAX && M->dout <= MS_OMAX;
+}
+
+// bf16 copies of W1 and W2 (Wb: W1 then W2) and room for the gradient parts
+// of a batch of B pairs.
+static int fused_prepare(nnm_ctx* c, uint32_t B) {
+    nnm_mlp_state* M = mlp_of(c);
+    const uint64_t w1n = (uint64_t)M->hid * M->din, w2n = (uint64_t)M->dout * M->hid;
+    if (!M->Wb) NNM_TRY_MLP(mlp_alloc(c, &M->Wb, w1n + w2n));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(grid_for_mlp(w1n)), dim3(256), 0, c->stream, M->W1, w1n, M->Wb);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(grid_for_mlp(w2n)), dim3(256), 0, c->stream, M->W2, w2n, M->Wb + w1n);
+    NNM_LAUNCH_CHECK(c);
+    const uint32_t nblk = (B + MS_ROWS / 2 - 1) / (MS_ROWS / 2);
+    if ((uint64_t)nblk * M->nparam > M->part_cap) {
+        nnm_free(c, M->part);
+        M->part = nullptr;
+        M->part_cap = 0;
+        NNM_TRY_MLP(mlp_alloc(c, &M->part, (size_t)nblk * M->nparam));
+        M->part_cap = (uint64_t)nblk * M->nparam;
+    }
+    return NNM_OK;
+}
+
+// One fused step on B pairs: gathered from the lists (gather) or the rows in
+// M->xs with M->t / M->s; then the gradient into M->g (lr < 0) or an Adam step.
+static int fused_step(nnm_ctx* c, uint32_t B, bool gather, uint64_t seed, float lr) {
+    nnm_mlp_state* M = mlp_of(c);
+    const uint32_t nblk = (B + MS_ROWS / 2 - 1) / (MS_ROWS / 2);
+    hipLaunchKernelGGL(k_mlp_step, dim3(nblk), dim3(256), 0, c->stream, B, M->din, M->hid, M->dout, M->Wb, M->b1,
+                       M->Wb + (uint64_t)M->hid * M->din, M->b2, gather ? c->X : nullptr, c->ld, c->n_act,
+                       c->K, c->g_ids, c->g_dist, c->label, M->alpha, seed, M->xs, M->t, M->s, M->loss, M->part,
+                       M->nparam);
+    NNM_LAUNCH_CHECK(c);
+    const uint64_t w1n = (uint64_t)M->hid * M->din, w2n = (uint64_t)M->dout * M->hid;
+    const bool adam = lr >= 0.f;
+    float c1 = 1.f, c2 = 1.f;
+    const float b1 = 0.9f, b2 = 0.999f;
+    if (adam) {
+        ++M->step;
+        c1 = 1.f - std::pow(b1, (float)M->step);
+        c2 = 1.f - std::pow(b2, (float)M->step);
+    }
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(grid_for_mlp(M->nparam)), dim3(256), 0, c->stream, M->nparam, nblk, M->part,
+                       M->g, adam ? 1 : 0, M->W1, M->m, M->v, lr, b1, b2, c1, c2, w1n, (uint64_t)M->hid, w2n, M->Wb);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
@@ -482,7 +796,13 @@ extern "C" int nnm_mlp_loss_grad(nnm_ctx* c, const float* xa, const float* xb, c
     NNM_TRY_MLP(nnm_h2d(c, M->xs, xs.data(), xs.size() * 4));
     NNM_TRY_MLP(nnm_h2d(c, M->t, t, B * 4));
     NNM_TRY_MLP(nnm_h2d(c, M->s, s, B * 4));
-    NNM_TRY_MLP(forward_backward(c, B, bf16 != 0));
+    if (bf16 && fused_ok(M)) {
+        // the fused step that training runs, without its Adam update
+        NNM_TRY_MLP(fused_prepare(c, B));
+        NNM_TRY_MLP(fused_step(c, B, false, 0, -1.f));
+    } else {
+        NNM_TRY_MLP(forward_backward(c, B, bf16 != 0));
+    }
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     std::vector<float> lv(B);
     NNM_TRY_MLP(nnm_d2h(c, lv.data(), M->loss, B * 4));
@@ -547,19 +867,26 @@ extern "C" int nnm_mlp_train(nnm_ctx* c, uint32_t steps, uint32_t batch, float l
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
     float lf = 0.f, ll = 0.f;
     std::vector<float> lv(batch);
+    const bool fused = fused_ok(M);
+    if (fused) NNM_TRY_MLP(fused_prepare(c, batch));
     for (uint32_t it = 0; it < steps; ++it) {
         if (2 * batch > M->cap_rows) return nnm_fail(c, NNM_ERR_STATE, "mlp batch larger than its buffers");
-        hipLaunchKernelGGL(k_mlp_pairs, dim3((batch + 255) / 256), dim3(256), 0, c->stream, batch, c->n_act, c->K,
-                           c->ld, c->X, c->g_ids, c->g_dist, c->label, M->alpha, nnm_rand(seed, it, 77), M->xs,
-                           M->t, M->s);
-        NNM_LAUNCH_CHECK(c);
-        NNM_TRY_MLP(forward_backward(c, batch, true));
-        ++M->step;
-        const float b1 = 0.9f, b2 = 0.999f;
-        const float c1 = 1.f - std::pow(b1, (float)M->step), c2 = 1.f - std::pow(b2, (float)M->step);
-        hipLaunchKernelGGL(k_adam, dim3((M->nparam + 255) / 256), dim3(256), 0, c->stream, M->W1, M->g, M->m, M->v,
-                           M->nparam, lr, b1, b2, c1, c2);
-        NNM_LAUNCH_CHECK(c);
+        if (fused) {
+            // pairs, forward, backward and the Adam step in two launches
+            NNM_TRY_MLP(fused_step(c, batch, true, nnm_rand(seed, it, 77), lr));
+        } else {
+            hipLaunchKernelGGL(k_mlp_pairs, dim3((batch + 255) / 256), dim3(256), 0, c->stream, batch, c->n_act,
+                               c->K, c->ld, c->X, c->g_ids, c->g_dist, c->label, M->alpha, nnm_rand(seed, it, 77),
+                               M->xs, M->t, M->s);
+            NNM_LAUNCH_CHECK(c);
+            NNM_TRY_MLP(forward_backward(c, batch, true));
+            ++M->step;
+            const float b1 = 0.9f, b2 = 0.999f;
+            const float c1 = 1.f - std::pow(b1, (float)M->step), c2 = 1.f - std::pow(b2, (float)M->step);
+            hipLaunchKernelGGL(k_adam, dim3((M->nparam + 255) / 256), dim3(256), 0, c->stream, M->W1, M->g, M->m,
+                               M->v, M->nparam, lr, b1, b2, c1, c2);
+            NNM_LAUNCH_CHECK(c);
+        }
         if (it == 0 || it + 1 == steps) {
             NNM_TRY_MLP(nnm_d2h(c, lv.data(), M->loss, batch * 4));
             double acc = 0;
