@@ -686,6 +686,42 @@ __device__ inline void offer_emit(const OfferSink& s, OfferWriter& w, bool p, ui
     w.used += k;
 }
 
+// Number of set bits of m below this lane (v_mbcnt).
+__device__ inline uint32_t nnm_mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Reserve cnt (wave-uniform, at most NNM_OCH) consecutive records of the
+// wave's log chunk for a batch of offers; *first is the batch's first record.
+// A chunk that cannot hold the whole batch is closed at its current count
+// (the passes read chunk_used records of every chunk, so its tail is never
+// read) and a new chunk is taken.  Returns false, with w.full set, when the
+// log is out of chunks.
+__device__ inline bool offer_reserve(const OfferSink& s, OfferWriter& w, uint32_t cnt, uint64_t* first) {
+    if (w.full) return false;
+    // the writer is wave-uniform: keep it in scalar registers
+    w.used = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.used);
+    w.chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.chunk);
+    if (w.used + cnt > NNM_OCH) {
+        const uint32_t lane = __lane_id();
+        if (w.chunk != 0xFFFFFFFFu && lane == 0) s.chunk_used[w.chunk] = w.used;
+        uint32_t ch = 0;
+        if (lane == 0) ch = (uint32_t)atomicAdd((unsigned long long*)&s.counters[CNT_OCHUNKS], 1ull);
+        ch = (uint32_t)__builtin_amdgcn_readfirstlane((int)ch);
+        if (ch >= s.nchunks) {
+            w.full = true;
+            w.chunk = 0xFFFFFFFFu;
+            w.used = NNM_OCH;
+            return false;
+        }
+        w.chunk = ch;
+        w.used = 0;
+    }
+    *first = (uint64_t)w.chunk * NNM_OCH + w.used;
+    w.used += cnt;
+    return true;
+}
+
 // Close the wave's chunk (its record count becomes visible to the passes).
 __device__ inline void offer_close(const OfferSink& s, OfferWriter& w) {
     if (w.chunk != 0xFFFFFFFFu && __lane_id() == 0) s.chunk_used[w.chunk] = w.used;

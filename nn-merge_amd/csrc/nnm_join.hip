@@ -512,18 +512,6 @@ typedef float nnm_f32x4 __attribute__((ext_vector_type(4)));
 #define KEY_BOUND 1    // offer the lower bound d - eps; exact re-rank later
 #define KEY_LEARN 2    // learned projection: estimate kscale * d_proj, margin tau
 
-// Offer key of a pair from the squared norms and the Gram entry.
-__device__ inline float offer_key(float an, float bn, float g, int mode, float kscale) {
-    float dd = fmaxf((an + bn) - 2.f * g, 0.f);
-    if (mode == KEY_BOUND) {
-        const float s = 0.00390625f * (sqrtf(an) + sqrtf(bn)) * 1.01f;
-        const float eps = s * (2.f * sqrtf(dd) + s) + 1e-5f * (an + bn);
-        dd = fmaxf(dd - eps, 0.f);
-    } else if (mode == KEY_LEARN) {
-        dd *= kscale;
-    }
-    return dd;
-}
 #define BJS 64         // bf16 window rows (one meta entry per lane)
 
 // One MFMA step of a 16x16 Gram block from one 8-byte unit per lane of each
@@ -554,24 +542,99 @@ struct Bf16Lds {
     uint32_t id[BJS];
 };
 
-// U: 8-byte units per row (bf16 rows of 4U elements; F8: e4m3 rows of 8U)
-template <int U, bool F8 = false>
-__global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb,
-                                                  const float* __restrict__ norms, int mode, float kscale, float tau) {
+// Offer key of a pair from the squared norms and the Gram entry.
+template <int MODE>
+__device__ inline float offer_key(float an, float bn, float g, float kscale) {
+    float dd = fmaxf((an + bn) - 2.f * g, 0.f);
+    if constexpr (MODE == KEY_BOUND) {
+        const float s = 0.00390625f * (sqrtf(an) + sqrtf(bn)) * 1.01f;
+        const float eps = s * (2.f * sqrtf(dd) + s) + 1e-5f * (an + bn);
+        dd = fmaxf(dd - eps, 0.f);
+    } else if constexpr (MODE == KEY_LEARN) {
+        dd *= kscale;
+    }
+    return dd;
+}
+
+// Rows of one 16-row block of a Gram block: lane (lr, lg) takes rows
+// 4 lg + q, q < 4 (its entries G[4 lg + q][lr] of the MFMA result).
+struct BlockRows {
+    uint32_t id[4], lab[4];
+    float thr[4], nrm[4];
+    bool in[4];  // the row is one of the vertex's new entries
+};
+
+// Pairs of one 16x16 Gram block (column cj of lane (lr, lg), rows 4 lg + q):
+// the pair rule, the distance count, and the offers below each end's
+// threshold, appended to the wave's log chunk as one batch (one reservation
+// per block, positions by ballot ranks).  diag: the block sits on the
+// diagonal, where only the pairs above it (column > row) are joined.
+// nd / no: wave-uniform distance and offer counts.
+template <int MODE>
+__device__ inline void block_offers(const JoinArgs& a, OfferWriter& w, const nnm_f32x4& acc, float kscale, float tau,
+                                    const BlockRows& r, uint32_t rbase, bool colok, uint32_t cj, bool diag,
+                                    uint32_t bid, uint32_t blab, float bthr, float bn, uint64_t& nd, uint64_t& no) {
+    const bool merge = a.rule == NNM_RULE_MERGE;
+    const float tb = tau * bthr;
+    bool pb[4], pa[4];
+    float dd[4];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        // bitwise: every term is cheap, branches would cost more
+        bool ok = colok & r.in[q] & (!diag | (cj > rbase + (uint32_t)q)) & (r.id[q] != bid);
+        ok = ok & !(merge & (r.lab[q] == blab) & (blab != NNM_RAW));
+        dd[q] = offer_key<MODE>(r.nrm[q], bn, acc[q], kscale);
+        nd += (uint32_t)__popcll(__ballot(ok));
+        pb[q] = ok & (dd[q] < tb);
+        pa[q] = ok & (dd[q] < tau * r.thr[q]);
+        cnt += (uint32_t)__popcll(__ballot(pb[q])) + (uint32_t)__popcll(__ballot(pa[q]));
+    }
+    if (!cnt) return;
+    uint64_t first;
+    if (!offer_reserve(a.sink, w, cnt, &first)) return;
+    no += cnt;
+    uint64_t* kp = a.sink.key + first;
+    uint32_t* tp = a.sink.tgt + first;
+    uint32_t off = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t mb = __ballot(pb[q]);
+        if (pb[q]) {
+            const uint32_t p = off + nnm_mbcnt(mb);
+            kp[p] = ((uint64_t)__float_as_uint(dd[q]) << 32) | r.id[q];
+            tp[p] = bid;
+        }
+        off += (uint32_t)__popcll(mb);
+        const uint64_t ma = __ballot(pa[q]);
+        if (pa[q]) {
+            const uint32_t p = off + nnm_mbcnt(ma);
+            kp[p] = ((uint64_t)__float_as_uint(dd[q]) << 32) | bid;
+            tp[p] = r.id[q];
+        }
+        off += (uint32_t)__popcll(ma);
+    }
+}
+
+// U: 8-byte units per row (bf16 rows of 4U elements; F8: e4m3 rows of 8U);
+// MODE: the offer key (KEY_EXACT, KEY_BOUND, KEY_LEARN).
+template <int U, bool F8, int MODE>
+__global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb, float kscale, float tau) {
     static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;  // dwords per row
     __shared__ __align__(16) Bf16Lds<U> L;
     const uint32_t js = min(a.js, (uint32_t)BJS);
     const uint32_t lane = threadIdx.x;
-    uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
+    uint64_t my_dists = 0, my_entries = 0, my_offers = 0;  // wave-uniform
     const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
     OfferWriter w;
 
     for (;;) {
         uint64_t base = 0;
         if (lane == 0) base = atomicAdd((unsigned long long*)&a.counters[CNT_WORK], (unsigned long long)a.chunk);
-        base = __shfl(base, 0, NNM_WAVE);
+        base = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+               (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
         if (base >= a.n) break;
         const uint32_t cnt = (uint32_t)min((uint64_t)a.chunk, a.n - base);
         uint32_t vtx = 0, cn = 0, co = 0;
@@ -587,10 +650,10 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
 #pragma unroll
             for (int b = 0; b < BFB; ++b) { off[b] = 0; nnv[b] = 0; mv[b] = 0; vid[b] = 0; }
             while (i < cnt && nbv < BFB) {
-                const uint32_t nn = (uint32_t)__shfl((int)cn, (int)i, NNM_WAVE);
-                const uint32_t m = nn + (uint32_t)__shfl((int)co, (int)i, NNM_WAVE);
+                const uint32_t nn = wave_read32(cn, i);
+                const uint32_t m = nn + wave_read32(co, i);
                 if (m > js || nn == 0) {
-                    if (m <= js && lane == 0) my_entries += m;
+                    if (m <= js) my_entries += m;
                     ++i;
                     continue;
                 }
@@ -608,7 +671,7 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
             uint32_t bvtx = 0;
 #pragma unroll
             for (int q = 0; q < BFB; ++q) {
-                const uint32_t t = (uint32_t)__shfl((int)vtx, (int)vid[q], NNM_WAVE);
+                const uint32_t t = wave_read32(vtx, vid[q]);
                 if (lane == (uint32_t)q) bvtx = t;
             }
             if (w.full) {  // the log is out of chunks: join the batch in a redo round
@@ -673,11 +736,23 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                 for (int q = 0; q < BFB; ++q)
                     if ((uint32_t)q == b) { ob = off[q]; nb_ = nnv[q]; mb = mv[q]; }
                 const uint32_t RB = (nb_ + 15) / 16, CB = (mb + 15) / 16;
-                for (uint32_t rb = 0; rb < RB; ++rb) {
+                for (uint32_t rb = 0; rb < RB && !w.full; ++rb) {
+                    // the lane's four rows of the row block (rows past the
+                    // window read its last row and are masked by in[])
+                    BlockRows rw;
+                    const uint32_t rbase = rb * 16 + 4 * lg;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t ia = min(ob + rbase + q, (uint32_t)BJS - 1);
+                        const uint4 ar = L.rec[ia];
+                        rw.id[q] = L.id[ia];
+                        rw.lab[q] = ar.x;
+                        rw.thr[q] = __uint_as_float(ar.y);
+                        rw.nrm[q] = __uint_as_float(ar.z);
+                        rw.in[q] = rbase + q < nb_;
+                    }
+                    const uint32_t ra = min(ob + rb * 16 + lr, (uint32_t)BJS - 1);
                     for (uint32_t cb = rb; cb < CB; ++cb) {
-                        // rows past the window read its last row: their
-                        // results are masked below (ri >= nb_ or cj >= mb)
-                        const uint32_t ra = min(ob + rb * 16 + lr, (uint32_t)BJS - 1);
                         const uint32_t rc = min(ob + cb * 16 + lr, (uint32_t)BJS - 1);
                         nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -687,35 +762,12 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
                             const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
                             acc = gram_step<F8>(av, bv, acc);
                         }
-                        // lane holds G[4 lg + q][lr] of the block
                         const uint32_t cj = cb * 16 + lr;
                         const bool colok = cj < mb;
-                        uint32_t bid = 0, blab = 0;
-                        float bthr = 0.f, bn = 0.f;
-                        if (colok) {
-                            const uint4 br = L.rec[ob + cj];
-                            bid = L.id[ob + cj];
-                            blab = br.x;
-                            bthr = __uint_as_float(br.y);
-                            bn = __uint_as_float(br.z);
-                        }
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            const uint32_t ri = rb * 16 + 4 * lg + q;
-                            const uint32_t ia = min(ob + ri, (uint32_t)BJS - 1);
-                            const uint4 ar = L.rec[ia];
-                            const uint32_t aid = L.id[ia], alab = ar.x;
-                            bool ok = colok && ri < nb_ && cj > ri && aid != bid;
-                            ok = ok && !(a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW);
-                            const float an = __uint_as_float(ar.z);
-                            const float dd = offer_key(an, bn, acc[q], mode, kscale);
-                            bd += ok ? 1u : 0u;
-                            const bool pb = ok && dd < tau * bthr;
-                            const bool pa = ok && dd < tau * __uint_as_float(ar.y);
-                            bo += (pb ? 1u : 0u) + (pa ? 1u : 0u);
-                            offer_emit(a.sink, w, pb, bid, aid, dd);
-                            offer_emit(a.sink, w, pa, aid, bid, dd);
-                        }
+                        const uint4 br = L.rec[rc];
+                        block_offers<MODE>(a, w, acc, kscale, tau, rw, rbase, colok, cj, cb == rb, L.id[rc], br.x,
+                                           __uint_as_float(br.y), __uint_as_float(br.z), bd, bo);
+                        if (w.full) break;
                     }
                 }
             }
@@ -724,12 +776,12 @@ __global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __
             } else {
                 my_dists += bd;
                 my_offers += bo;
-                if (lane == 0) my_entries += be;
+                my_entries += be;
             }
         }
     }
     offer_close(a.sink, w);
-    join_tally(a, my_dists, my_entries, my_offers, false);
+    join_tally(a, lane == 0 ? my_dists : 0, lane == 0 ? my_entries : 0, lane == 0 ? my_offers : 0, false);
 }
 
 // Long lists (BJS < m <= BLM) on BF16: one workgroup of 4 waves per vertex.
@@ -745,16 +797,16 @@ struct Bf16LongLds {
     uint32_t meta[4][BLM];
 };
 
-template <int U, bool F8 = false>
-__global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb,
-                                                        const float* __restrict__ norms, int mode, float kscale, float tau) {
+template <int U, bool F8, int MODE>
+__global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb, float kscale,
+                                                        float tau) {
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;
     __shared__ __align__(16) Bf16LongLds<U> L;
     __shared__ uint64_t vshare;
     __shared__ uint32_t wfull;  // a wave of the block ran out of log chunks
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    uint64_t my_dists = 0, my_entries = 0, my_offers = 0;
+    uint64_t my_dists = 0, my_entries = 0, my_offers = 0;  // wave-uniform
     const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
     OfferWriter w;
     uint64_t cur = 0, end = 0;
@@ -816,9 +868,21 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
         uint32_t blk = 0;
         uint64_t bd = 0, bo = 0;
         for (uint32_t rb = 0; rb < RB; ++rb) {
+            BlockRows rw;
+            const uint32_t rbase = rb * 16 + 4 * lg;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t ri = rbase + q;  // < BLM: nn <= m <= BLM rows, RB * 16 <= BLM
+                rw.id[q] = L.meta[0][ri];
+                rw.lab[q] = L.meta[1][ri];
+                rw.thr[q] = __uint_as_float(L.meta[2][ri]);
+                rw.nrm[q] = __uint_as_float(L.meta[3][ri]);
+                rw.in[q] = ri < nn;
+            }
+            const uint32_t ra = rb * 16 + lr;
             for (uint32_t cb = rb; cb < CB; ++cb, ++blk) {
                 if ((blk & 3u) != wave || w.full) continue;
-                const uint32_t ra = rb * 16 + lr, rc = cb * 16 + lr;
+                const uint32_t rc = cb * 16 + lr;
                 nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
@@ -829,29 +893,9 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
                 }
                 const uint32_t cj = cb * 16 + lr;
                 const bool colok = cj < m;
-                uint32_t bid = 0, blab = 0;
-                float bthr = 0.f, bn = 0.f;
-                if (colok) {
-                    bid = L.meta[0][cj];
-                    blab = L.meta[1][cj];
-                    bthr = __uint_as_float(L.meta[2][cj]);
-                    bn = __uint_as_float(L.meta[3][cj]);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t ri = rb * 16 + 4 * lg + q;
-                    const uint32_t aid = L.meta[0][ri], alab = L.meta[1][ri];
-                    bool ok = colok && ri < nn && cj > ri && aid != bid;
-                    ok = ok && !(a.rule == NNM_RULE_MERGE && alab == blab && alab != NNM_RAW);
-                    const float an = __uint_as_float(L.meta[3][ri]);
-                    const float dd = offer_key(an, bn, acc[q], mode, kscale);
-                    bd += ok ? 1u : 0u;
-                    const bool pb = ok && dd < tau * bthr;
-                    const bool pa = ok && dd < tau * __uint_as_float(L.meta[2][ri]);
-                    bo += (pb ? 1u : 0u) + (pa ? 1u : 0u);
-                    offer_emit(a.sink, w, pb, bid, aid, dd);
-                    offer_emit(a.sink, w, pa, aid, bid, dd);
-                }
+                const uint32_t cm = colok ? cj : 0u;
+                block_offers<MODE>(a, w, acc, kscale, tau, rw, rbase, colok, cj, cb == rb, L.meta[0][cm], L.meta[1][cm],
+                                   __uint_as_float(L.meta[2][cm]), __uint_as_float(L.meta[3][cm]), bd, bo);
             }
         }
         if (w.full && lane == 0) wfull = 1;
@@ -861,11 +905,11 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
         } else {
             my_dists += bd;
             my_offers += bo;
-            if (tid == 0) my_entries += m;
+            if (wave == 0) my_entries += m;
         }
     }
     offer_close(a.sink, w);
-    join_tally(a, my_dists, my_entries, my_offers, true);
+    join_tally(a, lane == 0 ? my_dists : 0, lane == 0 ? my_entries : 0, lane == 0 ? my_offers : 0, true);
 }
 
 // Lists longer than min_m entries (every list when min_m = 0): windows of 64.
@@ -969,13 +1013,11 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
     a.counters = c->counters;
     if (p->prec == NNM_PREC_BF16 || p->prec == NNM_PREC_LEARNED) {
         const uint16_t* rows = c->Xb;
-        const float* nrm = c->norms;
         uint32_t rl = c->ld;
         int mode = c->data_exact_bf16 ? KEY_EXACT : KEY_BOUND;
         float kscale = 1.f, tau = 1.f;
         if (p->prec == NNM_PREC_LEARNED) {
             rows = nnm_mlp_rows(c);
-            nrm = nnm_mlp_norms(c);
             rl = nnm_mlp_dout(c);
             mode = KEY_LEARN;
             kscale = nnm_mlp_scale(c);
@@ -989,14 +1031,14 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
                 const uint16_t* rows8 = reinterpret_cast<const uint16_t*>(nnm_mlp_rows8(c));
                 if (!rows8) return nnm_fail(c, NNM_ERR_STATE, "FP8 learned join without FP8 rows");
                 a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;
-                const float* nrm8 = nnm_mlp_norms8(c);
                 const float ks8 = nnm_mlp_scale8(c);
-                hipLaunchKernelGGL((k_join_bf16<4, true>), dim3(join_grid(c, k_join_bf16<4, true>)), dim3(64), 0,
-                                   c->stream, a, rows8, nrm8, mode, ks8, tau);
+                hipLaunchKernelGGL((k_join_bf16<4, true, KEY_LEARN>), dim3(join_grid(c, k_join_bf16<4, true, KEY_LEARN>)),
+                                   dim3(64), 0, c->stream, a, rows8, ks8, tau);
                 NNM_LAUNCH_CHECK(c);
                 if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));
-                hipLaunchKernelGGL((k_join_bf16_long<4, true>), dim3(join_grid256(c, k_join_bf16_long<4, true>)),
-                                   dim3(256), 0, c->stream, a, rows8, nrm8, mode, ks8, tau);
+                hipLaunchKernelGGL((k_join_bf16_long<4, true, KEY_LEARN>),
+                                   dim3(join_grid256(c, k_join_bf16_long<4, true, KEY_LEARN>)), dim3(256), 0, c->stream,
+                                   a, rows8, ks8, tau);
                 NNM_LAUNCH_CHECK(c);
                 // lists longer than BLM are refused before the learned phase
                 // (their FP32 join would offer exact distances against
@@ -1006,14 +1048,18 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
         a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;
-#define NNM_BF(U)                                                                                          \
-    case 4 * U:                                                                                            \
-        hipLaunchKernelGGL(k_join_bf16<U>, dim3(join_grid(c, k_join_bf16<U>)), dim3(64), 0, c->stream, a,  \
-                           rows, nrm, mode, kscale, tau);                                                  \
-        NNM_LAUNCH_CHECK(c);                                                                               \
-        if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));                                              \
-        hipLaunchKernelGGL(k_join_bf16_long<U>, dim3(join_grid256(c, k_join_bf16_long<U>)), dim3(256), 0,  \
-                           c->stream, a, rows, nrm, mode, kscale, tau);                                    \
+#define NNM_BFM(U, MODE)                                                                                            \
+    hipLaunchKernelGGL((k_join_bf16<U, false, MODE>), dim3(join_grid(c, k_join_bf16<U, false, MODE>)), dim3(64), 0, \
+                       c->stream, a, rows, kscale, tau);                                                           \
+    NNM_LAUNCH_CHECK(c);                                                                                            \
+    if (mid) NNM_HIP(c, hipEventRecord(mid, c->stream));                                                           \
+    hipLaunchKernelGGL((k_join_bf16_long<U, false, MODE>), dim3(join_grid256(c, k_join_bf16_long<U, false, MODE>)),  \
+                       dim3(256), 0, c->stream, a, rows, kscale, tau);
+#define NNM_BF(U)                                                    \
+    case 4 * U:                                                      \
+        if (mode == KEY_EXACT) { NNM_BFM(U, KEY_EXACT) }             \
+        else if (mode == KEY_BOUND) { NNM_BFM(U, KEY_BOUND) }        \
+        else { NNM_BFM(U, KEY_LEARN) }                               \
         break;
         switch (rl) {
             NNM_BF(4)
@@ -1024,6 +1070,7 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
             default: return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join row length");
         }
 #undef NNM_BF
+#undef NNM_BFM
         NNM_LAUNCH_CHECK(c);
         // lists beyond the long-list LDS window (rev_cap above 118) stay on
         // FP32 (plain BF16 joins only: a learned run with such lists is
