@@ -728,6 +728,266 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
     }
 }
 
+// ------------------------------------------------------------ paired owners
+//
+// Lists of at most 32 entries with exact keys (no re-rank, no FP64 tie
+// test, one key per pair): each wave merges two targets at once, one per
+// 32-lane half.  A half keeps its target's list in LDS (keys and payloads,
+// two buffers, empty keys past K), so the k-th key, the de-dup by id and
+// the merge read it by broadcast.  Per group of 32 candidates: filter by
+// the k-th key, drop ids the list holds, compact, sort (a network sized to
+// the survivors), drop repeats of one pair (equal keys, adjacent), then a
+// rank merge: every list entry and every candidate finds its rank in the
+// other sorted run by binary search, and the entries of rank below K form
+// the new list.  The result is UpdateKnnList's batch rule
+// (src/pvetex.h:40-85), as in own_merge: the K smallest distinct keys of the
+// list and the candidates.
+#define OWN2_WAVES 8u
+
+struct Own2Wave {
+    uint64_t lkey[2][NNM_WAVE];   // list keys: lane 32 h + j holds entry j of half h
+    uint32_t lpay[2][NNM_WAVE];   // payload: new flag | entered in this delivery << 1
+    uint64_t cand[NNM_WAVE];      // candidates of each half (compacted, then sorted)
+};
+
+// Number of keys below `key` in a[0..32) (sorted ascending).
+__device__ inline uint32_t own2_rank32(const uint64_t* a, uint64_t key) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t s = 16; s > 0; s >>= 1) pos += a[pos + s - 1] < key ? s : 0u;
+    return pos + (a[pos] < key ? 1u : 0u);  // 31 below it after the steps: a[31] decides
+}
+
+// Bitonic sort of lanes [0, 2^lg) of each half ascending (lg <= 5); the
+// lanes above hold the empty key.  The last stage of a 32-lane sort runs
+// with block size 64 so that both halves sort upward.
+__device__ inline uint64_t own2_sort(uint64_t key, uint32_t lg) {
+    if (lg >= 1) key = bitonic_pass64<2, 1>(key);
+    if (lg >= 2) key = bitonic_pass64<4, 2>(key);
+    if (lg >= 3) key = bitonic_pass64<8, 4>(key);
+    if (lg >= 4) key = bitonic_pass64<16, 8>(key);
+    if (lg >= 5) key = bitonic_pass64<64, 16>(key);
+    return key;
+}
+
+// Keys of the lanes with keep set moved to the first lanes of their half in
+// lane order (the empty key above them); *nh: the half's count.
+__device__ inline uint64_t own2_compact(uint64_t key, bool keep, uint64_t* buf, uint32_t* nh) {
+    const uint32_t lane = nnm_lane(), hl = lane & 31u, half = lane >> 5;
+    const uint64_t m = __ballot(keep);
+    const uint32_t nlo = (uint32_t)__popc((uint32_t)m), nhi = (uint32_t)__popc((uint32_t)(m >> 32));
+    *nh = half ? nhi : nlo;
+    const uint32_t r = nnm_mbcnt(m) - (half ? nlo : 0u);
+    own_wave_sync();  // earlier readers of buf are done
+    if (keep) buf[half * 32u + r] = key;
+    own_wave_sync();
+    return hl < *nh ? buf[lane] : NNM_EMPTY_KEY;
+}
+
+struct Own2Target {
+    uint64_t v = 0;      // target row
+    bool has = false;    // the half holds a target
+    uint32_t off = 0;    // its candidates: skey[off, off + len)
+    uint32_t len = 0;
+};
+
+// Merge the candidates of two targets (one per half) into their lists.
+__device__ inline void own2_merge(const OwnArgs& a, const Own2Target& tg, uint64_t lk0, uint32_t lp0, const uint64_t* skey,
+                                  Own2Wave& W, OwnTally& t) {
+    const uint32_t lane = nnm_lane(), hl = lane & 31u, half = lane >> 5;
+    const uint32_t K = a.K;
+    const uint64_t hmask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+    const uint64_t lmask = (K == 32 ? 0xFFFFFFFFull : ((1ull << K) - 1)) * 0x100000001ull;  // list lanes of both halves
+    own_wave_sync();  // the previous pair's readers are done
+    W.lkey[0][lane] = lk0;
+    W.lpay[0][lane] = lp0;
+    W.lkey[1][lane] = NNM_EMPTY_KEY;  // entries past K stay empty in both buffers
+    own_wave_sync();
+    const uint64_t f0 = __ballot(hl < K && (lp0 & 1u)), e0 = __ballot(hl < K && (lp0 & 2u));
+    const uint32_t glen = max((uint32_t)__builtin_amdgcn_readlane((int)tg.len, 0),
+                              (uint32_t)__builtin_amdgcn_readlane((int)tg.len, 32));
+    uint32_t buf = 0;
+    uint64_t entered = 0;  // lanes of the halves where a candidate entered
+    for (uint32_t g0 = 0; g0 < glen; g0 += 32) {
+        uint64_t ck = tg.has && g0 + hl < tg.len ? skey[tg.off + g0 + hl] : NNM_EMPTY_KEY;
+        const uint64_t* lkb = &W.lkey[buf][half * 32u];
+        bool valid = ck < lkb[K - 1];  // the empty key is above every key
+        if (!__ballot(valid)) continue;
+        // de-dup by id against the list (two entries per 16-byte read)
+        const uint32_t cid = nnm_key_id(ck);
+        bool dup = false;
+        for (uint32_t p = 0; p < K; p += 2) {
+            const uint4 q = *reinterpret_cast<const uint4*>(lkb + p);
+            dup = dup | (cid == q.x) | (cid == q.z);  // an empty entry (id ~0) matches no id
+        }
+        valid = valid & !dup;
+        if (!__ballot(valid)) continue;
+        uint32_t nh;
+        ck = own2_compact(ck, valid, W.cand, &nh);
+        const uint32_t nmax = max((uint32_t)__builtin_amdgcn_readlane((int)nh, 0),
+                                  (uint32_t)__builtin_amdgcn_readlane((int)nh, 32));
+        ck = own2_sort(ck, nmax > 1 ? 32u - (uint32_t)__builtin_clz(nmax - 1) : 0u);
+        {
+            // repeats of one pair are adjacent: keep the first
+            const uint64_t prev = shfl64(ck, hl ? lane - 1 : lane);
+            const bool keep = ck != NNM_EMPTY_KEY && (hl == 0 || prev != ck);
+            if (__ballot(keep) != __ballot(ck != NNM_EMPTY_KEY)) ck = own2_compact(ck, keep, W.cand, &nh);
+        }
+        // rank merge: candidate i (lane i < nh) and list entry j (lane j < K)
+        // go to rank i + (list entries below) and j + (candidates below)
+        own_wave_sync();
+        W.cand[lane] = ck;  // sorted, empty past nh
+        own_wave_sync();
+        const uint64_t lk = W.lkey[buf][lane];
+        const uint32_t lp = W.lpay[buf][lane];
+        const uint32_t below_c = own2_rank32(&W.cand[half * 32u], lk);
+        const uint32_t below_l = own2_rank32(lkb, ck);
+        const uint32_t nb = buf ^ 1u;
+        if (hl < K && hl + below_c < K) {
+            W.lkey[nb][half * 32u + hl + below_c] = lk;
+            W.lpay[nb][half * 32u + hl + below_c] = lp;
+        }
+        const bool in = hl < nh && hl + below_l < K;
+        if (in) {
+            W.lkey[nb][half * 32u + hl + below_l] = ck;
+            W.lpay[nb][half * 32u + hl + below_l] = 3u;  // enters new and entered now
+        }
+        entered |= __ballot(in);
+        buf = nb;
+        own_wave_sync();
+    }
+    // halves where a candidate entered write their lists back
+    const bool chg = (entered & hmask) != 0;
+    const uint64_t nk = W.lkey[buf][lane];
+    const uint32_t np = W.lpay[buf][lane];
+    if (tg.has && chg && hl < K) {
+        a.g_ids[tg.v * K + hl] = nnm_key_id(nk);
+        a.g_dist[tg.v * K + hl] = nnm_key_dist(nk);
+        a.g_flag[tg.v * K + hl] = (uint8_t)((np & 2u) ? 2u : (np & 1u));
+    }
+    // tallies: each changed half's change of entered entries and new flags
+    // (wrap-around sums, modulo 2^64)
+    const uint64_t f1 = __ballot(hl < K && (np & 1u)), e1 = __ballot(hl < K && (np & 2u));
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+        const uint64_t hm = h ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+        if (entered & hm) {
+            t.upd += (uint64_t)__popcll(e1 & hm & lmask) - (uint64_t)__popcll(e0 & hm & lmask);
+            t.flags += (uint64_t)__popcll(f1 & hm & lmask) - (uint64_t)__popcll(f0 & hm & lmask);
+        }
+    }
+}
+
+// A half's list entry (lane j < K of the half) as key and payload.
+__device__ inline void own2_load(const OwnArgs& a, const Own2Target& tg, uint64_t* key, uint32_t* pay) {
+    const uint32_t hl = nnm_lane() & 31u;
+    *key = NNM_EMPTY_KEY;
+    *pay = 0;
+    if (tg.has && hl < a.K) {
+        const uint64_t i = tg.v * a.K + hl;
+        const uint32_t f = a.g_flag[i];
+        *key = nnm_key(a.g_dist[i], a.g_ids[i]);
+        *pay = (f ? 1u : 0u) | (f == 2 ? 2u : 0u);
+    }
+}
+
+// Owner bins as in k_own (counting sort of each chunk by target in LDS);
+// the bin's targets with records are taken two at a time by the waves.
+__global__ __launch_bounds__(OWN2_WAVES * NNM_WAVE) void k_own2(OwnArgs a) {
+    constexpr uint32_t NT = OWN2_WAVES * NNM_WAVE;
+    constexpr uint32_t PER = OWN_CH / NT;
+    __shared__ uint64_t skey[OWN_CH];
+    __shared__ uint32_t hcnt[OWN_T];
+    __shared__ uint32_t hoff[OWN_T];
+    __shared__ __align__(16) Own2Wave W[OWN2_WAVES];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6, half = lane >> 5;
+    OwnTally t;
+    for (uint32_t bin = blockIdx.x; bin < a.nbins; bin += gridDim.x) {
+        const uint32_t s = bin ? a.bin_end[(uint64_t)(bin - 1) * a.bin_stride] : 0u;
+        const uint32_t e = a.bin_end[(uint64_t)bin * a.bin_stride];
+        const uint64_t t0 = (uint64_t)bin << NNM_OWN_BITS;
+        uint32_t c0 = s;
+        for (;;) {
+            const uint32_t ce = min(e, c0 + OWN_CH), m = ce - c0;
+            const bool last_chunk = ce >= e;
+            if (tid < OWN_T) hcnt[tid] = 0;
+            __syncthreads();
+            uint64_t key[PER];
+            uint32_t tl[PER], rk[PER];
+#pragma unroll
+            for (uint32_t k = 0; k < PER; ++k) {
+                const uint32_t j = tid + NT * k;
+                tl[k] = OWN_T;
+                rk[k] = 0;
+                key[k] = NNM_EMPTY_KEY;
+                if (j < m) {
+                    key[k] = a.key[(uint64_t)c0 + j];
+                    const uint64_t tg = a.tgt[(uint64_t)c0 + j];
+                    tl[k] = tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
+                    if (tl[k] < OWN_T) rk[k] = atomicAdd(&hcnt[tl[k]], 1u);
+                }
+            }
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t c = lane < OWN_T ? hcnt[lane] : 0u;
+                uint32_t x = c;
+#pragma unroll
+                for (int o = 1; o < NNM_WAVE; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
+                    if (lane >= (uint32_t)o) x += y;
+                }
+                if (lane < OWN_T) hoff[lane] = x - c;
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t k = 0; k < PER; ++k)
+                if (tl[k] < OWN_T) skey[hoff[tl[k]] + rk[k]] = key[k];
+            __syncthreads();
+            // targets with records: pairs (2p, 2p + 1) of the set bits, pair p
+            // to wave p % OWN2_WAVES; the next pair's lists load while the
+            // current pair merges
+            const uint64_t need = __ballot(lane < OWN_T && t0 + lane < a.n && hcnt[lane] != 0);
+            const uint32_t nneed = (uint32_t)__popcll(need);
+            auto target = [&](uint32_t p) {
+                Own2Target tg;
+                const uint32_t i = 2 * p + half;
+                if (i < nneed) {
+                    const uint32_t q = nth_set_bit(need, i);
+                    tg.v = t0 + q;
+                    tg.has = true;
+                    tg.off = hoff[q];
+                    tg.len = hcnt[q];
+                }
+                return tg;
+            };
+            uint32_t p = wave;
+            Own2Target cur = target(p);
+            uint64_t ck;
+            uint32_t cp;
+            own2_load(a, cur, &ck, &cp);
+            while (2 * p < nneed) {
+                const uint32_t pn = p + OWN2_WAVES;
+                const Own2Target nxt = target(pn);
+                uint64_t nk;
+                uint32_t np;
+                own2_load(a, nxt, &nk, &np);
+                own2_merge(a, cur, ck, cp, skey, W[wave], t);
+                cur = nxt;
+                ck = nk;
+                cp = np;
+                p = pn;
+            }
+            __syncthreads();  // the chunk's LDS is reused
+            if (last_chunk) break;
+            c0 = ce;
+        }
+    }
+    if (lane == 0) {
+        if (t.upd) atomicAdd((unsigned long long*)&a.counters[CNT_UPDATES], (unsigned long long)t.upd);
+        if (t.flags) atomicAdd((unsigned long long*)&a.counters[CNT_FLAGS], (unsigned long long)t.flags);
+    }
+}
+
 // ------------------------------------------------------------------ host
 
 namespace {
@@ -905,10 +1165,14 @@ int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
     void (*kern)(OwnArgs) = rerank ? (tie ? k_own<true, true, false> : k_own<true, false, false>)
                             : idd  ? (tie ? k_own<false, true, true> : k_own<false, false, true>)
                                    : (tie ? k_own<false, true, false> : k_own<false, false, false>);
+    // exact keys and lists of at most 32 entries: two targets per wave
+    const bool paired = !rerank && !tie && !idd && a.K <= 32;
+    if (paired) kern = k_own2;
+    const uint32_t threads = paired ? OWN2_WAVES * NNM_WAVE : 256u;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1) per_cu = 2;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(a.nbins, (uint32_t)c->num_cus * per_cu));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, c->stream, a);
     NNM_LAUNCH_CHECK(c);
     return NNM_OK;
 }
