@@ -803,110 +803,111 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;
     __shared__ __align__(16) Bf16LongLds<U> L;
-    __shared__ uint64_t vshare;
+    __shared__ uint32_t lq[256];    // long lists of the block's current work chunk
+    __shared__ uint32_t wcnt[4];
+    __shared__ uint64_t qbase;
     __shared__ uint32_t wfull;  // a wave of the block ran out of log chunks
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;  // wave-uniform
     const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
     OfferWriter w;
-    uint64_t cur = 0, end = 0;
     if (tid == 0) wfull = 0;
     for (;;) {
-        // block-wide queue over work items: thread 0 takes chunks of them
-        if (tid == 0) {
-            uint64_t i = cur, vx = ~0ull;
-            for (;;) {
-                if (i >= end) {
-                    i = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], (unsigned long long)(a.chunk * 16));
-                    end = min(a.n, i + a.chunk * 16);
-                    if (i >= a.n) break;
-                }
-                const uint32_t vi = work_vertex(a, i);
-                const uint32_t m = a.n_new[vi] + a.n_old[vi];
-                if (m > min(a.js, (uint32_t)BJS) && m <= BLM) {
-                    vx = vi;
-                    break;
-                }
-                ++i;
+        // block-wide queue: 256 work items at a time, one per thread; the
+        // long lists among them are collected in LDS and joined one by one
+        if (tid == 0) qbase = atomicAdd((unsigned long long*)&a.counters[CNT_WORK2], 256ull);
+        __syncthreads();
+        const uint64_t b0 = qbase;
+        if (b0 >= a.n) break;
+        bool lng = false;
+        uint32_t vi = 0;
+        if (b0 + tid < a.n) {
+            vi = work_vertex(a, b0 + tid);
+            const uint32_t mm = a.n_new[vi] + a.n_old[vi];
+            lng = mm > min(a.js, (uint32_t)BJS) && mm <= BLM;
+        }
+        const uint64_t bm = __ballot(lng);
+        if (lane == 0) wcnt[wave] = (uint32_t)__popcll(bm);
+        __syncthreads();
+        uint32_t wo = 0;
+        for (uint32_t q = 0; q < wave; ++q) wo += wcnt[q];
+        const uint32_t nl = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        if (lng) lq[wo + nnm_mbcnt(bm)] = vi;
+        __syncthreads();
+        for (uint32_t li = 0; li < nl; ++li) {
+            const uint64_t v = lq[li];
+            if (wfull) {  // the log is out of chunks: join the vertex in a redo round
+                if (wave == 0) offer_redo(a.sink, lane == 0, (uint32_t)v);
+                continue;
             }
-            vshare = vx;
-            cur = i + 1;
-        }
-        __syncthreads();
-        const uint64_t v = vshare;
-        const bool skip = wfull != 0;
-        __syncthreads();
-        if (v == ~0ull) break;
-        if (skip) {  // the log is out of chunks: join the vertex in a redo round
-            if (wave == 0) offer_redo(a.sink, lane == 0, (uint32_t)v);
-            continue;
-        }
-        const uint32_t nn = a.n_new[v], no = a.n_old[v], m = nn + no;
-        for (uint32_t j = tid; j < m; j += blockDim.x) {
-            const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
-            const uint4 mt = a.vmeta[id];
-            L.meta[0][j] = id;
-            L.meta[1][j] = mt.x;
-            L.meta[2][j] = mt.y;
-            L.meta[3][j] = mt.z;
-        }
-        __syncthreads();
-        // each wave copies every 4th row
-        for (uint32_t r = wave; r < m; r += 4) {
-            const uint32_t id = L.meta[0][r];
-            if (lane < DW) {
-                const uint32_t u = (lane >> 1) ^ (r & SWM);
-                const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
-                __builtin_amdgcn_global_load_lds((const void*)g,
-                                                 (__attribute__((address_space(3))) void*)(L.rows + r * U), 4, 0, 0);
+            const uint32_t nn = a.n_new[v], no = a.n_old[v], m = nn + no;
+            for (uint32_t j = tid; j < m; j += blockDim.x) {
+                const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
+                const uint4 mt = a.vmeta[id];
+                L.meta[0][j] = id;
+                L.meta[1][j] = mt.x;
+                L.meta[2][j] = mt.y;
+                L.meta[3][j] = mt.z;
             }
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        const uint32_t lr = lane & 15u, lg = lane >> 4;
-        const uint32_t RB = (nn + 15) / 16, CB = (m + 15) / 16;
-        uint32_t blk = 0;
-        uint64_t bd = 0, bo = 0;
-        for (uint32_t rb = 0; rb < RB; ++rb) {
-            BlockRows rw;
-            const uint32_t rbase = rb * 16 + 4 * lg;
+            __syncthreads();
+            // each wave copies every 4th row
+            for (uint32_t r = wave; r < m; r += 4) {
+                const uint32_t id = L.meta[0][r];
+                if (lane < DW) {
+                    const uint32_t u = (lane >> 1) ^ (r & SWM);
+                    const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
+                    __builtin_amdgcn_global_load_lds((const void*)g,
+                                                     (__attribute__((address_space(3))) void*)(L.rows + r * U), 4, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            const uint32_t lr = lane & 15u, lg = lane >> 4;
+            const uint32_t RB = (nn + 15) / 16, CB = (m + 15) / 16;
+            uint32_t blk = 0;
+            uint64_t bd = 0, bo = 0;
+            for (uint32_t rb = 0; rb < RB; ++rb) {
+                BlockRows rw;
+                const uint32_t rbase = rb * 16 + 4 * lg;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t ri = rbase + q;  // < BLM: nn <= m <= BLM rows, RB * 16 <= BLM
-                rw.id[q] = L.meta[0][ri];
-                rw.lab[q] = L.meta[1][ri];
-                rw.thr[q] = __uint_as_float(L.meta[2][ri]);
-                rw.nrm[q] = __uint_as_float(L.meta[3][ri]);
-                rw.in[q] = ri < nn;
-            }
-            const uint32_t ra = rb * 16 + lr;
-            for (uint32_t cb = rb; cb < CB; ++cb, ++blk) {
-                if ((blk & 3u) != wave || w.full) continue;
-                const uint32_t rc = cb * 16 + lr;
-                nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
-                    const uint32_t u = ks * 4 + lg;
-                    const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
-                    const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
-                    acc = gram_step<F8>(av, bv, acc);
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t ri = rbase + q;  // < BLM: nn <= m <= BLM rows, RB * 16 <= BLM
+                    rw.id[q] = L.meta[0][ri];
+                    rw.lab[q] = L.meta[1][ri];
+                    rw.thr[q] = __uint_as_float(L.meta[2][ri]);
+                    rw.nrm[q] = __uint_as_float(L.meta[3][ri]);
+                    rw.in[q] = ri < nn;
                 }
-                const uint32_t cj = cb * 16 + lr;
-                const bool colok = cj < m;
-                const uint32_t cm = colok ? cj : 0u;
-                block_offers<MODE>(a, w, acc, kscale, tau, rw, rbase, colok, cj, cb == rb, L.meta[0][cm], L.meta[1][cm],
-                                   __uint_as_float(L.meta[2][cm]), __uint_as_float(L.meta[3][cm]), bd, bo);
+                const uint32_t ra = rb * 16 + lr;
+                for (uint32_t cb = rb; cb < CB; ++cb, ++blk) {
+                    if ((blk & 3u) != wave || w.full) continue;
+                    const uint32_t rc = cb * 16 + lr;
+                    nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
+                        const uint32_t u = ks * 4 + lg;
+                        const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
+                        const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
+                        acc = gram_step<F8>(av, bv, acc);
+                    }
+                    const uint32_t cj = cb * 16 + lr;
+                    const bool colok = cj < m;
+                    const uint32_t cm = colok ? cj : 0u;
+                    block_offers<MODE>(a, w, acc, kscale, tau, rw, rbase, colok, cj, cb == rb, L.meta[0][cm], L.meta[1][cm],
+                                       __uint_as_float(L.meta[2][cm]), __uint_as_float(L.meta[3][cm]), bd, bo);
+                }
+            }
+            if (w.full && lane == 0) wfull = 1;
+            __syncthreads();  // LDS reused by the next vertex; every wave's offers issued
+            if (wfull) {
+                if (wave == 0) offer_redo(a.sink, lane == 0, (uint32_t)v);
+            } else {
+                my_dists += bd;
+                my_offers += bo;
+                if (wave == 0) my_entries += m;
             }
         }
-        if (w.full && lane == 0) wfull = 1;
-        __syncthreads();  // LDS reused by the next vertex; every wave's offers issued
-        if (wfull) {
-            if (wave == 0) offer_redo(a.sink, lane == 0, (uint32_t)v);
-        } else {
-            my_dists += bd;
-            my_offers += bo;
-            if (wave == 0) my_entries += m;
-        }
+        __syncthreads();  // lq, wcnt and qbase are rewritten for the next chunk
     }
     offer_close(a.sink, w);
     join_tally(a, lane == 0 ? my_dists : 0, lane == 0 ? my_entries : 0, lane == 0 ? my_offers : 0, true);
@@ -926,26 +927,35 @@ __global__ __launch_bounds__(64) void k_join_f32_windows(JoinArgs a, uint32_t mi
         base = __shfl(base, 0, NNM_WAVE);
         if (base >= a.n) break;
         const uint64_t end = min(a.n, base + JOIN_CHUNK * 16);
-        for (uint64_t i = base; i < end; ++i) {
-            const uint32_t v = work_vertex(a, i);
-            const uint32_t nn = a.n_new[v], no = a.n_old[v];
-            if (nn + no <= min_m) continue;
-            if (!nn) {
-                if (lane == 0) my_entries += no;
-                continue;
+        // the chunk's list sizes 64 at a time, one per lane; the lists this
+        // kernel takes are then joined in order by the whole wave
+        for (uint64_t i0 = base; i0 < end; i0 += NNM_WAVE) {
+            uint32_t lv = 0, lnn = 0, lno = 0;
+            if (i0 + lane < end) {
+                lv = work_vertex(a, i0 + lane);
+                lnn = a.n_new[lv];
+                lno = a.n_old[lv];
             }
-            if (w.full) {  // the log is out of chunks: join the vertex in a redo round
-                offer_redo(a.sink, lane == 0, v);
-                continue;
-            }
-            uint64_t bd = 0, bo = 0;
-            join_vertex_windows<M>(a, L, w, v, nn, no, lane, bd, bo);
-            if (w.full) {
-                offer_redo(a.sink, lane == 0, v);
-            } else {
-                my_dists += bd;
-                my_offers += bo;
-                if (lane == 0) my_entries += nn + no;
+            for (uint64_t take = __ballot(lnn + lno > min_m); take; take &= take - 1) {
+                const uint32_t src = (uint32_t)__builtin_ctzll(take);
+                const uint32_t v = wave_read32(lv, src), nn = wave_read32(lnn, src), no = wave_read32(lno, src);
+                if (!nn) {
+                    if (lane == 0) my_entries += no;
+                    continue;
+                }
+                if (w.full) {  // the log is out of chunks: join the vertex in a redo round
+                    offer_redo(a.sink, lane == 0, v);
+                    continue;
+                }
+                uint64_t bd = 0, bo = 0;
+                join_vertex_windows<M>(a, L, w, v, nn, no, lane, bd, bo);
+                if (w.full) {
+                    offer_redo(a.sink, lane == 0, v);
+                } else {
+                    my_dists += bd;
+                    my_offers += bo;
+                    if (lane == 0) my_entries += nn + no;
+                }
             }
         }
     }
