@@ -145,6 +145,7 @@ struct nnm_ctx {
     uint32_t* opart_blk = nullptr;    // first pass: per-block bin counts (bins x grid)
     uint32_t* opart_boff = nullptr;   // ... their exclusive scan
     uint32_t* opart_tmp = nullptr;    // scan scratch of the passes
+    uint32_t* opart_p1[2] = {nullptr, nullptr};  // second pass: counts per (owner bin, segment), their scan
     uint64_t* counters = nullptr;   // CNT_N
     float* kth = nullptr;           // n: k-th distance of every list at iteration start
     uint4* vmeta = nullptr;         // n: join record {label, kth x offer window, norm, 0}

@@ -32,17 +32,18 @@
 
 #include "nnm_internal.hpp"
 
-#define PT_THREADS 256u
+#define PT_THREADS 512u
 #define PT_PER 8u                          // records per thread in a tile
-#define PT_TILE (PT_THREADS * PT_PER)      // records per tile (half a log chunk)
+#define PT_TILE (PT_THREADS * PT_PER)      // records per tile (one log chunk)
 #define PT_BINS 1024u                      // LDS bins of a tile (relative to its smallest digit)
 #define PT_MAXBITS 10u                     // digit bits of one pass
 #define OWN_T (1u << NNM_OWN_BITS)         // targets per owner bin
 #define OWN_CH 4096u                       // records an owner sorts in LDS at a time
 #define NO_DIGIT 0xFFFFFFFFu
+#define P1_SEGS 8u                         // second pass: segments of each first-pass bin
 
-static_assert(2 * PT_TILE == NNM_OCH, "first-pass tiles are halves of log chunks");
-static_assert(PT_BINS == 4 * PT_THREADS, "four bins per thread in the LDS scan");
+static_assert(PT_TILE == NNM_OCH, "first-pass tiles are log chunks");
+static_assert(PT_BINS == 2 * PT_THREADS, "two bins per thread in the LDS scan");
 static_assert(OWN_T <= 64, "owner bins of at most 64 targets (one LDS counter per lane)");
 
 // ------------------------------------------------------------------ passes
@@ -50,9 +51,9 @@ static_assert(OWN_T <= 64, "owner bins of at most 64 targets (one LDS counter pe
 struct PassIn {
     const uint64_t* key;         // record keys
     const uint32_t* tgt;         // record targets
-    const uint32_t* chunk_used;  // first pass: records per log chunk (tiles 2c, 2c + 1 = chunk c)
+    const uint32_t* chunk_used;  // first pass: records per log chunk (tile c = chunk c)
     const uint64_t* counters;    // first pass: CNT_OCHUNKS
-    uint32_t nchunks;            // log capacity in chunks (2 tiles each, in any pass)
+    uint32_t nchunks;            // log capacity in chunks
     const uint32_t* total;       // later passes: record count of the input
     uint32_t nbins;              // bins of the pass (records of other digits are not moved)
 };
@@ -63,21 +64,17 @@ __device__ inline void tile_range(const PassIn& in, uint64_t t, uint64_t* b, uin
     *m = 0;
     if (in.chunk_used) {
         const uint64_t used = min(in.counters[CNT_OCHUNKS], (uint64_t)in.nchunks);
-        const uint64_t ch = t >> 1;
-        if (ch < used) {
-            const uint32_t u = in.chunk_used[ch], h = (uint32_t)(t & 1u) * PT_TILE;
-            *m = u > h ? min(u - h, PT_TILE) : 0u;
-        }
+        if (t < used) *m = min(in.chunk_used[t], PT_TILE);
     } else {
         const uint64_t tot = *in.total;
         if (*b < tot) *m = (uint32_t)min((uint64_t)PT_TILE, tot - *b);
     }
 }
 
-// Tiles of the pass input: two per reserved log chunk (first pass) or the
+// Tiles of the pass input: the reserved log chunks (first pass) or the
 // record count in tiles.
 __device__ inline uint64_t tile_count(const PassIn& in) {
-    if (in.chunk_used) return 2 * min(in.counters[CNT_OCHUNKS], (uint64_t)in.nchunks);
+    if (in.chunk_used) return min(in.counters[CNT_OCHUNKS], (uint64_t)in.nchunks);
     return ((uint64_t)*in.total + PT_TILE - 1) / PT_TILE;
 }
 
@@ -104,7 +101,36 @@ __device__ inline void tile_digits(const PassIn& in, uint64_t b, uint32_t m, uin
     }
 }
 
-// Bin counts of one pass: cnt[target >> shift] += records (reads the targets only).
+// Exclusive prefix of the PT_BINS LDS counts hist[] into hist[] (thread tid
+// owns bins 2 tid, 2 tid + 1); returns the total.  c2 gets the thread's counts.
+__device__ inline uint32_t block_prefix_bins(uint32_t* hist, uint32_t* wsum, uint32_t (&c2)[2]) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    c2[0] = hist[2 * tid];
+    c2[1] = hist[2 * tid + 1];
+    const uint32_t s2 = c2[0] + c2[1];
+    uint32_t x = s2;
+#pragma unroll
+    for (int o = 1; o < NNM_WAVE; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == NNM_WAVE - 1) wsum[wave] = x;
+    __syncthreads();
+    uint32_t run = x - s2, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < PT_THREADS / NNM_WAVE; ++w) {
+        run += w < wave ? wsum[w] : 0u;
+        tot += wsum[w];
+    }
+    hist[2 * tid] = run;
+    hist[2 * tid + 1] = run + c2[0];
+    __syncthreads();
+    return tot;
+}
+
+// Bin counts of one pass: cnt[target >> shift] += records (reads the targets
+// only).  Passes after the first when the owner-bin digits need more than
+// two passes.
 __global__ __launch_bounds__(PT_THREADS) void k_part_count(PassIn in, uint32_t shift, uint32_t* __restrict__ cnt) {
     __shared__ uint32_t hist[PT_BINS];
     __shared__ uint32_t dmin_s;
@@ -147,7 +173,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassIn in, uint32_t
     __shared__ uint32_t basev[PT_BINS];  // global position of each bin's run
     __shared__ uint32_t wsum[PT_THREADS / NNM_WAVE];
     __shared__ uint32_t dmin_s;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     const uint64_t nt = tile_count(in);
     for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
         uint64_t b;
@@ -189,34 +215,11 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassIn in, uint32_t
             }
         }
         __syncthreads();
-        // global runs of the nonempty bins, and the exclusive prefix of the
-        // counts (thread tid owns bins 4 tid .. 4 tid + 3)
-        uint32_t c4[4], s4 = 0;
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t i = 4 * tid + q;
-            c4[q] = hist[i];
-            if (c4[q]) basev[i] = atomicAdd(&cur[dmin + i], c4[q]);
-            s4 += c4[q];
-        }
-        uint32_t x = s4;
-#pragma unroll
-        for (int o = 1; o < NNM_WAVE; o <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
-            if (lane >= (uint32_t)o) x += y;
-        }
-        if (lane == NNM_WAVE - 1) wsum[wave] = x;
-        __syncthreads();
-        uint32_t wbase = 0;
-        for (uint32_t w = 0; w < wave; ++w) wbase += wsum[w];
-        uint32_t run = wbase + x - s4;
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            hist[4 * tid + q] = run;
-            run += c4[q];
-        }
-        const uint32_t nl = wsum[0] + wsum[1] + wsum[2] + wsum[3];  // records staged in LDS
-        __syncthreads();
+        // global runs of the nonempty bins
+        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS)
+            if (hist[i]) basev[i] = atomicAdd(&cur[dmin + i], hist[i]);
+        uint32_t c2[2];
+        const uint32_t nl = block_prefix_bins(hist, wsum, c2);  // records staged in LDS
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
             if (d[k] == NO_DIGIT) continue;
@@ -234,36 +237,6 @@ __global__ __launch_bounds__(PT_THREADS) void k_part_scatter(PassIn in, uint32_t
         }
         __syncthreads();
     }
-}
-
-// Exclusive prefix of the 1024 LDS counts hist[] into hist[] (thread tid owns
-// bins 4 tid .. 4 tid + 3); returns the total.  c4 gets the thread's counts.
-__device__ inline uint32_t block_prefix_bins(uint32_t* hist, uint32_t* wsum, uint32_t (&c4)[4]) {
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    uint32_t s4 = 0;
-#pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-        c4[q] = hist[4 * tid + q];
-        s4 += c4[q];
-    }
-    uint32_t x = s4;
-#pragma unroll
-    for (int o = 1; o < NNM_WAVE; o <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)x, (unsigned)o, NNM_WAVE);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    if (lane == NNM_WAVE - 1) wsum[wave] = x;
-    __syncthreads();
-    uint32_t run = x - s4;
-    for (uint32_t w = 0; w < wave; ++w) run += wsum[w];
-#pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-        hist[4 * tid + q] = run;
-        run += c4[q];
-    }
-    const uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
-    return tot;
 }
 
 // First pass (at most PT_BINS bins): each block counts the records of the
@@ -330,8 +303,8 @@ __global__ __launch_bounds__(PT_THREADS) void k_part0_scatter(PassIn in, uint32_
             if (d[k] != NO_DIGIT) rk[k] = atomicAdd(&hist[d[k]], 1u);
         }
         __syncthreads();
-        uint32_t c4[4];
-        const uint32_t nl = block_prefix_bins(hist, wsum, c4);
+        uint32_t c2[2];
+        const uint32_t nl = block_prefix_bins(hist, wsum, c2);
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
             if (d[k] == NO_DIGIT) continue;
@@ -348,9 +321,117 @@ __global__ __launch_bounds__(PT_THREADS) void k_part0_scatter(PassIn in, uint32_
             otgt[p] = tgv;
         }
         __syncthreads();
-#pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) curs[4 * tid + q] += c4[q];
+        curs[2 * tid] += c2[0];
+        curs[2 * tid + 1] += c2[1];
         __syncthreads();
+    }
+}
+
+// Second pass when each first-pass bin r holds nl = 2^b1 owner bins: the bin's
+// records (contiguous in the first pass's output, [boff[r * G], boff[(r + 1) *
+// G])) are split into P1_SEGS equal segments.  Counts per (bin, owner bin,
+// segment) go to cnt[(r * nl + l) * P1_SEGS + seg]; their exclusive scan in
+// that order is every segment's start in every owner bin, so the scatter
+// needs no atomics and owner bin ob ends where entry (ob + 1) * P1_SEGS
+// starts.
+__device__ inline void p1_segment(const uint32_t* boff, uint32_t G, uint32_t r, uint32_t seg, uint32_t* s,
+                                  uint32_t* e) {
+    const uint32_t bs = boff[(uint64_t)r * G], be = boff[(uint64_t)(r + 1) * G];
+    const uint32_t len = be - bs;
+    *s = bs + (uint32_t)((uint64_t)len * seg / P1_SEGS);
+    *e = bs + (uint32_t)((uint64_t)len * (seg + 1) / P1_SEGS);
+}
+
+__global__ __launch_bounds__(PT_THREADS) void k_part1_count(const uint32_t* __restrict__ tgt,
+                                                            const uint32_t* __restrict__ boff, uint32_t G,
+                                                            uint32_t nbins0, uint32_t b1, uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t hist[PT_BINS];
+    const uint32_t tid = threadIdx.x, nl = 1u << b1;
+    for (uint32_t u = blockIdx.x; u < nbins0 * P1_SEGS; u += gridDim.x) {
+        const uint32_t r = u / P1_SEGS, seg = u % P1_SEGS, ob0 = r << b1;
+        uint32_t s, e;
+        p1_segment(boff, G, r, seg, &s, &e);
+        for (uint32_t i = tid; i < nl; i += PT_THREADS) hist[i] = 0;
+        __syncthreads();
+        for (uint32_t i0 = s; i0 < e; i0 += PT_TILE) {
+            uint32_t d[PT_PER];
+#pragma unroll
+            for (uint32_t k = 0; k < PT_PER; ++k) {
+                const uint32_t i = i0 + tid + k * PT_THREADS;
+                d[k] = i < e ? (tgt[i] >> NNM_OWN_BITS) - ob0 : NO_DIGIT;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < PT_PER; ++k)
+                if (d[k] != NO_DIGIT) atomicAdd(&hist[d[k]], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < nl; i += PT_THREADS) cnt[((uint64_t)r * nl + i) * P1_SEGS + seg] = hist[i];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(PT_THREADS) void k_part1_scatter(const uint64_t* __restrict__ key,
+                                                              const uint32_t* __restrict__ tgt,
+                                                              const uint32_t* __restrict__ boff, uint32_t G,
+                                                              uint32_t nbins0, uint32_t b1,
+                                                              const uint32_t* __restrict__ off,
+                                                              uint64_t* __restrict__ okey, uint32_t* __restrict__ otgt) {
+    __shared__ uint64_t lkey[PT_TILE];
+    __shared__ uint32_t ltgt[PT_TILE];
+    __shared__ uint32_t hist[PT_BINS];   // per tile: counts, then their exclusive prefix
+    __shared__ uint32_t curs[PT_BINS];   // the segment's next output position in every owner bin
+    __shared__ uint32_t wsum[PT_THREADS / NNM_WAVE];
+    const uint32_t tid = threadIdx.x, nl = 1u << b1;
+    for (uint32_t u = blockIdx.x; u < nbins0 * P1_SEGS; u += gridDim.x) {
+        const uint32_t r = u / P1_SEGS, seg = u % P1_SEGS, ob0 = r << b1;
+        uint32_t s, e;
+        p1_segment(boff, G, r, seg, &s, &e);
+        for (uint32_t i = tid; i < nl; i += PT_THREADS) curs[i] = off[((uint64_t)r * nl + i) * P1_SEGS + seg];
+        for (uint32_t t0 = s; t0 < e; t0 += PT_TILE) {
+            const uint32_t m = min(PT_TILE, e - t0);
+            for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
+            __syncthreads();
+            uint32_t d[PT_PER], tg[PT_PER], rk[PT_PER];
+            uint64_t ky[PT_PER];
+#pragma unroll
+            for (uint32_t k = 0; k < PT_PER; ++k) {
+                const uint32_t j = tid + k * PT_THREADS;
+                tg[k] = 0;
+                ky[k] = 0;
+                d[k] = NO_DIGIT;
+                rk[k] = 0;
+                if (j < m) {
+                    tg[k] = tgt[t0 + j];
+                    ky[k] = key[t0 + j];
+                    d[k] = (tg[k] >> NNM_OWN_BITS) - ob0;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < PT_PER; ++k)
+                if (d[k] != NO_DIGIT) rk[k] = atomicAdd(&hist[d[k]], 1u);
+            __syncthreads();
+            uint32_t c2[2];
+            block_prefix_bins(hist, wsum, c2);
+#pragma unroll
+            for (uint32_t k = 0; k < PT_PER; ++k) {
+                if (d[k] == NO_DIGIT) continue;
+                const uint32_t lp = hist[d[k]] + rk[k];
+                lkey[lp] = ky[k];
+                ltgt[lp] = tg[k];
+            }
+            __syncthreads();
+            for (uint32_t lp = tid; lp < m; lp += PT_THREADS) {
+                const uint32_t tgv = ltgt[lp];
+                const uint32_t dd = (tgv >> NNM_OWN_BITS) - ob0;
+                const uint32_t p = curs[dd] + (lp - hist[dd]);
+                okey[p] = lkey[lp];
+                otgt[p] = tgv;
+            }
+            __syncthreads();
+            curs[2 * tid] += c2[0];
+            curs[2 * tid + 1] += c2[1];
+            __syncthreads();
+        }
     }
 }
 
@@ -1024,15 +1105,17 @@ int fill32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) { return nnm_fill_u3
 
 }  // namespace
 
-static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus * 4; }
+static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus * 2; }
 static uint64_t pass0_slots(nnm_ctx* c) { return (uint64_t)PT_BINS * pass_grid(c) + 1; }
+// second-pass table entries for n rows: (first-pass bins x 2^b1) owner bins x segments
+static uint64_t pass1_slots(uint64_t n) { return (((n >> NNM_OWN_BITS) + 2 + PT_BINS) * P1_SEGS) + 1; }
 
 void nnm_offer_free(nnm_ctx* c) {
     void** ps[] = {(void**)&c->okey[0], (void**)&c->okey[1], (void**)&c->otgt[0], (void**)&c->otgt[1],
                    (void**)&c->ochunk_used, (void**)&c->opart_cnt,
                    (void**)&c->opart_cur[0], (void**)&c->opart_cur[1], (void**)&c->opart_cur[2],
                    (void**)&c->work[0], (void**)&c->work[1], (void**)&c->opart_blk, (void**)&c->opart_boff,
-                   (void**)&c->opart_tmp};
+                   (void**)&c->opart_tmp, (void**)&c->opart_p1[0], (void**)&c->opart_p1[1]};
     for (void** p : ps) {
         nnm_free(c, *p);
         *p = nullptr;
@@ -1072,7 +1155,9 @@ int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
     // first pass: per-block counts and their scan; scan scratch for every pass
     NNM_TRY(nnm_alloc(c, (void**)&c->opart_blk, pass0_slots(c) * sizeof(uint32_t)));
     NNM_TRY(nnm_alloc(c, (void**)&c->opart_boff, (pass0_slots(c) + 1) * sizeof(uint32_t)));
-    const uint64_t scan_m = std::max<uint64_t>(pass0_slots(c), maxbins);
+    NNM_TRY(nnm_alloc(c, (void**)&c->opart_p1[0], pass1_slots(n) * sizeof(uint32_t)));
+    NNM_TRY(nnm_alloc(c, (void**)&c->opart_p1[1], (pass1_slots(n) + 1) * sizeof(uint32_t)));
+    const uint64_t scan_m = std::max<uint64_t>(std::max<uint64_t>(pass0_slots(c), maxbins), pass1_slots(n));
     NNM_TRY(nnm_alloc(c, (void**)&c->opart_tmp, ((scan_m + 2047) / 2048 + 2) * sizeof(uint32_t)));
     return NNM_OK;
 }
@@ -1092,7 +1177,20 @@ int nnm_offer_reset(nnm_ctx* c) { return nnm_fill_u64(c, c->counters + CNT_OCHUN
 
 
 int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
-    const PassPlan pl = plan_passes(c->n_act);
+    PassPlan pl = plan_passes(c->n_act);
+    // owner-bin digits of 11..20 bits: a first pass of the high half, then a
+    // second pass over each first-pass bin's 2^b1 owner bins in segments
+    // (k_part1_count, k_part1_scatter)
+    const uint32_t lbits = ceil_log2(std::max<uint64_t>(c->n_act, 2));
+    const uint32_t obits = lbits > NNM_OWN_BITS ? lbits - NNM_OWN_BITS : 0u;
+    const bool local = obits > PT_MAXBITS && obits <= 2 * PT_MAXBITS;
+    uint32_t b1 = 0;
+    if (local) {
+        b1 = (obits + 1) / 2;
+        pl.npass = 1;
+        pl.shift[0] = NNM_OWN_BITS + b1;
+        pl.bins[0] = (uint32_t)(((c->n_act - 1) >> pl.shift[0]) + 1);
+    }
     PassIn in;
     in.key = c->okey[0];
     in.tgt = c->otgt[0];
@@ -1134,6 +1232,26 @@ int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
         in.tgt = c->otgt[o];
         in.chunk_used = nullptr;
     }
+    uint32_t nob = pl.bins[pl.npass - 1];
+    if (local) {
+        // the first pass wrote okey[1] / otgt[1]; the owner bins go to okey[0] / otgt[0]
+        nob = (uint32_t)(((c->n_act - 1) >> NNM_OWN_BITS) + 1);
+        const uint32_t bins0 = pl.bins[0];
+        const uint64_t tab = ((uint64_t)bins0 << b1) * P1_SEGS;
+        const uint32_t units = bins0 * P1_SEGS;
+        const uint32_t grid = std::min<uint32_t>(units, (uint32_t)c->num_cus * 4);
+        hipLaunchKernelGGL(k_part1_count, dim3(grid), dim3(PT_THREADS), 0, c->stream, c->otgt[1], c->opart_boff, G,
+                           bins0, b1, c->opart_p1[0]);
+        NNM_LAUNCH_CHECK(c);
+        NNM_TRY(nnm_scan_u32(c, c->opart_p1[0], tab, c->opart_p1[1], c->opart_tmp));
+        hipLaunchKernelGGL(k_part1_scatter, dim3(grid), dim3(PT_THREADS), 0, c->stream, c->okey[1], c->otgt[1],
+                           c->opart_boff, G, bins0, b1, c->opart_p1[1], c->okey[0], c->otgt[0]);
+        NNM_LAUNCH_CHECK(c);
+        in.key = c->okey[0];
+        in.tgt = c->otgt[0];
+        bin_end = c->opart_p1[1] + P1_SEGS;  // owner bin ob ends where entry (ob + 1) x P1_SEGS starts
+        bin_stride = P1_SEGS;
+    }
     OwnArgs a;
     a.n = c->n_act;
     a.K = c->K;
@@ -1149,7 +1267,7 @@ int nnm_offer_deliver(nnm_ctx* c, const nnm_params* p) {
     a.tgt = in.tgt;
     a.bin_end = bin_end;
     a.bin_stride = bin_stride;
-    a.nbins = pl.bins[pl.npass - 1];
+    a.nbins = nob;
     a.counters = c->counters;
     int rerank = 0;
     if (p->prec == NNM_PREC_BF16)
