@@ -279,6 +279,8 @@ static void free_graph(nnm_ctx* c) {
     c->s_ids = nullptr; c->s_dist = nullptr; c->s_flag = nullptr;
     nnm_free(c, c->p_ids);
     c->p_ids = nullptr;
+    nnm_free(c, c->p_dist);
+    c->p_dist = nullptr;
     nnm_free(c, c->c_ids); nnm_free(c, c->c_dist); nnm_free(c, c->c_flag); nnm_free(c, c->has_cpn);
     c->c_ids = nullptr; c->c_dist = nullptr; c->c_flag = nullptr; c->has_cpn = nullptr;
     c->K = 0; c->cut = 0; c->cpn_live = 0;
@@ -812,9 +814,13 @@ __global__ void k_lists_estimate_fp8(uint64_t n, uint32_t K, const uint8_t* __re
 // exact distances outside the join.
 // prior (optional): the lists when the learned phase started; an entry whose
 // id is not in its row's prior list was found by the learned phase and is
-// marked new.
+// marked new.  prior_dist (optional, with prior): the exact distances of the
+// prior lists; an entry that was there keeps its exact distance from them
+// (the same function of the same rows, so the same bits) and only the
+// entries the learned phase found are computed.
 __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ X, const uint16_t* __restrict__ Xb_exact,
-                              uint32_t ld, int renew, const uint32_t* __restrict__ prior, uint32_t* __restrict__ g_ids,
+                              uint32_t ld, int renew, const uint32_t* __restrict__ prior,
+                              const float* __restrict__ prior_dist, uint32_t* __restrict__ g_ids,
                               float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
                               uint64_t* __restrict__ counters) {
     const uint32_t hl = nnm_lane() & 31u, hbase = nnm_lane() & 32u;
@@ -823,20 +829,33 @@ __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ 
         const uint64_t v = v0 + (nnm_lane() >> 5);
         const bool ok = v < n;
         uint32_t id = 0xFFFFFFFFu, f = 0, pr = 0xFFFFFFFFu;
-        float dd = 3.402823466e38f;
+        float dd = 3.402823466e38f, pdv = 0.f;
         if (ok && hl < K) {
             id = g_ids[v * K + hl];
             f = renew ? 1u : g_flag[v * K + hl];
             if (prior) pr = prior[v * K + hl];
-            dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
-                          : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
-            ++nd;
+            if (prior_dist) pdv = prior_dist[v * K + hl];
         }
+        bool found = false;
+        float known = 0.f;
         if (prior) {
             // membership in the half's prior row (every lane takes part in the shuffles)
-            bool found = false;
-            for (uint32_t p = 0; p < K; ++p) found |= (uint32_t)__shfl((int)pr, (int)(hbase + p), NNM_WAVE) == id;
+            for (uint32_t p = 0; p < K; ++p) {
+                const bool hit = (uint32_t)__shfl((int)pr, (int)(hbase + p), NNM_WAVE) == id;
+                const float pd = __shfl(pdv, (int)(hbase + p), NNM_WAVE);
+                known = hit ? pd : known;
+                found |= hit;
+            }
             if (!found) f = 1u;
+        }
+        if (ok && hl < K) {
+            if (prior_dist && found) {
+                dd = known;
+            } else {
+                dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
+                              : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
+                ++nd;
+            }
         }
         half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
@@ -1990,6 +2009,11 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     const bool fp8 = learned && (p->flags & NNM_FLAG_LEARNED_FP8);
     const bool mark_found = learned && (p->flags & NNM_FLAG_LEARNED_MARK_FOUND) && !(p->flags & NNM_FLAG_LEARNED_RENEW);
     if (mark_found && !c->p_ids) NNM_TRY(alloc_n(c, &c->p_ids, c->n * c->K));
+    // the exact distances of the lists at the learned phase's start, reused
+    // at the switch for the entries still there (one call runs the whole
+    // phase, so the lists it copies are exact)
+    const bool reuse_dist = mark_found && !single;
+    if (reuse_dist && !c->p_dist) NNM_TRY(alloc_n(c, &c->p_dist, c->n * c->K));
     if (fp8) {
         if (nnm_mlp_dout(c) != 32) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "FP8 learned join built for d_out = 32");
         NNM_TRY(nnm_mlp_ensure_fp8(c));
@@ -2023,6 +2047,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         if (mark_found)
             NNM_HIP(c, hipMemcpyAsync(c->p_ids, c->g_ids, (size_t)c->n_act * c->K * sizeof(uint32_t),
                                       hipMemcpyDeviceToDevice, c->stream));
+        if (reuse_dist)
+            NNM_HIP(c, hipMemcpyAsync(c->p_dist, c->g_dist, (size_t)c->n_act * c->K * sizeof(float),
+                                      hipMemcpyDeviceToDevice, c->stream));
         const uint32_t dout = nnm_mlp_dout(c);
         if (fp8)
             hipLaunchKernelGGL(k_lists_estimate_fp8, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream,
@@ -2040,7 +2067,8 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     auto to_exact = [&]() -> int {
         hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
                            (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld,
-                           (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, mark_found ? c->p_ids : nullptr, c->g_ids,
+                           (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, mark_found ? c->p_ids : nullptr,
+                           reuse_dist ? c->p_dist : nullptr, c->g_ids,
                            c->g_dist, c->g_flag, c->counters);
         NNM_LAUNCH_CHECK(c);
         estimates = false;

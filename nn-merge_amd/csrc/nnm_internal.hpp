@@ -109,6 +109,7 @@ struct nnm_ctx {
     float* s_dist = nullptr;
     uint8_t* s_flag = nullptr;
     uint32_t* p_ids = nullptr;      // lists at the learned phase's start (NNM_FLAG_LEARNED_MARK_FOUND)
+    float* p_dist = nullptr;        // ... and their exact distances
 
     // parked far entries of a merge (cpnGraph)
     uint32_t cut = 0;

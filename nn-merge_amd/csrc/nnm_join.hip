@@ -619,7 +619,7 @@ __device__ inline void block_offers(const JoinArgs& a, OfferWriter& w, const nnm
 // U: 8-byte units per row (bf16 rows of 4U elements; F8: e4m3 rows of 8U);
 // MODE: the offer key (KEY_EXACT, KEY_BOUND, KEY_LEARN).
 template <int U, bool F8, int MODE>
-__global__ __launch_bounds__(64) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb, float kscale, float tau) {
+__global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb, float kscale, float tau) {
     static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
     constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
     constexpr uint32_t DW = 2 * U;  // dwords per row

@@ -106,7 +106,10 @@ def test_c4_learned_merge_properties(c4, fp8, T, tail):
     print(f"\nlearned {'fp8' if fp8 else 'bf16'} rows: recall {rec:.4f} wall {secs:.3f}s descent {st['secs']:.3f}s "
           f"exact {exact} learned {st['learned_dists']} offers {st['offers']} train {tr}")
     assert st["learned_iters"] == T - tail and st["plain_iters"] == tail
-    assert st["reranked"] == N * K  # the switch recomputes every list entry once
+    # the switch recomputes the entries the learned phase found; the others
+    # keep their exact distances from the phase's start (checked with every
+    # other list distance against FP64 below)
+    assert 0 < st["reranked"] <= N * K
     assert st["redo_rows"] == 0 or st["redo_rounds"] > 0
     _properties(ctx, ids)
     assert rec >= 0.95
