@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--learn-tau", type=float, default=0.0,
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = the sweep "
                          "picks the fastest of 0.6 and 0.7 reaching the target)")
+    ap.add_argument("--learn-rho", type=float, default=0.0,
+                    help="learned: forward sampling rate of the learned phase (0 = --rho)")
     ap.add_argument("--fp8", action="store_true",
                     help="learned: run the learned phase on FP8 (OCP e4m3) rows of the projection")
     ap.add_argument("--switch", choices=["found", "keep", "renew"], default="found",
@@ -365,6 +367,8 @@ def main():
                                plain_tail=tail, flags=fl, offer_cap=args.offer_cap)
         if name == "learned" and tau > 0:
             p.learn_tau = tau
+        if name == "learned" and args.learn_rho > 0:
+            p.learn_rho = args.learn_rho
         return p
 
     def step(name, p):

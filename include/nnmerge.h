@@ -123,7 +123,9 @@ typedef struct nnm_params {
                            pair to a target when its estimate is below
                            learn_tau x the target's estimated k-th distance
                            (0 = 0.6); the plain tail uses the exact k-th */
-    uint32_t reserved;  /* 0 */
+    float learn_rho;    /* NNM_PREC_LEARNED: forward sampling rate of the
+                           learned phase (at most round(learn_rho*K) new
+                           entries; 0 = rho); the plain tail samples at rho */
     uint64_t seed;      /* every random choice derives from this seed */
 } nnm_params;
 

@@ -208,7 +208,7 @@ extern "C" void nnm_default_params(nnm_params* p) {
     p->plain_tail = 1;
     p->flags = 0;
     p->learn_tau = NNM_LEARN_TAU_DEFAULT;
-    p->reserved = 0;
+    p->learn_rho = 0.f;
     p->seed = 1;
 }
 
@@ -1753,7 +1753,9 @@ extern "C" int nnm_graph_dist_error(nnm_ctx* c, nnm_dist_error* e) {
 
 int nnm_ensure_iter_buffers(nnm_ctx* c, const nnm_params* p) {
     const uint32_t K = c->K;
-    uint32_t S = (uint32_t)std::lround((double)p->rho * K);
+    // the learned phase may sample more new entries than the plain iterations
+    const float rho = p->prec == NNM_PREC_LEARNED ? std::max(p->rho, p->learn_rho) : p->rho;
+    uint32_t S = (uint32_t)std::lround((double)rho * K);
     S = std::max<uint32_t>(1, std::min<uint32_t>(S, K));
     const uint32_t R = p->rev_cap;
     const uint32_t cap_new = S + R, cap_old = K + R;
@@ -1788,6 +1790,8 @@ static int validate_params(nnm_ctx* c, const nnm_params* p) {
     if (!c->g_ids) return nnm_fail(c, NNM_ERR_STATE, "no graph");
     if (p->K != c->K) return nnm_fail(c, NNM_ERR_INVALID, "params K differs from the graph's K");
     if (!(p->rho > 0.f && p->rho <= 1.f)) return nnm_fail(c, NNM_ERR_INVALID, "rho must be in (0, 1]");
+    if (!(p->learn_rho >= 0.f && p->learn_rho <= 1.f))
+        return nnm_fail(c, NNM_ERR_INVALID, "learn_rho must be 0 (rho) or in (0, 1]");
     if (p->rev_cap == 0 || p->rev_cap > 4096) return nnm_fail(c, NNM_ERR_INVALID, "rev_cap must be in [1, 4096]");
     if (p->offer_cap != 0 && (p->offer_cap < 4 || p->offer_cap > 65536))
         return nnm_fail(c, NNM_ERR_INVALID, "offer_cap must be 0 (default) or in [4, 65536]");
@@ -2080,6 +2084,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         if (p->prec == NNM_PREC_LEARNED) {
             if (p->max_iters && iter + p->plain_tail >= p->max_iters) tail = true;
             q.prec = tail ? NNM_PREC_BF16 : NNM_PREC_LEARNED;
+            if (!tail && p->learn_rho > 0.f) q.rho = p->learn_rho;
             if (q.prec == NNM_PREC_LEARNED && !estimates) NNM_TRY(to_estimates());
             if (q.prec != NNM_PREC_LEARNED && estimates) NNM_TRY(to_exact());
         }

@@ -974,7 +974,7 @@ __device__ inline void own2_load(const OwnArgs& a, const Own2Target& tg, uint64_
 
 // Owner bins as in k_own (counting sort of each chunk by target in LDS);
 // the bin's targets with records are taken two at a time by the waves.
-__global__ __launch_bounds__(OWN2_WAVES * NNM_WAVE) void k_own2(OwnArgs a) {
+__global__ __launch_bounds__(OWN2_WAVES * NNM_WAVE, 6) void k_own2(OwnArgs a) {
     constexpr uint32_t NT = OWN2_WAVES * NNM_WAVE;
     constexpr uint32_t PER = OWN_CH / NT;
     __shared__ uint64_t skey[OWN_CH];

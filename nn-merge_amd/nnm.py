@@ -30,7 +30,7 @@ class Params(C.Structure):
     _fields_ = [("K", C.c_uint32), ("rho", C.c_float), ("rev_cap", C.c_uint32), ("max_iters", C.c_uint32),
                 ("delta", C.c_float), ("rule", C.c_uint32), ("prec", C.c_uint32), ("rerank", C.c_uint32),
                 ("offer_cap", C.c_uint32), ("cut", C.c_uint32), ("plain_tail", C.c_uint32), ("flags", C.c_uint32),
-                ("learn_tau", C.c_float), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+                ("learn_tau", C.c_float), ("learn_rho", C.c_float), ("seed", C.c_uint64)]
 
 
 class Stats(C.Structure):

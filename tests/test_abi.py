@@ -34,7 +34,7 @@ def test_default_params_are_reference_constants():
     p = nnm.default_params()
     assert p.K == 20 and p.rev_cap == 100 and abs(p.delta - 0.002) < 1e-9 and p.rho == 1.0
     assert p.max_iters == 0 and p.rule == nnm.RULE_ALL and p.prec == nnm.PREC_FP32
-    assert abs(p.learn_tau - 0.6) < 1e-6 and p.reserved == 0
+    assert abs(p.learn_tau - 0.6) < 1e-6 and p.learn_rho == 0.0
 
 
 def test_params_struct_layout_matches_header():
