@@ -608,7 +608,7 @@ enum : int { DM_ANY = 0, DM_L2_BF16 = 1, DM_L2_F32 = 2 };
 template <int DM>
 __device__ inline float setup_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                    uint64_t a, uint64_t b, const nnm_metric_args& mt) {
-    if constexpr (DM == DM_L2_BF16) return nnm_l2_rows_bf16n(Xb + a * ld, Xb + b * ld, mt.bnorm[a], mt.bnorm[b], ld);
+    if constexpr (DM == DM_L2_BF16) return nnm_l2_rows_bf16(Xb + a * ld, Xb + b * ld, ld);
     else if constexpr (DM == DM_L2_F32) return nnm_l2_rows(X + a * ld, X + b * ld, ld);
     else return nnm_row_dist(X, Xb, ld, a, b, mt);
 }
@@ -825,7 +825,7 @@ __global__ void k_lists_estimate_fp8(uint64_t n, uint32_t K, const uint8_t* __re
 // (the same function of the same rows, so the same bits) and only the
 // entries the learned phase found are computed.
 __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ X, const uint16_t* __restrict__ Xb_exact,
-                              const float* __restrict__ bnorm, uint32_t ld, int renew, const uint32_t* __restrict__ prior,
+                              uint32_t ld, int renew, const uint32_t* __restrict__ prior,
                               const float* __restrict__ prior_dist, uint32_t* __restrict__ g_ids,
                               float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
                               uint64_t* __restrict__ counters) {
@@ -858,7 +858,7 @@ __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ 
             if (prior_dist && found) {
                 dd = known;
             } else {
-                dd = Xb_exact ? nnm_l2_rows_bf16n(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, bnorm[v], bnorm[id], ld)
+                dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
                               : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
                 ++nd;
             }
@@ -2076,7 +2076,7 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
     };
     auto to_exact = [&]() -> int {
         hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
-                           (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->norms, c->ld,
+                           (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld,
                            (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, mark_found ? c->p_ids : nullptr,
                            reuse_dist ? c->p_dist : nullptr, c->g_ids,
                            c->g_dist, c->g_flag, c->counters);

@@ -390,51 +390,6 @@ __device__ inline float nnm_l2_rows_bf16(const uint16_t* __restrict__ a, const u
     return acc;
 }
 
-// Exact squared L2 of two bf16 rows from their squared norms and the bf16
-// dot product (v_dot2c_f32_bf16, four accumulators): na + nb - 2 a.b.  Used
-// only where the bf16 rows are exact (integer rows whose norms, dot products
-// and distances stay below 2^24), where every product and partial sum is an
-// exact integer in either form, so the value is nnm_l2_rows_bf16's (and the
-// FP32 rows' nnm_l2_rows') to the bit, for 2 elements per instruction
-// instead of one element per 4 instructions.
-typedef __bf16 nnm_bf16x2 __attribute__((ext_vector_type(2)));
-__device__ inline float nnm_l2_rows_bf16n(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, float na,
-                                          float nb, uint32_t ld) {
-    const uint4* a4 = reinterpret_cast<const uint4*>(a);
-    const uint4* b4 = reinterpret_cast<const uint4*>(b);
-    float s[4] = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t n8 = ld / 8;
-    auto dot2 = [](uint32_t x, uint32_t y, float c) {
-        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(nnm_bf16x2, x), __builtin_bit_cast(nnm_bf16x2, y), c,
-                                               false);
-    };
-    uint32_t i = 0;
-    for (; i + 4 <= n8; i += 4) {
-        uint4 x[4], y[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            x[u] = a4[i + u];
-            y[u] = b4[i + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            s[u] = dot2(x[u].x, y[u].x, s[u]);
-            s[u] = dot2(x[u].y, y[u].y, s[u]);
-            s[u] = dot2(x[u].z, y[u].z, s[u]);
-            s[u] = dot2(x[u].w, y[u].w, s[u]);
-        }
-    }
-    for (; i < n8; ++i) {
-        const uint4 x = a4[i], y = b4[i];
-        s[0] = dot2(x.x, y.x, s[0]);
-        s[0] = dot2(x.y, y.y, s[0]);
-        s[0] = dot2(x.z, y.z, s[0]);
-        s[0] = dot2(x.w, y.w, s[0]);
-    }
-    const float g = (s[0] + s[1]) + (s[2] + s[3]);
-    return (na + nb) - 2.f * g;
-}
-
 // Row distance from the bf16 rows when given (values exact in bf16), else
 // from the FP32 rows: the same bits either way.
 __device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
@@ -515,9 +470,8 @@ __device__ inline float nnm_cos_finish(float dot, float l1, float l2) {
 struct nnm_metric_args {
     uint32_t metric;
     const float* sumsq;
-    const float* bnorm;  // squared norms of the bf16 rows (with them)
 };
-inline nnm_metric_args metric_args(const nnm_ctx* c) { return nnm_metric_args{c->metric, c->sumsq, c->norms}; }
+inline nnm_metric_args metric_args(const nnm_ctx* c) { return nnm_metric_args{c->metric, c->sumsq}; }
 
 __device__ inline float nnm_metric_rows(uint32_t metric, const float* __restrict__ a, const float* __restrict__ b,
                                         uint32_t d, float sa, float sb) {
@@ -557,10 +511,7 @@ __device__ inline float metric_step(float acc, float x, float y) {
 // Distance of rows a, b of X under the call's metric (bf16 rows only for L2).
 __device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                      uint64_t a, uint64_t b, const nnm_metric_args& m) {
-    if (m.metric == NNM_METRIC_L2) {
-        if (Xb && m.bnorm) return nnm_l2_rows_bf16n(Xb + a * ld, Xb + b * ld, m.bnorm[a], m.bnorm[b], ld);
-        return nnm_row_dist(X, Xb, ld, a, b);
-    }
+    if (m.metric == NNM_METRIC_L2) return nnm_row_dist(X, Xb, ld, a, b);
     return nnm_metric_rows(m.metric, X + a * ld, X + b * ld, ld, m.sumsq ? m.sumsq[a] : 0.f,
                            m.sumsq ? m.sumsq[b] : 0.f);
 }
