@@ -576,7 +576,9 @@ __device__ inline void block_offers(const JoinArgs& a, OfferWriter& w, const nnm
                                     uint32_t bid, uint32_t blab, float bthr, float bn, uint64_t& nd, uint64_t& no) {
     const bool merge = a.rule == NNM_RULE_MERGE;
     const float tb = tau * bthr;
-    bool pb[4], pa[4];
+    // the offer predicates are kept as wave masks (scalar registers) and
+    // turned back into lane predicates by inverse ballots: no per-lane copies
+    uint64_t mb[4], ma[4];
     float dd[4];
     uint32_t cnt = 0;
 #pragma unroll
@@ -585,10 +587,13 @@ __device__ inline void block_offers(const JoinArgs& a, OfferWriter& w, const nnm
         bool ok = colok & r.in[q] & (!diag | (cj > rbase + (uint32_t)q)) & (r.id[q] != bid);
         ok = ok & !(merge & (r.lab[q] == blab) & (blab != NNM_RAW));
         dd[q] = offer_key<MODE>(r.nrm[q], bn, acc[q], kscale);
-        nd += (uint32_t)__popcll(__ballot(ok));
-        pb[q] = ok & (dd[q] < tb);
-        pa[q] = ok & (dd[q] < tau * r.thr[q]);
-        cnt += (uint32_t)__popcll(__ballot(pb[q])) + (uint32_t)__popcll(__ballot(pa[q]));
+        // pairs outside the rule get an infinite key, so every ballot below
+        // is of one comparison (a wave mask as it is)
+        const float de = ok ? dd[q] : __builtin_inff();
+        nd += (uint32_t)__popcll(__ballot(de < __builtin_inff()));
+        mb[q] = __ballot(de < tb);
+        ma[q] = __ballot(de < tau * r.thr[q]);
+        cnt += (uint32_t)__popcll(mb[q]) + (uint32_t)__popcll(ma[q]);
     }
     if (!cnt) return;
     uint64_t first;
@@ -599,20 +604,18 @@ __device__ inline void block_offers(const JoinArgs& a, OfferWriter& w, const nnm
     uint32_t off = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint64_t mb = __ballot(pb[q]);
-        if (pb[q]) {
-            const uint32_t p = off + nnm_mbcnt(mb);
+        if (__builtin_amdgcn_inverse_ballot_w64(mb[q])) {
+            const uint32_t p = off + nnm_mbcnt(mb[q]);
             kp[p] = ((uint64_t)__float_as_uint(dd[q]) << 32) | r.id[q];
             tp[p] = bid;
         }
-        off += (uint32_t)__popcll(mb);
-        const uint64_t ma = __ballot(pa[q]);
-        if (pa[q]) {
-            const uint32_t p = off + nnm_mbcnt(ma);
+        off += (uint32_t)__popcll(mb[q]);
+        if (__builtin_amdgcn_inverse_ballot_w64(ma[q])) {
+            const uint32_t p = off + nnm_mbcnt(ma[q]);
             kp[p] = ((uint64_t)__float_as_uint(dd[q]) << 32) | bid;
             tp[p] = r.id[q];
         }
-        off += (uint32_t)__popcll(ma);
+        off += (uint32_t)__popcll(ma[q]);
     }
 }
 
