@@ -249,24 +249,79 @@ __global__ __launch_bounds__(PT_THREADS) void k_part0_count(PassIn in, uint32_t 
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
     __syncthreads();
-    const uint64_t nt = tile_count(in);
+    const uint64_t nt = tile_count(in);  // reserved log chunks (each PT_TILE records of capacity)
     for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        uint64_t b;
-        uint32_t m;
-        tile_range(in, t, &b, &m);
-        uint32_t d[PT_PER];
-        tile_digits(in, b, m, shift, tid, d);
+        // the whole chunk is read (no branches), records past its fill count are not counted
+        const uint32_t m = min(in.chunk_used[t], PT_TILE);
+        uint32_t tg[PT_PER];
 #pragma unroll
-        for (uint32_t k = 0; k < PT_PER; ++k)
-            if (d[k] != NO_DIGIT) atomicAdd(&hist[d[k]], 1u);
+        for (uint32_t k = 0; k < PT_PER; ++k) tg[k] = in.tgt[t * PT_TILE + tid + k * PT_THREADS];
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            const uint32_t d = tg[k] >> shift;
+            if (tid + k * PT_THREADS < m && d < in.nbins) atomicAdd(&hist[d], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = tid; i < in.nbins; i += PT_THREADS) bcnt[(uint64_t)i * gridDim.x + blockIdx.x] = hist[i];
 }
 
+// Records [b, b + m) of a tile into registers (thread tid's k-th: b + tid +
+// k PT_THREADS; past m, a repeat of record b + m - 1).  The loads carry no
+// branches, so they all go out before the first is waited for.
+__device__ inline void tile_load(const uint64_t* __restrict__ key, const uint32_t* __restrict__ tgt, uint64_t b,
+                                 uint32_t m, uint32_t tid, uint32_t (&tg)[PT_PER], uint64_t (&ky)[PT_PER]) {
+    if (m == 0) return;
+#pragma unroll
+    for (uint32_t k = 0; k < PT_PER; ++k) {
+        const uint32_t j = min(tid + k * PT_THREADS, m - 1);
+        tg[k] = tgt[b + j];
+        ky[k] = key[b + j];
+    }
+}
+
+// A tile in registers, by digit d[k] (NO_DIGIT: not moved), out to its bins:
+// ordered by digit in LDS, then written in runs from position curs[d] +
+// (rank within the tile's digit d).  The cursors advance by the tile's
+// counts.  Starts and ends at a block barrier.
+__device__ inline void tile_scatter(const uint32_t (&d)[PT_PER], const uint32_t (&tg)[PT_PER],
+                                    const uint64_t (&ky)[PT_PER], uint32_t shift, uint32_t base,
+                                    uint64_t* lkey, uint32_t* ltgt, uint32_t* hist, uint32_t* curs, uint32_t* wsum,
+                                    uint64_t* __restrict__ okey, uint32_t* __restrict__ otgt) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
+    __syncthreads();
+    uint32_t rk[PT_PER];
+#pragma unroll
+    for (uint32_t k = 0; k < PT_PER; ++k) rk[k] = d[k] != NO_DIGIT ? atomicAdd(&hist[d[k]], 1u) : 0u;
+    __syncthreads();
+    uint32_t c2[2];
+    const uint32_t nl = block_prefix_bins(hist, wsum, c2);
+#pragma unroll
+    for (uint32_t k = 0; k < PT_PER; ++k) {
+        if (d[k] == NO_DIGIT) continue;
+        const uint32_t lp = hist[d[k]] + rk[k];
+        lkey[lp] = ky[k];
+        ltgt[lp] = tg[k];
+    }
+    __syncthreads();
+    for (uint32_t lp = tid; lp < nl; lp += PT_THREADS) {
+        const uint32_t tgv = ltgt[lp];
+        const uint32_t r = (tgv >> shift) - base;
+        const uint32_t p = curs[r] + (lp - hist[r]);
+        okey[p] = lkey[lp];
+        otgt[p] = tgv;
+    }
+    __syncthreads();
+    curs[2 * tid] += c2[0];
+    curs[2 * tid + 1] += c2[1];
+    __syncthreads();
+}
+
 // First-pass scatter: the same blocks over the same tiles; block runs start
-// at the scanned per-block counts boff[bin * grid + block].  Each tile is
-// ordered by bin in LDS and written out in runs.
+// at the scanned per-block counts boff[bin * grid + block].  The next tile
+// (a whole log chunk, its fill count beside it) is read while the current one
+// is ordered and written; records past the fill count are not moved.
 __global__ __launch_bounds__(PT_THREADS) void k_part0_scatter(PassIn in, uint32_t shift,
                                                               const uint32_t* __restrict__ boff,
                                                               uint64_t* __restrict__ okey, uint32_t* __restrict__ otgt) {
@@ -277,53 +332,40 @@ __global__ __launch_bounds__(PT_THREADS) void k_part0_scatter(PassIn in, uint32_
     __shared__ uint32_t wsum[PT_THREADS / NNM_WAVE];
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < in.nbins; i += PT_THREADS) curs[i] = boff[(uint64_t)i * gridDim.x + blockIdx.x];
-    const uint64_t nt = tile_count(in);
-    for (uint64_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        uint64_t b;
-        uint32_t m;
-        tile_range(in, t, &b, &m);
-        if (m == 0) continue;
-        for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
-        __syncthreads();
-        uint32_t d[PT_PER], tg[PT_PER], rk[PT_PER];
-        uint64_t ky[PT_PER];
+    const uint64_t nt = tile_count(in);  // reserved log chunks (each PT_TILE records of capacity)
+    uint64_t t = blockIdx.x;
+    uint32_t m = 0;
+    uint32_t tg[PT_PER];
+    uint64_t ky[PT_PER];
+    if (t < nt) {
+        m = min(in.chunk_used[t], PT_TILE);
+        tile_load(in.key, in.tgt, t * PT_TILE, PT_TILE, tid, tg, ky);
+    }
+    // The first tile is waited for here: in the loop, the only loads in
+    // flight are then the next tile's, and the current tile's first use
+    // does not wait for them.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (; t < nt; t += gridDim.x) {
+        uint32_t d[PT_PER];
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
-            const uint32_t j = tid + k * PT_THREADS;
-            tg[k] = 0;
-            ky[k] = 0;
-            d[k] = NO_DIGIT;
-            rk[k] = 0;
-            if (j < m) {
-                tg[k] = in.tgt[b + j];
-                ky[k] = in.key[b + j];
-                d[k] = tg[k] >> shift;
-                if (d[k] >= in.nbins) d[k] = NO_DIGIT;
-            }
-            if (d[k] != NO_DIGIT) rk[k] = atomicAdd(&hist[d[k]], 1u);
+            d[k] = tid + k * PT_THREADS < m ? tg[k] >> shift : NO_DIGIT;
+            if (d[k] >= in.nbins) d[k] = NO_DIGIT;
         }
-        __syncthreads();
-        uint32_t c2[2];
-        const uint32_t nl = block_prefix_bins(hist, wsum, c2);
+        // the next tile (past the last, the last again, not used): loads
+        // without branches, waited for only after this tile is written
+        const uint64_t tn = t + gridDim.x, tl = min(tn, nt - 1);
+        const uint32_t mraw = in.chunk_used[tl];
+        uint32_t ntg[PT_PER];
+        uint64_t nky[PT_PER];
+        tile_load(in.key, in.tgt, tl * PT_TILE, PT_TILE, tid, ntg, nky);
+        if (m) tile_scatter(d, tg, ky, shift, 0u, lkey, ltgt, hist, curs, wsum, okey, otgt);
 #pragma unroll
         for (uint32_t k = 0; k < PT_PER; ++k) {
-            if (d[k] == NO_DIGIT) continue;
-            const uint32_t lp = hist[d[k]] + rk[k];
-            lkey[lp] = ky[k];
-            ltgt[lp] = tg[k];
+            tg[k] = ntg[k];
+            ky[k] = nky[k];
         }
-        __syncthreads();
-        for (uint32_t lp = tid; lp < nl; lp += PT_THREADS) {
-            const uint32_t tgv = ltgt[lp];
-            const uint32_t r = tgv >> shift;
-            const uint32_t p = curs[r] + (lp - hist[r]);
-            okey[p] = lkey[lp];
-            otgt[p] = tgv;
-        }
-        __syncthreads();
-        curs[2 * tid] += c2[0];
-        curs[2 * tid + 1] += c2[1];
-        __syncthreads();
+        m = tn < nt ? min(mraw, PT_TILE) : 0u;
     }
 }
 
@@ -354,15 +396,12 @@ __global__ __launch_bounds__(PT_THREADS) void k_part1_count(const uint32_t* __re
         for (uint32_t i = tid; i < nl; i += PT_THREADS) hist[i] = 0;
         __syncthreads();
         for (uint32_t i0 = s; i0 < e; i0 += PT_TILE) {
-            uint32_t d[PT_PER];
+            uint32_t tg[PT_PER];
 #pragma unroll
-            for (uint32_t k = 0; k < PT_PER; ++k) {
-                const uint32_t i = i0 + tid + k * PT_THREADS;
-                d[k] = i < e ? (tgt[i] >> NNM_OWN_BITS) - ob0 : NO_DIGIT;
-            }
+            for (uint32_t k = 0; k < PT_PER; ++k) tg[k] = tgt[min(i0 + tid + k * PT_THREADS, e - 1)];  // no branches
 #pragma unroll
             for (uint32_t k = 0; k < PT_PER; ++k)
-                if (d[k] != NO_DIGIT) atomicAdd(&hist[d[k]], 1u);
+                if (i0 + tid + k * PT_THREADS < e) atomicAdd(&hist[(tg[k] >> NNM_OWN_BITS) - ob0], 1u);
         }
         __syncthreads();
         for (uint32_t i = tid; i < nl; i += PT_THREADS) cnt[((uint64_t)r * nl + i) * P1_SEGS + seg] = hist[i];
@@ -386,51 +425,27 @@ __global__ __launch_bounds__(PT_THREADS) void k_part1_scatter(const uint64_t* __
         const uint32_t r = u / P1_SEGS, seg = u % P1_SEGS, ob0 = r << b1;
         uint32_t s, e;
         p1_segment(boff, G, r, seg, &s, &e);
+        uint32_t tg[PT_PER];
+        uint64_t ky[PT_PER];
+        tile_load(key, tgt, s, s < e ? min(PT_TILE, e - s) : 0u, tid, tg, ky);
         for (uint32_t i = tid; i < nl; i += PT_THREADS) curs[i] = off[((uint64_t)r * nl + i) * P1_SEGS + seg];
+        __builtin_amdgcn_s_waitcnt(0);  // the segment's first tile (then only the next tile's loads are in flight)
         for (uint32_t t0 = s; t0 < e; t0 += PT_TILE) {
             const uint32_t m = min(PT_TILE, e - t0);
-            for (uint32_t i = tid; i < PT_BINS; i += PT_THREADS) hist[i] = 0;
-            __syncthreads();
-            uint32_t d[PT_PER], tg[PT_PER], rk[PT_PER];
-            uint64_t ky[PT_PER];
-#pragma unroll
-            for (uint32_t k = 0; k < PT_PER; ++k) {
-                const uint32_t j = tid + k * PT_THREADS;
-                tg[k] = 0;
-                ky[k] = 0;
-                d[k] = NO_DIGIT;
-                rk[k] = 0;
-                if (j < m) {
-                    tg[k] = tgt[t0 + j];
-                    ky[k] = key[t0 + j];
-                    d[k] = (tg[k] >> NNM_OWN_BITS) - ob0;
-                }
-            }
+            uint32_t d[PT_PER];
 #pragma unroll
             for (uint32_t k = 0; k < PT_PER; ++k)
-                if (d[k] != NO_DIGIT) rk[k] = atomicAdd(&hist[d[k]], 1u);
-            __syncthreads();
-            uint32_t c2[2];
-            block_prefix_bins(hist, wsum, c2);
+                d[k] = tid + k * PT_THREADS < m ? (tg[k] >> NNM_OWN_BITS) - ob0 : NO_DIGIT;
+            const uint32_t tn = t0 + PT_TILE;
+            uint32_t ntg[PT_PER];
+            uint64_t nky[PT_PER];
+            tile_load(key, tgt, tn, tn < e ? min(PT_TILE, e - tn) : 0u, tid, ntg, nky);
+            tile_scatter(d, tg, ky, NNM_OWN_BITS, ob0, lkey, ltgt, hist, curs, wsum, okey, otgt);
 #pragma unroll
             for (uint32_t k = 0; k < PT_PER; ++k) {
-                if (d[k] == NO_DIGIT) continue;
-                const uint32_t lp = hist[d[k]] + rk[k];
-                lkey[lp] = ky[k];
-                ltgt[lp] = tg[k];
+                tg[k] = ntg[k];
+                ky[k] = nky[k];
             }
-            __syncthreads();
-            for (uint32_t lp = tid; lp < m; lp += PT_THREADS) {
-                const uint32_t tgv = ltgt[lp];
-                const uint32_t dd = (tgv >> NNM_OWN_BITS) - ob0;
-                const uint32_t p = curs[dd] + (lp - hist[dd]);
-                okey[p] = lkey[lp];
-                otgt[p] = tgv;
-            }
-            __syncthreads();
-            curs[2 * tid] += c2[0];
-            curs[2 * tid + 1] += c2[1];
-            __syncthreads();
         }
     }
 }
