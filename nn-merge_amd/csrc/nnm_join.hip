@@ -499,10 +499,12 @@ __global__ __launch_bounds__(64) void k_join_f32(JoinArgs a) {
 // |x_j|) (bf16 rounding moves each element by at most 2^-8 of itself), and the
 // update kernel recomputes every offered candidate in exact FP32.
 //
-// LDS: bf16 rows at a pitch of exactly one row, copied by 4-byte direct-to-LDS
-// loads (one row per instruction); the 8-byte unit u of row r sits at
-// u ^ (r & SWM), so the 16 lanes that read one MFMA operand column (16 rows,
-// one unit) hit 16 different bank pairs.
+// LDS: bf16 rows at a pitch of exactly one row, copied by 16-byte
+// direct-to-LDS loads (several rows per instruction) into a chunk-swizzled
+// layout (RowChunks).  Both join kernels read the Gram blocks in one order
+// (gram_block), so a pair's value is the same whichever kernel joins it: the
+// owners' de-dup of repeated offers relies on that for estimates, whose FP32
+// sums depend on the order.
 typedef short nnm_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float nnm_f32x4 __attribute__((ext_vector_type(4)));
 
@@ -541,6 +543,64 @@ struct Bf16Lds {
     uint4 rec[BJS];                // the rows' join records {label, k-th bits, norm bits, 0}
     uint32_t id[BJS];
 };
+
+// Window rows in LDS as 16-byte chunks (U / 2 per row, copied by 16-byte
+// direct-to-LDS loads): chunk c of row r sits at chunk slot c ^ sw(r).  sw
+// spreads the 16 rows of an MFMA operand over the eight 16-byte bank groups
+// of a 128-byte LDS line (rows that share a line get different slots).
+template <int U>
+struct RowChunks {
+    static constexpr uint32_t CH = U / 2;                                     // chunks per row
+    static constexpr uint32_t S = U == 4 ? 2u : (U == 8 || U == 24) ? 1u : 0u;
+    static constexpr uint32_t M = U == 4 ? 1u : (U == 8 || U == 24) ? 3u : 7u;
+    static constexpr uint32_t RPI = NNM_WAVE / CH;                            // rows per load instruction
+    __device__ static uint32_t sw(uint32_t r) { return (r >> S) & M; }
+};
+
+// Rows [0, rows) of a window into LDS (lane r of wcp: the id of row r): per
+// instruction RPI rows, lane l fetching chunk (l % CH) ^ sw(r) of row r0 + l / CH.
+template <int U>
+__device__ inline void stage_rows(uint2* dst, const uint16_t* __restrict__ Xb, uint32_t wcp, uint32_t rows,
+                                  uint32_t lane) {
+    using C = RowChunks<U>;
+    const uint32_t rr = lane / C::CH, c = lane % C::CH;
+    const char* xb = reinterpret_cast<const char*>(Xb);
+    for (uint32_t r0 = 0; r0 < rows; r0 += C::RPI) {
+        const uint32_t r = r0 + rr;
+        const uint32_t id = (uint32_t)__shfl((int)wcp, (int)min(r, NNM_WAVE - 1u), NNM_WAVE);
+        if (rr < C::RPI && r < rows) {
+            const char* g = xb + ((uint64_t)id * C::CH + (c ^ C::sw(r))) * 16u;
+            __builtin_amdgcn_global_load_lds((const void*)g, (__attribute__((address_space(3))) void*)(dst + r0 * U),
+                                             16, 0, 0);
+        }
+    }
+}
+
+// The 16x16 Gram block of window rows ra (lane row lr) and rc (lane column):
+// lane group lg supplies chunks lg, lg + 4, ... (both halves of a chunk feed
+// two MFMA steps), or for U = 4 unit lg.  Every k position is summed once;
+// A and B take the same positions.
+template <int U, bool F8>
+__device__ inline nnm_f32x4 gram_block(const uint2* rows, uint32_t ra, uint32_t rc, uint32_t lg) {
+    using C = RowChunks<U>;
+    nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (U >= 8) {
+        const uint4* r16 = reinterpret_cast<const uint4*>(rows);
+#pragma unroll
+        for (uint32_t kc = 0; kc < (uint32_t)U / 8; ++kc) {
+            const uint32_t c = kc * 4 + lg;
+            const uint4 av = r16[ra * C::CH + (c ^ C::sw(ra))];
+            const uint4 bv = r16[rc * C::CH + (c ^ C::sw(rc))];
+            acc = gram_step<F8>(make_uint2(av.x, av.y), make_uint2(bv.x, bv.y), acc);
+            acc = gram_step<F8>(make_uint2(av.z, av.w), make_uint2(bv.z, bv.w), acc);
+        }
+    } else {
+        const uint2 av = rows[ra * U + (lg ^ (C::sw(ra) << 1))];
+        const uint2 bv = rows[rc * U + (lg ^ (C::sw(rc) << 1))];
+        acc = gram_step<F8>(av, bv, acc);
+    }
+    return acc;
+}
 
 // Offer key of a pair from the squared norms and the Gram entry.
 template <int MODE>
@@ -624,13 +684,10 @@ __device__ inline void block_offers(const JoinArgs& a, OfferWriter& w, const nnm
 template <int U, bool F8, int MODE>
 __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t* __restrict__ Xb, float kscale, float tau) {
     static_assert(U % 4 == 0 && U <= 32, "bf16 rows of 16..128 elements");
-    constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
-    constexpr uint32_t DW = 2 * U;  // dwords per row
     __shared__ __align__(16) Bf16Lds<U> L;
     const uint32_t js = min(a.js, (uint32_t)BJS);
     const uint32_t lane = threadIdx.x;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;  // wave-uniform
-    const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
     OfferWriter w;
 
     for (;;) {
@@ -648,10 +705,12 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
         }
         uint32_t i = 0;
         while (i < cnt) {
+            // pack up to BFB lists into one window: list b's start row, new
+            // entries, length and work-chunk index sit in lane b; marks has
+            // a bit at every list's start row
             uint32_t nbv = 0, rows = 0, be = 0;
-            uint32_t off[BFB], nnv[BFB], mv[BFB], vid[BFB];
-#pragma unroll
-            for (int b = 0; b < BFB; ++b) { off[b] = 0; nnv[b] = 0; mv[b] = 0; vid[b] = 0; }
+            uint32_t l_off = 0, l_nn = 0, l_m = 0, l_vid = 0;
+            uint64_t marks = 0;
             while (i < cnt && nbv < BFB) {
                 const uint32_t nn = wave_read32(cn, i);
                 const uint32_t m = nn + wave_read32(co, i);
@@ -661,9 +720,13 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
                     continue;
                 }
                 if (rows + m > js) break;
-#pragma unroll
-                for (int b = 0; b < BFB; ++b)
-                    if ((uint32_t)b == nbv) { off[b] = rows; nnv[b] = nn; mv[b] = m; vid[b] = i; }
+                if (lane == nbv) {
+                    l_off = rows;
+                    l_nn = nn;
+                    l_m = m;
+                    l_vid = i;
+                }
+                marks |= 1ull << rows;
                 rows += m;
                 ++nbv;
                 ++i;
@@ -671,12 +734,7 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
             }
             if (nbv == 0) continue;
             // vertex of packed list b in lane b (for the redo list)
-            uint32_t bvtx = 0;
-#pragma unroll
-            for (int q = 0; q < BFB; ++q) {
-                const uint32_t t = wave_read32(vtx, vid[q]);
-                if (lane == (uint32_t)q) bvtx = t;
-            }
+            const uint32_t bvtx = (uint32_t)__shfl((int)vtx, (int)l_vid, NNM_WAVE);
             if (w.full) {  // the log is out of chunks: join the batch in a redo round
                 offer_redo(a.sink, lane < nbv, bvtx);
                 continue;
@@ -684,10 +742,11 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
             wave_lds_sync();
             uint32_t wid = 0;
             {
-                uint32_t ob = 0, nb_ = 0, vb = 0;
-#pragma unroll
-                for (int q = 0; q < BFB; ++q)
-                    if ((uint32_t)q < nbv && lane >= off[q]) { ob = off[q]; nb_ = nnv[q]; vb = vid[q]; }
+                // the list of window row `lane`: list starts at or below it, less one
+                const uint32_t lb = (uint32_t)__popcll(marks & ((2ull << lane) - 1ull)) - 1u;
+                const uint32_t ob = (uint32_t)__shfl((int)l_off, (int)lb, NNM_WAVE);
+                const uint32_t nb_ = (uint32_t)__shfl((int)l_nn, (int)lb, NNM_WAVE);
+                const uint32_t vb = (uint32_t)__shfl((int)l_vid, (int)lb, NNM_WAVE);
                 const uint64_t v = (uint32_t)__shfl((int)vtx, (int)vb, NNM_WAVE);
                 if (lane < rows) {
                     const uint32_t e = lane - ob;
@@ -697,7 +756,7 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
             // ids and join records -> LDS ahead of the rows (an LDS store after
             // a direct-to-LDS load would wait for it): the records and the
             // rows are in flight together, one wait for both
-            // The row loop reads the ids from a VALU copy made in the same
+            // The row copy reads the ids from a VALU copy made in the same
             // block: a register written by a memory load and read after the
             // block would make the compiler wait for every outstanding load
             // (the records) first.
@@ -708,36 +767,13 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
                                                  (__attribute__((address_space(3))) void*)L.rec, 16, 0, 0);
                 asm volatile("v_mov_b32 %0, %1" : "=v"(wcp) : "v"(wid));
             }
-            // rows -> LDS: lane l of a row fetches dword 2u + (l & 1), u = (l/2) ^ (r & SWM).
-            // Rows shorter than 64 dwords go RPI to an instruction (lanes
-            // [DW g, DW g + DW) take row r0 + g, which lands at the group's
-            // LDS base + DW g dwords: the next row).
-            constexpr uint32_t RPI = DW < NNM_WAVE ? NNM_WAVE / DW : 1u;
-            const uint32_t rg = lane / DW, rl = lane % DW;
-            for (uint32_t r0 = 0; r0 < rows; r0 += RPI) {
-                uint32_t id = wave_read32(wcp, r0);
-#pragma unroll
-                for (uint32_t k = 1; k < RPI; ++k) {
-                    const uint32_t t = wave_read32(wcp, r0 + k);  // lanes past the window hold 0
-                    if (rg == k) id = t;
-                }
-                const uint32_t r = r0 + rg;
-                if (rg < RPI && r < rows) {
-                    const uint32_t u = (rl >> 1) ^ (r & SWM);
-                    const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (rl & 1);
-                    __builtin_amdgcn_global_load_lds((const void*)g,
-                                                     (__attribute__((address_space(3))) void*)(L.rows + r0 * U), 4, 0, 0);
-                }
-            }
+            stage_rows<U>(L.rows, Xb, wcp, rows, lane);
             __builtin_amdgcn_s_waitcnt(0);
             wave_lds_sync();
             const uint32_t lr = lane & 15u, lg = lane >> 4;
             uint64_t bd = 0, bo = 0;
             for (uint32_t b = 0; b < nbv && !w.full; ++b) {
-                uint32_t ob = 0, nb_ = 0, mb = 0;
-#pragma unroll
-                for (int q = 0; q < BFB; ++q)
-                    if ((uint32_t)q == b) { ob = off[q]; nb_ = nnv[q]; mb = mv[q]; }
+                const uint32_t ob = wave_read32(l_off, b), nb_ = wave_read32(l_nn, b), mb = wave_read32(l_m, b);
                 const uint32_t RB = (nb_ + 15) / 16, CB = (mb + 15) / 16;
                 for (uint32_t rb = 0; rb < RB && !w.full; ++rb) {
                     // the lane's four rows of the row block (rows past the
@@ -757,14 +793,7 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
                     const uint32_t ra = min(ob + rb * 16 + lr, (uint32_t)BJS - 1);
                     for (uint32_t cb = rb; cb < CB; ++cb) {
                         const uint32_t rc = min(ob + cb * 16 + lr, (uint32_t)BJS - 1);
-                        nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                        for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
-                            const uint32_t u = ks * 4 + lg;
-                            const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
-                            const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
-                            acc = gram_step<F8>(av, bv, acc);
-                        }
+                        const nnm_f32x4 acc = gram_block<U, F8>(L.rows, ra, rc, lg);
                         const uint32_t cj = cb * 16 + lr;
                         const bool colok = cj < mb;
                         const uint4 br = L.rec[rc];
@@ -803,8 +832,7 @@ struct Bf16LongLds {
 template <int U, bool F8, int MODE>
 __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16_t* __restrict__ Xb, float kscale,
                                                         float tau) {
-    constexpr uint32_t SWM = (U % 16 == 0) ? 15u : ((U % 8 == 0) ? 7u : 3u);
-    constexpr uint32_t DW = 2 * U;
+    using C = RowChunks<U>;
     __shared__ __align__(16) Bf16LongLds<U> L;
     __shared__ uint32_t lq[256];    // long lists of the block's current work chunk
     __shared__ uint32_t wcnt[4];
@@ -812,7 +840,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
     __shared__ uint32_t wfull;  // a wave of the block ran out of log chunks
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint64_t my_dists = 0, my_entries = 0, my_offers = 0;  // wave-uniform
-    const uint32_t* xw = reinterpret_cast<const uint32_t*>(Xb);
+    const char* xb = reinterpret_cast<const char*>(Xb);
     OfferWriter w;
     if (tid == 0) wfull = 0;
     for (;;) {
@@ -853,14 +881,18 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
                 L.meta[3][j] = mt.z;
             }
             __syncthreads();
-            // each wave copies every 4th row
-            for (uint32_t r = wave; r < m; r += 4) {
-                const uint32_t id = L.meta[0][r];
-                if (lane < DW) {
-                    const uint32_t u = (lane >> 1) ^ (r & SWM);
-                    const uint32_t* g = xw + (uint64_t)id * DW + 2 * u + (lane & 1);
-                    __builtin_amdgcn_global_load_lds((const void*)g,
-                                                     (__attribute__((address_space(3))) void*)(L.rows + r * U), 4, 0, 0);
+            // rows in the short-list kernel's chunk layout: the waves take
+            // turns by load instruction (RPI rows each)
+            {
+                const uint32_t rr = lane / C::CH, c = lane % C::CH;
+                for (uint32_t r0 = wave * C::RPI; r0 < m; r0 += 4 * C::RPI) {
+                    const uint32_t r = r0 + rr;
+                    if (rr < C::RPI && r < m) {
+                        const char* g = xb + ((uint64_t)L.meta[0][r] * C::CH + (c ^ C::sw(r))) * 16u;
+                        __builtin_amdgcn_global_load_lds((const void*)g,
+                                                         (__attribute__((address_space(3))) void*)(L.rows + r0 * U), 16,
+                                                         0, 0);
+                    }
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);
@@ -885,14 +917,9 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
                 for (uint32_t cb = rb; cb < CB; ++cb, ++blk) {
                     if ((blk & 3u) != wave || w.full) continue;
                     const uint32_t rc = cb * 16 + lr;
-                    nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (uint32_t ks = 0; ks < (uint32_t)U / 4; ++ks) {
-                        const uint32_t u = ks * 4 + lg;
-                        const uint2 av = L.rows[ra * U + (u ^ (ra & SWM))];
-                        const uint2 bv = L.rows[rc * U + (u ^ (rc & SWM))];
-                        acc = gram_step<F8>(av, bv, acc);
-                    }
+                    // same chunk order as the short-list kernel: a pair's
+                    // value does not depend on the kernel that joins it
+                    const nnm_f32x4 acc = gram_block<U, F8>(L.rows, ra, rc, lg);
                     const uint32_t cj = cb * 16 + lr;
                     const bool colok = cj < m;
                     const uint32_t cm = colok ? cj : 0u;
