@@ -570,6 +570,111 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
+// Projection with both weight matrices in LDS (bf16, loaded once per block;
+// rows padded by 4 elements, so the 16 weight rows that one MFMA operand
+// reads fall in distinct banks).  PJ_WAVES waves, 16 data rows per wave at a
+// time, the next 16 rows' loads issued before the current ones' MFMAs.  Same
+// sums in the same order as k_mlp_project (bit-identical output).
+#define PJ_WAVES 8u
+template <uint32_t HID, uint32_t DOUT, uint32_t NC>
+__global__ __launch_bounds__(PJ_WAVES * NNM_WAVE) void k_mlp_project_lds(
+    const float* __restrict__ X, uint64_t n, float alpha, const uint16_t* __restrict__ W1b,
+    const float* __restrict__ b1, const uint16_t* __restrict__ W2b, const float* __restrict__ b2,
+    uint16_t* __restrict__ P, float* __restrict__ pnorm) {
+    constexpr uint32_t D = NC * 16, P1 = D + 4, P2 = HID + 4;
+    __shared__ __align__(16) short w1s[HID * P1];
+    __shared__ __align__(16) short w2s[DOUT * P2];
+    __shared__ __align__(16) short hsh[PJ_WAVES][16][HID + 4];
+    for (uint32_t i = threadIdx.x; i < HID * D / 4; i += blockDim.x) {
+        const uint32_t r = i / (D / 4), c = i % (D / 4);
+        *reinterpret_cast<uint2*>(&w1s[r * P1 + 4 * c]) = *reinterpret_cast<const uint2*>(W1b + (uint64_t)r * D + 4 * c);
+    }
+    for (uint32_t i = threadIdx.x; i < DOUT * HID / 4; i += blockDim.x) {
+        const uint32_t r = i / (HID / 4), c = i % (HID / 4);
+        *reinterpret_cast<uint2*>(&w2s[r * P2 + 4 * c]) =
+            *reinterpret_cast<const uint2*>(W2b + (uint64_t)r * HID + 4 * c);
+    }
+    __syncthreads();
+    const uint32_t wv = threadIdx.x / NNM_WAVE, lane = threadIdx.x % NNM_WAVE;
+    const uint32_t lr = lane & 15u, lg = lane >> 4;
+    const uint64_t nblk = (n + 15) / 16, step = (uint64_t)gridDim.x * PJ_WAVES;
+    uint64_t blk = (uint64_t)blockIdx.x * PJ_WAVES + wv;
+    float4 xn[NC];
+    auto load = [&](uint64_t b, float4(&x)[NC]) {
+        const uint64_t row = min(b * 16 + lr, n - 1);
+#pragma unroll
+        for (uint32_t c = 0; c < NC; ++c) x[c] = *reinterpret_cast<const float4*>(X + row * D + c * 16 + 4 * lg);
+    };
+    if (blk < nblk) load(blk, xn);
+    for (; blk < nblk; blk += step) {
+        const uint64_t r0 = blk * 16;
+        mlp_bf16x4 xa[NC];
+#pragma unroll
+        for (uint32_t c = 0; c < NC; ++c) {
+            xa[c][0] = mlp_to_bf16(alpha * xn[c].x);
+            xa[c][1] = mlp_to_bf16(alpha * xn[c].y);
+            xa[c][2] = mlp_to_bf16(alpha * xn[c].z);
+            xa[c][3] = mlp_to_bf16(alpha * xn[c].w);
+        }
+        if (blk + step < nblk) load(blk + step, xn);
+        // layer 1: H[16 x HID] = relu(a X W1^T + b1)
+        for (uint32_t n0 = 0; n0 < HID; n0 += 16) {
+            mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            const short* w1r = &w1s[(n0 + lr) * P1 + 4 * lg];
+#pragma unroll
+            for (uint32_t c = 0; c < NC; ++c) {
+                const uint2 w = *reinterpret_cast<const uint2*>(w1r + c * 16);
+                mlp_bf16x4 b;
+                b[0] = (short)(w.x & 0xFFFFu);
+                b[1] = (short)(w.x >> 16);
+                b[2] = (short)(w.y & 0xFFFFu);
+                b[3] = (short)(w.y >> 16);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xa[c], b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) hsh[wv][4 * lg + q][n0 + lr] = mlp_to_bf16(fmaxf(acc[q] + b1[n0 + lr], 0.f));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // layer 2: Z[16 x DOUT] = H W2^T + b2
+        float zq[DOUT / 16][4];
+#pragma unroll
+        for (uint32_t t = 0; t < DOUT / 16; ++t) {
+            mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (uint32_t k0 = 0; k0 < HID; k0 += 16) {
+                mlp_bf16x4 a, b;
+                const uint2 w = *reinterpret_cast<const uint2*>(&w2s[(t * 16 + lr) * P2 + k0 + 4 * lg]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[q] = hsh[wv][lr][k0 + 4 * lg + q];
+                b[0] = (short)(w.x & 0xFFFFu);
+                b[1] = (short)(w.x >> 16);
+                b[2] = (short)(w.y & 0xFFFFu);
+                b[3] = (short)(w.y >> 16);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) zq[t][q] = acc[q] + b2[t * 16 + lr];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t row = r0 + 4 * lg + q;
+            float nrm = 0.f;
+#pragma unroll
+            for (uint32_t t = 0; t < DOUT / 16; ++t) {
+                const short hb = mlp_to_bf16(zq[t][q]);
+                const float r = __uint_as_float((uint32_t)(uint16_t)hb << 16);
+                nrm = __builtin_fmaf(r, r, nrm);
+                if (row < n) P[row * DOUT + t * 16 + lr] = (uint16_t)hb;
+            }
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) nrm += __shfl_xor(nrm, o, NNM_WAVE);
+            if (lr == 0 && row < n) pnorm[row] = nrm;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // ---------------------------------------------------------------- host side
 
 static uint32_t grid_for_mlp(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + 255) / 256); }
@@ -934,7 +1039,9 @@ int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs) {
     hipLaunchKernelGGL((k_mlp_project<H, D, NC>), dim3(grid), dim3(256), 0, c->stream, c->X, c->n, c->ld, M->alpha, \
                        W1b, M->b1, W2b, M->b2, M->P, M->pnorm)
     if (M->hid == 256 && M->dout == 32 && c->ld == 128)
-        NNM_PROJECT(256, 32, 8);
+        hipLaunchKernelGGL((k_mlp_project_lds<256, 32, 8>), dim3((uint32_t)std::min<uint64_t>((c->n + 15) / 16, c->num_cus)),
+                           dim3(PJ_WAVES * NNM_WAVE), 0, c->stream, c->X, c->n, M->alpha, W1b, M->b1, W2b, M->b2, M->P,
+                           M->pnorm);
     else if (M->hid == 256 && M->dout == 32)
         NNM_PROJECT(256, 32, 0);
     else if (M->hid == 128 && M->dout == 32)

@@ -297,20 +297,42 @@ __device__ inline void tile_scatter(const uint32_t (&d)[PT_PER], const uint32_t 
     __syncthreads();
     uint32_t c2[2];
     const uint32_t nl = block_prefix_bins(hist, wsum, c2);
+    // the bin starts first (unconditional reads), then the stores: the
+    // reads are not held back behind the stores
+    uint32_t lp[PT_PER];
+#pragma unroll
+    for (uint32_t k = 0; k < PT_PER; ++k) lp[k] = hist[min(d[k], PT_BINS - 1)] + rk[k];
 #pragma unroll
     for (uint32_t k = 0; k < PT_PER; ++k) {
         if (d[k] == NO_DIGIT) continue;
-        const uint32_t lp = hist[d[k]] + rk[k];
-        lkey[lp] = ky[k];
-        ltgt[lp] = tg[k];
+        lkey[lp[k]] = ky[k];
+        ltgt[lp[k]] = tg[k];
     }
     __syncthreads();
-    for (uint32_t lp = tid; lp < nl; lp += PT_THREADS) {
-        const uint32_t tgv = ltgt[lp];
-        const uint32_t r = (tgv >> shift) - base;
-        const uint32_t p = curs[r] + (lp - hist[r]);
-        okey[p] = lkey[lp];
-        otgt[p] = tgv;
+    if (nl) {
+        // all LDS reads first (positions past nl read the last record and
+        // store nothing): one LDS round trip per level, not one per record
+        uint32_t tgv[PT_PER], p[PT_PER];
+        uint64_t kv[PT_PER];
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            const uint32_t lp = min(tid + k * PT_THREADS, nl - 1);
+            tgv[k] = ltgt[lp];
+            kv[k] = lkey[lp];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            const uint32_t lp = min(tid + k * PT_THREADS, nl - 1);
+            const uint32_t r = (tgv[k] >> shift) - base;
+            p[k] = curs[r] + (lp - hist[r]);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PT_PER; ++k) {
+            if (tid + k * PT_THREADS < nl) {
+                okey[p[k]] = kv[k];
+                otgt[p[k]] = tgv[k];
+            }
+        }
     }
     __syncthreads();
     curs[2 * tid] += c2[0];
