@@ -493,6 +493,28 @@ struct OwnArgs {
     uint64_t* counters;
 };
 
+// A chunk's records [c0, c0 + m) into registers, thread tid's k-th at
+// c0 + tid + k NT (past m: the last record again; nothing is loaded when m
+// is 0).  No branches between the loads: all are in flight before the
+// first wait.  tg gets the targets.
+template <uint32_t PER, uint32_t NT>
+__device__ inline void own_chunk_load(const OwnArgs& a, uint32_t c0, uint32_t m, uint64_t (&key)[PER],
+                                      uint32_t (&tg)[PER]) {
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        key[k] = NNM_EMPTY_KEY;
+        tg[k] = 0xFFFFFFFFu;
+    }
+    if (m == 0) return;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+        const uint64_t j = (uint64_t)c0 + min(tid + NT * k, m - 1);
+        key[k] = a.key[j];
+        tg[k] = a.tgt[j];
+    }
+}
+
 // FP64 squared L2 across the wave: lanes take elements lane, lane + 64, ...;
 // the partial sums are combined in a fixed tree (same result every call).
 __device__ inline double own_l2_f64(const float* __restrict__ a, const float* __restrict__ b, uint32_t d) {
@@ -783,23 +805,20 @@ __global__ __launch_bounds__(256) void k_own(OwnArgs a) {
         for (;;) {
             const uint32_t ce = min(e, c0 + OWN_CH), m = ce - c0;
             const bool last_chunk = ce >= e;
-            if (tid < OWN_T) hcnt[tid] = 0;
-            __syncthreads();
+            // the chunk's records, all loads in flight before the first wait
+            // (positions past m repeat the last record and are not counted)
             uint64_t key[OWN_CH / 256];
             uint32_t tl[OWN_CH / 256], rk[OWN_CH / 256];
+            if (tid < OWN_T) hcnt[tid] = 0;
+            own_chunk_load<OWN_CH / 256, 256>(a, c0, m, key, tl);
+            __syncthreads();
 #pragma unroll
             for (uint32_t k = 0; k < OWN_CH / 256; ++k) {
-                const uint32_t j = tid + 256 * k;
-                tl[k] = OWN_T;
-                rk[k] = 0;
-                key[k] = NNM_EMPTY_KEY;
-                if (j < m) {
-                    key[k] = a.key[(uint64_t)c0 + j];
-                    const uint64_t tg = a.tgt[(uint64_t)c0 + j];
-                    // the partition puts only this bin's targets here
-                    tl[k] = tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
-                    if (tl[k] < OWN_T) rk[k] = atomicAdd(&hcnt[tl[k]], 1u);
-                }
+                // the partition puts only this bin's targets here
+                const uint64_t tg = tl[k];
+                tl[k] = tid + 256 * k < m && tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
+                if (tl[k] == OWN_T) key[k] = NNM_EMPTY_KEY;
+                rk[k] = tl[k] < OWN_T ? atomicAdd(&hcnt[tl[k]], 1u) : 0u;
             }
             __syncthreads();
             if (wave == 0) {
@@ -1028,22 +1047,17 @@ __global__ __launch_bounds__(OWN2_WAVES * NNM_WAVE, 6) void k_own2(OwnArgs a) {
         for (;;) {
             const uint32_t ce = min(e, c0 + OWN_CH), m = ce - c0;
             const bool last_chunk = ce >= e;
-            if (tid < OWN_T) hcnt[tid] = 0;
-            __syncthreads();
             uint64_t key[PER];
             uint32_t tl[PER], rk[PER];
+            if (tid < OWN_T) hcnt[tid] = 0;
+            own_chunk_load<PER, NT>(a, c0, m, key, tl);
+            __syncthreads();
 #pragma unroll
             for (uint32_t k = 0; k < PER; ++k) {
-                const uint32_t j = tid + NT * k;
-                tl[k] = OWN_T;
-                rk[k] = 0;
-                key[k] = NNM_EMPTY_KEY;
-                if (j < m) {
-                    key[k] = a.key[(uint64_t)c0 + j];
-                    const uint64_t tg = a.tgt[(uint64_t)c0 + j];
-                    tl[k] = tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
-                    if (tl[k] < OWN_T) rk[k] = atomicAdd(&hcnt[tl[k]], 1u);
-                }
+                const uint64_t tg = tl[k];
+                tl[k] = tid + NT * k < m && tg >= t0 && tg < t0 + OWN_T ? (uint32_t)(tg - t0) : OWN_T;
+                if (tl[k] == OWN_T) key[k] = NNM_EMPTY_KEY;
+                rk[k] = tl[k] < OWN_T ? atomicAdd(&hcnt[tl[k]], 1u) : 0u;
             }
             __syncthreads();
             if (wave == 0) {
