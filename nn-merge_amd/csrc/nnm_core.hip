@@ -829,14 +829,18 @@ __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ 
                               const float* __restrict__ prior_dist, uint32_t* __restrict__ g_ids,
                               float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
                               uint64_t* __restrict__ counters) {
-    const uint32_t hl = nnm_lane() & 31u, hbase = nnm_lane() & 32u;
+    const uint32_t lane = nnm_lane(), hl = lane & 31u, hbase = lane & 32u, wv = threadIdx.x / NNM_WAVE;
+    // the wave's two prior rows (ids, distances) for the membership test
+    __shared__ __align__(16) uint32_t s_pid[256 / NNM_WAVE][NNM_WAVE];
+    __shared__ float s_pd[256 / NNM_WAVE][NNM_WAVE];
     uint64_t nd = 0;
     NNM_HALF_LOOP(v0, n) {
-        const uint64_t v = v0 + (nnm_lane() >> 5);
+        const uint64_t v = v0 + (lane >> 5);
         const bool ok = v < n;
+        const bool ent = ok && hl < K;
         uint32_t id = 0xFFFFFFFFu, f = 0, pr = 0xFFFFFFFFu;
         float dd = 3.402823466e38f, pdv = 0.f;
-        if (ok && hl < K) {
+        if (ent) {
             id = g_ids[v * K + hl];
             f = renew ? 1u : g_flag[v * K + hl];
             if (prior) pr = prior[v * K + hl];
@@ -845,13 +849,25 @@ __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ 
         bool found = false;
         float known = 0.f;
         if (prior) {
-            // membership in the half's prior row (every lane takes part in the shuffles)
-            for (uint32_t p = 0; p < K; ++p) {
-                const bool hit = (uint32_t)__shfl((int)pr, (int)(hbase + p), NNM_WAVE) == id;
-                const float pd = __shfl(pdv, (int)(hbase + p), NNM_WAVE);
-                known = hit ? pd : known;
-                found |= hit;
+            // membership in the half's prior row: its ids four at a time from
+            // LDS (entries past K hold ~0, which no entry's id equals)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();  // the previous rows' readers are done
+            s_pid[wv][lane] = pr;
+            s_pd[wv][lane] = pdv;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint32_t pos = NNM_WAVE;
+            for (uint32_t p = 0; p < K; p += 4) {
+                const uint4 q = *reinterpret_cast<const uint4*>(&s_pid[wv][hbase + p]);
+                pos = q.x == id ? p : pos;
+                pos = q.y == id ? p + 1 : pos;
+                pos = q.z == id ? p + 2 : pos;
+                pos = q.w == id ? p + 3 : pos;
             }
+            found = ent && pos < NNM_WAVE;
+            if (found) known = s_pd[wv][hbase + pos];
             if (!found) f = 1u;
         }
         if (ok && hl < K) {
