@@ -34,7 +34,7 @@
 // Offer log: records {key = dist bits << 32 | id, target} in two arrays,
 // written by each join wave into chunks of NNM_OCH records it reserves from
 // a shared counter.
-#define NNM_OCH 4096u
+#define NNM_OCH 8192u
 // Owner bins: the partition passes group offers by target >> NNM_OWN_BITS.
 #define NNM_OWN_BITS 5u
 // Default offer-log capacity per active row (nnm_params.offer_cap = 0).

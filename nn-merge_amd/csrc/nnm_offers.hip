@@ -32,10 +32,10 @@
 
 #include "nnm_internal.hpp"
 
-#define PT_THREADS 512u
+#define PT_THREADS 1024u
 #define PT_PER 8u                          // records per thread in a tile
 #define PT_TILE (PT_THREADS * PT_PER)      // records per tile (one log chunk)
-#define PT_BINS 1024u                      // LDS bins of a tile (relative to its smallest digit)
+#define PT_BINS 2048u                      // LDS bins of a tile (relative to its smallest digit)
 #define PT_MAXBITS 10u                     // digit bits of one pass
 #define OWN_T (1u << NNM_OWN_BITS)         // targets per owner bin
 #define OWN_CH 4096u                       // records an owner sorts in LDS at a time
@@ -1156,7 +1156,7 @@ int fill32(nnm_ctx* c, uint32_t* p, uint64_t n, uint32_t v) { return nnm_fill_u3
 
 }  // namespace
 
-static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus * 2; }
+static uint32_t pass_grid(nnm_ctx* c) { return (uint32_t)c->num_cus; }  // one tile workgroup per CU (LDS)
 static uint64_t pass0_slots(nnm_ctx* c) { return (uint64_t)PT_BINS * pass_grid(c) + 1; }
 // second-pass table entries for n rows: (first-pass bins x 2^b1) owner bins x segments
 static uint64_t pass1_slots(uint64_t n) { return (((n >> NNM_OWN_BITS) + 2 + PT_BINS) * P1_SEGS) + 1; }
@@ -1181,7 +1181,7 @@ int nnm_offer_buffers(nnm_ctx* c, const nnm_params* p) {
     NNM_HIP(c, hipStreamSynchronize(c->stream));
     nnm_offer_free(c);
     const uint64_t n = c->n;
-    // the default: a chunk per 4096 records at 256 records per row, plus room
+    // the default: a chunk per NNM_OCH records at 256 records per row, plus room
     // for the partly filled last chunk of every join wave (at most 32 waves
     // per CU over the three join kernels); an explicit offer_cap sizes the
     // log to exactly offer_cap records per row
