@@ -1573,52 +1573,55 @@ __device__ inline uint32_t wave_append_reverse(const uint32_t* __restrict__ head
     return out;
 }
 
-// Two rows per wave: a half copies its row's reverse lists when both fit
-// the cap; a row with a list above the cap is then taken by the whole wave
-// (the seeded selection).
+// Four rows per wave: a quarter copies its row's reverse lists when both fit
+// the cap (the row's counts and list sizes are read together: one round trip
+// before the copy); a row with a list above the cap is then taken by the
+// whole wave (the seeded selection).
 __global__ void k_lb_select(uint64_t n, const uint32_t* __restrict__ rcnt, const uint32_t* __restrict__ roff,
                             const uint32_t* __restrict__ rbuf, const uint32_t* __restrict__ obuf, uint32_t cap,
                             uint64_t seed,
                             uint32_t* __restrict__ l_new, uint32_t cap_new, uint32_t* __restrict__ l_old,
                             uint32_t cap_old, uint32_t* __restrict__ n_new, uint32_t* __restrict__ n_old,
                             const uint64_t* __restrict__ counters) {
-    const uint32_t lane = nnm_lane(), hl = lane & 31u;
+    const uint32_t lane = nnm_lane(), ql = lane & 15u;
     // sampling salt of this iteration (the iteration number lives on the device)
     const uint32_t salt = (uint32_t)(nnm_rand(seed, 0xC0FFEEull, counters[CNT_ITER]) >> 32);
-    NNM_HALF_LOOP(v0, n) {
-        const uint64_t v = v0 + (lane >> 5);
+    const uint64_t wpb = blockDim.x / NNM_WAVE;
+    for (uint64_t v0 = 4 * ((uint64_t)blockIdx.x * wpb + threadIdx.x / NNM_WAVE); v0 < n;
+         v0 += 4 * (uint64_t)gridDim.x * wpb) {
+        const uint64_t v = v0 + (lane >> 4);
         const bool ok = v < n;
-        const uint32_t cn = ok ? rcnt[v] : 0u, co = ok ? rcnt[n + v] : 0u;
-        const bool big = cn > cap || co > cap;
+        const uint64_t vc = ok ? v : n - 1;  // in bounds: the loads need no branch
+        const uint32_t cn = rcnt[vc], co = rcnt[n + vc], fn = n_new[vc], fo = n_old[vc];
+        const bool big = ok && (cn > cap || co > cap);
         if (ok && !big) {
-            const uint32_t fn = n_new[v], fo = n_old[v];
             const uint32_t* hn = rbuf + v * NNM_RB;
             const uint32_t* ho = rbuf + (n + v) * NNM_RB;
             const uint32_t* rn = cn > NNM_RB ? obuf + roff[v] : nullptr;
             const uint32_t* ro = co > NNM_RB ? obuf + roff[n + v] : nullptr;
             uint32_t* dn = l_new + v * cap_new + fn;
             uint32_t* dold = l_old + v * cap_old + fo;
-            for (uint32_t i = hl; i < cn; i += 32) dn[i] = i < NNM_RB ? hn[i] : rn[i - NNM_RB];
-            for (uint32_t i = hl; i < co; i += 32) dold[i] = i < NNM_RB ? ho[i] : ro[i - NNM_RB];
-            if (hl == 0) {
+            for (uint32_t i = ql; i < cn; i += 16) dn[i] = i < NNM_RB ? hn[i] : rn[i - NNM_RB];
+            for (uint32_t i = ql; i < co; i += 16) dold[i] = i < NNM_RB ? ho[i] : ro[i - NNM_RB];
+            if (ql == 0) {
                 n_new[v] = fn + cn;
                 n_old[v] = fo + co;
             }
         }
-        for (uint64_t bb = nnm_ballot(ok && big && hl == 0); bb; bb &= bb - 1) {
-            const uint64_t w = v0 + ((uint32_t)__builtin_ctzll(bb) >> 5);
-            const uint32_t fn = n_new[w], fo = n_old[w];
+        for (uint64_t bb = nnm_ballot(big && ql == 0); bb; bb &= bb - 1) {
+            const uint64_t w = v0 + ((uint32_t)__builtin_ctzll(bb) >> 4);
+            const uint32_t wfn = n_new[w], wfo = n_old[w];
             const uint32_t wn = rcnt[w], wo = rcnt[n + w];
             const uint32_t* rn = wn > NNM_RB ? obuf + roff[w] : nullptr;
             const uint32_t* ro = wo > NNM_RB ? obuf + roff[n + w] : nullptr;
             const uint32_t an = wave_append_reverse(rbuf + w * NNM_RB, rn, wn, cap, nnm_hash32((uint32_t)w, salt),
-                                                    l_new + w * cap_new + fn);
+                                                    l_new + w * cap_new + wfn);
             const uint32_t ao = wave_append_reverse(rbuf + (n + w) * NNM_RB, ro, wo, cap,
                                                     nnm_hash32((uint32_t)w, salt ^ 0xA511E9B3u),
-                                                    l_old + w * cap_old + fo);
+                                                    l_old + w * cap_old + wfo);
             if (lane == 0) {
-                n_new[w] = fn + an;
-                n_old[w] = fo + ao;
+                n_new[w] = wfn + an;
+                n_old[w] = wfo + ao;
             }
         }
     }
@@ -1874,7 +1877,7 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     hipLaunchKernelGGL(k_lb_spill_fill, dim3(grid_for(n, 256, 8192)), dim3(256), 0, c->stream, n, K, c->g_ids,
                        c->newmask, c->spill, c->roff, c->rcur, c->obuf);
     NNM_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, (n + 1) / 2)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
+    hipLaunchKernelGGL(k_lb_select, dim3(wave_grid(c, (n + 3) / 4)), dim3(256), 0, c->stream, n, c->rcnt, c->roff,
                        c->rbuf, c->obuf, c->rev_cap, p->seed, c->l_new, c->cap_new, c->l_old, c->cap_old, c->n_new,
                        c->n_old, c->counters);
     NNM_LAUNCH_CHECK(c);
