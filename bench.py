@@ -59,7 +59,9 @@ sys.path.insert(0, os.path.join(ROOT, "nn-merge_amd"))
 PEAK_HBM_GBS = 8000.0
 PEAK = {"fp32": 157.3, "bf16": 2500.0}  # TFLOP/s dense
 # offer thresholds the learned sweep tries (fractions of the estimated k-th distance)
-LEARN_TAUS = (0.6, 0.7)
+# learned-phase schedules the sweep tries: (sampling rate, offer threshold);
+# sampling 0 = the run's rho
+LEARN_SCHEDULES = ((0.0, 0.6), (0.0, 0.7), (1.0, 0.55), (1.0, 0.6))
 METRIC = "merge secs at recall@10>=0.95 (learned+BF16 vs plain FP32 GPU), Gdist/s"
 MAIN_KERNEL = {"fp32": r"k_join_f32<", "bf16": r"k_join_bf16<", "learned": r"k_join_bf16<"}
 COPY_BYTES = 1 << 30
@@ -282,9 +284,10 @@ def main():
                          "never offers")
     ap.add_argument("--learn-tau", type=float, default=0.0,
                     help="learned: offer threshold of the learned phase (x estimated k-th; 0 = the sweep "
-                         "picks the fastest of 0.6 and 0.7 reaching the target)")
+                         "picks the fastest schedule reaching the target)")
     ap.add_argument("--learn-rho", type=float, default=0.0,
-                    help="learned: forward sampling rate of the learned phase (0 = --rho)")
+                    help="learned: forward sampling rate of the learned phase (0 = --rho, or the sweep's "
+                         "choice when --learn-tau is 0 too)")
     ap.add_argument("--fp8", action="store_true",
                     help="learned: run the learned phase on FP8 (OCP e4m3) rows of the projection")
     ap.add_argument("--switch", choices=["found", "keep", "renew"], default="found",
@@ -358,7 +361,7 @@ def main():
     precs = {"fp32": nnm.PREC_FP32, "bf16": nnm.PREC_BF16, "learned": nnm.PREC_LEARNED}
     graph_flag = 0 if args.no_graph else nnm.FLAG_GRAPH
 
-    def params(name, T, tail, graph, tau=0.0):
+    def params(name, T, tail, graph, tau=0.0, lrho=0.0):
         fl = nnm.FLAG_AUTOTUNE | (graph_flag if graph else 0)
         if name == "learned":
             fl |= {"renew": nnm.FLAG_LEARNED_RENEW, "found": nnm.FLAG_LEARNED_MARK_FOUND, "keep": 0}[args.switch]
@@ -367,8 +370,8 @@ def main():
                                plain_tail=tail, flags=fl, offer_cap=args.offer_cap)
         if name == "learned" and tau > 0:
             p.learn_tau = tau
-        if name == "learned" and args.learn_rho > 0:
-            p.learn_rho = args.learn_rho
+        if name == "learned" and lrho > 0:
+            p.learn_rho = lrho
         return p
 
     def step(name, p):
@@ -393,39 +396,47 @@ def main():
     def choose(name):
         """Smallest-time setting of the path whose merged graph reaches the
         target recall: plain paths the smallest T; the learned path the
-        fastest (T, plain tail, offer threshold) among the smallest T of each
-        (tail, threshold)."""
+        fastest (T, plain tail, offer threshold, sampling rate) among the
+        smallest T of each (tail, schedule)."""
         if name != "learned":
             if args.fp32_iters:
-                return args.fp32_iters, 1, 0.0
+                return args.fp32_iters, 1, 0.0, 0.0
             for T in range(2, args.max_iters + 1):
                 step(name, params(name, T, 1, False))
                 rec = recall_now()[0]
                 sweep_log.append({"path": name, "T": T, "recall@10": round(rec, 5)})
                 log(f"[sweep {name}] T {T}: recall {rec:.4f}")
                 if rec >= args.target_recall:
-                    return T, 1, 0.0
-            return args.max_iters, 1, 0.0
+                    return T, 1, 0.0, 0.0
+            return args.max_iters, 1, 0.0, 0.0
         if args.learned_iters:
-            return args.learned_iters, args.plain_tail, args.learn_tau
+            return args.learned_iters, args.plain_tail, args.learn_tau, args.learn_rho
+        if args.learn_tau > 0 or args.learn_rho > 0:
+            scheds = ((args.learn_rho, args.learn_tau if args.learn_tau > 0 else 0.6),)
+        else:
+            scheds = LEARN_SCHEDULES
         best = None
-        for tau in ((args.learn_tau,) if args.learn_tau > 0 else LEARN_TAUS):
+        for lrho, tau in scheds:
             for tail in (2, 3):
                 for T in range(tail + 2, args.max_iters + 1):
-                    step(name, params(name, T, tail, False, tau))  # warm the tuned shapes
+                    step(name, params(name, T, tail, False, tau, lrho))  # warm the tuned shapes
                     t0 = time.perf_counter()
-                    step(name, params(name, T, tail, False, tau))
+                    step(name, params(name, T, tail, False, tau, lrho))
                     torch.cuda.synchronize()
                     secs = time.perf_counter() - t0
                     rec = recall_now()[0]
                     sweep_log.append({"path": name, "T": T, "plain_tail": tail, "learn_tau": tau,
-                                      "recall@10": round(rec, 5), "merge_secs_eager": round(secs, 4)})
-                    log(f"[sweep learned] tau {tau} T {T} tail {tail}: recall {rec:.4f} secs {secs:.3f}")
+                                      "learn_rho": lrho, "recall@10": round(rec, 5),
+                                      "merge_secs_eager": round(secs, 4)})
+                    log(f"[sweep learned] rho {lrho} tau {tau} T {T} tail {tail}: recall {rec:.4f} "
+                        f"secs {secs:.3f}")
                     if rec >= args.target_recall:
                         if best is None or secs < best[0]:
-                            best = (secs, T, tail, tau)
+                            best = (secs, T, tail, tau, lrho)
                         break
-        return (best[1], best[2], best[3]) if best else (args.max_iters, 3, LEARN_TAUS[0])
+        if best:
+            return best[1], best[2], best[3], best[4]
+        return args.max_iters, 3, LEARN_SCHEDULES[0][1], LEARN_SCHEDULES[0][0]
 
     def run(name):
         train = None
@@ -435,8 +446,8 @@ def main():
             train = ctx.mlp_train(steps=args.train_steps, batch=4096, lr=2e-3, keep=args.keep, seed=9)
             train["tau"] = float(ctx.mlp_weights()[5])
             log(f"mlp training: {train}")
-        T, tail, tau = choose(name)
-        p = params(name, T, tail, True, tau)
+        T, tail, tau, lrho = choose(name)
+        p = params(name, T, tail, True, tau, lrho)
         for w in range(args.warmup):
             r, i, s = step(name, p)
             log(f"[{name}] warmup {w}: descent {s['secs']:.3f}s iters {s['iters']} graph {s['graph_iters']}")
@@ -455,7 +466,7 @@ def main():
                 f"updates {s['updates']}")
         # one eager step for the kernel timings (HIP events on the library's
         # stream around the join and the delivery; graph replays carry none)
-        er, ei, es = step(name, params(name, T, tail, False, tau))
+        er, ei, es = step(name, params(name, T, tail, False, tau, lrho))
         log(f"[{name}] eager: descent {es['secs']:.3f}s join {es['join_secs']:.3f}s (short {es['join_main_secs']:.3f}s) "
             f"deliver {es['deliver_secs']:.3f}s")
         # distances: exact ones (the reference's nCmps: join distances on the
@@ -480,6 +491,7 @@ def main():
         if name == "learned":
             res["plain_tail"] = tail
             res["learn_tau"] = round(float(p.learn_tau), 4)
+            res["learn_rho"] = round(float(p.learn_rho), 4) if p.learn_rho > 0 else round(float(p.rho), 4)
             res["train"] = {k: round(v, 5) for k, v in train.items()}
         # dominant kernel: the packed short-list join of the eager step,
         # averaged per launch
@@ -537,6 +549,7 @@ def main():
                       "hip_graph": not args.no_graph,
                       "learned": {"plain_tail": runs.get("learned", {}).get("plain_tail"),
                                   "learn_tau": runs.get("learned", {}).get("learn_tau"),
+                                  "learn_rho": runs.get("learned", {}).get("learn_rho"),
                                   "switch": args.switch, "fp8_rows": bool(args.fp8)}},
            "value_counts": "exact distances per second of the chosen form (join distances of the plain iterations, "
                            "set-up distances, exact list distances at the switch); the learned phase's projected "
