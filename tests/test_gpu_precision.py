@@ -54,11 +54,14 @@ def test_bf16_merge_equals_fp32_on_integer_rows(ctx):
         ctx.set_partition([0, h, n], [1, 0], [0, 0])
         ctx.set_graph(sub)
         p = nnm.default_params(K=K, rule=nnm.RULE_MERGE, prec=prec, rho=0.5, max_iters=10, seed=3)
-        ctx.merge_init(p)
+        ctx.merge_init(p)  # bf16: the fresh candidates' distances from rows staged in LDS
+        init = ctx.get_graph()
         st = ctx.nndescent(p)
         ctx.merge_finish(True)
-        out[prec] = (ctx.get_graph(), st)
-    (g32, s32), (g16, s16) = out[nnm.PREC_FP32], out[nnm.PREC_BF16]
+        out[prec] = (init, ctx.get_graph(), st)
+    (i32, g32, s32), (i16, g16, s16) = out[nnm.PREC_FP32], out[nnm.PREC_BF16]
+    for a, b in zip(i32, i16):
+        assert np.array_equal(a, b)
     assert s32["dists"] == s16["dists"]
     for a, b in zip(g32, g16):
         assert np.array_equal(a, b)
