@@ -290,8 +290,10 @@ static void free_iter(nnm_ctx* c) {
     void** ps[] = {(void**)&c->l_new, (void**)&c->l_old, (void**)&c->n_new, (void**)&c->n_old,
                    (void**)&c->newmask, (void**)&c->rcnt, (void**)&c->roff, (void**)&c->rcur,
                    (void**)&c->rbuf, (void**)&c->ocnt, (void**)&c->obuf, (void**)&c->spill,
-                   (void**)&c->scan_tmp, (void**)&c->kth, (void**)&c->vmeta};
+                   (void**)&c->scan_tmp, (void**)&c->kth, (void**)&c->vmeta, (void**)&c->lrec};
     for (void** p : ps) { nnm_free(c, *p); *p = nullptr; }
+    c->lrec_n = c->lrec_gen = 0;
+    c->lrec_live = 0;
     nnm_offer_free(c);
     c->cap_new = c->cap_old = c->rev_cap = 0;
 }
@@ -1394,14 +1396,27 @@ extern "C" int nnm_merge_finish(nnm_ctx* c, int dedup) {
 // k-th list distance per vertex, and the join's packed per-vertex record
 // {label, k-th distance x offer window, norm of the joined rows, 0}: one
 // 16-byte random read per candidate-list entry in the join.
+// lrec (optional): the record goes to bytes 64..79 of the vertex's 128-byte
+// learned record instead (the learned join then reads it there).
 __global__ void k_kth(uint64_t n, uint32_t K, const float* __restrict__ g_dist, const uint32_t* __restrict__ label,
-                      const float* __restrict__ norms, float win, float* __restrict__ kth, uint4* __restrict__ vmeta) {
+                      const float* __restrict__ norms, float win, float* __restrict__ kth, uint4* __restrict__ vmeta,
+                      uint4* __restrict__ lrec) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v < n) {
         const float k = g_dist[v * K + K - 1];
         kth[v] = k;
-        vmeta[v] = make_uint4(label[v], __float_as_uint(k * win), norms ? __float_as_uint(norms[v]) : 0u, 0u);
+        const uint4 r = make_uint4(label[v], __float_as_uint(k * win), norms ? __float_as_uint(norms[v]) : 0u, 0u);
+        if (lrec) lrec[v * 8 + 4] = r;
+        else vmeta[v] = r;
     }
+}
+
+// Learned records: the 64-byte projected row of every vertex into bytes
+// 0..63 of its 128-byte record (one thread per 16-byte chunk).  A gathered
+// row and its join record then share one 128-byte line.
+__global__ void k_lrec_rows(uint64_t n, const uint4* __restrict__ P, uint4* __restrict__ lrec) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * 4; i += (uint64_t)gridDim.x * blockDim.x)
+        lrec[(i >> 2) * 8 + (i & 3)] = P[i];
 }
 
 // List build, pass 1 (src/abstractmerge.cpp:94-134): one lane per list
@@ -1856,8 +1871,27 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
                        : fp8 ? nnm_mlp_norms8(c)
                        : p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c)
                        : c->metric == NNM_METRIC_COSINE ? c->sumsq : nullptr;
+    // learned join on 32-d bf16 rows: rows and join records in one 128-byte
+    // record per vertex (the rows copied once per projection)
+    c->lrec_live = 0;
+    if (p->prec == NNM_PREC_LEARNED && !fp8 && nnm_mlp_dout(c) == 32 && nnm_mlp_proj_gen(c)) {
+        if (!c->lrec || c->lrec_n != c->n) {
+            nnm_free(c, c->lrec);
+            c->lrec = nullptr;
+            c->lrec_n = c->lrec_gen = 0;
+            NNM_TRY(alloc_n(c, &c->lrec, c->n * 8));
+            c->lrec_n = c->n;
+        }
+        if (c->lrec_gen != nnm_mlp_proj_gen(c)) {
+            hipLaunchKernelGGL(k_lrec_rows, dim3(grid_for(c->n * 4, 256, 8192)), dim3(256), 0, c->stream, c->n,
+                               reinterpret_cast<const uint4*>(nnm_mlp_rows(c)), c->lrec);
+            NNM_LAUNCH_CHECK(c);
+            c->lrec_gen = nnm_mlp_proj_gen(c);
+        }
+        c->lrec_live = 1;
+    }
     hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->label,
-                       nrm, nnm_near_tie_window(c, p), c->kth, c->vmeta);
+                       nrm, nnm_near_tie_window(c, p), c->kth, c->vmeta, c->lrec_live ? c->lrec : nullptr);
     NNM_LAUNCH_CHECK(c);
     const uint32_t lb_grid = wave_grid(c, (n + 1) / 2);
     if (K <= 32) {

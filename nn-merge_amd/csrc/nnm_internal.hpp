@@ -150,6 +150,10 @@ struct nnm_ctx {
     uint64_t* counters = nullptr;   // CNT_N
     float* kth = nullptr;           // n: k-th distance of every list at iteration start
     uint4* vmeta = nullptr;         // n: join record {label, kth x offer window, norm, 0}
+    uint4* lrec = nullptr;          // n x 128 bytes: learned row (bytes 0..63) + join record (64..79)
+    uint64_t lrec_n = 0;            // rows lrec holds
+    uint64_t lrec_gen = 0;          // projection generation its rows came from (0: none)
+    int lrec_live = 0;              // this iteration's learned join reads lrec
     uint32_t* grp_flags = nullptr;  // ngrp: per-group new-flag counts
     uint8_t* grp_active = nullptr;  // ngrp
 
@@ -234,6 +238,7 @@ int nnm_scan_u32(nnm_ctx* c, const uint32_t* in, uint64_t m, uint32_t* out, uint
 int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs);
 void nnm_mlp_invalidate_projection(nnm_ctx* c);
 const uint16_t* nnm_mlp_rows(nnm_ctx* c);
+uint64_t nnm_mlp_proj_gen(nnm_ctx* c);
 const float* nnm_mlp_norms(nnm_ctx* c);
 uint32_t nnm_mlp_dout(nnm_ctx* c);
 float nnm_mlp_scale(nnm_ctx* c);

@@ -55,6 +55,8 @@ struct JoinArgs {
     uint32_t K;
     uint32_t rule;
     const uint4* vmeta;  // per vertex {label, k-th distance x offer window, row norm, 0}
+    uint32_t mpitch;     // uint4 from one vertex's record to the next (1: packed)
+    uint32_t rpitch;     // bf16 joins: 16-byte chunks from one row to the next (0: packed rows)
     uint32_t js;      // short-list window rows (<= the kernel's compiled window)
     uint32_t chunk;   // rows per work-queue grab
     const uint32_t* l_new;
@@ -559,8 +561,9 @@ struct RowChunks {
 
 // Rows [0, rows) of a window into LDS (lane r of wcp: the id of row r): per
 // instruction RPI rows, lane l fetching chunk (l % CH) ^ sw(r) of row r0 + l / CH.
+// rp: 16-byte chunks from one row to the next in Xb.
 template <int U>
-__device__ inline void stage_rows(uint2* dst, const uint16_t* __restrict__ Xb, uint32_t wcp, uint32_t rows,
+__device__ inline void stage_rows(uint2* dst, const uint16_t* __restrict__ Xb, uint32_t rp, uint32_t wcp, uint32_t rows,
                                   uint32_t lane) {
     using C = RowChunks<U>;
     const uint32_t rr = lane / C::CH, c = lane % C::CH;
@@ -569,7 +572,7 @@ __device__ inline void stage_rows(uint2* dst, const uint16_t* __restrict__ Xb, u
         const uint32_t r = r0 + rr;
         const uint32_t id = (uint32_t)__shfl((int)wcp, (int)min(r, NNM_WAVE - 1u), NNM_WAVE);
         if (rr < C::RPI && r < rows) {
-            const char* g = xb + ((uint64_t)id * C::CH + (c ^ C::sw(r))) * 16u;
+            const char* g = xb + ((uint64_t)id * rp + (c ^ C::sw(r))) * 16u;
             __builtin_amdgcn_global_load_lds((const void*)g, (__attribute__((address_space(3))) void*)(dst + r0 * U),
                                              16, 0, 0);
         }
@@ -763,11 +766,11 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
             uint32_t wcp = 0;
             if (lane < rows) {
                 L.id[lane] = wid;
-                __builtin_amdgcn_global_load_lds((const void*)(a.vmeta + wid),
+                __builtin_amdgcn_global_load_lds((const void*)(a.vmeta + (uint64_t)wid * a.mpitch),
                                                  (__attribute__((address_space(3))) void*)L.rec, 16, 0, 0);
                 asm volatile("v_mov_b32 %0, %1" : "=v"(wcp) : "v"(wid));
             }
-            stage_rows<U>(L.rows, Xb, wcp, rows, lane);
+            stage_rows<U>(L.rows, Xb, a.rpitch ? a.rpitch : RowChunks<U>::CH, wcp, rows, lane);
             __builtin_amdgcn_s_waitcnt(0);
             wave_lds_sync();
             const uint32_t lr = lane & 15u, lg = lane >> 4;
@@ -874,7 +877,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
             const uint32_t nn = a.n_new[v], no = a.n_old[v], m = nn + no;
             for (uint32_t j = tid; j < m; j += blockDim.x) {
                 const uint32_t id = j < nn ? a.l_new[v * a.cap_new + j] : a.l_old[v * a.cap_old + (j - nn)];
-                const uint4 mt = a.vmeta[id];
+                const uint4 mt = a.vmeta[(uint64_t)id * a.mpitch];
                 L.meta[0][j] = id;
                 L.meta[1][j] = mt.x;
                 L.meta[2][j] = mt.y;
@@ -885,10 +888,11 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
             // turns by load instruction (RPI rows each)
             {
                 const uint32_t rr = lane / C::CH, c = lane % C::CH;
+                const uint32_t rp = a.rpitch ? a.rpitch : C::CH;
                 for (uint32_t r0 = wave * C::RPI; r0 < m; r0 += 4 * C::RPI) {
                     const uint32_t r = r0 + rr;
                     if (rr < C::RPI && r < m) {
-                        const char* g = xb + ((uint64_t)L.meta[0][r] * C::CH + (c ^ C::sw(r))) * 16u;
+                        const char* g = xb + ((uint64_t)L.meta[0][r] * rp + (c ^ C::sw(r))) * 16u;
                         __builtin_amdgcn_global_load_lds((const void*)g,
                                                          (__attribute__((address_space(3))) void*)(L.rows + r0 * U), 16,
                                                          0, 0);
@@ -1040,6 +1044,8 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
     a.rule = p->rule;
     a.chunk = chunk ? chunk : JOIN_CHUNK;
     a.vmeta = c->vmeta;
+    a.mpitch = 1;
+    a.rpitch = 0;
     a.l_new = c->l_new;
     a.cap_new = c->cap_new;
     a.l_old = c->l_old;
@@ -1085,6 +1091,14 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
                 // estimated k-th thresholds)
                 return NNM_OK;
             }
+        }
+        if (p->prec == NNM_PREC_LEARNED && rl == 32 && c->lrec_live) {
+            // rows and join records from the 128-byte learned records: one
+            // line per gathered row
+            rows = reinterpret_cast<const uint16_t*>(c->lrec);
+            a.vmeta = c->lrec + 4;
+            a.mpitch = 8;
+            a.rpitch = 8;
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
         a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;

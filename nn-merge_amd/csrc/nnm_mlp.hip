@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -48,7 +49,7 @@ struct nnm_mlp_state {
     uint16_t* P = nullptr;
     float* pnorm = nullptr;
     uint64_t projected_n = 0;
-    uint64_t proj_gen = 0;  // counts computed projections
+    uint64_t proj_gen = 0;  // number of the current projection (unique in the process; 0: none)
     // FP8 copy of the projection (OCP e4m3, scaled by s8 = 448 / max |P|),
     // the squared norms of the encoded rows, and the projection it encodes
     uint8_t* P8 = nullptr;
@@ -1058,7 +1059,10 @@ int nnm_mlp_ensure_projection(nnm_ctx* c, double* secs) {
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     if (secs) *secs = ms * 1e-3;
     M->projected_n = c->n;
-    ++M->proj_gen;
+    // numbered across models and contexts: copies made from an earlier
+    // model's projection never match a later one's number
+    static std::atomic<uint64_t> proj_seq{0};
+    M->proj_gen = ++proj_seq;
     return NNM_OK;
 }
 
@@ -1185,6 +1189,7 @@ extern "C" int nnm_mlp_project(nnm_ctx* c, float* out, double* secs) {
 
 // accessors for the join launcher
 const uint16_t* nnm_mlp_rows(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->P : nullptr; }
+uint64_t nnm_mlp_proj_gen(nnm_ctx* c) { return mlp_of(c) && mlp_of(c)->P ? mlp_of(c)->proj_gen : 0; }
 const float* nnm_mlp_norms(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->pnorm : nullptr; }
 uint32_t nnm_mlp_dout(nnm_ctx* c) { return mlp_of(c) ? mlp_of(c)->dout : 0; }
 float nnm_mlp_scale(nnm_ctx* c) { return mlp_of(c) ? 1.f / (mlp_of(c)->alpha * mlp_of(c)->alpha) : 1.f; }
