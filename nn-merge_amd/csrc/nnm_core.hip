@@ -330,6 +330,35 @@ __global__ void k_to_bf16(const float* __restrict__ X, uint16_t* __restrict__ Xb
     if (lane == 0) norms[row] = acc;
 }
 
+// int8 rows: value - shift for the d values, 0 past them; the squared norm
+// as an exact integer.  One wave per row.
+__global__ void k_to_i8(const float* __restrict__ X, int8_t* __restrict__ Xi8, float* __restrict__ norms, uint64_t n,
+                        uint32_t ld, uint32_t d, float shift) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / NNM_WAVE) + threadIdx.x / NNM_WAVE;
+    const uint32_t lane = threadIdx.x % NNM_WAVE;
+    if (row >= n) return;
+    int acc = 0;
+    for (uint32_t k = lane; k < ld; k += NNM_WAVE) {
+        const int q = k < d ? (int)(X[row * ld + k] - shift) : 0;
+        Xi8[row * ld + k] = (int8_t)q;
+        acc += q * q;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, NNM_WAVE);
+    if (lane == 0) norms[row] = (float)acc;
+}
+
+int nnm_ensure_i8(nnm_ctx* c) {
+    if (c->Xi8) return NNM_OK;
+    if (!c->data_i8) return nnm_fail(c, NNM_ERR_STATE, "int8 rows need integer data spanning at most 255");
+    NNM_TRY(alloc_n(c, &c->Xi8, c->n * c->ld));
+    NNM_TRY(alloc_n(c, &c->norms_i8, c->n));
+    hipLaunchKernelGGL(k_to_i8, dim3(grid_for(c->n, 4, 1u << 30)), dim3(256), 0, c->stream, c->X, c->Xi8, c->norms_i8,
+                       c->n, c->ld, c->d, c->i8_shift);
+    NNM_LAUNCH_CHECK(c);
+    return NNM_OK;
+}
+
 int nnm_ensure_bf16(nnm_ctx* c) {
     if (c->Xb) return NNM_OK;
     NNM_TRY(alloc_n(c, &c->Xb, c->n * c->ld));
@@ -381,6 +410,8 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
     nnm_dpg_free(c);
     nnm_free(c, c->X); c->X = nullptr;
     nnm_free(c, c->Xb); c->Xb = nullptr;
+    nnm_free(c, c->Xi8); c->Xi8 = nullptr;
+    nnm_free(c, c->norms_i8); c->norms_i8 = nullptr;
     nnm_free(c, c->norms); c->norms = nullptr;
     nnm_free(c, c->sumsq); c->sumsq = nullptr;
     c->n = n;
@@ -407,6 +438,10 @@ extern "C" int nnm_set_data(nnm_ctx* c, const float* X, uint64_t n, uint32_t d) 
         if ((double)d * span * span > 16777216.0 || 2.0 * d * amax * amax > 16777216.0) exact = 0;
     }
     c->data_exact_bf16 = exact;
+    // int8 rows (value - (min + 128)) are exact when the values span at most
+    // 255; the int8 join is built for rows of 128
+    c->data_i8 = exact && (double)vmax - (double)vmin <= 255.0 && c->ld == 128;
+    c->i8_shift = vmin + 128.f;
     c->data_min = vmin;
     c->data_max = vmax;
     // default partition: one built subgraph, one group
@@ -1867,7 +1902,9 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     // row norms in the join's metadata: bf16 norms (norm form), learned
     // projection norms, or the cosine metric's squared norms
     const bool fp8 = p->prec == NNM_PREC_LEARNED && (p->flags & NNM_FLAG_LEARNED_FP8);
-    const float* nrm = p->prec == NNM_PREC_BF16 ? c->norms
+    if (nnm_use_i8(c, p)) NNM_TRY(nnm_ensure_i8(c));
+    const float* nrm = nnm_use_i8(c, p) ? c->norms_i8
+                       : p->prec == NNM_PREC_BF16 ? c->norms
                        : fp8 ? nnm_mlp_norms8(c)
                        : p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c)
                        : c->metric == NNM_METRIC_COSINE ? c->sumsq : nullptr;

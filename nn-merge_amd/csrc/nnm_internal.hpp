@@ -84,6 +84,10 @@ struct nnm_ctx {
     uint16_t* Xb = nullptr;   // bf16 copy (lazy)
     float* norms = nullptr;   // squared norms of the bf16 rows (lazy)
     int data_exact_bf16 = 0;  // every value an integer in [-256, 256]
+    int data_i8 = 0;          // exact, rows of 128, values spanning at most 255: int8 rows exact
+    float i8_shift = 0.f;     // the int8 rows hold value - i8_shift (in [-128, 127])
+    int8_t* Xi8 = nullptr;    // int8 copy (lazy; padding 0)
+    float* norms_i8 = nullptr;  // squared norms of the int8 rows (exact integers)
     float data_min = 0.f, data_max = 0.f;  // value range of the rows (with the exactness scan)
     uint32_t metric = NNM_METRIC_L2;
     float* sumsq = nullptr;   // cosine: squared row norms (with the data)
@@ -187,6 +191,12 @@ int nnm_h2d(nnm_ctx* c, void* dst, const void* src, size_t bytes);
 int nnm_d2h(nnm_ctx* c, void* dst, const void* src, size_t bytes);
 int nnm_h2d_rows(nnm_ctx* c, float* dst, uint32_t ld, const float* src, uint32_t d, uint64_t n);
 int nnm_ensure_bf16(nnm_ctx* c);
+int nnm_ensure_i8(nnm_ctx* c);
+// The plain BF16 joins run on the int8 rows (exact integer Gram on MFMA,
+// the same keys bit for bit: every distance is an exact integer below 2^24).
+inline bool nnm_use_i8(const nnm_ctx* c, const nnm_params* p) {
+    return p->prec == NNM_PREC_BF16 && c->data_i8 && c->metric == NNM_METRIC_L2;
+}
 // Offer window above the k-th distance: 1 + 2 * (d + 2) * 2^-24 (twice the
 // relative error bound of an FP32 squared distance of d terms), so that
 // candidates within rounding of the k-th entry reach the FP64 tie test; 1

@@ -515,6 +515,7 @@ typedef float nnm_f32x4 __attribute__((ext_vector_type(4)));
 #define KEY_EXACT 0    // bf16 norm form is exact (integer rows)
 #define KEY_BOUND 1    // offer the lower bound d - eps; exact re-rank later
 #define KEY_LEARN 2    // learned projection: estimate kscale * d_proj, margin tau
+#define KEY_I8 3       // int8 rows (value - shift): exact integer Gram, exact key
 
 #define BJS 64         // bf16 window rows (one meta entry per lane)
 
@@ -582,12 +583,31 @@ __device__ inline void stage_rows(uint2* dst, const uint16_t* __restrict__ Xb, u
 // The 16x16 Gram block of window rows ra (lane row lr) and rc (lane column):
 // lane group lg supplies chunks lg, lg + 4, ... (both halves of a chunk feed
 // two MFMA steps), or for U = 4 unit lg.  Every k position is summed once;
-// A and B take the same positions.
-template <int U, bool F8>
+// A and B take the same positions.  I8: int8 rows, one 16x16x64 integer
+// MFMA per chunk (lane group lg's 16 values), the exact integer Gram.
+template <int U, bool F8, bool I8 = false>
 __device__ inline nnm_f32x4 gram_block(const uint2* rows, uint32_t ra, uint32_t rc, uint32_t lg) {
     using C = RowChunks<U>;
     nnm_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    if constexpr (U >= 8) {
+    if constexpr (I8) {
+        static_assert(U >= 8, "int8 rows of at least 64 values");
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        const uint4* r16 = reinterpret_cast<const uint4*>(rows);
+        v4i s = {0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t kc = 0; kc < (uint32_t)U / 8; ++kc) {
+            const uint32_t c = kc * 4 + lg;
+            const uint4 av = r16[ra * C::CH + (c ^ C::sw(ra))];
+            const uint4 bv = r16[rc * C::CH + (c ^ C::sw(rc))];
+            const v4i A = {(int)av.x, (int)av.y, (int)av.z, (int)av.w};
+            const v4i B = {(int)bv.x, (int)bv.y, (int)bv.z, (int)bv.w};
+            s = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B, s, 0, 0, 0);
+        }
+        acc[0] = (float)s[0];
+        acc[1] = (float)s[1];
+        acc[2] = (float)s[2];
+        acc[3] = (float)s[3];
+    } else if constexpr (U >= 8) {
         const uint4* r16 = reinterpret_cast<const uint4*>(rows);
 #pragma unroll
         for (uint32_t kc = 0; kc < (uint32_t)U / 8; ++kc) {
@@ -796,7 +816,7 @@ __global__ __launch_bounds__(64, 8) void k_join_bf16(JoinArgs a, const uint16_t*
                     const uint32_t ra = min(ob + rb * 16 + lr, (uint32_t)BJS - 1);
                     for (uint32_t cb = rb; cb < CB; ++cb) {
                         const uint32_t rc = min(ob + cb * 16 + lr, (uint32_t)BJS - 1);
-                        const nnm_f32x4 acc = gram_block<U, F8>(L.rows, ra, rc, lg);
+                        const nnm_f32x4 acc = gram_block<U, F8, MODE == KEY_I8>(L.rows, ra, rc, lg);
                         const uint32_t cj = cb * 16 + lr;
                         const bool colok = cj < mb;
                         const uint4 br = L.rec[rc];
@@ -923,7 +943,7 @@ __global__ __launch_bounds__(256) void k_join_bf16_long(JoinArgs a, const uint16
                     const uint32_t rc = cb * 16 + lr;
                     // same chunk order as the short-list kernel: a pair's
                     // value does not depend on the kernel that joins it
-                    const nnm_f32x4 acc = gram_block<U, F8>(L.rows, ra, rc, lg);
+                    const nnm_f32x4 acc = gram_block<U, F8, MODE == KEY_I8>(L.rows, ra, rc, lg);
                     const uint32_t cj = cb * 16 + lr;
                     const bool colok = cj < m;
                     const uint32_t cm = colok ? cj : 0u;
@@ -1101,6 +1121,12 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
             a.rpitch = 8;
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
+        if (nnm_use_i8(c, p)) {
+            // int8 rows of 128 values (128 bytes: one line per gathered row)
+            NNM_TRY(nnm_ensure_i8(c));
+            rows = reinterpret_cast<const uint16_t*>(c->Xi8);
+            mode = KEY_I8;
+        }
         a.js = js_req ? min(js_req, (uint32_t)BJS) : (uint32_t)BJS;
 #define NNM_BFM(U, MODE)                                                                                            \
     hipLaunchKernelGGL((k_join_bf16<U, false, MODE>), dim3(join_grid(c, k_join_bf16<U, false, MODE>)), dim3(64), 0, \
@@ -1115,13 +1141,17 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
         else if (mode == KEY_BOUND) { NNM_BFM(U, KEY_BOUND) }        \
         else { NNM_BFM(U, KEY_LEARN) }                               \
         break;
-        switch (rl) {
-            NNM_BF(4)
-            NNM_BF(8)
-            NNM_BF(16)
-            NNM_BF(24)
-            NNM_BF(32)
-            default: return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join row length");
+        if (mode == KEY_I8) {
+            NNM_BFM(16, KEY_I8)  // 128-byte rows: 16 units of 8 bytes
+        } else {
+            switch (rl) {
+                NNM_BF(4)
+                NNM_BF(8)
+                NNM_BF(16)
+                NNM_BF(24)
+                NNM_BF(32)
+                default: return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join row length");
+            }
         }
 #undef NNM_BF
 #undef NNM_BFM
