@@ -641,11 +641,15 @@ __device__ inline void wave_random_row(const float* __restrict__ X, const uint16
 // are the same.  DM_ANY: the run-time metric (FP32 rows, or bf16 rows when
 // given); DM_L2_BF16: L2 on bf16 rows (exact there); DM_L2_F32: L2 on FP32
 // rows.
-enum : int { DM_ANY = 0, DM_L2_BF16 = 1, DM_L2_F32 = 2 };
+// DM_L2_I8: L2 on the int8 rows (passed as Xb; exact there).
+enum : int { DM_ANY = 0, DM_L2_BF16 = 1, DM_L2_F32 = 2, DM_L2_I8 = 3 };
 template <int DM>
 __device__ inline float setup_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,
                                    uint64_t a, uint64_t b, const nnm_metric_args& mt) {
-    if constexpr (DM == DM_L2_BF16) return nnm_l2_rows_bf16(Xb + a * ld, Xb + b * ld, ld);
+    if constexpr (DM == DM_L2_I8) {
+        const int8_t* q = reinterpret_cast<const int8_t*>(Xb);
+        return nnm_l2_rows_i8(q + a * ld, q + b * ld, ld);
+    } else if constexpr (DM == DM_L2_BF16) return nnm_l2_rows_bf16(Xb + a * ld, Xb + b * ld, ld);
     else if constexpr (DM == DM_L2_F32) return nnm_l2_rows(X + a * ld, X + b * ld, ld);
     else return nnm_row_dist(X, Xb, ld, a, b, mt);
 }
@@ -862,7 +866,7 @@ __global__ void k_lists_estimate_fp8(uint64_t n, uint32_t K, const uint8_t* __re
 // (the same function of the same rows, so the same bits) and only the
 // entries the learned phase found are computed.
 __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ X, const uint16_t* __restrict__ Xb_exact,
-                              uint32_t ld, int renew, const uint32_t* __restrict__ prior,
+                              const int8_t* __restrict__ Xi8, uint32_t ld, int renew, const uint32_t* __restrict__ prior,
                               const float* __restrict__ prior_dist, uint32_t* __restrict__ g_ids,
                               float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
                               uint64_t* __restrict__ counters) {
@@ -911,8 +915,9 @@ __global__ void k_lists_exact(uint64_t n, uint32_t K, const float* __restrict__ 
             if (prior_dist && found) {
                 dd = known;
             } else {
-                dd = Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
-                              : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
+                dd = Xi8        ? nnm_l2_rows_i8(Xi8 + v * ld, Xi8 + (uint64_t)id * ld, ld)
+                     : Xb_exact ? nnm_l2_rows_bf16(Xb_exact + v * ld, Xb_exact + (uint64_t)id * ld, ld)
+                                : nnm_l2_rows(X + v * ld, X + (uint64_t)id * ld, ld);
                 ++nd;
             }
         }
@@ -1303,8 +1308,14 @@ extern "C" int nnm_graph_restore(nnm_ctx* c, int recompute, nnm_stats* st) {
             xb = c->Xb;
         }
         uint32_t* bad = reinterpret_cast<uint32_t*>(c->counters + CNT_TOTAL_REV);
-        const int dm = setup_dm(c, xb);
-        hipLaunchKernelGGL(dm == DM_L2_BF16 ? k_set_graph<DM_L2_BF16>
+        int dm = setup_dm(c, xb);
+        if (xb && c->data_i8) {  // the same exact distances from the int8 rows
+            NNM_TRY(nnm_ensure_i8(c));
+            xb = reinterpret_cast<const uint16_t*>(c->Xi8);
+            dm = DM_L2_I8;
+        }
+        hipLaunchKernelGGL(dm == DM_L2_I8 ? k_set_graph<DM_L2_I8>
+                           : dm == DM_L2_BF16 ? k_set_graph<DM_L2_BF16>
                                             : dm == DM_L2_F32 ? k_set_graph<DM_L2_F32> : k_set_graph<DM_ANY>,
                            dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X, xb, c->ld, c->n_act,
                            c->K, 1, 0, c->label, c->g_ids, c->g_dist, c->g_flag, c->counters, bad, metric_args(c));
@@ -1379,9 +1390,17 @@ extern "C" int nnm_merge_init(nnm_ctx* c, const nnm_params* p, nnm_stats* st) {
     }
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    const int dm = setup_dm(c, xb);
+    int dm = setup_dm(c, xb);
+    if (xb && c->data_i8 && K + 1 <= 32) {
+        // int8 rows (two rows per wave kernel): the same exact distances from
+        // half the bytes
+        NNM_TRY(nnm_ensure_i8(c));
+        xb = reinterpret_cast<const uint16_t*>(c->Xi8);
+        dm = DM_L2_I8;
+    }
     if (K + 1 <= 32)  // two rows per wave
-        hipLaunchKernelGGL(dm == DM_L2_BF16  ? k_merge_init_half<DM_L2_BF16>
+        hipLaunchKernelGGL(dm == DM_L2_I8 ? k_merge_init_half<DM_L2_I8>
+                           : dm == DM_L2_BF16  ? k_merge_init_half<DM_L2_BF16>
                            : dm == DM_L2_F32 ? k_merge_init_half<DM_L2_F32>
                                              : k_merge_init_half<DM_ANY>,
                            dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->X,
@@ -2165,8 +2184,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         return NNM_OK;
     };
     auto to_exact = [&]() -> int {
+        if (c->data_i8 && c->metric == NNM_METRIC_L2) NNM_TRY(nnm_ensure_i8(c));
         hipLaunchKernelGGL(k_lists_exact, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K, c->X,
-                           (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->ld,
+                           (c->data_exact_bf16 && c->Xb) ? c->Xb : nullptr, c->data_i8 ? c->Xi8 : nullptr, c->ld,
                            (p->flags & NNM_FLAG_LEARNED_RENEW) ? 1 : 0, mark_found ? c->p_ids : nullptr,
                            reuse_dist ? c->p_dist : nullptr, c->g_ids,
                            c->g_dist, c->g_flag, c->counters);

@@ -405,6 +405,39 @@ __device__ inline float nnm_l2_rows_bf16(const uint16_t* __restrict__ a, const u
     return acc;
 }
 
+// Exact squared L2 distance of int8 rows (value - shift, padding 0):
+// sum a^2 + sum b^2 - 2 sum ab in integers by four-way int8 dot products.
+// For integer rows of 128 values spanning at most 255 every such distance
+// is an integer below 2^24, so the float is exact: the bits of nnm_l2_rows.
+__device__ inline float nnm_l2_rows_i8(const int8_t* __restrict__ a, const int8_t* __restrict__ b, uint32_t ld) {
+    const uint4* a4 = reinterpret_cast<const uint4*>(a);
+    const uint4* b4 = reinterpret_cast<const uint4*>(b);
+    int aa = 0, bb = 0, ab = 0;
+    const uint32_t n16 = ld / 16;
+    for (uint32_t i0 = 0; i0 < n16; i0 += 4) {
+        uint4 x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t i = min(i0 + u, n16 - 1);  // past the row: its last chunk again, not summed
+            x[u] = a4[i];
+            y[u] = b4[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i0 + u >= n16) break;
+            const int xs[4] = {(int)x[u].x, (int)x[u].y, (int)x[u].z, (int)x[u].w};
+            const int ys[4] = {(int)y[u].x, (int)y[u].y, (int)y[u].z, (int)y[u].w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                aa = __builtin_amdgcn_sdot4(xs[h], xs[h], aa, false);
+                bb = __builtin_amdgcn_sdot4(ys[h], ys[h], bb, false);
+                ab = __builtin_amdgcn_sdot4(xs[h], ys[h], ab, false);
+            }
+        }
+    }
+    return (float)(aa + bb - 2 * ab);
+}
+
 // Row distance from the bf16 rows when given (values exact in bf16), else
 // from the FP32 rows: the same bits either way.
 __device__ inline float nnm_row_dist(const float* __restrict__ X, const uint16_t* __restrict__ Xb, uint32_t ld,

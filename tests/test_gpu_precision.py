@@ -108,3 +108,32 @@ def test_fp64_error_report_and_near_ties(ctx):
         assert rel.max() <= e["max_rel"] * (1 + 1e-9)
         assert e["max_rel"] <= (96 + 2) * 2.0 ** -24
         assert st["tie_swaps"] <= st["near_ties"]
+
+
+def test_graph_restore_recompute_int8_rows_equal_fp32(ctx):
+    """The saved lists' distances recomputed from the exact low-precision
+    rows (recompute=2: the int8 rows for integer data spanning at most 255
+    over 128 values) equal the FP32 recompute bit for bit, and the J-Merge
+    set-up from them too."""
+    X = nnm_data.sift_like(6000, 128, centers=16, seed=63)
+    n, h, K = 6000, 3000, 20
+    ctx.set_data(X)
+    ctx.set_partition([0, h, n], [1, 1], [0, 1])
+    ctx.random_init(K, seed=2)
+    ctx.nndescent(nnm.default_params(K=K, seed=2, max_iters=4))
+    ids, _, _ = ctx.get_graph()
+    ctx.set_partition([0, h, n], [1, 0], [0, 0])
+    ctx.set_graph(ids)
+    ctx.graph_save()
+    out = {}
+    for rc in (1, 2):
+        ctx.graph_restore(recompute=rc)
+        restored = ctx.get_graph()
+        ctx.merge_init(nnm.default_params(K=K, rule=nnm.RULE_MERGE, prec=nnm.PREC_BF16, seed=3))
+        out[rc] = (restored, ctx.get_graph())
+    for a, b in zip(out[1][0] + out[1][1], out[2][0] + out[2][1]):
+        assert np.array_equal(a, b)
+    v = np.repeat(np.arange(0, h, 37), K)
+    e = out[2][0][0][:h:37].ravel()
+    exact = np.array([orc.l2_seq(X[i], X[j]) for i, j in zip(v, e)], np.float32)
+    assert np.array_equal(out[2][0][1][:h:37].ravel(), exact)
