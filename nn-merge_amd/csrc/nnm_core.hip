@@ -1921,31 +1921,16 @@ static int build_lists(nnm_ctx* c, const nnm_params* p) {
     // row norms in the join's metadata: bf16 norms (norm form), learned
     // projection norms, or the cosine metric's squared norms
     const bool fp8 = p->prec == NNM_PREC_LEARNED && (p->flags & NNM_FLAG_LEARNED_FP8);
-    if (nnm_use_i8(c, p)) NNM_TRY(nnm_ensure_i8(c));
     const float* nrm = nnm_use_i8(c, p) ? c->norms_i8
                        : p->prec == NNM_PREC_BF16 ? c->norms
                        : fp8 ? nnm_mlp_norms8(c)
                        : p->prec == NNM_PREC_LEARNED ? nnm_mlp_norms(c)
                        : c->metric == NNM_METRIC_COSINE ? c->sumsq : nullptr;
     // learned join on 32-d bf16 rows: rows and join records in one 128-byte
-    // record per vertex (the rows copied once per projection)
-    c->lrec_live = 0;
-    if (p->prec == NNM_PREC_LEARNED && !fp8 && nnm_mlp_dout(c) == 32 && nnm_mlp_proj_gen(c)) {
-        if (!c->lrec || c->lrec_n != c->n) {
-            nnm_free(c, c->lrec);
-            c->lrec = nullptr;
-            c->lrec_n = c->lrec_gen = 0;
-            NNM_TRY(alloc_n(c, &c->lrec, c->n * 8));
-            c->lrec_n = c->n;
-        }
-        if (c->lrec_gen != nnm_mlp_proj_gen(c)) {
-            hipLaunchKernelGGL(k_lrec_rows, dim3(grid_for(c->n * 4, 256, 8192)), dim3(256), 0, c->stream, c->n,
-                               reinterpret_cast<const uint4*>(nnm_mlp_rows(c)), c->lrec);
-            NNM_LAUNCH_CHECK(c);
-            c->lrec_gen = nnm_mlp_proj_gen(c);
-        }
-        c->lrec_live = 1;
-    }
+    // record per vertex, prepared before the iterations (no allocation here:
+    // an iteration may be captured into a graph)
+    c->lrec_live = p->prec == NNM_PREC_LEARNED && !fp8 && nnm_mlp_dout(c) == 32 && c->lrec && c->lrec_n == c->n &&
+                   c->lrec_gen == nnm_mlp_proj_gen(c) && nnm_mlp_proj_gen(c) != 0;
     hipLaunchKernelGGL(k_kth, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, c->stream, n, K, c->g_dist, c->label,
                        nrm, nnm_near_tie_window(c, p), c->kth, c->vmeta, c->lrec_live ? c->lrec : nullptr);
     NNM_LAUNCH_CHECK(c);
@@ -2095,6 +2080,25 @@ struct IterGraphs {
     }
 };
 
+// The learned records' rows from the current projection (allocated and
+// copied outside the iterations, once per projection).
+static int prepare_lrec(nnm_ctx* c) {
+    if (!c->lrec || c->lrec_n != c->n) {
+        nnm_free(c, c->lrec);
+        c->lrec = nullptr;
+        c->lrec_n = c->lrec_gen = 0;
+        NNM_TRY(alloc_n(c, &c->lrec, c->n * 8));
+        c->lrec_n = c->n;
+    }
+    if (c->lrec_gen != nnm_mlp_proj_gen(c)) {
+        hipLaunchKernelGGL(k_lrec_rows, dim3(grid_for(c->n * 4, 256, 8192)), dim3(256), 0, c->stream, c->n,
+                           reinterpret_cast<const uint4*>(nnm_mlp_rows(c)), c->lrec);
+        NNM_LAUNCH_CHECK(c);
+        c->lrec_gen = nnm_mlp_proj_gen(c);
+    }
+    return NNM_OK;
+}
+
 static int capture_iteration(nnm_ctx* c, const nnm_params* p, uint32_t js, uint32_t chunk, hipGraphExec_t* out) {
     NNM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     const int rc = run_iteration(c, p, js, chunk, nullptr);
@@ -2138,6 +2142,9 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
         NNM_TRY(nnm_mlp_ensure_fp8(c));
     }
     if (p->prec != NNM_PREC_FP32) NNM_TRY(nnm_ensure_bf16(c));
+    // int8 rows for the plain joins on integer data (the learned run's tail too)
+    if (p->prec != NNM_PREC_FP32 && c->data_i8 && c->metric == NNM_METRIC_L2) NNM_TRY(nnm_ensure_i8(c));
+    if (learned && !fp8 && nnm_mlp_dout(c) == 32) NNM_TRY(prepare_lrec(c));
     stats_reset(st);
     NNM_TRY(fill_u64(c, c->counters, CNT_N, 0));
     NNM_HIP(c, hipEventRecord(c->ev0, c->stream));
@@ -2251,7 +2258,8 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
                 row_bytes = (uint64_t)nnm_mlp_dout(c) * (fp8 ? 1 : 2) + 16;
                 flops_per_dist = 2ull * nnm_mlp_dout(c);
             } else {
-                row_bytes = (uint64_t)c->d * 2 + 16;
+                // bf16 rows, or int8 rows on integer data
+                row_bytes = (uint64_t)c->d * (nnm_use_i8(c, &q) ? 1 : 2) + 16;
                 flops_per_dist = 2ull * c->d;
             }
             main_bytes += se * row_bytes + so * 12 + c->n_act * 8;

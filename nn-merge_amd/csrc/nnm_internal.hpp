@@ -193,9 +193,10 @@ int nnm_h2d_rows(nnm_ctx* c, float* dst, uint32_t ld, const float* src, uint32_t
 int nnm_ensure_bf16(nnm_ctx* c);
 int nnm_ensure_i8(nnm_ctx* c);
 // The plain BF16 joins run on the int8 rows (exact integer Gram on MFMA,
-// the same keys bit for bit: every distance is an exact integer below 2^24).
+// the same keys bit for bit: every distance is an exact integer below 2^24)
+// once the rows exist (made before a descent's iterations).
 inline bool nnm_use_i8(const nnm_ctx* c, const nnm_params* p) {
-    return p->prec == NNM_PREC_BF16 && c->data_i8 && c->metric == NNM_METRIC_L2;
+    return p->prec == NNM_PREC_BF16 && c->data_i8 && c->metric == NNM_METRIC_L2 && c->Xi8;
 }
 // Offer window above the k-th distance: 1 + 2 * (d + 2) * 2^-24 (twice the
 // relative error bound of an FP32 squared distance of d terms), so that

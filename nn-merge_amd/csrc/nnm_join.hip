@@ -1122,8 +1122,8 @@ int nnm_launch_join(nnm_ctx* c, const nnm_params* p, uint32_t js_req, uint32_t c
         }
         if (rl > 128) return nnm_fail(c, NNM_ERR_UNSUPPORTED, "bf16 join supports d <= 128");
         if (nnm_use_i8(c, p)) {
-            // int8 rows of 128 values (128 bytes: one line per gathered row)
-            NNM_TRY(nnm_ensure_i8(c));
+            // int8 rows of 128 values (128 bytes: one line per gathered row;
+            // made before the iterations, which may be captured into graphs)
             rows = reinterpret_cast<const uint16_t*>(c->Xi8);
             mode = KEY_I8;
         }
