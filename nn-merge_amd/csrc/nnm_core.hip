@@ -776,9 +776,12 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
 // 16-byte chunks (dout = 8 NC), every chunk loaded before the products;
 // NC = 0: any dout (a multiple of 8).  The products run in k order either way.
 template <uint32_t NC>
+// lrec (NC = 4 only): rows and norms from the 128-byte learned records (one
+// line per entry instead of a row and a norm gather; the same values).
 __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restrict__ P, const float* __restrict__ pn,
                                  uint32_t dout, float kscale, uint32_t* __restrict__ g_ids,
-                                 float* __restrict__ g_dist, uint8_t* __restrict__ g_flag) {
+                                 float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
+                                 const uint4* __restrict__ lrec) {
     const uint32_t hl = nnm_lane() & 31u;
     NNM_HALF_LOOP(v0, n) {
         const uint64_t v = v0 + (nnm_lane() >> 5);
@@ -788,8 +791,11 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
         if (ok && hl < K) {
             id = g_ids[v * K + hl];
             f = g_flag[v * K + hl];
-            const uint4* a = reinterpret_cast<const uint4*>(P + v * dout);
-            const uint4* b = reinterpret_cast<const uint4*>(P + (uint64_t)id * dout);
+            const bool rec = NC == 4 && lrec;
+            const uint4* a = rec ? lrec + v * 8 : reinterpret_cast<const uint4*>(P + v * dout);
+            const uint4* b = rec ? lrec + (uint64_t)id * 8 : reinterpret_cast<const uint4*>(P + (uint64_t)id * dout);
+            const float nv = rec ? __uint_as_float(lrec[v * 8 + 4].z) : pn[v];
+            const float nu = rec ? __uint_as_float(lrec[(uint64_t)id * 8 + 4].z) : pn[id];
             float dot = 0.f;
             auto step = [&](const uint4 x, const uint4 y) {
                 const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
@@ -812,7 +818,7 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
             } else {
                 for (uint32_t k = 0; k < dout / 8; ++k) step(a[k], b[k]);
             }
-            dd = fmaxf((pn[v] + pn[id]) - 2.f * dot, 0.f) * kscale;
+            dd = fmaxf((nv + nu) - 2.f * dot, 0.f) * kscale;
         }
         half_write_sorted_row(nnm_key(dd, id), f, ok, K, g_ids + v * K, g_dist + v * K, g_flag + v * K);
     }
@@ -1466,11 +1472,17 @@ __global__ void k_kth(uint64_t n, uint32_t K, const float* __restrict__ g_dist, 
 }
 
 // Learned records: the 64-byte projected row of every vertex into bytes
-// 0..63 of its 128-byte record (one thread per 16-byte chunk).  A gathered
-// row and its join record then share one 128-byte line.
-__global__ void k_lrec_rows(uint64_t n, const uint4* __restrict__ P, uint4* __restrict__ lrec) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * 4; i += (uint64_t)gridDim.x * blockDim.x)
-        lrec[(i >> 2) * 8 + (i & 3)] = P[i];
+// 0..63 of its 128-byte record and its squared norm into the join record's
+// norm field (bytes 72..75; k_kth rewrites the record with the same norm).
+// A gathered row, its norm and its join record then share one 128-byte line.
+// One thread per 16-byte chunk of the record's first 80 bytes.
+__global__ void k_lrec_rows(uint64_t n, const uint4* __restrict__ P, const float* __restrict__ pn,
+                            uint4* __restrict__ lrec) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * 5; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = i / 5;
+        const uint32_t c = (uint32_t)(i % 5);
+        lrec[v * 8 + c] = c < 4 ? P[v * 4 + c] : make_uint4(0u, 0u, __float_as_uint(pn[v]), 0u);
+    }
 }
 
 // List build, pass 1 (src/abstractmerge.cpp:94-134): one lane per list
@@ -2091,8 +2103,8 @@ static int prepare_lrec(nnm_ctx* c) {
         c->lrec_n = c->n;
     }
     if (c->lrec_gen != nnm_mlp_proj_gen(c)) {
-        hipLaunchKernelGGL(k_lrec_rows, dim3(grid_for(c->n * 4, 256, 8192)), dim3(256), 0, c->stream, c->n,
-                           reinterpret_cast<const uint4*>(nnm_mlp_rows(c)), c->lrec);
+        hipLaunchKernelGGL(k_lrec_rows, dim3(grid_for(c->n * 5, 256, 8192)), dim3(256), 0, c->stream, c->n,
+                           reinterpret_cast<const uint4*>(nnm_mlp_rows(c)), nnm_mlp_norms(c), c->lrec);
         NNM_LAUNCH_CHECK(c);
         c->lrec_gen = nnm_mlp_proj_gen(c);
     }
@@ -2185,7 +2197,11 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
             hipLaunchKernelGGL(dout == 32 ? k_lists_estimate<4> : k_lists_estimate<0>,
                                dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K,
                                nnm_mlp_rows(c), nnm_mlp_norms(c), dout, nnm_mlp_scale(c), c->g_ids, c->g_dist,
-                               c->g_flag);
+                               c->g_flag,
+                               (dout == 32 && c->lrec && c->lrec_n == c->n && c->lrec_gen == nnm_mlp_proj_gen(c) &&
+                                nnm_mlp_proj_gen(c) != 0)
+                                   ? c->lrec
+                                   : nullptr);
         NNM_LAUNCH_CHECK(c);
         estimates = true;
         return NNM_OK;
