@@ -137,3 +137,20 @@ def test_graph_restore_recompute_int8_rows_equal_fp32(ctx):
     e = out[2][0][0][:h:37].ravel()
     exact = np.array([orc.l2_seq(X[i], X[j]) for i, j in zip(v, e)], np.float32)
     assert np.array_equal(out[2][0][1][:h:37].ravel(), exact)
+
+
+def test_int8_rows_padded_dimension_equal_fp32(ctx):
+    """Integer rows of 100 values (row pitch 128: 28 zero-padded values in the
+    int8 rows): NN-Descent on the BF16 precision (int8 joins and set-up
+    distances) gives the FP32 lists bit for bit."""
+    X = nnm_data.sift_like(8000, 100, centers=32, seed=64)
+    out = {}
+    for prec in (nnm.PREC_FP32, nnm.PREC_BF16):
+        ctx.set_data(X)
+        ctx.random_init(20, seed=4)
+        st = ctx.nndescent(nnm.default_params(K=20, prec=prec, seed=4, max_iters=6))
+        out[prec] = (ctx.get_graph(), st)
+    (g32, s32), (g16, s16) = out[nnm.PREC_FP32], out[nnm.PREC_BF16]
+    assert s32["dists"] == s16["dists"]
+    for a, b in zip(g32, g16):
+        assert np.array_equal(a, b)
