@@ -775,9 +775,9 @@ __global__ void k_set_graph(const float* __restrict__ X, const uint16_t* __restr
 // learned join's key), rows re-sorted, flags kept.  NC > 0: rows of NC
 // 16-byte chunks (dout = 8 NC), every chunk loaded before the products;
 // NC = 0: any dout (a multiple of 8).  The products run in k order either way.
-template <uint32_t NC>
-// lrec (NC = 4 only): rows and norms from the 128-byte learned records (one
-// line per entry instead of a row and a norm gather; the same values).
+// REC (NC = 4 only): rows and norms from the 128-byte learned records lrec
+// (one line per entry instead of a row and a norm gather; the same values).
+template <uint32_t NC, bool REC = false>
 __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restrict__ P, const float* __restrict__ pn,
                                  uint32_t dout, float kscale, uint32_t* __restrict__ g_ids,
                                  float* __restrict__ g_dist, uint8_t* __restrict__ g_flag,
@@ -791,7 +791,7 @@ __global__ void k_lists_estimate(uint64_t n, uint32_t K, const uint16_t* __restr
         if (ok && hl < K) {
             id = g_ids[v * K + hl];
             f = g_flag[v * K + hl];
-            const bool rec = NC == 4 && lrec;
+            constexpr bool rec = NC == 4 && REC;
             const uint4* a = rec ? lrec + v * 8 : reinterpret_cast<const uint4*>(P + v * dout);
             const uint4* b = rec ? lrec + (uint64_t)id * 8 : reinterpret_cast<const uint4*>(P + (uint64_t)id * dout);
             const float nv = rec ? __uint_as_float(lrec[v * 8 + 4].z) : pn[v];
@@ -2193,15 +2193,14 @@ static int descent(nnm_ctx* c, const nnm_params* p, uint32_t only_iter, bool sin
             hipLaunchKernelGGL(k_lists_estimate_fp8, dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream,
                                c->n_act, c->K, nnm_mlp_rows8(c), nnm_mlp_norms8(c), nnm_mlp_scale8(c), c->g_ids,
                                c->g_dist, c->g_flag);
-        else
-            hipLaunchKernelGGL(dout == 32 ? k_lists_estimate<4> : k_lists_estimate<0>,
+        else {
+            const bool rec = dout == 32 && c->lrec && c->lrec_n == c->n && c->lrec_gen == nnm_mlp_proj_gen(c) &&
+                             nnm_mlp_proj_gen(c) != 0;
+            hipLaunchKernelGGL(rec ? (k_lists_estimate<4, true>) : dout == 32 ? k_lists_estimate<4> : k_lists_estimate<0>,
                                dim3(wave_grid(c, (c->n_act + 1) / 2)), dim3(256), 0, c->stream, c->n_act, c->K,
                                nnm_mlp_rows(c), nnm_mlp_norms(c), dout, nnm_mlp_scale(c), c->g_ids, c->g_dist,
-                               c->g_flag,
-                               (dout == 32 && c->lrec && c->lrec_n == c->n && c->lrec_gen == nnm_mlp_proj_gen(c) &&
-                                nnm_mlp_proj_gen(c) != 0)
-                                   ? c->lrec
-                                   : nullptr);
+                               c->g_flag, rec ? c->lrec : nullptr);
+        }
         NNM_LAUNCH_CHECK(c);
         estimates = true;
         return NNM_OK;
