@@ -1045,9 +1045,15 @@ void nnm_join_tune_space(nnm_ctx* c, uint32_t prec, std::vector<std::pair<uint32
     out->clear();
     const uint32_t top = (prec == NNM_PREC_FP32) ? fast_window(c->ld) : (uint32_t)BJS;
     const uint32_t cand[3] = {top, top * 2 / 3, top / 2};
-    for (uint32_t js : cand)
+    // the learned phase keeps the full window: its first iteration (where the
+    // tuning runs) has the longest new lists, and a narrower window chosen
+    // there sends later iterations' lists to the long-list kernel (measured
+    // at 5M: 24.2 ms of learned joins per merge at 64 rows, 26.1 ms at a
+    // narrower choice)
+    const uint32_t ncand = prec == NNM_PREC_LEARNED ? 1u : 3u;
+    for (uint32_t i = 0; i < ncand; ++i)
         for (uint32_t ch : {4u, 16u, 64u})
-            if (js >= 8) out->emplace_back(js, ch);
+            if (cand[i] >= 8) out->emplace_back(cand[i], ch);
 }
 
 uint32_t nnm_join_learned_max_list() { return BLM; }
