@@ -542,6 +542,10 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(ch["merge_secs"] * 1e3, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f32" if chosen == "fp32" else "bf16",
+           "dtype_note": ("f32 rows" if chosen == "fp32" else
+                          "learned phase: bf16 MFMA on the 32-d projections; plain joins and exact set-up "
+                          "distances on these integer rows: int8 MFMA and int8 dot products (exact, the bits of "
+                          "FP32)"),
            "data": "synthetic (seeded generator nnm_data.sift_like)",
            "config": {"workload": workload, "n": n, "d": d, "K": K, "rho": args.rho, "centers": args.centers,
                       "rank": args.rank, "target_recall": args.target_recall, "T": ch["T"],
